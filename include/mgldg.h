@@ -1,0 +1,200 @@
+/*
+ * mgldg.h — C ABI of the MI355X-native LDG-Poisson MG-PCG library (libmgldg.so).
+ *
+ * The reference (danfortunato/multigrid-ldg) has no C ABI: its "operator API"
+ * is a C++ template surface.  Each entry point below replaces one piece of that
+ * surface; the reference interface it stands in for is cited beside it
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = OK, negative = error.  The
+ *     message of the last error on a context is returned by mgldg_last_error.
+ *     No C++ exception crosses this boundary.
+ *   - Host buffers are caller-owned.  Functions with the _device suffix take
+ *     device pointers (HBM) and enqueue on the context's stream; they return
+ *     after the work is complete unless stated otherwise.
+ *   - Layouts follow the reference: element-major vectors with npl = (p+1)^N
+ *     nodal values per element, tensor node index with the first coordinate
+ *     slowest (src/cartesian/util/range.h:187-194); element order is the
+ *     quadtree depth-first order (src/cartesian/mesh/quadtree.h:176-186);
+ *     exported BSR blocks are row-major (src/cartesian/util/common.h:25-32).
+ *     Multiple right-hand sides are stored [ndof][nrhs] (right-hand side
+ *     fastest).
+ *   - A multigrid handle is not reentrant (the reference's Multigrid holds
+ *     mutable x/b/r scratch, src/cartesian/multigrid/multigrid.h:268-273).
+ *     Use one context per host thread.
+ */
+#ifndef MGLDG_H
+#define MGLDG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MGLDG_OK             0
+#define MGLDG_ERR_ARG       -1   /* invalid argument (replaces std::invalid_argument / out_of_range) */
+#define MGLDG_ERR_HIP       -2   /* HIP runtime failure */
+#define MGLDG_ERR_NOMEM     -3   /* host or device allocation failure */
+#define MGLDG_ERR_STATE     -4   /* handle in the wrong state */
+#define MGLDG_ERR_UNSUPPORTED -5 /* valid request this build does not implement */
+
+/* ---- solve status (pcg returns void in the reference: krylov.h:23) ---- */
+#define MGLDG_CONVERGED   0
+#define MGLDG_MAXIT       1   /* maxit reached before the tolerance */
+#define MGLDG_BREAKDOWN   2   /* d^T A d <= 0 or r^T z <= 0 */
+#define MGLDG_NONFINITE   3   /* NaN/Inf met in the recurrences */
+
+/* ---- enums (src/cartesian/multigrid/multigrid.h:24-63, quadtree.h:19-23) ---- */
+#define MGLDG_EQUAL_COARSENING 0   /* kEqualCoarsening */
+#define MGLDG_RAPID_COARSENING 1   /* kRapidCoarsening */
+#define MGLDG_DIRICHLET 0          /* kDirichlet (boundaryconditions.h:10-15) */
+#define MGLDG_NEUMANN   1          /* kNeumann */
+#define MGLDG_JACOBI        0      /* MG::kJacobi */
+#define MGLDG_GAUSS_SEIDEL  1      /* MG::kGaussSeidel, multicolour on the GPU */
+#define MGLDG_COARSE_CHOLESKY   0  /* MG::kCholesky */
+#define MGLDG_COARSE_RELAXATION 1  /* MG::kRelaxation */
+#define MGLDG_RAT    0             /* MG::kRAT */
+#define MGLDG_RATRAT 1             /* MG::kRATRAT */
+#define MGLDG_FP64   0             /* V-cycle in FP64 */
+#define MGLDG_MIXED  1             /* V-cycle in FP32, CG and true residual in FP64 */
+#define MGLDG_CRIT_REFERENCE 0     /* r^T z <= tol^2 r0^T z0 (krylov.h:42) */
+#define MGLDG_CRIT_TRUE      1     /* ||b - A x|| <= tol ||b||, confirmed in FP64 */
+
+typedef struct mgldg_ctx_s*      mgldg_ctx;
+typedef struct mgldg_quadtree_s* mgldg_quadtree;
+typedef struct mgldg_problem_s*  mgldg_problem;
+typedef struct mgldg_mg_s*       mgldg_mg;
+
+/* size function: N widths for the point x.  A cell splits iff every width of
+ * the cell exceeds h(center) (src/cartesian/mesh/quadtree.tpp:15). */
+typedef void (*mgldg_sizefn)(const double* x, double* h, void* user);
+
+/* ======================= context ======================= */
+/* No reference counterpart (the reference is single-threaded CPU code).
+ * stream: a hipStream_t, or NULL to create a private stream. */
+int mgldg_ctx_create(int device, void* stream, mgldg_ctx* out);
+int mgldg_ctx_destroy(mgldg_ctx ctx);
+const char* mgldg_last_error(mgldg_ctx ctx);
+int mgldg_ctx_synchronize(mgldg_ctx ctx);
+
+/* ======================= mesh ======================= */
+/* Quadtree<N>(h, strategy, periodic) and Quadtree<N>(domain, h, ...)
+ * (src/cartesian/mesh/quadtree.h:211-228, build quadtree.tpp:8-40).
+ * lo/hi may be NULL for [0,1]^N.  balance=1 additionally enforces 2:1 face
+ * balance of the leaves (not in the reference). */
+int mgldg_quadtree_create(mgldg_ctx ctx, int N, const double* lo, const double* hi,
+                          mgldg_sizefn h, void* user, int strategy, int periodic,
+                          int balance, mgldg_quadtree* out);
+/* info[0]=nodes info[1]=numLevels info[2]=leaves (Quadtree::size/numLevels) */
+int mgldg_quadtree_info(mgldg_quadtree qt, int* info);
+/* Quadtree::layer (quadtree.h:168-178): node ids of layer l in DFS order;
+ * ids may be NULL to query the count. */
+int mgldg_quadtree_layer(mgldg_quadtree qt, int l, int* ids, int* count);
+int mgldg_quadtree_destroy(mgldg_quadtree qt);
+
+/* ======================= assembly ======================= */
+/* Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
+ * (src/cartesian/mesh/mesh.h:186-265, src/cartesian/dg/ldgpoisson.h:60-77,
+ * ldgpoisson.tpp:14-161).  bctype[2N]: type of boundary -(2d+1) (low side of
+ * dimension d) at index 2d and -(2d+2) at 2d+1.  Boundary data g enters
+ * later as values at the boundary quadrature points (mgldg_problem_bc_qpoints). */
+int mgldg_ldg_create(mgldg_quadtree qt, int p, const int* bctype, double tau0, double tauD,
+                     mgldg_problem* out);
+/* sizes[0..6] = ne, nf, nbfaces, npl, nqb (boundary quadrature points), ndof, N */
+int mgldg_problem_sizes(mgldg_problem pr, int* sizes);
+/* Master::dgnodes for every element (master.h:35-43): xyz[ndof*N] */
+int mgldg_problem_nodes(mgldg_problem pr, double* xyz);
+/* boundary quadrature points in boundary-face order (ldgpoisson.tpp:119-129) */
+int mgldg_problem_bc_qpoints(mgldg_problem pr, double* xyz, int* tag);
+/* LDGPoisson::computeRHS (ldgpoisson.tpp:167-176) plus add_source_terms
+ * (:103-161): b = M f + B_g g.  Host buffers; f[ndof*nrhs] nodal values,
+ * g[nqb*nrhs] (either may be NULL = zero). */
+int mgldg_rhs(mgldg_problem pr, const double* f, const double* g, int nrhs, double* b);
+/* LDGOperators dump (ldgpoisson.h:100-109) as sorted CSR of row-major blocks.
+ * which: 0=A 1=M 2=T 3+d=G_d.  Call with NULL arrays to get *nnzb first. */
+int mgldg_problem_export(mgldg_problem pr, int which, int* rowptr, int* cols,
+                         double* blocks, long* nnzb);
+int mgldg_problem_destroy(mgldg_problem pr);
+
+/* ======================= multigrid setup ======================= */
+/* MG::Parameters (multigrid.h:48-62) plus the GPU forms. */
+typedef struct {
+    int relaxation;    /* MGLDG_JACOBI | MGLDG_GAUSS_SEIDEL               */
+    double omega;      /* damping, reference default 0.8                   */
+    int coarse;        /* MGLDG_COARSE_CHOLESKY | MGLDG_COARSE_RELAXATION  */
+    int npre, npost;   /* reference default 3, 3                            */
+    int restriction;   /* MGLDG_RAT | MGLDG_RATRAT                          */
+    int precision;     /* MGLDG_FP64 | MGLDG_MIXED: which V-cycles to build */
+} mgldg_mg_params;
+
+/* InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
+ * (interpolationhierarchy.h:30-82, multigrid.h:291-301, Level 83-171).
+ * The symbolic sparsity of every Galerkin product is formed on the host; the
+ * numeric block triple products, block inversions and colouring-ordered
+ * operator layouts are built on the GPU. */
+int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out);
+/* info[0]=levels, info[1]=ndof, info[2]=npl; then per level l < 16:
+ * info[3+2l]=block rows, info[4+2l]=stored blocks */
+int mgldg_mg_info(mgldg_mg mg, long* info, int ninfo);
+/* Export a level operator (copied back from the GPU): which 0=A 1=M 2=T 3+d=G_d
+ * (RATRAT levels only), 6 = interpolation T_l (rows level l, cols level l+1).
+ * Exported patterns are the host symbolic phase; values come from the GPU. */
+int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols,
+                    double* blocks, long* nnzb, int* nbrows, int* nbcols);
+/* colouring used by the multicolour Gauss-Seidel on level l */
+int mgldg_mg_colors(mgldg_mg mg, int level, int* colors, int* ncolors);
+int mgldg_mg_destroy(mgldg_mg mg);
+
+/* ======================= solve ======================= */
+typedef struct {
+    double tol;        /* reference default 1e-8 (dg_tests.cpp:43)            */
+    int maxit;         /* reference default 200                              */
+    int criterion;     /* MGLDG_CRIT_REFERENCE | MGLDG_CRIT_TRUE              */
+    int flexible;      /* 1: Polak-Ribiere beta (for the FP32 preconditioner) */
+    int precision;     /* MGLDG_FP64 | MGLDG_MIXED                            */
+    int use_graph;     /* 1: replay the V-cycle from a captured hipGraph       */
+} mgldg_solve_params;
+
+typedef struct {
+    int iterations;    /* PCG loop bodies executed (krylov.h:41-65)  */
+    int status;        /* MGLDG_CONVERGED / MAXIT / BREAKDOWN / NONFINITE */
+    double relres_true;/* ||b - A x||_2 / ||b||_2 in FP64 at exit      */
+    double relres_prec;/* sqrt(r^T z / r0^T z0) at exit                */
+    double seconds;    /* device time of the solve                     */
+} mgldg_solve_stats;
+
+/* pcg(A, b, x, precon=V-cycle, tol, maxit) (krylov.h:22-68 with the
+ * preconditioner lambda of tests/cartesian/dg_tests.cpp:300-305).
+ * Device pointers, x is the initial guess on entry. nrhs right-hand sides are
+ * solved together ([ndof][nrhs] layout); stats has nrhs entries. */
+int mgldg_solve_device(mgldg_mg mg, const double* b, double* x, int nrhs,
+                       const mgldg_solve_params* prm, mgldg_solve_stats* stats);
+/* same with host buffers (copies in and out of HBM) */
+int mgldg_solve(mgldg_mg mg, const double* b, double* x, int nrhs,
+                const mgldg_solve_params* prm, mgldg_solve_stats* stats);
+
+/* ======================= adjoint ======================= */
+/* No reference counterpart.  For a loss L(u) with u = A^{-1} b(f, g):
+ * lambda = A^{-1} dL/du by the same MG-PCG (A is symmetric), then
+ * dL/df = M lambda and dL/dg = B_g^T lambda.  Device pointers; any of
+ * lambda/dLdf/dLdg may be NULL.  Layout [ndof][nrhs], dLdg [nqb][nrhs]. */
+int mgldg_adjoint_device(mgldg_mg mg, const double* dLdu, double* lambda, double* dLdf,
+                         double* dLdg, int nrhs, const mgldg_solve_params* prm,
+                         mgldg_solve_stats* stats);
+/* b = M f + B_g g on the GPU (device pointers; f or g may be NULL) */
+int mgldg_rhs_device(mgldg_mg mg, const double* f, const double* g, int nrhs, double* b);
+
+/* ======================= unit-parity hooks ======================= */
+/* y = A_l x on level l (multiply_mv, sparseblockmatrix.tpp:288-301) */
+int mgldg_spmv_device(mgldg_mg mg, int level, int precision, const void* x, void* y, int nrhs);
+/* nsweeps relaxations on level l (Level::relax, multigrid.h:202-216), x in/out */
+int mgldg_smooth_device(mgldg_mg mg, int level, int precision, int nsweeps, int reverse,
+                        const void* b, void* x, int nrhs);
+/* x = V(b) from x = 0 (one preconditioner application, dg_tests.cpp:300-305) */
+int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, int nrhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,370 @@
+// capi.cpp — the extern "C" boundary of libmgldg (declared in include/mgldg.h).
+// No exception crosses it: every entry point maps errors to a status code and
+// records the message on the context.
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "../../include/mgldg.h"
+#include "kernels.hpp"
+#include "mg.hpp"
+
+using namespace mgldg;
+
+struct mgldg_ctx_s { Ctx c; };
+struct mgldg_quadtree_s { Ctx* ctx; Quadtree qt; };
+struct mgldg_problem_s { Ctx* ctx; ProblemH p; };
+struct mgldg_mg_s { Ctx* ctx; MGH m; };
+
+static thread_local std::string g_err;
+
+template <typename F>
+static int guard(Ctx* ctx, F&& f)
+{
+    try {
+        if (ctx) HIPCHK(hipSetDevice(ctx->device));
+        f();
+        return MGLDG_OK;
+    } catch (const Error& e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        (ctx ? ctx->err : g_err) = "host allocation failed";
+        return MGLDG_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return MGLDG_ERR_HIP;
+    }
+}
+
+#define ARG(cond, msg) do { if (!(cond)) throw Error(MGLDG_ERR_ARG, msg); } while (0)
+
+extern "C" {
+
+int mgldg_ctx_create(int device, void* stream, mgldg_ctx* out)
+{
+    if (!out) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    mgldg_ctx h = new (std::nothrow) mgldg_ctx_s();
+    if (!h) return MGLDG_ERR_NOMEM;
+    int st = guard(nullptr, [&] {
+        int n = 0;
+        HIPCHK(hipGetDeviceCount(&n));
+        ARG(device >= 0 && device < n, "no such device");
+        h->c.init(device, stream);
+    });
+    if (st != MGLDG_OK) { delete h; return st; }
+    *out = h;
+    return MGLDG_OK;
+}
+
+int mgldg_ctx_destroy(mgldg_ctx ctx) { delete ctx; return MGLDG_OK; }
+
+const char* mgldg_last_error(mgldg_ctx ctx) { return ctx ? ctx->c.err.c_str() : g_err.c_str(); }
+
+int mgldg_ctx_synchronize(mgldg_ctx ctx)
+{
+    if (!ctx) return MGLDG_ERR_ARG;
+    return guard(&ctx->c, [&] { HIPCHK(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+int mgldg_quadtree_create(mgldg_ctx ctx, int N, const double* lo, const double* hi, mgldg_sizefn h, void* user,
+                          int strategy, int periodic, int balance, mgldg_quadtree* out)
+{
+    if (!ctx || !out) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(&ctx->c, [&] {
+        ARG(N == 2 || N == 3, "dimension must be 2 or 3");
+        ARG(h != nullptr, "size function required");
+        ARG(strategy == MGLDG_EQUAL_COARSENING || strategy == MGLDG_RAPID_COARSENING, "unknown coarsening strategy");
+        if (lo && hi) for (int d = 0; d < N; ++d) ARG(hi[d] > lo[d], "empty domain");
+        auto* q = new mgldg_quadtree_s();
+        q->ctx = &ctx->c;
+        q->qt.N = N;
+        // the reference's domain constructor ignores the strategy (quadtree.h:221-228)
+        q->qt.rapid = (strategy == MGLDG_RAPID_COARSENING) && !(lo || hi);
+        q->qt.periodic = periodic != 0;
+        try {
+            q->qt.build(lo, hi, h, user);
+            if (balance) q->qt.balance();
+        } catch (...) { delete q; throw; }
+        *out = q;
+    });
+}
+
+int mgldg_quadtree_info(mgldg_quadtree qt, int* info)
+{
+    if (!qt || !info) return MGLDG_ERR_ARG;
+    return guard(qt->ctx, [&] {
+        info[0] = (int)qt->qt.nodes.size();
+        info[1] = qt->qt.num_levels;
+        info[2] = (int)qt->qt.layer(0).size();
+    });
+}
+
+int mgldg_quadtree_layer(mgldg_quadtree qt, int l, int* ids, int* count)
+{
+    if (!qt || !count) return MGLDG_ERR_ARG;
+    return guard(qt->ctx, [&] {
+        ARG(l >= 0 && l < qt->qt.num_levels, "layer out of range");
+        std::vector<int> v = qt->qt.layer(l);
+        *count = (int)v.size();
+        if (ids) std::memcpy(ids, v.data(), v.size() * sizeof(int));
+    });
+}
+
+int mgldg_quadtree_destroy(mgldg_quadtree qt) { delete qt; return MGLDG_OK; }
+
+int mgldg_ldg_create(mgldg_quadtree qt, int p, const int* bctype, double tau0, double tauD, mgldg_problem* out)
+{
+    if (!qt || !out) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(qt->ctx, [&] {
+        ARG(p >= 0 && p <= 5, "Invalid polynomial order.");   // dg_tests.cpp:384-385
+        auto* h = new mgldg_problem_s();
+        h->ctx = qt->ctx;
+        try { h->p.build(qt->ctx, &qt->qt, p, bctype, tau0, tauD); }
+        catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+int mgldg_problem_sizes(mgldg_problem pr, int* s)
+{
+    if (!pr || !s) return MGLDG_ERR_ARG;
+    const ProblemH& p = pr->p;
+    s[0] = p.ne; s[1] = p.host.mesh.nf; s[2] = p.host.mesh.nbf; s[3] = p.b; s[4] = p.nqb; s[5] = (int)p.ndof; s[6] = p.N;
+    return MGLDG_OK;
+}
+
+int mgldg_problem_nodes(mgldg_problem pr, double* xyz)
+{
+    if (!pr || !xyz) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] { pr->p.host.nodes_xyz(xyz); });
+}
+
+int mgldg_problem_bc_qpoints(mgldg_problem pr, double* xyz, int* tag)
+{
+    if (!pr || !xyz) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] { pr->p.host.bc_qpoints(xyz, tag); });
+}
+
+int mgldg_rhs(mgldg_problem pr, const double* f, const double* g, int nrhs, double* b)
+{
+    if (!pr || !b || nrhs < 1) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] { pr->p.host.rhs(f, g, nrhs, b); });
+}
+
+static void export_op(Ctx* ctx, const DevOp& op, int* rowptr, int* cols, double* blocks, long* nnzb)
+{
+    if (nnzb) *nnzb = op.nnzb();
+    if (rowptr) std::memcpy(rowptr, op.pat.rowptr.data(), op.pat.rowptr.size() * sizeof(int));
+    if (cols) std::memcpy(cols, op.pat.cols.data(), op.pat.cols.size() * sizeof(int));
+    if (blocks) {
+        op.val.download(blocks, (size_t)op.nnzb() * op.pat.br * op.pat.bc, ctx->stream);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+int mgldg_problem_export(mgldg_problem pr, int which, int* rowptr, int* cols, double* blocks, long* nnzb)
+{
+    if (!pr) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] {
+        ProblemH& p = pr->p;
+        const DevOp* op = which == 0 ? &p.A : which == 1 ? &p.M : which == 2 ? &p.T
+                        : (which >= 3 && which - 3 < p.N) ? &p.G[which - 3] : nullptr;
+        ARG(op, "unknown operator");
+        export_op(pr->ctx, *op, rowptr, cols, blocks, nnzb);
+    });
+}
+
+int mgldg_problem_destroy(mgldg_problem pr) { delete pr; return MGLDG_OK; }
+
+int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out)
+{
+    if (!pr || !prm || !out) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(pr->ctx, [&] {
+        MGParams m;
+        m.relaxation = prm->relaxation; m.omega = prm->omega; m.coarse = prm->coarse;
+        m.npre = prm->npre; m.npost = prm->npost; m.restriction = prm->restriction; m.precision = prm->precision;
+        auto* h = new mgldg_mg_s();
+        h->ctx = pr->ctx;
+        try { h->m.build(&pr->p, m); }
+        catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+int mgldg_mg_info(mgldg_mg mg, long* info, int ninfo)
+{
+    if (!mg || !info || ninfo < 3) return MGLDG_ERR_ARG;
+    const MGH& m = mg->m;
+    info[0] = (long)m.L.size(); info[1] = m.pr->ndof; info[2] = m.pr->b;
+    for (size_t l = 0; l < m.L.size() && 4 + 2 * (int)l < ninfo; ++l) { info[3 + 2 * l] = m.L[l].nb; info[4 + 2 * l] = m.L[l].A.nnzb(); }
+    return MGLDG_OK;
+}
+
+int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols, double* blocks, long* nnzb,
+                    int* nbrows, int* nbcols)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        MGH& m = mg->m;
+        ARG(level >= 0 && level < (int)m.L.size(), "level out of range");
+        Level& lv = m.L[level];
+        if (which == 6) {
+            ARG(level + 1 < (int)m.L.size(), "coarsest level has no interpolation");
+            const Interp& I = lv.I;
+            if (nnzb) *nnzb = I.nf;
+            if (nbrows) *nbrows = I.nf;
+            if (nbcols) *nbcols = I.nc;
+            if (rowptr) for (int i = 0; i <= I.nf; ++i) rowptr[i] = i;
+            if (cols) for (int i = 0; i < I.nf; ++i) cols[i] = I.parent[i];
+            if (blocks) {
+                const size_t bs = (size_t)I.bf * I.bc;
+                for (int i = 0; i < I.nf; ++i) std::memcpy(blocks + i * bs, I.pool.data() + (size_t)I.type[i] * bs, bs * sizeof(double));
+            }
+            return;
+        }
+        const DevOp* op = which == 0 ? &lv.A : nullptr;
+        if (which >= 1 && which <= 5) {
+            ARG(lv.has_ops, "operator only kept on RATRAT levels");
+            op = which == 1 ? &lv.M : which == 2 ? &lv.Tp : (which - 3 < m.pr->N) ? &lv.G[which - 3] : nullptr;
+        }
+        ARG(op, "unknown operator");
+        if (nbrows) *nbrows = op->pat.nbr;
+        if (nbcols) *nbcols = op->pat.nbc;
+        export_op(mg->ctx, *op, rowptr, cols, blocks, nnzb);
+    });
+}
+
+int mgldg_mg_colors(mgldg_mg mg, int level, int* colors, int* ncolors)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        const Level& lv = mg->m.L[level];
+        if (ncolors) *ncolors = lv.ncolor;
+        if (colors) std::memcpy(colors, lv.color.data(), lv.color.size() * sizeof(int));
+    });
+}
+
+int mgldg_mg_destroy(mgldg_mg mg) { delete mg; return MGLDG_OK; }
+
+static SolveParams to_sp(const mgldg_solve_params* p)
+{
+    SolveParams s;
+    if (p) { s.tol = p->tol; s.maxit = p->maxit; s.criterion = p->criterion; s.flexible = p->flexible; s.precision = p->precision; s.use_graph = p->use_graph; }
+    if (s.criterion != 0 && s.criterion != 1) throw Error(MGLDG_ERR_ARG, "unknown stopping criterion");
+    if (s.precision != 0 && s.precision != 1) throw Error(MGLDG_ERR_ARG, "unknown precision");
+    if (s.maxit < 0 || !(s.tol >= 0)) throw Error(MGLDG_ERR_ARG, "invalid tolerance or iteration limit");
+    return s;
+}
+
+static void fill_stats(mgldg_solve_stats* o, const SolveStats& s)
+{
+    o->iterations = s.iterations; o->status = s.status; o->relres_true = s.relres_true;
+    o->relres_prec = s.relres_prec; o->seconds = s.seconds;
+}
+
+int mgldg_solve_device(mgldg_mg mg, const double* b, double* x, int nrhs, const mgldg_solve_params* prm,
+                       mgldg_solve_stats* stats)
+{
+    if (!mg || !b || !x || nrhs < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        SolveParams sp = to_sp(prm);
+        MGH& m = mg->m;
+        const long n = m.pr->ndof;
+        hipStream_t s = mg->ctx->stream;
+        if (nrhs == 1) {
+            SolveStats st = m.solve(b, x, sp);
+            if (stats) fill_stats(stats, st);
+            return;
+        }
+        // columns of the [ndof][nrhs] layout, one after another
+        std::vector<double> hb((size_t)n * nrhs), hx((size_t)n * nrhs), col(n);
+        HIPCHK(hipMemcpyAsync(hb.data(), b, hb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hx.data(), x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int c = 0; c < nrhs; ++c) {
+            for (long i = 0; i < n; ++i) col[i] = hb[(size_t)i * nrhs + c];
+            m.bw.upload(col.data(), n, s);
+            for (long i = 0; i < n; ++i) col[i] = hx[(size_t)i * nrhs + c];
+            m.xw.upload(col.data(), n, s);
+            SolveStats st = m.solve(m.bw, m.xw, sp);
+            if (stats) fill_stats(&stats[c], st);
+            m.xw.download(col.data(), n, s);
+            HIPCHK(hipStreamSynchronize(s));
+            for (long i = 0; i < n; ++i) hx[(size_t)i * nrhs + c] = col[i];
+        }
+        HIPCHK(hipMemcpyAsync(x, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int mgldg_solve(mgldg_mg mg, const double* b, double* x, int nrhs, const mgldg_solve_params* prm, mgldg_solve_stats* stats)
+{
+    if (!mg || !b || !x || nrhs < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        const size_t n = (size_t)mg->m.pr->ndof * nrhs;
+        hipStream_t s = mg->ctx->stream;
+        DBuf<double> db, dx;
+        db.upload(b, n, s); dx.upload(x, n, s);
+        int st = mgldg_solve_device(mg, db, dx, nrhs, prm, stats);
+        if (st != MGLDG_OK) throw Error(st, mg->ctx->err);
+        dx.download(x, n, s);
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int mgldg_adjoint_device(mgldg_mg mg, const double*, double*, double*, double*, int, const mgldg_solve_params*,
+                         mgldg_solve_stats*)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    mg->ctx->err = "adjoint not built yet";
+    return MGLDG_ERR_UNSUPPORTED;
+}
+
+int mgldg_rhs_device(mgldg_mg mg, const double*, const double*, int, double*)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    mg->ctx->err = "device rhs not built yet";
+    return MGLDG_ERR_UNSUPPORTED;
+}
+
+int mgldg_spmv_device(mgldg_mg mg, int level, int precision, const void* x, void* y, int nrhs)
+{
+    if (!mg || !x || !y || nrhs != 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        if (precision == 1) ARG(mg->m.L[level].A32.p, "mixed precision not built");
+        mg->m.spmv(level, precision, x, y);
+        HIPCHK(hipStreamSynchronize(mg->ctx->stream));
+    });
+}
+
+int mgldg_smooth_device(mgldg_mg mg, int level, int precision, int nsweeps, int reverse, const void* b, void* x, int nrhs)
+{
+    if (!mg || !b || !x || nrhs != 1 || nsweeps < 0) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        if (precision == 1) ARG(mg->m.L[level].A32.p, "mixed precision not built");
+        mg->m.smooth(level, precision, nsweeps, reverse != 0, b, x);
+        HIPCHK(hipStreamSynchronize(mg->ctx->stream));
+    });
+}
+
+int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, int nrhs)
+{
+    if (!mg || !b || !x || nrhs != 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        mg->m.vcycle(precision, b, x);
+        HIPCHK(hipStreamSynchronize(mg->ctx->stream));
+    });
+}
+
+}  // extern "C"

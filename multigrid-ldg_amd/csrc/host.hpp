@@ -1,0 +1,168 @@
+// host.hpp — host-side (CPU) part of libmgldg: basis tables, quadtree, mesh,
+// LDG face/element assembly and the symbolic phase of every block-sparse
+// product.  Numeric products, inversions and the solve run on the GPU
+// (kernels.hip, mg.cpp).
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mgldg {
+
+constexpr int kQuad = 10;   // Gauss points per face dimension (src/cartesian/util/common.h:15)
+constexpr int kMaxP = 8;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---------------------------------------------------------------- basis
+// Nodal Gauss-Lobatto tensor basis on [0,1]^N (src/cartesian/dg/master.h:12-250).
+struct Basis {
+    int N = 0, P = 0, npl = 0, nfl = 0;
+    double nodes[kMaxP], denom[kMaxP];
+    double qx[kQuad], qw[kQuad];
+    std::vector<double> mass1, invmass1, diff1;   // P x P
+    std::vector<double> mass, invmass;            // npl x npl
+    std::vector<double> diff[3];                  // npl x npl per dimension
+    std::vector<double> fmass;                    // nfl x nfl ((N-1)-D mass)
+    void init(int N, int P);
+    // LagrangePoly<P>::eval (master.h:78-116)
+    void lagrange1(double x, double* out) const;
+    void lagrangeN(const double* x, double* out) const;
+};
+
+void gl_nodes(int P, double* out);
+void gauss_rule(int Q, double* x, double* w);
+
+// ---------------------------------------------------------------- quadtree
+struct QNode {
+    double lo[3], hi[3];
+    int level, height, parent;
+    int first_child;   // children are first_child .. first_child + 2^N - 1, or -1
+};
+
+// Quadtree<N> (src/cartesian/mesh/quadtree.{h,tpp}).
+struct Quadtree {
+    int N = 2;
+    bool rapid = false, periodic = false;
+    int num_levels = 1;
+    std::vector<QNode> nodes;
+
+    void build(const double* lo, const double* hi, void (*h)(const double*, double*, void*), void* user);
+    void balance();   // 2:1 face balance (not in the reference)
+    bool is_leaf(int id, int c) const;
+    std::vector<int> layer(int c) const;
+    void neighbors(int id, int dim, int dir, int c, std::vector<int>& out) const;
+
+  private:
+    void split(int id);
+    bool touches(const QNode& a, const QNode& b, int dim, int dir) const;
+    bool contains(const QNode& a, const QNode& b) const;
+    void search(int a, int dim, int dir, int id, int c, std::vector<int>& out) const;
+};
+
+// ---------------------------------------------------------------- host BSR
+// Compressed sparse block rows; blocks row-major br x bc.
+struct HostBSR {
+    int nbr = 0, nbc = 0, br = 0, bc = 0;
+    std::vector<int> rowptr, cols;
+    std::vector<double> val;
+    long nnzb() const { return (long)cols.size(); }
+    int find(int i, int j) const;                 // slot or -1
+    double* block(long k) { return val.data() + k * (long)br * bc; }
+    const double* block(long k) const { return val.data() + k * (long)br * bc; }
+};
+
+// Pattern from (row, col) pairs: sorted unique per row.
+struct PatternBuilder {
+    int nbr, nbc;
+    std::vector<std::vector<int>> rows;
+    PatternBuilder(int m, int n) : nbr(m), nbc(n), rows(m) {}
+    void add(int i, int j) { rows[i].push_back(j); }
+    void finish(HostBSR& out, int br, int bc);     // allocates zero values
+};
+
+// ---------------------------------------------------------------- mesh
+struct Face {
+    int left, right, dim;    // right < 0: boundary tag -(2d+1) / -(2d+2)
+    int canonical;
+    double sign;             // normal = sign * e_dim
+    double lo[3], hi[3];
+    int qoff;                // offset into Mesh::face_q (non-canonical interior), else -1
+};
+
+// Mesh<N,P>(qt) (src/cartesian/mesh/mesh.h:186-265)
+struct Mesh {
+    int N = 2, ne = 0, nf = 0, nbf = 0;
+    std::vector<int> gid;                 // element -> quadtree node
+    std::vector<double> lo, hi;           // ne x N
+    std::vector<Face> faces;
+    std::vector<int> bfaces;              // boundary faces in face order
+    std::vector<double> face_q;           // local quadrature points of non-canonical faces (left then right)
+    void build(const Quadtree& qt, const Basis& B);
+    double volume(int e) const;
+    double width(int e, int d) const { return hi[e * N + d] - lo[e * N + d]; }
+};
+
+// Mesh::lift (mesh.h:282-352).  L (npl x nq) may be null; LL/RR/RL npl x npl.
+void lift(const Mesh& m, const Basis& B, const Face& f, double* L, double* LL, double* RR, double* RL);
+
+// ---------------------------------------------------------------- LDG
+struct LdgHost {
+    int N = 2, p = 1, npl = 0, nq = 0;
+    double tau0 = 0, tauD = 0;
+    int bctype[6] = {0, 0, 0, 0, 0, 0};
+    Basis basis;
+    Mesh mesh;
+    HostBSR M, T, G[3];                    // assembled operators (A is formed on the GPU)
+    std::vector<double> Lb;                // boundary lift blocks, nbf x (npl x nq)
+    // LDGPoisson::discretize minus construct_laplacian (ldgpoisson.tpp:14-100)
+    void assemble(const Quadtree& qt, int p, const int* bct, double tau0, double tauD);
+    // b = M f + B_g g (ldgpoisson.tpp:103-176), host
+    void rhs(const double* f, const double* g, int nrhs, double* b) const;
+    void bc_qpoints(double* xyz, int* tag) const;
+    void nodes_xyz(double* xyz) const;
+};
+
+// Symbolic phase of C = sum over terms of opL(L) * X * R.  For each output
+// block: a run of terms (iL, iX, iR); -1 = identity.
+struct TermList {
+    std::vector<int> off;   // nout + 1
+    std::vector<int> l, x, r;
+};
+
+// A = T + sum_d G_d^T M G_d (LDGOperators::construct_laplacian, ldgpoisson.h:34-52):
+// pattern of A and the term lists.  Terms reference: L,R pool = concatenated G_d
+// blocks (offsets goff[d]), X pool = M blocks (one per row), and T blocks
+// (separate run with iL = iR = -1 taken from the T pool).
+struct LaplacianSymbolic {
+    HostBSR pattern;                    // values unused
+    TermList gmg;                       // G^T M G terms (pool: G concat / M)
+    std::vector<int> tslot;             // for each A block: slot in T or -1
+};
+void laplacian_symbolic(const HostBSR& T, const HostBSR* G, int N, const HostBSR& M, LaplacianSymbolic& out);
+
+// One-block-per-row interpolation (InterpolationHierarchy, interpolationhierarchy.h:30-82)
+struct Interp {
+    int nf = 0, nc = 0, bf = 0, bc = 0;
+    std::vector<int> parent;     // fine row -> coarse col
+    std::vector<int> type;       // fine row -> block index in pool
+    std::vector<double> pool;    // ntypes x (bf x bc) row-major
+    int ntypes = 0;
+    // restriction gather: coarse row -> fine rows
+    std::vector<int> kid_off, kids;
+    void make_gather();
+};
+
+// Pattern + terms of C = T^T X T (multiply_mm + multiply_mm_t,
+// sparseblockmatrix.tpp:234-280, with T one block per row).
+void rap_symbolic(const Interp& T, const HostBSR& X, HostBSR& C, TermList& terms);
+
+// greedy colouring in row order
+int greedy_colors(const HostBSR& A, std::vector<int>& color);
+
+}  // namespace mgldg

@@ -1,0 +1,684 @@
+// kernels.hip — CDNA4 (gfx950) kernels of libmgldg.
+//
+// Solve kernels are HBM-bandwidth bound block-sparse streams.  Mapping: one
+// block row is handled by B consecutive lanes of a 64-wide wavefront (lane r
+// owns output row r of the block), so a wavefront holds floor(64/B) block
+// rows and a 256-thread workgroup four wavefronts.  Blocks are stored in a
+// "panel" layout: element (r, c) of a B x B block lives at
+//     ((c / V) * B + r) * V + (c % V)
+// with V = 16 bytes / sizeof(T) when B is a multiple of it (else 2 or 1), so
+// at every step the B lanes of a row read B consecutive 16-byte vectors and
+// the whole wavefront issues one contiguous stream of dwordx4 loads.
+//
+// Reductions (PCG dot products) are deterministic: per-workgroup partials are
+// summed in index order by the last workgroup to finish.
+#include "kernels.hpp"
+
+#include <stdexcept>
+#include <string>
+
+namespace mgldg {
+namespace dev {
+
+// --------------------------------------------------------------------- helpers
+__host__ __device__ constexpr int panel_v(int tsize, int B)
+{
+    return tsize == 8 ? (B % 2 == 0 ? 2 : 1) : (B % 4 == 0 ? 4 : (B % 2 == 0 ? 2 : 1));
+}
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) VecT { T v[V]; };
+
+template <typename T, int V>
+__device__ __forceinline__ VecT<T, V> ldv(const T* p) { return *reinterpret_cast<const VecT<T, V>*>(p); }
+
+template <typename T, int B>
+struct Map {
+    static constexpr int V = panel_v(sizeof(T), B);
+    static constexpr int RPW = 64 / B;        // block rows per wavefront
+    static constexpr int RPB = 4 * RPW;       // block rows per 256-thread workgroup
+};
+
+static void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("kernel launch failed: ") + what + ": " + hipGetErrorString(e));
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// sum over the workgroup (result valid in thread 0); fixed order
+__device__ double block_sum(double v)
+{
+    __shared__ double ws[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += ws[i];
+    return t;
+}
+
+// Deterministic grid reduction: every workgroup calls this with its threads'
+// values.  Returns true in thread 0 of the last workgroup, with *total set.
+__device__ bool grid_sum(double v, double* part, unsigned* counter, double* total)
+{
+    __shared__ bool last;
+    double bs = block_sum(v);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = bs;
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double t = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
+        t += __hip_atomic_load(&part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = block_sum(t);
+    if (threadIdx.x == 0) { *total = t; *counter = 0u; }
+    return threadIdx.x == 0;
+}
+
+// --------------------------------------------------------------------- setup kernels
+#define TP_MAX 32
+__global__ __launch_bounds__(256) void k_triple(int p, int q, int m, int n, int transL,
+                                                const int* __restrict__ off, const int* __restrict__ li,
+                                                const int* __restrict__ xi, const int* __restrict__ ri,
+                                                const double* __restrict__ Lp, const double* __restrict__ Xp,
+                                                const double* __restrict__ Rp, const double* __restrict__ addpool,
+                                                const int* __restrict__ addslot, double* __restrict__ out)
+{
+    __shared__ double Ls[TP_MAX * TP_MAX], Xs[TP_MAX * TP_MAX], Rs[TP_MAX * TP_MAX], Ts[TP_MAX * TP_MAX];
+    const int o = blockIdx.x, tid = threadIdx.x;
+    const int pq = p * q;
+    double acc[4];
+    const int as = addslot ? addslot[o] : -1;
+    for (int u = 0; u < 4; ++u) {
+        int e = tid + u * 256;
+        acc[u] = (e < pq && as >= 0) ? addpool[(size_t)as * pq + e] : 0.0;
+    }
+    const int lm = transL ? m * p : p * m;
+    for (int t = off[o]; t < off[o + 1]; ++t) {
+        const int a = li[t], xx = xi[t], c = ri[t];
+        for (int e = tid; e < lm; e += 256) {
+            if (a >= 0) Ls[e] = Lp[(size_t)a * lm + e];
+            else { int rr = e / (transL ? p : m), cc = e % (transL ? p : m); Ls[e] = (rr == cc) ? 1.0 : 0.0; }
+        }
+        for (int e = tid; e < m * n; e += 256) Xs[e] = Xp[(size_t)xx * m * n + e];
+        for (int e = tid; e < n * q; e += 256) {
+            if (c >= 0) Rs[e] = Rp[(size_t)c * n * q + e];
+            else Rs[e] = (e / q == e % q) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < m * q; e += 256) {
+            int i = e / q, j = e % q; double s = 0;
+            for (int k = 0; k < n; ++k) s += Xs[i * n + k] * Rs[k * q + j];
+            Ts[e] = s;
+        }
+        __syncthreads();
+        for (int u = 0; u < 4; ++u) {
+            int e = tid + u * 256;
+            if (e >= pq) break;
+            int i = e / q, j = e % q; double s = 0;
+            if (transL) for (int k = 0; k < m; ++k) s += Ls[k * p + i] * Ts[k * q + j];
+            else        for (int k = 0; k < m; ++k) s += Ls[i * m + k] * Ts[k * q + j];
+            acc[u] += s;
+        }
+        __syncthreads();
+    }
+    for (int u = 0; u < 4; ++u) {
+        int e = tid + u * 256;
+        if (e < pq) out[(size_t)o * pq + e] = acc[u];
+    }
+}
+
+void triple(hipStream_t s, int nout, int p, int q, int m, int n, int transL, const int* off, const int* l,
+            const int* x, const int* r, const double* Lp, const double* Xp, const double* Rp,
+            const double* addpool, const int* addslot, double* out)
+{
+    if (nout <= 0) return;
+    if (p > TP_MAX || q > TP_MAX || m > TP_MAX || n > TP_MAX) throw std::runtime_error("triple: block larger than 32");
+    hipLaunchKernelGGL(k_triple, dim3(nout), dim3(256), 0, s, p, q, m, n, transL, off, l, x, r, Lp, Xp, Rp, addpool, addslot, out);
+    check_launch("triple");
+}
+
+// Gauss-Jordan inverse with partial pivoting, one workgroup per block.
+__global__ __launch_bounds__(256) void k_inverse(int b, const double* __restrict__ in, const int* __restrict__ slot,
+                                                 double* __restrict__ out)
+{
+    __shared__ double W[TP_MAX * 2 * TP_MAX];
+    __shared__ double fac[TP_MAX];
+    __shared__ int piv;
+    const int blk = blockIdx.x, tid = threadIdx.x, w = 2 * b;
+    const double* A = in + (size_t)(slot ? slot[blk] : blk) * b * b;
+    for (int e = tid; e < b * w; e += 256) {
+        int i = e / w, j = e % w;
+        W[e] = (j < b) ? A[i * b + j] : ((j - b) == i ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int k = 0; k < b; ++k) {
+        if (tid == 0) {
+            int pi = k; double best = fabs(W[k * w + k]);
+            for (int i = k + 1; i < b; ++i) if (fabs(W[i * w + k]) > best) { best = fabs(W[i * w + k]); pi = i; }
+            piv = pi;
+        }
+        __syncthreads();
+        const int pi = piv;
+        if (pi != k) for (int j = tid; j < w; j += 256) { double t = W[k * w + j]; W[k * w + j] = W[pi * w + j]; W[pi * w + j] = t; }
+        __syncthreads();
+        const double d = W[k * w + k];
+        __syncthreads();
+        for (int j = tid; j < w; j += 256) W[k * w + j] = (d != 0.0) ? W[k * w + j] / d : 0.0;
+        for (int i = tid; i < b; i += 256) fac[i] = (i == k) ? 0.0 : W[i * w + k];
+        __syncthreads();
+        for (int e = tid; e < b * w; e += 256) {
+            int i = e / w, j = e % w;
+            if (i != k) W[e] -= fac[i] * W[k * w + j];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < b * b; e += 256) out[(size_t)blk * b * b + e] = W[(e / b) * w + b + (e % b)];
+}
+
+void inverse(hipStream_t s, int nblk, int b, const double* in, const int* slot, double* out)
+{
+    if (nblk <= 0) return;
+    if (b > TP_MAX) throw std::runtime_error("inverse: block larger than 32");
+    hipLaunchKernelGGL(k_inverse, dim3(nblk), dim3(256), 0, s, b, in, slot, out);
+    check_launch("inverse");
+}
+
+template <typename T>
+__global__ void k_to_panel(long nblk, int B, int V, const double* __restrict__ in, T* __restrict__ out)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long bs = (long)B * B;
+    if (e >= nblk * bs) return;
+    long k = e / bs; int rc = (int)(e % bs), r = rc / B, c = rc % B;
+    out[k * bs + ((c / V) * B + r) * V + (c % V)] = (T)in[e];
+}
+
+void to_panel_f64(hipStream_t s, long nblk, int b, const double* in, double* out)
+{
+    long n = nblk * b * b; if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_panel<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nblk, b, panel_v(8, b), in, out);
+    check_launch("to_panel");
+}
+void to_panel_f32(hipStream_t s, long nblk, int b, const double* in, float* out)
+{
+    long n = nblk * b * b; if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_panel<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nblk, b, panel_v(4, b), in, out);
+    check_launch("to_panel");
+}
+
+__global__ void k_gather_blocks(int n, int bs, const int* __restrict__ idx, const double* __restrict__ in, double* __restrict__ out)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * bs) return;
+    long i = e / bs; int t = (int)(e % bs);
+    out[e] = in[(long)idx[i] * bs + t];
+}
+void gather_blocks(hipStream_t s, int n, int bs, const int* idx, const double* in, double* out)
+{
+    long tot = (long)n * bs; if (tot <= 0) return;
+    hipLaunchKernelGGL(k_gather_blocks, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, bs, idx, in, out);
+    check_launch("gather_blocks");
+}
+
+// --------------------------------------------------------------------- solve kernels
+template <typename T, int B>
+__device__ __forceinline__ T row_dot(const T* __restrict__ Ab, const T* __restrict__ xj)
+{
+    constexpr int V = Map<T, B>::V;
+    T acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
+#pragma unroll
+    for (int cc = 0; cc < B / V; ++cc) {
+        VecT<T, V> a = ldv<T, V>(Ab + cc * B * V);
+        VecT<T, V> xv = ldv<T, V>(xj + cc * V);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += a.v[v] * xv.v[v];
+    }
+    T s = acc[0];
+#pragma unroll
+    for (int v = 1; v < V; ++v) s += acc[v];
+    return s;
+}
+
+// y = A x  or  y = b - A x ; optional deterministic reduction of y.dotv
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                               const T* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
+                                               T* __restrict__ y, const T* __restrict__ dotv, double* dot_out,
+                                               double* part, unsigned* counter)
+{
+    using M = Map<T, B>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B;
+    const int row = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    double dv = 0;
+    if (slot < M::RPW && row < nrows) {
+        T acc = 0;
+        const int k1 = rowptr[row + 1];
+        for (int k = rowptr[row]; k < k1; ++k)
+            acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)cols[k] * B);
+        const size_t o = (size_t)row * B + r;
+        T out = b ? (T)(b[o] - acc) : acc;
+        y[o] = out;
+        if (dotv) dv = (double)out * (double)dotv[o];
+    }
+    if (dot_out) grid_sum(dv, part, counter, dot_out);
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_gs(int nrows, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                            const int* __restrict__ cols, const T* __restrict__ A,
+                                            const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
+                                            int zero_guess)
+{
+    using M = Map<T, B>;
+    __shared__ T tb[M::RPB * B];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B, lr = wave * M::RPW + slot;
+    const int idx = blockIdx.x * M::RPB + lr;
+    const bool valid = slot < M::RPW && idx < nrows;
+    const int i = valid ? rows[idx] : 0;
+    if (valid) {
+        T acc = 0;
+        if (!zero_guess) {
+            const int k1 = rowptr[i + 1];
+            for (int k = rowptr[i]; k < k1; ++k) {
+                const int j = cols[k];
+                if (j == i) continue;
+                acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)j * B);
+            }
+        }
+        tb[lr * B + r] = b[(size_t)i * B + r] - acc;
+    }
+    __syncthreads();
+    if (valid) {
+        const T s = row_dot<T, B>(Dinv + (size_t)i * (B * B) + r * M::V, tb + lr * B);
+        const size_t o = (size_t)i * B + r;
+        const T xi = zero_guess ? (T)0 : x[o];
+        x[o] = xi + omega * (s - xi);
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                                const T* __restrict__ A, const T* __restrict__ Dinv,
+                                                const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ xout,
+                                                T omega)
+{
+    using M = Map<T, B>;
+    __shared__ T tb[M::RPB * B];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B, lr = wave * M::RPW + slot;
+    const int i = blockIdx.x * M::RPB + lr;
+    const bool valid = slot < M::RPW && i < nrows;
+    if (valid) {
+        T acc = 0;
+        const int k1 = rowptr[i + 1];
+        for (int k = rowptr[i]; k < k1; ++k)
+            acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)cols[k] * B);
+        tb[lr * B + r] = b[(size_t)i * B + r] - acc;
+    }
+    __syncthreads();
+    if (valid) {
+        const T s = row_dot<T, B>(Dinv + (size_t)i * (B * B) + r * M::V, tb + lr * B);
+        const size_t o = (size_t)i * B + r;
+        xout[o] = x[o] + omega * s;
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_restrict(int nc, const int* __restrict__ kid_off, const int* __restrict__ kids,
+                                                  const int* __restrict__ type, const T* __restrict__ pool,
+                                                  const T* __restrict__ r, T* __restrict__ bc)
+{
+    using M = Map<T, B>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, c = lane - slot * B;
+    const int I = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    if (slot >= M::RPW || I >= nc) return;
+    T acc = 0;
+    for (int kk = kid_off[I]; kk < kid_off[I + 1]; ++kk) {
+        const int k = kids[kk];
+        const T* Tb = pool + (size_t)type[k] * (B * B);
+        const T* rk = r + (size_t)k * B;
+#pragma unroll
+        for (int q = 0; q < B; ++q) acc += Tb[q * B + c] * rk[q];
+    }
+    bc[(size_t)I * B + c] = acc;
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_prolong(int nf, const int* __restrict__ parent, const int* __restrict__ type,
+                                                 const T* __restrict__ pool, const T* __restrict__ xc, T* __restrict__ xf)
+{
+    using M = Map<T, B>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B;
+    const int k = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    if (slot >= M::RPW || k >= nf) return;
+    const T* Tb = pool + (size_t)type[k] * (B * B) + r * B;
+    const T* xp = xc + (size_t)parent[k] * B;
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < B; ++c) acc += Tb[c] * xp[c];
+    xf[(size_t)k * B + r] += acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse(int n, const double* __restrict__ Cinv, const T* __restrict__ b,
+                                                T* __restrict__ x, int subtract_mean)
+{
+    extern __shared__ double bs[];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) { double v = (double)b[i]; bs[i] = v; s += v; }
+    __shared__ double mean;
+    double tot = block_sum(s);
+    if (threadIdx.x == 0) mean = subtract_mean ? tot / n : 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) bs[i] -= mean;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) {
+        double y = 0;
+        for (int j = 0; j < n; ++j) y += Cinv[(size_t)i * n + j] * bs[j];
+        x[i] = (T)y;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_submean(long n, T* b)
+{
+    double s = 0;
+    for (long i = threadIdx.x; i < n; i += 256) s += (double)b[i];
+    __shared__ double mean;
+    double tot = block_sum(s);
+    if (threadIdx.x == 0) mean = tot / (double)n;
+    __syncthreads();
+    for (long i = threadIdx.x; i < n; i += 256) b[i] = (T)((double)b[i] - mean);
+}
+
+template <typename T>
+__global__ void k_fill(long n, T v, T* x) { long i = (long)blockIdx.x * blockDim.x + threadIdx.x; if (i < n) x[i] = v; }
+template <typename T>
+__global__ void k_copy(long n, const T* a, T* b) { long i = (long)blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = a[i]; }
+__global__ void k_d2f(long n, const double* a, float* b) { long i = (long)blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = (float)a[i]; }
+__global__ void k_f2d(long n, const float* a, double* b) { long i = (long)blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = (double)a[i]; }
+
+static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+// dispatch over the compiled block sizes
+#define MGLDG_DISPATCH(B, ...)                                                       \
+    switch (B) {                                                                     \
+        case 1: { constexpr int BB = 1; __VA_ARGS__; } break;                               \
+        case 3: { constexpr int BB = 3; __VA_ARGS__; } break;                               \
+        case 4: { constexpr int BB = 4; __VA_ARGS__; } break;                               \
+        case 6: { constexpr int BB = 6; __VA_ARGS__; } break;                               \
+        case 8: { constexpr int BB = 8; __VA_ARGS__; } break;                               \
+        case 9: { constexpr int BB = 9; __VA_ARGS__; } break;                               \
+        case 10: { constexpr int BB = 10; __VA_ARGS__; } break;                             \
+        case 16: { constexpr int BB = 16; __VA_ARGS__; } break;                             \
+        case 25: { constexpr int BB = 25; __VA_ARGS__; } break;                             \
+        case 27: { constexpr int BB = 27; __VA_ARGS__; } break;                             \
+        case 36: { constexpr int BB = 36; __VA_ARGS__; } break;                             \
+        default: throw std::runtime_error("block size " + std::to_string(B) + " not compiled"); \
+    }
+
+template <typename T>
+void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
+                   const T* b, T* y, const T* dotv, double* dot_out, double* part, unsigned* counter)
+{
+    if (nrows <= 0) return;
+    MGLDG_DISPATCH(B, {
+        using M = Map<T, BB>;
+        hipLaunchKernelGGL((k_apply<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, cols, A, x, b, y,
+                           dotv, dot_out, part, counter);
+    });
+    check_launch("apply");
+}
+
+template <typename T>
+void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* rows, const int* rowptr, const int* cols, const T* A,
+                      const T* Dinv, const T* b, T* x, T omega, int zero_guess)
+{
+    if (nrows <= 0) return;
+    MGLDG_DISPATCH(B, {
+        using M = Map<T, BB>;
+        hipLaunchKernelGGL((k_gs<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rows, rowptr, cols, A, Dinv, b,
+                           x, omega, zero_guess);
+    });
+    check_launch("gs_color");
+}
+
+template <typename T>
+void Ops<T>::jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A, const T* Dinv,
+                    const T* b, const T* x, T* xout, T omega)
+{
+    if (nrows <= 0) return;
+    MGLDG_DISPATCH(B, {
+        using M = Map<T, BB>;
+        hipLaunchKernelGGL((k_jacobi<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, cols, A, Dinv, b, x,
+                           xout, omega);
+    });
+    check_launch("jacobi");
+}
+
+template <typename T>
+void Ops<T>::restrict_(hipStream_t s, int B, int nc, const int* kid_off, const int* kids, const int* type, const T* pool,
+                       const T* r, T* bc)
+{
+    if (nc <= 0) return;
+    MGLDG_DISPATCH(B, {
+        using M = Map<T, BB>;
+        hipLaunchKernelGGL((k_restrict<T, BB>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc, kid_off, kids, type, pool, r, bc);
+    });
+    check_launch("restrict");
+}
+
+template <typename T>
+void Ops<T>::prolong(hipStream_t s, int B, int nf, const int* parent, const int* type, const T* pool, const T* xc, T* xf)
+{
+    if (nf <= 0) return;
+    MGLDG_DISPATCH(B, {
+        using M = Map<T, BB>;
+        hipLaunchKernelGGL((k_prolong<T, BB>), dim3(nblk(nf, M::RPB)), dim3(256), 0, s, nf, parent, type, pool, xc, xf);
+    });
+    check_launch("prolong");
+}
+
+template <typename T>
+void Ops<T>::coarse(hipStream_t s, int n, const double* Cinv, const T* b, T* x, int subtract_mean)
+{
+    if (n > 6144) throw std::runtime_error("coarse: dense coarse problem too large");
+    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(256), n * sizeof(double), s, n, Cinv, b, x, subtract_mean);
+    check_launch("coarse");
+}
+
+template <typename T>
+void Ops<T>::submean(hipStream_t s, long n, T* b)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_submean<T>, dim3(1), dim3(256), 0, s, n, b);
+    check_launch("submean");
+}
+
+template <typename T>
+void Ops<T>::fill(hipStream_t s, long n, T v, T* x)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n, 256)), dim3(256), 0, s, n, v, x);
+    check_launch("fill");
+}
+
+template <typename T>
+void Ops<T>::copy(hipStream_t s, long n, const T* a, T* b)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, s, n, a, b);
+    check_launch("copy");
+}
+
+template struct Ops<float>;
+template struct Ops<double>;
+
+void f64_to_f32(hipStream_t s, long n, const double* a, float* b)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(n, 256)), dim3(256), 0, s, n, a, b);
+    check_launch("f64_to_f32");
+}
+void f32_to_f64(hipStream_t s, long n, const float* a, double* b)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_f2d, dim3(nblk(n, 256)), dim3(256), 0, s, n, a, b);
+    check_launch("f32_to_f64");
+}
+
+// --------------------------------------------------------------------- PCG kernels
+constexpr int kRedBlocks = 1024;   // grid of the reduction kernels (grid-stride)
+
+__global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
+                                             double* out, double* part, unsigned* counter)
+{
+    double s = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) s += a[i] * b[i];
+    grid_sum(s, part, counter, out);
+}
+
+void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part, unsigned* counter)
+{
+    unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, s, n, a, b, out, part, counter);
+    check_launch("dot");
+}
+
+__global__ __launch_bounds__(256) void k_pcg_xr(long n, double* S, const double* __restrict__ d, const double* __restrict__ q,
+                                                double* __restrict__ x, double* __restrict__ r, double* part, unsigned* counter)
+{
+    const double alpha = S[S_DELTA] / S[S_DQ];
+    double s = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        x[i] += alpha * d[i];
+        double ri = r[i] - alpha * q[i];
+        r[i] = ri; s += ri * ri;
+    }
+    grid_sum(s, part, counter, S + S_RR);
+}
+
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, double* part, unsigned* counter)
+{
+    unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(256), 0, s, n, S, d, q, x, r, part, counter);
+    check_launch("pcg_xr");
+}
+
+__global__ void k_pcg_x(long n, const double* S, const double* __restrict__ d, double* __restrict__ x)
+{
+    const double alpha = S[S_DELTA] / S[S_DQ];
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] += alpha * d[i];
+}
+
+void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pcg_x, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, d, x);
+    check_launch("pcg_x");
+}
+
+// three reductions in one pass: r.z, r.zold, r.r ; then beta in the last block
+__global__ __launch_bounds__(256) void k_pcg_rz(long n, double* S, const double* __restrict__ r, const double* __restrict__ z,
+                                                const double* __restrict__ zold, double* part, unsigned* counter)
+{
+    double a = 0, b = 0, c = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double ri = r[i];
+        a += ri * z[i]; c += ri * ri;
+        if (zold) b += ri * zold[i];
+    }
+    // three sequential deterministic grid sums sharing one counter
+    __shared__ bool last;
+    double ba = block_sum(a), bb = block_sum(b), bc = block_sum(c);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = ba; part[3 * blockIdx.x + 1] = bb; part[3 * blockIdx.x + 2] = bc;
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double ta = 0, tb = 0, tc = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += 256) {
+        ta += __hip_atomic_load(&part[3 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tb += __hip_atomic_load(&part[3 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tc += __hip_atomic_load(&part[3 * i + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    ta = block_sum(ta); tb = block_sum(tb); tc = block_sum(tc);
+    if (threadIdx.x == 0) {
+        const double old = S[S_DELTA];
+        S[S_DELTA_OLD] = old;
+        S[S_DELTA] = ta;
+        S[S_RSOLD] = tb;
+        S[S_RR] = tc;
+        S[S_BETA] = zold ? (ta - tb) / old : ta / old;
+        *counter = 0u;
+    }
+}
+
+void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold, double* part, unsigned* counter)
+{
+    unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_pcg_rz, dim3(g), dim3(256), 0, s, n, S, r, z, zold, part, counter);
+    check_launch("pcg_rz");
+}
+
+__global__ void k_pcg_d(long n, const double* S, const double* __restrict__ z, double* __restrict__ d)
+{
+    const double beta = S[S_BETA];
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = z[i] + beta * d[i];
+}
+
+void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pcg_d, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, z, d);
+    check_launch("pcg_d");
+}
+
+__global__ void k_blockdiag(int nb, int B, int nrhs, const double* __restrict__ Mb, const double* __restrict__ x,
+                            double* __restrict__ y)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tot = (long)nb * B * nrhs;
+    if (e >= tot) return;
+    int c = (int)(e % nrhs); long rr = e / nrhs; int r = (int)(rr % B); long i = rr / B;
+    const double* blk = Mb + i * B * B + (long)r * B;
+    double s = 0;
+    for (int k = 0; k < B; ++k) s += blk[k] * x[(i * B + k) * nrhs + c];
+    y[e] = s;
+}
+
+void blockdiag(hipStream_t s, int nb, int B, const double* Mblk, const double* x, double* y, int nrhs)
+{
+    long tot = (long)nb * B * nrhs; if (tot <= 0) return;
+    hipLaunchKernelGGL(k_blockdiag, dim3(nblk(tot, 256)), dim3(256), 0, s, nb, B, nrhs, Mblk, x, y);
+    check_launch("blockdiag");
+}
+
+}  // namespace dev
+}  // namespace mgldg

@@ -1,0 +1,76 @@
+// kernels.hpp — host-callable launchers of the HIP kernels in kernels.hip.
+// All pointers are device pointers; every launcher enqueues on `s` and
+// returns without synchronising.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mgldg {
+namespace dev {
+
+// ---------------------------------------------------------------- setup
+// out[o] = sum_{t in [off[o], off[o+1])} opL(L[l[t]]) * X[x[t]] * R[r[t]]
+// (row-major FP64 blocks; l/r = -1 means identity; opL = transpose if transL).
+// Block shapes: L m x p (transL) or p x m, X m x n, R n x q, out p x q.
+// If addpool != nullptr, out[o] starts from addpool[addslot[o]] (addslot -1 = zero).
+void triple(hipStream_t s, int nout, int p, int q, int m, int n, int transL,
+            const int* off, const int* l, const int* x, const int* r,
+            const double* Lp, const double* Xp, const double* Rp,
+            const double* addpool, const int* addslot, double* out);
+// batched inverse of nblk row-major b x b FP64 blocks (Gauss-Jordan, partial pivoting)
+void inverse(hipStream_t s, int nblk, int b, const double* in, const int* slot, double* out);
+// row-major FP64 blocks -> panel layout of float or double (see kernels.hip)
+void to_panel_f64(hipStream_t s, long nblk, int b, const double* in, double* out);
+void to_panel_f32(hipStream_t s, long nblk, int b, const double* in, float* out);
+// gather diagonal blocks: out[i] = in[diag[i]]
+void gather_blocks(hipStream_t s, int n, int bs, const int* idx, const double* in, double* out);
+
+// ---------------------------------------------------------------- solve (T = float | double)
+template <typename T>
+struct Ops {
+    // y = A x (b == nullptr) or y = b - A x ; if dotv: partial sums of y.dotv
+    // are reduced into *dot_out (FP64) deterministically.
+    static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
+                      const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part, unsigned* counter);
+    // damped block Gauss-Seidel on the rows of one colour (multigrid.h:240-249)
+    static void gs_color(hipStream_t s, int B, int nrows, const int* rows, const int* rowptr, const int* cols,
+                         const T* A, const T* Dinv, const T* b, T* x, T omega, int zero_guess);
+    // damped block Jacobi, x_out = x + omega Dinv (b - A x) (multigrid.h:223-229)
+    static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
+                       const T* Dinv, const T* b, const T* x, T* xout, T omega);
+    // b_c = T^T r (multigrid.h:414-418) as a gather over children
+    static void restrict_(hipStream_t s, int B, int nc, const int* kid_off, const int* kids, const int* type,
+                          const T* pool, const T* r, T* bc);
+    // x_f += T x_c (multigrid.h:421-425)
+    static void prolong(hipStream_t s, int B, int nf, const int* parent, const int* type, const T* pool,
+                        const T* xc, T* xf);
+    // coarsest level: b -= mean(b); x = Cinv b, Cinv FP64 dense n x n (multigrid.h:365-368)
+    static void coarse(hipStream_t s, int n, const double* Cinv, const T* b, T* x, int subtract_mean);
+    static void submean(hipStream_t s, long n, T* b);   // b -= mean(b), one workgroup
+    static void fill(hipStream_t s, long n, T v, T* x);
+    static void copy(hipStream_t s, long n, const T* a, T* b);
+};
+
+void f64_to_f32(hipStream_t s, long n, const double* a, float* b);
+void f32_to_f64(hipStream_t s, long n, const float* a, double* b);
+
+// ---------------------------------------------------------------- PCG (FP64)
+// Scalar slots in a device array S:
+enum { S_DELTA = 0, S_DELTA_OLD, S_DQ, S_ALPHA, S_BETA, S_RR, S_RSOLD, S_DELTA0, S_BB, S_COUNT };
+// a.b reduced into *out
+void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part, unsigned* counter);
+// x += alpha d ; r -= alpha q ; S[S_RR] = r.r      (alpha = S[DELTA]/S[DQ])
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r,
+            double* part, unsigned* counter);
+// x += alpha d only
+void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x);
+// S[DELTA_OLD]=S[DELTA]; S[DELTA] = r.z ; if flexible S[RSOLD] = r.zold ; S[RR] = r.r ; beta
+void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold,
+            double* part, unsigned* counter);
+// d = z + beta d
+void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d);
+// M-block-diagonal apply y = M x (block size B, row-major FP64 blocks)
+void blockdiag(hipStream_t s, int nb, int B, const double* Mblk, const double* x, double* y, int nrhs);
+
+}  // namespace dev
+}  // namespace mgldg

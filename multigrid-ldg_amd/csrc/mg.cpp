@@ -1,0 +1,646 @@
+// mg.cpp — device-resident problem setup, GPU Galerkin hierarchy, V-cycle and
+// MG-preconditioned CG.  Reference structure:
+//   LDGPoisson / LDGOperators       src/cartesian/dg/ldgpoisson.{h,tpp}
+//   InterpolationHierarchy          src/cartesian/multigrid/interpolationhierarchy.h:21-96
+//   Level / Multigrid               src/cartesian/multigrid/multigrid.h:65-433
+//   pcg                             src/cartesian/multigrid/krylov.h:22-68
+#include "mg.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "kernels.hpp"
+
+namespace mgldg {
+
+using Clock = std::chrono::steady_clock;
+static double secs(Clock::time_point a) { return std::chrono::duration<double>(Clock::now() - a).count(); }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Error(-2, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+void DBuf<T>::alloc(size_t count)
+{
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; throw Error(-3, std::string("hipMalloc failed: ") + hipGetErrorString(e)); }
+    n = count;
+}
+template <typename T>
+void DBuf<T>::upload(const T* h, size_t count, hipStream_t s)
+{
+    if (count > n) alloc(count);
+    if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+void DBuf<T>::download(T* h, size_t count, hipStream_t s) const
+{
+    if (count) HIPCHK(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+template <typename T>
+void DBuf<T>::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr; n = 0;
+}
+template struct DBuf<double>;
+template struct DBuf<float>;
+template struct DBuf<int>;
+template struct DBuf<unsigned>;
+
+void Ctx::init(int dev, void* s)
+{
+    device = dev;
+    HIPCHK(hipSetDevice(dev));
+    if (s) { stream = (hipStream_t)s; own_stream = false; }
+    else { HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); own_stream = true; }
+    part.alloc(3 * 1024 + 64);
+    counter.alloc(16);
+    HIPCHK(hipMemsetAsync(counter.p, 0, 16 * sizeof(unsigned), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+}
+
+Ctx::~Ctx()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    part.release(); counter.release();
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void DevOp::upload_pattern(const HostBSR& h, hipStream_t s)
+{
+    pat.nbr = h.nbr; pat.nbc = h.nbc; pat.br = h.br; pat.bc = h.bc;
+    pat.rowptr = h.rowptr; pat.cols = h.cols; pat.val.clear();
+    rowptr.upload(h.rowptr.data(), h.rowptr.size(), s);
+    cols.upload(h.cols.data(), std::max<size_t>(h.cols.size(), 0), s);
+}
+
+// upload a term list and launch the triple-product kernel
+static void run_triple(hipStream_t s, const TermList& t, int p, int q, int m, int n, int transL, const double* Lp,
+                       const double* Xp, const double* Rp, const double* addpool, const std::vector<int>* addslot,
+                       double* out)
+{
+    int nout = (int)t.off.size() - 1;
+    if (nout <= 0) return;
+    DBuf<int> off, l, x, r, as;
+    off.upload(t.off.data(), t.off.size(), s);
+    if (!t.l.empty()) { l.upload(t.l.data(), t.l.size(), s); x.upload(t.x.data(), t.x.size(), s); r.upload(t.r.data(), t.r.size(), s); }
+    if (addslot) as.upload(addslot->data(), addslot->size(), s);
+    dev::triple(s, nout, p, q, m, n, transL, off, l, x, r, Lp, Xp, Rp, addpool, addslot ? as.p : nullptr, out);
+    HIPCHK(hipStreamSynchronize(s));   // term buffers are freed on return
+}
+
+// A = T + sum_d G_d^T M G_d on the GPU (construct_laplacian, ldgpoisson.h:34-52)
+static void build_laplacian(hipStream_t s, int N, int b, DevOp& T, DevOp* G, DevOp& M, DevOp& A)
+{
+    HostBSR Gp[3];
+    for (int d = 0; d < N; ++d) Gp[d] = G[d].pat;
+    LaplacianSymbolic sym;
+    laplacian_symbolic(T.pat, Gp, N, M.pat, sym);
+    A.upload_pattern(sym.pattern, s);
+    A.val.alloc((size_t)A.nnzb() * b * b);
+    // concatenate the G pools so one term list addresses all dimensions
+    long gtot = 0;
+    for (int d = 0; d < N; ++d) gtot += G[d].nnzb();
+    DBuf<double> gcat;
+    gcat.alloc((size_t)gtot * b * b);
+    long off = 0;
+    for (int d = 0; d < N; ++d) {
+        HIPCHK(hipMemcpyAsync(gcat.p + off * b * b, G[d].val.p, (size_t)G[d].nnzb() * b * b * sizeof(double), hipMemcpyDeviceToDevice, s));
+        off += G[d].nnzb();
+    }
+    run_triple(s, sym.gmg, b, b, b, b, 1, gcat, M.val, gcat, T.val, &sym.tslot, A.val);
+}
+
+void ProblemH::build(Ctx* c, const Quadtree* q, int p, const int* bct, double tau0, double tauD)
+{
+    ctx = c; qt = q;
+    auto t0 = Clock::now();
+    host.assemble(*q, p, bct, tau0, tauD);
+    setup_host_s = secs(t0);
+    N = host.N; b = host.npl; ne = host.mesh.ne; ndof = (long)ne * b; nqb = host.mesh.nbf * host.nq;
+    hipStream_t s = ctx->stream;
+    auto t1 = Clock::now();
+    auto up = [&](DevOp& D, const HostBSR& H) { D.upload_pattern(H, s); D.val.upload(H.val.data(), H.val.size(), s); };
+    up(M, host.M); up(T, host.T);
+    for (int d = 0; d < N; ++d) up(G[d], host.G[d]);
+    build_laplacian(s, N, b, T, G, M, A);
+    // boundary data for the device RHS / adjoint
+    Lb.upload(host.Lb.data(), host.Lb.size(), s);
+    std::vector<int> bl(host.mesh.nbf), bd(host.mesh.nbf), bt(host.mesh.nbf);
+    std::vector<double> bs(host.mesh.nbf);
+    for (int k = 0; k < host.mesh.nbf; ++k) {
+        const Face& f = host.mesh.faces[host.mesh.bfaces[k]];
+        bl[k] = f.left; bd[k] = f.dim; bt[k] = host.bctype[-f.right - 1]; bs[k] = f.sign;
+    }
+    bleft.upload(bl.data(), bl.size(), s); bdim.upload(bd.data(), bd.size(), s);
+    btype.upload(bt.data(), bt.size(), s); bsign.upload(bs.data(), bs.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+    setup_dev_s = secs(t1);
+}
+
+// ---------------------------------------------------------------------------- hierarchy
+// InterpolationHierarchy<N,P>(qt) (interpolationhierarchy.h:30-82): one block
+// per fine element, identity if the element persists, else the parent's
+// Lagrange basis evaluated at the child's nodes.  Distinct blocks are pooled.
+static void build_interp(const Quadtree& qt, const Basis& B, int l, Interp& I)
+{
+    std::vector<int> fine = qt.layer(l), coarse = qt.layer(l + 1);
+    std::vector<int> cmap(qt.nodes.size(), -1);
+    for (size_t k = 0; k < coarse.size(); ++k) cmap[coarse[k]] = (int)k;
+    const int b = B.npl, N = qt.N;
+    I.nf = (int)fine.size(); I.nc = (int)coarse.size(); I.bf = b; I.bc = b;
+    I.parent.resize(I.nf); I.type.resize(I.nf);
+    std::map<std::vector<double>, int> seen;
+    I.pool.clear(); I.ntypes = 0;
+    auto add_type = [&](const std::vector<double>& key, const std::vector<double>& blk) {
+        auto it = seen.find(key);
+        if (it != seen.end()) return it->second;
+        I.pool.insert(I.pool.end(), blk.begin(), blk.end());
+        seen[key] = I.ntypes;
+        return I.ntypes++;
+    };
+    std::vector<double> eye((size_t)b * b, 0.0), E((size_t)b * b);
+    for (int i = 0; i < b; ++i) eye[(size_t)i * b + i] = 1.0;
+    for (int f = 0; f < I.nf; ++f) {
+        int g = fine[f];
+        if (qt.is_leaf(g, l + 1)) {
+            I.parent[f] = cmap[g];
+            I.type[f] = add_type(std::vector<double>{-1.0}, eye);
+            continue;
+        }
+        int c = qt.nodes[g].parent;
+        I.parent[f] = cmap[c];
+        std::vector<double> key(2 * N);
+        for (int d = 0; d < N; ++d) {
+            double w = qt.nodes[c].hi[d] - qt.nodes[c].lo[d];
+            key[d] = (qt.nodes[g].lo[d] - qt.nodes[c].lo[d]) / w;
+            key[N + d] = (qt.nodes[g].hi[d] - qt.nodes[c].lo[d]) / w;
+        }
+        if (!seen.count(key)) {
+            int it[3];
+            for (int i = 0; i < b; ++i) {
+                int lin = i;
+                for (int d = N - 1; d >= 0; --d) { it[d] = lin % B.P; lin /= B.P; }
+                double x[3];
+                for (int d = 0; d < N; ++d) x[d] = key[d] + (key[N + d] - key[d]) * B.nodes[it[d]];
+                B.lagrangeN(x, &E[(size_t)i * b]);
+            }
+        }
+        I.type[f] = add_type(key, E);
+    }
+    I.make_gather();
+}
+
+// invert a dense SPD(-semidefinite) matrix on the host by Gauss-Jordan with
+// partial pivoting (zero pivots give zero rows, as LDLT::solve does)
+static std::vector<double> dense_inverse(std::vector<double> A, int n)
+{
+    std::vector<double> X((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) X[(size_t)i * n + i] = 1.0;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i) if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)p * n + k])) p = i;
+        if (p != k) for (int j = 0; j < n; ++j) { std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]); std::swap(X[(size_t)k * n + j], X[(size_t)p * n + j]); }
+        double d = A[(size_t)k * n + k];
+        double inv = (std::fabs(d) > 2.2250738585072014e-308) ? 1.0 / d : 0.0;
+        for (int j = 0; j < n; ++j) { A[(size_t)k * n + j] *= inv; X[(size_t)k * n + j] *= inv; }
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            double f = A[(size_t)i * n + k];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) { A[(size_t)i * n + j] -= f * A[(size_t)k * n + j]; X[(size_t)i * n + j] -= f * X[(size_t)k * n + j]; }
+        }
+    }
+    return X;
+}
+
+static std::vector<int> diag_slots(const HostBSR& P)
+{
+    std::vector<int> d(P.nbr);
+    for (int i = 0; i < P.nbr; ++i) {
+        d[i] = P.find(i, i);
+        if (d[i] < 0) throw Error(-4, "operator has no diagonal block");
+    }
+    return d;
+}
+
+// Level coarse constructor (multigrid.h:105-171): symbolic on the host,
+// numeric block triple products on the GPU
+static void coarsen(hipStream_t s, int N, Level& f, Level& c, int restriction)
+{
+    const Interp& I = f.I;
+    const int b = c.B;
+    if (restriction == 1) {
+        // M_c = T^T M_f T
+        HostBSR pat; TermList t;
+        rap_symbolic(I, f.M.pat, pat, t);
+        c.M.upload_pattern(pat, s);
+        c.M.val.alloc((size_t)pat.nnzb() * b * b);
+        run_triple(s, t, b, b, b, b, 1, f.pool64, f.M.val, f.pool64, nullptr, nullptr, c.M.val);
+        // inverse coarse mass blocks (the reference's Minv LDLT, multigrid.h:133-135)
+        std::vector<int> ds = diag_slots(pat);
+        DBuf<int> dsl; dsl.upload(ds.data(), ds.size(), s);
+        c.Minv.alloc((size_t)c.nb * b * b);
+        dev::inverse(s, c.nb, b, c.M.val, dsl, c.Minv);
+        HIPCHK(hipStreamSynchronize(s));
+        // G_c = M_c^{-1} T^T (M_f G_f) T
+        for (int d = 0; d < N; ++d) {
+            DevOp& Gf = f.G[d];
+            // MG = M_f G_f (block row scaling), pattern of G_f
+            TermList tm; tm.off.resize(Gf.nnzb() + 1); tm.l.assign(Gf.nnzb(), -1); tm.x.resize(Gf.nnzb()); tm.r.resize(Gf.nnzb());
+            std::vector<int> mslot = diag_slots(f.M.pat);
+            for (int i = 0; i < Gf.pat.nbr; ++i)
+                for (int k = Gf.pat.rowptr[i]; k < Gf.pat.rowptr[i + 1]; ++k) { tm.off[k] = k; tm.x[k] = mslot[i]; tm.r[k] = k; }
+            tm.off[Gf.nnzb()] = (int)Gf.nnzb();
+            DBuf<double> MG; MG.alloc((size_t)Gf.nnzb() * b * b);
+            run_triple(s, tm, b, b, b, b, 0, nullptr, f.M.val, Gf.val, nullptr, nullptr, MG);
+            HostBSR gp; TermList ty;
+            rap_symbolic(I, Gf.pat, gp, ty);
+            DBuf<double> Y; Y.alloc((size_t)gp.nnzb() * b * b);
+            run_triple(s, ty, b, b, b, b, 1, f.pool64, MG, f.pool64, nullptr, nullptr, Y);
+            // G_c(i,j) = Minv_c(i) Y(i,j)
+            TermList tg; tg.off.resize(gp.nnzb() + 1); tg.l.resize(gp.nnzb()); tg.x.resize(gp.nnzb()); tg.r.assign(gp.nnzb(), -1);
+            for (int i = 0; i < gp.nbr; ++i)
+                for (int k = gp.rowptr[i]; k < gp.rowptr[i + 1]; ++k) { tg.off[k] = k; tg.l[k] = i; tg.x[k] = k; }
+            tg.off[gp.nnzb()] = (int)gp.nnzb();
+            c.G[d].upload_pattern(gp, s);
+            c.G[d].val.alloc((size_t)gp.nnzb() * b * b);
+            run_triple(s, tg, b, b, b, b, 0, c.Minv, Y, nullptr, nullptr, nullptr, c.G[d].val);
+        }
+        // T_c = T^T T_f T
+        {
+            HostBSR tp; TermList tt;
+            rap_symbolic(I, f.Tp.pat, tp, tt);
+            c.Tp.upload_pattern(tp, s);
+            c.Tp.val.alloc((size_t)tp.nnzb() * b * b);
+            run_triple(s, tt, b, b, b, b, 1, f.pool64, f.Tp.val, f.pool64, nullptr, nullptr, c.Tp.val);
+        }
+        build_laplacian(s, N, b, c.Tp, c.G, c.M, c.A);
+        c.has_ops = true;
+    } else {
+        // A_c = T^T A_f T
+        HostBSR ap; TermList ta;
+        rap_symbolic(I, f.A.pat, ap, ta);
+        c.A.upload_pattern(ap, s);
+        c.A.val.alloc((size_t)ap.nnzb() * b * b);
+        run_triple(s, ta, b, b, b, b, 1, f.pool64, f.A.val, f.pool64, nullptr, nullptr, c.A.val);
+    }
+}
+
+// solve layouts, inverse diagonal blocks, colouring, vectors
+static void finalize_level(hipStream_t s, Level& lv, int precision)
+{
+    const int b = lv.B;
+    const long nnzb = lv.A.nnzb();
+    lv.A64.alloc((size_t)nnzb * b * b);
+    dev::to_panel_f64(s, nnzb, b, lv.A.val, lv.A64);
+    std::vector<int> ds = diag_slots(lv.A.pat);
+    DBuf<int> dsl; dsl.upload(ds.data(), ds.size(), s);
+    lv.Dinv_rm.alloc((size_t)lv.nb * b * b);
+    dev::inverse(s, lv.nb, b, lv.A.val, dsl, lv.Dinv_rm);
+    lv.D64.alloc((size_t)lv.nb * b * b);
+    dev::to_panel_f64(s, lv.nb, b, lv.Dinv_rm, lv.D64);
+    if (precision == 1) {
+        lv.A32.alloc((size_t)nnzb * b * b);
+        dev::to_panel_f32(s, nnzb, b, lv.A.val, lv.A32);
+        lv.D32.alloc((size_t)lv.nb * b * b);
+        dev::to_panel_f32(s, lv.nb, b, lv.Dinv_rm, lv.D32);
+    }
+    lv.ncolor = greedy_colors(lv.A.pat, lv.color);
+    lv.color_off.assign(lv.ncolor + 1, 0);
+    for (int c : lv.color) lv.color_off[c + 1]++;
+    for (int c = 0; c < lv.ncolor; ++c) lv.color_off[c + 1] += lv.color_off[c];
+    std::vector<int> rows(lv.nb), pos(lv.color_off.begin(), lv.color_off.end() - 1);
+    for (int i = 0; i < lv.nb; ++i) rows[pos[lv.color[i]]++] = i;
+    lv.color_rows.upload(rows.data(), rows.size(), s);
+    const size_t n = (size_t)lv.nb * b;
+    lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
+    if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+static void upload_interp(hipStream_t s, Level& lv, int precision)
+{
+    Interp& I = lv.I;
+    lv.parent.upload(I.parent.data(), I.parent.size(), s);
+    lv.type.upload(I.type.data(), I.type.size(), s);
+    lv.kid_off.upload(I.kid_off.data(), I.kid_off.size(), s);
+    lv.kids.upload(I.kids.data(), I.kids.size(), s);
+    lv.pool64.upload(I.pool.data(), I.pool.size(), s);
+    if (precision == 1) {
+        std::vector<float> p32(I.pool.begin(), I.pool.end());
+        lv.pool32.upload(p32.data(), p32.size(), s);
+        HIPCHK(hipStreamSynchronize(s));
+    }
+}
+
+void MGH::build(ProblemH* p, const MGParams& pm)
+{
+    pr = p; ctx = p->ctx; prm = pm;
+    if (prm.relaxation < 0 || prm.relaxation > 1) throw Error(-1, "Unknown relaxation method.");   // multigrid.h:213
+    if (prm.coarse < 0 || prm.coarse > 1) throw Error(-1, "Unknown coarse solver.");              // multigrid.h:193
+    if (prm.restriction < 0 || prm.restriction > 1) throw Error(-1, "Unknown restriction method."); // multigrid.h:164
+    if (prm.precision < 0 || prm.precision > 1) throw Error(-1, "Unknown precision.");
+    if (prm.npre < 0 || prm.npost < 0) throw Error(-1, "negative sweep count");
+    const Quadtree& qt = *p->qt;
+    if (qt.num_levels < 2) throw Error(-1, "the quadtree needs at least two levels");  // interpolationhierarchy.h:33
+    hipStream_t s = ctx->stream;
+    const int nl = qt.num_levels, b = p->b, N = p->N;
+    L.clear(); L.resize(nl);
+    double th = 0;
+    auto t0 = Clock::now();
+    // fine level: the problem's operators (device copies share nothing with the problem)
+    Level& f = L[0];
+    f.nb = p->ne; f.B = b;
+    auto clone = [&](DevOp& dst, const DevOp& src) {
+        dst.upload_pattern(src.pat, s);
+        dst.val.alloc(src.val.n);
+        HIPCHK(hipMemcpyAsync(dst.val.p, src.val.p, src.val.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    };
+    clone(f.A, p->A);
+    if (prm.restriction == 1) {
+        clone(f.M, p->M); clone(f.Tp, p->T);
+        for (int d = 0; d < N; ++d) clone(f.G[d], p->G[d]);
+        f.has_ops = true;
+    }
+    for (int l = 0; l + 1 < nl; ++l) {
+        auto th0 = Clock::now();
+        build_interp(qt, p->host.basis, l, L[l].I);
+        th += secs(th0);
+        upload_interp(s, L[l], prm.precision);
+        L[l + 1].nb = L[l].I.nc; L[l + 1].B = b;
+        coarsen(s, N, L[l], L[l + 1], prm.restriction);
+    }
+    for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision);
+    // coarsest level: dense inverse of the assembled operator (FP64)
+    Level& c = L[nl - 1];
+    if (prm.coarse == 0) {
+        const int n = c.nb * b;
+        if (n > 6144) throw Error(-5, "coarsest level too large for the dense solve");
+        std::vector<double> blk(c.A.val.n), D((size_t)n * n, 0.0);
+        c.A.val.download(blk.data(), blk.size(), s);
+        HIPCHK(hipStreamSynchronize(s));
+        for (int i = 0; i < c.nb; ++i)
+            for (int k = c.A.pat.rowptr[i]; k < c.A.pat.rowptr[i + 1]; ++k) {
+                int j = c.A.pat.cols[k];
+                for (int r = 0; r < b; ++r) for (int q = 0; q < b; ++q)
+                    D[(size_t)(i * b + r) * n + j * b + q] = blk[(size_t)k * b * b + r * b + q];
+            }
+        std::vector<double> X = dense_inverse(D, n);
+        c.Cinv.upload(X.data(), X.size(), s);
+    }
+    // PCG work space
+    const size_t nd = (size_t)p->ndof;
+    r.alloc(nd); z.alloc(nd); zold.alloc(nd); d.alloc(nd); q.alloc(nd); xw.alloc(nd); bw.alloc(nd);
+    S.alloc(16);
+    part.alloc(std::max<size_t>(3 * 1024 + 64, (size_t)L[0].nb + 64));
+    HIPCHK(hipMemsetAsync(S.p, 0, 16 * sizeof(double), s));
+    HIPCHK(hipHostMalloc((void**)&S_host, 16 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipStreamSynchronize(s));
+    setup_dev_s = secs(t0) - th;
+    setup_host_s = th;
+}
+
+MGH::~MGH()
+{
+    for (auto& g : graph) if (g) (void)hipGraphExecDestroy(g);
+    if (S_host) (void)hipHostFree(S_host);
+}
+
+template <> LevelVecs<double>& MGH::vecs<double>(int l) { return L[l].v64; }
+template <> LevelVecs<float>& MGH::vecs<float>(int l) { return L[l].v32; }
+
+template <typename T> static const T* Aop(const Level& lv);
+template <> const double* Aop<double>(const Level& lv) { return lv.A64.p; }
+template <> const float* Aop<float>(const Level& lv) { return lv.A32.p; }
+template <typename T> static const T* Dop(const Level& lv);
+template <> const double* Dop<double>(const Level& lv) { return lv.D64.p; }
+template <> const float* Dop<float>(const Level& lv) { return lv.D32.p; }
+template <typename T> static const T* Pool(const Level& lv);
+template <> const double* Pool<double>(const Level& lv) { return lv.pool64.p; }
+template <> const float* Pool<float>(const Level& lv) { return lv.pool32.p; }
+
+// Level::relax (multigrid.h:202-262); Gauss-Seidel is multicolour on the GPU:
+// colours in increasing order forward, decreasing order in reverse.
+template <typename T>
+void MGH::relax(int l, int n, bool reverse, bool zero_guess)
+{
+    Level& lv = L[l];
+    LevelVecs<T>& v = vecs<T>(l);
+    hipStream_t s = ctx->stream;
+    const T omega = (T)prm.omega;
+    if (Aop<T>(lv) == nullptr) throw Error(-4, "precision not built for this hierarchy");
+    if (prm.relaxation == 1) {
+        for (int sw = 0; sw < n; ++sw)
+            for (int cc = 0; cc < lv.ncolor; ++cc) {
+                int c = reverse ? lv.ncolor - 1 - cc : cc;
+                int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
+                dev::Ops<T>::gs_color(s, lv.B, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv),
+                                      v.b, v.x, omega, (zero_guess && sw == 0 && cc == 0) ? 1 : 0);
+            }
+    } else {
+        // Jacobi ping-pongs between x and xalt; the result ends in x
+        T* cur = v.x.p; T* alt = v.xalt.p;
+        for (int sw = 0; sw < n; ++sw) {
+            dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, cur, alt, omega);
+            std::swap(cur, alt);
+        }
+        if (cur != v.x.p) dev::Ops<T>::copy(s, (long)lv.nb * lv.B, cur, v.x);
+    }
+}
+
+// Multigrid::vcycle (multigrid.h:361-383), enqueued without host syncs
+template <typename T>
+void MGH::vcycle_from(int l0)
+{
+    hipStream_t s = ctx->stream;
+    const int nl = (int)L.size();
+    bool zero = true;   // x of the starting level is zero (preconditioner use)
+    for (int l = l0; l < nl - 1; ++l) {
+        Level& lv = L[l];
+        LevelVecs<T>& v = vecs<T>(l);
+        relax<T>(l, prm.npre, false, zero);
+        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr, nullptr);
+        LevelVecs<T>& vc = vecs<T>(l + 1);
+        dev::Ops<T>::restrict_(s, lv.B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
+        dev::Ops<T>::fill(s, (long)L[l + 1].nb * L[l + 1].B, (T)0, vc.x);
+        zero = true;
+    }
+    Level& c = L[nl - 1];
+    LevelVecs<T>& vc = vecs<T>(nl - 1);
+    if (prm.coarse == 0) {
+        dev::Ops<T>::coarse(s, c.nb * c.B, c.Cinv, vc.b, vc.x, 1);
+    } else {
+        // b -= mean(b) then relaxations from x = 0 (multigrid.h:184-187, 367)
+        dev::Ops<T>::submean(s, (long)c.nb * c.B, vc.b);
+        relax<T>(nl - 1, prm.npre, false, true);
+        relax<T>(nl - 1, prm.npost, true, false);
+    }
+    for (int l = nl - 2; l >= l0; --l) {
+        Level& lv = L[l];
+        LevelVecs<T>& v = vecs<T>(l);
+        dev::Ops<T>::prolong(s, lv.B, lv.nb, lv.parent, lv.type, Pool<T>(lv), vecs<T>(l + 1).x, v.x);
+        relax<T>(l, prm.npost, true, false);
+    }
+}
+
+// z = V(r): enqueue the preconditioner on the stream
+void MGH::precon_enqueue(int precision)
+{
+    hipStream_t s = ctx->stream;
+    const long n = pr->ndof;
+    if (precision == 1) {
+        if (!L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
+        dev::f64_to_f32(s, n, r, L[0].v32.b);
+        dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
+        vcycle_from<float>(0);
+        dev::f32_to_f64(s, n, L[0].v32.x, z);
+    } else {
+        dev::Ops<double>::copy(s, n, r, L[0].v64.b);
+        dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
+        vcycle_from<double>(0);
+        dev::Ops<double>::copy(s, n, L[0].v64.x, z);
+    }
+}
+
+void MGH::precon(int precision, bool use_graph)
+{
+    hipStream_t s = ctx->stream;
+    if (!use_graph) { precon_enqueue(precision); return; }
+    hipGraphExec_t& g = graph[precision];
+    if (!g) {
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try { precon_enqueue(precision); }
+        catch (...) { hipGraph_t junk; (void)hipStreamEndCapture(s, &junk); if (junk) (void)hipGraphDestroy(junk); throw; }
+        HIPCHK(hipStreamEndCapture(s, &gr));
+        HIPCHK(hipGraphInstantiate(&g, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+    }
+    HIPCHK(hipGraphLaunch(g, s));
+}
+
+// pcg (krylov.h:22-68) with the V-cycle preconditioner; FP64 recurrences
+SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
+{
+    hipStream_t s = ctx->stream;
+    Level& f = L[0];
+    const long n = pr->ndof;
+    double* part = this->part.p; unsigned* cnt = ctx->counter;
+    SolveStats st;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    auto fetchS = [&]() {
+        HIPCHK(hipMemcpyAsync(S_host, S.p, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    };
+    auto residual = [&](bool with_rr) {   // r = b - A x  (optionally r.r into S[RR])
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, x, b, r, with_rr ? r.p : nullptr,
+                                with_rr ? S.p + dev::S_RR : nullptr, part, cnt);
+    };
+    // ||b||
+    dev::dot(s, n, b, b, S.p + dev::S_BB, part, cnt);
+    residual(false);
+    precon(sp.precision, sp.use_graph != 0);
+    dev::pcg_rz(s, n, S, r, z, nullptr, part, cnt);        // S[DELTA] = r.z, S[RR] = r.r
+    dev::Ops<double>::copy(s, n, z, d);
+    if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
+    fetchS();
+    const double bnorm = std::sqrt(S_host[dev::S_BB]);
+    const double delta0 = S_host[dev::S_DELTA];
+    double delta = delta0, rr = S_host[dev::S_RR];
+    int i = 0;
+    for (;;) {
+        if (!std::isfinite(delta) || !std::isfinite(rr)) { st.status = 3; break; }
+        bool go = (sp.criterion == 0) ? (delta > sp.tol * sp.tol * delta0) : (std::sqrt(rr) > sp.tol * bnorm);
+        if (!go && sp.criterion == 1 && bnorm > 0) {
+            // confirm on the true residual; restart the direction if it fails
+            residual(true);
+            fetchS();
+            if (std::sqrt(S_host[dev::S_RR]) > sp.tol * bnorm && i < sp.maxit) {
+                precon(sp.precision, sp.use_graph != 0);
+                dev::pcg_rz(s, n, S, r, z, nullptr, part, cnt);
+                dev::Ops<double>::copy(s, n, z, d);
+                if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
+                fetchS();
+                delta = S_host[dev::S_DELTA]; rr = S_host[dev::S_RR];
+                go = true;
+            }
+        }
+        if (!go) { st.status = 0; break; }
+        if (i >= sp.maxit) { st.status = 1; break; }
+        // q = A d, S[DQ] = d.q
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, d, nullptr, q, d, S.p + dev::S_DQ, part, cnt);
+        if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
+            dev::pcg_x(s, n, S, d, x);
+            residual(true);
+        } else {
+            dev::pcg_xr(s, n, S, d, q, x, r, part, cnt);
+        }
+        precon(sp.precision, sp.use_graph != 0);
+        dev::pcg_rz(s, n, S, r, z, sp.flexible ? zold.p : nullptr, part, cnt);
+        if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
+        dev::pcg_d(s, n, S, z, d);
+        ++i;
+        fetchS();
+        if (!(S_host[dev::S_DQ] > 0)) { st.status = std::isfinite(S_host[dev::S_DQ]) ? 2 : 3; break; }
+        delta = S_host[dev::S_DELTA]; rr = S_host[dev::S_RR];
+    }
+    // true residual at exit
+    residual(true);
+    HIPCHK(hipEventRecord(e1, s));
+    fetchS();
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    st.iterations = i;
+    st.relres_true = bnorm > 0 ? std::sqrt(S_host[dev::S_RR]) / bnorm : 0.0;
+    st.relres_prec = delta0 > 0 ? std::sqrt(std::fabs(delta / delta0)) : 0.0;
+    st.seconds = ms * 1e-3;
+    if (st.status == 0 && sp.criterion == 1 && st.relres_true > sp.tol) st.status = 1;
+    return st;
+}
+
+void MGH::spmv(int l, int precision, const void* x, void* y)
+{
+    Level& lv = L[l];
+    hipStream_t s = ctx->stream;
+    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr, nullptr);
+    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr, nullptr);
+}
+
+void MGH::smooth(int l, int precision, int n, bool reverse, const void* b, void* x)
+{
+    Level& lv = L[l];
+    hipStream_t s = ctx->stream;
+    const long m = (long)lv.nb * lv.B;
+    if (precision == 1) {
+        dev::Ops<float>::copy(s, m, (const float*)b, lv.v32.b); dev::Ops<float>::copy(s, m, (const float*)x, lv.v32.x);
+        relax<float>(l, n, reverse, false);
+        dev::Ops<float>::copy(s, m, lv.v32.x, (float*)x);
+    } else {
+        dev::Ops<double>::copy(s, m, (const double*)b, lv.v64.b); dev::Ops<double>::copy(s, m, (const double*)x, lv.v64.x);
+        relax<double>(l, n, reverse, false);
+        dev::Ops<double>::copy(s, m, lv.v64.x, (double*)x);
+    }
+}
+
+void MGH::vcycle(int precision, const double* b, double* x)
+{
+    hipStream_t s = ctx->stream;
+    const long n = pr->ndof;
+    dev::Ops<double>::copy(s, n, b, r);
+    precon(precision, false);
+    dev::Ops<double>::copy(s, n, z, x);
+}
+
+}  // namespace mgldg

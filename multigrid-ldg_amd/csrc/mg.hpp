@@ -1,0 +1,149 @@
+// mg.hpp — device-resident LDG problem, multigrid hierarchy and MG-PCG driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+
+namespace mgldg {
+
+void hip_check(hipError_t e, const char* what);
+#define HIPCHK(x) ::mgldg::hip_check((x), #x)
+
+// owning device buffer
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept { if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; } return *this; }
+    ~DBuf() { release(); }
+    void alloc(size_t count);
+    void upload(const T* h, size_t count, hipStream_t s);
+    void download(T* h, size_t count, hipStream_t s) const;
+    void release();
+    operator T*() const { return p; }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    DBuf<double> part;       // reduction partials
+    DBuf<unsigned> counter;  // reduction counter (kept at zero between kernels)
+    void init(int dev, void* stream);
+    ~Ctx();
+};
+
+// a block-sparse operator with host pattern and device row-major FP64 values
+struct DevOp {
+    HostBSR pat;             // pattern (values empty)
+    DBuf<int> rowptr, cols;
+    DBuf<double> val;        // nnzb x br x bc row-major
+    void upload_pattern(const HostBSR& h, hipStream_t s);
+    long nnzb() const { return pat.nnzb(); }
+};
+
+struct QuadtreeH {
+    Ctx* ctx;
+    Quadtree qt;
+};
+
+struct ProblemH {
+    Ctx* ctx;
+    const Quadtree* qt;                // kept alive by the caller (as the reference's raw pointers)
+    LdgHost host;
+    DevOp M, T, G[3], A;
+    DBuf<double> Lb;                   // boundary lifting blocks (npl x nq per boundary face)
+    DBuf<int> bleft, bdim, btype;      // per boundary face: left element, dim, bc type
+    DBuf<double> bsign;
+    int N = 2, b = 0, ne = 0, nqb = 0;
+    long ndof = 0;
+    double setup_host_s = 0, setup_dev_s = 0;
+    void build(Ctx* c, const Quadtree* q, int p, const int* bct, double tau0, double tauD);
+};
+
+struct MGParams {
+    int relaxation = 1;
+    double omega = 0.8;
+    int coarse = 0;
+    int npre = 3, npost = 3;
+    int restriction = 1;
+    int precision = 0;   // 0: FP64 only, 1: also FP32 copies for the mixed V-cycle
+};
+
+template <typename T>
+struct LevelVecs {
+    DBuf<T> x, b, r, xalt;
+};
+
+struct Level {
+    int nb = 0, B = 0;
+    DevOp A;                            // row-major FP64 values (setup / export)
+    DevOp M, Tp, G[3];                  // RATRAT operators (when present)
+    bool has_ops = false;
+    DBuf<double> A64, D64, Dinv_rm;     // panel layouts and row-major inverse
+    DBuf<float> A32, D32;
+    DBuf<double> Minv;                  // inverse coarse mass blocks (RATRAT)
+    std::vector<int> color, color_off;  // colouring and rows grouped by colour
+    int ncolor = 0;
+    DBuf<int> color_rows;
+    LevelVecs<double> v64;
+    LevelVecs<float> v32;
+    // interpolation to the next coarser level
+    Interp I;
+    DBuf<int> parent, type, kid_off, kids;
+    DBuf<double> pool64;
+    DBuf<float> pool32;
+    DBuf<double> Cinv;                  // coarsest: dense inverse (n x n)
+};
+
+struct SolveParams {
+    double tol = 1e-8;
+    int maxit = 200;
+    int criterion = 0;
+    int flexible = 0;
+    int precision = 0;
+    int use_graph = 0;
+};
+
+struct SolveStats {
+    int iterations = 0, status = 0;
+    double relres_true = 0, relres_prec = 0, seconds = 0;
+};
+
+struct MGH {
+    Ctx* ctx;
+    ProblemH* pr;
+    MGParams prm;
+    std::vector<Level> L;
+    // PCG work vectors (FP64) and scalars
+    DBuf<double> r, z, zold, d, q, S, xw, bw;
+    DBuf<double> part;                 // reduction partials (one per workgroup of the largest launch)
+    double* S_host = nullptr;          // pinned
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double setup_host_s = 0, setup_dev_s = 0;
+
+    void build(ProblemH* p, const MGParams& prm);
+    ~MGH();
+
+    template <typename T> void relax(int l, int n, bool reverse, bool zero_guess);
+    template <typename T> void vcycle_from(int l);
+    template <typename T> LevelVecs<T>& vecs(int l);
+    void precon(int precision, bool use_graph);           // z = V(r)
+    void precon_enqueue(int precision);
+    SolveStats solve(const double* b, double* x, const SolveParams& sp);   // device, one RHS
+    void spmv(int l, int precision, const void* x, void* y);
+    void smooth(int l, int precision, int n, bool reverse, const void* b, void* x);
+    void vcycle(int precision, const double* b, double* x);
+};
+
+}  // namespace mgldg

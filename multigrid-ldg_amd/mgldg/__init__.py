@@ -1,0 +1,234 @@
+"""mgldg — MI355X-native LDG-Poisson setup + MG-preconditioned CG.
+
+Python mirror of the reference's C++ surface (danfortunato/multigrid-ldg):
+
+    reference (src/cartesian/...)                  here
+    Quadtree<N>(h, strategy, periodic)             Quadtree(ctx, N, h, coarsening=..., periodic=...)
+    Mesh<N,P>(qt) + LDGPoisson<N,P>(mesh, bcs,     LDGPoisson(qt, p, bc=..., tau0=..., tauD=...)
+        tau0, tauD), computeRHS(f)                     .rhs(f, g)
+    InterpolationHierarchy<N,P>(qt) +              Multigrid(prob, relaxation=..., omega=..., npre=...,
+        Multigrid<N,P>(ops, hierarchy, params)         npost=..., restriction=..., precision=...)
+    pcg(A, b, x, precon, tol, maxit)               pcg(mg, b, x0, tol, maxit, ...)
+
+Everything above the ctypes calls is argument plumbing; the work happens in
+libmgldg.so (host C++ + HIP kernels for gfx950).  There is no CPU fallback:
+if the library or a GPU is missing, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._lib import lib, LIB_PATH, MGParams, SolveParams, SolveStats, SIZEFN, check, MGLDGError  # noqa: F401
+
+STATUS = {0: "converged", 1: "maxit", 2: "breakdown", 3: "nonfinite"}
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pi(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Context:
+    """One device + one HIP stream (no reference counterpart)."""
+
+    def __init__(self, device=0, stream=None):
+        self.h = C.c_void_p()
+        check(None, lib().mgldg_ctx_create(device, stream, C.byref(self.h)))
+        self.device = device
+
+    def sync(self):
+        check(self, lib().mgldg_ctx_synchronize(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mgldg_ctx_destroy(self.h)
+            self.h = None
+
+
+class Quadtree:
+    """Quadtree<N>(h, strategy, periodic) (src/cartesian/mesh/quadtree.h:211-228).
+
+    h(x) -> widths (scalar or N values): a cell splits iff every width exceeds
+    h(center) (quadtree.tpp:15).  balance=True enforces 2:1 leaf balance."""
+
+    def __init__(self, ctx, N, h, lo=None, hi=None, coarsening="equal", periodic=False, balance=False):
+        self.ctx, self.N = ctx, N
+
+        def cb(x, out, user):
+            v = np.broadcast_to(np.asarray(h(np.array([x[i] for i in range(N)])), dtype=np.float64), (N,))
+            for i in range(N):
+                out[i] = float(v[i])
+
+        self._cb = SIZEFN(cb)
+        lo_a = None if lo is None else np.ascontiguousarray(lo, dtype=np.float64)
+        hi_a = None if hi is None else np.ascontiguousarray(hi, dtype=np.float64)
+        self.h = C.c_void_p()
+        strat = {"equal": 0, "rapid": 1}[coarsening]
+        check(ctx, lib().mgldg_quadtree_create(ctx.h, N, _pd(lo_a), _pd(hi_a), self._cb, None, strat,
+                                               int(periodic), int(balance), C.byref(self.h)))
+        info = np.zeros(3, dtype=np.int32)
+        check(ctx, lib().mgldg_quadtree_info(self.h, _pi(info)))
+        self.nnodes, self.num_levels, self.nleaves = [int(v) for v in info]
+
+    def layer(self, l):
+        n = C.c_int()
+        check(self.ctx, lib().mgldg_quadtree_layer(self.h, l, None, C.byref(n)))
+        ids = np.zeros(n.value, dtype=np.int32)
+        check(self.ctx, lib().mgldg_quadtree_layer(self.h, l, _pi(ids), C.byref(n)))
+        return ids
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mgldg_quadtree_destroy(self.h)
+            self.h = None
+
+
+class LDGPoisson:
+    """Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
+    (src/cartesian/mesh/mesh.h:186-265, src/cartesian/dg/ldgpoisson.h:60-77).
+
+    bc: 'dirichlet' | 'neumann' | list of 2N types per boundary tag."""
+
+    OPS = {"A": 0, "M": 1, "T": 2, "G0": 3, "G1": 4, "G2": 5}
+
+    def __init__(self, qt, p, bc="dirichlet", tau0=0.0, tauD=1000.0):
+        self.qt, self.ctx, self.p = qt, qt.ctx, p
+        if isinstance(bc, str):
+            bc = [{"dirichlet": 0, "neumann": 1}[bc]] * (2 * qt.N)
+        bct = np.ascontiguousarray(bc, dtype=np.int32)
+        self.h = C.c_void_p()
+        check(self.ctx, lib().mgldg_ldg_create(qt.h, p, _pi(bct), tau0, tauD, C.byref(self.h)))
+        s = np.zeros(7, dtype=np.int32)
+        check(self.ctx, lib().mgldg_problem_sizes(self.h, _pi(s)))
+        self.ne, self.nf, self.nbf, self.npl, self.nqb, self.ndof, self.N = [int(v) for v in s]
+
+    def nodes(self):
+        x = np.zeros((self.ndof, self.N))
+        check(self.ctx, lib().mgldg_problem_nodes(self.h, _pd(x)))
+        return x
+
+    def bc_qpoints(self):
+        x = np.zeros((self.nqb, self.N))
+        t = np.zeros(self.nqb, dtype=np.int32)
+        check(self.ctx, lib().mgldg_problem_bc_qpoints(self.h, _pd(x), _pi(t)))
+        return x, t
+
+    def rhs(self, f=None, g=None):
+        """computeRHS (ldgpoisson.tpp:167-176): b = M f + B_g g; f [ndof(,k)], g [nqb(,k)]"""
+        arrs = [a for a in (f, g) if a is not None]
+        k = 1 if not arrs or np.ndim(arrs[0]) == 1 else np.shape(arrs[0])[1]
+        f_a = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        g_a = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        b = np.zeros((self.ndof, k)) if k > 1 else np.zeros(self.ndof)
+        check(self.ctx, lib().mgldg_rhs(self.h, _pd(f_a), _pd(g_a), k, _pd(b)))
+        return b
+
+    def export(self, which):
+        """(rowptr, cols, blocks[nnzb, npl, npl]) of A, M, T or G_d"""
+        w = self.OPS[which]
+        nz = C.c_long()
+        check(self.ctx, lib().mgldg_problem_export(self.h, w, None, None, None, C.byref(nz)))
+        rp = np.zeros(self.ne + 1, dtype=np.int32)
+        cols = np.zeros(nz.value, dtype=np.int32)
+        blk = np.zeros((nz.value, self.npl, self.npl))
+        check(self.ctx, lib().mgldg_problem_export(self.h, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz)))
+        return rp, cols, blk
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mgldg_problem_destroy(self.h)
+            self.h = None
+
+
+class Multigrid:
+    """InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
+    (src/cartesian/multigrid/interpolationhierarchy.h:30-82, multigrid.h:291-301).
+
+    precision='fp64' builds the FP64 V-cycle only; 'mixed' also builds the FP32
+    copies used by the mixed-precision V-cycle."""
+
+    def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
+                 restriction="ratrat", precision="fp64"):
+        self.prob, self.ctx = prob, prob.ctx
+        self.params = MGParams({"jacobi": 0, "gs": 1}[relaxation], omega,
+                               {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
+                               {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision])
+        self.h = C.c_void_p()
+        check(self.ctx, lib().mgldg_mg_create(prob.h, C.byref(self.params), C.byref(self.h)))
+        info = np.zeros(3 + 2 * 32, dtype=np.int64)
+        check(self.ctx, lib().mgldg_mg_info(self.h, info.ctypes.data_as(C.POINTER(C.c_long)), len(info)))
+        self.nlevels = int(info[0])
+        self.level_rows = [int(info[3 + 2 * l]) for l in range(self.nlevels)]
+        self.level_nnzb = [int(info[4 + 2 * l]) for l in range(self.nlevels)]
+
+    def export(self, level, which):
+        w = {"A": 0, "M": 1, "T": 2, "G0": 3, "G1": 4, "G2": 5, "interp": 6}[which]
+        nz, nbr, nbc = C.c_long(), C.c_int(), C.c_int()
+        check(self.ctx, lib().mgldg_mg_export(self.h, level, w, None, None, None, C.byref(nz), C.byref(nbr), C.byref(nbc)))
+        rp = np.zeros(nbr.value + 1, dtype=np.int32)
+        cols = np.zeros(nz.value, dtype=np.int32)
+        b = self.prob.npl
+        blk = np.zeros((nz.value, b, b))
+        check(self.ctx, lib().mgldg_mg_export(self.h, level, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz), None, None))
+        return rp, cols, blk, nbr.value, nbc.value
+
+    def colors(self, level):
+        nc = C.c_int()
+        c = np.zeros(self.level_rows[level], dtype=np.int32)
+        check(self.ctx, lib().mgldg_mg_colors(self.h, level, _pi(c), C.byref(nc)))
+        return c, nc.value
+
+    # unit-parity hooks on device tensors (torch) -----------------------------
+    def spmv(self, level, x, y, precision="fp64"):
+        check(self.ctx, lib().mgldg_spmv_device(self.h, level, _prec(precision), _ptr(x), _ptr(y), 1))
+
+    def smooth(self, level, b, x, nsweeps=1, reverse=False, precision="fp64"):
+        check(self.ctx, lib().mgldg_smooth_device(self.h, level, _prec(precision), nsweeps, int(reverse), _ptr(b), _ptr(x), 1))
+
+    def vcycle(self, b, x, precision="fp64"):
+        check(self.ctx, lib().mgldg_vcycle_device(self.h, _prec(precision), _ptr(b), _ptr(x), 1))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mgldg_mg_destroy(self.h)
+            self.h = None
+
+
+def _prec(p):
+    return {"fp64": 0, "mixed": 1, "fp32": 1}[p]
+
+
+def _ptr(t):
+    """device pointer of a torch tensor (contiguous, on the GPU)"""
+    if not hasattr(t, "data_ptr") or not t.is_cuda or not t.is_contiguous():
+        raise MGLDGError(-1, "expected a contiguous device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def pcg(mg, b, x0=None, tol=1e-8, maxit=200, criterion="reference", flexible=False, precision="fp64",
+        use_graph=False):
+    """pcg(A, b, x, precon=V-cycle, tol, maxit) (src/cartesian/multigrid/krylov.h:22-68).
+
+    b: numpy array (host path) or torch device tensor (device path, x0 updated
+    in place).  Returns (x, stats dict)."""
+    sp = SolveParams(tol, maxit, {"reference": 0, "true": 1}[criterion], int(flexible), _prec(precision),
+                     int(use_graph))
+    is_dev = hasattr(b, "data_ptr")
+    nrhs = 1 if b.ndim == 1 else b.shape[1]
+    st = (SolveStats * nrhs)()
+    if is_dev:
+        import torch
+        x = torch.zeros_like(b) if x0 is None else x0
+        torch.cuda.synchronize()
+        check(mg.ctx, lib().mgldg_solve_device(mg.h, _ptr(b), _ptr(x), nrhs, C.byref(sp), st))
+    else:
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(bb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        check(mg.ctx, lib().mgldg_solve(mg.h, _pd(bb), _pd(x), nrhs, C.byref(sp), st))
+    stats = [dict(iterations=s.iterations, status=STATUS.get(s.status, s.status), relres_true=s.relres_true,
+                  relres_prec=s.relres_prec, seconds=s.seconds) for s in st]
+    return x, (stats[0] if nrhs == 1 else stats)
