@@ -194,6 +194,22 @@ int mgldg_smooth_device(mgldg_mg mg, int level, int precision, int nsweeps, int 
 /* x = V(b) from x = 0 (one preconditioner application, dg_tests.cpp:300-305) */
 int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, int nrhs);
 
+/* ======================= measurement hooks ======================= */
+/* No reference counterpart (the reference only prints a Timer).  Times
+ * `nrep` smoother sweeps (all colours) or SpMVs on level l with HIP events on
+ * the context stream and reports the average duration of one kernel launch
+ * (ms), the launches timed and the algorithmic HBM bytes one launch moves:
+ *   GS colour launch: off-diagonal blocks + inverse diagonal blocks + column
+ *   and row indices + b, x (read) and x (write) of its rows, per colour.
+ *   SpMV: all blocks + indices + x (read) + y (write). */
+int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double* ms_per_launch,
+                        double* bytes_per_launch, int* launches);
+int mgldg_profile_spmv(mgldg_mg mg, int level, int precision, int nrep, double* ms_per_launch,
+                       double* bytes_per_launch);
+/* setup timings: t[0] host assembly, t[1] device assembly (A), t[2] hierarchy
+ * host symbolic, t[3] hierarchy device numeric (seconds) */
+int mgldg_setup_times(mgldg_mg mg, double* t);
+
 #ifdef __cplusplus
 }
 #endif

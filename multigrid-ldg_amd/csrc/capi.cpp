@@ -367,4 +367,33 @@ int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, 
     });
 }
 
+int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double* ms, double* bytes, int* launches)
+{
+    if (!mg || nrep < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        if (precision == 1) ARG(mg->m.L[level].A32.p, "mixed precision not built");
+        mg->m.profile_relax(level, precision, nrep, ms, bytes, launches);
+    });
+}
+
+int mgldg_profile_spmv(mgldg_mg mg, int level, int precision, int nrep, double* ms, double* bytes)
+{
+    if (!mg || nrep < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        if (precision == 1) ARG(mg->m.L[level].A32.p, "mixed precision not built");
+        mg->m.profile_spmv(level, precision, nrep, ms, bytes);
+    });
+}
+
+int mgldg_setup_times(mgldg_mg mg, double* t)
+{
+    if (!mg || !t) return MGLDG_ERR_ARG;
+    t[0] = mg->m.pr->setup_host_s; t[1] = mg->m.pr->setup_dev_s; t[2] = mg->m.setup_host_s; t[3] = mg->m.setup_dev_s;
+    return MGLDG_OK;
+}
+
 }  // extern "C"

@@ -643,4 +643,61 @@ void MGH::vcycle(int precision, const double* b, double* x)
     dev::Ops<double>::copy(s, n, z, x);
 }
 
+// ---------------------------------------------------------------------------- measurement
+void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches)
+{
+    Level& lv = L[l];
+    hipStream_t s = ctx->stream;
+    const long n = (long)lv.nb * lv.B;
+    const int sa = precision == 1 ? 4 : 8, sv = sa;
+    if (precision == 1) { dev::Ops<float>::fill(s, n, 1.f, lv.v32.b); dev::Ops<float>::fill(s, n, 0.f, lv.v32.x); }
+    else { dev::Ops<double>::fill(s, n, 1.0, lv.v64.b); dev::Ops<double>::fill(s, n, 0.0, lv.v64.x); }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    // one warm sweep, then the timed ones
+    if (precision == 1) relax<float>(l, 1, false, false); else relax<double>(l, 1, false, false);
+    HIPCHK(hipEventRecord(e0, s));
+    if (precision == 1) relax<float>(l, nrep, false, false); else relax<double>(l, nrep, false, false);
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    const int per = prm.relaxation == 1 ? lv.ncolor : 1;
+    const double b2 = (double)lv.B * lv.B;
+    double by;
+    if (prm.relaxation == 1)   // off-diagonal blocks + Dinv + cols + rowptr + colour rows + b, x in, x out
+        by = (double)(lv.A.nnzb() - lv.nb) * b2 * sa + lv.nb * b2 * sa + 4.0 * lv.A.nnzb() + 8.0 * lv.nb + 4.0 * lv.nb + 3.0 * n * sv;
+    else                       // all blocks + Dinv + indices + b, x in, x out
+        by = (double)lv.A.nnzb() * b2 * sa + lv.nb * b2 * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 3.0 * n * sv;
+    *ms = t / (double)(nrep * per);
+    *bytes = by / per;
+    if (launches) *launches = nrep * per;
+}
+
+void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes)
+{
+    Level& lv = L[l];
+    hipStream_t s = ctx->stream;
+    const long n = (long)lv.nb * lv.B;
+    const int sa = precision == 1 ? 4 : 8;
+    if (precision == 1) dev::Ops<float>::fill(s, n, 1.f, lv.v32.x); else dev::Ops<double>::fill(s, n, 1.0, lv.v64.x);
+    auto once = [&] {
+        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr, nullptr);
+        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr, nullptr);
+    };
+    once();
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < nrep; ++i) once();
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    *ms = t / nrep;
+    *bytes = (double)lv.A.nnzb() * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 2.0 * n * sa;
+}
+
 }  // namespace mgldg

@@ -144,6 +144,8 @@ struct MGH {
     void spmv(int l, int precision, const void* x, void* y);
     void smooth(int l, int precision, int n, bool reverse, const void* b, void* x);
     void vcycle(int precision, const double* b, double* x);
+    void profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches);
+    void profile_spmv(int l, int precision, int nrep, double* ms, double* bytes);
 };
 
 }  // namespace mgldg

@@ -24,6 +24,16 @@ from ._lib import lib, LIB_PATH, MGParams, SolveParams, SolveStats, SIZEFN, chec
 STATUS = {0: "converged", 1: "maxit", 2: "breakdown", 3: "nonfinite"}
 
 
+def _release(obj, fname):
+    h = getattr(obj, "h", None)
+    if h:
+        try:
+            getattr(lib(), fname)(h)
+        except Exception:
+            pass
+        obj.h = None
+
+
 def _pd(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -44,9 +54,7 @@ class Context:
         check(self, lib().mgldg_ctx_synchronize(self.h))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mgldg_ctx_destroy(self.h)
-            self.h = None
+        _release(self, "mgldg_ctx_destroy")
 
 
 class Quadtree:
@@ -82,9 +90,7 @@ class Quadtree:
         return ids
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mgldg_quadtree_destroy(self.h)
-            self.h = None
+        _release(self, "mgldg_quadtree_destroy")
 
 
 class LDGPoisson:
@@ -139,9 +145,7 @@ class LDGPoisson:
         return rp, cols, blk
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mgldg_problem_destroy(self.h)
-            self.h = None
+        _release(self, "mgldg_problem_destroy")
 
 
 class Multigrid:
@@ -192,10 +196,24 @@ class Multigrid:
     def vcycle(self, b, x, precision="fp64"):
         check(self.ctx, lib().mgldg_vcycle_device(self.h, _prec(precision), _ptr(b), _ptr(x), 1))
 
+    def profile_relax(self, level=0, precision="fp64", nrep=10):
+        """(ms per smoother launch, algorithmic bytes per launch, launches)"""
+        ms, by, n = C.c_double(), C.c_double(), C.c_int()
+        check(self.ctx, lib().mgldg_profile_relax(self.h, level, _prec(precision), nrep, C.byref(ms), C.byref(by), C.byref(n)))
+        return ms.value, by.value, n.value
+
+    def profile_spmv(self, level=0, precision="fp64", nrep=10):
+        ms, by = C.c_double(), C.c_double()
+        check(self.ctx, lib().mgldg_profile_spmv(self.h, level, _prec(precision), nrep, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def setup_times(self):
+        t = np.zeros(4)
+        check(self.ctx, lib().mgldg_setup_times(self.h, _pd(t)))
+        return dict(assembly_host=t[0], assembly_device=t[1], hierarchy_host=t[2], hierarchy_device=t[3])
+
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mgldg_mg_destroy(self.h)
-            self.h = None
+        _release(self, "mgldg_mg_destroy")
 
 
 def _prec(p):

@@ -59,6 +59,9 @@ SIGNATURES = {
     "mgldg_spmv_device": (I, [V, I, I, V, V, I]),
     "mgldg_smooth_device": (I, [V, I, I, I, I, V, V, I]),
     "mgldg_vcycle_device": (I, [V, I, V, V, I]),
+    "mgldg_profile_relax": (I, [V, I, I, I, PD, PD, PI]),
+    "mgldg_profile_spmv": (I, [V, I, I, I, PD, PD]),
+    "mgldg_setup_times": (I, [V, PD]),
 }
 
 _lib = None
@@ -71,6 +74,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise MGLDGError(-5, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    # torch ships its own libamdhip64 (same soname, different file name).  Load
+    # torch first so that libmgldg binds to the runtime torch already uses and
+    # the process holds a single HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L_ = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L_, name)
