@@ -126,6 +126,12 @@ typedef struct {
     int npre, npost;   /* reference default 3, 3                            */
     int restriction;   /* MGLDG_RAT | MGLDG_RATRAT                          */
     int precision;     /* MGLDG_FP64 | MGLDG_MIXED: which V-cycles to build */
+    int coarse_dense_max; /* 0: off.  Otherwise the V-cycle below the first level
+                             (not the finest) with at most this many dofs is
+                             formed once at setup as a dense FP64 matrix (the
+                             reference's Multigrid::vcycle_matrix, multigrid.h:
+                             386-400, on that level) and applied as one GEMV:
+                             the same linear map in one launch. <= 8192. */
 } mgldg_mg_params;
 
 /* InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)

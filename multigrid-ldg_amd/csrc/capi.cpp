@@ -188,6 +188,8 @@ int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out)
         MGParams m;
         m.relaxation = prm->relaxation; m.omega = prm->omega; m.coarse = prm->coarse;
         m.npre = prm->npre; m.npost = prm->npost; m.restriction = prm->restriction; m.precision = prm->precision;
+        m.coarse_dense_max = prm->coarse_dense_max;
+        ARG(m.coarse_dense_max >= 0 && m.coarse_dense_max <= 8192, "coarse_dense_max must be in [0, 8192]");
         auto* h = new mgldg_mg_s();
         h->ctx = pr->ctx;
         try { h->m.build(&pr->p, m); }

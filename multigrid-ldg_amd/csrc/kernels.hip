@@ -65,27 +65,39 @@ __device__ double block_sum(double v)
     return t;
 }
 
-// Deterministic grid reduction: every workgroup calls this with its threads'
-// values.  Returns true in thread 0 of the last workgroup, with *total set.
-__device__ bool grid_sum(double v, double* part, unsigned* counter, double* total)
+// Deterministic two-pass reductions: every workgroup writes its partial sum;
+// a one-workgroup kernel adds the partials in index order.  (A fused
+// "last workgroup finishes" scheme needs agent-scope fences, which write back
+// the L2 of every XCD and cost more than the extra launch.)
+__device__ __forceinline__ void block_partial(double v, double* part)
 {
-    __shared__ bool last;
     double bs = block_sum(v);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = bs;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double t = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
-        t += __hip_atomic_load(&part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t = block_sum(t);
-    if (threadIdx.x == 0) { *total = t; *counter = 0u; }
-    return threadIdx.x == 0;
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// sums nval interleaved partial streams (part[i * nval + k], i < n)
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ part, int n, int nval, double* S, int mode,
+                                                   double* out)
+{
+    double t[3] = {0, 0, 0};
+    for (int i = threadIdx.x; i < n; i += 1024)
+        for (int k = 0; k < nval; ++k) t[k] += part[(size_t)i * nval + k];
+    for (int k = 0; k < nval; ++k) t[k] = block_sum(t[k]);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) { *out = t[0]; return; }
+    // mode 1: PCG r.z / r.zold / r.r update with beta (krylov.h:54-57)
+    const double old = S[S_DELTA];
+    S[S_DELTA_OLD] = old;
+    S[S_DELTA] = t[0];
+    S[S_RSOLD] = t[1];
+    S[S_RR] = t[2];
+    S[S_BETA] = (mode == 2) ? (t[0] - t[1]) / old : t[0] / old;
+}
+
+static void finalize(hipStream_t s, const double* part, int n, int nval, double* S, int mode, double* out)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, part, n, nval, S, mode, out);
+    check_launch("finalize");
 }
 
 // --------------------------------------------------------------------- setup kernels
@@ -259,8 +271,7 @@ __device__ __forceinline__ T row_dot(const T* __restrict__ Ab, const T* __restri
 template <typename T, int B>
 __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
                                                const T* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
-                                               T* __restrict__ y, const T* __restrict__ dotv, double* dot_out,
-                                               double* part, unsigned* counter)
+                                               T* __restrict__ y, const T* __restrict__ dotv, double* part)
 {
     using M = Map<T, B>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -277,15 +288,18 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
         y[o] = out;
         if (dotv) dv = (double)out * (double)dotv[o];
     }
-    if (dot_out) grid_sum(dv, part, counter, dot_out);
+    if (part) block_partial(dv, part);
 }
 
 template <typename T, int B>
 __global__ __launch_bounds__(256) void k_gs(int nrows, const int* __restrict__ rows, const int* __restrict__ rowptr,
                                             const int* __restrict__ cols, const T* __restrict__ A,
                                             const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
-                                            int zero_guess)
+                                            const int* __restrict__ color, int cur, int skip)
 {
+    // skip = 1: first sweep from x = 0, forward: x_j of colours > cur (and x_i)
+    // are still zero; skip = 2: the same in reverse (colours < cur).  Their
+    // blocks contribute exactly 0 and are not read.
     using M = Map<T, B>;
     __shared__ T tb[M::RPB * B];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -295,13 +309,12 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int* __restrict__ r
     const int i = valid ? rows[idx] : 0;
     if (valid) {
         T acc = 0;
-        if (!zero_guess) {
-            const int k1 = rowptr[i + 1];
-            for (int k = rowptr[i]; k < k1; ++k) {
-                const int j = cols[k];
-                if (j == i) continue;
-                acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)j * B);
-            }
+        const int k1 = rowptr[i + 1];
+        for (int k = rowptr[i]; k < k1; ++k) {
+            const int j = cols[k];
+            if (j == i) continue;
+            if (skip) { const int cj = color[j]; if (skip == 1 ? cj > cur : cj < cur) continue; }
+            acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)j * B);
         }
         tb[lr * B + r] = b[(size_t)i * B + r] - acc;
     }
@@ -309,7 +322,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int* __restrict__ r
     if (valid) {
         const T s = row_dot<T, B>(Dinv + (size_t)i * (B * B) + r * M::V, tb + lr * B);
         const size_t o = (size_t)i * B + r;
-        const T xi = zero_guess ? (T)0 : x[o];
+        const T xi = skip ? (T)0 : x[o];
         x[o] = xi + omega * (s - xi);
     }
 }
@@ -439,26 +452,29 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
 
 template <typename T>
 void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
-                   const T* b, T* y, const T* dotv, double* dot_out, double* part, unsigned* counter)
+                   const T* b, T* y, const T* dotv, double* dot_out, double* part)
 {
     if (nrows <= 0) return;
+    unsigned g = 0;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_apply<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, cols, A, x, b, y,
-                           dotv, dot_out, part, counter);
+        g = nblk(nrows, M::RPB);
+        hipLaunchKernelGGL((k_apply<T, BB>), dim3(g), dim3(256), 0, s, nrows, rowptr, cols, A, x, b, y,
+                           dotv, dot_out ? part : nullptr);
     });
     check_launch("apply");
+    if (dot_out) finalize(s, part, (int)g, 1, nullptr, 0, dot_out);
 }
 
 template <typename T>
 void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* rows, const int* rowptr, const int* cols, const T* A,
-                      const T* Dinv, const T* b, T* x, T omega, int zero_guess)
+                      const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
 {
     if (nrows <= 0) return;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
         hipLaunchKernelGGL((k_gs<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rows, rowptr, cols, A, Dinv, b,
-                           x, omega, zero_guess);
+                           x, omega, color, cur, skip);
     });
     check_launch("gs_color");
 }
@@ -547,26 +563,291 @@ void f32_to_f64(hipStream_t s, long n, const float* a, double* b)
     check_launch("f32_to_f64");
 }
 
+// --------------------------------------------------------------------- k right-hand sides
+template <typename T, int B>
+__device__ __forceinline__ T pan(const T* blk, int r, int c)
+{
+    constexpr int V = Map<T, B>::V;
+    return blk[((c / V) * B + r) * V + (c % V)];
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void mk_apply(int k, int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                                const T* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
+                                                T* __restrict__ y)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nrows * k) return;
+    const int i = (int)(e / k), c = (int)(e % k);
+    T acc[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) acc[r] = 0;
+    for (int t = rowptr[i]; t < rowptr[i + 1]; ++t) {
+        const T* Ab = A + (size_t)t * B * B;
+        const T* xj = x + (size_t)cols[t] * B * k + c;
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const T xv = xj[(size_t)q * k];
+#pragma unroll
+            for (int r = 0; r < B; ++r) acc[r] += pan<T, B>(Ab, r, q) * xv;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const size_t o = ((size_t)i * B + r) * k + c;
+        y[o] = b ? (T)(b[o] - acc[r]) : acc[r];
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void mk_gs(int k, int nrows, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                             const int* __restrict__ cols, const T* __restrict__ A, const T* __restrict__ Dinv,
+                                             const T* __restrict__ b, T* x, T omega, const int* __restrict__ color, int cur,
+                                             int skip)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nrows * k) return;
+    const int i = rows[(int)(e / k)], c = (int)(e % k);
+    T t[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) t[r] = b[((size_t)i * B + r) * k + c];
+    for (int s = rowptr[i]; s < rowptr[i + 1]; ++s) {
+        const int j = cols[s];
+        if (j == i) continue;
+        if (skip) { const int cj = color[j]; if (skip == 1 ? cj > cur : cj < cur) continue; }
+        const T* Ab = A + (size_t)s * B * B;
+        const T* xj = x + (size_t)j * B * k + c;
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const T xv = xj[(size_t)q * k];
+#pragma unroll
+            for (int r = 0; r < B; ++r) t[r] -= pan<T, B>(Ab, r, q) * xv;
+        }
+    }
+    const T* D = Dinv + (size_t)i * B * B;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        T sdot = 0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) sdot += pan<T, B>(D, r, q) * t[q];
+        const size_t o = ((size_t)i * B + r) * k + c;
+        const T xi = skip ? (T)0 : x[o];
+        x[o] = xi + omega * (sdot - xi);
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void mk_jacobi(int k, int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                                 const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                                 const T* __restrict__ x, T* __restrict__ xout, T omega)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nrows * k) return;
+    const int i = (int)(e / k), c = (int)(e % k);
+    T t[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) t[r] = b[((size_t)i * B + r) * k + c];
+    for (int s = rowptr[i]; s < rowptr[i + 1]; ++s) {
+        const T* Ab = A + (size_t)s * B * B;
+        const T* xj = x + (size_t)cols[s] * B * k + c;
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const T xv = xj[(size_t)q * k];
+#pragma unroll
+            for (int r = 0; r < B; ++r) t[r] -= pan<T, B>(Ab, r, q) * xv;
+        }
+    }
+    const T* D = Dinv + (size_t)i * B * B;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        T sdot = 0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) sdot += pan<T, B>(D, r, q) * t[q];
+        const size_t o = ((size_t)i * B + r) * k + c;
+        xout[o] = x[o] + omega * sdot;
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void mk_restrict(int k, int nc, const int* __restrict__ kid_off, const int* __restrict__ kids,
+                                                   const int* __restrict__ type, const T* __restrict__ pool,
+                                                   const T* __restrict__ r, T* __restrict__ bc)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nc * k) return;
+    const int I = (int)(e / k), c = (int)(e % k);
+    T acc[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = 0;
+    for (int t = kid_off[I]; t < kid_off[I + 1]; ++t) {
+        const int f = kids[t];
+        const T* Tb = pool + (size_t)type[f] * B * B;
+        const T* rf = r + (size_t)f * B * k + c;
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const T rv = rf[(size_t)q * k];
+#pragma unroll
+            for (int p = 0; p < B; ++p) acc[p] += Tb[q * B + p] * rv;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) bc[((size_t)I * B + q) * k + c] = acc[q];
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void mk_prolong(int k, int nf, const int* __restrict__ parent, const int* __restrict__ type,
+                                                  const T* __restrict__ pool, const T* __restrict__ xc, T* __restrict__ xf)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nf * k) return;
+    const int f = (int)(e / k), c = (int)(e % k);
+    const T* Tb = pool + (size_t)type[f] * B * B;
+    const T* xp = xc + (size_t)parent[f] * B * k + c;
+    T xv[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) xv[q] = xp[(size_t)q * k];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) acc += Tb[r * B + q] * xv[q];
+        xf[((size_t)f * B + r) * k + c] += acc;
+    }
+}
+
+// one workgroup per column
+template <typename T>
+__global__ __launch_bounds__(256) void mk_coarse(int n, int k, const double* __restrict__ Cinv, T* b, T* x, int subtract_mean)
+{
+    extern __shared__ double bs[];
+    const int c = blockIdx.x;
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) { double v = (double)b[(size_t)i * k + c]; bs[i] = v; s += v; }
+    __shared__ double mean;
+    double tot = block_sum(s);
+    if (threadIdx.x == 0) mean = subtract_mean ? tot / n : 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) bs[i] -= mean;
+    __syncthreads();
+    if (!Cinv) { for (int i = threadIdx.x; i < n; i += 256) b[(size_t)i * k + c] = (T)bs[i]; return; }
+    for (int i = threadIdx.x; i < n; i += 256) {
+        double y = 0;
+        for (int j = 0; j < n; ++j) y += Cinv[(size_t)i * n + j] * bs[j];
+        x[(size_t)i * k + c] = (T)y;
+    }
+}
+
+template <typename T>
+void MOps<T>::apply(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
+                    const T* b, T* y)
+{
+    long n = (long)nrows * k; if (n <= 0) return;
+    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_apply<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, rowptr, cols, A, x, b, y));
+    check_launch("mk_apply");
+}
+template <typename T>
+void MOps<T>::gs_color(hipStream_t s, int B, int k, int nrows, const int* rows, const int* rowptr, const int* cols, const T* A,
+                       const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
+{
+    long n = (long)nrows * k; if (n <= 0) return;
+    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_gs<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, rows, rowptr, cols, A,
+                                         Dinv, b, x, omega, color, cur, skip));
+    check_launch("mk_gs");
+}
+template <typename T>
+void MOps<T>::jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* Dinv,
+                     const T* b, const T* x, T* xout, T omega)
+{
+    long n = (long)nrows * k; if (n <= 0) return;
+    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_jacobi<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, rowptr, cols, A,
+                                         Dinv, b, x, xout, omega));
+    check_launch("mk_jacobi");
+}
+template <typename T>
+void MOps<T>::restrict_(hipStream_t s, int B, int k, int nc, const int* kid_off, const int* kids, const int* type,
+                        const T* pool, const T* r, T* bc)
+{
+    long n = (long)nc * k; if (n <= 0) return;
+    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_restrict<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nc, kid_off, kids, type,
+                                         pool, r, bc));
+    check_launch("mk_restrict");
+}
+template <typename T>
+void MOps<T>::prolong(hipStream_t s, int B, int k, int nf, const int* parent, const int* type, const T* pool, const T* xc,
+                      T* xf)
+{
+    long n = (long)nf * k; if (n <= 0) return;
+    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_prolong<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nf, parent, type, pool,
+                                         xc, xf));
+    check_launch("mk_prolong");
+}
+template <typename T>
+void MOps<T>::coarse(hipStream_t s, int n, int k, const double* Cinv, T* b, T* x, int subtract_mean)
+{
+    if (n > 6144) throw std::runtime_error("coarse: dense coarse problem too large");
+    if (k <= 0) return;
+    hipLaunchKernelGGL(mk_coarse<T>, dim3(k), dim3(256), n * sizeof(double), s, n, k, Cinv, b, x, subtract_mean);
+    check_launch("mk_coarse");
+}
+template struct MOps<float>;
+template struct MOps<double>;
+
+// y = W x: one wavefront per row, lanes stride the columns
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_f64(int n, const double* __restrict__ W, const T* __restrict__ x, T* __restrict__ y)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* w = W + (size_t)row * n;
+    double acc = 0;
+    for (int j = lane; j < n; j += 64) acc += w[j] * (double)x[j];
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = (T)acc;
+}
+template <typename T>
+void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gemv_f64<T>, dim3(nblk(n, 4)), dim3(256), 0, s, n, W, x, y);
+    check_launch("gemv_f64");
+}
+template void gemv_f64<float>(hipStream_t, int, const double*, const float*, float*);
+template void gemv_f64<double>(hipStream_t, int, const double*, const double*, double*);
+
+__global__ void k_eye(int n, double* X)
+{
+    long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)n * n) return;
+    X[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+void eye(hipStream_t s, int n, double* X)
+{
+    long t = (long)n * n; if (t <= 0) return;
+    hipLaunchKernelGGL(k_eye, dim3(nblk(t, 256)), dim3(256), 0, s, n, X);
+    check_launch("eye");
+}
+
 // --------------------------------------------------------------------- PCG kernels
 constexpr int kRedBlocks = 1024;   // grid of the reduction kernels (grid-stride)
 
 __global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
-                                             double* out, double* part, unsigned* counter)
+                                             double* part)
 {
     double s = 0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) s += a[i] * b[i];
-    grid_sum(s, part, counter, out);
+    block_partial(s, part);
 }
 
-void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part, unsigned* counter)
+void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part)
 {
     unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, s, n, a, b, out, part, counter);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, s, n, a, b, part);
     check_launch("dot");
+    finalize(s, part, (int)g, 1, nullptr, 0, out);
 }
 
 __global__ __launch_bounds__(256) void k_pcg_xr(long n, double* S, const double* __restrict__ d, const double* __restrict__ q,
-                                                double* __restrict__ x, double* __restrict__ r, double* part, unsigned* counter)
+                                                double* __restrict__ x, double* __restrict__ r, double* part)
 {
     const double alpha = S[S_DELTA] / S[S_DQ];
     double s = 0;
@@ -575,14 +856,15 @@ __global__ __launch_bounds__(256) void k_pcg_xr(long n, double* S, const double*
         double ri = r[i] - alpha * q[i];
         r[i] = ri; s += ri * ri;
     }
-    grid_sum(s, part, counter, S + S_RR);
+    block_partial(s, part);
 }
 
-void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, double* part, unsigned* counter)
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, double* part)
 {
     unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(256), 0, s, n, S, d, q, x, r, part, counter);
+    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(256), 0, s, n, S, d, q, x, r, part);
     check_launch("pcg_xr");
+    finalize(s, part, (int)g, 1, S, 0, S + S_RR);
 }
 
 __global__ void k_pcg_x(long n, const double* S, const double* __restrict__ d, double* __restrict__ x)
@@ -600,8 +882,8 @@ void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x)
 }
 
 // three reductions in one pass: r.z, r.zold, r.r ; then beta in the last block
-__global__ __launch_bounds__(256) void k_pcg_rz(long n, double* S, const double* __restrict__ r, const double* __restrict__ z,
-                                                const double* __restrict__ zold, double* part, unsigned* counter)
+__global__ __launch_bounds__(256) void k_pcg_rz(long n, const double* __restrict__ r, const double* __restrict__ z,
+                                                const double* __restrict__ zold, double* part)
 {
     double a = 0, b = 0, c = 0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
@@ -609,41 +891,16 @@ __global__ __launch_bounds__(256) void k_pcg_rz(long n, double* S, const double*
         a += ri * z[i]; c += ri * ri;
         if (zold) b += ri * zold[i];
     }
-    // three sequential deterministic grid sums sharing one counter
-    __shared__ bool last;
     double ba = block_sum(a), bb = block_sum(b), bc = block_sum(c);
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = ba; part[3 * blockIdx.x + 1] = bb; part[3 * blockIdx.x + 2] = bc;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double ta = 0, tb = 0, tc = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += 256) {
-        ta += __hip_atomic_load(&part[3 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tb += __hip_atomic_load(&part[3 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tc += __hip_atomic_load(&part[3 * i + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    ta = block_sum(ta); tb = block_sum(tb); tc = block_sum(tc);
-    if (threadIdx.x == 0) {
-        const double old = S[S_DELTA];
-        S[S_DELTA_OLD] = old;
-        S[S_DELTA] = ta;
-        S[S_RSOLD] = tb;
-        S[S_RR] = tc;
-        S[S_BETA] = zold ? (ta - tb) / old : ta / old;
-        *counter = 0u;
-    }
+    if (threadIdx.x == 0) { part[3 * blockIdx.x] = ba; part[3 * blockIdx.x + 1] = bb; part[3 * blockIdx.x + 2] = bc; }
 }
 
-void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold, double* part, unsigned* counter)
+void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold, double* part)
 {
     unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_pcg_rz, dim3(g), dim3(256), 0, s, n, S, r, z, zold, part, counter);
+    hipLaunchKernelGGL(k_pcg_rz, dim3(g), dim3(256), 0, s, n, r, z, zold, part);
     check_launch("pcg_rz");
+    finalize(s, part, (int)g, 3, S, zold ? 2 : 1, nullptr);
 }
 
 __global__ void k_pcg_d(long n, const double* S, const double* __restrict__ z, double* __restrict__ d)

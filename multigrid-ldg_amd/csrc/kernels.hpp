@@ -28,13 +28,15 @@ void gather_blocks(hipStream_t s, int n, int bs, const int* idx, const double* i
 // ---------------------------------------------------------------- solve (T = float | double)
 template <typename T>
 struct Ops {
-    // y = A x (b == nullptr) or y = b - A x ; if dotv: partial sums of y.dotv
-    // are reduced into *dot_out (FP64) deterministically.
+    // y = A x (b == nullptr) or y = b - A x ; if dot_out: y.dotv is reduced
+    // into *dot_out (FP64, deterministic; part holds one partial per workgroup)
     static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
-                      const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part, unsigned* counter);
-    // damped block Gauss-Seidel on the rows of one colour (multigrid.h:240-249)
+                      const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
+    // damped block Gauss-Seidel on the rows of colour `cur` (multigrid.h:240-249).
+    // skip = 1 (2): first forward (reverse) sweep from x = 0, where the
+    // colours after `cur` in sweep order are still zero and are not read.
     static void gs_color(hipStream_t s, int B, int nrows, const int* rows, const int* rowptr, const int* cols,
-                         const T* A, const T* Dinv, const T* b, T* x, T omega, int zero_guess);
+                         const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
     // damped block Jacobi, x_out = x + omega Dinv (b - A x) (multigrid.h:223-229)
     static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);
@@ -51,6 +53,30 @@ struct Ops {
     static void copy(hipStream_t s, long n, const T* a, T* b);
 };
 
+// ---------------------------------------------------------------- k right-hand sides
+// Vectors hold k columns, [row][B][k] (right-hand side fastest).  One thread
+// owns one (block row, column) pair and keeps the B entries in registers.
+template <typename T>
+struct MOps {
+    static void apply(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A,
+                      const T* x, const T* b, T* y);
+    static void gs_color(hipStream_t s, int B, int k, int nrows, const int* rows, const int* rowptr, const int* cols,
+                         const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
+    static void jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A,
+                       const T* Dinv, const T* b, const T* x, T* xout, T omega);
+    static void restrict_(hipStream_t s, int B, int k, int nc, const int* kid_off, const int* kids, const int* type,
+                          const T* pool, const T* r, T* bc);
+    static void prolong(hipStream_t s, int B, int k, int nf, const int* parent, const int* type, const T* pool,
+                        const T* xc, T* xf);
+    // per column: b -= mean(b) (optional) then x = Cinv b (Cinv FP64 n x n); Cinv null: only the mean
+    static void coarse(hipStream_t s, int n, int k, const double* Cinv, T* b, T* x, int subtract_mean);
+};
+// y = W x with W FP64 n x n row-major (the collapsed coarse V-cycle); x, y of type T
+template <typename T>
+void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y);
+// identity columns: X[i][c] = (i == c), n x n
+void eye(hipStream_t s, int n, double* X);
+
 void f64_to_f32(hipStream_t s, long n, const double* a, float* b);
 void f32_to_f64(hipStream_t s, long n, const float* a, double* b);
 
@@ -58,15 +84,15 @@ void f32_to_f64(hipStream_t s, long n, const float* a, double* b);
 // Scalar slots in a device array S:
 enum { S_DELTA = 0, S_DELTA_OLD, S_DQ, S_ALPHA, S_BETA, S_RR, S_RSOLD, S_DELTA0, S_BB, S_COUNT };
 // a.b reduced into *out
-void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part, unsigned* counter);
+void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part);
 // x += alpha d ; r -= alpha q ; S[S_RR] = r.r      (alpha = S[DELTA]/S[DQ])
 void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r,
-            double* part, unsigned* counter);
+            double* part);
 // x += alpha d only
 void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x);
 // S[DELTA_OLD]=S[DELTA]; S[DELTA] = r.z ; if flexible S[RSOLD] = r.zold ; S[RR] = r.r ; beta
 void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold,
-            double* part, unsigned* counter);
+            double* part);
 // d = z + beta d
 void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d);
 // M-block-diagonal apply y = M x (block size B, row-major FP64 blocks)

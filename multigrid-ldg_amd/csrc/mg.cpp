@@ -321,6 +321,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision)
     std::vector<int> rows(lv.nb), pos(lv.color_off.begin(), lv.color_off.end() - 1);
     for (int i = 0; i < lv.nb; ++i) rows[pos[lv.color[i]]++] = i;
     lv.color_rows.upload(rows.data(), rows.size(), s);
+    lv.color_dev.upload(lv.color.data(), lv.color.size(), s);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
     if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
@@ -397,6 +398,7 @@ void MGH::build(ProblemH* p, const MGParams& pm)
         std::vector<double> X = dense_inverse(D, n);
         c.Cinv.upload(X.data(), X.size(), s);
     }
+    build_collapse();
     // PCG work space
     const size_t nd = (size_t)p->ndof;
     r.alloc(nd); z.alloc(nd); zold.alloc(nd); d.alloc(nd); q.alloc(nd); xw.alloc(nd); bw.alloc(nd);
@@ -443,11 +445,13 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
             for (int cc = 0; cc < lv.ncolor; ++cc) {
                 int c = reverse ? lv.ncolor - 1 - cc : cc;
                 int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
+                const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
                 dev::Ops<T>::gs_color(s, lv.B, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv),
-                                      v.b, v.x, omega, (zero_guess && sw == 0 && cc == 0) ? 1 : 0);
+                                      v.b, v.x, omega, lv.color_dev, c, skip);
             }
     } else {
         // Jacobi ping-pongs between x and xalt; the result ends in x
+        if (zero_guess) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
         T* cur = v.x.p; T* alt = v.xalt.p;
         for (int sw = 0; sw < n; ++sw) {
             dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, cur, alt, omega);
@@ -463,33 +467,114 @@ void MGH::vcycle_from(int l0)
 {
     hipStream_t s = ctx->stream;
     const int nl = (int)L.size();
+    const int bottom = cut > l0 ? cut : nl - 1;
     bool zero = true;   // x of the starting level is zero (preconditioner use)
-    for (int l = l0; l < nl - 1; ++l) {
+    for (int l = l0; l < bottom; ++l) {
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
         relax<T>(l, prm.npre, false, zero);
-        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr, nullptr);
+        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr);
         LevelVecs<T>& vc = vecs<T>(l + 1);
         dev::Ops<T>::restrict_(s, lv.B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
-        dev::Ops<T>::fill(s, (long)L[l + 1].nb * L[l + 1].B, (T)0, vc.x);
+        // x_{l+1} = 0 (multigrid.h:376): implicit in the first sweep of a
+        // Gauss-Seidel pre-smoother, explicit otherwise
+        if (l + 1 < bottom && !(prm.relaxation == 1 && prm.npre > 0))
+            dev::Ops<T>::fill(s, (long)L[l + 1].nb * L[l + 1].B, (T)0, vc.x);
         zero = true;
     }
-    Level& c = L[nl - 1];
-    LevelVecs<T>& vc = vecs<T>(nl - 1);
-    if (prm.coarse == 0) {
+    Level& c = L[bottom];
+    LevelVecs<T>& vc = vecs<T>(bottom);
+    if (bottom == cut) {
+        // the V-cycle below `cut` as one precomputed linear map (FP64)
+        dev::gemv_f64<T>(s, c.nb * c.B, W, vc.b, vc.x);
+    } else if (prm.coarse == 0) {
         dev::Ops<T>::coarse(s, c.nb * c.B, c.Cinv, vc.b, vc.x, 1);
     } else {
         // b -= mean(b) then relaxations from x = 0 (multigrid.h:184-187, 367)
         dev::Ops<T>::submean(s, (long)c.nb * c.B, vc.b);
-        relax<T>(nl - 1, prm.npre, false, true);
-        relax<T>(nl - 1, prm.npost, true, false);
+        relax<T>(bottom, prm.npre, false, true);
+        relax<T>(bottom, prm.npost, true, false);
     }
-    for (int l = nl - 2; l >= l0; --l) {
+    for (int l = bottom - 1; l >= l0; --l) {
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
         dev::Ops<T>::prolong(s, lv.B, lv.nb, lv.parent, lv.type, Pool<T>(lv), vecs<T>(l + 1).x, v.x);
         relax<T>(l, prm.npost, true, false);
     }
+}
+
+// relaxation on k columns (FP64), used to form the collapsed coarse operator
+void MGH::mrelax(int l, int k, int nsw, bool reverse, bool zero_guess, double* x, const double* b, double* alt)
+{
+    Level& lv = L[l];
+    hipStream_t s = ctx->stream;
+    const double omega = prm.omega;
+    if (prm.relaxation == 1) {
+        for (int sw = 0; sw < nsw; ++sw)
+            for (int cc = 0; cc < lv.ncolor; ++cc) {
+                int c = reverse ? lv.ncolor - 1 - cc : cc;
+                int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
+                const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
+                dev::MOps<double>::gs_color(s, lv.B, k, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, lv.A64, lv.D64,
+                                            b, x, omega, lv.color_dev, c, skip);
+            }
+    } else {
+        const long n = (long)lv.nb * lv.B * k;
+        if (zero_guess) dev::Ops<double>::fill(s, n, 0.0, x);
+        double* cur = x; double* nxt = alt;
+        for (int sw = 0; sw < nsw; ++sw) {
+            dev::MOps<double>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, lv.D64, b, cur, nxt, omega);
+            std::swap(cur, nxt);
+        }
+        if (cur != x) dev::Ops<double>::copy(s, n, cur, x);
+    }
+}
+
+// The V-cycle below a level is a fixed linear map of that level's right-hand
+// side (the reference forms it column by column as Multigrid::vcycle_matrix,
+// multigrid.h:386-400).  Form it once for the first level with at most
+// coarse_dense_max dofs by running the sub-cycle on all identity columns at
+// once, and apply it as one dense FP64 GEMV in every V-cycle.
+void MGH::build_collapse()
+{
+    cut = -1;
+    const int nl = (int)L.size();
+    if (prm.coarse_dense_max <= 0) return;
+    for (int l = 1; l + 2 < nl; ++l)
+        if ((long)L[l].nb * L[l].B <= prm.coarse_dense_max) { cut = l; break; }
+    if (cut < 0) return;
+    hipStream_t s = ctx->stream;
+    const int k = L[cut].nb * L[cut].B;
+    std::vector<DBuf<double>> X(nl), Bv(nl), R(nl);
+    for (int l = cut; l < nl; ++l) {
+        const size_t n = (size_t)L[l].nb * L[l].B * k;
+        X[l].alloc(n); Bv[l].alloc(n); R[l].alloc(n);
+    }
+    dev::eye(s, k, Bv[cut]);
+    for (int l = cut; l + 1 < nl; ++l) {
+        Level& lv = L[l];
+        mrelax(l, k, prm.npre, false, true, X[l], Bv[l], R[l]);
+        if (prm.npre == 0) dev::Ops<double>::fill(s, (long)lv.nb * lv.B * k, 0.0, X[l]);
+        dev::MOps<double>::apply(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, X[l], Bv[l], R[l]);
+        dev::MOps<double>::restrict_(s, lv.B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, lv.pool64, R[l], Bv[l + 1]);
+    }
+    Level& c = L[nl - 1];
+    const int nc = c.nb * c.B;
+    if (prm.coarse == 0) {
+        dev::MOps<double>::coarse(s, nc, k, c.Cinv, Bv[nl - 1], X[nl - 1], 1);
+    } else {
+        dev::MOps<double>::coarse(s, nc, k, nullptr, Bv[nl - 1], nullptr, 1);
+        mrelax(nl - 1, k, prm.npre, false, true, X[nl - 1], Bv[nl - 1], R[nl - 1]);
+        if (prm.npre == 0) dev::Ops<double>::fill(s, (long)nc * k, 0.0, X[nl - 1]);
+        mrelax(nl - 1, k, prm.npost, true, false, X[nl - 1], Bv[nl - 1], R[nl - 1]);
+    }
+    for (int l = nl - 2; l >= cut; --l) {
+        Level& lv = L[l];
+        dev::MOps<double>::prolong(s, lv.B, k, lv.nb, lv.parent, lv.type, lv.pool64, X[l + 1], X[l]);
+        mrelax(l, k, prm.npost, true, false, X[l], Bv[l], R[l]);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    W = std::move(X[cut]);
 }
 
 // z = V(r): enqueue the preconditioner on the stream
@@ -500,12 +585,12 @@ void MGH::precon_enqueue(int precision)
     if (precision == 1) {
         if (!L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
         dev::f64_to_f32(s, n, r, L[0].v32.b);
-        dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
+        if (!(prm.relaxation == 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
         vcycle_from<float>(0);
         dev::f32_to_f64(s, n, L[0].v32.x, z);
     } else {
         dev::Ops<double>::copy(s, n, r, L[0].v64.b);
-        dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
+        if (!(prm.relaxation == 1 && prm.npre > 0)) dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
         vcycle_from<double>(0);
         dev::Ops<double>::copy(s, n, L[0].v64.x, z);
     }
@@ -534,7 +619,7 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     hipStream_t s = ctx->stream;
     Level& f = L[0];
     const long n = pr->ndof;
-    double* part = this->part.p; unsigned* cnt = ctx->counter;
+    double* part = this->part.p;
     SolveStats st;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
@@ -545,13 +630,13 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     };
     auto residual = [&](bool with_rr) {   // r = b - A x  (optionally r.r into S[RR])
         dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, x, b, r, with_rr ? r.p : nullptr,
-                                with_rr ? S.p + dev::S_RR : nullptr, part, cnt);
+                                with_rr ? S.p + dev::S_RR : nullptr, part);
     };
     // ||b||
-    dev::dot(s, n, b, b, S.p + dev::S_BB, part, cnt);
+    dev::dot(s, n, b, b, S.p + dev::S_BB, part);
     residual(false);
     precon(sp.precision, sp.use_graph != 0);
-    dev::pcg_rz(s, n, S, r, z, nullptr, part, cnt);        // S[DELTA] = r.z, S[RR] = r.r
+    dev::pcg_rz(s, n, S, r, z, nullptr, part);        // S[DELTA] = r.z, S[RR] = r.r
     dev::Ops<double>::copy(s, n, z, d);
     if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
     fetchS();
@@ -568,7 +653,7 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
             fetchS();
             if (std::sqrt(S_host[dev::S_RR]) > sp.tol * bnorm && i < sp.maxit) {
                 precon(sp.precision, sp.use_graph != 0);
-                dev::pcg_rz(s, n, S, r, z, nullptr, part, cnt);
+                dev::pcg_rz(s, n, S, r, z, nullptr, part);
                 dev::Ops<double>::copy(s, n, z, d);
                 if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
                 fetchS();
@@ -579,15 +664,15 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
         if (!go) { st.status = 0; break; }
         if (i >= sp.maxit) { st.status = 1; break; }
         // q = A d, S[DQ] = d.q
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, d, nullptr, q, d, S.p + dev::S_DQ, part, cnt);
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, d, nullptr, q, d, S.p + dev::S_DQ, part);
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
             dev::pcg_x(s, n, S, d, x);
             residual(true);
         } else {
-            dev::pcg_xr(s, n, S, d, q, x, r, part, cnt);
+            dev::pcg_xr(s, n, S, d, q, x, r, part);
         }
         precon(sp.precision, sp.use_graph != 0);
-        dev::pcg_rz(s, n, S, r, z, sp.flexible ? zold.p : nullptr, part, cnt);
+        dev::pcg_rz(s, n, S, r, z, sp.flexible ? zold.p : nullptr, part);
         if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
         dev::pcg_d(s, n, S, z, d);
         ++i;
@@ -614,8 +699,8 @@ void MGH::spmv(int l, int precision, const void* x, void* y)
 {
     Level& lv = L[l];
     hipStream_t s = ctx->stream;
-    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr, nullptr);
-    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr, nullptr);
+    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr);
+    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr);
 }
 
 void MGH::smooth(int l, int precision, int n, bool reverse, const void* b, void* x)
@@ -683,8 +768,8 @@ void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes
     const int sa = precision == 1 ? 4 : 8;
     if (precision == 1) dev::Ops<float>::fill(s, n, 1.f, lv.v32.x); else dev::Ops<double>::fill(s, n, 1.0, lv.v64.x);
     auto once = [&] {
-        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr, nullptr);
-        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr, nullptr);
+        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr);
+        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr);
     };
     once();
     hipEvent_t e0, e1;

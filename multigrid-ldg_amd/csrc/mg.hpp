@@ -78,6 +78,7 @@ struct MGParams {
     int npre = 3, npost = 3;
     int restriction = 1;
     int precision = 0;   // 0: FP64 only, 1: also FP32 copies for the mixed V-cycle
+    int coarse_dense_max = 0;   // collapse the V-cycle below the first level with at most this many dofs (0: off)
 };
 
 template <typename T>
@@ -95,7 +96,7 @@ struct Level {
     DBuf<double> Minv;                  // inverse coarse mass blocks (RATRAT)
     std::vector<int> color, color_off;  // colouring and rows grouped by colour
     int ncolor = 0;
-    DBuf<int> color_rows;
+    DBuf<int> color_rows, color_dev;
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     // interpolation to the next coarser level
@@ -131,8 +132,12 @@ struct MGH {
     double* S_host = nullptr;          // pinned
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     double setup_host_s = 0, setup_dev_s = 0;
+    int cut = -1;                      // first collapsed level (-1: none)
+    DBuf<double> W;                    // collapsed V-cycle below `cut`, FP64 dense n x n
 
     void build(ProblemH* p, const MGParams& prm);
+    void build_collapse();
+    void mrelax(int l, int k, int n, bool reverse, bool zero_guess, double* x, const double* b, double* alt);
     ~MGH();
 
     template <typename T> void relax(int l, int n, bool reverse, bool zero_guess);

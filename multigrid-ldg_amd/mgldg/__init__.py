@@ -24,14 +24,18 @@ from ._lib import lib, LIB_PATH, MGParams, SolveParams, SolveStats, SIZEFN, chec
 STATUS = {0: "converged", 1: "maxit", 2: "breakdown", 3: "nonfinite"}
 
 
-def _release(obj, fname):
-    h = getattr(obj, "h", None)
-    if h:
-        try:
-            getattr(lib(), fname)(h)
-        except Exception:
-            pass
-        obj.h = None
+class _Handle:
+    """Owns one library handle; the destroy function is bound at creation so
+    that release also works while the interpreter shuts down."""
+
+    def _own(self, destroy):
+        self._destroy = destroy
+
+    def __del__(self):
+        h, d = getattr(self, "h", None), getattr(self, "_destroy", None)
+        if h and d is not None:
+            d(h)
+        self.h = None
 
 
 def _pd(a):
@@ -42,22 +46,20 @@ def _pi(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
 
 
-class Context:
+class Context(_Handle):
     """One device + one HIP stream (no reference counterpart)."""
 
     def __init__(self, device=0, stream=None):
         self.h = C.c_void_p()
         check(None, lib().mgldg_ctx_create(device, stream, C.byref(self.h)))
+        self._own(lib().mgldg_ctx_destroy)
         self.device = device
 
     def sync(self):
         check(self, lib().mgldg_ctx_synchronize(self.h))
 
-    def __del__(self):
-        _release(self, "mgldg_ctx_destroy")
 
-
-class Quadtree:
+class Quadtree(_Handle):
     """Quadtree<N>(h, strategy, periodic) (src/cartesian/mesh/quadtree.h:211-228).
 
     h(x) -> widths (scalar or N values): a cell splits iff every width exceeds
@@ -78,6 +80,7 @@ class Quadtree:
         strat = {"equal": 0, "rapid": 1}[coarsening]
         check(ctx, lib().mgldg_quadtree_create(ctx.h, N, _pd(lo_a), _pd(hi_a), self._cb, None, strat,
                                                int(periodic), int(balance), C.byref(self.h)))
+        self._own(lib().mgldg_quadtree_destroy)
         info = np.zeros(3, dtype=np.int32)
         check(ctx, lib().mgldg_quadtree_info(self.h, _pi(info)))
         self.nnodes, self.num_levels, self.nleaves = [int(v) for v in info]
@@ -89,11 +92,8 @@ class Quadtree:
         check(self.ctx, lib().mgldg_quadtree_layer(self.h, l, _pi(ids), C.byref(n)))
         return ids
 
-    def __del__(self):
-        _release(self, "mgldg_quadtree_destroy")
 
-
-class LDGPoisson:
+class LDGPoisson(_Handle):
     """Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
     (src/cartesian/mesh/mesh.h:186-265, src/cartesian/dg/ldgpoisson.h:60-77).
 
@@ -108,6 +108,7 @@ class LDGPoisson:
         bct = np.ascontiguousarray(bc, dtype=np.int32)
         self.h = C.c_void_p()
         check(self.ctx, lib().mgldg_ldg_create(qt.h, p, _pi(bct), tau0, tauD, C.byref(self.h)))
+        self._own(lib().mgldg_problem_destroy)
         s = np.zeros(7, dtype=np.int32)
         check(self.ctx, lib().mgldg_problem_sizes(self.h, _pi(s)))
         self.ne, self.nf, self.nbf, self.npl, self.nqb, self.ndof, self.N = [int(v) for v in s]
@@ -144,11 +145,8 @@ class LDGPoisson:
         check(self.ctx, lib().mgldg_problem_export(self.h, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz)))
         return rp, cols, blk
 
-    def __del__(self):
-        _release(self, "mgldg_problem_destroy")
 
-
-class Multigrid:
+class Multigrid(_Handle):
     """InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
     (src/cartesian/multigrid/interpolationhierarchy.h:30-82, multigrid.h:291-301).
 
@@ -156,13 +154,15 @@ class Multigrid:
     copies used by the mixed-precision V-cycle."""
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
-                 restriction="ratrat", precision="fp64"):
+                 restriction="ratrat", precision="fp64", coarse_dense_max=4096):
         self.prob, self.ctx = prob, prob.ctx
         self.params = MGParams({"jacobi": 0, "gs": 1}[relaxation], omega,
                                {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
-                               {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision])
+                               {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision],
+                               coarse_dense_max)
         self.h = C.c_void_p()
         check(self.ctx, lib().mgldg_mg_create(prob.h, C.byref(self.params), C.byref(self.h)))
+        self._own(lib().mgldg_mg_destroy)
         info = np.zeros(3 + 2 * 32, dtype=np.int64)
         check(self.ctx, lib().mgldg_mg_info(self.h, info.ctypes.data_as(C.POINTER(C.c_long)), len(info)))
         self.nlevels = int(info[0])
@@ -211,9 +211,6 @@ class Multigrid:
         t = np.zeros(4)
         check(self.ctx, lib().mgldg_setup_times(self.h, _pd(t)))
         return dict(assembly_host=t[0], assembly_device=t[1], hierarchy_host=t[2], hierarchy_device=t[3])
-
-    def __del__(self):
-        _release(self, "mgldg_mg_destroy")
 
 
 def _prec(p):

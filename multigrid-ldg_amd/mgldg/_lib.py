@@ -9,7 +9,7 @@ SIZEFN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_voi
 
 class MGParams(C.Structure):
     _fields_ = [("relaxation", C.c_int), ("omega", C.c_double), ("coarse", C.c_int), ("npre", C.c_int),
-                ("npost", C.c_int), ("restriction", C.c_int), ("precision", C.c_int)]
+                ("npost", C.c_int), ("restriction", C.c_int), ("precision", C.c_int), ("coarse_dense_max", C.c_int)]
 
 
 class SolveParams(C.Structure):

@@ -81,12 +81,13 @@ def test_hierarchy_matches_oracle(gctx, oracle, mesh, p, rapid, restriction):
                 assert np.array_equal(a[1], b[1]) and rel(a[2], b[2]) < 1e-12, (l, w)
 
 
+@pytest.mark.parametrize("collapse", [0, 4096])
 @pytest.mark.parametrize("relax", ["gs", "jacobi"])
-def test_kernels_match_oracle(gctx, oracle, relax):
+def test_kernels_match_oracle(gctx, oracle, relax, collapse):
     import mgldg
     import torch
     qt, pr, oqt, opr = build_pair(gctx, oracle, "adaptive", 2)
-    mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, precision="mixed")
+    mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, precision="mixed", coarse_dense_max=collapse)
     omg = oracle.Multigrid(opr, "mcgs" if relax == "gs" else "jacobi", npre=2, npost=2)
     rng = np.random.default_rng(7)
     dev = torch.device("cuda")
@@ -114,12 +115,13 @@ def test_kernels_match_oracle(gctx, oracle, relax):
     assert rel(z.cpu().numpy(), zo) < 1e-5
 
 
+@pytest.mark.parametrize("collapse", [0, 4096])
 @pytest.mark.parametrize("mesh,p", [("uniform8", 1), ("adaptive", 2), ("uniform3d", 1)])
 @pytest.mark.parametrize("criterion", ["reference", "true"])
-def test_pcg_matches_oracle(gctx, oracle, mesh, p, criterion):
+def test_pcg_matches_oracle(gctx, oracle, mesh, p, criterion, collapse):
     import mgldg
     qt, pr, oqt, opr = build_pair(gctx, oracle, mesh, p)
-    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, precision="mixed")
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, precision="mixed", coarse_dense_max=collapse)
     omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2)
     rng = np.random.default_rng(11)
     f = rng.standard_normal(pr.ndof)
@@ -168,6 +170,25 @@ def test_graph_replay_matches_eager(gctx, oracle):
             assert np.array_equal(x1, x0)
             # eager work between replays
             mgldg.pcg(mg, 2 * b, tol=1e-6, precision=prec)
+
+
+@pytest.mark.parametrize("coarse", ["cholesky", "relaxation"])
+def test_collapsed_vcycle_equals_levelwise(gctx, oracle, coarse):
+    """the dense collapsed tail reproduces the level-by-level V-cycle"""
+    import mgldg
+    import torch
+    qt, pr, oqt, opr = build_pair(gctx, oracle, "adaptive", 2, rapid=True)
+    b = torch.tensor(np.random.default_rng(2).standard_normal(pr.ndof), device="cuda")
+    outs = []
+    for cdm in (0, 64, 4096):
+        mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, coarse=coarse, coarse_dense_max=cdm)
+        z = torch.zeros_like(b)
+        mg.vcycle(b, z)
+        outs.append(z.cpu().numpy())
+    omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2, coarse=coarse)
+    zo = omg.vcycle(b.cpu().numpy())
+    for z in outs:
+        assert rel(z, zo) < 1e-12
 
 
 def test_solve_statuses(gctx, oracle):
