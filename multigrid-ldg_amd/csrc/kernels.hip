@@ -267,6 +267,73 @@ __device__ __forceinline__ T row_dot(const T* __restrict__ Ab, const T* __restri
     return s;
 }
 
+// Blocks kept in flight per lane: loads of up to UNR blocks of a row are
+// issued before their FMAs (one HBM round trip per UNR blocks instead of one
+// per block).  Sized so that A and x operands stay near 32-64 VGPRs per block.
+template <typename T, int B>
+__host__ __device__ constexpr int unroll_of()
+{
+    return sizeof(T) == 4 ? (B <= 16 ? 4 : (B <= 27 ? 2 : 1)) : (B <= 10 ? 4 : (B <= 16 ? 2 : 1));
+}
+
+// Lane r of the row owning block row i returns sum_k A_k[r][:] . x_{col(k)}
+// over the blocks of the row (FILTER: without the diagonal block and, with
+// skip = 1 / 2, without columns whose colour comes after `cur` in sweep
+// order).  The row's column indices are loaded once (one per lane) and
+// broadcast with lane shuffles; the trip count is uniform across the
+// wavefront so every shuffle runs with all lanes active.
+template <typename T, int B, bool FILTER>
+__device__ __forceinline__ T row_sum(bool valid, int i, int k0, int k1, const int* __restrict__ cols,
+                                     const T* __restrict__ A, const T* __restrict__ x, int r, int lane0,
+                                     const int* __restrict__ color, int cur, int skip)
+{
+    constexpr int V = Map<T, B>::V;
+    constexpr int U = unroll_of<T, B>();
+    T acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
+    const int len = valid ? k1 - k0 : 0;
+    int wl = len;
+    for (int o = 32; o > 0; o >>= 1) wl = max(wl, __shfl_xor(wl, o, 64));
+    for (int kc = 0; kc < wl; kc += B) {
+        const int mycol = (r < len - kc) ? cols[k0 + kc + r] : -1;
+        const int nc = min(B, wl - kc);
+        for (int u0 = 0; u0 < nc; u0 += U) {
+            VecT<T, V> a[U][B / V], xv[U][B / V];
+            bool use[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = u0 + u;
+                const int j = __shfl(mycol, lane0 + (kk < B ? kk : B - 1), 64);
+                bool ok = kk < nc && j >= 0;
+                if (FILTER && ok) {
+                    ok = j != i;
+                    if (ok && skip) { const int cj = color[j]; ok = (skip == 1) ? cj <= cur : cj >= cur; }
+                }
+                use[u] = ok;
+                if (ok) {
+                    const T* Ab = A + (size_t)(k0 + kc + kk) * (B * B) + r * V;
+                    const T* xj = x + (size_t)j * B;
+#pragma unroll
+                    for (int cc = 0; cc < B / V; ++cc) { a[u][cc] = ldv<T, V>(Ab + cc * B * V); xv[u][cc] = ldv<T, V>(xj + cc * V); }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (use[u]) {
+#pragma unroll
+                    for (int cc = 0; cc < B / V; ++cc)
+#pragma unroll
+                        for (int v = 0; v < V; ++v) acc[v] += a[u][cc].v[v] * xv[u][cc].v[v];
+                }
+        }
+    }
+    T s = acc[0];
+#pragma unroll
+    for (int v = 1; v < V; ++v) s += acc[v];
+    return s;
+}
+
 // y = A x  or  y = b - A x ; optional deterministic reduction of y.dotv
 template <typename T, int B>
 __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
@@ -277,12 +344,11 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, r = lane - slot * B;
     const int row = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    const bool valid = slot < M::RPW && row < nrows;
+    const int k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
+    const T acc = row_sum<T, B, false>(valid, row, k0, k1, cols, A, x, r, slot * B, nullptr, 0, 0);
     double dv = 0;
-    if (slot < M::RPW && row < nrows) {
-        T acc = 0;
-        const int k1 = rowptr[row + 1];
-        for (int k = rowptr[row]; k < k1; ++k)
-            acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)cols[k] * B);
+    if (valid) {
         const size_t o = (size_t)row * B + r;
         T out = b ? (T)(b[o] - acc) : acc;
         y[o] = out;
@@ -307,17 +373,9 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int* __restrict__ r
     const int idx = blockIdx.x * M::RPB + lr;
     const bool valid = slot < M::RPW && idx < nrows;
     const int i = valid ? rows[idx] : 0;
-    if (valid) {
-        T acc = 0;
-        const int k1 = rowptr[i + 1];
-        for (int k = rowptr[i]; k < k1; ++k) {
-            const int j = cols[k];
-            if (j == i) continue;
-            if (skip) { const int cj = color[j]; if (skip == 1 ? cj > cur : cj < cur) continue; }
-            acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)j * B);
-        }
-        tb[lr * B + r] = b[(size_t)i * B + r] - acc;
-    }
+    const int k0 = valid ? rowptr[i] : 0, k1 = valid ? rowptr[i + 1] : 0;
+    const T acc = row_sum<T, B, true>(valid, i, k0, k1, cols, A, x, r, slot * B, color, cur, skip);
+    if (valid) tb[lr * B + r] = b[(size_t)i * B + r] - acc;
     __syncthreads();
     if (valid) {
         const T s = row_dot<T, B>(Dinv + (size_t)i * (B * B) + r * M::V, tb + lr * B);
