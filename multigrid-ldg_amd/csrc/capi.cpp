@@ -284,24 +284,11 @@ int mgldg_solve_device(mgldg_mg mg, const double* b, double* x, int nrhs, const 
             if (stats) fill_stats(stats, st);
             return;
         }
-        // columns of the [ndof][nrhs] layout, one after another
-        std::vector<double> hb((size_t)n * nrhs), hx((size_t)n * nrhs), col(n);
-        HIPCHK(hipMemcpyAsync(hb.data(), b, hb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(hx.data(), x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int c = 0; c < nrhs; ++c) {
-            for (long i = 0; i < n; ++i) col[i] = hb[(size_t)i * nrhs + c];
-            m.bw.upload(col.data(), n, s);
-            for (long i = 0; i < n; ++i) col[i] = hx[(size_t)i * nrhs + c];
-            m.xw.upload(col.data(), n, s);
-            SolveStats st = m.solve(m.bw, m.xw, sp);
-            if (stats) fill_stats(&stats[c], st);
-            m.xw.download(col.data(), n, s);
-            HIPCHK(hipStreamSynchronize(s));
-            for (long i = 0; i < n; ++i) hx[(size_t)i * nrhs + c] = col[i];
-        }
-        HIPCHK(hipMemcpyAsync(x, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        // all columns of the [ndof][nrhs] layout in one batched solve
+        ARG(nrhs <= 256, "at most 256 right-hand sides per call");
+        std::vector<SolveStats> st = m.solve_batched(b, x, nrhs, sp);
+        if (stats) for (int c = 0; c < nrhs; ++c) fill_stats(&stats[c], st[c]);
+        (void)n; (void)s;
     });
 }
 
@@ -320,19 +307,22 @@ int mgldg_solve(mgldg_mg mg, const double* b, double* x, int nrhs, const mgldg_s
     });
 }
 
-int mgldg_adjoint_device(mgldg_mg mg, const double*, double*, double*, double*, int, const mgldg_solve_params*,
-                         mgldg_solve_stats*)
+int mgldg_adjoint_device(mgldg_mg mg, const double* dLdu, double* lambda, double* dLdf, double* dLdg, int nrhs,
+                         const mgldg_solve_params* prm, mgldg_solve_stats* stats)
 {
-    if (!mg) return MGLDG_ERR_ARG;
-    mg->ctx->err = "adjoint not built yet";
-    return MGLDG_ERR_UNSUPPORTED;
+    if (!mg || !dLdu || nrhs < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        SolveParams sp = to_sp(prm);
+        ARG(nrhs <= 256, "at most 256 right-hand sides per call");
+        std::vector<SolveStats> st = mg->m.adjoint(dLdu, lambda, dLdf, dLdg, nrhs, sp);
+        if (stats) for (int c = 0; c < nrhs; ++c) fill_stats(&stats[c], st[c]);
+    });
 }
 
-int mgldg_rhs_device(mgldg_mg mg, const double*, const double*, int, double*)
+int mgldg_rhs_device(mgldg_mg mg, const double* f, const double* g, int nrhs, double* b)
 {
-    if (!mg) return MGLDG_ERR_ARG;
-    mg->ctx->err = "device rhs not built yet";
-    return MGLDG_ERR_UNSUPPORTED;
+    if (!mg || !b || nrhs < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] { mg->m.pr->rhs_device(f, g, nrhs, b); });
 }
 
 int mgldg_spmv_device(mgldg_mg mg, int level, int precision, const void* x, void* y, int nrhs)

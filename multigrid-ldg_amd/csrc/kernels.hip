@@ -872,6 +872,29 @@ void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y)
 template void gemv_f64<float>(hipStream_t, int, const double*, const float*, float*);
 template void gemv_f64<double>(hipStream_t, int, const double*, const double*, double*);
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_f64(int n, int k, const double* __restrict__ W, const T* __restrict__ x,
+                                                  T* __restrict__ y)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)n * k) return;
+    const int i = (int)(e / k), c = (int)(e % k);
+    const double* w = W + (size_t)i * n;
+    double acc = 0;
+    for (int j = 0; j < n; ++j) acc += w[j] * (double)x[(size_t)j * k + c];
+    y[e] = (T)acc;
+}
+template <typename T>
+void gemm_f64(hipStream_t s, int n, int k, const double* W, const T* x, T* y)
+{
+    if (k == 1) { gemv_f64<T>(s, n, W, x, y); return; }
+    long t = (long)n * k; if (t <= 0) return;
+    hipLaunchKernelGGL(k_gemm_f64<T>, dim3(nblk(t, 256)), dim3(256), 0, s, n, k, W, x, y);
+    check_launch("gemm_f64");
+}
+template void gemm_f64<float>(hipStream_t, int, int, const double*, const float*, float*);
+template void gemm_f64<double>(hipStream_t, int, int, const double*, const double*, double*);
+
 __global__ void k_eye(int n, double* X)
 {
     long e = (long)blockIdx.x * 256 + threadIdx.x;
@@ -973,6 +996,234 @@ void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d)
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pcg_d, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, z, d);
     check_launch("pcg_d");
+}
+
+// ---------------------------------------------------------------------- batched PCG
+// thread (group g, column c), c = tid % k, g = tid / k; k <= 256
+__device__ __forceinline__ void col_partials(int k, int nval, const double* vals, double* part)
+{
+    __shared__ double red[3][256];
+    for (int v = 0; v < nval; ++v) red[v][threadIdx.x] = vals[v];
+    __syncthreads();
+    const int groups = 256 / k;
+    if (threadIdx.x < k)
+        for (int v = 0; v < nval; ++v) {
+            double t = 0;
+            for (int g = 0; g < groups; ++g) t += red[v][g * k + threadIdx.x];
+            part[((size_t)blockIdx.x * nval + v) * k + threadIdx.x] = t;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_coldot(long n, int k, const double* __restrict__ a, const double* __restrict__ b,
+                                                double* part)
+{
+    const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
+    double acc = 0;
+    if (g < groups)
+        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) acc += a[i * k + c] * b[i * k + c];
+    col_partials(k, 1, &acc, part);
+}
+
+// sums per-column partials; mode 0: out[c * ostride] = sum ; mode 1/2: PCG r.z update
+__global__ void k_colfinalize(const double* __restrict__ part, int nb, int k, int nval, double* out, int ostride,
+                              double* S, const int* active, int mode)
+{
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    double t[3] = {0, 0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int v = 0; v < nval; ++v) t[v] += part[((size_t)b * nval + v) * k + c];
+    if (mode == 0) { out[(size_t)c * ostride] = t[0]; return; }
+    double* Sc = S + (size_t)c * S_COUNT;
+    if (active && !active[c]) return;
+    const double old = Sc[S_DELTA];
+    Sc[S_DELTA_OLD] = old;
+    Sc[S_DELTA] = t[0];
+    Sc[S_RSOLD] = t[1];
+    Sc[S_RR] = t[2];
+    Sc[S_BETA] = (mode == 2) ? (t[0] - t[1]) / old : t[0] / old;
+}
+
+static unsigned red_grid(long n, int k)
+{
+    const int groups = 256 / k;
+    return (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + groups - 1) / groups));
+}
+
+void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part)
+{
+    if (k > 256) throw std::runtime_error("coldot: more than 256 columns");
+    unsigned g = red_grid(n, k);
+    hipLaunchKernelGGL(k_coldot, dim3(g), dim3(256), 0, s, n, k, a, b, part);
+    check_launch("coldot");
+    hipLaunchKernelGGL(k_colfinalize, dim3(1), dim3(256), 0, s, part, (int)g, k, 1, out, ostride, nullptr, nullptr, 0);
+    check_launch("colfinalize");
+}
+
+__global__ __launch_bounds__(256) void k_bpcg_xr(long n, int k, const double* S, const int* active, const double* __restrict__ d,
+                                                 const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
+                                                 double* part)
+{
+    const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
+    double acc = 0;
+    if (g < groups && active[c]) {
+        const double alpha = S[(size_t)c * S_COUNT + S_DELTA] / S[(size_t)c * S_COUNT + S_DQ];
+        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) {
+            const long o = i * k + c;
+            x[o] += alpha * d[o];
+            const double ri = r[o] - alpha * q[o];
+            r[o] = ri; acc += ri * ri;
+        }
+    }
+    col_partials(k, 1, &acc, part);
+}
+
+void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const double* d, const double* q, double* x, double* r,
+             double* part)
+{
+    unsigned g = red_grid(n, k);
+    hipLaunchKernelGGL(k_bpcg_xr, dim3(g), dim3(256), 0, s, n, k, S, active, d, q, x, r, part);
+    check_launch("bpcg_xr");
+    hipLaunchKernelGGL(k_colfinalize, dim3(1), dim3(256), 0, s, part, (int)g, k, 1, S + S_RR, (int)S_COUNT, nullptr, nullptr, 0);
+    check_launch("colfinalize");
+}
+
+__global__ void k_bpcg_x(long n, int k, const double* S, const int* active, const double* __restrict__ d, double* __restrict__ x)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * k) return;
+    const int c = (int)(e % k);
+    if (!active[c]) return;
+    x[e] += S[(size_t)c * S_COUNT + S_DELTA] / S[(size_t)c * S_COUNT + S_DQ] * d[e];
+}
+
+void bpcg_x(hipStream_t s, long n, int k, const double* S, const int* active, const double* d, double* x)
+{
+    hipLaunchKernelGGL(k_bpcg_x, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, S, active, d, x);
+    check_launch("bpcg_x");
+}
+
+__global__ __launch_bounds__(256) void k_bpcg_rz(long n, int k, const double* __restrict__ r, const double* __restrict__ z,
+                                                 const double* __restrict__ zold, double* part)
+{
+    const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
+    double v[3] = {0, 0, 0};
+    if (g < groups)
+        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) {
+            const long o = i * k + c;
+            const double ri = r[o];
+            v[0] += ri * z[o]; v[2] += ri * ri;
+            if (zold) v[1] += ri * zold[o];
+        }
+    col_partials(k, 3, v, part);
+}
+
+void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const double* r, const double* z, const double* zold,
+             double* part)
+{
+    unsigned g = red_grid(n, k);
+    hipLaunchKernelGGL(k_bpcg_rz, dim3(g), dim3(256), 0, s, n, k, r, z, zold, part);
+    check_launch("bpcg_rz");
+    hipLaunchKernelGGL(k_colfinalize, dim3(1), dim3(256), 0, s, part, (int)g, k, 3, nullptr, 0, S, active, zold ? 2 : 1);
+    check_launch("colfinalize");
+}
+
+__global__ void k_bpcg_d(long n, int k, const double* S, const int* active, const double* __restrict__ z, double* __restrict__ d)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * k) return;
+    const int c = (int)(e % k);
+    if (!active[c]) return;
+    d[e] = z[e] + S[(size_t)c * S_COUNT + S_BETA] * d[e];
+}
+
+void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const double* z, double* d)
+{
+    hipLaunchKernelGGL(k_bpcg_d, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, S, active, z, d);
+    check_launch("bpcg_d");
+}
+
+// ---------------------------------------------------------------------- boundary data map
+__global__ void k_bc_forward(int ne, int B, int nq, int k, int N, const int* __restrict__ bf_off, const int* __restrict__ bf_list,
+                             const int* __restrict__ bdim, const int* __restrict__ btype, const double* __restrict__ bsign,
+                             double tauD, const double* __restrict__ Lb, const double* __restrict__ g, double* __restrict__ r,
+                             double* __restrict__ J0, double* __restrict__ J1, double* __restrict__ J2)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)ne * B * k) return;
+    const int c = (int)(e % k);
+    const long ei = e / k;
+    const int i = (int)(ei % B), el = (int)(ei / B);
+    double rr = 0, jj[3] = {0, 0, 0};
+    for (int t = bf_off[el]; t < bf_off[el + 1]; ++t) {
+        const int f = bf_list[t];
+        const double* L = Lb + (size_t)f * B * nq + (size_t)i * nq;
+        const double* gf = g + (size_t)f * nq * k + c;
+        double lg = 0;
+        for (int q = 0; q < nq; ++q) lg += L[q] * gf[(size_t)q * k];
+        if (btype[f] == 0) { rr += tauD * lg; jj[bdim[f]] += bsign[f] * lg; }
+        else rr += lg;
+    }
+    r[e] = rr;
+    J0[e] = jj[0];
+    if (N > 1) J1[e] = jj[1];
+    if (N > 2) J2[e] = jj[2];
+}
+
+void bc_forward(hipStream_t s, int ne, int B, int nq, int k, int N, const int* bf_off, const int* bf_list, const int* bdim,
+                const int* btype, const double* bsign, double tauD, const double* Lb, const double* g, double* r, double* J0,
+                double* J1, double* J2)
+{
+    long t = (long)ne * B * k; if (t <= 0) return;
+    hipLaunchKernelGGL(k_bc_forward, dim3(nblk(t, 256)), dim3(256), 0, s, ne, B, nq, k, N, bf_off, bf_list, bdim, btype, bsign,
+                       tauD, Lb, g, r, J0, J1, J2);
+    check_launch("bc_forward");
+}
+
+__global__ void k_bc_adjoint(int nbf, int B, int nq, int k, const int* __restrict__ bleft, const int* __restrict__ bdim,
+                             const int* __restrict__ btype, const double* __restrict__ bsign, double tauD,
+                             const double* __restrict__ Lb, const double* __restrict__ lam, const double* __restrict__ G0,
+                             const double* __restrict__ G1, const double* __restrict__ G2, double* __restrict__ dg)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nbf * nq * k) return;
+    const int c = (int)(e % k);
+    const long fq = e / k;
+    const int q = (int)(fq % nq), f = (int)(fq / nq);
+    const int l = bleft[f], d = bdim[f];
+    const double* Gl = d == 0 ? G0 : (d == 1 ? G1 : G2);
+    const bool dir = btype[f] == 0;
+    const double n = bsign[f];
+    const double* L = Lb + (size_t)f * B * nq + q;
+    double acc = 0;
+    for (int i = 0; i < B; ++i) {
+        const size_t o = ((size_t)l * B + i) * k + c;
+        const double v = dir ? tauD * lam[o] - n * Gl[o] : lam[o];
+        acc += L[(size_t)i * nq] * v;
+    }
+    dg[e] = acc;
+}
+
+void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, const int* bleft, const int* bdim, const int* btype,
+                const double* bsign, double tauD, const double* Lb, const double* lam, const double* G0, const double* G1,
+                const double* G2, double* dg)
+{
+    long t = (long)nbf * nq * k; if (t <= 0) return;
+    hipLaunchKernelGGL(k_bc_adjoint, dim3(nblk(t, 256)), dim3(256), 0, s, nbf, B, nq, k, bleft, bdim, btype, bsign, tauD, Lb,
+                       lam, G0, G1, G2, dg);
+    check_launch("bc_adjoint");
+}
+
+__global__ void k_axpby(long n, double as, const double* a, double bs, const double* b, double* y)
+{
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = as * a[i] + bs * b[i];
+}
+void axpby(hipStream_t s, long n, double a_scale, const double* a, double b_scale, const double* b, double* y)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, s, n, a_scale, a, b_scale, b, y);
+    check_launch("axpby");
 }
 
 __global__ void k_blockdiag(int nb, int B, int nrhs, const double* __restrict__ Mb, const double* __restrict__ x,

@@ -71,6 +71,9 @@ struct MOps {
     // per column: b -= mean(b) (optional) then x = Cinv b (Cinv FP64 n x n); Cinv null: only the mean
     static void coarse(hipStream_t s, int n, int k, const double* Cinv, T* b, T* x, int subtract_mean);
 };
+// Y = W X for k columns ([n][k] layout), W FP64 n x n row-major
+template <typename T>
+void gemm_f64(hipStream_t s, int n, int k, const double* W, const T* x, T* y);
 // y = W x with W FP64 n x n row-major (the collapsed coarse V-cycle); x, y of type T
 template <typename T>
 void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y);
@@ -95,6 +98,32 @@ void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, 
             double* part);
 // d = z + beta d
 void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d);
+// ---------------------------------------------------------------- batched PCG (k columns, [n][k])
+// Per-column scalars live at S[c * S_COUNT + slot]; active[c] = 0 freezes column c.
+// per-column a.b into out[c * ostride] (deterministic two-pass)
+void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part);
+void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const double* d, const double* q, double* x,
+             double* r, double* part);
+void bpcg_x(hipStream_t s, long n, int k, const double* S, const int* active, const double* d, double* x);
+void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const double* r, const double* z,
+             const double* zold, double* part);
+void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const double* z, double* d);
+// ---------------------------------------------------------------- boundary data map B_g (ldgpoisson.tpp:103-161)
+// Forward gather per element e (its boundary faces bf_off[e]..bf_off[e+1]):
+//   r_e  = sum_f w_f L_f g_f     (w_f = tauD for Dirichlet, 1 for Neumann)
+//   J_d,e = sum_{Dirichlet f, dim d} n_f L_f g_f
+// L_f is npl x nq row-major; g [nbf*nq][k]; r, J [ndof][k] (J: N arrays).
+void bc_forward(hipStream_t s, int ne, int B, int nq, int k, int N, const int* bf_off, const int* bf_list,
+                const int* bdim, const int* btype, const double* bsign, double tauD, const double* Lb,
+                const double* g, double* r, double* J0, double* J1, double* J2);
+// Adjoint per boundary face f, point q: dg = L_f^T v_f with
+//   v_f = tauD lam_l - n_f (G_dim lam)_l (Dirichlet), lam_l (Neumann).
+// Glam: N arrays [ndof][k] holding G_d lam.
+void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, const int* bleft, const int* bdim, const int* btype,
+                const double* bsign, double tauD, const double* Lb, const double* lam, const double* G0,
+                const double* G1, const double* G2, double* dg);
+// y = a + alpha b (k columns flattened)
+void axpby(hipStream_t s, long n, double a_scale, const double* a, double b_scale, const double* b, double* y);
 // M-block-diagonal apply y = M x (block size B, row-major FP64 blocks)
 void blockdiag(hipStream_t s, int nb, int B, const double* Mblk, const double* x, double* y, int nrhs);
 

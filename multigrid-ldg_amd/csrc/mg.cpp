@@ -142,8 +142,73 @@ void ProblemH::build(Ctx* c, const Quadtree* q, int p, const int* bct, double ta
     }
     bleft.upload(bl.data(), bl.size(), s); bdim.upload(bd.data(), bd.size(), s);
     btype.upload(bt.data(), bt.size(), s); bsign.upload(bs.data(), bs.size(), s);
+    std::vector<int> off(ne + 1, 0), lst(host.mesh.nbf);
+    for (int k = 0; k < host.mesh.nbf; ++k) off[bl[k] + 1]++;
+    for (int e = 0; e < ne; ++e) off[e + 1] += off[e];
+    {
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (int k = 0; k < host.mesh.nbf; ++k) lst[pos[bl[k]]++] = k;
+    }
+    bf_off.upload(off.data(), off.size(), s); bf_list.upload(lst.data(), std::max<size_t>(lst.size(), 1), s);
+    // G_d (panel) and G_d^T (pattern transpose, transposed blocks, panel) for B_g and its adjoint
+    for (int d = 0; d < N; ++d) {
+        Gpan[d].alloc(G[d].val.n);
+        dev::to_panel_f64(s, G[d].nnzb(), b, G[d].val, Gpan[d]);
+        const HostBSR& Gh = host.G[d];
+        PatternBuilder pb(ne, ne);
+        for (int i = 0; i < ne; ++i) for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) pb.add(Gh.cols[t], i);
+        HostBSR T; pb.finish(T, b, b);
+        for (int i = 0; i < ne; ++i)
+            for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) {
+                double* dst = T.block(T.find(Gh.cols[t], i));
+                const double* src = Gh.block(t);
+                for (int r = 0; r < b; ++r) for (int q = 0; q < b; ++q) dst[(size_t)q * b + r] = src[(size_t)r * b + q];
+            }
+        GT[d].upload_pattern(T, s);
+        DBuf<double> rm; rm.upload(T.val.data(), T.val.size(), s);
+        GT[d].val.alloc(T.val.size());
+        dev::to_panel_f64(s, T.nnzb(), b, rm, GT[d].val);
+        HIPCHK(hipStreamSynchronize(s));
+    }
     HIPCHK(hipStreamSynchronize(s));
     setup_dev_s = secs(t1);
+}
+
+void ProblemH::rhs_device(const double* f, const double* g, int k, double* bout)
+{
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)ndof * k;
+    if (f) dev::blockdiag(s, ne, b, M.val, f, bout, k);
+    else dev::Ops<double>::fill(s, (long)n, 0.0, bout);
+    if (!g) return;
+    DBuf<double> r, J[3], t;
+    r.alloc(n); t.alloc(n);
+    for (int d = 0; d < N; ++d) J[d].alloc(n);
+    dev::bc_forward(s, ne, b, host.nq, k, N, bf_off, bf_list, bdim, btype, bsign, host.tauD, Lb, g, r, J[0], J[1], J[2]);
+    dev::axpby(s, (long)n, 1.0, bout, 1.0, r, bout);
+    for (int d = 0; d < N; ++d) {
+        dev::MOps<double>::apply(s, b, k, ne, GT[d].rowptr, GT[d].cols, GT[d].val, J[d], nullptr, t);
+        dev::axpby(s, (long)n, 1.0, bout, -1.0, t, bout);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+void ProblemH::adjoint_maps(const double* lam, int k, double* dLdf, double* dLdg)
+{
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)ndof * k;
+    if (dLdf) dev::blockdiag(s, ne, b, M.val, lam, dLdf, k);
+    if (dLdg) {
+        DBuf<double> Gl[3];
+        for (int d = 0; d < N; ++d) {
+            Gl[d].alloc(n);
+            dev::MOps<double>::apply(s, b, k, ne, G[d].rowptr, G[d].cols, Gpan[d], lam, nullptr, Gl[d]);
+        }
+        dev::bc_adjoint(s, host.mesh.nbf, b, host.nq, k, bleft, bdim, btype, bsign, host.tauD, Lb, lam, Gl[0],
+                        N > 1 ? Gl[1].p : nullptr, N > 2 ? Gl[2].p : nullptr, dLdg);
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------- hierarchy
@@ -414,6 +479,7 @@ void MGH::build(ProblemH* p, const MGParams& pm)
 MGH::~MGH()
 {
     for (auto& g : graph) if (g) (void)hipGraphExecDestroy(g);
+    for (auto& g : bgraph) if (g) (void)hipGraphExecDestroy(g);
     if (S_host) (void)hipHostFree(S_host);
 }
 
@@ -692,6 +758,261 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     st.relres_prec = delta0 > 0 ? std::sqrt(std::fabs(delta / delta0)) : 0.0;
     st.seconds = ms * 1e-3;
     if (st.status == 0 && sp.criterion == 1 && st.relres_true > sp.tol) st.status = 1;
+    return st;
+}
+
+// ---------------------------------------------------------------------------- batched
+template <typename T> static T* bx(MGH::BatchLevel& b);
+template <> double* bx<double>(MGH::BatchLevel& b) { return b.x64.p; }
+template <> float* bx<float>(MGH::BatchLevel& b) { return b.x32.p; }
+template <typename T> static T* bb(MGH::BatchLevel& b);
+template <> double* bb<double>(MGH::BatchLevel& b) { return b.b64.p; }
+template <> float* bb<float>(MGH::BatchLevel& b) { return b.b32.p; }
+template <typename T> static T* br(MGH::BatchLevel& b);
+template <> double* br<double>(MGH::BatchLevel& b) { return b.r64.p; }
+template <> float* br<float>(MGH::BatchLevel& b) { return b.r32.p; }
+template <typename T> static T* balt(MGH::BatchLevel& b);
+template <> double* balt<double>(MGH::BatchLevel& b) { return b.alt64.p; }
+template <> float* balt<float>(MGH::BatchLevel& b) { return b.alt32.p; }
+
+void MGH::ensure_batch(int k, int precision)
+{
+    if (k > 256) throw Error(-1, "at most 256 right-hand sides per batch");
+    if (precision == 1 && !L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
+    if (bk == k && bprec >= precision) return;
+    for (auto& g : bgraph) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    const int nl = (int)L.size();
+    BL.clear(); BL.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        const size_t n = (size_t)L[l].nb * L[l].B * k;
+        BL[l].x64.alloc(n); BL[l].b64.alloc(n); BL[l].r64.alloc(n); BL[l].alt64.alloc(n);
+        if (precision == 1) { BL[l].x32.alloc(n); BL[l].b32.alloc(n); BL[l].r32.alloc(n); BL[l].alt32.alloc(n); }
+    }
+    const size_t nd = (size_t)pr->ndof * k;
+    Br.alloc(nd); Bz.alloc(nd); Bzold.alloc(nd); Bd.alloc(nd); Bq.alloc(nd);
+    BS.alloc((size_t)k * dev::S_COUNT);
+    Bpart.alloc((size_t)1024 * 3 * k + 64);
+    Bactive.alloc(k);
+    bk = k; bprec = precision;
+}
+
+template <typename T>
+void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
+{
+    Level& lv = L[l];
+    BatchLevel& v = BL[l];
+    hipStream_t s = ctx->stream;
+    const T omega = (T)prm.omega;
+    if (prm.relaxation == 1) {
+        for (int sw = 0; sw < n; ++sw)
+            for (int cc = 0; cc < lv.ncolor; ++cc) {
+                int c = reverse ? lv.ncolor - 1 - cc : cc;
+                int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
+                const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
+                dev::MOps<T>::gs_color(s, lv.B, k, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv),
+                                       bb<T>(v), bx<T>(v), omega, lv.color_dev, c, skip);
+            }
+    } else {
+        const long m = (long)lv.nb * lv.B * k;
+        if (zero_guess) dev::Ops<T>::fill(s, m, (T)0, bx<T>(v));
+        T* cur = bx<T>(v); T* nxt = balt<T>(v);
+        for (int sw = 0; sw < n; ++sw) {
+            dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
+            std::swap(cur, nxt);
+        }
+        if (cur != bx<T>(v)) dev::Ops<T>::copy(s, m, cur, bx<T>(v));
+    }
+}
+
+// Multigrid::vcycle (multigrid.h:361-383) on k columns, x0 = 0
+template <typename T>
+void MGH::bvcycle(int k)
+{
+    hipStream_t s = ctx->stream;
+    const int nl = (int)L.size();
+    const int bottom = cut > 0 ? cut : nl - 1;
+    for (int l = 0; l < bottom; ++l) {
+        Level& lv = L[l];
+        brelax<T>(l, k, prm.npre, false, true);
+        if (prm.npre == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B * k, (T)0, bx<T>(BL[l]));
+        dev::MOps<T>::apply(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), bx<T>(BL[l]), bb<T>(BL[l]), br<T>(BL[l]));
+        dev::MOps<T>::restrict_(s, lv.B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
+    }
+    Level& c = L[bottom];
+    const int n = c.nb * c.B;
+    if (bottom == cut) {
+        dev::gemm_f64<T>(s, n, k, W, bb<T>(BL[bottom]), bx<T>(BL[bottom]));
+    } else if (prm.coarse == 0) {
+        dev::MOps<T>::coarse(s, n, k, c.Cinv, bb<T>(BL[bottom]), bx<T>(BL[bottom]), 1);
+    } else {
+        dev::MOps<T>::coarse(s, n, k, nullptr, bb<T>(BL[bottom]), nullptr, 1);
+        brelax<T>(bottom, k, prm.npre, false, true);
+        if (prm.npre == 0) dev::Ops<T>::fill(s, (long)n * k, (T)0, bx<T>(BL[bottom]));
+        brelax<T>(bottom, k, prm.npost, true, false);
+    }
+    for (int l = bottom - 1; l >= 0; --l) {
+        Level& lv = L[l];
+        dev::MOps<T>::prolong(s, lv.B, k, lv.nb, lv.parent, lv.type, Pool<T>(lv), bx<T>(BL[l + 1]), bx<T>(BL[l]));
+        brelax<T>(l, k, prm.npost, true, false);
+    }
+}
+
+void MGH::bprecon(int precision, int k, bool use_graph)
+{
+    hipStream_t s = ctx->stream;
+    const long n = pr->ndof * (long)k;
+    auto enqueue = [&] {
+        if (precision == 1) {
+            dev::f64_to_f32(s, n, Br, BL[0].b32);
+            bvcycle<float>(k);
+            dev::f32_to_f64(s, n, BL[0].x32, Bz);
+        } else {
+            dev::Ops<double>::copy(s, n, Br, BL[0].b64);
+            bvcycle<double>(k);
+            dev::Ops<double>::copy(s, n, BL[0].x64, Bz);
+        }
+    };
+    if (!use_graph) { enqueue(); return; }
+    hipGraphExec_t& g = bgraph[precision];
+    if (!g) {
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try { enqueue(); }
+        catch (...) { hipGraph_t junk; (void)hipStreamEndCapture(s, &junk); if (junk) (void)hipGraphDestroy(junk); throw; }
+        HIPCHK(hipStreamEndCapture(s, &gr));
+        HIPCHK(hipGraphInstantiate(&g, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+    }
+    HIPCHK(hipGraphLaunch(g, s));
+}
+
+// k independent PCG recurrences (krylov.h:22-68 per column) sharing every
+// operator application; per-column scalars and convergence masks
+std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, const SolveParams& sp)
+{
+    ensure_batch(k, sp.precision);
+    hipStream_t s = ctx->stream;
+    Level& f = L[0];
+    const long n = pr->ndof;
+    const int NS = dev::S_COUNT;
+    std::vector<double> Sh((size_t)k * NS);
+    std::vector<int> act(k, 1), it(k, 0);
+    std::vector<SolveStats> st(k);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    auto fetch = [&] {
+        HIPCHK(hipMemcpyAsync(Sh.data(), BS.p, Sh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    };
+    auto push_active = [&] { HIPCHK(hipMemcpyAsync(Bactive.p, act.data(), k * sizeof(int), hipMemcpyHostToDevice, s)); };
+    auto residual = [&] { dev::MOps<double>::apply(s, f.B, k, f.nb, f.A.rowptr, f.A.cols, f.A64, x, b, Br); };
+    HIPCHK(hipMemsetAsync(BS.p, 0, BS.n * sizeof(double), s));
+    push_active();
+    dev::coldot(s, n, k, b, b, BS.p + dev::S_BB, NS, Bpart);
+    residual();
+    bprecon(sp.precision, k, sp.use_graph != 0);
+    dev::bpcg_rz(s, n, k, BS, Bactive, Br, Bz, nullptr, Bpart);
+    dev::Ops<double>::copy(s, n * k, Bz, Bd);
+    if (sp.flexible) dev::Ops<double>::copy(s, n * k, Bz, Bzold);
+    fetch();
+    std::vector<double> bnorm(k), delta0(k);
+    for (int c = 0; c < k; ++c) { bnorm[c] = std::sqrt(Sh[(size_t)c * NS + dev::S_BB]); delta0[c] = Sh[(size_t)c * NS + dev::S_DELTA]; }
+    auto converged = [&](int c) {
+        const double* Sc = &Sh[(size_t)c * NS];
+        return sp.criterion == 0 ? !(Sc[dev::S_DELTA] > sp.tol * sp.tol * delta0[c]) : !(std::sqrt(Sc[dev::S_RR]) > sp.tol * bnorm[c]);
+    };
+    auto update_active = [&](bool after_iter) {
+        bool changed = false;
+        for (int c = 0; c < k; ++c) {
+            if (!act[c]) continue;
+            const double* Sc = &Sh[(size_t)c * NS];
+            int status = -1;
+            if (!std::isfinite(Sc[dev::S_DELTA]) || !std::isfinite(Sc[dev::S_RR])) status = 3;
+            else if (after_iter && !(Sc[dev::S_DQ] > 0)) status = std::isfinite(Sc[dev::S_DQ]) ? 2 : 3;
+            else if (converged(c)) status = 0;
+            if (status >= 0) { act[c] = 0; st[c].status = status; changed = true; }
+        }
+        if (changed) push_active();
+    };
+    update_active(false);
+    int i = 0;
+    for (;;) {
+        bool any = false;
+        for (int c = 0; c < k; ++c) any |= act[c] != 0;
+        if (!any && sp.criterion == 1 && i < sp.maxit) {
+            // confirm converged columns on the true residual; restart those that fail
+            residual();
+            dev::coldot(s, n, k, Br, Br, BS.p + dev::S_RR, NS, Bpart);
+            fetch();
+            std::vector<int> redo;
+            for (int c = 0; c < k; ++c)
+                if (st[c].status == 0 && std::sqrt(Sh[(size_t)c * NS + dev::S_RR]) > sp.tol * bnorm[c]) redo.push_back(c);
+            if (!redo.empty()) {
+                for (int c : redo) act[c] = 1;
+                push_active();
+                bprecon(sp.precision, k, sp.use_graph != 0);
+                dev::bpcg_rz(s, n, k, BS, Bactive, Br, Bz, nullptr, Bpart);
+                fetch();
+                for (int c : redo) Sh[(size_t)c * NS + dev::S_BETA] = 0.0;
+                HIPCHK(hipMemcpyAsync(BS.p, Sh.data(), Sh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+                dev::bpcg_d(s, n, k, BS, Bactive, Bz, Bd);
+                if (sp.flexible) dev::Ops<double>::copy(s, n * k, Bz, Bzold);
+                any = true;
+            }
+        }
+        if (!any) break;
+        if (i >= sp.maxit) { for (int c = 0; c < k; ++c) if (act[c]) st[c].status = 1; break; }
+        dev::MOps<double>::apply(s, f.B, k, f.nb, f.A.rowptr, f.A.cols, f.A64, Bd, nullptr, Bq);
+        dev::coldot(s, n, k, Bd, Bq, BS.p + dev::S_DQ, NS, Bpart);
+        if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
+            dev::bpcg_x(s, n, k, BS, Bactive, Bd, x);
+            residual();
+        } else {
+            dev::bpcg_xr(s, n, k, BS, Bactive, Bd, Bq, x, Br, Bpart);
+        }
+        bprecon(sp.precision, k, sp.use_graph != 0);
+        dev::bpcg_rz(s, n, k, BS, Bactive, Br, Bz, sp.flexible ? Bzold.p : nullptr, Bpart);
+        if (sp.flexible) dev::Ops<double>::copy(s, n * k, Bz, Bzold);
+        dev::bpcg_d(s, n, k, BS, Bactive, Bz, Bd);
+        ++i;
+        for (int c = 0; c < k; ++c) it[c] += act[c];
+        fetch();
+        update_active(true);
+    }
+    residual();
+    dev::coldot(s, n, k, Br, Br, BS.p + dev::S_RR, NS, Bpart);
+    HIPCHK(hipEventRecord(e1, s));
+    fetch();
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    for (int c = 0; c < k; ++c) {
+        const double* Sc = &Sh[(size_t)c * NS];
+        st[c].iterations = it[c];
+        st[c].relres_true = bnorm[c] > 0 ? std::sqrt(Sc[dev::S_RR]) / bnorm[c] : 0.0;
+        st[c].relres_prec = delta0[c] > 0 ? std::sqrt(std::fabs(Sc[dev::S_DELTA] / delta0[c])) : 0.0;
+        st[c].seconds = ms * 1e-3;
+        if (st[c].status == 0 && sp.criterion == 1 && st[c].relres_true > sp.tol) st[c].status = 1;
+    }
+    return st;
+}
+
+// adjoint solve: A is symmetric (T symmetric, G^T M G symmetric), so the
+// adjoint system is solved by the same MG-PCG
+std::vector<SolveStats> MGH::adjoint(const double* dLdu, double* lambda, double* dLdf, double* dLdg, int k,
+                                     const SolveParams& sp)
+{
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)pr->ndof * k;
+    DBuf<double> own;
+    double* lam = lambda;
+    if (!lam) { own.alloc(n); lam = own.p; }
+    dev::Ops<double>::fill(s, (long)n, 0.0, lam);
+    std::vector<SolveStats> st;
+    if (k == 1) st.push_back(solve(dLdu, lam, sp));
+    else st = solve_batched(dLdu, lam, k, sp);
+    pr->adjoint_maps(lam, k, dLdf, dLdg);
     return st;
 }
 

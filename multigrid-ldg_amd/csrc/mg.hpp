@@ -65,10 +65,17 @@ struct ProblemH {
     DBuf<double> Lb;                   // boundary lifting blocks (npl x nq per boundary face)
     DBuf<int> bleft, bdim, btype;      // per boundary face: left element, dim, bc type
     DBuf<double> bsign;
+    DBuf<int> bf_off, bf_list;         // boundary faces of each element (CSR)
+    DBuf<double> Gpan[3];              // G_d in the solve (panel) layout
+    DevOp GT[3];                       // G_d^T, values in the panel layout
     int N = 2, b = 0, ne = 0, nqb = 0;
     long ndof = 0;
     double setup_host_s = 0, setup_dev_s = 0;
     void build(Ctx* c, const Quadtree* q, int p, const int* bct, double tau0, double tauD);
+    // b = M f + B_g g on the device, k columns (f or g may be null)
+    void rhs_device(const double* f, const double* g, int k, double* bout);
+    // dL/df = M lam, dL/dg = B_g^T lam (either output may be null)
+    void adjoint_maps(const double* lam, int k, double* dLdf, double* dLdg);
 };
 
 struct MGParams {
@@ -135,8 +142,23 @@ struct MGH {
     int cut = -1;                      // first collapsed level (-1: none)
     DBuf<double> W;                    // collapsed V-cycle below `cut`, FP64 dense n x n
 
+    // batched (k right-hand sides) state
+    struct BatchLevel { DBuf<double> x64, b64, r64, alt64; DBuf<float> x32, b32, r32, alt32; };
+    int bk = 0, bprec = -1;
+    std::vector<BatchLevel> BL;
+    DBuf<double> Br, Bz, Bzold, Bd, Bq, BS, Bpart;
+    DBuf<int> Bactive;
+    hipGraphExec_t bgraph[2] = {nullptr, nullptr};
+
     void build(ProblemH* p, const MGParams& prm);
     void build_collapse();
+    void ensure_batch(int k, int precision);
+    template <typename T> void brelax(int l, int k, int n, bool reverse, bool zero_guess);
+    template <typename T> void bvcycle(int k);
+    void bprecon(int precision, int k, bool use_graph);     // Bz = V(Br) on k columns
+    std::vector<SolveStats> solve_batched(const double* b, double* x, int k, const SolveParams& sp);
+    // lambda = A^{-1} dLdu (same MG-PCG), dLdf = M lambda, dLdg = B_g^T lambda
+    std::vector<SolveStats> adjoint(const double* dLdu, double* lambda, double* dLdf, double* dLdg, int k, const SolveParams& sp);
     void mrelax(int l, int k, int n, bool reverse, bool zero_guess, double* x, const double* b, double* alt);
     ~MGH();
 

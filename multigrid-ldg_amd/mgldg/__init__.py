@@ -247,3 +247,72 @@ def pcg(mg, b, x0=None, tol=1e-8, maxit=200, criterion="reference", flexible=Fal
     stats = [dict(iterations=s.iterations, status=STATUS.get(s.status, s.status), relres_true=s.relres_true,
                   relres_prec=s.relres_prec, seconds=s.seconds) for s in st]
     return x, (stats[0] if nrhs == 1 else stats)
+
+
+def rhs_device(mg, f=None, g=None):
+    """b = M f + B_g g on the GPU (ldgpoisson.tpp:103-176).  f [ndof(,k)] and g
+    [nqb(,k)] are FP64 device tensors (either may be None)."""
+    import torch
+    ref = f if f is not None else g
+    k = 1 if ref.ndim == 1 else ref.shape[1]
+    n = mg.prob.ndof
+    b = torch.empty((n,) if k == 1 else (n, k), dtype=torch.float64, device=ref.device)
+    torch.cuda.synchronize()
+    check(mg.ctx, lib().mgldg_rhs_device(mg.h, None if f is None else _ptr(f), None if g is None else _ptr(g), k, _ptr(b)))
+    return b
+
+
+def adjoint(mg, dLdu, tol=1e-12, maxit=500, criterion="true", flexible=False, precision="fp64", use_graph=False):
+    """Adjoint solve through the same MG-PCG: lambda = A^{-1} dL/du, then
+    dL/df = M lambda and dL/dg = B_g^T lambda.  Returns (lambda, dLdf, dLdg, stats)."""
+    import torch
+    k = 1 if dLdu.ndim == 1 else dLdu.shape[1]
+    sp = SolveParams(tol, maxit, {"reference": 0, "true": 1}[criterion], int(flexible), _prec(precision), int(use_graph))
+    st = (SolveStats * k)()
+    lam = torch.empty_like(dLdu)
+    dLdf = torch.empty_like(dLdu)
+    nq = mg.prob.nqb
+    dLdg = torch.empty((nq,) if k == 1 else (nq, k), dtype=torch.float64, device=dLdu.device)
+    torch.cuda.synchronize()
+    check(mg.ctx, lib().mgldg_adjoint_device(mg.h, _ptr(dLdu.contiguous()), _ptr(lam), _ptr(dLdf), _ptr(dLdg), k,
+                                             C.byref(sp), st))
+    stats = [dict(iterations=s.iterations, status=STATUS.get(s.status, s.status), relres_true=s.relres_true) for s in st]
+    return lam, dLdf, dLdg, (stats[0] if k == 1 else stats)
+
+
+def _make_autograd():
+    import torch
+
+    class LDGSolve(torch.autograd.Function):
+        """u = A^{-1} (M f + B_g g) with a hand-written adjoint backward.
+
+        forward : device RHS, MG-PCG (FP64 or the mixed form)
+        backward: dL/df = M A^{-1} dL/du, dL/dg = B_g^T A^{-1} dL/du (A symmetric)"""
+
+        @staticmethod
+        def forward(ctx, f, g, mg, opts):
+            b = rhs_device(mg, f, g)
+            u = torch.zeros_like(b)
+            pcg(mg, b, u, **opts)
+            ctx.mg, ctx.opts = mg, opts
+            return u
+
+        @staticmethod
+        def backward(ctx, gu):
+            _, dLdf, dLdg, _ = adjoint(ctx.mg, gu.contiguous(), **ctx.opts)
+            return dLdf, dLdg, None, None
+
+    return LDGSolve
+
+
+_ldg_solve_cls = None
+
+
+def ldg_solve(f, g, mg, **opts):
+    """Differentiable solve u(f, g) (torch.autograd).  opts as for pcg()."""
+    global _ldg_solve_cls
+    if _ldg_solve_cls is None:
+        _ldg_solve_cls = _make_autograd()
+    o = dict(tol=1e-12, maxit=500, criterion="true")
+    o.update(opts)
+    return _ldg_solve_cls.apply(f, g, mg, o)

@@ -1,0 +1,104 @@
+"""GPU: batched right-hand sides and the adjoint, against single solves and the
+oracle (FP64)."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import build_pair, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("collapse", [0, 4096])
+def test_batched_matches_single(gctx, oracle, precision, collapse):
+    import mgldg
+    import torch
+    qt, pr, _, opr = build_pair(gctx, oracle, "adaptive", 2, rapid=True)
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, precision="mixed", coarse_dense_max=collapse)
+    rng = np.random.default_rng(21)
+    k = 5
+    F = rng.standard_normal((pr.ndof, k))
+    F[:, 3] = 0.0                                   # a zero right-hand side converges at once
+    B = pr.rhs(F)
+    opts = dict(tol=1e-10, maxit=200, criterion="true", precision=precision, flexible=precision == "mixed")
+    Bt = torch.tensor(B, device="cuda")
+    Xt = torch.zeros_like(Bt)
+    _, stats = mgldg.pcg(mg, Bt, Xt, **opts)
+    X = Xt.cpu().numpy()
+    for c in range(k):
+        x1, s1 = mgldg.pcg(mg, B[:, c].copy(), **opts)
+        assert stats[c]["status"] == "converged"
+        assert abs(stats[c]["iterations"] - s1["iterations"]) <= (0 if precision == "fp64" else 1)
+        if c == 3:
+            assert stats[c]["iterations"] == 0 and np.all(X[:, c] == 0)
+            continue
+        assert stats[c]["relres_true"] <= 1e-10
+        assert rel(X[:, c], x1) < 1e-8
+
+
+def test_batched_nonfinite_column_is_isolated(gctx, oracle):
+    import mgldg
+    import torch
+    qt, pr, _, _ = build_pair(gctx, oracle, "uniform8", 1)
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2)
+    B = pr.rhs(np.random.default_rng(2).standard_normal((pr.ndof, 3)))
+    B[7, 1] = np.nan
+    Bt = torch.tensor(B, device="cuda")
+    Xt = torch.zeros_like(Bt)
+    _, st = mgldg.pcg(mg, Bt, Xt, tol=1e-10, criterion="true")
+    assert st[1]["status"] == "nonfinite"
+    assert st[0]["status"] == "converged" and st[2]["status"] == "converged"
+
+
+@pytest.mark.parametrize("mesh,p,k", [("uniform8", 1, 1), ("adaptive", 2, 1), ("adaptive", 2, 3)])
+def test_device_rhs_and_adjoint_match_oracle(gctx, oracle, mesh, p, k):
+    import mgldg
+    import torch
+    qt, pr, oqt, opr = build_pair(gctx, oracle, mesh, p, tau0=0.5)
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2)
+    omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2)
+    rng = np.random.default_rng(4)
+    F = rng.standard_normal((pr.ndof, k)); G = rng.standard_normal((pr.nqb, k)); D = rng.standard_normal((pr.ndof, k))
+    sq = (lambda a: a[:, 0]) if k == 1 else (lambda a: a)
+    ft, gt, dt = [torch.tensor(sq(a).copy(), device="cuda") for a in (F, G, D)]
+    b = mgldg.rhs_device(mg, ft, gt).cpu().numpy().reshape(pr.ndof, k)
+    for c in range(k):
+        assert rel(b[:, c], opr.rhs(F[:, c], G[:, c])) < 1e-13
+    lam, dLdf, dLdg, st = mgldg.adjoint(mg, dt, tol=1e-12)
+    lam, dLdf, dLdg = [a.cpu().numpy().reshape(-1, k) for a in (lam, dLdf, dLdg)]
+    for c in range(k):
+        lo, _, _, _ = oracle.pcg(opr, omg, D[:, c], tol=1e-13, maxit=300, criterion="true")
+        assert rel(lam[:, c], lo) < 1e-9
+        assert rel(dLdf[:, c], opr.rhs(lo, np.zeros(opr.nqb))) < 1e-9
+        assert rel(dLdg[:, c], opr.rhs_g_adjoint(lo)) < 1e-9
+
+
+def test_autograd_gradients_against_finite_differences(gctx, oracle):
+    """L = 0.5 ||u(f, g) - u*||^2: the adjoint gradient equals central finite
+    differences (eps = 1e-6, FP64) along random directions."""
+    import mgldg
+    import torch
+    qt, pr, _, _ = build_pair(gctx, oracle, "adaptive", 2, tau0=0.5)
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2)
+    rng = np.random.default_rng(9)
+    dev = "cuda"
+    f = torch.tensor(rng.standard_normal(pr.ndof), device=dev, requires_grad=True)
+    g = torch.tensor(rng.standard_normal(pr.nqb), device=dev, requires_grad=True)
+    ustar = torch.tensor(rng.standard_normal(pr.ndof), device=dev)
+
+    def loss(ff, gg):
+        u = mgldg.ldg_solve(ff, gg, mg, tol=1e-13)
+        return 0.5 * torch.sum((u - ustar) ** 2)
+
+    L = loss(f, g)
+    L.backward()
+    # L is quadratic in (f, g), so central differences carry no truncation
+    # error; what remains is the solve error (tol 1e-13) divided by 2 eps.
+    for eps, bound in ((1e-6, 1e-4), (1e-3, 1e-7)):
+        for _ in range(4):
+            df = torch.tensor(rng.standard_normal(pr.ndof), device=dev)
+            dg = torch.tensor(rng.standard_normal(pr.nqb), device=dev)
+            with torch.no_grad():
+                fd = (loss(f + eps * df, g + eps * dg) - loss(f - eps * df, g - eps * dg)) / (2 * eps)
+            an = torch.dot(f.grad, df) + torch.dot(g.grad, dg)
+            assert abs(fd.item() - an.item()) <= bound * abs(an.item()), (eps, fd.item(), an.item())
