@@ -29,6 +29,7 @@ for p in (ROOT, os.path.join(ROOT, "multigrid-ldg_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+CALIB_BYTES = 256 * 1024 * 1024   # plain copy used to scale the FETCH_SIZE / WRITE_SIZE counters
 X0 = (0.3, 0.7)
 C2_SCALE = 1.8          # refine while |x - x0| < 1.8 * 2^-l (l = depth below the 16x16 root grid)
 
@@ -60,6 +61,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true", help="setup + fine-level smoother sweeps only (profiling)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-maxit", type=int, default=60)
+    ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides of the batched form")
+    ap.add_argument("--solve-only", action="store_true", help="setup + three mixed solves (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -97,12 +100,22 @@ def main():
     spmv64 = bys / (mss * 1e-3) / 1e9
     log(f"GS colour sweep fp32: {ms32 * 1e3:.1f} us/launch, {by32 / 1e6:.2f} MB -> {gs32:.0f} GB/s; "
         f"fp64: {ms64 * 1e3:.1f} us, {gs64:.0f} GB/s; SpMV fp64 {mss * 1e3:.1f} us {spmv64:.0f} GB/s")
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "k_gs_c2_fine_pmc_traffic.json")
+    if os.path.exists(tp):   # measured offline: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated
+        traffic = json.load(open(tp))["kernels"].get("void mgldg::dev::k_gs<float, 16>", {}).get("traffic_bytes")
     roofline = {"bound": "hbm", "kernel": "k_gs<float,16> (fine-level multicolour block GS launch)",
                 "achieved": round(gs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gs32 / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": by32, "us_per_launch": round(ms32 * 1e3, 3),
+                "traffic": traffic, "bytes_per_launch": by32, "us_per_launch": round(ms32 * 1e3, 3),
                 "fp64_gs_GBs": round(gs64, 1), "fp64_spmv_GBs": round(spmv64, 1)}
     if args.roofline_only:
-        print(json.dumps({"roofline": roofline}), flush=True)
+        # calibration for the PMC traffic counters: a plain device copy of known size
+        src = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline": roofline, "calibration_copy_bytes": CALIB_BYTES}), flush=True)
         return
 
     rng = np.random.default_rng(0)
@@ -110,6 +123,14 @@ def main():
     b_host = prob.rhs(f)
     dev = torch.device("cuda", 0)
     b = torch.tensor(b_host, device=dev)
+    if args.solve_only:
+        # profiling mode: three mixed solves, V-cycle replayed from its hipGraph
+        for _ in range(3):
+            _, st = mgldg.pcg(mg, b, torch.zeros_like(b), tol=1e-10, maxit=500, criterion="true", precision="mixed",
+                              flexible=True, use_graph=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"solve": st}), flush=True)
+        return
 
     def run(precision, steps, warm):
         kw = dict(tol=1e-10, maxit=500, criterion="true", precision=precision, flexible=(precision == "mixed"),
@@ -141,6 +162,36 @@ def main():
     ndof = prob.ndof
     v64 = ndof * args.steps / dt64
     v32 = ndof * args.steps / dt32
+
+    # batched form: k right-hand sides share every operator application
+    k = args.nrhs
+    Fb = rng.standard_normal((prob.ndof, k))
+    Bb = torch.tensor(prob.rhs(Fb), device=dev)
+
+    def run_batched(precision, steps, warm):
+        kw = dict(tol=1e-10, maxit=500, criterion="true", precision=precision, flexible=(precision == "mixed"),
+                  use_graph=True)
+        for _ in range(warm):
+            mgldg.pcg(mg, Bb, torch.zeros_like(Bb), **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            X = torch.zeros_like(Bb)
+            _, st = mgldg.pcg(mg, Bb, X, **kw)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, st
+
+    bsteps = max(1, args.steps // 2)
+    dtb64, stb64 = run_batched("fp64", bsteps, 1)
+    dtb32, stb32 = run_batched("mixed", bsteps, 1)
+    for st in stb64 + stb32:
+        if st["status"] != "converged" or st["relres_true"] > 1e-10:
+            log("BATCHED NOT CONVERGED", st)
+            sys.exit(1)
+    vb64 = prob.ndof * k * bsteps / dtb64
+    vb32 = prob.ndof * k * bsteps / dtb32
+    log(f"batched k={k}: fp64 {dtb64 / bsteps * 1e3:.1f} ms/solve ({vb64:.3g} DoF-solves/s), "
+        f"mixed {dtb32 / bsteps * 1e3:.1f} ms ({vb32:.3g}); max iterations {max(s['iterations'] for s in stb32)}")
 
     cpu = None
     err = {}
@@ -190,6 +241,11 @@ def main():
         "mixed": {"value": round(v32, 1), "ms_per_step": round(dt32 / args.steps * 1e3, 3),
                   "iterations": st32[-1]["iterations"], "relres_true": st32[-1]["relres_true"]},
         "speedup_mixed_vs_fp64": round(v32 / v64, 3),
+        "batched": {"nrhs": k, "fp64": {"value": round(vb64, 1), "ms_per_step": round(dtb64 / bsteps * 1e3, 3),
+                                        "max_iterations": max(s["iterations"] for s in stb64)},
+                    "mixed": {"value": round(vb32, 1), "ms_per_step": round(dtb32 / bsteps * 1e3, 3),
+                              "max_iterations": max(s["iterations"] for s in stb32)},
+                    "mixed_batched_vs_fp64_single": round(vb32 / v64, 3)},
         "error_vs_fp64_oracle": err,
         "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
         "roofline": roofline,

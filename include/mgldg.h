@@ -52,6 +52,8 @@ extern "C" {
 #define MGLDG_NEUMANN   1          /* kNeumann */
 #define MGLDG_JACOBI        0      /* MG::kJacobi */
 #define MGLDG_GAUSS_SEIDEL  1      /* MG::kGaussSeidel, multicolour on the GPU */
+#define MGLDG_GAUSS_SEIDEL_LEX 2   /* MG::kGaussSeidel in the reference's own (lexicographic)
+                                      order, run as wavefronts of independent rows */
 #define MGLDG_COARSE_CHOLESKY   0  /* MG::kCholesky */
 #define MGLDG_COARSE_RELAXATION 1  /* MG::kRelaxation */
 #define MGLDG_RAT    0             /* MG::kRAT */
@@ -120,7 +122,7 @@ int mgldg_problem_destroy(mgldg_problem pr);
 /* ======================= multigrid setup ======================= */
 /* MG::Parameters (multigrid.h:48-62) plus the GPU forms. */
 typedef struct {
-    int relaxation;    /* MGLDG_JACOBI | MGLDG_GAUSS_SEIDEL               */
+    int relaxation;    /* MGLDG_JACOBI | MGLDG_GAUSS_SEIDEL | MGLDG_GAUSS_SEIDEL_LEX */
     double omega;      /* damping, reference default 0.8                   */
     int coarse;        /* MGLDG_COARSE_CHOLESKY | MGLDG_COARSE_RELAXATION  */
     int npre, npost;   /* reference default 3, 3                            */

@@ -715,4 +715,22 @@ int greedy_colors(const HostBSR& A, std::vector<int>& color)
     return nc;
 }
 
+int wavefronts(const HostBSR& A, bool reverse, std::vector<int>& level)
+{
+    const int n = A.nbr;
+    level.assign(n, 0);
+    int nl = 0;
+    for (int t = 0; t < n; ++t) {
+        const int i = reverse ? n - 1 - t : t;
+        int lv = 0;
+        for (int k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+            const int j = A.cols[k];
+            if (reverse ? j > i : j < i) lv = std::max(lv, level[j] + 1);
+        }
+        level[i] = lv;
+        nl = std::max(nl, lv + 1);
+    }
+    return nl;
+}
+
 }  // namespace mgldg

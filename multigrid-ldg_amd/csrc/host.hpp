@@ -164,5 +164,9 @@ void rap_symbolic(const Interp& T, const HostBSR& X, HostBSR& C, TermList& terms
 
 // greedy colouring in row order
 int greedy_colors(const HostBSR& A, std::vector<int>& color);
+// wavefront index of every row for the lexicographic sweep in row order
+// (reverse = false: a row after all its lower-index neighbours) or in reverse
+// order; returns the number of wavefronts
+int wavefronts(const HostBSR& A, bool reverse, std::vector<int>& level);
 
 }  // namespace mgldg

@@ -361,7 +361,7 @@ static void coarsen(hipStream_t s, int N, Level& f, Level& c, int restriction)
 }
 
 // solve layouts, inverse diagonal blocks, colouring, vectors
-static void finalize_level(hipStream_t s, Level& lv, int precision)
+static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicographic)
 {
     const int b = lv.B;
     const long nnzb = lv.A.nnzb();
@@ -379,14 +379,27 @@ static void finalize_level(hipStream_t s, Level& lv, int precision)
         lv.D32.alloc((size_t)lv.nb * b * b);
         dev::to_panel_f32(s, lv.nb, b, lv.Dinv_rm, lv.D32);
     }
-    lv.ncolor = greedy_colors(lv.A.pat, lv.color);
-    lv.color_off.assign(lv.ncolor + 1, 0);
-    for (int c : lv.color) lv.color_off[c + 1]++;
-    for (int c = 0; c < lv.ncolor; ++c) lv.color_off[c + 1] += lv.color_off[c];
-    std::vector<int> rows(lv.nb), pos(lv.color_off.begin(), lv.color_off.end() - 1);
-    for (int i = 0; i < lv.nb; ++i) rows[pos[lv.color[i]]++] = i;
-    lv.color_rows.upload(rows.data(), rows.size(), s);
-    lv.color_dev.upload(lv.color.data(), lv.color.size(), s);
+    auto group = [&](const std::vector<int>& color, int nc, std::vector<int>& off, DBuf<int>& rows_d, DBuf<int>& col_d) {
+        off.assign(nc + 1, 0);
+        for (int c : color) off[c + 1]++;
+        for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
+        std::vector<int> rows(lv.nb), pos(off.begin(), off.end() - 1);
+        for (int i = 0; i < lv.nb; ++i) rows[pos[color[i]]++] = i;
+        rows_d.upload(rows.data(), rows.size(), s);
+        col_d.upload(color.data(), color.size(), s);
+        HIPCHK(hipStreamSynchronize(s));
+    };
+    if (lexicographic) {
+        // wavefronts of the lexicographic order: a row waits for its earlier
+        // (forward) or later (reverse) neighbours only, so processing the
+        // wavefronts in order reproduces the sequential sweep exactly
+        lv.ncolor = wavefronts(lv.A.pat, false, lv.color);
+        lv.rncolor = wavefronts(lv.A.pat, true, lv.rcolor);
+        group(lv.rcolor, lv.rncolor, lv.rcolor_off, lv.rcolor_rows, lv.rcolor_dev);
+    } else {
+        lv.ncolor = greedy_colors(lv.A.pat, lv.color);
+    }
+    group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
     if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
@@ -411,7 +424,7 @@ static void upload_interp(hipStream_t s, Level& lv, int precision)
 void MGH::build(ProblemH* p, const MGParams& pm)
 {
     pr = p; ctx = p->ctx; prm = pm;
-    if (prm.relaxation < 0 || prm.relaxation > 1) throw Error(-1, "Unknown relaxation method.");   // multigrid.h:213
+    if (prm.relaxation < 0 || prm.relaxation > 2) throw Error(-1, "Unknown relaxation method.");   // multigrid.h:213
     if (prm.coarse < 0 || prm.coarse > 1) throw Error(-1, "Unknown coarse solver.");              // multigrid.h:193
     if (prm.restriction < 0 || prm.restriction > 1) throw Error(-1, "Unknown restriction method."); // multigrid.h:164
     if (prm.precision < 0 || prm.precision > 1) throw Error(-1, "Unknown precision.");
@@ -445,7 +458,7 @@ void MGH::build(ProblemH* p, const MGParams& pm)
         L[l + 1].nb = L[l].I.nc; L[l + 1].B = b;
         coarsen(s, N, L[l], L[l + 1], prm.restriction);
     }
-    for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision);
+    for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision, prm.relaxation == 2);
     // coarsest level: dense inverse of the assembled operator (FP64)
     Level& c = L[nl - 1];
     if (prm.coarse == 0) {
@@ -496,6 +509,27 @@ template <typename T> static const T* Pool(const Level& lv);
 template <> const double* Pool<double>(const Level& lv) { return lv.pool64.p; }
 template <> const float* Pool<float>(const Level& lv) { return lv.pool32.p; }
 
+// The GS launches of one sweep in order.  Multicolour: colours ascending
+// (forward) or descending (reverse).  Lexicographic: forward wavefronts
+// ascending, or reverse-order wavefronts ascending.  skip marks the first
+// sweep from x = 0 (rows after the current launch in sweep order are zero).
+template <typename F>
+void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) const
+{
+    const bool lex = prm.relaxation == 2;
+    const bool use_r = lex && reverse;
+    const int nc = use_r ? lv.rncolor : lv.ncolor;
+    const std::vector<int>& off = use_r ? lv.rcolor_off : lv.color_off;
+    const int* rows = use_r ? lv.rcolor_rows.p : lv.color_rows.p;
+    const int* cdev = use_r ? lv.rcolor_dev.p : lv.color_dev.p;
+    const bool desc = reverse && !lex;
+    for (int cc = 0; cc < nc; ++cc) {
+        const int c = desc ? nc - 1 - cc : cc;
+        const int skip = first_from_zero ? (desc ? 2 : 1) : 0;
+        fn(rows + off[c], off[c + 1] - off[c], cdev, c, skip);
+    }
+}
+
 // Level::relax (multigrid.h:202-262); Gauss-Seidel is multicolour on the GPU:
 // colours in increasing order forward, decreasing order in reverse.
 template <typename T>
@@ -506,15 +540,12 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
     hipStream_t s = ctx->stream;
     const T omega = (T)prm.omega;
     if (Aop<T>(lv) == nullptr) throw Error(-4, "precision not built for this hierarchy");
-    if (prm.relaxation == 1) {
+    if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            for (int cc = 0; cc < lv.ncolor; ++cc) {
-                int c = reverse ? lv.ncolor - 1 - cc : cc;
-                int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
-                const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
-                dev::Ops<T>::gs_color(s, lv.B, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv),
-                                      v.b, v.x, omega, lv.color_dev, c, skip);
-            }
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
+                dev::Ops<T>::gs_color(s, lv.B, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega,
+                                      cdev, c, skip);
+            });
     } else {
         // Jacobi ping-pongs between x and xalt; the result ends in x
         if (zero_guess) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
@@ -544,7 +575,7 @@ void MGH::vcycle_from(int l0)
         dev::Ops<T>::restrict_(s, lv.B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): implicit in the first sweep of a
         // Gauss-Seidel pre-smoother, explicit otherwise
-        if (l + 1 < bottom && !(prm.relaxation == 1 && prm.npre > 0))
+        if (l + 1 < bottom && !(prm.relaxation >= 1 && prm.npre > 0))
             dev::Ops<T>::fill(s, (long)L[l + 1].nb * L[l + 1].B, (T)0, vc.x);
         zero = true;
     }
@@ -575,15 +606,12 @@ void MGH::mrelax(int l, int k, int nsw, bool reverse, bool zero_guess, double* x
     Level& lv = L[l];
     hipStream_t s = ctx->stream;
     const double omega = prm.omega;
-    if (prm.relaxation == 1) {
+    if (prm.relaxation >= 1) {
         for (int sw = 0; sw < nsw; ++sw)
-            for (int cc = 0; cc < lv.ncolor; ++cc) {
-                int c = reverse ? lv.ncolor - 1 - cc : cc;
-                int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
-                const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
-                dev::MOps<double>::gs_color(s, lv.B, k, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, lv.A64, lv.D64,
-                                            b, x, omega, lv.color_dev, c, skip);
-            }
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
+                dev::MOps<double>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, lv.A64, lv.D64, b, x, omega, cdev, c,
+                                            skip);
+            });
     } else {
         const long n = (long)lv.nb * lv.B * k;
         if (zero_guess) dev::Ops<double>::fill(s, n, 0.0, x);
@@ -651,12 +679,12 @@ void MGH::precon_enqueue(int precision)
     if (precision == 1) {
         if (!L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
         dev::f64_to_f32(s, n, r, L[0].v32.b);
-        if (!(prm.relaxation == 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
+        if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
         vcycle_from<float>(0);
         dev::f32_to_f64(s, n, L[0].v32.x, z);
     } else {
         dev::Ops<double>::copy(s, n, r, L[0].v64.b);
-        if (!(prm.relaxation == 1 && prm.npre > 0)) dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
+        if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
         vcycle_from<double>(0);
         dev::Ops<double>::copy(s, n, L[0].v64.x, z);
     }
@@ -803,15 +831,12 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     BatchLevel& v = BL[l];
     hipStream_t s = ctx->stream;
     const T omega = (T)prm.omega;
-    if (prm.relaxation == 1) {
+    if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            for (int cc = 0; cc < lv.ncolor; ++cc) {
-                int c = reverse ? lv.ncolor - 1 - cc : cc;
-                int c0 = lv.color_off[c], c1 = lv.color_off[c + 1];
-                const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
-                dev::MOps<T>::gs_color(s, lv.B, k, c1 - c0, lv.color_rows.p + c0, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv),
-                                       bb<T>(v), bx<T>(v), omega, lv.color_dev, c, skip);
-            }
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
+                dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), bx<T>(v),
+                                       omega, cdev, c, skip);
+            });
     } else {
         const long m = (long)lv.nb * lv.B * k;
         if (zero_guess) dev::Ops<T>::fill(s, m, (T)0, bx<T>(v));
@@ -1069,10 +1094,10 @@ void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* byte
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-    const int per = prm.relaxation == 1 ? lv.ncolor : 1;
+    const int per = prm.relaxation >= 1 ? lv.ncolor : 1;
     const double b2 = (double)lv.B * lv.B;
     double by;
-    if (prm.relaxation == 1)   // off-diagonal blocks + Dinv + cols + rowptr + colour rows + b, x in, x out
+    if (prm.relaxation >= 1)   // off-diagonal blocks + Dinv + cols + rowptr + colour rows + b, x in, x out
         by = (double)(lv.A.nnzb() - lv.nb) * b2 * sa + lv.nb * b2 * sa + 4.0 * lv.A.nnzb() + 8.0 * lv.nb + 4.0 * lv.nb + 3.0 * n * sv;
     else                       // all blocks + Dinv + indices + b, x in, x out
         by = (double)lv.A.nnzb() * b2 * sa + lv.nb * b2 * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 3.0 * n * sv;

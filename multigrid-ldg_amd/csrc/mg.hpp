@@ -101,9 +101,15 @@ struct Level {
     DBuf<double> A64, D64, Dinv_rm;     // panel layouts and row-major inverse
     DBuf<float> A32, D32;
     DBuf<double> Minv;                  // inverse coarse mass blocks (RATRAT)
-    std::vector<int> color, color_off;  // colouring and rows grouped by colour
+    // Gauss-Seidel schedule: rows grouped by colour (multicolour) or by
+    // wavefront of the lexicographic order (forward; the reverse sweep has its
+    // own wavefronts in rcolor*)
+    std::vector<int> color, color_off;
     int ncolor = 0;
     DBuf<int> color_rows, color_dev;
+    std::vector<int> rcolor, rcolor_off;
+    int rncolor = 0;
+    DBuf<int> rcolor_rows, rcolor_dev;
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     // interpolation to the next coarser level
@@ -163,6 +169,8 @@ struct MGH {
     ~MGH();
 
     template <typename T> void relax(int l, int n, bool reverse, bool zero_guess);
+    // the ordered GS launches of one sweep: fn(rows, nrows, colour array, current colour, skip mode)
+    template <typename F> void sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) const;
     template <typename T> void vcycle_from(int l);
     template <typename T> LevelVecs<T>& vecs(int l);
     void precon(int precision, bool use_graph);           // z = V(r)

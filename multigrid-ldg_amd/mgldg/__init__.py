@@ -156,7 +156,7 @@ class Multigrid(_Handle):
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
                  restriction="ratrat", precision="fp64", coarse_dense_max=4096):
         self.prob, self.ctx = prob, prob.ctx
-        self.params = MGParams({"jacobi": 0, "gs": 1}[relaxation], omega,
+        self.params = MGParams({"jacobi": 0, "gs": 1, "gs-lex": 2}[relaxation], omega,
                                {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
                                {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision],
                                coarse_dense_max)

@@ -138,22 +138,54 @@ def test_pcg_matches_oracle(gctx, oracle, mesh, p, criterion, collapse):
     assert rel(xm, xo) < 1e-8
 
 
-def test_config1_iteration_parity(gctx, oracle):
-    """C1: 32x32 p=1, V(2,2) block GS, MG-PCG to 1e-10.  GPU (multicolour) equals
-    the oracle under the same colouring; the reference's lexicographic count is
-    within +-2 of it."""
+@pytest.mark.parametrize("criterion", ["reference", "true"])
+def test_config1_iteration_parity(gctx, oracle, criterion):
+    """C1: 32x32 p=1, V(2,2) block GS, MG-PCG to 1e-10.  The lexicographic GS
+    (the reference's order, run as wavefronts) gives the reference algorithm's
+    iterates and iteration count; the multicolour GS equals the oracle under
+    the same colouring."""
     import mgldg
     qt = mgldg.Quadtree(gctx, 2, lambda x: 1 / 32)
     pr = mgldg.LDGPoisson(qt, 1)
-    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2)
     oqt = oracle.Quadtree(2, lambda x: 1 / 32)
     opr = oracle.Problem(oqt, 1)
     b = pr.rhs(sine_f(pr.nodes()))
-    x, st = mgldg.pcg(mg, b, tol=1e-10, maxit=200)
-    _, it_mc, _, _ = oracle.pcg(opr, oracle.Multigrid(opr, "mcgs", npre=2, npost=2), b, tol=1e-10, maxit=200)
-    _, it_lex, _, _ = oracle.pcg(opr, oracle.Multigrid(opr, "gs", npre=2, npost=2), b, tol=1e-10, maxit=200)
+    xl, it_lex, _, _ = oracle.pcg(opr, oracle.Multigrid(opr, "gs", npre=2, npost=2), b, tol=1e-10, maxit=200,
+                                  criterion=criterion)
+    _, it_mc, _, _ = oracle.pcg(opr, oracle.Multigrid(opr, "mcgs", npre=2, npost=2), b, tol=1e-10, maxit=200,
+                                criterion=criterion)
+    for collapse in (0, 4096):
+        mgl = mgldg.Multigrid(pr, "gs-lex", npre=2, npost=2, precision="mixed", coarse_dense_max=collapse)
+        x, st = mgldg.pcg(mgl, b, tol=1e-10, maxit=200, criterion=criterion)
+        assert st["iterations"] == it_lex and rel(x, xl) < 1e-10
+        xm, stm = mgldg.pcg(mgl, b, tol=1e-10, maxit=200, criterion=criterion, precision="mixed", flexible=True)
+        assert abs(stm["iterations"] - it_lex) <= 1 and rel(xm, xl) < 1e-8
+    mgc = mgldg.Multigrid(pr, "gs", npre=2, npost=2)
+    x, st = mgldg.pcg(mgc, b, tol=1e-10, maxit=200, criterion=criterion)
     assert st["iterations"] == it_mc
-    assert abs(st["iterations"] - it_lex) <= 2
+
+
+@pytest.mark.parametrize("mesh,p,rapid", [("uniform8", 2, False), ("adaptive", 2, True)])
+def test_lexicographic_gs_matches_reference_order(gctx, oracle, mesh, p, rapid):
+    """wavefront-scheduled GS equals the oracle's sequential lexicographic sweep
+    (multigrid.h:236-262) forward and reverse, on every level"""
+    import mgldg
+    import torch
+    qt, pr, oqt, opr = build_pair(gctx, oracle, mesh, p, rapid=rapid)
+    mg = mgldg.Multigrid(pr, "gs-lex", npre=2, npost=2)
+    omg = oracle.Multigrid(opr, "gs", npre=2, npost=2)
+    rng = np.random.default_rng(13)
+    for l in range(mg.nlevels):
+        n = mg.level_rows[l] * pr.npl
+        x, b = rng.standard_normal(n), rng.standard_normal(n)
+        for reverse in (False, True):
+            xs = torch.tensor(x, device="cuda")
+            mg.smooth(l, torch.tensor(b, device="cuda"), xs, 2, reverse)
+            assert rel(xs.cpu().numpy(), omg.relax(l, b, x, 2, reverse)) < 1e-12
+    bb = pr.rhs(rng.standard_normal(pr.ndof))
+    xo, ito, _, _ = oracle.pcg(opr, omg, bb, tol=1e-10, maxit=200, criterion="true")
+    x, st = mgldg.pcg(mg, bb, tol=1e-10, maxit=200, criterion="true")
+    assert st["iterations"] == ito and rel(x, xo) < 1e-9
 
 
 def test_graph_replay_matches_eager(gctx, oracle):
