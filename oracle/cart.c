@@ -823,12 +823,19 @@ static void mesh_lift(const omesh* Msh, const oface* f, double* L, double* R, do
 /* ======================================================================
  * LDG assembly (src/cartesian/dg/ldgpoisson.{h,tpp})
  * ====================================================================== */
+/* the operators of one discretisation (LDGOperators, ldgpoisson.h:15-53);
+ * the first member of every problem type */
 typedef struct {
+    int N, ne, b, ndof;
+    bsm M, T, A, G[3];
+} oops;
+
+typedef struct {
+    oops ops;
     omesh mesh;
     int b, ndof, nqb;
     int bctype[6];
     double tau0, tauD;
-    bsm M, T, A, G[3];
 } oproblem;
 
 /* LDGOperators::construct_laplacian (ldgpoisson.h:34-52): A = T + sum_d G_d^T M G_d */
@@ -867,18 +874,18 @@ void* orc_problem_new(void* qtp, int p, const int* bctype, double tau0, double t
     for (int k = 0; k < 2*N; ++k) pr->bctype[k] = bctype ? bctype[k] : 0;
     for (int k = 0; k < 2*N; ++k) if (pr->bctype[k] != 0 && pr->bctype[k] != 1) { orc_problem_free(pr); return NULL; }
 
-    bsm_init(&pr->M, ne, ne, b, b); bsm_init(&pr->T, ne, ne, b, b);
-    for (int d = 0; d < N; ++d) bsm_init(&pr->G[d], ne, ne, b, b);
+    bsm_init(&pr->ops.M, ne, ne, b, b); bsm_init(&pr->ops.T, ne, ne, b, b);
+    for (int d = 0; d < N; ++d) bsm_init(&pr->ops.G[d], ne, ne, b, b);
     double* blk = (double*)xmalloc(sizeof(double)*b*b);
     /* construct_mass_matrix (ldgpoisson.tpp:27-39) */
     for (int e = 0; e < ne; ++e) {
         double vol = 1.0; for (int i = 0; i < N; ++i) vol *= Msh->ehi[e*N+i] - Msh->elo[e*N+i];
-        bsm_add(&pr->M, e, e, vol, B->mass);
+        bsm_add(&pr->ops.M, e, e, vol, B->mass);
     }
     /* construct_broken_gradient (ldgpoisson.tpp:42-60) */
     for (int d = 0; d < N; ++d) for (int e = 0; e < ne; ++e) {
         double jac = 1.0/(Msh->ehi[e*N+d] - Msh->elo[e*N+d]);
-        bsm_add(&pr->G[d], e, e, jac, B->diff[d]);
+        bsm_add(&pr->ops.G[d], e, e, jac, B->diff[d]);
     }
     /* construct_lifting_terms (ldgpoisson.tpp:63-100) */
     double* LL = (double*)xmalloc(sizeof(double)*b*b); double* RR = (double*)xmalloc(sizeof(double)*b*b); double* RL = (double*)xmalloc(sizeof(double)*b*b);
@@ -891,34 +898,35 @@ void* orc_problem_new(void* qtp, int p, const int* bctype, double tau0, double t
             int r = f->right, l = f->left;
             double vol = 1.0; for (int i = 0; i < N; ++i) vol *= Msh->ehi[r*N+i] - Msh->elo[r*N+i];
             for (int k = 0; k < b*b; ++k) MI[k] = B->invmass[k]*(1.0/vol);
-            gemm(0, 0, b, b, b, n, MI, b, RR, b, 0.0, blk, b); bsm_add(&pr->G[f->dim], r, r, 1.0, blk);
-            gemm(0, 0, b, b, b, -n, MI, b, RL, b, 0.0, blk, b); bsm_add(&pr->G[f->dim], r, l, 1.0, blk);
-            bsm_add(&pr->T, r, r, tau0, RR);
-            bsm_add(&pr->T, l, l, tau0, LL);
-            bsm_add(&pr->T, r, l, -tau0, RL);
+            gemm(0, 0, b, b, b, n, MI, b, RR, b, 0.0, blk, b); bsm_add(&pr->ops.G[f->dim], r, r, 1.0, blk);
+            gemm(0, 0, b, b, b, -n, MI, b, RL, b, 0.0, blk, b); bsm_add(&pr->ops.G[f->dim], r, l, 1.0, blk);
+            bsm_add(&pr->ops.T, r, r, tau0, RR);
+            bsm_add(&pr->ops.T, l, l, tau0, LL);
+            bsm_add(&pr->ops.T, r, l, -tau0, RL);
             for (int i = 0; i < b; ++i) for (int j = 0; j < b; ++j) blk[i*b+j] = RL[j*b+i];
-            bsm_add(&pr->T, l, r, -tau0, blk);
+            bsm_add(&pr->ops.T, l, r, -tau0, blk);
         } else {
             int tag = -f->right - 1;
             if (pr->bctype[tag] == 0) {
                 int l = f->left;
                 double vol = 1.0; for (int i = 0; i < N; ++i) vol *= Msh->ehi[l*N+i] - Msh->elo[l*N+i];
                 for (int k = 0; k < b*b; ++k) MI[k] = B->invmass[k]*(1.0/vol);
-                gemm(0, 0, b, b, b, -n, MI, b, LL, b, 0.0, blk, b); bsm_add(&pr->G[f->dim], l, l, 1.0, blk);
-                bsm_add(&pr->T, l, l, tauD, LL);
+                gemm(0, 0, b, b, b, -n, MI, b, LL, b, 0.0, blk, b); bsm_add(&pr->ops.G[f->dim], l, l, 1.0, blk);
+                bsm_add(&pr->ops.T, l, l, tauD, LL);
             }
         }
     }
     free(LL); free(RR); free(RL); free(MI); free(blk);
-    construct_laplacian(&pr->A, &pr->T, &pr->M, pr->G, N);
+    construct_laplacian(&pr->ops.A, &pr->ops.T, &pr->ops.M, pr->ops.G, N);
+    pr->ops.N = N; pr->ops.ne = ne; pr->ops.b = b; pr->ops.ndof = ne*b;
     return pr;
 }
 
 void orc_problem_free(void* p)
 {
     oproblem* pr = (oproblem*)p; if (!pr) return;
-    bsm_free(&pr->M); bsm_free(&pr->T); bsm_free(&pr->A);
-    for (int d = 0; d < 3; ++d) bsm_free(&pr->G[d]);
+    bsm_free(&pr->ops.M); bsm_free(&pr->ops.T); bsm_free(&pr->ops.A);
+    for (int d = 0; d < 3; ++d) bsm_free(&pr->ops.G[d]);
     mesh_free(&pr->mesh); free(pr);
 }
 
@@ -990,11 +998,11 @@ int orc_problem_rhs(void* p, const double* fnod, const double* g, double* out)
     /* rhs_e -= sum_d sum_{i in rowsInCol(e)} G_d(i,e)^T Jg_d(i) */
     double* t = (double*)xmalloc(sizeof(double)*pr->ndof);
     for (int d = 0; d < N; ++d) {
-        bsm_mv_t(&pr->G[d], Jg[d], t);
+        bsm_mv_t(&pr->ops.G[d], Jg[d], t);
         for (long i = 0; i < pr->ndof; ++i) rhs[i] -= t[i];
     }
     for (int e = 0; e < ne; ++e) {
-        const double* Me = bsm_get(&pr->M, e, e);
+        const double* Me = bsm_get(&pr->ops.M, e, e);
         for (int i = 0; i < b; ++i) { double s = 0.0; if (fnod) for (int j = 0; j < b; ++j) s += Me[i*b+j]*fnod[(long)e*b+j]; out[(long)e*b+i] = rhs[(long)e*b+i] + s; }
     }
     free(t); free(L); free(Lg); free(rhs); for (int d = 0; d < N; ++d) free(Jg[d]);
@@ -1006,7 +1014,7 @@ int orc_problem_rhs_g_adjoint(void* p, const double* lam, double* dg)
 {
     oproblem* pr = (oproblem*)p; omesh* M = &pr->mesh; int N = M->N, b = pr->b, nq = M->nq;
     double* Glam[3] = {0, 0, 0};
-    for (int d = 0; d < N; ++d) { Glam[d] = (double*)xmalloc(sizeof(double)*pr->ndof); bsm_mv(&pr->G[d], 1.0, lam, 0.0, Glam[d]); }
+    for (int d = 0; d < N; ++d) { Glam[d] = (double*)xmalloc(sizeof(double)*pr->ndof); bsm_mv(&pr->ops.G[d], 1.0, lam, 0.0, Glam[d]); }
     double* L = (double*)xmalloc(sizeof(double)*b*nq);
     for (int k = 0; k < M->nbf; ++k) {
         const oface* f = &M->faces[M->bface[k]];
@@ -1023,7 +1031,7 @@ int orc_problem_rhs_g_adjoint(void* p, const double* lam, double* dg)
 long orc_problem_export(void* p, int which, int* rowptr, int* cols, double* blocks)
 {
     oproblem* pr = (oproblem*)p;
-    const bsm* A = which == 0 ? &pr->A : which == 1 ? &pr->M : which == 2 ? &pr->T : (which - 3 < pr->mesh.N && which >= 3) ? &pr->G[which-3] : NULL;
+    const bsm* A = which == 0 ? &pr->ops.A : which == 1 ? &pr->ops.M : which == 2 ? &pr->ops.T : (which - 3 < pr->mesh.N && which >= 3) ? &pr->ops.G[which-3] : NULL;
     if (!A) return -1;
     return bsm_export(A, rowptr, cols, blocks);
 }
@@ -1044,19 +1052,20 @@ typedef struct {
 } olevel;
 
 typedef struct {
-    oproblem* pr;
+    oops* ops;
+    int N;
     orc_mg_params prm;
     int L;                 /* number of levels */
     olevel* lev;
-    bsm* Tl;               /* L-1 interpolation operators */
+    bsm* Tl;               /* L-1 interpolation operators (one block per fine row) */
 } omg;
 
 /* InterpolationHierarchy<N,P>(qt) (interpolationhierarchy.h:30-82) */
-static void build_interp(omg* mg)
+static bsm* build_interp_qt(const oproblem* pr, int* nlev)
 {
-    oproblem* pr = mg->pr; qtree* qt = pr->mesh.qt; const basis_t* B = &pr->mesh.B;
+    const qtree* qt = pr->mesh.qt; const basis_t* B = &pr->mesh.B;
     int N = qt->N, P = B->P, b = B->npl, it[3];
-    mg->Tl = (bsm*)xcalloc(qt->numLevels - 1, sizeof(bsm));
+    bsm* Tl = (bsm*)xcalloc(qt->numLevels - 1, sizeof(bsm));
     int* cg2l = (int*)xmalloc(sizeof(int)*qt->n);
     double* E = (double*)xmalloc(sizeof(double)*b*b);
     double* I = (double*)xcalloc((size_t)b*b, sizeof(double));
@@ -1065,10 +1074,10 @@ static void build_interp(omg* mg)
         ivec fine = {0}, coarse = {0};
         qt_dfs_layer(qt, 0, l, &fine); qt_dfs_layer(qt, 0, l+1, &coarse);
         for (int k = 0; k < coarse.n; ++k) cg2l[coarse.v[k]] = k;
-        bsm_init(&mg->Tl[l], fine.n, coarse.n, b, b);
+        bsm_init(&Tl[l], fine.n, coarse.n, b, b);
         for (int fl = 0; fl < fine.n; ++fl) {
             int fg = fine.v[fl];
-            if (qt_isleaf(qt, fg, l+1)) { bsm_add(&mg->Tl[l], fl, cg2l[fg], 1.0, I); continue; }
+            if (qt_isleaf(qt, fg, l+1)) { bsm_add(&Tl[l], fl, cg2l[fg], 1.0, I); continue; }
             int cg = qt->nd[fg].parent;
             double rlo[3], rhi[3];
             for (int k = 0; k < N; ++k) {
@@ -1081,11 +1090,13 @@ static void build_interp(omg* mg)
                 for (int k = 0; k < N; ++k) x[k] = rlo[k] + (rhi[k] - rlo[k])*B->nodes[it[k]];
                 lagrange_eval_nd(B, x, E + i*b);
             }
-            bsm_add(&mg->Tl[l], fl, cg2l[cg], 1.0, E);
+            bsm_add(&Tl[l], fl, cg2l[cg], 1.0, E);
         }
         free(fine.v); free(coarse.v);
     }
     free(cg2l); free(E); free(I);
+    *nlev = qt->numLevels;
+    return Tl;
 }
 
 /* greedy colouring in block-row order: smallest colour unused by any
@@ -1101,7 +1112,6 @@ static void color_level(olevel* lv)
         int c = 0; while (used[c]) ++c;
         lv->color[i] = c; if (c + 1 > maxc) maxc = c + 1;
     }
-    /* a neighbour j > i that precedes i in no row is still checked through row j */
     lv->ncolor = maxc;
 }
 
@@ -1114,24 +1124,25 @@ static void level_factor(olevel* lv)
     color_level(lv);
 }
 
-/* Level coarse constructor (multigrid.h:105-171) */
+/* Level coarse constructor (multigrid.h:105-171), fine blocks bf, coarse bc
+ * (T blocks bf x bc; the reference's Level<N,P>(ops<N,P1>, T<N,P1,P>)) */
 static void coarsen(olevel* c, const olevel* f, const bsm* T, int restriction, int N)
 {
-    int b = f->b;
+    int bf = T->br, b = T->bc;
     memset(c, 0, sizeof(*c));
     c->nb = T->n; c->b = b;
     if (restriction == 1) {
         /* M_c = T^T M_f T (multigrid.h:117-130) */
         bsm_init(&c->M, T->n, T->n, b, b);
-        double* TTM = (double*)xmalloc(sizeof(double)*b*b); double* TTMT = (double*)xmalloc(sizeof(double)*b*b);
+        double* TTM = (double*)xmalloc(sizeof(double)*b*bf); double* TTMT = (double*)xmalloc(sizeof(double)*b*b);
         for (int k = 0; k < f->M.m; ++k) {
             const double* Mk = bsm_get(&f->M, k, k);
             for (int a = 0; a < T->len[k]; ++a) {
                 int i = T->col[k][a];
-                gemm(1, 0, b, b, b, 1.0, T->pool + T->blk[k][a], b, Mk, b, 0.0, TTM, b);
+                gemm(1, 0, b, bf, bf, 1.0, T->pool + T->blk[k][a], b, Mk, bf, 0.0, TTM, bf);
                 for (int cc = 0; cc < T->len[k]; ++cc) {
                     int j = T->col[k][cc];
-                    gemm(0, 0, b, b, b, 1.0, TTM, b, T->pool + T->blk[k][cc], b, 0.0, TTMT, b);
+                    gemm(0, 0, b, b, bf, 1.0, TTM, bf, T->pool + T->blk[k][cc], b, 0.0, TTMT, b);
                     bsm_add(&c->M, i, j, 1.0, TTMT);
                 }
             }
@@ -1162,22 +1173,17 @@ static void coarsen(olevel* c, const olevel* f, const bsm* T, int restriction, i
     level_factor(c);
 }
 
-void* orc_mg_new(void* p, const orc_mg_params* prm)
+/* Multigrid(ops, hierarchy, params) (multigrid.h:291-301, setup 404-411) for
+ * any hierarchy of one-block-per-row interpolation operators Tl[0..L-2] */
+static omg* mg_setup(oops* ops, int N, bsm* Tl, int L, const orc_mg_params* prm)
 {
-    oproblem* pr = (oproblem*)p;
-    if (!pr || !prm) return NULL;
-    if (prm->relaxation < 0 || prm->relaxation > 2 || prm->coarse < 0 || prm->coarse > 1 || prm->restriction < 0 || prm->restriction > 1) return NULL;
-    if (pr->mesh.qt->numLevels < 2) return NULL;   /* interpolationhierarchy.h:33 */
     omg* mg = (omg*)xcalloc(1, sizeof(omg));
-    mg->pr = pr; mg->prm = *prm;
-    build_interp(mg);
-    int L = pr->mesh.qt->numLevels, N = pr->mesh.N;
-    mg->L = L;
+    mg->ops = ops; mg->N = N; mg->prm = *prm; mg->Tl = Tl; mg->L = L;
     mg->lev = (olevel*)xcalloc(L, sizeof(olevel));
     olevel* f = &mg->lev[0];
-    f->nb = pr->mesh.ne; f->b = pr->b;
-    bsm_copy(&f->A, &pr->A); bsm_copy(&f->M, &pr->M); bsm_copy(&f->T, &pr->T);
-    for (int d = 0; d < N; ++d) bsm_copy(&f->G[d], &pr->G[d]);
+    f->nb = ops->ne; f->b = ops->b;
+    bsm_copy(&f->A, &ops->A); bsm_copy(&f->M, &ops->M); bsm_copy(&f->T, &ops->T);
+    for (int d = 0; d < N; ++d) bsm_copy(&f->G[d], &ops->G[d]);
     f->has_ops = 1;
     level_factor(f);
     for (int l = 0; l + 1 < L; ++l) coarsen(&mg->lev[l+1], &mg->lev[l], &mg->Tl[l], prm->restriction, N);
@@ -1192,6 +1198,21 @@ void* orc_mg_new(void* p, const orc_mg_params* prm)
         ldlt_factor(&c->dense, D, n); c->has_dense = 1; free(D);
     }
     return mg;
+}
+
+static int params_ok(const orc_mg_params* prm)
+{
+    return prm && !(prm->relaxation < 0 || prm->relaxation > 2 || prm->coarse < 0 || prm->coarse > 1 || prm->restriction < 0 || prm->restriction > 1);
+}
+
+void* orc_mg_new(void* p, const orc_mg_params* prm)
+{
+    oproblem* pr = (oproblem*)p;
+    if (!pr || !params_ok(prm)) return NULL;
+    if (pr->mesh.qt->numLevels < 2) return NULL;   /* interpolationhierarchy.h:33 */
+    int L = 0;
+    bsm* Tl = build_interp_qt(pr, &L);
+    return mg_setup(&pr->ops, pr->mesh.N, Tl, L, prm);
 }
 
 void orc_mg_free(void* p)
@@ -1210,6 +1231,13 @@ void orc_mg_free(void* p)
 
 int orc_mg_nlevels(void* p) { return ((omg*)p)->L; }
 
+int orc_mg_level_dims(void* p, int level, int* nb, int* b)
+{
+    omg* mg = (omg*)p; if (level < 0 || level >= mg->L) return -1;
+    *nb = mg->lev[level].nb; *b = mg->lev[level].b;
+    return 0;
+}
+
 long orc_mg_export(void* p, int level, int which, int* rowptr, int* cols, double* blocks, int* nbr, int* nbc)
 {
     omg* mg = (omg*)p;
@@ -1220,7 +1248,7 @@ long orc_mg_export(void* p, int level, int which, int* rowptr, int* cols, double
     else if (which == 0) A = &lv->A;
     else if (!lv->has_ops) return -1;
     else if (which == 1) A = &lv->M; else if (which == 2) A = &lv->T;
-    else if (which >= 3 && which - 3 < mg->pr->mesh.N) A = &lv->G[which-3];
+    else if (which >= 3 && which - 3 < mg->N) A = &lv->G[which-3];
     if (!A) return -1;
     if (nbr) *nbr = A->m;
     if (nbc) *nbc = A->n;
@@ -1343,7 +1371,7 @@ static void precon(omg* mg, const double* r, double* z, long n)
 int orc_pcg(void* p, void* m, const double* b, double* x, double tol, int maxit,
             int criterion, int flexible, double* stats, double* hist)
 {
-    oproblem* pr = (oproblem*)p; omg* mg = (omg*)m; const bsm* A = &pr->A; long n = pr->ndof;
+    oops* ops = (oops*)p; omg* mg = (omg*)m; const bsm* A = &ops->A; long n = ops->ndof;
     double* r = (double*)xmalloc(sizeof(double)*n); double* d = (double*)xmalloc(sizeof(double)*n);
     double* q = (double*)xmalloc(sizeof(double)*n); double* s = (double*)xmalloc(sizeof(double)*n);
     double* sold = flexible ? (double*)xmalloc(sizeof(double)*n) : NULL;

@@ -76,6 +76,7 @@ typedef struct {
 void* orc_mg_new(void* pr, const orc_mg_params* prm);
 void  orc_mg_free(void* mg);
 int   orc_mg_nlevels(void* mg);
+int   orc_mg_level_dims(void* mg, int level, int* nb, int* b);   /* block rows, block size */
 /* which: 0=A 1=M 2=T 3..5=G_d of level l ; 6 = interpolation T_l (level l -> l+1 cols) */
 long  orc_mg_export(void* mg, int level, int which, int* rowptr, int* cols, double* blocks,
                     int* nbrows, int* nbcols);
@@ -92,8 +93,36 @@ int   orc_mg_spmv(void* mg, int level, const double* x, double* y);
    mg may be NULL -> identity preconditioner (krylov.h:78-82).
    stats[0]=iterations, stats[1]=true relres, stats[2]=sqrt(r^T z / r0^T z0) ;
    hist (may be NULL) receives ||r|| per iteration (maxit entries). */
+/* pr: any problem handle (Cartesian or simplex) */
 int   orc_pcg(void* pr, void* mg, const double* b, double* x, double tol, int maxit,
               int criterion, int flexible, double* stats, double* hist);
+
+/* ---- unstructured simplex path (src/unstructured/{mesh,dg,multigrid}) ---- */
+/* Koornwinder nodal master element Master<N,P=p+1>: returns npl; nodes npl*N,
+   mass npl*npl, diff N*npl*npl (any may be NULL) */
+int   orc_sx_master(int N, int P, double* nodes, double* mass, double* diff);
+/* mesh + LDG assembly (mesh.h:185-298, ldgpoisson.tpp:14-99).  tris 0-based
+   (N+1 per simplex); vmask[v] bit k = boundary function k holds at vertex v
+   (nbnd functions; 0 = one boundary tag); bctype[max(nbnd,1)] 0=Dirichlet 1=Neumann */
+void* orc_sx_problem_new(int N, int np, const double* pts, int nt, const int* tris, int nbnd, const int* vmask,
+                         int p, const int* bctype, double tau0, double tauD);
+void  orc_sx_problem_free(void* pr);
+/* sizes[0..6] = ne, nf, nbf, npl, nqb, ndof, nq */
+int   orc_sx_sizes(void* pr, int* sizes);
+int   orc_sx_nodes(void* pr, double* xyz);
+int   orc_sx_faces(void* pr, int* lr, double* normal, int* verts);
+int   orc_sx_bc_qpoints(void* pr, double* xyz, int* tag);
+int   orc_sx_lift(void* pr, int face, double* L, double* LL, double* RR, double* RL);
+int   orc_sx_rhs(void* pr, const double* f, const double* g, double* b);
+int   orc_sx_rhs_g_adjoint(void* pr, const double* lam, double* dg);
+long  orc_sx_export(void* pr, int which, int* rowptr, int* cols, double* blocks);
+/* dual graph handed to MGridGen (agglomeration.h:57-93 for part = NULL, :133-185
+   for the agglomerates of part); returns the number of directed edges */
+int   orc_sx_agglom_graph(void* pr, const int* part, int nparts, int* xadj, int* adjncy,
+                          double* adjwgt, double* vvol, double* vsurf);
+/* agglomerated MG (interpolationhierarchy.h:88-204): parts[l] maps the
+   elements of level l to nparts[l] agglomerates; usable with orc_mg_* and orc_pcg */
+void* orc_sx_mg_new(void* pr, int nlev, const int* const* parts, const int* nparts, const orc_mg_params* prm);
 
 /* generic BSR helpers used by known-answer tests (sparseblockmatrix.tpp:288-301) */
 int   orc_bsr_spmv(int nbrows, int b, const int* rowptr, const int* cols,
