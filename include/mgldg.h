@@ -67,6 +67,7 @@ typedef struct mgldg_ctx_s*      mgldg_ctx;
 typedef struct mgldg_quadtree_s* mgldg_quadtree;
 typedef struct mgldg_problem_s*  mgldg_problem;
 typedef struct mgldg_mg_s*       mgldg_mg;
+typedef struct mgldg_wireframe_s* mgldg_wireframe;
 
 /* size function: N widths for the point x.  A cell splits iff every width of
  * the cell exceeds h(center) (src/cartesian/mesh/quadtree.tpp:15). */
@@ -95,6 +96,18 @@ int mgldg_quadtree_info(mgldg_quadtree qt, int* info);
 int mgldg_quadtree_layer(mgldg_quadtree qt, int l, int* ids, int* count);
 int mgldg_quadtree_destroy(mgldg_quadtree qt);
 
+/* Wireframe<N>(p, t) and Wireframe<N>(file) (src/unstructured/mesh/wireframe.h:12-77):
+ * the points (np x N) and simplices (nt x (N+1)) of an unstructured mesh.
+ * mgldg_wireframe_create takes 0-based vertex ids; mgldg_wireframe_read reads
+ * the reference's .mesh text format (header "np nt", a blank line, np point
+ * lines, a blank line, nt lines of 1-based vertex ids). */
+int mgldg_wireframe_create(mgldg_ctx ctx, int N, int np, const double* points, int nt,
+                           const int* simplices, mgldg_wireframe* out);
+int mgldg_wireframe_read(mgldg_ctx ctx, int N, const char* path, mgldg_wireframe* out);
+/* info[0]=N info[1]=np info[2]=nt; points / simplices (0-based) may be NULL */
+int mgldg_wireframe_get(mgldg_wireframe wf, int* info, double* points, int* simplices);
+int mgldg_wireframe_destroy(mgldg_wireframe wf);
+
 /* ======================= assembly ======================= */
 /* Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
  * (src/cartesian/mesh/mesh.h:186-265, src/cartesian/dg/ldgpoisson.h:60-77,
@@ -103,11 +116,24 @@ int mgldg_quadtree_destroy(mgldg_quadtree qt);
  * later as values at the boundary quadrature points (mgldg_problem_bc_qpoints). */
 int mgldg_ldg_create(mgldg_quadtree qt, int p, const int* bctype, double tau0, double tauD,
                      mgldg_problem* out);
+/* Mesh<N,P>(wireframe, bndfuns) + BoundaryConditions + LDGPoisson<N,P>(mesh,
+ * bcs, tau0, tauD) on simplices (src/unstructured/mesh/mesh.h:185-298,
+ * src/unstructured/dg/ldgpoisson.tpp:14-170).  The boundary predicates
+ * bndfuns[k] enter as vertex masks: bit k of vmask[v] says whether predicate k
+ * holds at vertex v; a boundary face gets key k+1 for the first k holding at
+ * all of its vertices (mesh.h:276-288).  nbnd = 0 (vmask NULL): every boundary
+ * face has key 1.  bctype[max(nbnd,1)]: MGLDG_DIRICHLET / MGLDG_NEUMANN per key.
+ * Faces are numbered in first-appearance order (simplex i, local face j); the
+ * reference's order is that of a hash map. */
+int mgldg_ldg_create_simplex(mgldg_wireframe wf, int p, int nbnd, const int* vmask,
+                             const int* bctype, double tau0, double tauD, mgldg_problem* out);
 /* sizes[0..6] = ne, nf, nbfaces, npl, nqb (boundary quadrature points), ndof, N */
 int mgldg_problem_sizes(mgldg_problem pr, int* sizes);
 /* Master::dgnodes for every element (master.h:35-43): xyz[ndof*N] */
 int mgldg_problem_nodes(mgldg_problem pr, double* xyz);
-/* boundary quadrature points in boundary-face order (ldgpoisson.tpp:119-129) */
+/* boundary quadrature points in boundary-face order (ldgpoisson.tpp:119-129);
+ * tag: the boundary key of each point (quadtree: -(2d+1) / -(2d+2); simplex:
+ * 1..nbnd) */
 int mgldg_problem_bc_qpoints(mgldg_problem pr, double* xyz, int* tag);
 /* LDGPoisson::computeRHS (ldgpoisson.tpp:167-176) plus add_source_terms
  * (:103-161): b = M f + B_g g.  Host buffers; f[ndof*nrhs] nodal values,
@@ -141,10 +167,31 @@ typedef struct {
  * The symbolic sparsity of every Galerkin product is formed on the host; the
  * numeric block triple products, block inversions and colouring-ordered
  * operator layouts are built on the GPU. */
+/* On a simplex problem: the library's own agglomeration (below) followed by
+ * the agglomerated hierarchy. */
 int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out);
+/* Agglomeration<N,P>(mesh, maxAgglom) (src/unstructured/mesh/agglomeration.h:21-34)
+ * with the library's MGridGen-style agglomerator (clusters of at most 2^N
+ * agglomerates of the finer level, chosen for small perimeter^N/volume^(N-1)),
+ * stopping at one agglomerate, max_levels partitions, or a level that no
+ * longer shrinks.  Outputs: *nlevels partitions; nparts[l] agglomerates made
+ * by partition l; parts[l*ne + i] = agglomerate of level-l element i
+ * (capacity ne * max_levels; may be NULL). */
+int mgldg_agglomerate(mgldg_problem pr, int max_levels, int* nlevels, int* nparts, int* parts);
+/* InterpolationHierarchy<N,P>(agglomeration) + multigrid for a simplex problem
+ * (src/unstructured/multigrid/interpolationhierarchy.h:88-204; the reference
+ * has no unstructured Multigrid and its constructor does not compile, :96,150,217).
+ * parts[l*ne + i], l < nlev: level-l element i -> level-(l+1) agglomerate
+ * (level 0 = the mesh, ne its element count).  Level 0 blocks are P_p simplex
+ * bases; coarse levels are tensor Q_p Gauss-Lobatto bases on the agglomerates'
+ * bounding boxes (of their vertices). */
+int mgldg_mg_create_agglomerated(mgldg_problem pr, const mgldg_mg_params* prm, int nlev,
+                                 const int* parts, mgldg_mg* out);
 /* info[0]=levels, info[1]=ndof, info[2]=npl; then per level l < 16:
  * info[3+2l]=block rows, info[4+2l]=stored blocks */
 int mgldg_mg_info(mgldg_mg mg, long* info, int ninfo);
+/* block rows, block size and stored blocks of level l */
+int mgldg_mg_level(mgldg_mg mg, int level, int* nb, int* B, long* nnzb);
 /* Export a level operator (copied back from the GPU): which 0=A 1=M 2=T 3+d=G_d
  * (RATRAT levels only), 6 = interpolation T_l (rows level l, cols level l+1).
  * Exported patterns are the host symbolic phase; values come from the GPU. */

@@ -3,7 +3,9 @@
 // records the message on the context.
 #include <cstring>
 #include <exception>
+#include <fstream>
 #include <new>
+#include <sstream>
 
 #include "../../include/mgldg.h"
 #include "kernels.hpp"
@@ -15,6 +17,7 @@ struct mgldg_ctx_s { Ctx c; };
 struct mgldg_quadtree_s { Ctx* ctx; Quadtree qt; };
 struct mgldg_problem_s { Ctx* ctx; ProblemH p; };
 struct mgldg_mg_s { Ctx* ctx; MGH m; };
+struct mgldg_wireframe_s { Ctx* ctx; int N; std::vector<double> p; std::vector<int> t; };
 
 static thread_local std::string g_err;
 
@@ -115,6 +118,91 @@ int mgldg_quadtree_layer(mgldg_quadtree qt, int l, int* ids, int* count)
 
 int mgldg_quadtree_destroy(mgldg_quadtree qt) { delete qt; return MGLDG_OK; }
 
+int mgldg_wireframe_create(mgldg_ctx ctx, int N, int np, const double* points, int nt, const int* simplices,
+                           mgldg_wireframe* out)
+{
+    if (!ctx || !out) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(&ctx->c, [&] {
+        ARG(N == 2 || N == 3, "dimension must be 2 or 3");
+        ARG(np >= N + 1 && nt >= 1 && points && simplices, "empty wireframe");
+        auto* w = new mgldg_wireframe_s();
+        w->ctx = &ctx->c; w->N = N;
+        w->p.assign(points, points + (size_t)np * N);
+        w->t.assign(simplices, simplices + (size_t)nt * (N + 1));
+        for (int v : w->t) if (v < 0 || v >= np) { delete w; throw Error(MGLDG_ERR_ARG, "vertex id out of range"); }
+        *out = w;
+    });
+}
+
+// Wireframe<N>(file) (wireframe.h:22-68): header "np nt", one skipped line,
+// point lines up to an empty line, simplex lines (1-based) up to an empty line
+int mgldg_wireframe_read(mgldg_ctx ctx, int N, const char* path, mgldg_wireframe* out)
+{
+    if (!ctx || !out || !path) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(&ctx->c, [&] {
+        ARG(N == 2 || N == 3, "dimension must be 2 or 3");
+        std::ifstream in(path);
+        ARG(in.is_open(), "cannot open mesh file");
+        std::string line;
+        int np = 0, nt = 0;
+        ARG((bool)std::getline(in, line), "empty mesh file");
+        { std::istringstream ss(line); ARG((bool)(ss >> np >> nt), "bad mesh header"); }
+        ARG(np >= N + 1 && nt >= 1, "bad mesh header");
+        std::getline(in, line);
+        std::vector<double> p;
+        std::vector<int> t;
+        p.reserve((size_t)np * N); t.reserve((size_t)nt * (N + 1));
+        while (std::getline(in, line) && !line.empty()) {
+            std::istringstream ss(line);
+            double v; int k = 0;
+            while (ss >> v) { p.push_back(v); ++k; }
+            ARG(k == N, "point line with the wrong number of coordinates");
+        }
+        while (std::getline(in, line) && !line.empty()) {
+            std::istringstream ss(line);
+            int v, k = 0;
+            while (ss >> v) { t.push_back(v - 1); ++k; }
+            ARG(k == N + 1, "simplex line with the wrong number of vertices");
+        }
+        ARG((long)p.size() == (long)np * N && (long)t.size() == (long)nt * (N + 1), "mesh file counts do not match its header");
+        for (int v : t) ARG(v >= 0 && v < np, "vertex id out of range");
+        auto* w = new mgldg_wireframe_s();
+        w->ctx = &ctx->c; w->N = N; w->p = std::move(p); w->t = std::move(t);
+        *out = w;
+    });
+}
+
+int mgldg_wireframe_get(mgldg_wireframe wf, int* info, double* points, int* simplices)
+{
+    if (!wf) return MGLDG_ERR_ARG;
+    const int np = (int)(wf->p.size() / wf->N), nt = (int)(wf->t.size() / (wf->N + 1));
+    if (info) { info[0] = wf->N; info[1] = np; info[2] = nt; }
+    if (points) std::memcpy(points, wf->p.data(), wf->p.size() * sizeof(double));
+    if (simplices) std::memcpy(simplices, wf->t.data(), wf->t.size() * sizeof(int));
+    return MGLDG_OK;
+}
+
+int mgldg_wireframe_destroy(mgldg_wireframe wf) { delete wf; return MGLDG_OK; }
+
+int mgldg_ldg_create_simplex(mgldg_wireframe wf, int p, int nbnd, const int* vmask, const int* bctype, double tau0,
+                             double tauD, mgldg_problem* out)
+{
+    if (!wf || !out) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(wf->ctx, [&] {
+        ARG(p >= 0 && p <= 5, "Invalid polynomial order.");
+        ARG(nbnd >= 0 && nbnd <= 30 && (nbnd == 0 || vmask), "invalid boundary predicates");
+        const int N = wf->N, np = (int)(wf->p.size() / N), nt = (int)(wf->t.size() / (N + 1));
+        auto* h = new mgldg_problem_s();
+        h->ctx = wf->ctx;
+        try { h->p.build_simplex(wf->ctx, N, np, wf->p.data(), nt, wf->t.data(), nbnd, vmask, p, bctype, tau0, tauD); }
+        catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+
 int mgldg_ldg_create(mgldg_quadtree qt, int p, const int* bctype, double tau0, double tauD, mgldg_problem* out)
 {
     if (!qt || !out) return MGLDG_ERR_ARG;
@@ -133,26 +221,26 @@ int mgldg_problem_sizes(mgldg_problem pr, int* s)
 {
     if (!pr || !s) return MGLDG_ERR_ARG;
     const ProblemH& p = pr->p;
-    s[0] = p.ne; s[1] = p.host.mesh.nf; s[2] = p.host.mesh.nbf; s[3] = p.b; s[4] = p.nqb; s[5] = (int)p.ndof; s[6] = p.N;
+    s[0] = p.ne; s[1] = p.host->nf; s[2] = p.host->nbf; s[3] = p.b; s[4] = p.nqb; s[5] = (int)p.ndof; s[6] = p.N;
     return MGLDG_OK;
 }
 
 int mgldg_problem_nodes(mgldg_problem pr, double* xyz)
 {
     if (!pr || !xyz) return MGLDG_ERR_ARG;
-    return guard(pr->ctx, [&] { pr->p.host.nodes_xyz(xyz); });
+    return guard(pr->ctx, [&] { pr->p.host->nodes_xyz(xyz); });
 }
 
 int mgldg_problem_bc_qpoints(mgldg_problem pr, double* xyz, int* tag)
 {
     if (!pr || !xyz) return MGLDG_ERR_ARG;
-    return guard(pr->ctx, [&] { pr->p.host.bc_qpoints(xyz, tag); });
+    return guard(pr->ctx, [&] { pr->p.host->bc_qpoints(xyz, tag); });
 }
 
 int mgldg_rhs(mgldg_problem pr, const double* f, const double* g, int nrhs, double* b)
 {
     if (!pr || !b || nrhs < 1) return MGLDG_ERR_ARG;
-    return guard(pr->ctx, [&] { pr->p.host.rhs(f, g, nrhs, b); });
+    return guard(pr->ctx, [&] { pr->p.host->rhs(f, g, nrhs, b); });
 }
 
 static void export_op(Ctx* ctx, const DevOp& op, int* rowptr, int* cols, double* blocks, long* nnzb)
@@ -180,21 +268,76 @@ int mgldg_problem_export(mgldg_problem pr, int which, int* rowptr, int* cols, do
 
 int mgldg_problem_destroy(mgldg_problem pr) { delete pr; return MGLDG_OK; }
 
+static MGParams mg_params(const mgldg_mg_params* prm)
+{
+    MGParams m;
+    m.relaxation = prm->relaxation; m.omega = prm->omega; m.coarse = prm->coarse;
+    m.npre = prm->npre; m.npost = prm->npost; m.restriction = prm->restriction; m.precision = prm->precision;
+    m.coarse_dense_max = prm->coarse_dense_max;
+    ARG(m.coarse_dense_max >= 0 && m.coarse_dense_max <= 8192, "coarse_dense_max must be in [0, 8192]");
+    return m;
+}
+
+static void mg_build(mgldg_problem pr, const MGParams& m, const std::vector<std::vector<int>>* parts, mgldg_mg* out)
+{
+    auto* h = new mgldg_mg_s();
+    h->ctx = pr->ctx;
+    try { h->m.build(&pr->p, m, parts); }
+    catch (...) { delete h; throw; }
+    *out = h;
+}
+
 int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out)
 {
     if (!pr || !prm || !out) return MGLDG_ERR_ARG;
     *out = nullptr;
     return guard(pr->ctx, [&] {
-        MGParams m;
-        m.relaxation = prm->relaxation; m.omega = prm->omega; m.coarse = prm->coarse;
-        m.npre = prm->npre; m.npost = prm->npost; m.restriction = prm->restriction; m.precision = prm->precision;
-        m.coarse_dense_max = prm->coarse_dense_max;
-        ARG(m.coarse_dense_max >= 0 && m.coarse_dense_max <= 8192, "coarse_dense_max must be in [0, 8192]");
-        auto* h = new mgldg_mg_s();
-        h->ctx = pr->ctx;
-        try { h->m.build(&pr->p, m); }
-        catch (...) { delete h; throw; }
-        *out = h;
+        MGParams m = mg_params(prm);
+        if (const SimplexLdg* sx = pr->p.simplex()) {
+            std::vector<std::vector<int>> parts = agglomerate(sx->mesh, 10, 1 << sx->N);
+            ARG(!parts.empty(), "the mesh cannot be agglomerated");
+            mg_build(pr, m, &parts, out);
+        } else {
+            mg_build(pr, m, nullptr, out);
+        }
+    });
+}
+
+int mgldg_agglomerate(mgldg_problem pr, int max_levels, int* nlevels, int* nparts, int* parts)
+{
+    if (!pr || !nlevels || max_levels < 1) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] {
+        const SimplexLdg* sx = pr->p.simplex();
+        ARG(sx, "agglomeration applies to simplex problems");
+        std::vector<std::vector<int>> q = agglomerate(sx->mesh, max_levels, 1 << sx->N);
+        *nlevels = (int)q.size();
+        for (size_t l = 0; l < q.size(); ++l) {
+            int nc = 0;
+            for (int v : q[l]) nc = std::max(nc, v + 1);
+            if (nparts) nparts[l] = nc;
+            if (parts) std::memcpy(parts + l * (size_t)sx->ne, q[l].data(), q[l].size() * sizeof(int));
+        }
+    });
+}
+
+int mgldg_mg_create_agglomerated(mgldg_problem pr, const mgldg_mg_params* prm, int nlev, const int* parts, mgldg_mg* out)
+{
+    if (!pr || !prm || !out || !parts || nlev < 1) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(pr->ctx, [&] {
+        MGParams m = mg_params(prm);
+        const SimplexLdg* sx = pr->p.simplex();
+        ARG(sx, "agglomerated multigrid applies to simplex problems");
+        std::vector<std::vector<int>> q(nlev);
+        int n = sx->ne;
+        for (int l = 0; l < nlev; ++l) {
+            const int* src = parts + (size_t)l * sx->ne;
+            q[l].assign(src, src + n);
+            int nc = 0;
+            for (int v : q[l]) { ARG(v >= 0 && v < n, "agglomerate id out of range"); nc = std::max(nc, v + 1); }
+            n = nc;
+        }
+        mg_build(pr, m, &q, out);
     });
 }
 
@@ -204,6 +347,16 @@ int mgldg_mg_info(mgldg_mg mg, long* info, int ninfo)
     const MGH& m = mg->m;
     info[0] = (long)m.L.size(); info[1] = m.pr->ndof; info[2] = m.pr->b;
     for (size_t l = 0; l < m.L.size() && 4 + 2 * (int)l < ninfo; ++l) { info[3 + 2 * l] = m.L[l].nb; info[4 + 2 * l] = m.L[l].A.nnzb(); }
+    return MGLDG_OK;
+}
+
+int mgldg_mg_level(mgldg_mg mg, int level, int* nb, int* B, long* nnzb)
+{
+    if (!mg || level < 0 || level >= (int)mg->m.L.size()) return MGLDG_ERR_ARG;
+    const Level& lv = mg->m.L[level];
+    if (nb) *nb = lv.nb;
+    if (B) *B = lv.B;
+    if (nnzb) *nnzb = lv.A.nnzb();
     return MGLDG_OK;
 }
 

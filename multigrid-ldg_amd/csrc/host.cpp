@@ -553,33 +553,43 @@ void LdgHost::assemble(const Quadtree& qt, int p_, const int* bct, double t0, do
             add_block(T, f.left, f.left, tauD, LL.data());
         }
     }
-    // boundary lifting blocks for the RHS and its adjoint
-    Lb.assign((size_t)mesh.nbf * b * nq, 0.0);
-    for (int k = 0; k < mesh.nbf; ++k) lift(mesh, basis, mesh.faces[mesh.bfaces[k]], &Lb[(size_t)k * b * nq], nullptr, nullptr, nullptr);
+    // boundary lifting blocks and face data for the RHS and its adjoint
+    this->ne = ne; nf = mesh.nf; nbf = mesh.nbf;
+    Lb.assign((size_t)nbf * b * nq, 0.0);
+    bleft.resize(nbf); btype.resize(nbf); btag.resize(nbf); bnormal.assign((size_t)nbf * N, 0.0);
+    for (int k = 0; k < nbf; ++k) {
+        const Face& f = mesh.faces[mesh.bfaces[k]];
+        lift(mesh, basis, f, &Lb[(size_t)k * b * nq], nullptr, nullptr, nullptr);
+        bleft[k] = f.left; btype[k] = bctype[-f.right - 1]; btag[k] = f.right;
+        bnormal[(size_t)k * N + f.dim] = f.sign;
+    }
 }
 
 // add_source_terms + computeRHS (ldgpoisson.tpp:103-176), host, [ndof][nrhs]
-void LdgHost::rhs(const double* f, const double* g, int nrhs, double* out) const
+void LdgOps::rhs(const double* f, const double* g, int nrhs, double* out) const
 {
-    const int b = npl, ne = mesh.ne;
+    const int b = npl;
     const size_t ndof = (size_t)ne * b;
-    std::vector<double> r(ndof), J[3], t(ndof), col(ndof), Lg(b);
+    std::vector<double> r(ndof), J[3], Lg(b);
     for (int c = 0; c < nrhs; ++c) {
         std::fill(r.begin(), r.end(), 0.0);
         for (int d = 0; d < N; ++d) J[d].assign(ndof, 0.0);
         if (g) {
-            for (int k = 0; k < mesh.nbf; ++k) {
-                const Face& fc = mesh.faces[mesh.bfaces[k]];
+            for (int k = 0; k < nbf; ++k) {
                 const double* L = &Lb[(size_t)k * b * nq];
                 for (int i = 0; i < b; ++i) {
                     double s = 0;
                     for (int q = 0; q < nq; ++q) s += L[(size_t)i * nq + q] * g[((size_t)k * nq + q) * nrhs + c];
                     Lg[i] = s;
                 }
-                const size_t off = (size_t)fc.left * b;
-                if (bctype[-fc.right - 1] == 0)
-                    for (int i = 0; i < b; ++i) { J[fc.dim][off + i] += fc.sign * Lg[i]; r[off + i] += tauD * Lg[i]; }
-                else
+                const size_t off = (size_t)bleft[k] * b;
+                if (btype[k] == 0) {
+                    for (int d = 0; d < N; ++d) {
+                        const double n = bnormal[(size_t)k * N + d];
+                        if (n != 0.0) for (int i = 0; i < b; ++i) J[d][off + i] += n * Lg[i];
+                    }
+                    for (int i = 0; i < b; ++i) r[off + i] += tauD * Lg[i];
+                } else
                     for (int i = 0; i < b; ++i) r[off + i] += Lg[i];
             }
             // r_e -= sum_d sum_i G_d(i,e)^T J_d(i)
@@ -597,7 +607,7 @@ void LdgHost::rhs(const double* f, const double* g, int nrhs, double* out) const
             }
         }
         for (int e = 0; e < ne; ++e) {
-            const double* Me = M.block(e);   // M is diagonal: slot e
+            const double* Me = M.block(M.find(e, e));
             for (int i = 0; i < b; ++i) {
                 double s = 0;
                 if (f) for (int j = 0; j < b; ++j) s += Me[(size_t)i * b + j] * f[((size_t)e * b + j) * nrhs + c];

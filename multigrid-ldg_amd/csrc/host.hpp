@@ -112,20 +112,33 @@ struct Mesh {
 void lift(const Mesh& m, const Basis& B, const Face& f, double* L, double* LL, double* RR, double* RL);
 
 // ---------------------------------------------------------------- LDG
-struct LdgHost {
-    int N = 2, p = 1, npl = 0, nq = 0;
+// The assembled discretisation handed to the device, whatever the mesh:
+// LDGOperators {M, G_d, T} (ldgpoisson.h:15-53; A is formed on the GPU) and
+// the boundary data of add_source_terms (ldgpoisson.tpp:103-161).
+struct LdgOps {
+    int N = 2, p = 1, npl = 0, nq = 0, ne = 0, nf = 0, nbf = 0;
     double tau0 = 0, tauD = 0;
+    HostBSR M, T, G[3];
+    std::vector<double> Lb;                // boundary lift blocks, nbf x (npl x nq)
+    std::vector<int> bleft, btype, btag;   // per boundary face: element, 0 Dirichlet / 1 Neumann, boundary key
+    std::vector<double> bnormal;           // per boundary face: outward unit normal (N)
+    virtual ~LdgOps() = default;
+    // b = M f + B_g g (ldgpoisson.tpp:103-176), host, [ndof][nrhs]
+    void rhs(const double* f, const double* g, int nrhs, double* b) const;
+    // boundary quadrature points (nbf*nq x N) and their boundary keys
+    virtual void bc_qpoints(double* xyz, int* tag) const = 0;
+    virtual void nodes_xyz(double* xyz) const = 0;
+};
+
+// Quadtree discretisation: Mesh<N,P>(qt) + LDGPoisson (src/cartesian/dg/ldgpoisson.tpp)
+struct LdgHost : LdgOps {
     int bctype[6] = {0, 0, 0, 0, 0, 0};
     Basis basis;
     Mesh mesh;
-    HostBSR M, T, G[3];                    // assembled operators (A is formed on the GPU)
-    std::vector<double> Lb;                // boundary lift blocks, nbf x (npl x nq)
     // LDGPoisson::discretize minus construct_laplacian (ldgpoisson.tpp:14-100)
     void assemble(const Quadtree& qt, int p, const int* bct, double tau0, double tauD);
-    // b = M f + B_g g (ldgpoisson.tpp:103-176), host
-    void rhs(const double* f, const double* g, int nrhs, double* b) const;
-    void bc_qpoints(double* xyz, int* tag) const;
-    void nodes_xyz(double* xyz) const;
+    void bc_qpoints(double* xyz, int* tag) const override;
+    void nodes_xyz(double* xyz) const override;
 };
 
 // Symbolic phase of C = sum over terms of opL(L) * X * R.  For each output

@@ -412,42 +412,44 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     }
 }
 
-template <typename T, int B>
+// Interpolation blocks are BF x BC row-major (fine simplex or box basis x
+// coarse box basis); BF == BC on quadtree hierarchies.
+template <typename T, int BF, int BC>
 __global__ __launch_bounds__(256) void k_restrict(int nc, const int* __restrict__ kid_off, const int* __restrict__ kids,
                                                   const int* __restrict__ type, const T* __restrict__ pool,
                                                   const T* __restrict__ r, T* __restrict__ bc)
 {
-    using M = Map<T, B>;
+    using M = Map<T, BC>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / B, c = lane - slot * B;
+    const int slot = lane / BC, c = lane - slot * BC;
     const int I = blockIdx.x * M::RPB + wave * M::RPW + slot;
     if (slot >= M::RPW || I >= nc) return;
     T acc = 0;
     for (int kk = kid_off[I]; kk < kid_off[I + 1]; ++kk) {
         const int k = kids[kk];
-        const T* Tb = pool + (size_t)type[k] * (B * B);
-        const T* rk = r + (size_t)k * B;
+        const T* Tb = pool + (size_t)type[k] * (BF * BC);
+        const T* rk = r + (size_t)k * BF;
 #pragma unroll
-        for (int q = 0; q < B; ++q) acc += Tb[q * B + c] * rk[q];
+        for (int q = 0; q < BF; ++q) acc += Tb[q * BC + c] * rk[q];
     }
-    bc[(size_t)I * B + c] = acc;
+    bc[(size_t)I * BC + c] = acc;
 }
 
-template <typename T, int B>
+template <typename T, int BF, int BC>
 __global__ __launch_bounds__(256) void k_prolong(int nf, const int* __restrict__ parent, const int* __restrict__ type,
                                                  const T* __restrict__ pool, const T* __restrict__ xc, T* __restrict__ xf)
 {
-    using M = Map<T, B>;
+    using M = Map<T, BF>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / B, r = lane - slot * B;
+    const int slot = lane / BF, r = lane - slot * BF;
     const int k = blockIdx.x * M::RPB + wave * M::RPW + slot;
     if (slot >= M::RPW || k >= nf) return;
-    const T* Tb = pool + (size_t)type[k] * (B * B) + r * B;
-    const T* xp = xc + (size_t)parent[k] * B;
+    const T* Tb = pool + (size_t)type[k] * (BF * BC) + r * BC;
+    const T* xp = xc + (size_t)parent[k] * BC;
     T acc = 0;
 #pragma unroll
-    for (int c = 0; c < B; ++c) acc += Tb[c] * xp[c];
-    xf[(size_t)k * B + r] += acc;
+    for (int c = 0; c < BC; ++c) acc += Tb[c] * xp[c];
+    xf[(size_t)k * BF + r] += acc;
 }
 
 template <typename T>
@@ -508,6 +510,22 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
         default: throw std::runtime_error("block size " + std::to_string(B) + " not compiled"); \
     }
 
+// dispatch over the compiled (fine, coarse) interpolation block shapes: the
+// square sizes and P_p simplex -> Q_p box for 2-D p = 1..3 and 3-D p = 1, 2
+#define MGLDG_DISPATCH2(BF, BC, ...)                                                 \
+    if ((BF) == (BC)) {                                                              \
+        MGLDG_DISPATCH(BF, { constexpr int FF = BB, CC = BB; __VA_ARGS__; });        \
+    } else {                                                                         \
+        switch ((BF) * 1000 + (BC)) {                                                \
+            case 3004: { constexpr int FF = 3, CC = 4; __VA_ARGS__; } break;         \
+            case 6009: { constexpr int FF = 6, CC = 9; __VA_ARGS__; } break;         \
+            case 10016: { constexpr int FF = 10, CC = 16; __VA_ARGS__; } break;      \
+            case 4008: { constexpr int FF = 4, CC = 8; __VA_ARGS__; } break;         \
+            case 10027: { constexpr int FF = 10, CC = 27; __VA_ARGS__; } break;      \
+            default: throw std::runtime_error("interpolation block " + std::to_string(BF) + "x" + std::to_string(BC) + " not compiled"); \
+        }                                                                            \
+    }
+
 template <typename T>
 void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
                    const T* b, T* y, const T* dotv, double* dot_out, double* part)
@@ -551,24 +569,25 @@ void Ops<T>::jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const in
 }
 
 template <typename T>
-void Ops<T>::restrict_(hipStream_t s, int B, int nc, const int* kid_off, const int* kids, const int* type, const T* pool,
-                       const T* r, T* bc)
+void Ops<T>::restrict_(hipStream_t s, int BF, int BC, int nc, const int* kid_off, const int* kids, const int* type,
+                       const T* pool, const T* r, T* bc)
 {
     if (nc <= 0) return;
-    MGLDG_DISPATCH(B, {
-        using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_restrict<T, BB>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc, kid_off, kids, type, pool, r, bc);
+    MGLDG_DISPATCH2(BF, BC, {
+        using M = Map<T, CC>;
+        hipLaunchKernelGGL((k_restrict<T, FF, CC>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc, kid_off, kids, type, pool, r, bc);
     });
     check_launch("restrict");
 }
 
 template <typename T>
-void Ops<T>::prolong(hipStream_t s, int B, int nf, const int* parent, const int* type, const T* pool, const T* xc, T* xf)
+void Ops<T>::prolong(hipStream_t s, int BF, int BC, int nf, const int* parent, const int* type, const T* pool, const T* xc,
+                     T* xf)
 {
     if (nf <= 0) return;
-    MGLDG_DISPATCH(B, {
-        using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_prolong<T, BB>), dim3(nblk(nf, M::RPB)), dim3(256), 0, s, nf, parent, type, pool, xc, xf);
+    MGLDG_DISPATCH2(BF, BC, {
+        using M = Map<T, FF>;
+        hipLaunchKernelGGL((k_prolong<T, FF, CC>), dim3(nblk(nf, M::RPB)), dim3(256), 0, s, nf, parent, type, pool, xc, xf);
     });
     check_launch("prolong");
 }
@@ -726,7 +745,7 @@ __global__ __launch_bounds__(256) void mk_jacobi(int k, int nrows, const int* __
     }
 }
 
-template <typename T, int B>
+template <typename T, int BF, int BC>
 __global__ __launch_bounds__(256) void mk_restrict(int k, int nc, const int* __restrict__ kid_off, const int* __restrict__ kids,
                                                    const int* __restrict__ type, const T* __restrict__ pool,
                                                    const T* __restrict__ r, T* __restrict__ bc)
@@ -734,42 +753,42 @@ __global__ __launch_bounds__(256) void mk_restrict(int k, int nc, const int* __r
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long)nc * k) return;
     const int I = (int)(e / k), c = (int)(e % k);
-    T acc[B];
+    T acc[BC];
 #pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] = 0;
+    for (int q = 0; q < BC; ++q) acc[q] = 0;
     for (int t = kid_off[I]; t < kid_off[I + 1]; ++t) {
         const int f = kids[t];
-        const T* Tb = pool + (size_t)type[f] * B * B;
-        const T* rf = r + (size_t)f * B * k + c;
+        const T* Tb = pool + (size_t)type[f] * BF * BC;
+        const T* rf = r + (size_t)f * BF * k + c;
 #pragma unroll
-        for (int q = 0; q < B; ++q) {
+        for (int q = 0; q < BF; ++q) {
             const T rv = rf[(size_t)q * k];
 #pragma unroll
-            for (int p = 0; p < B; ++p) acc[p] += Tb[q * B + p] * rv;
+            for (int p = 0; p < BC; ++p) acc[p] += Tb[q * BC + p] * rv;
         }
     }
 #pragma unroll
-    for (int q = 0; q < B; ++q) bc[((size_t)I * B + q) * k + c] = acc[q];
+    for (int q = 0; q < BC; ++q) bc[((size_t)I * BC + q) * k + c] = acc[q];
 }
 
-template <typename T, int B>
+template <typename T, int BF, int BC>
 __global__ __launch_bounds__(256) void mk_prolong(int k, int nf, const int* __restrict__ parent, const int* __restrict__ type,
                                                   const T* __restrict__ pool, const T* __restrict__ xc, T* __restrict__ xf)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long)nf * k) return;
     const int f = (int)(e / k), c = (int)(e % k);
-    const T* Tb = pool + (size_t)type[f] * B * B;
-    const T* xp = xc + (size_t)parent[f] * B * k + c;
-    T xv[B];
+    const T* Tb = pool + (size_t)type[f] * BF * BC;
+    const T* xp = xc + (size_t)parent[f] * BC * k + c;
+    T xv[BC];
 #pragma unroll
-    for (int q = 0; q < B; ++q) xv[q] = xp[(size_t)q * k];
+    for (int q = 0; q < BC; ++q) xv[q] = xp[(size_t)q * k];
 #pragma unroll
-    for (int r = 0; r < B; ++r) {
+    for (int r = 0; r < BF; ++r) {
         T acc = 0;
 #pragma unroll
-        for (int q = 0; q < B; ++q) acc += Tb[r * B + q] * xv[q];
-        xf[((size_t)f * B + r) * k + c] += acc;
+        for (int q = 0; q < BC; ++q) acc += Tb[r * BC + q] * xv[q];
+        xf[((size_t)f * BF + r) * k + c] += acc;
     }
 }
 
@@ -822,21 +841,21 @@ void MOps<T>::jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, 
     check_launch("mk_jacobi");
 }
 template <typename T>
-void MOps<T>::restrict_(hipStream_t s, int B, int k, int nc, const int* kid_off, const int* kids, const int* type,
+void MOps<T>::restrict_(hipStream_t s, int BF, int BC, int k, int nc, const int* kid_off, const int* kids, const int* type,
                         const T* pool, const T* r, T* bc)
 {
     long n = (long)nc * k; if (n <= 0) return;
-    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_restrict<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nc, kid_off, kids, type,
-                                         pool, r, bc));
+    MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_restrict<T, FF, CC>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nc, kid_off,
+                                               kids, type, pool, r, bc));
     check_launch("mk_restrict");
 }
 template <typename T>
-void MOps<T>::prolong(hipStream_t s, int B, int k, int nf, const int* parent, const int* type, const T* pool, const T* xc,
-                      T* xf)
+void MOps<T>::prolong(hipStream_t s, int BF, int BC, int k, int nf, const int* parent, const int* type, const T* pool,
+                      const T* xc, T* xf)
 {
     long n = (long)nf * k; if (n <= 0) return;
-    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_prolong<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nf, parent, type, pool,
-                                         xc, xf));
+    MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_prolong<T, FF, CC>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nf, parent,
+                                               type, pool, xc, xf));
     check_launch("mk_prolong");
 }
 template <typename T>
@@ -1145,8 +1164,8 @@ void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, co
 
 // ---------------------------------------------------------------------- boundary data map
 __global__ void k_bc_forward(int ne, int B, int nq, int k, int N, const int* __restrict__ bf_off, const int* __restrict__ bf_list,
-                             const int* __restrict__ bdim, const int* __restrict__ btype, const double* __restrict__ bsign,
-                             double tauD, const double* __restrict__ Lb, const double* __restrict__ g, double* __restrict__ r,
+                             const int* __restrict__ btype, const double* __restrict__ bnormal, double tauD,
+                             const double* __restrict__ Lb, const double* __restrict__ g, double* __restrict__ r,
                              double* __restrict__ J0, double* __restrict__ J1, double* __restrict__ J2)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1161,8 +1180,10 @@ __global__ void k_bc_forward(int ne, int B, int nq, int k, int N, const int* __r
         const double* gf = g + (size_t)f * nq * k + c;
         double lg = 0;
         for (int q = 0; q < nq; ++q) lg += L[q] * gf[(size_t)q * k];
-        if (btype[f] == 0) { rr += tauD * lg; jj[bdim[f]] += bsign[f] * lg; }
-        else rr += lg;
+        if (btype[f] == 0) {
+            rr += tauD * lg;
+            for (int d = 0; d < N; ++d) jj[d] += bnormal[(size_t)f * N + d] * lg;
+        } else rr += lg;
     }
     r[e] = rr;
     J0[e] = jj[0];
@@ -1170,18 +1191,18 @@ __global__ void k_bc_forward(int ne, int B, int nq, int k, int N, const int* __r
     if (N > 2) J2[e] = jj[2];
 }
 
-void bc_forward(hipStream_t s, int ne, int B, int nq, int k, int N, const int* bf_off, const int* bf_list, const int* bdim,
-                const int* btype, const double* bsign, double tauD, const double* Lb, const double* g, double* r, double* J0,
-                double* J1, double* J2)
+void bc_forward(hipStream_t s, int ne, int B, int nq, int k, int N, const int* bf_off, const int* bf_list,
+                const int* btype, const double* bnormal, double tauD, const double* Lb, const double* g, double* r,
+                double* J0, double* J1, double* J2)
 {
     long t = (long)ne * B * k; if (t <= 0) return;
-    hipLaunchKernelGGL(k_bc_forward, dim3(nblk(t, 256)), dim3(256), 0, s, ne, B, nq, k, N, bf_off, bf_list, bdim, btype, bsign,
+    hipLaunchKernelGGL(k_bc_forward, dim3(nblk(t, 256)), dim3(256), 0, s, ne, B, nq, k, N, bf_off, bf_list, btype, bnormal,
                        tauD, Lb, g, r, J0, J1, J2);
     check_launch("bc_forward");
 }
 
-__global__ void k_bc_adjoint(int nbf, int B, int nq, int k, const int* __restrict__ bleft, const int* __restrict__ bdim,
-                             const int* __restrict__ btype, const double* __restrict__ bsign, double tauD,
+__global__ void k_bc_adjoint(int nbf, int B, int nq, int k, int N, const int* __restrict__ bleft,
+                             const int* __restrict__ btype, const double* __restrict__ bnormal, double tauD,
                              const double* __restrict__ Lb, const double* __restrict__ lam, const double* __restrict__ G0,
                              const double* __restrict__ G1, const double* __restrict__ G2, double* __restrict__ dg)
 {
@@ -1190,26 +1211,31 @@ __global__ void k_bc_adjoint(int nbf, int B, int nq, int k, const int* __restric
     const int c = (int)(e % k);
     const long fq = e / k;
     const int q = (int)(fq % nq), f = (int)(fq / nq);
-    const int l = bleft[f], d = bdim[f];
-    const double* Gl = d == 0 ? G0 : (d == 1 ? G1 : G2);
+    const int l = bleft[f];
     const bool dir = btype[f] == 0;
-    const double n = bsign[f];
+    double n[3] = {0, 0, 0};
+    for (int d = 0; d < N; ++d) n[d] = bnormal[(size_t)f * N + d];
     const double* L = Lb + (size_t)f * B * nq + q;
     double acc = 0;
     for (int i = 0; i < B; ++i) {
         const size_t o = ((size_t)l * B + i) * k + c;
-        const double v = dir ? tauD * lam[o] - n * Gl[o] : lam[o];
+        double v = lam[o];
+        if (dir) {
+            v = tauD * v - n[0] * G0[o];
+            if (N > 1) v -= n[1] * G1[o];
+            if (N > 2) v -= n[2] * G2[o];
+        }
         acc += L[(size_t)i * nq] * v;
     }
     dg[e] = acc;
 }
 
-void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, const int* bleft, const int* bdim, const int* btype,
-                const double* bsign, double tauD, const double* Lb, const double* lam, const double* G0, const double* G1,
+void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, int N, const int* bleft, const int* btype,
+                const double* bnormal, double tauD, const double* Lb, const double* lam, const double* G0, const double* G1,
                 const double* G2, double* dg)
 {
     long t = (long)nbf * nq * k; if (t <= 0) return;
-    hipLaunchKernelGGL(k_bc_adjoint, dim3(nblk(t, 256)), dim3(256), 0, s, nbf, B, nq, k, bleft, bdim, btype, bsign, tauD, Lb,
+    hipLaunchKernelGGL(k_bc_adjoint, dim3(nblk(t, 256)), dim3(256), 0, s, nbf, B, nq, k, N, bleft, btype, bnormal, tauD, Lb,
                        lam, G0, G1, G2, dg);
     check_launch("bc_adjoint");
 }

@@ -40,11 +40,12 @@ struct Ops {
     // damped block Jacobi, x_out = x + omega Dinv (b - A x) (multigrid.h:223-229)
     static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);
-    // b_c = T^T r (multigrid.h:414-418) as a gather over children
-    static void restrict_(hipStream_t s, int B, int nc, const int* kid_off, const int* kids, const int* type,
+    // b_c = T^T r (multigrid.h:414-418) as a gather over children; T blocks
+    // are BF x BC (fine x coarse basis)
+    static void restrict_(hipStream_t s, int BF, int BC, int nc, const int* kid_off, const int* kids, const int* type,
                           const T* pool, const T* r, T* bc);
     // x_f += T x_c (multigrid.h:421-425)
-    static void prolong(hipStream_t s, int B, int nf, const int* parent, const int* type, const T* pool,
+    static void prolong(hipStream_t s, int BF, int BC, int nf, const int* parent, const int* type, const T* pool,
                         const T* xc, T* xf);
     // coarsest level: b -= mean(b); x = Cinv b, Cinv FP64 dense n x n (multigrid.h:365-368)
     static void coarse(hipStream_t s, int n, const double* Cinv, const T* b, T* x, int subtract_mean);
@@ -64,10 +65,10 @@ struct MOps {
                          const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
     static void jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);
-    static void restrict_(hipStream_t s, int B, int k, int nc, const int* kid_off, const int* kids, const int* type,
-                          const T* pool, const T* r, T* bc);
-    static void prolong(hipStream_t s, int B, int k, int nf, const int* parent, const int* type, const T* pool,
-                        const T* xc, T* xf);
+    static void restrict_(hipStream_t s, int BF, int BC, int k, int nc, const int* kid_off, const int* kids,
+                          const int* type, const T* pool, const T* r, T* bc);
+    static void prolong(hipStream_t s, int BF, int BC, int k, int nf, const int* parent, const int* type,
+                        const T* pool, const T* xc, T* xf);
     // per column: b -= mean(b) (optional) then x = Cinv b (Cinv FP64 n x n); Cinv null: only the mean
     static void coarse(hipStream_t s, int n, int k, const double* Cinv, T* b, T* x, int subtract_mean);
 };
@@ -111,16 +112,17 @@ void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, co
 // ---------------------------------------------------------------- boundary data map B_g (ldgpoisson.tpp:103-161)
 // Forward gather per element e (its boundary faces bf_off[e]..bf_off[e+1]):
 //   r_e  = sum_f w_f L_f g_f     (w_f = tauD for Dirichlet, 1 for Neumann)
-//   J_d,e = sum_{Dirichlet f, dim d} n_f L_f g_f
-// L_f is npl x nq row-major; g [nbf*nq][k]; r, J [ndof][k] (J: N arrays).
+//   J_d,e = sum_{Dirichlet f} n_{f,d} L_f g_f
+// L_f is npl x nq row-major; g [nbf*nq][k]; r, J [ndof][k] (J: N arrays);
+// bnormal nbf x N outward unit normals.
 void bc_forward(hipStream_t s, int ne, int B, int nq, int k, int N, const int* bf_off, const int* bf_list,
-                const int* bdim, const int* btype, const double* bsign, double tauD, const double* Lb,
+                const int* btype, const double* bnormal, double tauD, const double* Lb,
                 const double* g, double* r, double* J0, double* J1, double* J2);
 // Adjoint per boundary face f, point q: dg = L_f^T v_f with
-//   v_f = tauD lam_l - n_f (G_dim lam)_l (Dirichlet), lam_l (Neumann).
+//   v_f = tauD lam_l - sum_d n_{f,d} (G_d lam)_l (Dirichlet), lam_l (Neumann).
 // Glam: N arrays [ndof][k] holding G_d lam.
-void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, const int* bleft, const int* bdim, const int* btype,
-                const double* bsign, double tauD, const double* Lb, const double* lam, const double* G0,
+void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, int N, const int* bleft, const int* btype,
+                const double* bnormal, double tauD, const double* Lb, const double* lam, const double* G0,
                 const double* G1, const double* G2, double* dg);
 // y = a + alpha b (k columns flattened)
 void axpby(hipStream_t s, long n, double a_scale, const double* a, double b_scale, const double* b, double* y);

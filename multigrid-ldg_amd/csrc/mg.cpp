@@ -123,51 +123,67 @@ void ProblemH::build(Ctx* c, const Quadtree* q, int p, const int* bct, double ta
 {
     ctx = c; qt = q;
     auto t0 = Clock::now();
-    host.assemble(*q, p, bct, tau0, tauD);
+    auto h = std::make_unique<LdgHost>();
+    h->assemble(*q, p, bct, tau0, tauD);
+    host = std::move(h);
     setup_host_s = secs(t0);
-    N = host.N; b = host.npl; ne = host.mesh.ne; ndof = (long)ne * b; nqb = host.mesh.nbf * host.nq;
+    upload();
+}
+
+void ProblemH::build_simplex(Ctx* c, int N_, int np, const double* pts, int nt, const int* tris, int nbnd,
+                             const int* vmask, int p, const int* bct, double tau0, double tauD)
+{
+    ctx = c; qt = nullptr;
+    auto t0 = Clock::now();
+    auto h = std::make_unique<SimplexLdg>();
+    h->assemble(N_, np, pts, nt, tris, nbnd, vmask, p, bct, tau0, tauD);
+    host = std::move(h);
+    setup_host_s = secs(t0);
+    upload();
+}
+
+void ProblemH::upload()
+{
+    const LdgOps& H = *host;
+    N = H.N; b = H.npl; ne = H.ne; ndof = (long)ne * b; nqb = H.nbf * H.nq;
     hipStream_t s = ctx->stream;
     auto t1 = Clock::now();
-    auto up = [&](DevOp& D, const HostBSR& H) { D.upload_pattern(H, s); D.val.upload(H.val.data(), H.val.size(), s); };
-    up(M, host.M); up(T, host.T);
-    for (int d = 0; d < N; ++d) up(G[d], host.G[d]);
+    auto up = [&](DevOp& D, const HostBSR& X) { D.upload_pattern(X, s); D.val.upload(X.val.data(), X.val.size(), s); };
+    up(M, H.M); up(T, H.T);
+    for (int d = 0; d < N; ++d) up(G[d], H.G[d]);
     build_laplacian(s, N, b, T, G, M, A);
     // boundary data for the device RHS / adjoint
-    Lb.upload(host.Lb.data(), host.Lb.size(), s);
-    std::vector<int> bl(host.mesh.nbf), bd(host.mesh.nbf), bt(host.mesh.nbf);
-    std::vector<double> bs(host.mesh.nbf);
-    for (int k = 0; k < host.mesh.nbf; ++k) {
-        const Face& f = host.mesh.faces[host.mesh.bfaces[k]];
-        bl[k] = f.left; bd[k] = f.dim; bt[k] = host.bctype[-f.right - 1]; bs[k] = f.sign;
-    }
-    bleft.upload(bl.data(), bl.size(), s); bdim.upload(bd.data(), bd.size(), s);
-    btype.upload(bt.data(), bt.size(), s); bsign.upload(bs.data(), bs.size(), s);
-    std::vector<int> off(ne + 1, 0), lst(host.mesh.nbf);
-    for (int k = 0; k < host.mesh.nbf; ++k) off[bl[k] + 1]++;
+    const int nbf = H.nbf;
+    Lb.upload(H.Lb.data(), std::max<size_t>(H.Lb.size(), 1), s);
+    bleft.upload(H.bleft.data(), std::max(nbf, 1), s);
+    btype.upload(H.btype.data(), std::max(nbf, 1), s);
+    bnormal.upload(H.bnormal.data(), std::max<size_t>(H.bnormal.size(), 1), s);
+    std::vector<int> off(ne + 1, 0), lst(std::max(nbf, 1));
+    for (int k = 0; k < nbf; ++k) off[H.bleft[k] + 1]++;
     for (int e = 0; e < ne; ++e) off[e + 1] += off[e];
     {
         std::vector<int> pos(off.begin(), off.end() - 1);
-        for (int k = 0; k < host.mesh.nbf; ++k) lst[pos[bl[k]]++] = k;
+        for (int k = 0; k < nbf; ++k) lst[pos[H.bleft[k]]++] = k;
     }
-    bf_off.upload(off.data(), off.size(), s); bf_list.upload(lst.data(), std::max<size_t>(lst.size(), 1), s);
+    bf_off.upload(off.data(), off.size(), s); bf_list.upload(lst.data(), lst.size(), s);
     // G_d (panel) and G_d^T (pattern transpose, transposed blocks, panel) for B_g and its adjoint
     for (int d = 0; d < N; ++d) {
         Gpan[d].alloc(G[d].val.n);
         dev::to_panel_f64(s, G[d].nnzb(), b, G[d].val, Gpan[d]);
-        const HostBSR& Gh = host.G[d];
+        const HostBSR& Gh = H.G[d];
         PatternBuilder pb(ne, ne);
         for (int i = 0; i < ne; ++i) for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) pb.add(Gh.cols[t], i);
-        HostBSR T; pb.finish(T, b, b);
+        HostBSR Tr; pb.finish(Tr, b, b);
         for (int i = 0; i < ne; ++i)
             for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) {
-                double* dst = T.block(T.find(Gh.cols[t], i));
+                double* dst = Tr.block(Tr.find(Gh.cols[t], i));
                 const double* src = Gh.block(t);
                 for (int r = 0; r < b; ++r) for (int q = 0; q < b; ++q) dst[(size_t)q * b + r] = src[(size_t)r * b + q];
             }
-        GT[d].upload_pattern(T, s);
-        DBuf<double> rm; rm.upload(T.val.data(), T.val.size(), s);
-        GT[d].val.alloc(T.val.size());
-        dev::to_panel_f64(s, T.nnzb(), b, rm, GT[d].val);
+        GT[d].upload_pattern(Tr, s);
+        DBuf<double> rm; rm.upload(Tr.val.data(), Tr.val.size(), s);
+        GT[d].val.alloc(Tr.val.size());
+        dev::to_panel_f64(s, Tr.nnzb(), b, rm, GT[d].val);
         HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -184,7 +200,7 @@ void ProblemH::rhs_device(const double* f, const double* g, int k, double* bout)
     DBuf<double> r, J[3], t;
     r.alloc(n); t.alloc(n);
     for (int d = 0; d < N; ++d) J[d].alloc(n);
-    dev::bc_forward(s, ne, b, host.nq, k, N, bf_off, bf_list, bdim, btype, bsign, host.tauD, Lb, g, r, J[0], J[1], J[2]);
+    dev::bc_forward(s, ne, b, host->nq, k, N, bf_off, bf_list, btype, bnormal, host->tauD, Lb, g, r, J[0], J[1], J[2]);
     dev::axpby(s, (long)n, 1.0, bout, 1.0, r, bout);
     for (int d = 0; d < N; ++d) {
         dev::MOps<double>::apply(s, b, k, ne, GT[d].rowptr, GT[d].cols, GT[d].val, J[d], nullptr, t);
@@ -204,7 +220,7 @@ void ProblemH::adjoint_maps(const double* lam, int k, double* dLdf, double* dLdg
             Gl[d].alloc(n);
             dev::MOps<double>::apply(s, b, k, ne, G[d].rowptr, G[d].cols, Gpan[d], lam, nullptr, Gl[d]);
         }
-        dev::bc_adjoint(s, host.mesh.nbf, b, host.nq, k, bleft, bdim, btype, bsign, host.tauD, Lb, lam, Gl[0],
+        dev::bc_adjoint(s, host->nbf, b, host->nq, k, N, bleft, btype, bnormal, host->tauD, Lb, lam, Gl[0],
                         N > 1 ? Gl[1].p : nullptr, N > 2 ? Gl[2].p : nullptr, dLdg);
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -298,65 +314,66 @@ static std::vector<int> diag_slots(const HostBSR& P)
 }
 
 // Level coarse constructor (multigrid.h:105-171): symbolic on the host,
-// numeric block triple products on the GPU
+// numeric block triple products on the GPU.  Interpolation blocks are bf x bc
+// (bf = bc on quadtree hierarchies).
 static void coarsen(hipStream_t s, int N, Level& f, Level& c, int restriction)
 {
     const Interp& I = f.I;
-    const int b = c.B;
+    const int bf = f.B, bc = c.B;
     if (restriction == 1) {
         // M_c = T^T M_f T
         HostBSR pat; TermList t;
         rap_symbolic(I, f.M.pat, pat, t);
         c.M.upload_pattern(pat, s);
-        c.M.val.alloc((size_t)pat.nnzb() * b * b);
-        run_triple(s, t, b, b, b, b, 1, f.pool64, f.M.val, f.pool64, nullptr, nullptr, c.M.val);
+        c.M.val.alloc((size_t)pat.nnzb() * bc * bc);
+        run_triple(s, t, bc, bc, bf, bf, 1, f.pool64, f.M.val, f.pool64, nullptr, nullptr, c.M.val);
         // inverse coarse mass blocks (the reference's Minv LDLT, multigrid.h:133-135)
         std::vector<int> ds = diag_slots(pat);
         DBuf<int> dsl; dsl.upload(ds.data(), ds.size(), s);
-        c.Minv.alloc((size_t)c.nb * b * b);
-        dev::inverse(s, c.nb, b, c.M.val, dsl, c.Minv);
+        c.Minv.alloc((size_t)c.nb * bc * bc);
+        dev::inverse(s, c.nb, bc, c.M.val, dsl, c.Minv);
         HIPCHK(hipStreamSynchronize(s));
         // G_c = M_c^{-1} T^T (M_f G_f) T
+        std::vector<int> mslot = diag_slots(f.M.pat);
         for (int d = 0; d < N; ++d) {
             DevOp& Gf = f.G[d];
             // MG = M_f G_f (block row scaling), pattern of G_f
             TermList tm; tm.off.resize(Gf.nnzb() + 1); tm.l.assign(Gf.nnzb(), -1); tm.x.resize(Gf.nnzb()); tm.r.resize(Gf.nnzb());
-            std::vector<int> mslot = diag_slots(f.M.pat);
             for (int i = 0; i < Gf.pat.nbr; ++i)
                 for (int k = Gf.pat.rowptr[i]; k < Gf.pat.rowptr[i + 1]; ++k) { tm.off[k] = k; tm.x[k] = mslot[i]; tm.r[k] = k; }
             tm.off[Gf.nnzb()] = (int)Gf.nnzb();
-            DBuf<double> MG; MG.alloc((size_t)Gf.nnzb() * b * b);
-            run_triple(s, tm, b, b, b, b, 0, nullptr, f.M.val, Gf.val, nullptr, nullptr, MG);
+            DBuf<double> MG; MG.alloc((size_t)Gf.nnzb() * bf * bf);
+            run_triple(s, tm, bf, bf, bf, bf, 0, nullptr, f.M.val, Gf.val, nullptr, nullptr, MG);
             HostBSR gp; TermList ty;
             rap_symbolic(I, Gf.pat, gp, ty);
-            DBuf<double> Y; Y.alloc((size_t)gp.nnzb() * b * b);
-            run_triple(s, ty, b, b, b, b, 1, f.pool64, MG, f.pool64, nullptr, nullptr, Y);
+            DBuf<double> Y; Y.alloc((size_t)gp.nnzb() * bc * bc);
+            run_triple(s, ty, bc, bc, bf, bf, 1, f.pool64, MG, f.pool64, nullptr, nullptr, Y);
             // G_c(i,j) = Minv_c(i) Y(i,j)
             TermList tg; tg.off.resize(gp.nnzb() + 1); tg.l.resize(gp.nnzb()); tg.x.resize(gp.nnzb()); tg.r.assign(gp.nnzb(), -1);
             for (int i = 0; i < gp.nbr; ++i)
                 for (int k = gp.rowptr[i]; k < gp.rowptr[i + 1]; ++k) { tg.off[k] = k; tg.l[k] = i; tg.x[k] = k; }
             tg.off[gp.nnzb()] = (int)gp.nnzb();
             c.G[d].upload_pattern(gp, s);
-            c.G[d].val.alloc((size_t)gp.nnzb() * b * b);
-            run_triple(s, tg, b, b, b, b, 0, c.Minv, Y, nullptr, nullptr, nullptr, c.G[d].val);
+            c.G[d].val.alloc((size_t)gp.nnzb() * bc * bc);
+            run_triple(s, tg, bc, bc, bc, bc, 0, c.Minv, Y, nullptr, nullptr, nullptr, c.G[d].val);
         }
         // T_c = T^T T_f T
         {
             HostBSR tp; TermList tt;
             rap_symbolic(I, f.Tp.pat, tp, tt);
             c.Tp.upload_pattern(tp, s);
-            c.Tp.val.alloc((size_t)tp.nnzb() * b * b);
-            run_triple(s, tt, b, b, b, b, 1, f.pool64, f.Tp.val, f.pool64, nullptr, nullptr, c.Tp.val);
+            c.Tp.val.alloc((size_t)tp.nnzb() * bc * bc);
+            run_triple(s, tt, bc, bc, bf, bf, 1, f.pool64, f.Tp.val, f.pool64, nullptr, nullptr, c.Tp.val);
         }
-        build_laplacian(s, N, b, c.Tp, c.G, c.M, c.A);
+        build_laplacian(s, N, bc, c.Tp, c.G, c.M, c.A);
         c.has_ops = true;
     } else {
         // A_c = T^T A_f T
         HostBSR ap; TermList ta;
         rap_symbolic(I, f.A.pat, ap, ta);
         c.A.upload_pattern(ap, s);
-        c.A.val.alloc((size_t)ap.nnzb() * b * b);
-        run_triple(s, ta, b, b, b, b, 1, f.pool64, f.A.val, f.pool64, nullptr, nullptr, c.A.val);
+        c.A.val.alloc((size_t)ap.nnzb() * bc * bc);
+        run_triple(s, ta, bc, bc, bf, bf, 1, f.pool64, f.A.val, f.pool64, nullptr, nullptr, c.A.val);
     }
 }
 
@@ -421,7 +438,7 @@ static void upload_interp(hipStream_t s, Level& lv, int precision)
     }
 }
 
-void MGH::build(ProblemH* p, const MGParams& pm)
+void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<int>>* parts)
 {
     pr = p; ctx = p->ctx; prm = pm;
     if (prm.relaxation < 0 || prm.relaxation > 2) throw Error(-1, "Unknown relaxation method.");   // multigrid.h:213
@@ -429,16 +446,39 @@ void MGH::build(ProblemH* p, const MGParams& pm)
     if (prm.restriction < 0 || prm.restriction > 1) throw Error(-1, "Unknown restriction method."); // multigrid.h:164
     if (prm.precision < 0 || prm.precision > 1) throw Error(-1, "Unknown precision.");
     if (prm.npre < 0 || prm.npost < 0) throw Error(-1, "negative sweep count");
-    const Quadtree& qt = *p->qt;
-    if (qt.num_levels < 2) throw Error(-1, "the quadtree needs at least two levels");  // interpolationhierarchy.h:33
+    const SimplexLdg* sx = p->simplex();
+    int nl = 0;
+    Basis Q;   // coarse box basis of an agglomerated hierarchy
+    if (sx) {
+        if (!parts || parts->empty()) throw Error(-1, "an agglomerated hierarchy needs at least one partition");
+        // RATRAT inverts the coarse mass matrices T^T M T, which Q_p box spaces
+        // over a few P_p simplices can make singular (a lone simplex has fewer
+        // nodes than box functions; two triangles' four vertices need not
+        // determine a bilinear function)
+        if (prm.restriction != 0) throw Error(-1, "agglomerated hierarchies coarsen by RAT (MGLDG_RAT) only");
+        int n = sx->ne;
+        for (const auto& q : *parts) {
+            if ((int)q.size() != n) throw Error(-1, "partition size does not match its level");
+            int nc = 0;
+            for (int v : q) { if (v < 0 || v >= n) throw Error(-1, "agglomerate id out of range"); nc = std::max(nc, v + 1); }
+            n = nc;
+        }
+        nl = (int)parts->size() + 1;
+        Q.init(sx->N, sx->p + 1);
+    } else {
+        if (parts) throw Error(-1, "partitions apply to simplex problems only");
+        const Quadtree& qt = *p->qt;
+        if (qt.num_levels < 2) throw Error(-1, "the quadtree needs at least two levels");  // interpolationhierarchy.h:33
+        nl = qt.num_levels;
+    }
     hipStream_t s = ctx->stream;
-    const int nl = qt.num_levels, b = p->b, N = p->N;
+    const int N = p->N;
     L.clear(); L.resize(nl);
     double th = 0;
     auto t0 = Clock::now();
     // fine level: the problem's operators (device copies share nothing with the problem)
     Level& f = L[0];
-    f.nb = p->ne; f.B = b;
+    f.nb = p->ne; f.B = p->b;
     auto clone = [&](DevOp& dst, const DevOp& src) {
         dst.upload_pattern(src.pat, s);
         dst.val.alloc(src.val.n);
@@ -452,17 +492,18 @@ void MGH::build(ProblemH* p, const MGParams& pm)
     }
     for (int l = 0; l + 1 < nl; ++l) {
         auto th0 = Clock::now();
-        build_interp(qt, p->host.basis, l, L[l].I);
+        if (sx) build_interp_agglo(sx->mesh, sx->E, Q, *parts, l, L[l].I);
+        else build_interp(*p->qt, p->cart()->basis, l, L[l].I);
         th += secs(th0);
         upload_interp(s, L[l], prm.precision);
-        L[l + 1].nb = L[l].I.nc; L[l + 1].B = b;
+        L[l + 1].nb = L[l].I.nc; L[l + 1].B = L[l].I.bc;
         coarsen(s, N, L[l], L[l + 1], prm.restriction);
     }
     for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision, prm.relaxation == 2);
     // coarsest level: dense inverse of the assembled operator (FP64)
     Level& c = L[nl - 1];
     if (prm.coarse == 0) {
-        const int n = c.nb * b;
+        const int b = c.B, n = c.nb * b;
         if (n > 6144) throw Error(-5, "coarsest level too large for the dense solve");
         std::vector<double> blk(c.A.val.n), D((size_t)n * n, 0.0);
         c.A.val.download(blk.data(), blk.size(), s);
@@ -572,7 +613,7 @@ void MGH::vcycle_from(int l0)
         relax<T>(l, prm.npre, false, zero);
         dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr);
         LevelVecs<T>& vc = vecs<T>(l + 1);
-        dev::Ops<T>::restrict_(s, lv.B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
+        dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): implicit in the first sweep of a
         // Gauss-Seidel pre-smoother, explicit otherwise
         if (l + 1 < bottom && !(prm.relaxation >= 1 && prm.npre > 0))
@@ -595,7 +636,7 @@ void MGH::vcycle_from(int l0)
     for (int l = bottom - 1; l >= l0; --l) {
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
-        dev::Ops<T>::prolong(s, lv.B, lv.nb, lv.parent, lv.type, Pool<T>(lv), vecs<T>(l + 1).x, v.x);
+        dev::Ops<T>::prolong(s, lv.B, L[l + 1].B, lv.nb, lv.parent, lv.type, Pool<T>(lv), vecs<T>(l + 1).x, v.x);
         relax<T>(l, prm.npost, true, false);
     }
 }
@@ -650,7 +691,7 @@ void MGH::build_collapse()
         mrelax(l, k, prm.npre, false, true, X[l], Bv[l], R[l]);
         if (prm.npre == 0) dev::Ops<double>::fill(s, (long)lv.nb * lv.B * k, 0.0, X[l]);
         dev::MOps<double>::apply(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, X[l], Bv[l], R[l]);
-        dev::MOps<double>::restrict_(s, lv.B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, lv.pool64, R[l], Bv[l + 1]);
+        dev::MOps<double>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, lv.pool64, R[l], Bv[l + 1]);
     }
     Level& c = L[nl - 1];
     const int nc = c.nb * c.B;
@@ -664,7 +705,7 @@ void MGH::build_collapse()
     }
     for (int l = nl - 2; l >= cut; --l) {
         Level& lv = L[l];
-        dev::MOps<double>::prolong(s, lv.B, k, lv.nb, lv.parent, lv.type, lv.pool64, X[l + 1], X[l]);
+        dev::MOps<double>::prolong(s, lv.B, L[l + 1].B, k, lv.nb, lv.parent, lv.type, lv.pool64, X[l + 1], X[l]);
         mrelax(l, k, prm.npost, true, false, X[l], Bv[l], R[l]);
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -861,7 +902,7 @@ void MGH::bvcycle(int k)
         brelax<T>(l, k, prm.npre, false, true);
         if (prm.npre == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B * k, (T)0, bx<T>(BL[l]));
         dev::MOps<T>::apply(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), bx<T>(BL[l]), bb<T>(BL[l]), br<T>(BL[l]));
-        dev::MOps<T>::restrict_(s, lv.B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
+        dev::MOps<T>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
     }
     Level& c = L[bottom];
     const int n = c.nb * c.B;
@@ -877,7 +918,7 @@ void MGH::bvcycle(int k)
     }
     for (int l = bottom - 1; l >= 0; --l) {
         Level& lv = L[l];
-        dev::MOps<T>::prolong(s, lv.B, k, lv.nb, lv.parent, lv.type, Pool<T>(lv), bx<T>(BL[l + 1]), bx<T>(BL[l]));
+        dev::MOps<T>::prolong(s, lv.B, L[l + 1].B, k, lv.nb, lv.parent, lv.type, Pool<T>(lv), bx<T>(BL[l + 1]), bx<T>(BL[l]));
         brelax<T>(l, k, prm.npost, true, false);
     }
 }

@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "host.hpp"
+#include "simplex.hpp"
 
 namespace mgldg {
 
@@ -59,19 +59,24 @@ struct QuadtreeH {
 
 struct ProblemH {
     Ctx* ctx;
-    const Quadtree* qt;                // kept alive by the caller (as the reference's raw pointers)
-    LdgHost host;
+    const Quadtree* qt = nullptr;      // quadtree problems: kept alive by the caller (as the reference's raw pointers)
+    std::unique_ptr<LdgOps> host;      // LdgHost (quadtree) or SimplexLdg
     DevOp M, T, G[3], A;
     DBuf<double> Lb;                   // boundary lifting blocks (npl x nq per boundary face)
-    DBuf<int> bleft, bdim, btype;      // per boundary face: left element, dim, bc type
-    DBuf<double> bsign;
+    DBuf<int> bleft, btype;            // per boundary face: left element, bc type
+    DBuf<double> bnormal;              // per boundary face: outward unit normal (N)
     DBuf<int> bf_off, bf_list;         // boundary faces of each element (CSR)
     DBuf<double> Gpan[3];              // G_d in the solve (panel) layout
     DevOp GT[3];                       // G_d^T, values in the panel layout
     int N = 2, b = 0, ne = 0, nqb = 0;
     long ndof = 0;
     double setup_host_s = 0, setup_dev_s = 0;
+    LdgHost* cart() const { return dynamic_cast<LdgHost*>(host.get()); }
+    SimplexLdg* simplex() const { return dynamic_cast<SimplexLdg*>(host.get()); }
     void build(Ctx* c, const Quadtree* q, int p, const int* bct, double tau0, double tauD);
+    void build_simplex(Ctx* c, int N, int np, const double* pts, int nt, const int* tris, int nbnd, const int* vmask,
+                       int p, const int* bct, double tau0, double tauD);
+    void upload();                     // device copies of the host operators, A = T + sum G^T M G
     // b = M f + B_g g on the device, k columns (f or g may be null)
     void rhs_device(const double* f, const double* g, int k, double* bout);
     // dL/df = M lam, dL/dg = B_g^T lam (either output may be null)
@@ -156,7 +161,9 @@ struct MGH {
     DBuf<int> Bactive;
     hipGraphExec_t bgraph[2] = {nullptr, nullptr};
 
-    void build(ProblemH* p, const MGParams& prm);
+    // parts: agglomeration partitions (level l -> l + 1) of a simplex problem;
+    // null: the quadtree hierarchy of a quadtree problem
+    void build(ProblemH* p, const MGParams& prm, const std::vector<std::vector<int>>* parts = nullptr);
     void build_collapse();
     void ensure_batch(int k, int precision);
     template <typename T> void brelax(int l, int k, int n, bool reverse, bool zero_guess);

@@ -1,0 +1,704 @@
+// simplex.cpp — unstructured (simplex) host path.  See simplex.hpp.
+#include "simplex.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace mgldg {
+
+static long factorial(int n) { long r = 1; for (int i = 2; i <= n; ++i) r *= i; return r; }
+
+// multi-indices with entries >= 0 and sum <= P-1, last coordinate fastest
+// (SimplexRangeIterator, src/unstructured/util/range.h:291-309)
+static std::vector<int> simplex_multi_indices(int N, int P)
+{
+    std::vector<int> out;
+    int cur[3] = {0, 0, 0};
+    for (;;) {
+        out.insert(out.end(), cur, cur + N);
+        int s = N - 1;
+        for (; s >= 0; --s) {
+            int other = 0;
+            for (int k = 0; k < N; ++k) if (k != s) other += cur[k];
+            if (++cur[s] < P - other) break;
+            cur[s] = 0;
+        }
+        if (s < 0) break;
+    }
+    return out;
+}
+
+// Gauss-Jordan inverse with partial pivoting in long double
+static std::vector<long double> invert_ld(std::vector<long double> A, int n)
+{
+    std::vector<long double> X((size_t)n * n, 0.0L);
+    for (int i = 0; i < n; ++i) X[(size_t)i * n + i] = 1.0L;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i) if (fabsl(A[(size_t)i * n + k]) > fabsl(A[(size_t)p * n + k])) p = i;
+        if (A[(size_t)p * n + k] == 0.0L) throw Error(-1, "singular matrix in the simplex basis");
+        if (p != k) for (int j = 0; j < n; ++j) { std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]); std::swap(X[(size_t)k * n + j], X[(size_t)p * n + j]); }
+        const long double inv = 1.0L / A[(size_t)k * n + k];
+        for (int j = 0; j < n; ++j) { A[(size_t)k * n + j] *= inv; X[(size_t)k * n + j] *= inv; }
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const long double f = A[(size_t)i * n + k];
+            if (f == 0.0L) continue;
+            for (int j = 0; j < n; ++j) { A[(size_t)i * n + j] -= f * A[(size_t)k * n + j]; X[(size_t)i * n + j] -= f * X[(size_t)k * n + j]; }
+        }
+    }
+    return X;
+}
+
+static long double mono_ld(const long double* x, const int* e, int N)
+{
+    long double v = 1.0L;
+    for (int d = 0; d < N; ++d) for (int k = 0; k < e[d]; ++k) v *= x[d];
+    return v;
+}
+
+void SimplexBasis::init(int N_, int P_)
+{
+    N = N_; P = P_;
+    if (N < 1 || N > 3 || P < 1 || P > 6) throw Error(-1, "simplex basis: unsupported dimension or degree");
+    expo = simplex_multi_indices(N, P);
+    npl = (int)expo.size() / N;
+    volume = 1.0 / (double)factorial(N);
+    nodes.assign((size_t)npl * N, 0.0);
+    for (int i = 0; i < npl; ++i)
+        for (int d = 0; d < N; ++d)
+            nodes[(size_t)i * N + d] = (P == 1) ? 1.0 / 3.0 : (double)expo[(size_t)i * N + d] / (P - 1);
+    const int n = npl;
+    // Vandermonde of the monomials at the nodes, long double
+    std::vector<long double> V((size_t)n * n), xl((size_t)n * N);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < N; ++d)
+            xl[(size_t)i * N + d] = (P == 1) ? 1.0L / 3.0L : (long double)expo[(size_t)i * N + d] / (P - 1);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) V[(size_t)i * n + k] = mono_ld(&xl[(size_t)i * N], &expo[(size_t)k * N], N);
+    std::vector<long double> C = invert_ld(V, n);   // phi_i = sum_k C[k][i] m_k
+    coef.resize((size_t)n * n);
+    for (size_t t = 0; t < coef.size(); ++t) coef[t] = (double)C[t];
+    // exact moments int x^a over the unit simplex = prod a_d! / (|a| + N)!
+    auto moment = [&](const int* a, const int* b) {
+        long double num = 1.0L; int tot = 0;
+        for (int d = 0; d < N; ++d) { int s = a[d] + b[d]; tot += s; for (int k = 2; k <= s; ++k) num *= k; }
+        long double den = 1.0L;
+        for (int k = 2; k <= tot + N; ++k) den *= k;
+        return num / den;
+    };
+    std::vector<long double> Mm((size_t)n * n), mass_ld((size_t)n * n, 0.0L);
+    for (int k = 0; k < n; ++k) for (int l = 0; l < n; ++l) Mm[(size_t)k * n + l] = moment(&expo[(size_t)k * N], &expo[(size_t)l * N]);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            long double s = 0.0L;
+            for (int k = 0; k < n; ++k) {
+                const long double cki = C[(size_t)k * n + i];
+                if (cki == 0.0L) continue;
+                for (int l = 0; l < n; ++l) s += cki * Mm[(size_t)k * n + l] * C[(size_t)l * n + j];
+            }
+            mass_ld[(size_t)i * n + j] = s;
+        }
+    mass.resize((size_t)n * n);
+    for (size_t t = 0; t < mass.size(); ++t) mass[t] = (double)mass_ld[t];
+    std::vector<long double> Mi = invert_ld(mass_ld, n);
+    invmass.resize((size_t)n * n);
+    for (size_t t = 0; t < invmass.size(); ++t) invmass[t] = (double)Mi[t];
+    for (int d = 0; d < N; ++d) {
+        diff[d].assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                long double s = 0.0L;
+                for (int k = 0; k < n; ++k) {
+                    const int* e = &expo[(size_t)k * N];
+                    if (e[d] == 0) continue;
+                    int e2[3] = {e[0], N > 1 ? e[1] : 0, N > 2 ? e[2] : 0};
+                    e2[d] -= 1;
+                    s += C[(size_t)k * n + j] * (long double)e[d] * mono_ld(&xl[(size_t)i * N], e2, N);
+                }
+                diff[d][(size_t)i * n + j] = (double)s;
+            }
+    }
+}
+
+void SimplexBasis::eval(const double* x, double* phi) const
+{
+    long double xl[3] = {x[0], N > 1 ? x[1] : 0.0, N > 2 ? x[2] : 0.0};
+    std::vector<long double> m(npl);
+    for (int k = 0; k < npl; ++k) m[k] = mono_ld(xl, &expo[(size_t)k * N], N);
+    for (int i = 0; i < npl; ++i) {
+        long double s = 0.0L;
+        for (int k = 0; k < npl; ++k) s += (long double)coef[(size_t)k * npl + i] * m[k];
+        phi[i] = (double)s;
+    }
+}
+
+// ======================================================================= mesh
+static double det_n(const double* J, int N)
+{
+    if (N == 1) return J[0];
+    if (N == 2) return J[0] * J[3] - J[1] * J[2];
+    return J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) + J[2] * (J[3] * J[7] - J[4] * J[6]);
+}
+
+static void inv_n(const double* J, int N, double det, double* I)
+{
+    if (N == 1) { I[0] = 1.0 / det; return; }
+    if (N == 2) { I[0] = J[3] / det; I[1] = -J[1] / det; I[2] = -J[2] / det; I[3] = J[0] / det; return; }
+    I[0] = (J[4] * J[8] - J[5] * J[7]) / det; I[1] = (J[2] * J[7] - J[1] * J[8]) / det; I[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+    I[3] = (J[5] * J[6] - J[3] * J[8]) / det; I[4] = (J[0] * J[8] - J[2] * J[6]) / det; I[5] = (J[2] * J[3] - J[0] * J[5]) / det;
+    I[6] = (J[3] * J[7] - J[4] * J[6]) / det; I[7] = (J[1] * J[6] - J[0] * J[7]) / det; I[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+}
+
+void SimplexMesh::build(int N_, int np_, const double* p, int nt, const int* t, int nbnd, const int* vmask)
+{
+    N = N_; np = np_; ne = nt;
+    if (N < 2 || N > 3) throw Error(-1, "simplex meshes are 2-D or 3-D");
+    if (np < N + 1 || nt < 1 || !p || !t) throw Error(-1, "empty simplex mesh");
+    if (nbnd < 0 || nbnd > 30 || (nbnd > 0 && !vmask)) throw Error(-1, "invalid boundary predicates");
+    pts.assign(p, p + (size_t)np * N);
+    tris.assign(t, t + (size_t)nt * (N + 1));
+    for (int v : tris) if (v < 0 || v >= np) throw Error(-1, "simplex vertex index out of range");
+    vol.resize(ne); J.resize((size_t)ne * N * N); Jinv.resize((size_t)ne * N * N);
+    for (int e = 0; e < ne; ++e) {
+        double* Je = &J[(size_t)e * N * N];
+        const int* te = &tris[(size_t)e * (N + 1)];
+        for (int r = 0; r < N; ++r)
+            for (int i = 0; i < N; ++i) Je[r * N + i] = pts[(size_t)te[i + 1] * N + r] - pts[(size_t)te[0] * N + r];
+        const double d = det_n(Je, N);
+        if (!(std::fabs(d) > 0.0)) throw Error(-1, "degenerate simplex");
+        vol[e] = std::fabs(d) / (double)factorial(N);   // sqrt(det(J^T J)) / N! (geometry.h:65-74)
+        inv_n(Je, N, d, &Jinv[(size_t)e * N * N]);
+    }
+    // faces: sorted vertex N-tuples (mesh.h:210-226), numbered by first appearance
+    struct Ent { int v[3]; long pos; };
+    const long nent = (long)ne * (N + 1);
+    std::vector<Ent> ent(nent);
+    for (int e = 0; e < ne; ++e)
+        for (int j = 0; j < N + 1; ++j) {
+            Ent& x = ent[(long)e * (N + 1) + j];
+            x.v[2] = -1;
+            for (int k = 0; k < N; ++k) x.v[k] = tris[(size_t)e * (N + 1) + (k + j) % (N + 1)];
+            std::sort(x.v, x.v + N);
+            x.pos = (long)e * (N + 1) + j;
+        }
+    std::sort(ent.begin(), ent.end(), [](const Ent& a, const Ent& b) {
+        for (int k = 0; k < 3; ++k) if (a.v[k] != b.v[k]) return a.v[k] < b.v[k];
+        return a.pos < b.pos;
+    });
+    struct Grp { long first; int e1, e2; int v[3]; };
+    std::vector<Grp> grp;
+    grp.reserve(nent / 2 + 16);
+    for (long i = 0; i < nent;) {
+        long j = i + 1;
+        while (j < nent && ent[j].v[0] == ent[i].v[0] && ent[j].v[1] == ent[i].v[1] && ent[j].v[2] == ent[i].v[2]) ++j;
+        if (j - i > 2) throw Error(-1, "non-manifold simplex mesh (a face shared by more than two simplices)");
+        Grp g;
+        g.first = ent[i].pos;
+        g.e1 = (int)(ent[i].pos / (N + 1));
+        g.e2 = (j - i == 2) ? (int)(ent[i + 1].pos / (N + 1)) : -1;
+        std::memcpy(g.v, ent[i].v, sizeof g.v);
+        grp.push_back(g);
+        i = j;
+    }
+    std::sort(grp.begin(), grp.end(), [](const Grp& a, const Grp& b) { return a.first < b.first; });
+    nf = (int)grp.size();
+    faces.assign(nf, SimplexFace());
+    bfaces.clear();
+    for (int fi = 0; fi < nf; ++fi) {
+        const Grp& g = grp[fi];
+        SimplexFace& f = faces[fi];
+        const int mn = std::min(g.e1, g.e2), mx = std::max(g.e1, g.e2);
+        f.left = (mn >= 0) ? mn : mx;          // mesh.h:234-237
+        f.right = (mn >= 0) ? mx : mn;
+        for (int k = 0; k < N; ++k) f.v[k] = g.v[k];
+        // normal from the cofactors of the face Jacobian (mesh.h:249-263)
+        const double* x0 = &pts[(size_t)f.v[0] * N];
+        const double* x1 = &pts[(size_t)f.v[1] * N];
+        double n[3] = {0, 0, 0};
+        if (N == 2) { n[0] = x1[1] - x0[1]; n[1] = -(x1[0] - x0[0]); }
+        else {
+            const double* x2 = &pts[(size_t)f.v[2] * N];
+            const double a[3] = {x1[0] - x0[0], x1[1] - x0[1], x1[2] - x0[2]};
+            const double c[3] = {x2[0] - x0[0], x2[1] - x0[1], x2[2] - x0[2]};
+            n[0] = a[1] * c[2] - a[2] * c[1]; n[1] = a[2] * c[0] - a[0] * c[2]; n[2] = a[0] * c[1] - a[1] * c[0];
+        }
+        double nn = 0;
+        for (int d = 0; d < N; ++d) nn += n[d] * n[d];
+        nn = std::sqrt(nn);
+        f.area = nn / (double)factorial(N - 1);
+        // outward from the left element (mesh.h:266-273)
+        const int* tl = &tris[(size_t)f.left * (N + 1)];
+        int opp = tl[0];
+        for (int i = 0; i < N + 1; ++i) {
+            bool in = false;
+            for (int k = 0; k < N; ++k) in |= (tl[i] == f.v[k]);
+            if (!in) opp = tl[i];
+        }
+        double dot = 0;
+        for (int d = 0; d < N; ++d) dot += n[d] * (pts[(size_t)opp * N + d] - x0[d]);
+        const double sgn = dot > 0 ? -1.0 : 1.0;
+        for (int d = 0; d < N; ++d) f.normal[d] = sgn * n[d] / nn;
+        if (f.right < 0) {
+            int key = 1;
+            if (nbnd > 0) {   // first predicate holding at all face vertices (mesh.h:276-288)
+                key = 0;
+                for (int k = 0; k < nbnd && !key; ++k) {
+                    bool all = true;
+                    for (int q = 0; q < N; ++q) all &= ((vmask[f.v[q]] >> k) & 1) != 0;
+                    if (all) key = k + 1;
+                }
+                if (!key) throw Error(-1, "Boundary face does not satisfy any boundary function.");
+            }
+            f.right = -key;
+            bfaces.push_back(fi);
+        }
+    }
+    nbf = (int)bfaces.size();
+}
+
+void SimplexMesh::face_point(const SimplexFace& f, const double* ref, double* x) const
+{
+    const double* x0 = &pts[(size_t)f.v[0] * N];
+    for (int d = 0; d < N; ++d) {
+        double s = x0[d];
+        for (int k = 0; k < N - 1; ++k) s += (pts[(size_t)f.v[k + 1] * N + d] - x0[d]) * ref[k];
+        x[d] = s;
+    }
+}
+
+void SimplexMesh::to_local(int e, const double* x, double* loc) const
+{
+    const double* I = &Jinv[(size_t)e * N * N];
+    const double* x0 = &pts[(size_t)tris[(size_t)e * (N + 1)] * N];
+    for (int r = 0; r < N; ++r) {
+        double s = 0;
+        for (int c = 0; c < N; ++c) s += I[r * N + c] * (x[c] - x0[c]);
+        loc[r] = s;
+    }
+}
+
+// ======================================================================= assembly
+// Face rule on the reference (N-1)-simplex, weights summing to one
+// (Quadrature<N-1,Q>, src/unstructured/dg/master.h:359-401): the Q = 10 point
+// Gauss rule on [0,1] for triangle meshes.
+static void face_rule(int N, std::vector<double>& x, std::vector<double>& w)
+{
+    if (N == 2) {
+        x.resize(kQuad); w.resize(kQuad);
+        gauss_rule(kQuad, x.data(), w.data());
+        return;
+    }
+    throw Error(-1, "3-D simplex meshes are not supported yet (face quadrature)");
+}
+
+static void gemm_sq(int n, double alpha, const double* A, const double* B, double* C)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0;
+            for (int k = 0; k < n; ++k) s += A[i * n + k] * B[k * n + j];
+            C[i * n + j] = alpha * s;
+        }
+}
+
+static void add_blk(HostBSR& A, int i, int j, double alpha, const double* X)
+{
+    int k = A.find(i, j);
+    double* p = A.block(k);
+    const size_t n = (size_t)A.br * A.bc;
+    for (size_t t = 0; t < n; ++t) p[t] += alpha * X[t];
+}
+
+void SimplexLdg::assemble(int N_, int np, const double* pts, int nt, const int* tris, int nbnd, const int* vmask, int p_,
+                          const int* bct, double t0, double tD)
+{
+    N = N_; p = p_; tau0 = t0; tauD = tD;
+    if (p < 0 || p > 5) throw Error(-1, "polynomial degree out of range");
+    mesh.build(N, np, pts, nt, tris, nbnd, vmask);
+    const int nkeys = std::max(nbnd, 1);
+    bctype.assign(nkeys, 0);
+    for (int k = 0; k < nkeys; ++k) {
+        bctype[k] = bct ? bct[k] : 0;
+        if (bctype[k] != 0 && bctype[k] != 1) throw Error(-1, "unknown boundary condition");   // ldgpoisson.tpp:145
+    }
+    E.init(N, p + 1);
+    F.init(N - 1, p + 1);
+    face_rule(N, qx, qw);
+    npl = E.npl; nq = (int)qw.size();
+    ne = mesh.ne; nf = mesh.nf; nbf = mesh.nbf;
+    const int b = npl, nfl = F.npl;
+    auto dirichlet = [&](const SimplexFace& f) { return f.right < 0 && bctype[-f.right - 1] == 0; };
+    // slices of interior faces: face node -> element node, by position (mesh.h:32-66)
+    std::vector<int> sl((size_t)nf * nfl, 0), sr((size_t)nf * nfl, 0);
+    for (int fi = 0; fi < nf; ++fi) {
+        const SimplexFace& f = mesh.faces[fi];
+        if (f.right < 0) continue;
+        for (int i = 0; i < nfl; ++i) {
+            if (E.P == 1) { sl[(size_t)fi * nfl] = 0; sr[(size_t)fi * nfl] = 0; break; }
+            double x[3], loc[3];
+            mesh.face_point(f, &F.nodes[(size_t)i * (N - 1)], x);
+            for (int side = 0; side < 2; ++side) {
+                mesh.to_local(side ? f.right : f.left, x, loc);
+                int m = -1;
+                for (int j = 0; j < b && m < 0; ++j) {
+                    bool ok = true;
+                    for (int d = 0; d < N; ++d) ok &= std::fabs(loc[d] - E.nodes[(size_t)j * N + d]) < 1e-6;
+                    if (ok) m = j;
+                }
+                if (m < 0) throw Error(-1, "face nodes do not match element nodes (non-conforming mesh)");
+                (side ? sr : sl)[(size_t)fi * nfl + i] = m;
+            }
+        }
+    }
+    // patterns (addToBlock creates structural blocks for every normal component)
+    {
+        PatternBuilder pm(ne, ne), pt(ne, ne);
+        std::vector<PatternBuilder> pg(N, PatternBuilder(ne, ne));
+        for (int e = 0; e < ne; ++e) { pm.add(e, e); for (int d = 0; d < N; ++d) pg[d].add(e, e); }
+        for (const SimplexFace& f : mesh.faces) {
+            if (f.right >= 0) {
+                for (int d = 0; d < N; ++d) { pg[d].add(f.right, f.right); pg[d].add(f.right, f.left); }
+                pt.add(f.right, f.right); pt.add(f.left, f.left); pt.add(f.right, f.left); pt.add(f.left, f.right);
+            } else if (dirichlet(f)) {
+                pt.add(f.left, f.left);
+            }
+        }
+        pm.finish(M, b, b); pt.finish(T, b, b);
+        for (int d = 0; d < N; ++d) pg[d].finish(G[d], b, b);
+    }
+    // mass and broken gradient (ldgpoisson.tpp:26-58)
+    std::vector<double> W((size_t)b * b), MI((size_t)b * b), LL((size_t)b * b), RR((size_t)b * b), RL((size_t)b * b),
+        RLt((size_t)b * b);
+    for (int e = 0; e < ne; ++e) add_blk(M, e, e, mesh.vol[e] / E.volume, E.mass.data());
+    for (int e = 0; e < ne; ++e) {
+        const double* I = &mesh.Jinv[(size_t)e * N * N];
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) add_blk(G[i], e, e, I[j * N + i], E.diff[j].data());
+    }
+    // boundary evaluation of the left element's basis at the face rule (mesh.h:67-76, 336-347)
+    auto boundary_phi = [&](const SimplexFace& f, std::vector<double>& phi, std::vector<double>& w) {
+        phi.resize((size_t)nq * b); w.resize(nq);
+        for (int q = 0; q < nq; ++q) {
+            double x[3], loc[3];
+            mesh.face_point(f, &qx[(size_t)q * (N - 1)], x);
+            mesh.to_local(f.left, x, loc);
+            E.eval(loc, &phi[(size_t)q * b]);
+            w[q] = qw[q] * f.area * F.volume;   // as written in mesh.h:343-344
+        }
+    };
+    std::vector<double> phi, w;
+    // lifting and penalty terms (ldgpoisson.tpp:60-99)
+    for (int fi = 0; fi < nf; ++fi) {
+        const SimplexFace& f = mesh.faces[fi];
+        if (f.right >= 0) {
+            std::fill(LL.begin(), LL.end(), 0.0); std::fill(RR.begin(), RR.end(), 0.0); std::fill(RL.begin(), RL.end(), 0.0);
+            const double jac = f.area / F.volume;
+            const int* a = &sl[(size_t)fi * nfl]; const int* c = &sr[(size_t)fi * nfl];
+            for (int i = 0; i < nfl; ++i)
+                for (int j = 0; j < nfl; ++j) {
+                    const double v = F.mass[(size_t)i * nfl + j] * jac;
+                    LL[(size_t)a[i] * b + a[j]] += v;
+                    RR[(size_t)c[i] * b + c[j]] += v;
+                    RL[(size_t)c[i] * b + a[j]] += v;
+                }
+            const double iv = E.volume / mesh.vol[f.right];
+            for (size_t k = 0; k < MI.size(); ++k) MI[k] = E.invmass[k] * iv;
+            for (int d = 0; d < N; ++d) {
+                const double n = f.normal[d];
+                gemm_sq(b, n, MI.data(), RR.data(), W.data());  add_blk(G[d], f.right, f.right, 1.0, W.data());
+                gemm_sq(b, -n, MI.data(), RL.data(), W.data()); add_blk(G[d], f.right, f.left, 1.0, W.data());
+            }
+            add_blk(T, f.right, f.right, tau0, RR.data());
+            add_blk(T, f.left, f.left, tau0, LL.data());
+            add_blk(T, f.right, f.left, -tau0, RL.data());
+            for (int i = 0; i < b; ++i) for (int j = 0; j < b; ++j) RLt[(size_t)i * b + j] = RL[(size_t)j * b + i];
+            add_blk(T, f.left, f.right, -tau0, RLt.data());
+        } else if (dirichlet(f)) {
+            boundary_phi(f, phi, w);
+            for (int i = 0; i < b; ++i)
+                for (int j = 0; j < b; ++j) {
+                    double s = 0;
+                    for (int q = 0; q < nq; ++q) s += phi[(size_t)q * b + i] * w[q] * phi[(size_t)q * b + j];
+                    LL[(size_t)i * b + j] = s;
+                }
+            const double iv = E.volume / mesh.vol[f.left];
+            for (size_t k = 0; k < MI.size(); ++k) MI[k] = E.invmass[k] * iv;
+            for (int d = 0; d < N; ++d) {
+                gemm_sq(b, -f.normal[d], MI.data(), LL.data(), W.data());
+                add_blk(G[d], f.left, f.left, 1.0, W.data());
+            }
+            add_blk(T, f.left, f.left, tauD, LL.data());
+        }
+    }
+    // boundary lifting blocks and face data for the RHS and its adjoint
+    Lb.assign((size_t)nbf * b * nq, 0.0);
+    bleft.resize(nbf); btype.resize(nbf); btag.resize(nbf); bnormal.assign((size_t)nbf * N, 0.0);
+    for (int k = 0; k < nbf; ++k) {
+        const SimplexFace& f = mesh.faces[mesh.bfaces[k]];
+        boundary_phi(f, phi, w);
+        double* L = &Lb[(size_t)k * b * nq];
+        for (int i = 0; i < b; ++i)
+            for (int q = 0; q < nq; ++q) L[(size_t)i * nq + q] = phi[(size_t)q * b + i] * w[q];
+        bleft[k] = f.left; btag[k] = -f.right; btype[k] = bctype[-f.right - 1];
+        for (int d = 0; d < N; ++d) bnormal[(size_t)k * N + d] = f.normal[d];
+    }
+}
+
+void SimplexLdg::bc_qpoints(double* xyz, int* tag) const
+{
+    for (int k = 0; k < nbf; ++k) {
+        const SimplexFace& f = mesh.faces[mesh.bfaces[k]];
+        for (int q = 0; q < nq; ++q) {
+            mesh.face_point(f, &qx[(size_t)q * (N - 1)], xyz + ((size_t)k * nq + q) * N);
+            if (tag) tag[(size_t)k * nq + q] = -f.right;
+        }
+    }
+}
+
+void SimplexLdg::nodes_xyz(double* xyz) const
+{
+    for (int e = 0; e < ne; ++e) {
+        const double* Je = &mesh.J[(size_t)e * N * N];
+        const double* x0 = &mesh.pts[(size_t)mesh.tris[(size_t)e * (N + 1)] * N];
+        for (int i = 0; i < npl; ++i)
+            for (int r = 0; r < N; ++r) {
+                double s = x0[r];
+                for (int c = 0; c < N; ++c) s += Je[r * N + c] * E.nodes[(size_t)i * N + c];
+                xyz[((size_t)e * npl + i) * N + r] = s;
+            }
+    }
+}
+
+// ======================================================================= agglomeration
+void agglomeration_graph(const SimplexMesh& m, const int* orig, int nparts, AggGraph& g)
+{
+    const int nv = orig ? nparts : m.ne;
+    auto P = [&](int e) { return orig ? orig[e] : e; };
+    g.vvol.assign(nv, 0.0); g.vsurf.assign(nv, 0.0);
+    for (int e = 0; e < m.ne; ++e) g.vvol[P(e)] += m.vol[e];
+    // faces of each vertex in face order, +-(f+1) for the left / right side
+    std::vector<int> cnt(nv + 1, 0);
+    for (const SimplexFace& f : m.faces) {
+        const int l = P(f.left);
+        if (f.right >= 0) { const int r = P(f.right); if (l != r) { cnt[l + 1]++; cnt[r + 1]++; } }
+        else cnt[l + 1]++;
+    }
+    for (int i = 0; i < nv; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> lst(std::max(cnt[nv], 1)), pos(cnt.begin(), cnt.end() - 1);
+    for (int fi = 0; fi < m.nf; ++fi) {
+        const SimplexFace& f = m.faces[fi];
+        const int l = P(f.left);
+        if (f.right >= 0) { const int r = P(f.right); if (l != r) { lst[pos[l]++] = fi + 1; lst[pos[r]++] = -(fi + 1); } }
+        else lst[pos[l]++] = fi + 1;
+    }
+    g.xadj.assign(nv + 1, 0); g.adj.clear(); g.w.clear();
+    std::vector<int> seen(nv, -1);
+    for (int i = 0; i < nv; ++i) {
+        for (int t = cnt[i]; t < cnt[i + 1]; ++t) {
+            const int jj = lst[t], fi = std::abs(jj) - 1;
+            const SimplexFace& f = m.faces[fi];
+            const int on = jj > 0 ? f.right : f.left;
+            if (on >= 0) {
+                const int an = P(on);
+                if (seen[an] >= 0) { g.w[seen[an]] += f.area; continue; }
+                seen[an] = (int)g.adj.size();
+                g.adj.push_back(an); g.w.push_back(f.area);
+            } else g.vsurf[i] += f.area;
+        }
+        g.xadj[i + 1] = (int)g.adj.size();
+        for (int t = g.xadj[i]; t < g.xadj[i + 1]; ++t) seen[g.adj[t]] = -1;
+    }
+}
+
+int agglomerate_graph(const AggGraph& g, int N, int maxsize, std::vector<int>& part)
+{
+    const int nv = (int)g.vvol.size();
+    // cluster state: members counted in graph vertices
+    std::vector<int> cl(nv);
+    std::iota(cl.begin(), cl.end(), 0);
+    std::vector<int> size(nv, 1);
+    std::vector<double> vol(g.vvol), per(nv);
+    for (int i = 0; i < nv; ++i) {
+        double s = g.vsurf[i];
+        for (int t = g.xadj[i]; t < g.xadj[i + 1]; ++t) s += g.w[t];
+        per[i] = s;
+    }
+    auto ratio = [&](double P, double V) { return std::pow(P, N) / std::pow(V, N - 1); };
+    // current cluster graph (CSR), starts as g
+    std::vector<int> xadj(g.xadj), adj(g.adj);
+    std::vector<double> w(g.w);
+    int nc = nv;
+    for (int round = 0; round < 64; ++round) {
+        // visit the worst-shaped clusters first
+        std::vector<int> order(nc);
+        std::iota(order.begin(), order.end(), 0);
+        std::vector<double> ar(nc);
+        for (int c = 0; c < nc; ++c) ar[c] = ratio(per[c], vol[c]);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ar[a] > ar[b]; });
+        std::vector<int> mate(nc, -1);
+        int merges = 0;
+        for (int u : order) {
+            if (mate[u] >= 0) continue;
+            int best = -1; double bestv = 0;
+            for (int t = xadj[u]; t < xadj[u + 1]; ++t) {
+                const int v = adj[t];
+                if (v == u || mate[v] >= 0 || size[u] + size[v] > maxsize) continue;
+                const double val = ratio(per[u] + per[v] - 2.0 * w[t], vol[u] + vol[v]);
+                if (best < 0 || val < bestv || (val == bestv && v < best)) { best = v; bestv = val; }
+            }
+            if (best >= 0) { mate[u] = best; mate[best] = u; ++merges; }
+        }
+        if (!merges) break;
+        // contract: new ids in order of the smaller member
+        std::vector<int> nid(nc, -1);
+        int k = 0;
+        for (int c = 0; c < nc; ++c) {
+            if (nid[c] >= 0) continue;
+            nid[c] = k;
+            if (mate[c] >= 0) nid[mate[c]] = k;
+            ++k;
+        }
+        std::vector<int> nsize(k, 0);
+        std::vector<double> nvol(k, 0.0), nper(k, 0.0);
+        for (int c = 0; c < nc; ++c) { nsize[nid[c]] += size[c]; nvol[nid[c]] += vol[c]; nper[nid[c]] += per[c]; }
+        for (int c = 0; c < nc; ++c)
+            if (mate[c] > c)
+                for (int t = xadj[c]; t < xadj[c + 1]; ++t) if (adj[t] == mate[c]) nper[nid[c]] -= 2.0 * w[t];
+        // contracted adjacency with summed weights
+        std::vector<std::vector<int>> mem(k);
+        for (int c = 0; c < nc; ++c) mem[nid[c]].push_back(c);
+        std::vector<int> nx(k + 1, 0), na;
+        std::vector<double> nw;
+        std::vector<int> slot(k, -1);
+        for (int a = 0; a < k; ++a) {
+            for (int c : mem[a])
+                for (int t = xadj[c]; t < xadj[c + 1]; ++t) {
+                    const int b = nid[adj[t]];
+                    if (b == a) continue;
+                    if (slot[b] >= 0) { nw[slot[b]] += w[t]; continue; }
+                    slot[b] = (int)na.size();
+                    na.push_back(b); nw.push_back(w[t]);
+                }
+            nx[a + 1] = (int)na.size();
+            for (int t = nx[a]; t < nx[a + 1]; ++t) slot[na[t]] = -1;
+        }
+        for (int i = 0; i < nv; ++i) cl[i] = nid[cl[i]];
+        xadj.swap(nx); adj.swap(na); w.swap(nw);
+        size.swap(nsize); vol.swap(nvol); per.swap(nper);
+        nc = k;
+    }
+    part = cl;
+    return nc;
+}
+
+std::vector<std::vector<int>> agglomerate(const SimplexMesh& m, int max_levels, int maxsize)
+{
+    std::vector<std::vector<int>> parts;
+    std::vector<int> orig(m.ne);
+    std::iota(orig.begin(), orig.end(), 0);
+    int n = m.ne;
+    for (int l = 0; l < max_levels && n > 1; ++l) {
+        AggGraph g;
+        agglomeration_graph(m, l == 0 ? nullptr : orig.data(), n, g);
+        std::vector<int> part;
+        const int nc = agglomerate_graph(g, m.N, maxsize, part);
+        if (nc >= n) break;
+        for (int e = 0; e < m.ne; ++e) orig[e] = part[orig[e]];
+        parts.push_back(std::move(part));
+        n = nc;
+    }
+    return parts;
+}
+
+void build_interp_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
+                        const std::vector<std::vector<int>>& parts, int l, Interp& I)
+{
+    const int N = m.N;
+    // element -> agglomerate of level l and of level l + 1
+    std::vector<int> ol(m.ne), oc(m.ne);
+    std::iota(ol.begin(), ol.end(), 0);
+    int nl = m.ne;
+    for (int k = 0; k < l; ++k) { for (int e = 0; e < m.ne; ++e) ol[e] = parts[k][ol[e]]; nl = (int)parts[k + 1].size(); }
+    const std::vector<int>& part = parts[l];
+    if ((int)part.size() != nl) throw Error(-1, "partition size does not match its level");
+    int nc = 0;
+    for (int v : part) { if (v < 0) throw Error(-1, "negative agglomerate id"); nc = std::max(nc, v + 1); }
+    for (int e = 0; e < m.ne; ++e) oc[e] = part[ol[e]];
+    // bounding boxes of the original vertices (AgglomeratedElement, agglomeration.h:258-276)
+    auto boxes = [&](const std::vector<int>& o, int n, std::vector<double>& lo, std::vector<double>& hi) {
+        lo.assign((size_t)n * N, 1e300); hi.assign((size_t)n * N, -1e300);
+        for (int e = 0; e < m.ne; ++e)
+            for (int v = 0; v < N + 1; ++v)
+                for (int d = 0; d < N; ++d) {
+                    const double x = m.pts[(size_t)m.tris[(size_t)e * (N + 1) + v] * N + d];
+                    lo[(size_t)o[e] * N + d] = std::min(lo[(size_t)o[e] * N + d], x);
+                    hi[(size_t)o[e] * N + d] = std::max(hi[(size_t)o[e] * N + d], x);
+                }
+    };
+    std::vector<double> clo, chi, flo, fhi;
+    boxes(oc, nc, clo, chi);
+    std::vector<int> cnt(nc, 0);
+    for (int v : part) cnt[v]++;
+    for (int c = 0; c < nc; ++c) if (!cnt[c]) throw Error(-1, "empty agglomerate in partition");
+    const int bc = Q.npl, bf = (l == 0) ? E.npl : bc;
+    I.nf = nl; I.nc = nc; I.bf = bf; I.bc = bc;
+    I.parent.assign(part.begin(), part.end());
+    I.type.resize(nl);
+    I.pool.clear();
+    I.ntypes = 0;
+    std::vector<double> blk((size_t)bf * bc);
+    if (l == 0) {
+        // T0: box Q_p Lagrange basis at the fine simplex nodes (interpolationhierarchy.h:99-138)
+        for (int e = 0; e < m.ne; ++e) {
+            const int a = part[e];
+            double rp[4][3];
+            for (int v = 0; v < N + 1; ++v)
+                for (int d = 0; d < N; ++d)
+                    rp[v][d] = (m.pts[(size_t)m.tris[(size_t)e * (N + 1) + v] * N + d] - clo[(size_t)a * N + d]) /
+                               (chi[(size_t)a * N + d] - clo[(size_t)a * N + d]);
+            for (int i = 0; i < bf; ++i) {
+                double x[3];
+                for (int d = 0; d < N; ++d) {
+                    double s = rp[0][d];
+                    for (int c = 0; c < N; ++c) s += (rp[c + 1][d] - rp[0][d]) * E.nodes[(size_t)i * N + c];
+                    x[d] = s;
+                }
+                Q.lagrangeN(x, &blk[(size_t)i * bc]);
+            }
+            I.type[e] = I.ntypes++;
+            I.pool.insert(I.pool.end(), blk.begin(), blk.end());
+        }
+    } else {
+        // T_l: identity for a lone finer agglomerate, else box -> box (:143-203)
+        boxes(ol, nl, flo, fhi);
+        std::fill(blk.begin(), blk.end(), 0.0);
+        for (int i = 0; i < bc; ++i) blk[(size_t)i * bc + i] = 1.0;
+        I.pool.insert(I.pool.end(), blk.begin(), blk.end());
+        I.ntypes = 1;
+        int it[3];
+        for (int i = 0; i < nl; ++i) {
+            const int a = part[i];
+            if (cnt[a] == 1) { I.type[i] = 0; continue; }
+            for (int q = 0; q < bc; ++q) {
+                int lin = q;
+                for (int d = N - 1; d >= 0; --d) { it[d] = lin % Q.P; lin /= Q.P; }
+                double x[3];
+                for (int d = 0; d < N; ++d) {
+                    const double wd = chi[(size_t)a * N + d] - clo[(size_t)a * N + d];
+                    const double rl = (flo[(size_t)i * N + d] - clo[(size_t)a * N + d]) / wd;
+                    const double rh = (fhi[(size_t)i * N + d] - clo[(size_t)a * N + d]) / wd;
+                    x[d] = rl + (rh - rl) * Q.nodes[it[d]];
+                }
+                Q.lagrangeN(x, &blk[(size_t)q * bc]);
+            }
+            I.type[i] = I.ntypes++;
+            I.pool.insert(I.pool.end(), blk.begin(), blk.end());
+        }
+    }
+    I.make_gather();
+}
+
+}  // namespace mgldg

@@ -1,0 +1,92 @@
+// simplex.hpp — host side of the unstructured (simplex) path: nodal simplex
+// basis, mesh from points and simplices, LDG assembly into LdgOps, the dual
+// graph of an agglomeration level, an MGridGen-style agglomerator and the
+// agglomerated interpolation blocks (P_p simplex -> Q_p box, box -> box).
+#pragma once
+
+#include "host.hpp"
+
+namespace mgldg {
+
+// Nodal basis of P_p on the reference N-simplex {x >= 0, sum x <= 1} at the
+// equispaced nodes of Master<N,P> (src/unstructured/dg/master.h:80-95; P = p+1,
+// node order of SimplexRangeIterator, range.h:291-309, last coordinate
+// fastest; p = 0: one node at (1/3, ..., 1/3)).  Built from the monomials of
+// degree <= p with exact simplex moments (long double), not from orthogonal
+// polynomials: the nodal basis, its mass and its derivative matrices are the
+// same functions either way.
+struct SimplexBasis {
+    int N = 2, P = 1, npl = 1;
+    double volume = 0.5;                  // 1/N!
+    std::vector<double> nodes;            // npl x N
+    std::vector<int> expo;                // npl x N monomial exponents
+    std::vector<double> coef;             // phi_i = sum_k coef[k * npl + i] x^expo_k
+    std::vector<double> mass, invmass;    // npl x npl
+    std::vector<double> diff[3];          // diff[d](i, j) = d phi_j / d x_d at node i
+    void init(int N, int P);
+    void eval(const double* x, double* phi) const;
+};
+
+// Simplex mesh (Mesh<N,P>(wireframe, bndfuns), src/unstructured/mesh/mesh.h:185-298)
+struct SimplexFace {
+    int left = -1, right = -1;   // right < 0: boundary, key -right
+    int v[3] = {0, 0, 0};        // sorted vertex ids
+    double normal[3] = {0, 0, 0};
+    double area = 0;
+};
+
+struct SimplexMesh {
+    int N = 2, np = 0, ne = 0, nf = 0, nbf = 0;
+    std::vector<double> pts;     // np x N
+    std::vector<int> tris;       // ne x (N+1), 0-based
+    std::vector<double> vol;     // per element
+    std::vector<double> J, Jinv; // per element, N x N row-major (J(:,i) = p_{i+1} - p_0)
+    std::vector<SimplexFace> faces;
+    std::vector<int> bfaces;     // boundary faces in face order
+    // faces in first-appearance order (element i, local face j); vmask[v] bit
+    // k = boundary predicate k holds at vertex v (nbnd predicates; 0: key 1)
+    void build(int N, int np, const double* p, int nt, const int* t, int nbnd, const int* vmask);
+    void face_point(const SimplexFace& f, const double* ref, double* x) const;
+    void to_local(int e, const double* x, double* loc) const;
+};
+
+// LDGPoisson<N,P> on a simplex mesh (src/unstructured/dg/ldgpoisson.tpp:14-170)
+struct SimplexLdg : LdgOps {
+    SimplexBasis E, F;              // element and face bases
+    SimplexMesh mesh;
+    std::vector<double> qx, qw;     // face rule: nq points (N-1 coords) on the reference face
+    std::vector<int> bctype;        // per boundary key - 1
+    void assemble(int N, int np, const double* pts, int nt, const int* tris, int nbnd, const int* vmask, int p,
+                  const int* bct, double tau0, double tauD);
+    void bc_qpoints(double* xyz, int* tag) const override;
+    void nodes_xyz(double* xyz) const override;
+};
+
+// Dual graph of one agglomeration level (src/unstructured/mesh/agglomeration.h:57-93,
+// 133-185): vertices are the agglomerates of `orig` (orig[e] = agglomerate of
+// mesh element e; null: the elements), edges carry the shared face area,
+// vvol the volume and vsurf the boundary-face area.
+struct AggGraph {
+    std::vector<int> xadj, adj;
+    std::vector<double> w, vvol, vsurf;
+};
+void agglomeration_graph(const SimplexMesh& m, const int* orig, int nparts, AggGraph& g);
+
+// MGridGen-style agglomeration of one graph: clusters of at most maxsize
+// vertices grown by rounds of matching that pick, for each cluster, the
+// neighbour giving the smallest aspect ratio perimeter^N / volume^(N-1).
+// Returns the number of agglomerates; part[v] in [0, nparts).
+int agglomerate_graph(const AggGraph& g, int N, int maxsize, std::vector<int>& part);
+
+// Agglomeration<N,P>(mesh, maxAgglom) (agglomeration.h:21-34): successive
+// partitions, level l elements -> level l+1 agglomerates, until one
+// agglomerate remains, max_levels are made, or a level no longer shrinks.
+std::vector<std::vector<int>> agglomerate(const SimplexMesh& m, int max_levels, int maxsize);
+
+// Agglomerated interpolation (interpolationhierarchy.h:88-204): level l -> l+1
+// blocks; level 0 rows are P_p simplex bases (bf = E.npl), coarse spaces are
+// tensor Q_p Gauss-Lobatto bases (Q) on the agglomerates' bounding boxes.
+void build_interp_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
+                        const std::vector<std::vector<int>>& parts, int l, Interp& I);
+
+}  // namespace mgldg

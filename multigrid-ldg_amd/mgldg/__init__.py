@@ -93,21 +93,12 @@ class Quadtree(_Handle):
         return ids
 
 
-class LDGPoisson(_Handle):
-    """Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
-    (src/cartesian/mesh/mesh.h:186-265, src/cartesian/dg/ldgpoisson.h:60-77).
-
-    bc: 'dirichlet' | 'neumann' | list of 2N types per boundary tag."""
+class _Problem(_Handle):
+    """Operations shared by the quadtree and simplex discretisations."""
 
     OPS = {"A": 0, "M": 1, "T": 2, "G0": 3, "G1": 4, "G2": 5}
 
-    def __init__(self, qt, p, bc="dirichlet", tau0=0.0, tauD=1000.0):
-        self.qt, self.ctx, self.p = qt, qt.ctx, p
-        if isinstance(bc, str):
-            bc = [{"dirichlet": 0, "neumann": 1}[bc]] * (2 * qt.N)
-        bct = np.ascontiguousarray(bc, dtype=np.int32)
-        self.h = C.c_void_p()
-        check(self.ctx, lib().mgldg_ldg_create(qt.h, p, _pi(bct), tau0, tauD, C.byref(self.h)))
+    def _sizes(self):
         self._own(lib().mgldg_problem_destroy)
         s = np.zeros(7, dtype=np.int32)
         check(self.ctx, lib().mgldg_problem_sizes(self.h, _pi(s)))
@@ -146,28 +137,134 @@ class LDGPoisson(_Handle):
         return rp, cols, blk
 
 
+class LDGPoisson(_Problem):
+    """Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
+    (src/cartesian/mesh/mesh.h:186-265, src/cartesian/dg/ldgpoisson.h:60-77).
+
+    bc: 'dirichlet' | 'neumann' | list of 2N types per boundary tag."""
+
+    simplex = False
+
+    def __init__(self, qt, p, bc="dirichlet", tau0=0.0, tauD=1000.0):
+        self.qt, self.ctx, self.p = qt, qt.ctx, p
+        if isinstance(bc, str):
+            bc = [{"dirichlet": 0, "neumann": 1}[bc]] * (2 * qt.N)
+        bct = np.ascontiguousarray(bc, dtype=np.int32)
+        self.h = C.c_void_p()
+        check(self.ctx, lib().mgldg_ldg_create(qt.h, p, _pi(bct), tau0, tauD, C.byref(self.h)))
+        self._sizes()
+
+
+class Wireframe(_Handle):
+    """Wireframe<N>(p, t) / Wireframe<N>(file) (src/unstructured/mesh/wireframe.h:12-77).
+
+    points [np, N]; simplices [nt, N+1] with 0-based vertex ids."""
+
+    def __init__(self, ctx, points=None, simplices=None, path=None, N=2):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        if path is not None:
+            check(ctx, lib().mgldg_wireframe_read(ctx.h, N, os.fsencode(path), C.byref(self.h)))
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64)
+            tri = np.ascontiguousarray(simplices, dtype=np.int32)
+            check(ctx, lib().mgldg_wireframe_create(ctx.h, pts.shape[1], pts.shape[0], _pd(pts), tri.shape[0], _pi(tri),
+                                                    C.byref(self.h)))
+        self._own(lib().mgldg_wireframe_destroy)
+        info = np.zeros(3, dtype=np.int32)
+        check(ctx, lib().mgldg_wireframe_get(self.h, _pi(info), None, None))
+        self.N, self.np, self.nt = [int(v) for v in info]
+
+    @classmethod
+    def read(cls, ctx, path, N=2):
+        return cls(ctx, path=path, N=N)
+
+    def arrays(self):
+        p = np.zeros((self.np, self.N))
+        t = np.zeros((self.nt, self.N + 1), dtype=np.int32)
+        check(self.ctx, lib().mgldg_wireframe_get(self.h, None, _pd(p), _pi(t)))
+        return p, t
+
+
+class SimplexLDGPoisson(_Problem):
+    """Mesh<N,P>(wireframe, bndfuns) + BoundaryConditions + LDGPoisson<N,P>
+    (src/unstructured/mesh/mesh.h:185-298, src/unstructured/dg/ldgpoisson.tpp:14-170).
+
+    bndfuns: predicates on a vertex (keys 1..len, first match); bc: 'dirichlet'
+    | 'neumann' | one type per key.  vmask may be given instead of bndfuns."""
+
+    simplex = True
+
+    def __init__(self, wf, p, bndfuns=None, bc="dirichlet", tau0=0.0, tauD=1000.0, vmask=None):
+        self.wf, self.ctx, self.p = wf, wf.ctx, p
+        nb = len(bndfuns) if bndfuns else 0
+        if vmask is None and nb:
+            pts, _ = wf.arrays()
+            vmask = np.zeros(len(pts), dtype=np.int32)
+            for k, fn in enumerate(bndfuns):
+                vmask |= np.array([bool(fn(x)) for x in pts], dtype=np.int32) << k
+        elif vmask is not None:
+            vmask = np.ascontiguousarray(vmask, dtype=np.int32)
+            nb = int(vmask.max()).bit_length() if nb == 0 else nb
+        if isinstance(bc, str):
+            bc = [{"dirichlet": 0, "neumann": 1}[bc]] * max(nb, 1)
+        bct = np.ascontiguousarray(bc, dtype=np.int32)
+        self.h = C.c_void_p()
+        check(self.ctx, lib().mgldg_ldg_create_simplex(wf.h, p, nb, _pi(vmask), _pi(bct), tau0, tauD, C.byref(self.h)))
+        self._sizes()
+
+
+def agglomerate(prob, max_levels=10):
+    """The library's Agglomeration<N,P>(mesh, max_levels) (agglomeration.h:21-34):
+    list of partitions, level l elements -> level l+1 agglomerates."""
+    nl = C.c_int()
+    nparts = np.zeros(max_levels, dtype=np.int32)
+    parts = np.zeros(max_levels * prob.ne, dtype=np.int32)
+    check(prob.ctx, lib().mgldg_agglomerate(prob.h, max_levels, C.byref(nl), _pi(nparts), _pi(parts)))
+    out, n = [], prob.ne
+    for l in range(nl.value):
+        out.append(parts[l * prob.ne:l * prob.ne + n].copy())
+        n = int(nparts[l])
+    return out
+
+
 class Multigrid(_Handle):
     """InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
     (src/cartesian/multigrid/interpolationhierarchy.h:30-82, multigrid.h:291-301).
 
     precision='fp64' builds the FP64 V-cycle only; 'mixed' also builds the FP32
-    copies used by the mixed-precision V-cycle."""
+    copies used by the mixed-precision V-cycle.
+
+    On a simplex problem the hierarchy is agglomerated
+    (src/unstructured/multigrid/interpolationhierarchy.h:88-204): by the
+    library's own agglomeration, or from `partitions` (list of arrays, level l
+    elements -> level l+1 agglomerates)."""
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
-                 restriction="ratrat", precision="fp64", coarse_dense_max=4096):
+                 restriction="ratrat", precision="fp64", coarse_dense_max=4096, partitions=None):
         self.prob, self.ctx = prob, prob.ctx
         self.params = MGParams({"jacobi": 0, "gs": 1, "gs-lex": 2}[relaxation], omega,
                                {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
                                {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision],
                                coarse_dense_max)
         self.h = C.c_void_p()
-        check(self.ctx, lib().mgldg_mg_create(prob.h, C.byref(self.params), C.byref(self.h)))
+        if partitions is not None:
+            parts = np.zeros(len(partitions) * prob.ne, dtype=np.int32)
+            for l, q in enumerate(partitions):
+                parts[l * prob.ne:l * prob.ne + len(q)] = q
+            check(self.ctx, lib().mgldg_mg_create_agglomerated(prob.h, C.byref(self.params), len(partitions), _pi(parts),
+                                                               C.byref(self.h)))
+        else:
+            check(self.ctx, lib().mgldg_mg_create(prob.h, C.byref(self.params), C.byref(self.h)))
         self._own(lib().mgldg_mg_destroy)
-        info = np.zeros(3 + 2 * 32, dtype=np.int64)
+        info = np.zeros(3, dtype=np.int64)
         check(self.ctx, lib().mgldg_mg_info(self.h, info.ctypes.data_as(C.POINTER(C.c_long)), len(info)))
         self.nlevels = int(info[0])
-        self.level_rows = [int(info[3 + 2 * l]) for l in range(self.nlevels)]
-        self.level_nnzb = [int(info[4 + 2 * l]) for l in range(self.nlevels)]
+        self.level_rows, self.level_B, self.level_nnzb = [], [], []
+        for l in range(self.nlevels):
+            nb, B, nz = C.c_int(), C.c_int(), C.c_long()
+            check(self.ctx, lib().mgldg_mg_level(self.h, l, C.byref(nb), C.byref(B), C.byref(nz)))
+            self.level_rows.append(nb.value); self.level_B.append(B.value); self.level_nnzb.append(nz.value)
 
     def export(self, level, which):
         w = {"A": 0, "M": 1, "T": 2, "G0": 3, "G1": 4, "G2": 5, "interp": 6}[which]
@@ -175,8 +272,9 @@ class Multigrid(_Handle):
         check(self.ctx, lib().mgldg_mg_export(self.h, level, w, None, None, None, C.byref(nz), C.byref(nbr), C.byref(nbc)))
         rp = np.zeros(nbr.value + 1, dtype=np.int32)
         cols = np.zeros(nz.value, dtype=np.int32)
-        b = self.prob.npl
-        blk = np.zeros((nz.value, b, b))
+        br = self.level_B[level]
+        bc = self.level_B[level + 1] if which == "interp" else br
+        blk = np.zeros((nz.value, br, bc))
         check(self.ctx, lib().mgldg_mg_export(self.h, level, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz), None, None))
         return rp, cols, blk, nbr.value, nbc.value
 

@@ -41,6 +41,11 @@ SIGNATURES = {
     "mgldg_quadtree_layer": (I, [V, I, PI, PI]),
     "mgldg_quadtree_destroy": (I, [V]),
     "mgldg_ldg_create": (I, [V, I, PI, D, D, C.POINTER(V)]),
+    "mgldg_wireframe_create": (I, [V, I, I, PD, I, PI, C.POINTER(V)]),
+    "mgldg_wireframe_read": (I, [V, I, C.c_char_p, C.POINTER(V)]),
+    "mgldg_wireframe_get": (I, [V, PI, PD, PI]),
+    "mgldg_wireframe_destroy": (I, [V]),
+    "mgldg_ldg_create_simplex": (I, [V, I, I, PI, PI, D, D, C.POINTER(V)]),
     "mgldg_problem_sizes": (I, [V, PI]),
     "mgldg_problem_nodes": (I, [V, PD]),
     "mgldg_problem_bc_qpoints": (I, [V, PD, PI]),
@@ -49,6 +54,9 @@ SIGNATURES = {
     "mgldg_problem_destroy": (I, [V]),
     "mgldg_mg_create": (I, [V, C.POINTER(MGParams), C.POINTER(V)]),
     "mgldg_mg_info": (I, [V, PL, I]),
+    "mgldg_mg_level": (I, [V, I, PI, PI, PL]),
+    "mgldg_agglomerate": (I, [V, I, PI, PI, PI]),
+    "mgldg_mg_create_agglomerated": (I, [V, C.POINTER(MGParams), I, PI, C.POINTER(V)]),
     "mgldg_mg_export": (I, [V, I, I, PI, PI, PD, PL, PI, PI]),
     "mgldg_mg_colors": (I, [V, I, PI, PI]),
     "mgldg_mg_destroy": (I, [V]),

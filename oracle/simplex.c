@@ -626,7 +626,12 @@ int orc_sx_agglom_graph(void* p, const int* part, int nparts, int* xadj, int* ad
 void* orc_sx_mg_new(void* p, int nlev, const int* const* parts, const int* nparts, const orc_mg_params* prm)
 {
     sxproblem* sx = (sxproblem*)p;
-    if (!sx || !params_ok(prm) || nlev < 1) return NULL;
+    /* RATRAT inverts the coarse mass matrices T^T M T, which Q_p box spaces
+     * over a few P_p simplices can make singular (a lone simplex has fewer
+     * nodes than box functions; two triangles' four vertices need not
+     * determine a bilinear function), so agglomerated hierarchies coarsen by
+     * RAT only */
+    if (!sx || !params_ok(prm) || nlev < 1 || prm->restriction != 0) return NULL;
     int N = sx->N, P = sx->P, bf = sx->E.npl;
     basis_t Bq; basis_init(&Bq, N, P);
     int bc = Bq.npl;
