@@ -6,6 +6,14 @@ f ~ N(0,1) per dof (seed 0), V(2,2) multicolour block Gauss-Seidel, RATRAT
 Galerkin hierarchy with rapid (height) coarsening, MG-preconditioned CG until
 the true residual ||b - A x|| / ||b|| <= 1e-10.
 
+A second workload is reported in the same line ("unstructured"): the unit
+disk, Delaunay triangulation of 200,000 uniform random interior points
+(radius < 1 - h/2, seed 0) and 1,571 boundary points at spacing h ~ 0.004,
+p = 2 (401,569 triangles, 2,409,414 dofs), tau0 = 100, tauD = 1000 (the
+reference's unstructured driver), the library's agglomerated hierarchy (RAT),
+V(2,2) block Jacobi (omega 0.6), f ~ N(0,1) per dof; single solves in FP64
+and mixed precision and a batch of 32 right-hand sides solved together.
+
 One step = one full solve from x = 0 with b resident in HBM.  `value` is the
 mixed-precision form (FP32 V-cycle inside FP64 CG); the plain FP64 GPU solve
 is timed beside it.  The roofline object describes the dominant kernel, the
@@ -53,6 +61,83 @@ def cpu_model():
     return platform.processor()
 
 
+def disk_mesh(n_int=200_000, h=0.004, seed=0):
+    """Unit disk: boundary points at spacing ~h on r = 1 and n_int uniform
+    random points in r < 1 - h/2, Delaunay-triangulated (scipy)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    nb = int(round(2 * np.pi / h))
+    t = 2 * np.pi * np.arange(nb) / nb
+    bnd = np.stack([np.cos(t), np.sin(t)], 1)
+    r = (1 - h / 2) * np.sqrt(rng.random(n_int))
+    a = 2 * np.pi * rng.random(n_int)
+    pts = np.concatenate([bnd, np.stack([r * np.cos(a), r * np.sin(a)], 1)])
+    return pts, Delaunay(pts).simplices.astype(np.int32)
+
+
+def unstructured_leg(mgldg, torch, nrhs, warm):
+    """The unit-disk workload: setup times, single FP64 / mixed solves, and
+    nrhs right-hand sides batched in FP64 and in mixed precision."""
+    t0 = time.perf_counter()
+    pts, tri = disk_mesh()
+    t_mesh = time.perf_counter() - t0
+    ctx = mgldg.Context(0)
+    t0 = time.perf_counter()
+    wf = mgldg.Wireframe(ctx, pts, tri)
+    prob = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
+    t_prob = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    parts = mgldg.agglomerate(prob)
+    t_agg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mg = mgldg.Multigrid(prob, "jacobi", omega=0.6, npre=2, npost=2, restriction="rat", precision="mixed",
+                         partitions=parts)
+    t_mg = time.perf_counter() - t0
+    log(f"unstructured: {prob.ne} triangles, {prob.ndof} dofs, levels {mg.level_rows}; setup mesh {t_mesh:.2f}s "
+        f"problem {t_prob:.2f}s agglomeration {t_agg:.2f}s hierarchy {t_mg:.2f}s")
+    rng = np.random.default_rng(0)
+    F = torch.tensor(rng.standard_normal((prob.ndof, nrhs)), device="cuda")
+    B = mgldg.rhs_device(mg, F)
+    b = B[:, 0].contiguous()
+    out = {}
+    xs = {}
+    for form, rhs in (("single", b), ("batched", B)):
+        for prec in ("fp64", "mixed"):
+            kw = dict(tol=1e-10, maxit=1000, criterion="true", precision=prec, flexible=(prec == "mixed"),
+                      use_graph=True)
+            for _ in range(warm):
+                mgldg.pcg(mg, rhs, torch.zeros_like(rhs), **kw)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            x = torch.zeros_like(rhs)
+            _, st = mgldg.pcg(mg, rhs, x, **kw)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            sts = st if isinstance(st, list) else [st]
+            for q in sts:
+                if q["status"] != "converged" or q["relres_true"] > 1e-10:
+                    log("UNSTRUCTURED NOT CONVERGED", form, prec, q)
+                    sys.exit(1)
+            k = len(sts)
+            out[f"{form}_{prec}"] = {"value": round(prob.ndof * k / dt, 1), "ms_per_step": round(dt * 1e3, 3),
+                                     "iterations": max(q["iterations"] for q in sts)}
+            xs[(form, prec)] = x
+            log(f"unstructured {form} {prec}: {dt * 1e3:.1f} ms, {prob.ndof * k / dt:.3g} DoF-solves/s, "
+                f"iterations {out[f'{form}_{prec}']['iterations']}")
+    x64, x32 = xs[("batched", "fp64")], xs[("batched", "mixed")]
+    diff = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    return {
+        "workload": "unit disk, Delaunay of 200k random interior + 1571 boundary points, p=2, tau0=100, tauD=1000",
+        "elements": prob.ne, "ndof": prob.ndof, "levels": mg.nlevels, "level_rows": mg.level_rows,
+        "smoother": "V(2,2) block Jacobi, omega 0.6", "coarsening": "own agglomeration (<= 4 per level), RAT",
+        "nrhs": nrhs, **out,
+        "mixed_batched_vs_fp64_single": round(out["batched_mixed"]["value"] / out["single_fp64"]["value"], 3),
+        "mixed_vs_fp64_batched_rel_diff": diff,
+        "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
+                    "hierarchy": round(t_mg, 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +148,7 @@ def main():
     ap.add_argument("--cpu-maxit", type=int, default=60)
     ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides of the batched form")
     ap.add_argument("--solve-only", action="store_true", help="setup + three mixed solves (profiling)")
+    ap.add_argument("--no-unstructured", action="store_true", help="skip the unit-disk workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -219,6 +305,8 @@ def main():
                "oracle_iterations": ito}
         log(f"cpu: {cpu_rate:.0f} DoF-solves/s, iterations {ito}, errors {err}")
 
+    unstructured = None if args.no_unstructured else unstructured_leg(mgldg, torch, k, args.warmup)
+
     line = {
         "metric": "DoF-solves/s to 1e-10 residual, mixed vs FP64 MG-PCG; smoother HBM GB/s",
         "value": round(v32, 1),
@@ -250,6 +338,7 @@ def main():
         "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "unstructured": unstructured,
     }
     print(json.dumps(line), flush=True)
 

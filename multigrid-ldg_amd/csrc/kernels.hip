@@ -1240,6 +1240,18 @@ void bc_adjoint(hipStream_t s, int nbf, int B, int nq, int k, int N, const int* 
     check_launch("bc_adjoint");
 }
 
+__global__ void k_add_at(long n, const long* __restrict__ idx, double v, double* __restrict__ a)
+{
+    long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) a[idx[t]] += v;
+}
+void add_at(hipStream_t s, long n, const long* idx, double v, double* a)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add_at, dim3(nblk(n, 256)), dim3(256), 0, s, n, idx, v, a);
+    check_launch("add_at");
+}
+
 __global__ void k_axpby(long n, double as, const double* a, double bs, const double* b, double* y)
 {
     long i = (long)blockIdx.x * 256 + threadIdx.x;

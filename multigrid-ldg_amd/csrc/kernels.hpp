@@ -82,6 +82,8 @@ void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y);
 void eye(hipStream_t s, int n, double* X);
 
 void f64_to_f32(hipStream_t s, long n, const double* a, float* b);
+// a[idx[t]] += v for t < n (distinct indices)
+void add_at(hipStream_t s, long n, const long* idx, double v, double* a);
 void f32_to_f64(hipStream_t s, long n, const float* a, double* b);
 
 // ---------------------------------------------------------------- PCG (FP64)

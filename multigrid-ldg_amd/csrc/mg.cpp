@@ -54,6 +54,7 @@ template struct DBuf<double>;
 template struct DBuf<float>;
 template struct DBuf<int>;
 template struct DBuf<unsigned>;
+template struct DBuf<long>;
 
 void Ctx::init(int dev, void* s)
 {
@@ -438,6 +439,22 @@ static void upload_interp(hipStream_t s, Level& lv, int precision)
     }
 }
 
+// directions of an agglomerate's coarse space beyond its rank have zero
+// interpolation columns: give them a unit diagonal in A so they stay decoupled
+static void pad_dropped(hipStream_t s, Level& c, const std::vector<int>& rank)
+{
+    std::vector<int> ds = diag_slots(c.A.pat);
+    std::vector<long> idx;
+    const int b = c.B;
+    for (int a = 0; a < c.nb; ++a)
+        for (int d = rank[a]; d < b; ++d) idx.push_back((long)ds[a] * b * b + (long)d * b + d);
+    if (idx.empty()) return;
+    DBuf<long> di;
+    di.upload(idx.data(), idx.size(), s);
+    dev::add_at(s, (long)idx.size(), di, 1.0, c.A.val);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<int>>* parts)
 {
     pr = p; ctx = p->ctx; prm = pm;
@@ -448,7 +465,9 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
     if (prm.npre < 0 || prm.npost < 0) throw Error(-1, "negative sweep count");
     const SimplexLdg* sx = p->simplex();
     int nl = 0;
-    Basis Q;   // coarse box basis of an agglomerated hierarchy
+    std::vector<Interp> AI;                 // agglomerated hierarchy: all interpolations
+    std::vector<std::vector<int>> crank;    // kept coarse directions per agglomerate
+    submean = sx ? 0 : 1;
     if (sx) {
         if (!parts || parts->empty()) throw Error(-1, "an agglomerated hierarchy needs at least one partition");
         // RATRAT inverts the coarse mass matrices T^T M T, which Q_p box spaces
@@ -464,7 +483,6 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
             n = nc;
         }
         nl = (int)parts->size() + 1;
-        Q.init(sx->N, sx->p + 1);
     } else {
         if (parts) throw Error(-1, "partitions apply to simplex problems only");
         const Quadtree& qt = *p->qt;
@@ -490,14 +508,22 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
         for (int d = 0; d < N; ++d) clone(f.G[d], p->G[d]);
         f.has_ops = true;
     }
+    if (sx) {
+        auto th0 = Clock::now();
+        Basis Q;   // coarse box basis
+        Q.init(sx->N, sx->p + 1);
+        build_interps_agglo(sx->mesh, sx->E, Q, *parts, AI, crank);
+        th += secs(th0);
+    }
     for (int l = 0; l + 1 < nl; ++l) {
         auto th0 = Clock::now();
-        if (sx) build_interp_agglo(sx->mesh, sx->E, Q, *parts, l, L[l].I);
+        if (sx) L[l].I = std::move(AI[l]);
         else build_interp(*p->qt, p->cart()->basis, l, L[l].I);
         th += secs(th0);
         upload_interp(s, L[l], prm.precision);
         L[l + 1].nb = L[l].I.nc; L[l + 1].B = L[l].I.bc;
         coarsen(s, N, L[l], L[l + 1], prm.restriction);
+        if (sx) pad_dropped(s, L[l + 1], crank[l]);
     }
     for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision, prm.relaxation == 2);
     // coarsest level: dense inverse of the assembled operator (FP64)
@@ -626,10 +652,10 @@ void MGH::vcycle_from(int l0)
         // the V-cycle below `cut` as one precomputed linear map (FP64)
         dev::gemv_f64<T>(s, c.nb * c.B, W, vc.b, vc.x);
     } else if (prm.coarse == 0) {
-        dev::Ops<T>::coarse(s, c.nb * c.B, c.Cinv, vc.b, vc.x, 1);
+        dev::Ops<T>::coarse(s, c.nb * c.B, c.Cinv, vc.b, vc.x, submean);
     } else {
         // b -= mean(b) then relaxations from x = 0 (multigrid.h:184-187, 367)
-        dev::Ops<T>::submean(s, (long)c.nb * c.B, vc.b);
+        if (submean) dev::Ops<T>::submean(s, (long)c.nb * c.B, vc.b);
         relax<T>(bottom, prm.npre, false, true);
         relax<T>(bottom, prm.npost, true, false);
     }
@@ -696,9 +722,9 @@ void MGH::build_collapse()
     Level& c = L[nl - 1];
     const int nc = c.nb * c.B;
     if (prm.coarse == 0) {
-        dev::MOps<double>::coarse(s, nc, k, c.Cinv, Bv[nl - 1], X[nl - 1], 1);
+        dev::MOps<double>::coarse(s, nc, k, c.Cinv, Bv[nl - 1], X[nl - 1], submean);
     } else {
-        dev::MOps<double>::coarse(s, nc, k, nullptr, Bv[nl - 1], nullptr, 1);
+        if (submean) dev::MOps<double>::coarse(s, nc, k, nullptr, Bv[nl - 1], nullptr, 1);
         mrelax(nl - 1, k, prm.npre, false, true, X[nl - 1], Bv[nl - 1], R[nl - 1]);
         if (prm.npre == 0) dev::Ops<double>::fill(s, (long)nc * k, 0.0, X[nl - 1]);
         mrelax(nl - 1, k, prm.npost, true, false, X[nl - 1], Bv[nl - 1], R[nl - 1]);
@@ -909,9 +935,9 @@ void MGH::bvcycle(int k)
     if (bottom == cut) {
         dev::gemm_f64<T>(s, n, k, W, bb<T>(BL[bottom]), bx<T>(BL[bottom]));
     } else if (prm.coarse == 0) {
-        dev::MOps<T>::coarse(s, n, k, c.Cinv, bb<T>(BL[bottom]), bx<T>(BL[bottom]), 1);
+        dev::MOps<T>::coarse(s, n, k, c.Cinv, bb<T>(BL[bottom]), bx<T>(BL[bottom]), submean);
     } else {
-        dev::MOps<T>::coarse(s, n, k, nullptr, bb<T>(BL[bottom]), nullptr, 1);
+        if (submean) dev::MOps<T>::coarse(s, n, k, nullptr, bb<T>(BL[bottom]), nullptr, 1);
         brelax<T>(bottom, k, prm.npre, false, true);
         if (prm.npre == 0) dev::Ops<T>::fill(s, (long)n * k, (T)0, bx<T>(BL[bottom]));
         brelax<T>(bottom, k, prm.npost, true, false);

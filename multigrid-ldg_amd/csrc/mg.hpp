@@ -151,6 +151,7 @@ struct MGH {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     double setup_host_s = 0, setup_dev_s = 0;
     int cut = -1;                      // first collapsed level (-1: none)
+    int submean = 1;                   // coarsest b -= mean(b) (the quadtree Multigrid, multigrid.h:365-368)
     DBuf<double> W;                    // collapsed V-cycle below `cut`, FP64 dense n x n
 
     // batched (k right-hand sides) state

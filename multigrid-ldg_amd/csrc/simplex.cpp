@@ -613,92 +613,186 @@ std::vector<std::vector<int>> agglomerate(const SimplexMesh& m, int max_levels, 
     return parts;
 }
 
-void build_interp_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
-                        const std::vector<std::vector<int>>& parts, int l, Interp& I)
+// symmetric eigenproblem by cyclic Jacobi rotations: A = V diag(w) V^T with
+// the eigenvalues descending and V's columns the eigenvectors (n x n row-major)
+static void sym_eig(int n, const double* A, double* w, double* V)
 {
-    const int N = m.N;
-    // element -> agglomerate of level l and of level l + 1
-    std::vector<int> ol(m.ne), oc(m.ne);
-    std::iota(ol.begin(), ol.end(), 0);
-    int nl = m.ne;
-    for (int k = 0; k < l; ++k) { for (int e = 0; e < m.ne; ++e) ol[e] = parts[k][ol[e]]; nl = (int)parts[k + 1].size(); }
-    const std::vector<int>& part = parts[l];
-    if ((int)part.size() != nl) throw Error(-1, "partition size does not match its level");
-    int nc = 0;
-    for (int v : part) { if (v < 0) throw Error(-1, "negative agglomerate id"); nc = std::max(nc, v + 1); }
-    for (int e = 0; e < m.ne; ++e) oc[e] = part[ol[e]];
-    // bounding boxes of the original vertices (AgglomeratedElement, agglomeration.h:258-276)
-    auto boxes = [&](const std::vector<int>& o, int n, std::vector<double>& lo, std::vector<double>& hi) {
-        lo.assign((size_t)n * N, 1e300); hi.assign((size_t)n * N, -1e300);
-        for (int e = 0; e < m.ne; ++e)
+    std::vector<double> S(A, A + (size_t)n * n);
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) V[i * n + j] = (i == j);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) { const double v = S[i * n + j] * S[i * n + j]; tot += v; if (i != j) off += v; }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = S[p * n + q];
+                if (apq == 0.0) continue;
+                const double th = (S[q * n + q] - S[p * n + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double skp = S[k * n + p], skq = S[k * n + q];
+                    S[k * n + p] = c * skp - sn * skq; S[k * n + q] = sn * skp + c * skq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double spk = S[p * n + k], sqk = S[q * n + k];
+                    S[p * n + k] = c * spk - sn * sqk; S[q * n + k] = sn * spk + c * sqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - sn * vkq; V[k * n + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) w[i] = S[i * n + i];
+    for (int i = 0; i < n; ++i) {
+        int mx = i;
+        for (int j = i + 1; j < n; ++j) if (w[j] > w[mx]) mx = j;
+        if (mx != i) {
+            std::swap(w[i], w[mx]);
+            for (int k = 0; k < n; ++k) std::swap(V[k * n + i], V[k * n + mx]);
+        }
+    }
+}
+
+constexpr double kDrop = 1e-10;   // relative eigenvalue of a dropped coarse direction
+
+// C = [v_k / sqrt(w_k) for w_k > kDrop w_0, then zero columns]; returns the rank
+static int orth_basis(int n, const double* G, double* C)
+{
+    std::vector<double> w(n), V((size_t)n * n);
+    sym_eig(n, G, w.data(), V.data());
+    int r = 0;
+    while (r < n && w[r] > kDrop * w[0]) ++r;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) C[(size_t)i * n + k] = (k < r) ? V[(size_t)i * n + k] / std::sqrt(w[k]) : 0.0;
+    return r;
+}
+
+// C (m x n) = op(A) B, op = transpose if ta; A is (ta ? k x m : m x k), B k x n
+static void mm(bool ta, int m, int n, int k, const double* A, const double* B, double* C, bool acc = false)
+{
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = acc ? C[(size_t)i * n + j] : 0.0;
+            for (int t = 0; t < k; ++t) s += (ta ? A[(size_t)t * m + i] : A[(size_t)i * k + t]) * B[(size_t)t * n + j];
+            C[(size_t)i * n + j] = s;
+        }
+}
+
+void build_interps_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
+                         const std::vector<std::vector<int>>& parts, std::vector<Interp>& Iv,
+                         std::vector<std::vector<int>>& rank)
+{
+    const int N = m.N, bf = E.npl, bc = Q.npl, nlev = (int)parts.size();
+    Iv.assign(nlev, Interp());
+    rank.assign(nlev, std::vector<int>());
+    std::vector<int> orig(m.ne);   // mesh element -> agglomerate of the current level
+    std::iota(orig.begin(), orig.end(), 0);
+    // per agglomerate of the current level: Gram matrix of its box basis and
+    // its orthonormal basis (empty on the mesh level)
+    std::vector<double> Gp, Cp, lop, hip;
+    int nprev = m.ne;
+    std::vector<double> W1((size_t)bc * bc), W2((size_t)bc * bc), Me((size_t)bf * bf);
+    int it[3];
+    for (int l = 0; l < nlev; ++l) {
+        const std::vector<int>& part = parts[l];
+        if ((int)part.size() != nprev) throw Error(-1, "partition size does not match its level");
+        int nc = 0;
+        for (int v : part) { if (v < 0) throw Error(-1, "negative agglomerate id"); nc = std::max(nc, v + 1); }
+        std::vector<int> cnt(nc, 0);
+        for (int v : part) cnt[v]++;
+        for (int c = 0; c < nc; ++c) if (!cnt[c]) throw Error(-1, "empty agglomerate in partition");
+        // bounding boxes of the original vertices (AgglomeratedElement, agglomeration.h:258-276)
+        std::vector<double> lo((size_t)nc * N, 1e300), hi((size_t)nc * N, -1e300);
+        for (int e = 0; e < m.ne; ++e) {
+            const int a = part[orig[e]];
             for (int v = 0; v < N + 1; ++v)
                 for (int d = 0; d < N; ++d) {
                     const double x = m.pts[(size_t)m.tris[(size_t)e * (N + 1) + v] * N + d];
-                    lo[(size_t)o[e] * N + d] = std::min(lo[(size_t)o[e] * N + d], x);
-                    hi[(size_t)o[e] * N + d] = std::max(hi[(size_t)o[e] * N + d], x);
+                    lo[(size_t)a * N + d] = std::min(lo[(size_t)a * N + d], x);
+                    hi[(size_t)a * N + d] = std::max(hi[(size_t)a * N + d], x);
                 }
-    };
-    std::vector<double> clo, chi, flo, fhi;
-    boxes(oc, nc, clo, chi);
-    std::vector<int> cnt(nc, 0);
-    for (int v : part) cnt[v]++;
-    for (int c = 0; c < nc; ++c) if (!cnt[c]) throw Error(-1, "empty agglomerate in partition");
-    const int bc = Q.npl, bf = (l == 0) ? E.npl : bc;
-    I.nf = nl; I.nc = nc; I.bf = bf; I.bc = bc;
-    I.parent.assign(part.begin(), part.end());
-    I.type.resize(nl);
-    I.pool.clear();
-    I.ntypes = 0;
-    std::vector<double> blk((size_t)bf * bc);
-    if (l == 0) {
-        // T0: box Q_p Lagrange basis at the fine simplex nodes (interpolationhierarchy.h:99-138)
-        for (int e = 0; e < m.ne; ++e) {
-            const int a = part[e];
-            double rp[4][3];
-            for (int v = 0; v < N + 1; ++v)
-                for (int d = 0; d < N; ++d)
-                    rp[v][d] = (m.pts[(size_t)m.tris[(size_t)e * (N + 1) + v] * N + d] - clo[(size_t)a * N + d]) /
-                               (chi[(size_t)a * N + d] - clo[(size_t)a * N + d]);
-            for (int i = 0; i < bf; ++i) {
-                double x[3];
-                for (int d = 0; d < N; ++d) {
-                    double s = rp[0][d];
-                    for (int c = 0; c < N; ++c) s += (rp[c + 1][d] - rp[0][d]) * E.nodes[(size_t)i * N + c];
-                    x[d] = s;
-                }
-                Q.lagrangeN(x, &blk[(size_t)i * bc]);
-            }
-            I.type[e] = I.ntypes++;
-            I.pool.insert(I.pool.end(), blk.begin(), blk.end());
         }
-    } else {
-        // T_l: identity for a lone finer agglomerate, else box -> box (:143-203)
-        boxes(ol, nl, flo, fhi);
-        std::fill(blk.begin(), blk.end(), 0.0);
-        for (int i = 0; i < bc; ++i) blk[(size_t)i * bc + i] = 1.0;
-        I.pool.insert(I.pool.end(), blk.begin(), blk.end());
-        I.ntypes = 1;
-        int it[3];
-        for (int i = 0; i < nl; ++i) {
+        // reference blocks (the box basis at the finer nodes), one per finer row
+        const int rb = (l == 0) ? bf : bc;
+        std::vector<double> Eb((size_t)nprev * rb * bc);
+        for (int i = 0; i < nprev; ++i) {
             const int a = part[i];
-            if (cnt[a] == 1) { I.type[i] = 0; continue; }
-            for (int q = 0; q < bc; ++q) {
-                int lin = q;
-                for (int d = N - 1; d >= 0; --d) { it[d] = lin % Q.P; lin /= Q.P; }
-                double x[3];
-                for (int d = 0; d < N; ++d) {
-                    const double wd = chi[(size_t)a * N + d] - clo[(size_t)a * N + d];
-                    const double rl = (flo[(size_t)i * N + d] - clo[(size_t)a * N + d]) / wd;
-                    const double rh = (fhi[(size_t)i * N + d] - clo[(size_t)a * N + d]) / wd;
-                    x[d] = rl + (rh - rl) * Q.nodes[it[d]];
+            double* Ei = &Eb[(size_t)i * rb * bc];
+            if (l == 0) {
+                // T0 (:99-138): the fine simplex mapped into the box's [0,1]^N, then its nodes
+                double rp[4][3];
+                for (int v = 0; v < N + 1; ++v)
+                    for (int d = 0; d < N; ++d)
+                        rp[v][d] = (m.pts[(size_t)m.tris[(size_t)i * (N + 1) + v] * N + d] - lo[(size_t)a * N + d]) /
+                                   (hi[(size_t)a * N + d] - lo[(size_t)a * N + d]);
+                for (int q = 0; q < bf; ++q) {
+                    double x[3];
+                    for (int d = 0; d < N; ++d) {
+                        double sv = rp[0][d];
+                        for (int c = 0; c < N; ++c) sv += (rp[c + 1][d] - rp[0][d]) * E.nodes[(size_t)q * N + c];
+                        x[d] = sv;
+                    }
+                    Q.lagrangeN(x, Ei + (size_t)q * bc);
                 }
-                Q.lagrangeN(x, &blk[(size_t)q * bc]);
+            } else if (cnt[a] == 1) {
+                for (int k = 0; k < bc * bc; ++k) Ei[k] = (k / bc == k % bc) ? 1.0 : 0.0;   // identity (:155-161)
+            } else {
+                // box -> box (:164-199)
+                for (int q = 0; q < bc; ++q) {
+                    int lin = q;
+                    for (int d = N - 1; d >= 0; --d) { it[d] = lin % Q.P; lin /= Q.P; }
+                    double x[3];
+                    for (int d = 0; d < N; ++d) {
+                        const double wd = hi[(size_t)a * N + d] - lo[(size_t)a * N + d];
+                        const double rl = (lop[(size_t)i * N + d] - lo[(size_t)a * N + d]) / wd;
+                        const double rh = (hip[(size_t)i * N + d] - lo[(size_t)a * N + d]) / wd;
+                        x[d] = rl + (rh - rl) * Q.nodes[it[d]];
+                    }
+                    Q.lagrangeN(x, Ei + (size_t)q * bc);
+                }
             }
-            I.type[i] = I.ntypes++;
-            I.pool.insert(I.pool.end(), blk.begin(), blk.end());
         }
+        // Gram matrices of the coarse box bases over the mesh
+        std::vector<double> G((size_t)nc * bc * bc, 0.0);
+        std::vector<double> Wf((size_t)rb * bc);
+        for (int i = 0; i < nprev; ++i) {
+            const double* Gi;
+            if (l == 0) {
+                const double sc = m.vol[i] / E.volume;
+                for (size_t k = 0; k < Me.size(); ++k) Me[k] = E.mass[k] * sc;
+                Gi = Me.data();
+            } else Gi = &Gp[(size_t)i * bc * bc];
+            const double* Ei = &Eb[(size_t)i * rb * bc];
+            mm(false, rb, bc, rb, Gi, Ei, Wf.data());
+            mm(true, bc, bc, rb, Ei, Wf.data(), &G[(size_t)part[i] * bc * bc], true);
+        }
+        std::vector<double> C((size_t)nc * bc * bc);
+        rank[l].resize(nc);
+        for (int a = 0; a < nc; ++a) rank[l][a] = orth_basis(bc, &G[(size_t)a * bc * bc], &C[(size_t)a * bc * bc]);
+        // rows in the orthonormal bases: E_i C_a from the mesh, (C_i^T G_i) E_i C_a above
+        Interp& I = Iv[l];
+        I.nf = nprev; I.nc = nc; I.bf = rb; I.bc = bc;
+        I.parent.assign(part.begin(), part.end());
+        I.type.resize(nprev);
+        I.pool.assign((size_t)nprev * rb * bc, 0.0);
+        I.ntypes = nprev;
+        for (int i = 0; i < nprev; ++i) {
+            const int a = part[i];
+            I.type[i] = i;
+            double* out = &I.pool[(size_t)i * rb * bc];
+            mm(false, rb, bc, bc, &Eb[(size_t)i * rb * bc], &C[(size_t)a * bc * bc], Wf.data());
+            if (l == 0) { std::copy(Wf.begin(), Wf.end(), out); continue; }
+            mm(true, bc, bc, bc, &Cp[(size_t)i * bc * bc], &Gp[(size_t)i * bc * bc], W2.data());
+            mm(false, bc, bc, bc, W2.data(), Wf.data(), out);
+        }
+        I.make_gather();
+        Gp.swap(G); Cp.swap(C); lop.swap(lo); hip.swap(hi);
+        for (int e = 0; e < m.ne; ++e) orig[e] = part[orig[e]];
+        nprev = nc;
     }
-    I.make_gather();
 }
 
 }  // namespace mgldg

@@ -83,10 +83,19 @@ int agglomerate_graph(const AggGraph& g, int N, int maxsize, std::vector<int>& p
 // agglomerate remains, max_levels are made, or a level no longer shrinks.
 std::vector<std::vector<int>> agglomerate(const SimplexMesh& m, int max_levels, int maxsize);
 
-// Agglomerated interpolation (interpolationhierarchy.h:88-204): level l -> l+1
-// blocks; level 0 rows are P_p simplex bases (bf = E.npl), coarse spaces are
-// tensor Q_p Gauss-Lobatto bases (Q) on the agglomerates' bounding boxes.
-void build_interp_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
-                        const std::vector<std::vector<int>>& parts, int l, Interp& I);
+// Agglomerated interpolation (interpolationhierarchy.h:88-204): the
+// operators of every level, I[l]: level l -> l+1.  Level 0 rows are P_p
+// simplex bases (bf = E.npl).  Each agglomerate's coarse space is the span of
+// the tensor Q_p Gauss-Lobatto basis (Q) of its bounding box over its
+// elements, represented in an orthonormal basis of that span in the mass
+// inner product: eigenvectors of the box basis' Gram matrix over the mesh,
+// scaled by w^{-1/2}, directions with w < 1e-10 w_max dropped (zero columns;
+// rank[l][a] = kept directions of agglomerate a of level l+1, first).  Block
+// smoothers and Galerkin products do not see such block-diagonal changes of
+// basis, while the box bases themselves are singular on small or thin
+// agglomerates.
+void build_interps_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
+                         const std::vector<std::vector<int>>& parts, std::vector<Interp>& I,
+                         std::vector<std::vector<int>>& rank);
 
 }  // namespace mgldg

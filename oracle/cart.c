@@ -1058,6 +1058,7 @@ typedef struct {
     int L;                 /* number of levels */
     olevel* lev;
     bsm* Tl;               /* L-1 interpolation operators (one block per fine row) */
+    int submean;           /* coarsest b -= mean(b) (the quadtree Multigrid, multigrid.h:365-368) */
 } omg;
 
 /* InterpolationHierarchy<N,P>(qt) (interpolationhierarchy.h:30-82) */
@@ -1126,7 +1127,7 @@ static void level_factor(olevel* lv)
 
 /* Level coarse constructor (multigrid.h:105-171), fine blocks bf, coarse bc
  * (T blocks bf x bc; the reference's Level<N,P>(ops<N,P1>, T<N,P1,P>)) */
-static void coarsen(olevel* c, const olevel* f, const bsm* T, int restriction, int N)
+static void coarsen(olevel* c, const olevel* f, const bsm* T, int restriction, int N, const int* crank)
 {
     int bf = T->br, b = T->bc;
     memset(c, 0, sizeof(*c));
@@ -1170,15 +1171,21 @@ static void coarsen(olevel* c, const olevel* f, const bsm* T, int restriction, i
         /* A_c = T^T A T (multigrid.h:158-162) */
         bsm t1; bsm_mul(&t1, &f->A, T); bsm_mul_t(&c->A, T, &t1); bsm_free(&t1);
     }
+    /* coarse directions beyond crank[i] (zero columns of T) are decoupled:
+     * unit diagonal */
+    if (crank) for (int i = 0; i < c->nb; ++i) {
+        double* D = bsm_get(&c->A, i, i);
+        for (int d = crank[i]; d < b; ++d) D[d*b + d] += 1.0;
+    }
     level_factor(c);
 }
 
 /* Multigrid(ops, hierarchy, params) (multigrid.h:291-301, setup 404-411) for
  * any hierarchy of one-block-per-row interpolation operators Tl[0..L-2] */
-static omg* mg_setup(oops* ops, int N, bsm* Tl, int L, const orc_mg_params* prm)
+static omg* mg_setup(oops* ops, int N, bsm* Tl, int L, const orc_mg_params* prm, int* const* crank, int submean)
 {
     omg* mg = (omg*)xcalloc(1, sizeof(omg));
-    mg->ops = ops; mg->N = N; mg->prm = *prm; mg->Tl = Tl; mg->L = L;
+    mg->ops = ops; mg->N = N; mg->prm = *prm; mg->Tl = Tl; mg->L = L; mg->submean = submean;
     mg->lev = (olevel*)xcalloc(L, sizeof(olevel));
     olevel* f = &mg->lev[0];
     f->nb = ops->ne; f->b = ops->b;
@@ -1186,7 +1193,7 @@ static omg* mg_setup(oops* ops, int N, bsm* Tl, int L, const orc_mg_params* prm)
     for (int d = 0; d < N; ++d) bsm_copy(&f->G[d], &ops->G[d]);
     f->has_ops = 1;
     level_factor(f);
-    for (int l = 0; l + 1 < L; ++l) coarsen(&mg->lev[l+1], &mg->lev[l], &mg->Tl[l], prm->restriction, N);
+    for (int l = 0; l + 1 < L; ++l) coarsen(&mg->lev[l+1], &mg->lev[l], &mg->Tl[l], prm->restriction, N, crank ? crank[l] : NULL);
     /* coarsest level with more than one block: dense LDLT of the assembled operator */
     olevel* c = &mg->lev[L-1];
     if (prm->coarse == 0 && c->nb > 1) {
@@ -1212,7 +1219,7 @@ void* orc_mg_new(void* p, const orc_mg_params* prm)
     if (pr->mesh.qt->numLevels < 2) return NULL;   /* interpolationhierarchy.h:33 */
     int L = 0;
     bsm* Tl = build_interp_qt(pr, &L);
-    return mg_setup(&pr->ops, pr->mesh.N, Tl, L, prm);
+    return mg_setup(&pr->ops, pr->mesh.N, Tl, L, prm, NULL, 1);
 }
 
 void orc_mg_free(void* p)
@@ -1317,8 +1324,10 @@ static void mg_vcycle(omg* mg, int i)
 {
     olevel* lv = &mg->lev[i]; long n = (long)lv->nb*lv->b;
     if (i == mg->L - 1) {
-        double mean = 0.0; for (long k = 0; k < n; ++k) mean += lv->bv[k]; mean /= (double)n;
-        for (long k = 0; k < n; ++k) lv->bv[k] -= mean;
+        if (mg->submean) {
+            double mean = 0.0; for (long k = 0; k < n; ++k) mean += lv->bv[k]; mean /= (double)n;
+            for (long k = 0; k < n; ++k) lv->bv[k] -= mean;
+        }
         lv_solve(lv, &mg->prm);
         return;
     }

@@ -619,10 +619,81 @@ int orc_sx_agglom_graph(void* p, const int* part, int nparts, int* xadj, int* ad
     return ne_;
 }
 
+/* symmetric eigenproblem by cyclic Jacobi rotations: A = V diag(w) V^T,
+ * eigenvalues in descending order, V's columns the eigenvectors */
+static void sym_eig(int n, const double* A, double* w, double* V)
+{
+    double* S = (double*)xmalloc(sizeof(double)*n*n);
+    memcpy(S, A, sizeof(double)*n*n);
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) V[i*n + j] = (i == j);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) { tot += S[i*n + j]*S[i*n + j]; if (i != j) off += S[i*n + j]*S[i*n + j]; }
+        if (off <= 1e-30*tot) break;
+        for (int p = 0; p < n; ++p) for (int q = p + 1; q < n; ++q) {
+            double apq = S[p*n + q];
+            if (apq == 0.0) continue;
+            double th = (S[q*n + q] - S[p*n + p])/(2.0*apq);
+            double t = (th >= 0 ? 1.0 : -1.0)/(fabs(th) + sqrt(th*th + 1.0));
+            double c = 1.0/sqrt(t*t + 1.0), sn = t*c;
+            for (int k = 0; k < n; ++k) {   /* S <- S J */
+                double skp = S[k*n + p], skq = S[k*n + q];
+                S[k*n + p] = c*skp - sn*skq; S[k*n + q] = sn*skp + c*skq;
+            }
+            for (int k = 0; k < n; ++k) {   /* S <- J^T S */
+                double spk = S[p*n + k], sqk = S[q*n + k];
+                S[p*n + k] = c*spk - sn*sqk; S[q*n + k] = sn*spk + c*sqk;
+            }
+            for (int k = 0; k < n; ++k) {   /* V <- V J */
+                double vkp = V[k*n + p], vkq = V[k*n + q];
+                V[k*n + p] = c*vkp - sn*vkq; V[k*n + q] = sn*vkp + c*vkq;
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) w[i] = S[i*n + i];
+    /* selection sort, descending */
+    for (int i = 0; i < n; ++i) {
+        int m = i; for (int j = i + 1; j < n; ++j) if (w[j] > w[m]) m = j;
+        if (m != i) {
+            double t = w[i]; w[i] = w[m]; w[m] = t;
+            for (int k = 0; k < n; ++k) { double u = V[k*n + i]; V[k*n + i] = V[k*n + m]; V[k*n + m] = u; }
+        }
+    }
+    free(S);
+}
+
+/* Relative eigenvalue below which a direction of an agglomerate's coarse
+ * space counts as zero. */
+#define SX_DROP 1e-10
+
+/* C = [v_k / sqrt(w_k) for w_k > SX_DROP w_0, then zero columns]: an
+ * orthonormal basis of the span in the inner product G; returns the rank */
+static int orth_basis(int n, const double* G, double* C)
+{
+    double w[SX_MAXNPL], V[SX_MAXNPL*SX_MAXNPL];
+    sym_eig(n, G, w, V);
+    int r = 0;
+    while (r < n && w[r] > SX_DROP*w[0]) ++r;
+    for (int i = 0; i < n; ++i) for (int k = 0; k < n; ++k) C[i*n + k] = (k < r) ? V[i*n + k]/sqrt(w[k]) : 0.0;
+    return r;
+}
+
 /* Agglomerated InterpolationHierarchy (interpolationhierarchy.h:88-204) and
  * Multigrid on it.  parts[l] partitions the elements of level l (level 0 =
  * the mesh) into nparts[l] agglomerates; nlev partitions give nlev+1 levels.
- * Coarse spaces: tensor Q_p (Gauss-Lobatto nodes) on agglomerate bounding boxes. */
+ *
+ * Coarse spaces: tensor Q_p (Gauss-Lobatto) on the agglomerates' bounding
+ * boxes, as in the reference, restricted to the mesh.  On a small or thin
+ * agglomerate these box functions are nearly or exactly linearly dependent
+ * (a lone P_p simplex has fewer nodes than Q_p has functions), so each
+ * agglomerate's space is represented in an orthonormal basis of its span in
+ * the mass inner product (eigenvectors of its Gram matrix over the mesh,
+ * scaled by w^{-1/2}), directions with w < SX_DROP w_max dropped: their
+ * columns of T are zero and their diagonal entries of A are one.  Block
+ * smoothers and Galerkin products are unchanged by such block-diagonal
+ * changes of basis, so where the reference's spaces are well posed the
+ * V-cycle is the same.  The coarsest solve is exact without the quadtree
+ * Multigrid's mean subtraction. */
 void* orc_sx_mg_new(void* p, int nlev, const int* const* parts, const int* nparts, const orc_mg_params* prm)
 {
     sxproblem* sx = (sxproblem*)p;
@@ -636,11 +707,15 @@ void* orc_sx_mg_new(void* p, int nlev, const int* const* parts, const int* npart
     basis_t Bq; basis_init(&Bq, N, P);
     int bc = Bq.npl;
     bsm* Tl = (bsm*)xcalloc(nlev, sizeof(bsm));
-    /* bounding boxes of each level's agglomerates (of their original vertices) */
+    int** crank = (int**)xcalloc(nlev, sizeof(int*));
+    /* per agglomerate of the current coarse level: Gram matrix of its box
+     * basis (G) and its orthonormal basis (C) */
+    double *Gp = NULL, *Cp = NULL;
     double* lo_prev = NULL; double* hi_prev = NULL; int nprev = sx->ne;
     int* orig = (int*)xmalloc(sizeof(int)*sx->ne);   /* original element -> agglomerate of the current level */
     for (int e = 0; e < sx->ne; ++e) orig[e] = e;
-    double Erow[SX_MAXNPL*SX_MAXNPL]; int it[3];
+    double Erow[SX_MAXNPL*SX_MAXNPL], W1[SX_MAXNPL*SX_MAXNPL], W2[SX_MAXNPL*SX_MAXNPL], Me[SX_MAXNPL*SX_MAXNPL];
+    int it[3];
     for (int l = 0; l < nlev; ++l) {
         int nc = nparts[l]; const int* part = parts[l];
         double* lo = (double*)xmalloc(sizeof(double)*nc*N); double* hi = (double*)xmalloc(sizeof(double)*nc*N);
@@ -655,28 +730,26 @@ void* orc_sx_mg_new(void* p, int nlev, const int* const* parts, const int* npart
         }
         int* cntc = (int*)xcalloc(nc, sizeof(int));
         for (int i = 0; i < nprev; ++i) cntc[part[i]]++;
-        if (l == 0) {
-            /* T0: coarse Q_p Lagrange basis at the fine simplex nodes (:99-138) */
-            bsm_init(&Tl[0], sx->ne, nc, bf, bc);
-            for (int e = 0; e < sx->ne; ++e) {
-                int a = part[e];
-                /* the fine simplex relative to the box mapped to [0,1]^N, then its nodes (:112-128) */
+        /* reference interpolation blocks (fine rows x box columns), one per fine row */
+        int rb = (l == 0) ? bf : bc;
+        double* E = (double*)xmalloc(sizeof(double)*nprev*rb*bc);
+        for (int i = 0; i < nprev; ++i) {
+            int a = part[i]; double* Ei = E + (size_t)i*rb*bc;
+            if (l == 0) {
+                /* T0: the box's Q_p Lagrange basis at the fine simplex nodes, the fine
+                 * simplex first mapped into the box's [0,1]^N (:112-128) */
                 double rp[4][3];
                 for (int v = 0; v < N + 1; ++v) for (int r = 0; r < N; ++r)
-                    rp[v][r] = (sx->p[sx->t[e*(N+1) + v]*N + r] - lo[a*N + r])/(hi[a*N + r] - lo[a*N + r]);
-                for (int i = 0; i < bf; ++i) {
+                    rp[v][r] = (sx->p[sx->t[i*(N+1) + v]*N + r] - lo[a*N + r])/(hi[a*N + r] - lo[a*N + r]);
+                for (int q = 0; q < bf; ++q) {
                     double x[3];
-                    for (int r = 0; r < N; ++r) { double s = rp[0][r]; for (int c = 0; c < N; ++c) s += (rp[c+1][r] - rp[0][r])*sx->E.nodes[i*N + c]; x[r] = s; }
-                    lagrange_eval_nd(&Bq, x, Erow + i*bc);
+                    for (int r = 0; r < N; ++r) { double s = rp[0][r]; for (int c = 0; c < N; ++c) s += (rp[c+1][r] - rp[0][r])*sx->E.nodes[q*N + c]; x[r] = s; }
+                    lagrange_eval_nd(&Bq, x, Ei + q*bc);
                 }
-                bsm_add(&Tl[0], e, a, 1.0, Erow);
-            }
-        } else {
-            /* T_l: identity for a single finer agglomerate, else box-to-box Q_p (:143-203) */
-            bsm_init(&Tl[l], nprev, nc, bc, bc);
-            for (int i = 0; i < nprev; ++i) {
-                int a = part[i];
-                if (cntc[a] == 1) { for (int k = 0; k < bc*bc; ++k) Erow[k] = (k/bc == k%bc); bsm_add(&Tl[l], i, a, 1.0, Erow); continue; }
+            } else if (cntc[a] == 1) {
+                for (int k = 0; k < bc*bc; ++k) Ei[k] = (k/bc == k%bc);   /* identity (:155-161) */
+            } else {
+                /* box -> box: the coarse box's basis at the finer box's nodes (:164-199) */
                 for (int q = 0; q < bc; ++q) {
                     unlin(q, N, P, it); double x[3];
                     for (int r = 0; r < N; ++r) {
@@ -684,15 +757,44 @@ void* orc_sx_mg_new(void* p, int nlev, const int* const* parts, const int* npart
                         double rh = (hi_prev[i*N + r] - lo[a*N + r])/(hi[a*N + r] - lo[a*N + r]);
                         x[r] = rl + (rh - rl)*Bq.nodes[it[r]];
                     }
-                    lagrange_eval_nd(&Bq, x, Erow + q*bc);
+                    lagrange_eval_nd(&Bq, x, Ei + q*bc);
                 }
-                bsm_add(&Tl[l], i, a, 1.0, Erow);
             }
         }
-        free(cntc);
+        /* Gram matrices of the coarse box bases over the mesh: sum over the
+         * finer rows of E^T G_fine E (G_fine = element mass on level 0) */
+        double* G = (double*)xcalloc((size_t)nc*bc*bc, sizeof(double));
+        for (int i = 0; i < nprev; ++i) {
+            const double* Gi;
+            if (l == 0) { for (int k = 0; k < bf*bf; ++k) Me[k] = sx->E.mass[k]*(sx->vol[i]/sx->E.volume); Gi = Me; }
+            else Gi = Gp + (size_t)i*bc*bc;
+            const double* Ei = E + (size_t)i*rb*bc;
+            gemm(0, 0, rb, bc, rb, 1.0, Gi, rb, Ei, bc, 0.0, W1, bc);
+            gemm(1, 0, bc, bc, rb, 1.0, Ei, bc, W1, bc, 1.0, G + (size_t)part[i]*bc*bc, bc);
+        }
+        double* C = (double*)xmalloc(sizeof(double)*(size_t)nc*bc*bc);
+        crank[l] = (int*)xmalloc(sizeof(int)*nc);
+        for (int a = 0; a < nc; ++a) crank[l][a] = orth_basis(bc, G + (size_t)a*bc*bc, C + (size_t)a*bc*bc);
+        /* T_l rows in the orthonormal bases: (C_i^T G_i) E_i C_a on coarse
+         * levels, E_i C_a from the mesh */
+        bsm_init(&Tl[l], nprev, nc, rb, bc);
+        for (int i = 0; i < nprev; ++i) {
+            int a = part[i]; const double* Ei = E + (size_t)i*rb*bc;
+            gemm(0, 0, rb, bc, bc, 1.0, Ei, bc, C + (size_t)a*bc*bc, bc, 0.0, W1, bc);
+            if (l == 0) { bsm_add(&Tl[l], i, a, 1.0, W1); continue; }
+            gemm(1, 0, bc, bc, bc, 1.0, Cp + (size_t)i*bc*bc, bc, Gp + (size_t)i*bc*bc, bc, 0.0, W2, bc);
+            gemm(0, 0, bc, bc, bc, 1.0, W2, bc, W1, bc, 0.0, Erow, bc);
+            bsm_add(&Tl[l], i, a, 1.0, Erow);
+        }
+        free(E); free(cntc); free(Gp); free(Cp);
+        Gp = G; Cp = C;
         for (int e = 0; e < sx->ne; ++e) orig[e] = part[orig[e]];
         free(lo_prev); free(hi_prev); lo_prev = lo; hi_prev = hi; nprev = nc;
     }
+    free(Gp); free(Cp);
     free(lo_prev); free(hi_prev); free(orig); basis_free(&Bq);
-    return mg_setup(&sx->ops, N, Tl, nlev + 1, prm);
+    omg* mg = mg_setup(&sx->ops, N, Tl, nlev + 1, prm, crank, 0);
+    for (int l = 0; l < nlev; ++l) free(crank[l]);
+    free(crank);
+    return mg;
 }

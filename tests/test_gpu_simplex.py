@@ -4,14 +4,13 @@ reference MGridGen's partitions (tests/golden/partitions.json).
 
 Bars: patterns bit-exact; assembled operators within 1e-13 relative (the
 library builds the nodal simplex basis from monomials, the oracle from
-Koornwinder polynomials as the reference does); level operators within
-1e-12.  Smoothers and V-cycles apply diagonal block inverses (the library: an
-explicit inverse; the oracle: LDLT solves, as the reference), so on a level
-whose diagonal blocks have condition number kappa they agree to
-max(1e-12, 10 kappa eps): the Q_p box spaces of agglomerates that fill a small
-part of their bounding box reach kappa ~ 1e8.  PCG iteration counts within
-one of the oracle's under the same colouring; solutions to 1e-8 relative at
-tol 1e-10."""
+Koornwinder polynomials as the reference does).  Coarse levels use
+M-orthonormal bases of the agglomerates' Q_p box spaces that each side picks
+by its own eigen-decomposition, so coarse quantities are compared through
+basis-free forms (projectors, lifted operators, kernels conjugated by the
+block-orthogonal map between the bases) within 1e-9; V-cycles within 1e-9
+(FP64) and 1e-5 (FP32 V-cycle).  PCG iteration counts within one of the
+oracle's under the same colouring; solutions to 1e-8 relative at tol 1e-10."""
 import json
 import math
 import os
@@ -38,11 +37,6 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def block_bar(omg, l):
-    """max(1e-12, 10 kappa eps) for the diagonal blocks of level l"""
-    rp, cols, blk, nbr, _ = omg.export(l, "A")
-    kappa = max(np.linalg.cond(blk[k]) for i in range(nbr) for k in range(rp[i], rp[i + 1]) if cols[k] == i)
-    return max(1e-12, 10 * kappa * np.finfo(float).eps)
 
 
 def build_pair(gctx, oracle, name, p, tau0=100.0, tauD=1000.0, bndfuns=None, bc=None):
@@ -101,61 +95,103 @@ def test_mixed_boundary_keys(gctx, oracle):
     assert rel(pr.rhs(None, g), opr.rhs(None, g)) < 1e-13
 
 
-@pytest.mark.parametrize("name,p", [("circle2", 1), ("circle2", 2), ("square_32", 2), ("square_16", 3)])
+def dense(exported):
+    rp, cols, blk, nbr, nbc = exported
+    br, bc = blk.shape[1], blk.shape[2]
+    D = np.zeros((nbr * br, nbc * bc))
+    for i in range(nbr):
+        for k in range(rp[i], rp[i + 1]):
+            D[i * br:(i + 1) * br, cols[k] * bc:(cols[k] + 1) * bc] += blk[k]
+    return D
+
+
+def composites(mg, nlevels):
+    """Q_l = T_0 ... T_{l-1}: level-l coefficients -> fine nodal values"""
+    Q = [None]
+    for l in range(nlevels - 1):
+        T = dense(mg.export(l, "interp"))
+        Q.append(T if l == 0 else Q[-1] @ T)
+    return Q
+
+
+@pytest.mark.parametrize("name,p", [("circle2", 1), ("circle2", 2), ("square_16", 2), ("square_16", 3)])
 def test_agglomerated_hierarchy_matches_oracle(gctx, oracle, name, p):
+    """Coarse bases are M-orthonormal bases of the agglomerates' box spaces,
+    chosen independently by the library and the oracle, so the comparison is
+    through basis-free quantities: the M-orthogonal projector onto each level
+    (Q Q^T M) and the coarse operator lifted to the mesh (Q A_l Q^T)."""
     import mgldg
     wf, pr, opr = build_pair(gctx, oracle, name, p)
     parts = partitions(name)
     mg = mgldg.Multigrid(pr, "gs", restriction="rat", partitions=parts, coarse_dense_max=0)
     omg = oracle.SxMultigrid(opr, parts, relaxation="mcgs", restriction="rat")
     assert mg.nlevels == omg.nlevels == len(parts) + 1
+    M = opr.dense("M")
+    Qg, Qo = composites(mg, mg.nlevels), composites(omg, mg.nlevels)
+    Ag = [dense(mg.export(l, "A")) for l in range(mg.nlevels)]
     for l in range(mg.nlevels):
         assert (mg.level_rows[l], mg.level_B[l]) == omg.level_dims(l)
         a, b = mg.export(l, "A"), omg.export(l, "A")
-        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), l
-        assert rel(a[2], b[2]) < 1e-12, l
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), l     # block patterns
         c1, n1 = mg.colors(l)
         c2, n2 = omg.colors(l)
         assert n1 == n2 and np.array_equal(c1, c2)
-        if l + 1 < mg.nlevels:
-            ti, to = mg.export(l, "interp"), omg.export(l, "interp")
-            assert np.array_equal(ti[1], to[1]) and rel(ti[2], to[2]) < 1e-13
+        if l == 0:
+            assert rel(a[2], b[2]) < 1e-13
+            continue
+        Q = Qg[l]
+        G = Q.T @ M @ Q
+        d = np.diag(G)
+        assert np.abs(G - np.diag(d)).max() < 1e-9 and np.all((np.abs(d - 1) < 1e-9) | (np.abs(d) < 1e-9)), l
+        assert rel(Q @ Q.T, Qo[l] @ Qo[l].T) < 1e-9, l
+        # Galerkin on the GPU: A_l = T^T A_{l-1} T, unit diagonal on dropped directions
+        T = dense(mg.export(l - 1, "interp"))
+        dropped = np.abs(T).max(axis=0) == 0
+        assert rel(Ag[l], T.T @ Ag[l - 1] @ T + np.diag(dropped.astype(float))) < 1e-12, l
+        Ao = dense(omg.export(l, "A"))
+        assert rel(Q @ Ag[l] @ Q.T, Qo[l] @ Ao @ Qo[l].T) < 1e-9, l
     with pytest.raises(mgldg.MGLDGError):
         mgldg.Multigrid(pr, "gs", restriction="ratrat", partitions=parts)
 
 
 @pytest.mark.parametrize("relax", ["gs", "jacobi"])
 def test_agglomerated_kernels_match_oracle(gctx, oracle, relax):
+    """Level kernels commute with the block-orthogonal map O = Q^T M Q_oracle
+    between the two coarse bases: spmv(O x) = O spmv_oracle(x), likewise the
+    smoothers (block smoothers do not depend on the basis within a block)."""
     import mgldg
     import torch
     wf, pr, opr = build_pair(gctx, oracle, "circle2", 2)
     parts = partitions("circle2")
-    mg = mgldg.Multigrid(pr, relax, omega=0.6 if relax == "jacobi" else 0.8, npre=2, npost=2, restriction="rat",
+    om = 0.6 if relax == "jacobi" else 0.8
+    mg = mgldg.Multigrid(pr, relax, omega=om, npre=2, npost=2, restriction="rat",
                          precision="mixed", partitions=parts, coarse_dense_max=0)
     omg = oracle.SxMultigrid(opr, parts, relaxation="mcgs" if relax == "gs" else "jacobi",
-                             omega=0.6 if relax == "jacobi" else 0.8, npre=2, npost=2, restriction="rat")
+                             omega=om, npre=2, npost=2, restriction="rat")
+    M = opr.dense("M")
+    Qg, Qo = composites(mg, mg.nlevels), composites(omg, mg.nlevels)
     rng = np.random.default_rng(7)
-    bars = [block_bar(omg, l) for l in range(mg.nlevels)]
     for l in range(mg.nlevels):
         n = mg.level_rows[l] * mg.level_B[l]
+        O = np.eye(n) if l == 0 else Qg[l].T @ M @ Qo[l]
+        bar = 1e-12 if l == 0 else 1e-9
         x = rng.standard_normal(n)
-        xt = torch.tensor(x, device="cuda")
+        xt = torch.tensor(O @ x, device="cuda")
         y = torch.zeros_like(xt)
         mg.spmv(l, xt, y)
-        assert rel(y.cpu().numpy(), omg.spmv(l, x)) < 1e-13
+        assert rel(y.cpu().numpy(), O @ omg.spmv(l, x)) < bar, l
         b = rng.standard_normal(n)
         for reverse in (False, True):
-            xs = torch.tensor(x, device="cuda")
-            mg.smooth(l, torch.tensor(b, device="cuda"), xs, 2, reverse)
-            assert rel(xs.cpu().numpy(), omg.relax(l, b, x, 2, reverse)) < bars[l], l
+            xs = torch.tensor(O @ x, device="cuda")
+            mg.smooth(l, torch.tensor(O @ b, device="cuda"), xs, 2, reverse)
+            assert rel(xs.cpu().numpy(), O @ omg.relax(l, b, x, 2, reverse)) < bar, (l, reverse)
     b = rng.standard_normal(pr.ndof)
     z = torch.zeros(pr.ndof, dtype=torch.float64, device="cuda")
     mg.vcycle(torch.tensor(b, device="cuda"), z)
     zo = omg.vcycle(b)
-    assert rel(z.cpu().numpy(), zo) < max(bars)
+    assert rel(z.cpu().numpy(), zo) < 1e-9
     mg.vcycle(torch.tensor(b, device="cuda"), z, precision="mixed")
-    print("mixed V-cycle vs FP64 oracle:", rel(z.cpu().numpy(), zo))
-    assert rel(z.cpu().numpy(), zo) < 1e-2
+    assert rel(z.cpu().numpy(), zo) < 1e-5
 
 
 @pytest.mark.parametrize("collapse", [0, 4096])

@@ -234,41 +234,32 @@ def test_mgridgen_partitions_fixture():
             assert sizes.min() >= 1 and sizes.max() <= 4      # minsize 1, maxsize 2^N
 
 
-def boxes(pts, tris, orig):
-    n = int(orig.max()) + 1
-    lo = np.full((n, 2), np.inf)
-    hi = np.full((n, 2), -np.inf)
-    for e, t in enumerate(tris):
-        lo[orig[e]] = np.minimum(lo[orig[e]], pts[t].min(axis=0))
-        hi[orig[e]] = np.maximum(hi[orig[e]], pts[t].max(axis=0))
-    return lo, hi
-
-
-def box_nodes(lo, hi, p):
-    g = po.basis_tables(p + 1)["nodes"]
-    return np.array([[lo[0] + (hi[0] - lo[0]) * g[i], lo[1] + (hi[1] - lo[1]) * g[j]]
-                     for i in range(p + 1) for j in range(p + 1)])
-
-
-@pytest.mark.parametrize("p", [1, 2])
-def test_agglomerated_interpolation_exact_on_Qp(p):
-    """T0 evaluates the coarse Q_p function at the fine nodes; T_l maps box to box."""
-    pts, tris = mesh("circle2")
-    parts = partitions("circle2")
-    pr = po.SxProblem(pts, tris, p, bndfuns=[on_circle], tau0=100, tauD=1000)
+@pytest.mark.parametrize("name,p", [("circle2", 1), ("circle2", 2), ("square_16", 2)])
+def test_agglomerated_spaces(name, p):
+    """Each level's space holds the Q_p box functions of its agglomerates
+    (interpolationhierarchy.h:99-199), here checked through global Q_p
+    polynomials: the M-orthogonal projection onto the level reproduces them.
+    The coarse bases are M-orthonormal: Q_l^T M Q_l is diagonal with ones for
+    the kept directions and zeros for the dropped ones."""
+    pts, tris = mesh(name)
+    parts = partitions(name)
+    pr = po.SxProblem(pts, tris, p, tau0=100, tauD=1000)
     mg = po.SxMultigrid(pr, parts, relaxation="jacobi", omega=0.6, restriction="rat")
     assert mg.nlevels == len(parts) + 1
     assert mg.level_dims(0) == (pr.ne, pr.b) and mg.level_dims(1)[1] == (p + 1) ** 2
-    q = lambda X: (1 + X[:, 0]) ** p * (0.3 - X[:, 1]) ** p + 0.5 * X[:, 0] ** p * X[:, 1]   # in Q_p
-    orig = np.arange(pr.ne)
-    fine_vals = q(pr.nodes())
-    for l, part in enumerate(parts):
-        orig = part[orig]
-        lo, hi = boxes(pts, tris, orig)
-        coarse = np.concatenate([q(box_nodes(lo[a], hi[a], p)) for a in range(len(lo))])
-        T = mg.dense(l, "interp")
-        np.testing.assert_allclose(T @ coarse, fine_vals, atol=1e-12 * np.abs(fine_vals).max())
-        fine_vals = coarse
+    X = pr.nodes()
+    M = pr.dense("M")
+    qs = [(1 + X[:, 0]) ** p * (0.3 - X[:, 1]) ** p + 0.5 * X[:, 0] ** p * X[:, 1],
+          X[:, 0] ** p * X[:, 1] ** p - 0.2 * X[:, 1]]   # in Q_p
+    Q = np.eye(pr.ndof)
+    for l in range(mg.nlevels - 1):
+        Q = Q @ mg.dense(l, "interp")
+        G = Q.T @ M @ Q
+        d = np.diag(G)
+        assert np.abs(G - np.diag(d)).max() < 1e-9
+        assert np.all((np.abs(d - 1) < 1e-9) | (np.abs(d) < 1e-9))
+        for q in qs:
+            assert np.abs(Q @ (Q.T @ (M @ q)) - q).max() < 1e-7 * np.abs(q).max(), l
 
 
 def test_agglomerated_galerkin():
@@ -279,7 +270,10 @@ def test_agglomerated_galerkin():
     for l in range(mg.nlevels - 1):
         T = mg.dense(l, "interp")
         Ac = mg.dense(l + 1, "A")
-        np.testing.assert_allclose(Ac, T.T @ A @ T, atol=1e-11 * np.abs(Ac).max())
+        D = Ac - T.T @ A @ T
+        # dropped directions: zero columns of T, unit diagonal of A_c
+        dropped = np.abs(T).max(axis=0) == 0
+        np.testing.assert_allclose(D, np.diag(dropped.astype(float)), atol=1e-11 * np.abs(Ac).max())
         assert np.linalg.eigvalsh(Ac).min() > 0
         A = Ac
 
