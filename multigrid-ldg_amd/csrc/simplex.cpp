@@ -282,8 +282,12 @@ void SimplexMesh::to_local(int e, const double* x, double* loc) const
 
 // ======================================================================= assembly
 // Face rule on the reference (N-1)-simplex, weights summing to one
-// (Quadrature<N-1,Q>, src/unstructured/dg/master.h:359-401): the Q = 10 point
-// Gauss rule on [0,1] for triangle meshes.
+// (Quadrature<N-1,10>, src/unstructured/dg/master.h:359-442): the 10-point
+// Gauss rule on [0,1] for triangle meshes; for tetrahedra Dunavant's degree-9
+// rule with 19 points on the unit triangle (D. A. Dunavant, Int. J. Numer.
+// Meth. Eng. 21 (1985)), in the reference table's point order: the centroid,
+// four orbits (a,b), (b,a), (b,b) and one orbit (a,b), (c,a), (b,c), (a,c),
+// (c,b), (b,a), with the table's 15-digit values.
 static void face_rule(int N, std::vector<double>& x, std::vector<double>& w)
 {
     if (N == 2) {
@@ -291,7 +295,19 @@ static void face_rule(int N, std::vector<double>& x, std::vector<double>& w)
         gauss_rule(kQuad, x.data(), w.data());
         return;
     }
-    throw Error(-1, "3-D simplex meshes are not supported yet (face quadrature)");
+    struct Orbit3 { double a, b, w; };
+    static const Orbit3 o3[4] = {
+        {2.06349616025250E-02, 4.89682519198738E-01, 0.031334700227139},
+        {1.25820817014127E-01, 4.37089591492937E-01, 0.077827541004774},
+        {6.23592928761935E-01, 1.88203535619033E-01, 0.079647738927210},
+        {9.10540973211095E-01, 4.47295133944530E-02, 0.025577675658698}};
+    const double a6 = 3.68384120547360E-02, b6 = 2.21962989160766E-01, c6 = 7.41198598784498E-01;
+    const double w6 = 0.043283539377289;
+    x.clear(); w.clear();
+    auto put = [&](double u, double v, double wt) { x.push_back(u); x.push_back(v); w.push_back(wt); };
+    put(3.33333333333333E-01, 3.33333333333333E-01, 0.097135796282799);
+    for (const Orbit3& o : o3) { put(o.a, o.b, o.w); put(o.b, o.a, o.w); put(o.b, o.b, o.w); }
+    put(a6, b6, w6); put(c6, a6, w6); put(b6, c6, w6); put(a6, c6, w6); put(c6, b6, w6); put(b6, a6, w6);
 }
 
 static void gemm_sq(int n, double alpha, const double* A, const double* B, double* C)

@@ -13,5 +13,5 @@ compiled from its sources into ``oracle/_ref`` (``make -C oracle ref``).
 """
 from .pyoracle import (  # noqa: F401
     lib, build, Quadtree, Problem, Multigrid, pcg, basis_tables, bsr_spmv, bsr_to_dense,
-    MGParams, read_mesh, sx_master, SxProblem, SxMultigrid, mgridgen, mgridgen_available, agglomerate,
+    MGParams, read_mesh, sx_master, sx_face_rule, SxProblem, SxMultigrid, mgridgen, mgridgen_available, agglomerate,
 )

@@ -101,6 +101,8 @@ int   orc_pcg(void* pr, void* mg, const double* b, double* x, double tol, int ma
 /* Koornwinder nodal master element Master<N,P=p+1>: returns npl; nodes npl*N,
    mass npl*npl, diff N*npl*npl (any may be NULL) */
 int   orc_sx_master(int N, int P, double* nodes, double* mass, double* diff);
+/* face rule of an N-D simplex mesh (Quadrature<N-1,10>): returns nq; x nq*(N-1), w nq */
+int   orc_sx_face_rule(int N, double* x, double* w);
 /* mesh + LDG assembly (mesh.h:185-298, ldgpoisson.tpp:14-99).  tris 0-based
    (N+1 per simplex); vmask[v] bit k = boundary function k holds at vertex v
    (nbnd functions; 0 = one boundary tag); bctype[max(nbnd,1)] 0=Dirichlet 1=Neumann */

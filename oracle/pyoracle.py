@@ -54,6 +54,7 @@ def lib():
         "orc_bsr_spmv": (I, [I, I, pi, pi, pd, pd, pd]),
         "orc_mg_level_dims": (I, [V, I, pi, pi]),
         "orc_sx_master": (I, [I, I, pd, pd, pd]),
+        "orc_sx_face_rule": (I, [I, pd, pd]),
         "orc_sx_problem_new": (V, [I, I, pd, I, pi, I, pi, I, pi, D, D]),
         "orc_sx_problem_free": (None, [V]), "orc_sx_sizes": (I, [V, pi]),
         "orc_sx_nodes": (I, [V, pd]), "orc_sx_faces": (I, [V, pi, pd, pi]),
@@ -348,6 +349,14 @@ def sx_master(N, p):
     nodes, mass, diff = np.zeros((n, N)), np.zeros((n, n)), np.zeros((N, n, n))
     L.orc_sx_master(N, p + 1, _pd(nodes), _pd(mass), _pd(diff))
     return nodes, mass, diff
+
+
+def sx_face_rule(N):
+    """Quadrature<N-1,10> of the simplex path: (points [nq, N-1], weights [nq])"""
+    n = lib().orc_sx_face_rule(N, None, None)
+    x, w = np.zeros((n, N - 1)), np.zeros(n)
+    lib().orc_sx_face_rule(N, _pd(x), _pd(w))
+    return x, w
 
 
 class SxProblem:

@@ -231,13 +231,47 @@ static double det_n(const double* J, int N)
     return J[0]*(J[4]*J[8] - J[5]*J[7]) - J[1]*(J[3]*J[8] - J[5]*J[6]) + J[2]*(J[3]*J[7] - J[4]*J[6]);
 }
 
+/* The reference's Quadrature<2,10> (master.h:276, 289, 302): Dunavant's
+ * degree-9 rule with 19 points on the unit triangle, weights summing to 1, as
+ * the centroid, four orbits (a,b), (b,a), (b,b) and one orbit (a,b), (c,a),
+ * (b,c), (a,c), (c,b), (b,a), with the table's 15-digit values. */
+static int triangle_rule19(double* x, double* w)
+{
+    static const double o3[4][3] = {   /* a, b, weight */
+        {2.06349616025250E-02, 4.89682519198738E-01, 0.031334700227139},
+        {1.25820817014127E-01, 4.37089591492937E-01, 0.077827541004774},
+        {6.23592928761935E-01, 1.88203535619033E-01, 0.079647738927210},
+        {9.10540973211095E-01, 4.47295133944530E-02, 0.025577675658698}};
+    const double a6 = 3.68384120547360E-02, b6 = 2.21962989160766E-01, c6 = 7.41198598784498E-01, w6 = 0.043283539377289;
+    int n = 0;
+    x[0] = 3.33333333333333E-01; x[1] = 3.33333333333333E-01; w[n++] = 0.097135796282799;
+    for (int k = 0; k < 4; ++k) {
+        double a = o3[k][0], b = o3[k][1];
+        x[2*n] = a; x[2*n+1] = b; w[n++] = o3[k][2];
+        x[2*n] = b; x[2*n+1] = a; w[n++] = o3[k][2];
+        x[2*n] = b; x[2*n+1] = b; w[n++] = o3[k][2];
+    }
+    const double p6[6][2] = {{a6, b6}, {c6, a6}, {b6, c6}, {a6, c6}, {c6, b6}, {b6, a6}};
+    for (int k = 0; k < 6; ++k) { x[2*n] = p6[k][0]; x[2*n+1] = p6[k][1]; w[n++] = w6; }
+    return n;
+}
+
 /* face quadrature on the reference (N-1)-simplex with weights summing to 1
- * (Quadrature<N-1,Q>, master.h:359-442): for triangles' faces the Q = 10
- * Gauss rule on [0,1]; 3-D faces are not restated here */
+ * (Quadrature<N-1,Q>, Q = 10, master.h:359-442): the 10-point Gauss rule on
+ * [0,1] for triangle meshes, the 19-point triangle rule for tetrahedra */
 static int face_quadrature(int N, double* x, double* w)
 {
     if (N == 2) { orc_gauss(QPTS, x, w); return QPTS; }
+    if (N == 3) return triangle_rule19(x, w);
     return -1;
+}
+
+int orc_sx_face_rule(int N, double* x, double* w)
+{
+    double xx[SX_MAXQ*2], ww[SX_MAXQ];
+    int n = face_quadrature(N, xx, ww);
+    if (n > 0) { if (x) memcpy(x, xx, sizeof(double)*n*(N - 1)); if (w) memcpy(w, ww, sizeof(double)*n); }
+    return n;
 }
 
 /* phys = p[v0] + sum_k (p[v_{k+1}] - p[v0]) ref[k]: the face simplex map

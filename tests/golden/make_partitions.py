@@ -16,7 +16,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from oracle import pyoracle as po  # noqa: E402
 
-MESHES = ["circle", "circle2", "square_4", "square_8", "square_16", "square_32"]
+MESHES = ["circle", "circle2", "square_4", "square_8", "square_16", "square_32", "ball"]
 
 
 def main():

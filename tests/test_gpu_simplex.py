@@ -311,3 +311,44 @@ def test_invalid_simplex_inputs(gctx, oracle):
     pts, tris = oracle.read_mesh(path)
     with pytest.raises(mgldg.MGLDGError):
         mgldg.Wireframe(gctx, pts, np.where(tris == 0, 10 ** 6, tris))
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_ball_3d_matches_oracle(gctx, oracle, p):
+    """Tetrahedra (data/ball.mesh): operators, RHS and the agglomerated
+    MG-PCG (MGridGen partitions, at most 8 per agglomerate, 10 x 27
+    interpolation blocks at p = 2) against the oracle."""
+    import mgldg
+    path = os.path.join(MESHES, "ball.mesh")
+    wf = mgldg.Wireframe.read(gctx, path, N=3)
+    pts, tets = oracle.read_mesh(path)
+    pr = mgldg.SimplexLDGPoisson(wf, p, tau0=100.0, tauD=1000.0)
+    opr = oracle.SxProblem(pts, tets, p, tau0=100.0, tauD=1000.0)
+    assert (pr.ne, pr.nf, pr.nbf, pr.npl, pr.nqb) == (opr.ne, opr.nf, opr.nbf, opr.b, opr.nqb)
+    for w in ["M", "T", "A", "G0", "G1", "G2"]:
+        a, b = pr.export(w), opr.export(w)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), w
+        assert rel(a[2], b[2]) < 1e-13, w
+    xq, tag = pr.bc_qpoints()
+    oq, otag = opr.bc_qpoints()
+    assert rel(xq, oq) < 1e-15 and np.array_equal(tag, otag)
+    rng = np.random.default_rng(12)
+    f, g = rng.standard_normal(pr.ndof), rng.standard_normal(pr.nqb)
+    b = opr.rhs(f, g)
+    assert rel(pr.rhs(f, g), b) < 1e-13
+    parts = partitions("ball")
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, restriction="rat", precision="mixed", partitions=parts)
+    assert mg.level_B == [pr.npl] + [(p + 1) ** 3] * len(parts)
+    omg = oracle.SxMultigrid(opr, parts, relaxation="mcgs", npre=2, npost=2, restriction="rat")
+    xo, ito, _, _ = oracle.pcg(opr, omg, b, tol=1e-10, maxit=200, criterion="true")
+    x, st = mgldg.pcg(mg, b, tol=1e-10, maxit=200, criterion="true")
+    assert st["status"] == "converged" and abs(st["iterations"] - ito) <= 1
+    assert rel(x, xo) < 1e-8
+    xm, stm = mgldg.pcg(mg, b, tol=1e-10, maxit=200, criterion="true", precision="mixed", flexible=True)
+    assert stm["status"] == "converged" and rel(xm, xo) < 1e-8
+    # the library's own agglomeration (at most 8 per agglomerate)
+    own = mgldg.agglomerate(pr)
+    assert max(np.bincount(q).max() for q in own) <= 8
+    mg2 = mgldg.Multigrid(pr, "gs", npre=2, npost=2, restriction="rat")
+    _, st2 = mgldg.pcg(mg2, b, tol=1e-10, maxit=200, criterion="true")
+    assert st2["status"] == "converged"

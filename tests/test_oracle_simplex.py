@@ -306,3 +306,78 @@ def test_agglomerated_pcg_matches_direct(relaxation):
     ud = np.linalg.solve(pr.dense("A"), b)
     assert rr <= 1e-10 and it < 60
     assert np.abs(u - ud).max() < 1e-8 * np.abs(ud).max()
+
+
+# ------------------------------------------------------------------ face rules and 3-D
+def test_face_rules_match_reference_tables():
+    """Quadrature<N-1,10> (master.h:359-442): the 10-point Gauss rule on [0,1]
+    and the 19-point triangle rule equal the reference's literal tables
+    (fixture tests/golden/simplex_quadrature.json) and integrate monomials up
+    to their degree exactly (weights normalised to the unit simplex)."""
+    with open(os.path.join(HERE, "golden", "simplex_quadrature.json")) as f:
+        rules = json.load(f)["rules"]
+    x1, w1 = po.sx_face_rule(2)
+    np.testing.assert_allclose(x1.ravel(), rules["1"]["10"]["nodes"], rtol=0, atol=2e-16)
+    np.testing.assert_allclose(w1, rules["1"]["10"]["weights"], rtol=0, atol=2e-16)
+    x2, w2 = po.sx_face_rule(3)
+    assert np.array_equal(x2.ravel(), np.array(rules["2"]["10"]["nodes"]))
+    assert np.array_equal(w2, np.array(rules["2"]["10"]["weights"]))
+    for i in range(20):
+        assert (w1 * x1[:, 0] ** i).sum() == pytest.approx(1 / (i + 1), abs=1e-14)
+    for i in range(10):
+        for j in range(10 - i):
+            exact = 2 * math.factorial(i) * math.factorial(j) / math.factorial(i + j + 2)
+            assert abs((w2 * x2[:, 0] ** i * x2[:, 1] ** j).sum() - exact) < 1e-13, (i, j)
+
+
+def test_ball_mesh_3d():
+    pts, tets = mesh("ball")
+    pr = po.SxProblem(pts, tets, 1, tau0=100, tauD=1000)
+    assert (pr.ne, pr.N) == (6327, 3)
+    assert 2 * pr.nf - pr.nbf == 4 * pr.ne
+    lr, nrm, verts = pr.faces()
+    np.testing.assert_allclose(np.linalg.norm(nrm, axis=1), 1, atol=1e-13)
+    for f in range(0, pr.nf, 7):
+        l, r = lr[f]
+        assert r < 0 or l < r
+        opp = [v for v in tets[l] if v not in verts[f]][0]
+        assert np.dot(nrm[f], pts[opp] - pts[verts[f][0]]) < 0
+    bverts = np.unique(verts[lr[:, 1] < 0])
+    np.testing.assert_allclose(np.linalg.norm(pts[bverts], axis=1), 1, atol=1e-5)
+    # lift sums: interior faces integrate 1 to the face area; boundary faces
+    # to half of it, the reference's weight w * area * |reference triangle|
+    # (mesh.h:343-344; exact for segments, half for triangles)
+    for f in list(range(0, pr.nf, 97)):
+        L, LL, RR, RL = pr.lift(f)
+        a, b, c = pts[verts[f]]
+        area = 0.5 * np.linalg.norm(np.cross(b - a, c - a))
+        if lr[f, 1] >= 0:
+            assert LL.sum() == pytest.approx(area, rel=1e-12) and RL.sum() == pytest.approx(area, rel=1e-12)
+        else:
+            assert LL.sum() == pytest.approx(area / 2, rel=1e-12)
+
+
+def test_ball_assembly_3d():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    pts, tets = mesh("ball")
+    pr = po.SxProblem(pts, tets, 1, tau0=100, tauD=1000)
+    rp, cols, blk = pr.export("A")
+    A = sp.bsr_matrix((blk, cols, rp), shape=(pr.ndof, pr.ndof)).tocsr()
+    assert abs(A - A.T).max() < 1e-12 * abs(A).max()
+    prn = po.SxProblem(pts, tets, 1, bctype=[1], tau0=100, tauD=1000)
+    rp, cols, blk = prn.export("A")
+    An = sp.bsr_matrix((blk, cols, rp), shape=(prn.ndof, prn.ndof))
+    assert np.abs(An @ np.ones(prn.ndof)).max() < 1e-10 * abs(An).max()
+    # the halved boundary terms keep the operator SPD and the iteration
+    # converging but cost the 3-D patch test its exactness
+    g = lambda X: 0.3 + X[:, 0] - 2 * X[:, 1] + 0.5 * X[:, 2]
+    xq, _ = pr.bc_qpoints()
+    b = pr.rhs(np.zeros(pr.ndof), g(xq))
+    u = spl.spsolve(A.tocsc(), b)
+    err = np.abs(u - g(pr.nodes())).max()
+    assert 1e-4 < err < 1e-2
+    mg = po.SxMultigrid(pr, partitions("ball"), relaxation="mcgs", npre=2, npost=2, restriction="rat")
+    x, it, rr, _ = po.pcg(pr, mg, b, tol=1e-10, maxit=200, criterion="true")
+    assert rr <= 1e-10 and it < 80
+    assert np.abs(x - u).max() < 1e-8 * np.abs(u).max()
