@@ -61,6 +61,18 @@ def cpu_model():
     return platform.processor()
 
 
+def plan_summary(plan):
+    """the autotuner's per-level choices: 0 = vector kernels, n = MFMA tiles of
+    16 x n columns, with the fine level's timings (ms: vector, 1, 2, 4 tiles)"""
+    out = {}
+    for prec in ("fp64", "fp32"):
+        lv = plan[prec]
+        out[prec] = {"apply": [q["apply"] for q in lv], "smooth": [q["smooth"] for q in lv],
+                     "fine_apply_ms": [round(v, 4) for v in lv[0]["apply_ms"]],
+                     "fine_smooth_ms": [round(v, 4) for v in lv[0]["smooth_ms"]]}
+    return out
+
+
 def disk_mesh(n_int=200_000, h=0.004, seed=0):
     """Unit disk: boundary points at spacing ~h on r = 1 and n_int uniform
     random points in r < 1 - h/2, Delaunay-triangulated (scipy)."""
@@ -124,13 +136,15 @@ def unstructured_leg(mgldg, torch, nrhs, warm):
             xs[(form, prec)] = x
             log(f"unstructured {form} {prec}: {dt * 1e3:.1f} ms, {prob.ndof * k / dt:.3g} DoF-solves/s, "
                 f"iterations {out[f'{form}_{prec}']['iterations']}")
+    plan = plan_summary(mg.batch_plan())
+    log(f"unstructured batched kernel plan: {plan}")
     x64, x32 = xs[("batched", "fp64")], xs[("batched", "mixed")]
     diff = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
     return {
         "workload": "unit disk, Delaunay of 200k random interior + 1571 boundary points, p=2, tau0=100, tauD=1000",
         "elements": prob.ne, "ndof": prob.ndof, "levels": mg.nlevels, "level_rows": mg.level_rows,
         "smoother": "V(2,2) block Jacobi, omega 0.6", "coarsening": "own agglomeration (<= 4 per level), RAT",
-        "nrhs": nrhs, **out,
+        "nrhs": nrhs, **out, "kernel_plan": plan,
         "mixed_batched_vs_fp64_single": round(out["batched_mixed"]["value"] / out["single_fp64"]["value"], 3),
         "mixed_vs_fp64_batched_rel_diff": diff,
         "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
@@ -276,8 +290,10 @@ def main():
             sys.exit(1)
     vb64 = prob.ndof * k * bsteps / dtb64
     vb32 = prob.ndof * k * bsteps / dtb32
+    plan = mg.batch_plan()
     log(f"batched k={k}: fp64 {dtb64 / bsteps * 1e3:.1f} ms/solve ({vb64:.3g} DoF-solves/s), "
-        f"mixed {dtb32 / bsteps * 1e3:.1f} ms ({vb32:.3g}); max iterations {max(s['iterations'] for s in stb32)}")
+        f"mixed {dtb32 / bsteps * 1e3:.1f} ms ({vb32:.3g}); max iterations {max(s['iterations'] for s in stb32)}; "
+        f"kernel plan {plan_summary(plan)}")
 
     cpu = None
     err = {}
@@ -333,7 +349,8 @@ def main():
                                         "max_iterations": max(s["iterations"] for s in stb64)},
                     "mixed": {"value": round(vb32, 1), "ms_per_step": round(dtb32 / bsteps * 1e3, 3),
                               "max_iterations": max(s["iterations"] for s in stb32)},
-                    "mixed_batched_vs_fp64_single": round(vb32 / v64, 3)},
+                    "mixed_batched_vs_fp64_single": round(vb32 / v64, 3),
+                    "kernel_plan": plan_summary(plan)},
         "error_vs_fp64_oracle": err,
         "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
         "roofline": roofline,

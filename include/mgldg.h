@@ -192,6 +192,18 @@ int mgldg_mg_create_agglomerated(mgldg_problem pr, const mgldg_mg_params* prm, i
 int mgldg_mg_info(mgldg_mg mg, long* info, int ninfo);
 /* block rows, block size and stored blocks of level l */
 int mgldg_mg_level(mgldg_mg mg, int level, int* nb, int* B, long* nnzb);
+/* No reference counterpart.  Kernel family of the batched (nrhs > 1) block
+ * applications and smoothers: mode -1 (default) = picked per level by the
+ * library's autotuner when a batch size is first used; 0 = vector kernels;
+ * 1, 2, 4 = MFMA tiles of 16, 32, 64 columns on levels whose block size has
+ * them (6, 8, 9, 10, 16).  Takes effect at the next batch-size change. */
+int mgldg_mg_batch_kernels(mgldg_mg mg, int mode);
+/* the choices in use for the current batch: plan[2*l] (operator
+ * application) and plan[2*l+1] (smoother) per level, FP64 then FP32 (levels
+ * entries each), 0 = vector kernels, nt = MFMA tiles of 16 x nt columns;
+ * ms (may be NULL): the autotuner's timings, 4 candidates (vector, 1, 2, 4
+ * tiles; 0 = not tried) x 2 operations per level and precision */
+int mgldg_mg_batch_plan(mgldg_mg mg, int* plan, float* ms);
 /* Export a level operator (copied back from the GPU): which 0=A 1=M 2=T 3+d=G_d
  * (RATRAT levels only), 6 = interpolation T_l (rows level l, cols level l+1).
  * Exported patterns are the host symbolic phase; values come from the GPU. */

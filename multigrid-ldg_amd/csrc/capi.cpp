@@ -360,6 +360,33 @@ int mgldg_mg_level(mgldg_mg mg, int level, int* nb, int* B, long* nnzb)
     return MGLDG_OK;
 }
 
+int mgldg_mg_batch_kernels(mgldg_mg mg, int mode)
+{
+    if (!mg || !(mode == -1 || mode == 0 || mode == 1 || mode == 2 || mode == 4)) return MGLDG_ERR_ARG;
+    mg->m.bforce = mode;
+    mg->m.bk = 0;          // re-plan at the next batched call
+    return MGLDG_OK;
+}
+
+int mgldg_mg_batch_plan(mgldg_mg mg, int* plan, float* ms)
+{
+    if (!mg || !plan) return MGLDG_ERR_ARG;
+    const MGH& m = mg->m;
+    const int nl = (int)m.L.size();
+    for (int p = 0; p < 2; ++p)
+        for (int l = 0; l < nl; ++l) {
+            const bool has = (int)m.btune[p].size() > l;
+            plan[(p * nl + l) * 2] = has ? m.btune[p][l].apply : 0;
+            plan[(p * nl + l) * 2 + 1] = has ? m.btune[p][l].smooth : 0;
+            if (ms)
+                for (int c = 0; c < 4; ++c) {
+                    ms[((p * nl + l) * 2) * 4 + c] = has ? m.btune[p][l].apply_ms[c] : 0.f;
+                    ms[((p * nl + l) * 2 + 1) * 4 + c] = has ? m.btune[p][l].smooth_ms[c] : 0.f;
+                }
+        }
+    return MGLDG_OK;
+}
+
 int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols, double* blocks, long* nnzb,
                     int* nbrows, int* nbcols)
 {

@@ -72,6 +72,20 @@ struct MOps {
     // per column: b -= mean(b) (optional) then x = Cinv b (Cinv FP64 n x n); Cinv null: only the mean
     static void coarse(hipStream_t s, int n, int k, const double* Cinv, T* b, T* x, int subtract_mean);
 };
+// The same operators on the matrix cores (mfma.hip): one wavefront per block
+// row and nt tiles of 16 columns, blocks padded to 16 x 16 in registers.
+// Compiled for B in {6, 8, 9, 10, 16} and nt in {1, 2, 4}.
+bool mf_supported(int B);
+template <typename T>
+struct MF {
+    static void apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
+                      const T* x, const T* b, T* y);
+    static void gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* rows, const int* rowptr,
+                         const int* cols, const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color,
+                         int cur, int skip);
+    static void jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
+                       const T* Dinv, const T* b, const T* x, T* xout, T omega);
+};
 // Y = W X for k columns ([n][k] layout), W FP64 n x n row-major
 template <typename T>
 void gemm_f64(hipStream_t s, int n, int k, const double* W, const T* x, T* y);

@@ -1,0 +1,307 @@
+// mfma.hip — k right-hand sides on the matrix cores.
+//
+// For a block row i the batched operators are small dense products
+//   Y_i = sum_j A_ij X_j     (A_ij: B x B, X_j: B x k, k columns contiguous)
+// One wavefront owns one block row and NT tiles of 16 columns.  Each block is
+// a 16 x 16 x 4-step MFMA product (v_mfma_f32_16x16x4f32 / _f64_16x16x4f64),
+// blocks of 6..16 rows padded with zeros in registers: lane l supplies
+// A[l % 16][4s + l / 16] and X[4s + l / 16][l % 16] for K step s.  The
+// accumulator lane layout is the instruction's (f32: register i of lane l
+// holds row 4 (l / 16) + i; f64: row 4 i + l / 16; column l % 16).
+// The smoothers write the residual tile to LDS and read it back in the
+// B-operand layout for the D^-1 product.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "kernels.hpp"
+
+namespace mgldg {
+namespace dev {
+
+namespace {
+
+__host__ __device__ constexpr int pv(int tsize, int B)
+{
+    return tsize == 8 ? (B % 2 == 0 ? 2 : 1) : (B % 4 == 0 ? 4 : (B % 2 == 0 ? 2 : 1));
+}
+
+// element (r, c) of a block in the solve (panel) layout
+template <typename T, int B>
+__device__ __forceinline__ T panel_at(const T* blk, int r, int c)
+{
+    constexpr int V = pv(sizeof(T), B);
+    return blk[((c / V) * B + r) * V + (c % V)];
+}
+
+template <typename T> struct Mf;
+template <> struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+template <> struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
+};
+
+// acc[t] += A_blk X_j (rows padded to 16, K padded to a multiple of 4)
+template <typename T, int B, int NT>
+__device__ __forceinline__ void block_mac(typename Mf<T>::v4* acc, const T* __restrict__ blk, const T* __restrict__ xj,
+                                          int k, int col0, int lane)
+{
+    const int r16 = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < (B + 3) / 4; ++s) {
+        const int c = 4 * s + q;
+        const T a = (r16 < B && c < B) ? panel_at<T, B>(blk, r16, c) : (T)0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = col0 + 16 * t + r16;
+            const T xb = (c < B && col < k) ? xj[(size_t)c * k + col] : (T)0;
+            acc[t] = Mf<T>::mac(a, xb, acc[t]);
+        }
+    }
+}
+
+// y = A x (b null) or y = b - A x
+template <typename T, int B, int NT>
+__global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
+                                                const int* __restrict__ cols, const T* __restrict__ A,
+                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y)
+{
+    using V4 = typename Mf<T>::v4;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int i = w / ngroups, g = w - i * ngroups;
+    if (i >= nrows) return;
+    const int col0 = g * 16 * NT;
+    V4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+    const int k1 = rowptr[i + 1];
+    for (int kk = rowptr[i]; kk < k1; ++kk)
+        block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)cols[kk] * B * k, k, col0, lane);
+    const int col_l = lane & 15;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int col = col0 + 16 * t + col_l;
+        if (col >= k) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = Mf<T>::row(lane, e);
+            if (r >= B) continue;
+            const size_t o = ((size_t)i * B + r) * k + col;
+            y[o] = b ? (T)(b[o] - acc[t][e]) : acc[t][e];
+        }
+    }
+}
+
+// the D^-1 product of a smoother: r (acc layout, 16 x 16 NT) -> LDS -> D_i r
+template <typename T, int B, int NT>
+__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* rt, const T* __restrict__ D,
+                                           T* lds, int lane)
+{
+    using V4 = typename Mf<T>::v4;
+    const int col_l = lane & 15;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) lds[Mf<T>::row(lane, e) * (16 * NT) + 16 * t + col_l] = rt[t][e];
+    __syncthreads();
+    const int r16 = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) out[t] = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < (B + 3) / 4; ++s) {
+        const int c = 4 * s + q;
+        const T a = (r16 < B && c < B) ? panel_at<T, B>(D, r16, c) : (T)0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const T xb = (c < B) ? lds[c * (16 * NT) + 16 * t + r16] : (T)0;
+            out[t] = Mf<T>::mac(a, xb, out[t]);
+        }
+    }
+}
+
+// damped block GS on the rows of one colour (multigrid.h:240-249):
+// x_i += omega (D_i^-1 (b_i - sum_{j != i} A_ij x_j) - x_i); skip as in the
+// vector kernel (first sweep from zero: later colours are not read)
+template <typename T, int B, int NT>
+__global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, const int* __restrict__ rows,
+                                             const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                             const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                             T* x, T omega, const int* __restrict__ color, int cur, int skip)
+{
+    using V4 = typename Mf<T>::v4;
+    __shared__ T lds_all[4 * 16 * 16 * NT];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    T* lds = lds_all + wv * 16 * 16 * NT;
+    const int w = blockIdx.x * 4 + wv;
+    const int ri = w / ngroups, g = w - ri * ngroups;
+    const bool valid = ri < nrows;          // every wave reaches the barrier in dinv_times
+    const int i = valid ? rows[ri] : 0;
+    const int col0 = g * 16 * NT, col_l = lane & 15;
+    V4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+    if (valid) {
+        const int k1 = rowptr[i + 1];
+        for (int kk = rowptr[i]; kk < k1; ++kk) {
+            const int j = cols[kk];
+            if (j == i) continue;
+            if (skip) { const int cj = color[j]; if (skip == 1 ? cj > cur : cj < cur) continue; }
+            block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)j * B * k, k, col0, lane);
+        }
+    }
+    V4 rt[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int col = col0 + 16 * t + col_l;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = Mf<T>::row(lane, e);
+            rt[t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[t][e]) : (T)0;
+        }
+    }
+    V4 out[NT];
+    dinv_times<T, B, NT>(out, rt, Dinv + (size_t)i * B * B, lds, lane);
+    if (!valid) return;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int col = col0 + 16 * t + col_l;
+        if (col >= k) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = Mf<T>::row(lane, e);
+            if (r >= B) continue;
+            const size_t o = ((size_t)i * B + r) * k + col;
+            const T xi = skip ? (T)0 : x[o];
+            x[o] = xi + omega * (out[t][e] - xi);
+        }
+    }
+}
+
+// damped block Jacobi (multigrid.h:223-229): xout = x + omega D^-1 (b - A x)
+template <typename T, int B, int NT>
+__global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
+                                                 const int* __restrict__ cols, const T* __restrict__ A,
+                                                 const T* __restrict__ Dinv, const T* __restrict__ b,
+                                                 const T* __restrict__ x, T* __restrict__ xout, T omega)
+{
+    using V4 = typename Mf<T>::v4;
+    __shared__ T lds_all[4 * 16 * 16 * NT];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    T* lds = lds_all + wv * 16 * 16 * NT;
+    const int w = blockIdx.x * 4 + wv;
+    const int i0 = w / ngroups, g = w - i0 * ngroups;
+    const bool valid = i0 < nrows;
+    const int i = valid ? i0 : 0;
+    const int col0 = g * 16 * NT, col_l = lane & 15;
+    V4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+    if (valid) {
+        const int k1 = rowptr[i + 1];
+        for (int kk = rowptr[i]; kk < k1; ++kk)
+            block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)cols[kk] * B * k, k, col0, lane);
+    }
+    V4 rt[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int col = col0 + 16 * t + col_l;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = Mf<T>::row(lane, e);
+            rt[t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[t][e]) : (T)0;
+        }
+    }
+    V4 out[NT];
+    dinv_times<T, B, NT>(out, rt, Dinv + (size_t)i * B * B, lds, lane);
+    if (!valid) return;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int col = col0 + 16 * t + col_l;
+        if (col >= k) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = Mf<T>::row(lane, e);
+            if (r >= B) continue;
+            const size_t o = ((size_t)i * B + r) * k + col;
+            xout[o] = x[o] + omega * out[t][e];
+        }
+    }
+}
+
+inline unsigned nblocks(long waves) { return (unsigned)((waves + 3) / 4); }
+
+void check(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("kernel launch failed: ") + what + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+#define MGLDG_MF_DISPATCH(B, NT, ...)                                                          \
+    switch ((B) * 8 + (NT)) {                                                                  \
+        case 6 * 8 + 1: { constexpr int BB = 6, TT = 1; __VA_ARGS__; } break;                  \
+        case 6 * 8 + 2: { constexpr int BB = 6, TT = 2; __VA_ARGS__; } break;                  \
+        case 6 * 8 + 4: { constexpr int BB = 6, TT = 4; __VA_ARGS__; } break;                  \
+        case 8 * 8 + 1: { constexpr int BB = 8, TT = 1; __VA_ARGS__; } break;                  \
+        case 8 * 8 + 2: { constexpr int BB = 8, TT = 2; __VA_ARGS__; } break;                  \
+        case 8 * 8 + 4: { constexpr int BB = 8, TT = 4; __VA_ARGS__; } break;                  \
+        case 9 * 8 + 1: { constexpr int BB = 9, TT = 1; __VA_ARGS__; } break;                  \
+        case 9 * 8 + 2: { constexpr int BB = 9, TT = 2; __VA_ARGS__; } break;                  \
+        case 9 * 8 + 4: { constexpr int BB = 9, TT = 4; __VA_ARGS__; } break;                  \
+        case 10 * 8 + 1: { constexpr int BB = 10, TT = 1; __VA_ARGS__; } break;                \
+        case 10 * 8 + 2: { constexpr int BB = 10, TT = 2; __VA_ARGS__; } break;                \
+        case 10 * 8 + 4: { constexpr int BB = 10, TT = 4; __VA_ARGS__; } break;                \
+        case 16 * 8 + 1: { constexpr int BB = 16, TT = 1; __VA_ARGS__; } break;                \
+        case 16 * 8 + 2: { constexpr int BB = 16, TT = 2; __VA_ARGS__; } break;                \
+        case 16 * 8 + 4: { constexpr int BB = 16, TT = 4; __VA_ARGS__; } break;                \
+        default: throw std::runtime_error("MFMA block kernels: block size " + std::to_string(B) + " / tiles " + std::to_string(NT) + " not compiled"); \
+    }
+
+bool mf_supported(int B) { return B == 6 || B == 8 || B == 9 || B == 10 || B == 16; }
+
+template <typename T>
+void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
+                  const T* b, T* y)
+{
+    if (nrows <= 0 || k <= 0) return;
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
+                                                nrows, ng, rowptr, cols, A, x, b, y));
+    check("mf_apply");
+}
+
+template <typename T>
+void MF<T>::gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* rows, const int* rowptr, const int* cols,
+                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
+{
+    if (nrows <= 0 || k <= 0) return;
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
+                                                nrows, ng, rows, rowptr, cols, A, Dinv, b, x, omega, color, cur, skip));
+    check("mf_gs");
+}
+
+template <typename T>
+void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
+                   const T* Dinv, const T* b, const T* x, T* xout, T omega)
+{
+    if (nrows <= 0 || k <= 0) return;
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_jacobi<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
+                                                nrows, ng, rowptr, cols, A, Dinv, b, x, xout, omega));
+    check("mf_jacobi");
+}
+
+template struct MF<float>;
+template struct MF<double>;
+
+}  // namespace dev
+}  // namespace mgldg

@@ -889,6 +889,76 @@ void MGH::ensure_batch(int k, int precision)
     Bpart.alloc((size_t)1024 * 3 * k + 64);
     Bactive.alloc(k);
     bk = k; bprec = precision;
+    tune_batch(k, precision);
+}
+
+template <typename T>
+void MGH::bapply(int l, int k, const T* x, const T* b, T* y)
+{
+    Level& lv = L[l];
+    const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].apply : 0;
+    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), x, b, y);
+    else dev::MOps<T>::apply(ctx->stream, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), x, b, y);
+}
+
+// Autotuner of the batched block kernels: on every level and precision in
+// use, time one operator application and one smoother sweep with the vector
+// kernels and with MFMA tiles of 16, 32 and 64 columns (those that fit k),
+// on the level's own operators and batch buffers, and keep the fastest.
+// bforce >= 0 fixes the choice instead (0: vector kernels).
+void MGH::tune_batch(int k, int precision)
+{
+    hipStream_t s = ctx->stream;
+    const int nl = (int)L.size();
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    auto timed = [&](auto&& fn) {
+        fn();                                    // warm-up
+        HIPCHK(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) fn();
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        return ms / 3;
+    };
+    const int cand[4] = {0, 1, 2, 4};
+    for (int p = 0; p <= precision; ++p) {
+        btune[p].assign(nl, BTune());
+        for (int l = 0; l < nl; ++l) {
+            Level& lv = L[l];
+            BTune& t = btune[p][l];
+            if (!dev::mf_supported(lv.B)) continue;
+            if (bforce >= 0) { t.apply = t.smooth = bforce; continue; }
+            const long m = (long)lv.nb * lv.B * k;
+            auto run = [&](auto tag) {
+                using T = decltype(tag);
+                BatchLevel& v = BL[l];
+                dev::Ops<T>::fill(s, m, (T)1, bx<T>(v));
+                dev::Ops<T>::fill(s, m, (T)1, bb<T>(v));
+                float best_a = 1e30f, best_s = 1e30f;
+                for (int ci = 0; ci < 4; ++ci) {
+                    const int nt = cand[ci];
+                    if (nt > 1 && 16 * nt / 2 >= k) continue;      // more tiles than columns
+                    t.apply = nt; t.smooth = nt;
+                    const float ma = timed([&] { bapply<T>(l, k, bx<T>(v), bb<T>(v), br<T>(v)); });
+                    const float msm = timed([&] { brelax<T>(l, k, 1, false, false); });
+                    t.apply_ms[ci] = ma; t.smooth_ms[ci] = msm;
+                    if (ma < best_a) { best_a = ma; t.apply = nt; }
+                    if (msm < best_s) { best_s = msm; t.smooth = nt; }
+                }
+                int ba = 0, bs = 0;
+                for (int ci = 0; ci < 4; ++ci) {
+                    if (t.apply_ms[ci] > 0 && t.apply_ms[ci] <= best_a) ba = cand[ci];
+                    if (t.smooth_ms[ci] > 0 && t.smooth_ms[ci] <= best_s) bs = cand[ci];
+                }
+                t.apply = ba; t.smooth = bs;
+            };
+            if (p == 0) run(double());
+            else run(float());
+        }
+    }
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
 }
 
 template <typename T>
@@ -898,18 +968,23 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     BatchLevel& v = BL[l];
     hipStream_t s = ctx->stream;
     const T omega = (T)prm.omega;
+    const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].smooth : 0;
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
             sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
-                dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), bx<T>(v),
-                                       omega, cdev, c, skip);
+                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
+                                             bx<T>(v), omega, cdev, c, skip);
+                else dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
+                                            bx<T>(v), omega, cdev, c, skip);
             });
     } else {
         const long m = (long)lv.nb * lv.B * k;
         if (zero_guess) dev::Ops<T>::fill(s, m, (T)0, bx<T>(v));
         T* cur = bx<T>(v); T* nxt = balt<T>(v);
         for (int sw = 0; sw < n; ++sw) {
-            dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
+            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt,
+                                       omega);
+            else dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
             std::swap(cur, nxt);
         }
         if (cur != bx<T>(v)) dev::Ops<T>::copy(s, m, cur, bx<T>(v));
@@ -927,7 +1002,7 @@ void MGH::bvcycle(int k)
         Level& lv = L[l];
         brelax<T>(l, k, prm.npre, false, true);
         if (prm.npre == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B * k, (T)0, bx<T>(BL[l]));
-        dev::MOps<T>::apply(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), bx<T>(BL[l]), bb<T>(BL[l]), br<T>(BL[l]));
+        bapply<T>(l, k, bx<T>(BL[l]), bb<T>(BL[l]), br<T>(BL[l]));
         dev::MOps<T>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
     }
     Level& c = L[bottom];
@@ -984,7 +1059,6 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
 {
     ensure_batch(k, sp.precision);
     hipStream_t s = ctx->stream;
-    Level& f = L[0];
     const long n = pr->ndof;
     const int NS = dev::S_COUNT;
     std::vector<double> Sh((size_t)k * NS);
@@ -998,7 +1072,7 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
         HIPCHK(hipStreamSynchronize(s));
     };
     auto push_active = [&] { HIPCHK(hipMemcpyAsync(Bactive.p, act.data(), k * sizeof(int), hipMemcpyHostToDevice, s)); };
-    auto residual = [&] { dev::MOps<double>::apply(s, f.B, k, f.nb, f.A.rowptr, f.A.cols, f.A64, x, b, Br); };
+    auto residual = [&] { bapply<double>(0, k, x, b, Br); };
     HIPCHK(hipMemsetAsync(BS.p, 0, BS.n * sizeof(double), s));
     push_active();
     dev::coldot(s, n, k, b, b, BS.p + dev::S_BB, NS, Bpart);
@@ -1055,7 +1129,7 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
         }
         if (!any) break;
         if (i >= sp.maxit) { for (int c = 0; c < k; ++c) if (act[c]) st[c].status = 1; break; }
-        dev::MOps<double>::apply(s, f.B, k, f.nb, f.A.rowptr, f.A.cols, f.A64, Bd, nullptr, Bq);
+        bapply<double>(0, k, Bd, nullptr, Bq);
         dev::coldot(s, n, k, Bd, Bq, BS.p + dev::S_DQ, NS, Bpart);
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
             dev::bpcg_x(s, n, k, BS, Bactive, Bd, x);

@@ -167,6 +167,13 @@ struct MGH {
     void build(ProblemH* p, const MGParams& prm, const std::vector<std::vector<int>>* parts = nullptr);
     void build_collapse();
     void ensure_batch(int k, int precision);
+    // kernel family of each batched operator: 0 = vector kernels (MOps),
+    // nt > 0 = MFMA tiles of 16 x nt columns; per precision (0 FP64, 1 FP32) and level
+    struct BTune { int apply = 0, smooth = 0; float apply_ms[4] = {0, 0, 0, 0}, smooth_ms[4] = {0, 0, 0, 0}; };
+    std::vector<BTune> btune[2];
+    int bforce = -1;                   // -1: autotuned; 0: vector kernels; 1, 2, 4: MFMA tiles where compiled
+    void tune_batch(int k, int precision);
+    template <typename T> void bapply(int l, int k, const T* x, const T* b, T* y);
     template <typename T> void brelax(int l, int k, int n, bool reverse, bool zero_guess);
     template <typename T> void bvcycle(int k);
     void bprecon(int precision, int k, bool use_graph);     // Bz = V(Br) on k columns

@@ -305,6 +305,27 @@ class Multigrid(_Handle):
         check(self.ctx, lib().mgldg_profile_spmv(self.h, level, _prec(precision), nrep, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def batch_kernels(self, mode="auto"):
+        """kernel family of batched solves: 'auto' (autotuned per level), 'vector',
+        or the number of 16-column MFMA tiles (1, 2, 4)"""
+        m = {"auto": -1, "vector": 0}.get(mode, mode)
+        check(self.ctx, lib().mgldg_mg_batch_kernels(self.h, int(m)))
+
+    def batch_plan(self):
+        """per precision ('fp64', 'fp32') and level: (apply, smoother) kernel
+        choice (0 vector, nt MFMA tiles) and the autotuner's timings in ms"""
+        nl = self.nlevels
+        plan = np.zeros(4 * nl, dtype=np.int32)
+        ms = np.zeros(32 * nl, dtype=np.float32)
+        check(self.ctx, lib().mgldg_mg_batch_plan(self.h, _pi(plan), ms.ctypes.data_as(C.POINTER(C.c_float))))
+        out = {}
+        for p, name in enumerate(("fp64", "fp32")):
+            out[name] = [dict(apply=int(plan[(p * nl + l) * 2]), smooth=int(plan[(p * nl + l) * 2 + 1]),
+                              apply_ms=ms[((p * nl + l) * 2) * 4:((p * nl + l) * 2) * 4 + 4].tolist(),
+                              smooth_ms=ms[((p * nl + l) * 2 + 1) * 4:((p * nl + l) * 2 + 1) * 4 + 4].tolist())
+                         for l in range(nl)]
+        return out
+
     def setup_times(self):
         t = np.zeros(4)
         check(self.ctx, lib().mgldg_setup_times(self.h, _pd(t)))

@@ -55,6 +55,8 @@ SIGNATURES = {
     "mgldg_mg_create": (I, [V, C.POINTER(MGParams), C.POINTER(V)]),
     "mgldg_mg_info": (I, [V, PL, I]),
     "mgldg_mg_level": (I, [V, I, PI, PI, PL]),
+    "mgldg_mg_batch_kernels": (I, [V, I]),
+    "mgldg_mg_batch_plan": (I, [V, PI, C.POINTER(C.c_float)]),
     "mgldg_agglomerate": (I, [V, I, PI, PI, PI]),
     "mgldg_mg_create_agglomerated": (I, [V, C.POINTER(MGParams), I, PI, C.POINTER(V)]),
     "mgldg_mg_export": (I, [V, I, I, PI, PI, PD, PL, PI, PI]),

@@ -102,3 +102,47 @@ def test_autograd_gradients_against_finite_differences(gctx, oracle):
                 fd = (loss(f + eps * df, g + eps * dg) - loss(f - eps * df, g - eps * dg)) / (2 * eps)
             an = torch.dot(f.grad, df) + torch.dot(g.grad, dg)
             assert abs(fd.item() - an.item()) <= bound * abs(an.item()), (eps, fd.item(), an.item())
+
+
+@pytest.mark.parametrize("case", ["quadtree_p3", "quadtree_p2", "simplex_p2"])
+def test_mfma_batched_kernels(gctx, oracle, case):
+    """The MFMA block kernels (forced 1, 2, 4 tiles of 16 columns, and the
+    autotuned plan) give the vector kernels' batched solves: block sizes 16
+    (p = 3 quadtree), 9 (p = 2 quadtree) and 6 / 9 (p = 2 triangles with box
+    coarse levels), GS and Jacobi, FP64 and FP32 V-cycles."""
+    import os
+    import mgldg
+    import torch
+    if case.startswith("quadtree"):
+        p = 3 if case.endswith("p3") else 2
+        qt, pr, _, opr = build_pair(gctx, oracle, "adaptive", p, rapid=True)
+        relax = "gs"
+        mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, precision="mixed", coarse_dense_max=0)
+    else:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", "circle2.mesh")
+        wf = mgldg.Wireframe.read(gctx, path)
+        pr = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
+        relax = "jacobi"
+        mg = mgldg.Multigrid(pr, relax, omega=0.6, npre=2, npost=2, restriction="rat", precision="mixed",
+                             coarse_dense_max=0)
+    rng = np.random.default_rng(31)
+    k = 40                                         # a partial last tile
+    B = torch.tensor(pr.rhs(rng.standard_normal((pr.ndof, k))), device="cuda")
+    res = {}
+    for mode in ("vector", 1, 2, 4, "auto"):
+        mg.batch_kernels(mode)
+        for prec in ("fp64", "mixed"):
+            X = torch.zeros_like(B)
+            _, st = mgldg.pcg(mg, B, X, tol=1e-10, maxit=300, criterion="true", precision=prec,
+                              flexible=prec == "mixed")
+            assert all(s["status"] == "converged" for s in st), (mode, prec)
+            res[(mode, prec)] = (X.cpu().numpy(), [s["iterations"] for s in st])
+        if mode == "auto":
+            plan = mg.batch_plan()
+            assert any(lv["apply"] > 0 or lv["smooth"] > 0 for lv in plan["fp32"]) or pr.npl not in (6, 8, 9, 10, 16)
+    for prec in ("fp64", "mixed"):
+        X0, it0 = res[("vector", prec)]
+        for mode in (1, 2, 4, "auto"):
+            X1, it1 = res[(mode, prec)]
+            assert max(abs(a - b) for a, b in zip(it0, it1)) <= 1, (mode, prec)
+            assert rel(X1, X0) < 1e-8, (mode, prec)
