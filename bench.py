@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides of the batched form")
     ap.add_argument("--solve-only", action="store_true", help="setup + three mixed solves (profiling)")
     ap.add_argument("--no-unstructured", action="store_true", help="skip the unit-disk workload")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="setup + two mixed and one FP64 batched solves of --nrhs columns (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -230,6 +232,16 @@ def main():
                               flexible=True, use_graph=True)
         torch.cuda.synchronize()
         print(json.dumps({"solve": st}), flush=True)
+        return
+
+    if args.batched_only:
+        # profiling mode: batched solves (autotuned block kernels, V-cycle from its hipGraph)
+        Bp = torch.tensor(prob.rhs(rng.standard_normal((prob.ndof, args.nrhs))), device=dev)
+        for prec in ("mixed", "mixed", "fp64"):
+            _, st = mgldg.pcg(mg, Bp, torch.zeros_like(Bp), tol=1e-10, maxit=500, criterion="true", precision=prec,
+                              flexible=prec == "mixed", use_graph=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"batched": st[0], "kernel_plan": plan_summary(mg.batch_plan())}), flush=True)
         return
 
     def run(precision, steps, warm):

@@ -1043,24 +1043,34 @@ __global__ __launch_bounds__(256) void k_coldot(long n, int k, const double* __r
     col_partials(k, 1, &acc, part);
 }
 
-// sums per-column partials; mode 0: out[c * ostride] = sum ; mode 1/2: PCG r.z update
-__global__ void k_colfinalize(const double* __restrict__ part, int nb, int k, int nval, double* out, int ostride,
-                              double* S, const int* active, int mode)
+// sums per-column partials, one workgroup per column (fixed-order strided
+// sums and a fixed tree: deterministic); mode 0: out[c * ostride] = sum ;
+// mode 1/2: PCG r.z update
+__global__ __launch_bounds__(256) void k_colfinalize(const double* __restrict__ part, int nb, int k, int nval, double* out,
+                                                     int ostride, double* S, const int* active, int mode)
 {
-    const int c = threadIdx.x;
-    if (c >= k) return;
+    __shared__ double red[3][256];
+    const int c = blockIdx.x, tid = threadIdx.x;
     double t[3] = {0, 0, 0};
-    for (int b = 0; b < nb; ++b)
+    for (int b = tid; b < nb; b += 256)
         for (int v = 0; v < nval; ++v) t[v] += part[((size_t)b * nval + v) * k + c];
-    if (mode == 0) { out[(size_t)c * ostride] = t[0]; return; }
+    for (int v = 0; v < nval; ++v) red[v][tid] = t[v];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int v = 0; v < nval; ++v) red[v][tid] += red[v][tid + w];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    if (mode == 0) { out[(size_t)c * ostride] = red[0][0]; return; }
     double* Sc = S + (size_t)c * S_COUNT;
     if (active && !active[c]) return;
     const double old = Sc[S_DELTA];
     Sc[S_DELTA_OLD] = old;
-    Sc[S_DELTA] = t[0];
-    Sc[S_RSOLD] = t[1];
-    Sc[S_RR] = t[2];
-    Sc[S_BETA] = (mode == 2) ? (t[0] - t[1]) / old : t[0] / old;
+    Sc[S_DELTA] = red[0][0];
+    Sc[S_RSOLD] = red[1][0];
+    Sc[S_RR] = red[2][0];
+    Sc[S_BETA] = (mode == 2) ? (red[0][0] - red[1][0]) / old : red[0][0] / old;
 }
 
 static unsigned red_grid(long n, int k)
@@ -1075,7 +1085,7 @@ void coldot(hipStream_t s, long n, int k, const double* a, const double* b, doub
     unsigned g = red_grid(n, k);
     hipLaunchKernelGGL(k_coldot, dim3(g), dim3(256), 0, s, n, k, a, b, part);
     check_launch("coldot");
-    hipLaunchKernelGGL(k_colfinalize, dim3(1), dim3(256), 0, s, part, (int)g, k, 1, out, ostride, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 1, out, ostride, nullptr, nullptr, 0);
     check_launch("colfinalize");
 }
 
@@ -1103,7 +1113,7 @@ void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const d
     unsigned g = red_grid(n, k);
     hipLaunchKernelGGL(k_bpcg_xr, dim3(g), dim3(256), 0, s, n, k, S, active, d, q, x, r, part);
     check_launch("bpcg_xr");
-    hipLaunchKernelGGL(k_colfinalize, dim3(1), dim3(256), 0, s, part, (int)g, k, 1, S + S_RR, (int)S_COUNT, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 1, S + S_RR, (int)S_COUNT, nullptr, nullptr, 0);
     check_launch("colfinalize");
 }
 
@@ -1143,7 +1153,7 @@ void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const d
     unsigned g = red_grid(n, k);
     hipLaunchKernelGGL(k_bpcg_rz, dim3(g), dim3(256), 0, s, n, k, r, z, zold, part);
     check_launch("bpcg_rz");
-    hipLaunchKernelGGL(k_colfinalize, dim3(1), dim3(256), 0, s, part, (int)g, k, 3, nullptr, 0, S, active, zold ? 2 : 1);
+    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 3, nullptr, 0, S, active, zold ? 2 : 1);
     check_launch("colfinalize");
 }
 

@@ -86,6 +86,15 @@ struct MF {
     static void jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);
 };
+// Y = W X on the matrix cores (FP64 16x16x4) for k columns: W in the 16 x 4
+// tile layout made by gemm_tiles (gemm_tiles_size(n) doubles), the K range
+// split over slices summed in order (part: gemm_part_size(n, k) doubles)
+int gemm_tiles_size(int n);
+void gemm_tiles(hipStream_t s, int n, const double* W, double* Wt);
+int gemm_slices(int n, int k);
+long gemm_part_size(int n, int k);
+template <typename T>
+void gemm_mf(hipStream_t s, int n, int k, const double* Wt, const T* x, T* y, double* part);
 // Y = W X for k columns ([n][k] layout), W FP64 n x n row-major
 template <typename T>
 void gemm_f64(hipStream_t s, int n, int k, const double* W, const T* x, T* y);

@@ -235,6 +235,62 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     }
 }
 
+// W (n x n, row-major FP64) -> 16 x 4 tiles in the A-operand lane order:
+// Wt[(R * nK + K) * 64 + l] = W[16 R + l % 16][4 K + l / 16], zero-padded
+__global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, double* __restrict__ Wt)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nR * nK * 64) return;
+    const int l = (int)(e % 64);
+    const long t = e / 64;
+    const int K = (int)(t % nK), R = (int)(t / nK);
+    const int r = 16 * R + (l & 15), c = 4 * K + (l >> 4);
+    Wt[e] = (r < n && c < n) ? W[(size_t)r * n + c] : 0.0;
+}
+
+// partial products of Y = W X over K slices: one wavefront per (16 x 16 output
+// tile, K slice); part[(slice * nRpad + row) * k + col]
+template <typename T>
+__global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int nslices, int kpers,
+                                               const double* __restrict__ Wt, const T* __restrict__ x,
+                                               double* __restrict__ part)
+{
+    using V4 = Mf<double>::v4;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nC = (k + 15) / 16;
+    const int tile = w / nslices, sl = w - tile * nslices;
+    if (tile >= nR * nC) return;
+    const int R = tile / nC, Cc = tile - R * nC;
+    const int col = 16 * Cc + (lane & 15), q = lane >> 4;
+    const int K0 = sl * kpers, K1 = min(nK, K0 + kpers);
+    V4 acc = V4{0, 0, 0, 0};
+    const double* wt = Wt + (size_t)R * nK * 64 + lane;
+    for (int K = K0; K < K1; ++K) {
+        const int c = 4 * K + q;
+        const double a = wt[(size_t)K * 64];
+        const double b = (c < n && col < k) ? (double)x[(size_t)c * k + col] : 0.0;
+        acc = Mf<double>::mac(a, b, acc);
+    }
+    if (col >= k) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int r = 16 * R + Mf<double>::row(lane, e);
+        part[((size_t)sl * (16 * nR) + r) * k + col] = acc[e];
+    }
+}
+
+// y = sum over slices, in slice order
+template <typename T>
+__global__ void k_gemm_sum(int n, int k, int nR, int nslices, const double* __restrict__ part, T* __restrict__ y)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)n * k) return;
+    double s = 0;
+    for (int sl = 0; sl < nslices; ++sl) s += part[(size_t)sl * (16 * nR) * k + e];
+    y[e] = (T)s;
+}
+
 inline unsigned nblocks(long waves) { return (unsigned)((waves + 3) / 4); }
 
 void check(const char* what)
@@ -299,6 +355,44 @@ void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* ro
                                                 nrows, ng, rowptr, cols, A, Dinv, b, x, xout, omega));
     check("mf_jacobi");
 }
+
+int gemm_tiles_size(int n) { return ((n + 15) / 16) * ((n + 3) / 4) * 64; }
+
+void gemm_tiles(hipStream_t s, int n, const double* W, double* Wt)
+{
+    const int nR = (n + 15) / 16, nK = (n + 3) / 4;
+    const long t = (long)nR * nK * 64;
+    if (t <= 0) return;
+    hipLaunchKernelGGL(k_tile_w, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, nR, nK, W, Wt);
+    check("tile_w");
+}
+
+int gemm_slices(int n, int k)
+{
+    const int tiles = ((n + 15) / 16) * ((k + 15) / 16);
+    int sl = (2048 + tiles - 1) / tiles;          // about 2048 wavefronts
+    const int nK = (n + 3) / 4;
+    if (sl > 32) sl = 32;
+    if (sl > nK) sl = nK;
+    return sl < 1 ? 1 : sl;
+}
+
+long gemm_part_size(int n, int k) { return (long)gemm_slices(n, k) * 16 * ((n + 15) / 16) * k; }
+
+template <typename T>
+void gemm_mf(hipStream_t s, int n, int k, const double* Wt, const T* x, T* y, double* part)
+{
+    if (n <= 0 || k <= 0) return;
+    const int nR = (n + 15) / 16, nK = (n + 3) / 4, nC = (k + 15) / 16;
+    const int sl = gemm_slices(n, k), kpers = (nK + sl - 1) / sl;
+    hipLaunchKernelGGL(mf_gemm<T>, dim3(nblocks((long)nR * nC * sl)), dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    check("mf_gemm");
+    const long t = (long)n * k;
+    hipLaunchKernelGGL(k_gemm_sum<T>, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, k, nR, sl, part, y);
+    check("gemm_sum");
+}
+template void gemm_mf<float>(hipStream_t, int, int, const double*, const float*, float*, double*);
+template void gemm_mf<double>(hipStream_t, int, int, const double*, const double*, double*, double*);
 
 template struct MF<float>;
 template struct MF<double>;

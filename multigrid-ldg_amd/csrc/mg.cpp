@@ -736,6 +736,9 @@ void MGH::build_collapse()
     }
     HIPCHK(hipStreamSynchronize(s));
     W = std::move(X[cut]);
+    Wt.alloc((size_t)dev::gemm_tiles_size(k));
+    dev::gemm_tiles(s, k, W, Wt);
+    HIPCHK(hipStreamSynchronize(s));
 }
 
 // z = V(r): enqueue the preconditioner on the stream
@@ -888,6 +891,7 @@ void MGH::ensure_batch(int k, int precision)
     BS.alloc((size_t)k * dev::S_COUNT);
     Bpart.alloc((size_t)1024 * 3 * k + 64);
     Bactive.alloc(k);
+    if (cut >= 0) Bgemm.alloc((size_t)dev::gemm_part_size(L[cut].nb * L[cut].B, k));
     bk = k; bprec = precision;
     tune_batch(k, precision);
 }
@@ -1008,7 +1012,7 @@ void MGH::bvcycle(int k)
     Level& c = L[bottom];
     const int n = c.nb * c.B;
     if (bottom == cut) {
-        dev::gemm_f64<T>(s, n, k, W, bb<T>(BL[bottom]), bx<T>(BL[bottom]));
+        dev::gemm_mf<T>(s, n, k, Wt, bb<T>(BL[bottom]), bx<T>(BL[bottom]), Bgemm);
     } else if (prm.coarse == 0) {
         dev::MOps<T>::coarse(s, n, k, c.Cinv, bb<T>(BL[bottom]), bx<T>(BL[bottom]), submean);
     } else {

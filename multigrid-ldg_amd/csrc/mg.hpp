@@ -153,12 +153,13 @@ struct MGH {
     int cut = -1;                      // first collapsed level (-1: none)
     int submean = 1;                   // coarsest b -= mean(b) (the quadtree Multigrid, multigrid.h:365-368)
     DBuf<double> W;                    // collapsed V-cycle below `cut`, FP64 dense n x n
+    DBuf<double> Wt;                   // the same in the MFMA tile layout (batched form)
 
     // batched (k right-hand sides) state
     struct BatchLevel { DBuf<double> x64, b64, r64, alt64; DBuf<float> x32, b32, r32, alt32; };
     int bk = 0, bprec = -1;
     std::vector<BatchLevel> BL;
-    DBuf<double> Br, Bz, Bzold, Bd, Bq, BS, Bpart;
+    DBuf<double> Br, Bz, Bzold, Bd, Bq, BS, Bpart, Bgemm;
     DBuf<int> Bactive;
     hipGraphExec_t bgraph[2] = {nullptr, nullptr};
 
