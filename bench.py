@@ -152,6 +152,115 @@ def unstructured_leg(mgldg, torch, nrhs, warm):
     }
 
 
+def bumps(X, seed, n=16):
+    """a smooth field: n Gaussian bumps with seeded centres, widths and signs"""
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 2))
+    w = 0.03 + 0.1 * rng.random(n)
+    a = rng.choice([-1.0, 1.0], n) * (0.5 + rng.random(n))
+    f = np.zeros(len(X))
+    for i in range(n):
+        f += a[i] * np.exp(-((X[:, 0] - c[i, 0]) ** 2 + (X[:, 1] - c[i, 1]) ** 2) / (2 * w[i] ** 2))
+    return f
+
+
+def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1):
+    """The adjoint workload: uniform 1024 x 1024 quadtree, p = 2.  u* solves
+    f* (16 Gaussian bumps, seed 0); u solves f (16 bumps, seed 1) with
+    g = sin(2 pi (x + y)) on the boundary; L = 0.5 ||u - u*||^2.  dL/df and
+    dL/dg come from one adjoint MG-PCG and are checked against central
+    differences (eps, FP64) along ndir random directions (half in f, half in
+    g), the perturbed solves batched 32 columns at a time."""
+    ctx = mgldg.Context(0)
+    t0 = time.perf_counter()
+    qt = mgldg.Quadtree(ctx, 2, lambda x: 1.0 / 1024)
+    prob = mgldg.LDGPoisson(qt, 2, tau0=0.0, tauD=1000.0)
+    mg = mgldg.Multigrid(prob, "gs", npre=2, npost=2, precision="mixed")
+    t_setup = time.perf_counter() - t0
+    log(f"adjoint: {prob.ne} elements, {prob.ndof} dofs, {mg.nlevels} levels, setup {t_setup:.2f}s")
+    X = prob.nodes()
+    xq, _ = prob.bc_qpoints()
+    dev = torch.device("cuda", 0)
+    fs = torch.tensor(bumps(X, 0), device=dev)
+    f = torch.tensor(bumps(X, 1), device=dev)
+    g = torch.tensor(np.sin(2 * np.pi * (xq[:, 0] + xq[:, 1])), device=dev)
+    zg = torch.zeros_like(g)
+    # smooth right-hand sides put the FP64 floor of ||b - A x|| / ||b|| near
+    # kappa(A) eps ~ 1e-10 at h = 1/1024: solves stop at 1e-10 or at that floor
+    floor = 1e-9
+
+    def ok(sts, what):
+        for q in (sts if isinstance(sts, list) else [sts]):
+            if q["relres_true"] > floor or q["status"] not in ("converged", "maxit"):
+                log("ADJOINT LEG SOLVE FAILED", what, q)
+                sys.exit(1)
+    kw = dict(tol=1e-10, maxit=300, criterion="true")
+    us, st = mgldg.pcg(mg, mgldg.rhs_device(mg, fs, zg), **kw)
+    ok(st, "u*")
+    u, st = mgldg.pcg(mg, mgldg.rhs_device(mg, f, g), **kw)
+    ok(st, "u")
+    r = u - us
+    L0 = 0.5 * float(r @ r)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        akw = dict(tol=1e-10, maxit=300, criterion="true", precision=prec, flexible=prec == "mixed", use_graph=True)
+        for _ in range(warm):
+            mgldg.adjoint(mg, r, **akw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        lam, dLdf, dLdg, ast = mgldg.adjoint(mg, r, **akw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        ok(ast, "adjoint " + prec)
+        out[prec] = {"value": round(prob.ndof / dt, 1), "ms_per_step": round(dt * 1e3, 3), "iterations": ast["iterations"],
+                     "status": ast["status"], "relres_true": ast["relres_true"]}
+        if prec == "fp64":
+            grads = (dLdf, dLdg)
+        log(f"adjoint {prec}: {dt * 1e3:.1f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {ast['iterations']}")
+    dLdf, dLdg = grads
+    # central differences along random smooth directions (4 Gaussian bumps in f,
+    # low Fourier modes of g), 32 perturbed solves per batch.  L is quadratic in
+    # (f, g), so central differences carry no truncation error: what they see
+    # besides the gradient is the solves' rounding divided by 2 eps
+    rng = np.random.default_rng(7)
+    nf = ndir // 2
+    cols = []
+    for d in range(ndir):
+        if d < nf:
+            v = torch.tensor(bumps(X, 100 + d, n=4), device=dev)
+            cols.append(("f", v, float(dLdf @ v)))
+        else:
+            a, m, ph = rng.standard_normal(3), rng.integers(1, 4, 3), 2 * np.pi * rng.random(3)
+            gv = sum(a[i] * np.sin(2 * np.pi * m[i] * (xq[:, 0] if i % 2 else xq[:, 1]) + ph[i]) for i in range(3))
+            v = torch.tensor(gv, device=dev)
+            cols.append(("g", v, float(dLdg @ v)))
+    fd_out = {}
+    t_fd = time.perf_counter()
+    for e in (eps, 1e-3):
+        errs = []
+        for c0 in range(0, ndir, 16):
+            chunk = cols[c0:c0 + 16]
+            F = torch.stack([f + sgn * e * v if kind == "f" else f for kind, v, _ in chunk for sgn in (1, -1)], 1)
+            G = torch.stack([g + sgn * e * v if kind == "g" else g for kind, v, _ in chunk for sgn in (1, -1)], 1)
+            U, sts = mgldg.pcg(mg, mgldg.rhs_device(mg, F.contiguous(), G.contiguous()), **kw)
+            ok(sts, "finite differences")
+            for j, (kind, v, ad) in enumerate(chunk):
+                rp, rm = U[:, 2 * j] - us, U[:, 2 * j + 1] - us
+                fd = (0.5 * float(rp @ rp) - 0.5 * float(rm @ rm)) / (2 * e)
+                errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+        fd_out[f"{e:g}"] = {"max_rel_err": max(errs), "median_rel_err": float(np.median(errs))}
+        log(f"adjoint: L = {L0:.6e}; central differences (eps {e:g}) on {ndir} directions: max rel err "
+            f"{max(errs):.3e}, median {float(np.median(errs)):.3e}")
+    t_fd = time.perf_counter() - t_fd
+    return {
+        "workload": "uniform 1024x1024 quadtree, p=2, L = 0.5||u - u*||^2, f* / f = 16 Gaussian bumps (seeds 0 / 1), "
+                    "g = sin(2 pi (x+y))",
+        "ndof": prob.ndof, "levels": mg.nlevels, "setup_s": round(t_setup, 3), "adjoint": out, "loss": L0,
+        "fd_check": {"directions": ndir, "kind": "smooth random (4 bumps in f, 3 Fourier modes in g)",
+                     "by_eps": fd_out, "solve_tol": 1e-10, "relres_floor_accepted": floor, "seconds": round(t_fd, 2)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +272,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides of the batched form")
     ap.add_argument("--solve-only", action="store_true", help="setup + three mixed solves (profiling)")
     ap.add_argument("--no-unstructured", action="store_true", help="skip the unit-disk workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
     ap.add_argument("--batched-only", action="store_true",
                     help="setup + two mixed and one FP64 batched solves of --nrhs columns (profiling)")
     args = ap.parse_args()
@@ -177,6 +288,9 @@ def main():
     if not os.path.exists(mgldg.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(mgldg, torch, warm=args.warmup)}), flush=True)
+        return
 
     torch.cuda.init()
     ctx = mgldg.Context(0)
@@ -334,6 +448,7 @@ def main():
         log(f"cpu: {cpu_rate:.0f} DoF-solves/s, iterations {ito}, errors {err}")
 
     unstructured = None if args.no_unstructured else unstructured_leg(mgldg, torch, k, args.warmup)
+    adjoint = None if args.no_adjoint else adjoint_leg(mgldg, torch, warm=args.warmup)
 
     line = {
         "metric": "DoF-solves/s to 1e-10 residual, mixed vs FP64 MG-PCG; smoother HBM GB/s",
@@ -368,6 +483,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "unstructured": unstructured,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 
