@@ -261,6 +261,135 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1):
     }
 
 
+def ball_mesh(n=41, jitter=0.01, seed=0):
+    """Unit ball: the cubic lattice of spacing 1/n, vertices jittered by
+    jitter/n (seed), each lattice cube whose corners lie in r <= 1 split into
+    the 6 tetrahedra of its Kuhn (Freudenthal) triangulation -- a Delaunay
+    triangulation of the unjittered lattice."""
+    import itertools
+    rng = np.random.default_rng(seed)
+    s = 1.0 / n
+    m = 2 * n + 1
+    g = np.arange(-n, n + 1)
+    P = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * s
+    c = np.stack(np.meshgrid(np.arange(-n, n), np.arange(-n, n), np.arange(-n, n), indexing="ij"), -1).reshape(-1, 3)
+    corners = np.array(list(itertools.product([0, 1], repeat=3)))
+    c = c[np.all([np.linalg.norm((c + o) * s, axis=1) <= 1.0 for o in corners], axis=0)]
+    vid = lambda w: ((w[:, 0] + n) * m + (w[:, 1] + n)) * m + (w[:, 2] + n)
+    tets = []
+    for perm in itertools.permutations(range(3)):
+        cur = c.copy()
+        v = [vid(cur)]
+        for d in perm:
+            cur = cur.copy()
+            cur[:, d] += 1
+            v.append(vid(cur))
+        tets.append(np.stack(v, 1))
+    T = np.concatenate(tets)
+    used, inv = np.unique(T, return_inverse=True)
+    P = P[used] + jitter * s * rng.uniform(-1, 1, (len(used), 3))
+    return P, inv.reshape(T.shape).astype(np.int32)
+
+
+def ball_leg(mgldg, torch, nrhs=16, warm=1):
+    """The 3-D workload: unit ball of tetrahedra (ball_mesh), p = 2, tau0 =
+    100, tauD = 1000, the library's agglomeration (at most 8 per agglomerate),
+    V(2,2) multicolour block GS, RAT.  nrhs forward right-hand sides of uneven
+    support (Gaussian bumps of random centre and width 0.03..0.3) and the
+    nrhs adjoint solves of the linear losses L_i = <w_i, u_i> (w_i smooth,
+    so dL_i/du_i = w_i is known at once) go in one batch of 2 nrhs columns:
+    mixed precision, V-cycle replayed from its hipGraph; beside it the plain
+    FP64 single-column solve."""
+    t0 = time.perf_counter()
+    pts, tets = ball_mesh()
+    t_mesh = time.perf_counter() - t0
+    ctx = mgldg.Context(0)
+    t0 = time.perf_counter()
+    wf = mgldg.Wireframe(ctx, pts, tets)
+    prob = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
+    t_prob = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    parts = mgldg.agglomerate(prob)
+    t_agg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mg = mgldg.Multigrid(prob, "gs", omega=0.8, npre=2, npost=2, restriction="rat", precision="mixed",
+                         partitions=parts)
+    t_mg = time.perf_counter() - t0
+    log(f"ball: {prob.ne} tetrahedra, {prob.ndof} dofs, levels {mg.level_rows}; setup mesh {t_mesh:.2f}s "
+        f"problem {t_prob:.2f}s agglomeration {t_agg:.2f}s hierarchy {t_mg:.2f}s")
+    X = prob.nodes()
+    rng = np.random.default_rng(3)
+    dev = torch.device("cuda", 0)
+    cols = []
+    for i in range(nrhs):        # forward: localised sources of uneven support
+        c = rng.uniform(-0.6, 0.6, 3)
+        w = 0.03 + 0.27 * rng.random()
+        cols.append(np.exp(-((X - c) ** 2).sum(1) / (2 * w * w)))
+    F = torch.tensor(np.stack(cols, 1), device=dev)
+    B = mgldg.rhs_device(mg, F)
+    Wt = []
+    for i in range(nrhs):        # adjoint: dL_i/du_i = w_i, smooth weights
+        a = rng.standard_normal(3)
+        Wt.append(np.cos(np.pi * (X @ a)))
+    Badj = torch.tensor(np.stack(Wt, 1), device=dev)
+    Ball = torch.cat([B, Badj], 1).contiguous()
+    out = {}
+    kw = dict(tol=1e-10, maxit=400, criterion="true")
+    b0 = Ball[:, 0].contiguous()
+    for prec in ("fp64", "mixed"):
+        skw = dict(kw, precision=prec, flexible=prec == "mixed", use_graph=True)
+        for _ in range(warm):
+            mgldg.pcg(mg, b0, torch.zeros_like(b0), **skw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, st = mgldg.pcg(mg, b0, torch.zeros_like(b0), **skw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        if st["status"] != "converged":
+            log("BALL NOT CONVERGED", prec, st)
+            sys.exit(1)
+        out[f"single_{prec}"] = {"value": round(prob.ndof / dt, 1), "ms_per_step": round(dt * 1e3, 3),
+                                 "iterations": st["iterations"]}
+        log(f"ball single {prec}: {dt * 1e3:.1f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {st['iterations']}")
+    bkw = dict(kw, precision="mixed", flexible=True, use_graph=True)
+    mgldg.pcg(mg, Ball[:, :2].contiguous(), **bkw)            # plan the batch kernels on a small batch first
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    Xall, sts = mgldg.pcg(mg, Ball, torch.zeros_like(Ball), **bkw)
+    dLdf, dLdg = mgldg.adjoint_maps(mg, Xall[:, nrhs:])           # dL_i/df = M lambda_i, dL_i/dg = B_g^T lambda_i
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    for q in sts:
+        if q["status"] != "converged" or q["relres_true"] > 1e-10:
+            log("BALL BATCH NOT CONVERGED", q)
+            sys.exit(1)
+    k = 2 * nrhs
+    out["batched_mixed"] = {"value": round(prob.ndof * k / dt, 1), "ms_per_step": round(dt * 1e3, 3),
+                            "iterations": max(q["iterations"] for q in sts), "columns": k}
+    log(f"ball batched mixed ({nrhs} forward + {nrhs} adjoint): {dt * 1e3:.1f} ms, {prob.ndof * k / dt:.3g} DoF-solves/s, "
+        f"iterations {out['batched_mixed']['iterations']}")
+    x1, _ = mgldg.pcg(mg, b0, **dict(kw, precision="fp64"))
+    diff = float(torch.linalg.norm(Xall[:, 0] - x1) / torch.linalg.norm(x1))
+    # adjoint check on one column: dL/df . v = <w, A^{-1} M v> for a smooth v
+    v = torch.tensor(np.sin(3 * X[:, 0]) * np.cos(2 * X[:, 2]), device=dev)
+    uv, _ = mgldg.pcg(mg, mgldg.rhs_device(mg, v), **dict(kw, precision="fp64"))
+    adj_err = abs(float(dLdf[:, 0] @ v) - float(Badj[:, 0] @ uv)) / abs(float(Badj[:, 0] @ uv))
+    del dLdg
+    return {
+        "workload": "unit ball, Kuhn split of a 1%-jittered cubic lattice (spacing 1/41) inside r <= 1, p=2, "
+                    "tau0=100, tauD=1000",
+        "elements": prob.ne, "ndof": prob.ndof, "levels": mg.nlevels, "level_rows": mg.level_rows,
+        "smoother": "V(2,2) multicolour block GS, omega 0.8", "coarsening": "own agglomeration (<= 8 per level), RAT",
+        "nrhs": nrhs, **out,
+        "mixed_batched_vs_fp64_single": round(out["batched_mixed"]["value"] / out["single_fp64"]["value"], 3),
+        "mixed_batched_vs_fp64_single_rel_diff": diff,
+        "adjoint_check_rel_err": adj_err,
+        "kernel_plan": plan_summary(mg.batch_plan()),
+        "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
+                    "hierarchy": round(t_mg, 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +403,8 @@ def main():
     ap.add_argument("--no-unstructured", action="store_true", help="skip the unit-disk workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
+    ap.add_argument("--no-ball", action="store_true", help="skip the 3-D ball workload")
+    ap.add_argument("--ball-only", action="store_true", help="only the 3-D ball workload")
     ap.add_argument("--batched-only", action="store_true",
                     help="setup + two mixed and one FP64 batched solves of --nrhs columns (profiling)")
     args = ap.parse_args()
@@ -290,6 +421,9 @@ def main():
         __graft_entry__.build()
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(mgldg, torch, warm=args.warmup)}), flush=True)
+        return
+    if args.ball_only:
+        print(json.dumps({"ball": ball_leg(mgldg, torch, warm=args.warmup)}), flush=True)
         return
 
     torch.cuda.init()

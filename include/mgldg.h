@@ -249,6 +249,10 @@ int mgldg_solve(mgldg_mg mg, const double* b, double* x, int nrhs,
 int mgldg_adjoint_device(mgldg_mg mg, const double* dLdu, double* lambda, double* dLdf,
                          double* dLdg, int nrhs, const mgldg_solve_params* prm,
                          mgldg_solve_stats* stats);
+/* No reference counterpart.  The maps of the adjoint alone, for adjoint
+ * states lambda solved elsewhere (e.g. batched beside forward solves):
+ * dL/df = M lambda, dL/dg = B_g^T lambda (either output may be NULL). */
+int mgldg_adjoint_maps_device(mgldg_mg mg, const double* lambda, double* dLdf, double* dLdg, int nrhs);
 /* b = M f + B_g g on the GPU (device pointers; f or g may be NULL) */
 int mgldg_rhs_device(mgldg_mg mg, const double* f, const double* g, int nrhs, double* b);
 

@@ -499,6 +499,12 @@ int mgldg_adjoint_device(mgldg_mg mg, const double* dLdu, double* lambda, double
     });
 }
 
+int mgldg_adjoint_maps_device(mgldg_mg mg, const double* lambda, double* dLdf, double* dLdg, int nrhs)
+{
+    if (!mg || !lambda || nrhs < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] { mg->m.pr->adjoint_maps(lambda, nrhs, dLdf, dLdg); });
+}
+
 int mgldg_rhs_device(mgldg_mg mg, const double* f, const double* g, int nrhs, double* b)
 {
     if (!mg || !b || nrhs < 1) return MGLDG_ERR_ARG;

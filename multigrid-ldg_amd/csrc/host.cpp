@@ -323,17 +323,28 @@ int HostBSR::find(int i, int j) const
 void PatternBuilder::finish(HostBSR& out, int br, int bc)
 {
     out.nbr = nbr; out.nbc = nbc; out.br = br; out.bc = bc;
-    out.rowptr.assign(nbr + 1, 0);
-    for (int i = 0; i < nbr; ++i) {
-        auto& r = rows[i];
-        std::sort(r.begin(), r.end());
-        r.erase(std::unique(r.begin(), r.end()), r.end());
-        out.rowptr[i + 1] = out.rowptr[i] + (int)r.size();
+    // counting sort of the pairs by row, then sort and deduplicate each row
+    std::vector<int> off(nbr + 1, 0);
+    for (int i : ri) off[i + 1]++;
+    for (int i = 0; i < nbr; ++i) off[i + 1] += off[i];
+    std::vector<int> tmp(ci.size());
+    {
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (size_t t = 0; t < ri.size(); ++t) tmp[pos[ri[t]]++] = ci[t];
     }
-    out.cols.resize(out.rowptr[nbr]);
-    for (int i = 0; i < nbr; ++i) std::copy(rows[i].begin(), rows[i].end(), out.cols.begin() + out.rowptr[i]);
+    ri.clear(); ri.shrink_to_fit(); ci.clear(); ci.shrink_to_fit();
+    out.rowptr.assign(nbr + 1, 0);
+    int w = 0;
+    for (int i = 0; i < nbr; ++i) {
+        auto b = tmp.begin() + off[i], e = tmp.begin() + off[i + 1];
+        std::sort(b, e);
+        e = std::unique(b, e);
+        for (auto it = b; it != e; ++it) tmp[w++] = *it;
+        out.rowptr[i + 1] = w;
+    }
+    tmp.resize(w);
+    out.cols.swap(tmp);
     out.val.assign((size_t)out.cols.size() * br * bc, 0.0);
-    rows.clear(); rows.shrink_to_fit();
 }
 
 // ======================================================================= mesh

@@ -80,9 +80,9 @@ struct HostBSR {
 // Pattern from (row, col) pairs: sorted unique per row.
 struct PatternBuilder {
     int nbr, nbc;
-    std::vector<std::vector<int>> rows;
-    PatternBuilder(int m, int n) : nbr(m), nbc(n), rows(m) {}
-    void add(int i, int j) { rows[i].push_back(j); }
+    std::vector<int> ri, ci;
+    PatternBuilder(int m, int n) : nbr(m), nbc(n) {}
+    void add(int i, int j) { ri.push_back(i); ci.push_back(j); }
     void finish(HostBSR& out, int br, int bc);     // allocates zero values
 };
 

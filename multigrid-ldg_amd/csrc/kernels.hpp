@@ -73,9 +73,11 @@ struct MOps {
     static void coarse(hipStream_t s, int n, int k, const double* Cinv, T* b, T* x, int subtract_mean);
 };
 // The same operators on the matrix cores (mfma.hip): one wavefront per block
-// row and nt tiles of 16 columns, blocks padded to 16 x 16 in registers.
-// Compiled for B in {6, 8, 9, 10, 16} and nt in {1, 2, 4}.
+// row and nt tiles of 16 columns, blocks padded to 16 x 16 (B <= 16) or
+// 32 x 32 (B = 27) in registers.  Compiled for B in {6, 8, 9, 10, 16} with
+// nt in {1, 2, 4} and B = 27 with nt in {1, 2}.
 bool mf_supported(int B);
+int mf_max_tiles(int B);
 template <typename T>
 struct MF {
     static void apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,

@@ -3,9 +3,10 @@
 // For a block row i the batched operators are small dense products
 //   Y_i = sum_j A_ij X_j     (A_ij: B x B, X_j: B x k, k columns contiguous)
 // One wavefront owns one block row and NT tiles of 16 columns.  Each block is
-// a 16 x 16 x 4-step MFMA product (v_mfma_f32_16x16x4f32 / _f64_16x16x4f64),
-// blocks of 6..16 rows padded with zeros in registers: lane l supplies
-// A[l % 16][4s + l / 16] and X[4s + l / 16][l % 16] for K step s.  The
+// a product of RT = ceil(B / 16) row tiles of 16 x 16 x 4-step MFMAs
+// (v_mfma_f32_16x16x4f32 / _f64_16x16x4f64), blocks of 6..32 rows padded
+// with zeros in registers: for row tile rt lane l supplies
+// A[16 rt + l % 16][4s + l / 16] and X[4s + l / 16][l % 16] for K step s.  The
 // accumulator lane layout is the instruction's (f32: register i of lane l
 // holds row 4 (l / 16) + i; f64: row 4 i + l / 16; column l % 16).
 // The smoothers write the residual tile to LDS and read it back in the
@@ -47,21 +48,28 @@ template <> struct Mf<double> {
     static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
 };
 
-// acc[t] += A_blk X_j (rows padded to 16, K padded to a multiple of 4)
+// acc[rt * NT + t] += A_blk X_j (rows padded to 16 RT, K to a multiple of 4)
 template <typename T, int B, int NT>
 __device__ __forceinline__ void block_mac(typename Mf<T>::v4* acc, const T* __restrict__ blk, const T* __restrict__ xj,
                                           int k, int col0, int lane)
 {
+    constexpr int RT = (B + 15) / 16;
     const int r16 = lane & 15, q = lane >> 4;
 #pragma unroll
     for (int s = 0; s < (B + 3) / 4; ++s) {
         const int c = 4 * s + q;
-        const T a = (r16 < B && c < B) ? panel_at<T, B>(blk, r16, c) : (T)0;
+        T xb[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int col = col0 + 16 * t + r16;
-            const T xb = (c < B && col < k) ? xj[(size_t)c * k + col] : (T)0;
-            acc[t] = Mf<T>::mac(a, xb, acc[t]);
+            xb[t] = (c < B && col < k) ? xj[(size_t)c * k + col] : (T)0;
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int r = 16 * rt + r16;
+            const T a = (r < B && c < B) ? panel_at<T, B>(blk, r, c) : (T)0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[rt * NT + t] = Mf<T>::mac(a, xb[t], acc[rt * NT + t]);
         }
     }
 }
@@ -77,51 +85,62 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int i = w / ngroups, g = w - i * ngroups;
     if (i >= nrows) return;
+    constexpr int RT = (B + 15) / 16;
     const int col0 = g * 16 * NT;
-    V4 acc[NT];
+    V4 acc[RT * NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+    for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int k1 = rowptr[i + 1];
     for (int kk = rowptr[i]; kk < k1; ++kk)
         block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)cols[kk] * B * k, k, col0, lane);
     const int col_l = lane & 15;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int col = col0 + 16 * t + col_l;
-        if (col >= k) continue;
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = Mf<T>::row(lane, e);
-            if (r >= B) continue;
-            const size_t o = ((size_t)i * B + r) * k + col;
-            y[o] = b ? (T)(b[o] - acc[t][e]) : acc[t][e];
+        for (int t = 0; t < NT; ++t) {
+            const int col = col0 + 16 * t + col_l;
+            if (col >= k) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * rt + Mf<T>::row(lane, e);
+                if (r >= B) continue;
+                const size_t o = ((size_t)i * B + r) * k + col;
+                y[o] = b ? (T)(b[o] - acc[rt * NT + t][e]) : acc[rt * NT + t][e];
+            }
         }
-    }
 }
 
-// the D^-1 product of a smoother: r (acc layout, 16 x 16 NT) -> LDS -> D_i r
+// the D^-1 product of a smoother: r (acc layout, 16 RT x 16 NT) -> LDS -> D_i r
 template <typename T, int B, int NT>
-__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* rt, const T* __restrict__ D,
+__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* rt_in, const T* __restrict__ D,
                                            T* lds, int lane)
 {
     using V4 = typename Mf<T>::v4;
+    constexpr int RT = (B + 15) / 16;
     const int col_l = lane & 15;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) lds[Mf<T>::row(lane, e) * (16 * NT) + 16 * t + col_l] = rt[t][e];
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                lds[(16 * rt + Mf<T>::row(lane, e)) * (16 * NT) + 16 * t + col_l] = rt_in[rt * NT + t][e];
     __syncthreads();
     const int r16 = lane & 15, q = lane >> 4;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) out[t] = V4{0, 0, 0, 0};
+    for (int t = 0; t < RT * NT; ++t) out[t] = V4{0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < (B + 3) / 4; ++s) {
         const int c = 4 * s + q;
-        const T a = (r16 < B && c < B) ? panel_at<T, B>(D, r16, c) : (T)0;
+        T xb[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const T xb = (c < B) ? lds[c * (16 * NT) + 16 * t + r16] : (T)0;
-            out[t] = Mf<T>::mac(a, xb, out[t]);
+        for (int t = 0; t < NT; ++t) xb[t] = (c < B) ? lds[c * (16 * NT) + 16 * t + r16] : (T)0;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int r = 16 * rt + r16;
+            const T a = (r < B && c < B) ? panel_at<T, B>(D, r, c) : (T)0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) out[rt * NT + t] = Mf<T>::mac(a, xb[t], out[rt * NT + t]);
         }
     }
 }
@@ -136,17 +155,18 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
                                              T* x, T omega, const int* __restrict__ color, int cur, int skip)
 {
     using V4 = typename Mf<T>::v4;
-    __shared__ T lds_all[4 * 16 * 16 * NT];
+    constexpr int RT = (B + 15) / 16;
+    __shared__ T lds_all[4 * 16 * RT * 16 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    T* lds = lds_all + wv * 16 * 16 * NT;
+    T* lds = lds_all + wv * 16 * RT * 16 * NT;
     const int w = blockIdx.x * 4 + wv;
     const int ri = w / ngroups, g = w - ri * ngroups;
     const bool valid = ri < nrows;          // every wave reaches the barrier in dinv_times
     const int i = valid ? rows[ri] : 0;
     const int col0 = g * 16 * NT, col_l = lane & 15;
-    V4 acc[NT];
+    V4 acc[RT * NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+    for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     if (valid) {
         const int k1 = rowptr[i + 1];
         for (int kk = rowptr[i]; kk < k1; ++kk) {
@@ -156,32 +176,36 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
             block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)j * B * k, k, col0, lane);
         }
     }
-    V4 rt[NT];
+    V4 res[RT * NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int col = col0 + 16 * t + col_l;
+    for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = Mf<T>::row(lane, e);
-            rt[t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[t][e]) : (T)0;
+        for (int t = 0; t < NT; ++t) {
+            const int col = col0 + 16 * t + col_l;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * rr + Mf<T>::row(lane, e);
+                res[rr * NT + t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[rr * NT + t][e]) : (T)0;
+            }
         }
-    }
-    V4 out[NT];
-    dinv_times<T, B, NT>(out, rt, Dinv + (size_t)i * B * B, lds, lane);
+    V4 out[RT * NT];
+    dinv_times<T, B, NT>(out, res, Dinv + (size_t)i * B * B, lds, lane);
     if (!valid) return;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int col = col0 + 16 * t + col_l;
-        if (col >= k) continue;
+    for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = Mf<T>::row(lane, e);
-            if (r >= B) continue;
-            const size_t o = ((size_t)i * B + r) * k + col;
-            const T xi = skip ? (T)0 : x[o];
-            x[o] = xi + omega * (out[t][e] - xi);
+        for (int t = 0; t < NT; ++t) {
+            const int col = col0 + 16 * t + col_l;
+            if (col >= k) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * rr + Mf<T>::row(lane, e);
+                if (r >= B) continue;
+                const size_t o = ((size_t)i * B + r) * k + col;
+                const T xi = skip ? (T)0 : x[o];
+                x[o] = xi + omega * (out[rr * NT + t][e] - xi);
+            }
         }
-    }
 }
 
 // damped block Jacobi (multigrid.h:223-229): xout = x + omega D^-1 (b - A x)
@@ -192,47 +216,52 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
                                                  const T* __restrict__ x, T* __restrict__ xout, T omega)
 {
     using V4 = typename Mf<T>::v4;
-    __shared__ T lds_all[4 * 16 * 16 * NT];
+    constexpr int RT = (B + 15) / 16;
+    __shared__ T lds_all[4 * 16 * RT * 16 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    T* lds = lds_all + wv * 16 * 16 * NT;
+    T* lds = lds_all + wv * 16 * RT * 16 * NT;
     const int w = blockIdx.x * 4 + wv;
     const int i0 = w / ngroups, g = w - i0 * ngroups;
     const bool valid = i0 < nrows;
     const int i = valid ? i0 : 0;
     const int col0 = g * 16 * NT, col_l = lane & 15;
-    V4 acc[NT];
+    V4 acc[RT * NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+    for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     if (valid) {
         const int k1 = rowptr[i + 1];
         for (int kk = rowptr[i]; kk < k1; ++kk)
             block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)cols[kk] * B * k, k, col0, lane);
     }
-    V4 rt[NT];
+    V4 res[RT * NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int col = col0 + 16 * t + col_l;
+    for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = Mf<T>::row(lane, e);
-            rt[t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[t][e]) : (T)0;
+        for (int t = 0; t < NT; ++t) {
+            const int col = col0 + 16 * t + col_l;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * rr + Mf<T>::row(lane, e);
+                res[rr * NT + t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[rr * NT + t][e]) : (T)0;
+            }
         }
-    }
-    V4 out[NT];
-    dinv_times<T, B, NT>(out, rt, Dinv + (size_t)i * B * B, lds, lane);
+    V4 out[RT * NT];
+    dinv_times<T, B, NT>(out, res, Dinv + (size_t)i * B * B, lds, lane);
     if (!valid) return;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int col = col0 + 16 * t + col_l;
-        if (col >= k) continue;
+    for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = Mf<T>::row(lane, e);
-            if (r >= B) continue;
-            const size_t o = ((size_t)i * B + r) * k + col;
-            xout[o] = x[o] + omega * out[t][e];
+        for (int t = 0; t < NT; ++t) {
+            const int col = col0 + 16 * t + col_l;
+            if (col >= k) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * rr + Mf<T>::row(lane, e);
+                if (r >= B) continue;
+                const size_t o = ((size_t)i * B + r) * k + col;
+                xout[o] = x[o] + omega * out[rr * NT + t][e];
+            }
         }
-    }
 }
 
 // W (n x n, row-major FP64) -> 16 x 4 tiles in the A-operand lane order:
@@ -318,10 +347,13 @@ void check(const char* what)
         case 16 * 8 + 1: { constexpr int BB = 16, TT = 1; __VA_ARGS__; } break;                \
         case 16 * 8 + 2: { constexpr int BB = 16, TT = 2; __VA_ARGS__; } break;                \
         case 16 * 8 + 4: { constexpr int BB = 16, TT = 4; __VA_ARGS__; } break;                \
+        case 27 * 8 + 1: { constexpr int BB = 27, TT = 1; __VA_ARGS__; } break;                \
+        case 27 * 8 + 2: { constexpr int BB = 27, TT = 2; __VA_ARGS__; } break;                \
         default: throw std::runtime_error("MFMA block kernels: block size " + std::to_string(B) + " / tiles " + std::to_string(NT) + " not compiled"); \
     }
 
-bool mf_supported(int B) { return B == 6 || B == 8 || B == 9 || B == 10 || B == 16; }
+bool mf_supported(int B) { return B == 6 || B == 8 || B == 9 || B == 10 || B == 16 || B == 27; }
+int mf_max_tiles(int B) { return B > 16 ? 2 : 4; }
 
 template <typename T>
 void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,

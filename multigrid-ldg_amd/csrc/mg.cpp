@@ -933,7 +933,7 @@ void MGH::tune_batch(int k, int precision)
             Level& lv = L[l];
             BTune& t = btune[p][l];
             if (!dev::mf_supported(lv.B)) continue;
-            if (bforce >= 0) { t.apply = t.smooth = bforce; continue; }
+            if (bforce >= 0) { t.apply = t.smooth = std::min(bforce, dev::mf_max_tiles(lv.B)); continue; }
             const long m = (long)lv.nb * lv.B * k;
             auto run = [&](auto tag) {
                 using T = decltype(tag);
@@ -944,6 +944,7 @@ void MGH::tune_batch(int k, int precision)
                 for (int ci = 0; ci < 4; ++ci) {
                     const int nt = cand[ci];
                     if (nt > 1 && 16 * nt / 2 >= k) continue;      // more tiles than columns
+                    if (nt > dev::mf_max_tiles(lv.B)) continue;
                     t.apply = nt; t.smooth = nt;
                     const float ma = timed([&] { bapply<T>(l, k, bx<T>(v), bb<T>(v), br<T>(v)); });
                     const float msm = timed([&] { brelax<T>(l, k, 1, false, false); });

@@ -2,11 +2,29 @@
 #include "simplex.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 namespace mgldg {
+
+// fn(lo, hi) over [0, n) on up to 16 threads (disjoint index ranges)
+template <typename F>
+static void parallel_for(long n, F&& fn)
+{
+    const long nthr = std::min<long>({16L, (long)std::max(1u, std::thread::hardware_concurrency()), std::max(1L, n / 256)});
+    if (nthr <= 1) { fn(0L, n); return; }
+    std::vector<std::thread> th;
+    const long chunk = (n + nthr - 1) / nthr;
+    for (long t = 0; t < nthr; ++t) {
+        const long lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+    }
+    for (auto& x : th) x.join();
+}
 
 static long factorial(int n) { long r = 1; for (int i = 2; i <= n; ++i) r *= i; return r; }
 
@@ -126,7 +144,7 @@ void SimplexBasis::init(int N_, int P_)
 void SimplexBasis::eval(const double* x, double* phi) const
 {
     long double xl[3] = {x[0], N > 1 ? x[1] : 0.0, N > 2 ? x[2] : 0.0};
-    std::vector<long double> m(npl);
+    long double m[64];   // npl <= C(p+3, 3) = 56 for p <= 5
     for (int k = 0; k < npl; ++k) m[k] = mono_ld(xl, &expo[(size_t)k * N], N);
     for (int i = 0; i < npl; ++i) {
         long double s = 0.0L;
@@ -369,25 +387,25 @@ void SimplexLdg::assemble(int N_, int np, const double* pts, int nt, const int* 
             }
         }
     }
-    // patterns (addToBlock creates structural blocks for every normal component)
+    // patterns (addToBlock creates structural blocks for every normal
+    // component, so the N gradient patterns are one)
     {
-        PatternBuilder pm(ne, ne), pt(ne, ne);
-        std::vector<PatternBuilder> pg(N, PatternBuilder(ne, ne));
-        for (int e = 0; e < ne; ++e) { pm.add(e, e); for (int d = 0; d < N; ++d) pg[d].add(e, e); }
+        PatternBuilder pm(ne, ne), pt(ne, ne), pg(ne, ne);
+        for (int e = 0; e < ne; ++e) { pm.add(e, e); pg.add(e, e); }
         for (const SimplexFace& f : mesh.faces) {
             if (f.right >= 0) {
-                for (int d = 0; d < N; ++d) { pg[d].add(f.right, f.right); pg[d].add(f.right, f.left); }
+                pg.add(f.right, f.right); pg.add(f.right, f.left);
                 pt.add(f.right, f.right); pt.add(f.left, f.left); pt.add(f.right, f.left); pt.add(f.left, f.right);
             } else if (dirichlet(f)) {
                 pt.add(f.left, f.left);
             }
         }
         pm.finish(M, b, b); pt.finish(T, b, b);
-        for (int d = 0; d < N; ++d) pg[d].finish(G[d], b, b);
+        pg.finish(G[0], b, b);
+        for (int d = 1; d < N; ++d) G[d] = G[0];
     }
     // mass and broken gradient (ldgpoisson.tpp:26-58)
-    std::vector<double> W((size_t)b * b), MI((size_t)b * b), LL((size_t)b * b), RR((size_t)b * b), RL((size_t)b * b),
-        RLt((size_t)b * b);
+    std::vector<double> RLt((size_t)b * b);
     for (int e = 0; e < ne; ++e) add_blk(M, e, e, mesh.vol[e] / E.volume, E.mass.data());
     for (int e = 0; e < ne; ++e) {
         const double* I = &mesh.Jinv[(size_t)e * N * N];
@@ -405,62 +423,88 @@ void SimplexLdg::assemble(int N_, int np, const double* pts, int nt, const int* 
             w[q] = qw[q] * f.area * F.volume;   // as written in mesh.h:343-344
         }
     };
-    std::vector<double> phi, w;
-    // lifting and penalty terms (ldgpoisson.tpp:60-99)
-    for (int fi = 0; fi < nf; ++fi) {
-        const SimplexFace& f = mesh.faces[fi];
-        if (f.right >= 0) {
-            std::fill(LL.begin(), LL.end(), 0.0); std::fill(RR.begin(), RR.end(), 0.0); std::fill(RL.begin(), RL.end(), 0.0);
-            const double jac = f.area / F.volume;
-            const int* a = &sl[(size_t)fi * nfl]; const int* c = &sr[(size_t)fi * nfl];
-            for (int i = 0; i < nfl; ++i)
-                for (int j = 0; j < nfl; ++j) {
-                    const double v = F.mass[(size_t)i * nfl + j] * jac;
-                    LL[(size_t)a[i] * b + a[j]] += v;
-                    RR[(size_t)c[i] * b + c[j]] += v;
-                    RL[(size_t)c[i] * b + a[j]] += v;
+    // lifting and penalty terms (ldgpoisson.tpp:60-99).  Per chunk of faces
+    // the face blocks LL, RR, RL and the products M_e^{-1} RR, M_e^{-1} RL
+    // (interior) or M_e^{-1} LL (Dirichlet) are formed in parallel, then added
+    // into the operators in face order.
+    const size_t bb = (size_t)b * b;
+    const long chunk = 32768;
+    std::vector<double> fbuf;
+    for (long c0 = 0; c0 < nf; c0 += chunk) {
+        const long c1 = std::min<long>(nf, c0 + chunk);
+        fbuf.resize((size_t)(c1 - c0) * 5 * bb);
+        parallel_for(c1 - c0, [&](long i0, long i1) {
+            std::vector<double> MIt(bb), ph, wq;
+            for (long t = i0; t < i1; ++t) {
+                const int fi = (int)(c0 + t);
+                const SimplexFace& f = mesh.faces[fi];
+                double* LLb = &fbuf[(size_t)t * 5 * bb];
+                double *RRb = LLb + bb, *RLb = LLb + 2 * bb, *W1 = LLb + 3 * bb, *W2 = LLb + 4 * bb;
+                if (f.right >= 0) {
+                    std::fill(LLb, LLb + 3 * bb, 0.0);
+                    const double jac = f.area / F.volume;
+                    const int* a = &sl[(size_t)fi * nfl]; const int* c = &sr[(size_t)fi * nfl];
+                    for (int i = 0; i < nfl; ++i)
+                        for (int j = 0; j < nfl; ++j) {
+                            const double v = F.mass[(size_t)i * nfl + j] * jac;
+                            LLb[(size_t)a[i] * b + a[j]] += v;
+                            RRb[(size_t)c[i] * b + c[j]] += v;
+                            RLb[(size_t)c[i] * b + a[j]] += v;
+                        }
+                    const double iv = E.volume / mesh.vol[f.right];
+                    for (size_t k = 0; k < bb; ++k) MIt[k] = E.invmass[k] * iv;
+                    gemm_sq(b, 1.0, MIt.data(), RRb, W1);
+                    gemm_sq(b, 1.0, MIt.data(), RLb, W2);
+                } else if (dirichlet(f)) {
+                    boundary_phi(f, ph, wq);
+                    for (int i = 0; i < b; ++i)
+                        for (int j = 0; j < b; ++j) {
+                            double sv = 0;
+                            for (int q = 0; q < nq; ++q) sv += ph[(size_t)q * b + i] * wq[q] * ph[(size_t)q * b + j];
+                            LLb[(size_t)i * b + j] = sv;
+                        }
+                    const double iv = E.volume / mesh.vol[f.left];
+                    for (size_t k = 0; k < bb; ++k) MIt[k] = E.invmass[k] * iv;
+                    gemm_sq(b, 1.0, MIt.data(), LLb, W1);
                 }
-            const double iv = E.volume / mesh.vol[f.right];
-            for (size_t k = 0; k < MI.size(); ++k) MI[k] = E.invmass[k] * iv;
-            for (int d = 0; d < N; ++d) {
-                const double n = f.normal[d];
-                gemm_sq(b, n, MI.data(), RR.data(), W.data());  add_blk(G[d], f.right, f.right, 1.0, W.data());
-                gemm_sq(b, -n, MI.data(), RL.data(), W.data()); add_blk(G[d], f.right, f.left, 1.0, W.data());
             }
-            add_blk(T, f.right, f.right, tau0, RR.data());
-            add_blk(T, f.left, f.left, tau0, LL.data());
-            add_blk(T, f.right, f.left, -tau0, RL.data());
-            for (int i = 0; i < b; ++i) for (int j = 0; j < b; ++j) RLt[(size_t)i * b + j] = RL[(size_t)j * b + i];
-            add_blk(T, f.left, f.right, -tau0, RLt.data());
-        } else if (dirichlet(f)) {
-            boundary_phi(f, phi, w);
-            for (int i = 0; i < b; ++i)
-                for (int j = 0; j < b; ++j) {
-                    double s = 0;
-                    for (int q = 0; q < nq; ++q) s += phi[(size_t)q * b + i] * w[q] * phi[(size_t)q * b + j];
-                    LL[(size_t)i * b + j] = s;
+        });
+        for (long t = 0; t < c1 - c0; ++t) {
+            const SimplexFace& f = mesh.faces[c0 + t];
+            const double* LLb = &fbuf[(size_t)t * 5 * bb];
+            const double *RRb = LLb + bb, *RLb = LLb + 2 * bb, *W1 = LLb + 3 * bb, *W2 = LLb + 4 * bb;
+            if (f.right >= 0) {
+                for (int d = 0; d < N; ++d) {
+                    const double n = f.normal[d];
+                    add_blk(G[d], f.right, f.right, n, W1);
+                    add_blk(G[d], f.right, f.left, -n, W2);
                 }
-            const double iv = E.volume / mesh.vol[f.left];
-            for (size_t k = 0; k < MI.size(); ++k) MI[k] = E.invmass[k] * iv;
-            for (int d = 0; d < N; ++d) {
-                gemm_sq(b, -f.normal[d], MI.data(), LL.data(), W.data());
-                add_blk(G[d], f.left, f.left, 1.0, W.data());
+                add_blk(T, f.right, f.right, tau0, RRb);
+                add_blk(T, f.left, f.left, tau0, LLb);
+                add_blk(T, f.right, f.left, -tau0, RLb);
+                for (int i = 0; i < b; ++i) for (int j = 0; j < b; ++j) RLt[(size_t)i * b + j] = RLb[(size_t)j * b + i];
+                add_blk(T, f.left, f.right, -tau0, RLt.data());
+            } else if (dirichlet(f)) {
+                for (int d = 0; d < N; ++d) add_blk(G[d], f.left, f.left, -f.normal[d], W1);
+                add_blk(T, f.left, f.left, tauD, LLb);
             }
-            add_blk(T, f.left, f.left, tauD, LL.data());
         }
     }
     // boundary lifting blocks and face data for the RHS and its adjoint
     Lb.assign((size_t)nbf * b * nq, 0.0);
     bleft.resize(nbf); btype.resize(nbf); btag.resize(nbf); bnormal.assign((size_t)nbf * N, 0.0);
-    for (int k = 0; k < nbf; ++k) {
-        const SimplexFace& f = mesh.faces[mesh.bfaces[k]];
-        boundary_phi(f, phi, w);
-        double* L = &Lb[(size_t)k * b * nq];
-        for (int i = 0; i < b; ++i)
-            for (int q = 0; q < nq; ++q) L[(size_t)i * nq + q] = phi[(size_t)q * b + i] * w[q];
-        bleft[k] = f.left; btag[k] = -f.right; btype[k] = bctype[-f.right - 1];
-        for (int d = 0; d < N; ++d) bnormal[(size_t)k * N + d] = f.normal[d];
-    }
+    parallel_for(nbf, [&](long k0, long k1) {
+        std::vector<double> ph, wq;
+        for (long k = k0; k < k1; ++k) {
+            const SimplexFace& f = mesh.faces[mesh.bfaces[k]];
+            boundary_phi(f, ph, wq);
+            double* L = &Lb[(size_t)k * b * nq];
+            for (int i = 0; i < b; ++i)
+                for (int q = 0; q < nq; ++q) L[(size_t)i * nq + q] = ph[(size_t)q * b + i] * wq[q];
+            bleft[k] = f.left; btag[k] = -f.right; btype[k] = bctype[-f.right - 1];
+            for (int d = 0; d < N; ++d) bnormal[(size_t)k * N + d] = f.normal[d];
+        }
+    });
 }
 
 void SimplexLdg::bc_qpoints(double* xyz, int* tag) const
@@ -674,9 +718,41 @@ static void sym_eig(int n, const double* A, double* w, double* V)
 
 constexpr double kDrop = 1e-10;   // relative eigenvalue of a dropped coarse direction
 
-// C = [v_k / sqrt(w_k) for w_k > kDrop w_0, then zero columns]; returns the rank
+// An orthonormal basis of the span in the inner product G, as columns of C
+// (zero columns for dropped directions, kept ones first); returns the rank.
+// The Cholesky factor G = L L^T gives C = L^{-T} (C^T G C = I) whenever it
+// provably drops nothing: ||G^{-1}||_2 = ||C||_2^2 <= ||C||_F^2, so
+// 1/||C||_F^2 > kDrop ||G||_F >= kDrop w_max means every eigenvalue passes.
+// Otherwise the eigen-decomposition: C = [v_k / sqrt(w_k) for w_k > kDrop w_0].
 static int orth_basis(int n, const double* G, double* C)
 {
+    std::vector<double> Lc((size_t)n * n, 0.0);
+    bool ok = true;
+    for (int j = 0; j < n && ok; ++j) {
+        double d = G[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= Lc[(size_t)j * n + k] * Lc[(size_t)j * n + k];
+        if (!(d > 0.0)) { ok = false; break; }
+        const double ljj = std::sqrt(d);
+        Lc[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = G[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= Lc[(size_t)i * n + k] * Lc[(size_t)j * n + k];
+            Lc[(size_t)i * n + j] = v / ljj;
+        }
+    }
+    if (ok) {
+        // C = L^{-T}: solve L^T C = I column by column (C upper triangular)
+        std::fill(C, C + (size_t)n * n, 0.0);
+        for (int c = 0; c < n; ++c)
+            for (int i = c; i >= 0; --i) {
+                double v = (i == c) ? 1.0 : 0.0;
+                for (int k = i + 1; k <= c; ++k) v -= Lc[(size_t)k * n + i] * C[(size_t)k * n + c];
+                C[(size_t)i * n + c] = v / Lc[(size_t)i * n + i];
+            }
+        double gf = 0, cf = 0;
+        for (size_t t = 0; t < (size_t)n * n; ++t) { gf += G[t] * G[t]; cf += C[t] * C[t]; }
+        if (1.0 / cf > kDrop * std::sqrt(gf)) return n;
+    }
     std::vector<double> w(n), V((size_t)n * n);
     sym_eig(n, G, w.data(), V.data());
     int r = 0;
@@ -686,16 +762,35 @@ static int orth_basis(int n, const double* G, double* C)
     return r;
 }
 
-// C (m x n) = op(A) B, op = transpose if ta; A is (ta ? k x m : m x k), B k x n
-static void mm(bool ta, int m, int n, int k, const double* A, const double* B, double* C, bool acc = false)
+// C (m x n) = A B (+ C), A m x k, B k x n, row-major
+static void mm_nn(int m, int n, int k, const double* A, const double* B, double* C, bool acc = false)
 {
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j < n; ++j) {
-            double s = acc ? C[(size_t)i * n + j] : 0.0;
-            for (int t = 0; t < k; ++t) s += (ta ? A[(size_t)t * m + i] : A[(size_t)i * k + t]) * B[(size_t)t * n + j];
-            C[(size_t)i * n + j] = s;
+    for (int i = 0; i < m; ++i) {
+        double* c = C + (size_t)i * n;
+        if (!acc) std::fill(c, c + n, 0.0);
+        for (int t = 0; t < k; ++t) {
+            const double a = A[(size_t)i * k + t];
+            const double* b = B + (size_t)t * n;
+            for (int j = 0; j < n; ++j) c[j] += a * b[j];
         }
+    }
 }
+
+// C (m x n) = A^T B (+ C), A k x m, B k x n, row-major
+static void mm_tn(int m, int n, int k, const double* A, const double* B, double* C, bool acc = false)
+{
+    if (!acc) std::fill(C, C + (size_t)m * n, 0.0);
+    for (int t = 0; t < k; ++t) {
+        const double* a = A + (size_t)t * m;
+        const double* b = B + (size_t)t * n;
+        for (int i = 0; i < m; ++i) {
+            const double ai = a[i];
+            double* c = C + (size_t)i * n;
+            for (int j = 0; j < n; ++j) c[j] += ai * b[j];
+        }
+    }
+}
+
 
 void build_interps_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basis& Q,
                          const std::vector<std::vector<int>>& parts, std::vector<Interp>& Iv,
@@ -710,8 +805,6 @@ void build_interps_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basi
     // its orthonormal basis (empty on the mesh level)
     std::vector<double> Gp, Cp, lop, hip;
     int nprev = m.ne;
-    std::vector<double> W1((size_t)bc * bc), W2((size_t)bc * bc), Me((size_t)bf * bf);
-    int it[3];
     for (int l = 0; l < nlev; ++l) {
         const std::vector<int>& part = parts[l];
         if ((int)part.size() != nprev) throw Error(-1, "partition size does not match its level");
@@ -734,76 +827,98 @@ void build_interps_agglo(const SimplexMesh& m, const SimplexBasis& E, const Basi
         // reference blocks (the box basis at the finer nodes), one per finer row
         const int rb = (l == 0) ? bf : bc;
         std::vector<double> Eb((size_t)nprev * rb * bc);
-        for (int i = 0; i < nprev; ++i) {
-            const int a = part[i];
-            double* Ei = &Eb[(size_t)i * rb * bc];
-            if (l == 0) {
-                // T0 (:99-138): the fine simplex mapped into the box's [0,1]^N, then its nodes
-                double rp[4][3];
-                for (int v = 0; v < N + 1; ++v)
-                    for (int d = 0; d < N; ++d)
-                        rp[v][d] = (m.pts[(size_t)m.tris[(size_t)i * (N + 1) + v] * N + d] - lo[(size_t)a * N + d]) /
-                                   (hi[(size_t)a * N + d] - lo[(size_t)a * N + d]);
-                for (int q = 0; q < bf; ++q) {
-                    double x[3];
-                    for (int d = 0; d < N; ++d) {
-                        double sv = rp[0][d];
-                        for (int c = 0; c < N; ++c) sv += (rp[c + 1][d] - rp[0][d]) * E.nodes[(size_t)q * N + c];
-                        x[d] = sv;
+        parallel_for(nprev, [&](long i0, long i1) {
+            int it[3];
+            for (long i = i0; i < i1; ++i) {
+                const int a = part[i];
+                double* Ei = &Eb[(size_t)i * rb * bc];
+                if (l == 0) {
+                    // T0 (:99-138): the fine simplex mapped into the box's [0,1]^N, then its nodes
+                    double rp[4][3];
+                    for (int v = 0; v < N + 1; ++v)
+                        for (int d = 0; d < N; ++d)
+                            rp[v][d] = (m.pts[(size_t)m.tris[(size_t)i * (N + 1) + v] * N + d] - lo[(size_t)a * N + d]) /
+                                       (hi[(size_t)a * N + d] - lo[(size_t)a * N + d]);
+                    for (int q = 0; q < bf; ++q) {
+                        double x[3];
+                        for (int d = 0; d < N; ++d) {
+                            double sv = rp[0][d];
+                            for (int c = 0; c < N; ++c) sv += (rp[c + 1][d] - rp[0][d]) * E.nodes[(size_t)q * N + c];
+                            x[d] = sv;
+                        }
+                        Q.lagrangeN(x, Ei + (size_t)q * bc);
                     }
-                    Q.lagrangeN(x, Ei + (size_t)q * bc);
-                }
-            } else if (cnt[a] == 1) {
-                for (int k = 0; k < bc * bc; ++k) Ei[k] = (k / bc == k % bc) ? 1.0 : 0.0;   // identity (:155-161)
-            } else {
-                // box -> box (:164-199)
-                for (int q = 0; q < bc; ++q) {
-                    int lin = q;
-                    for (int d = N - 1; d >= 0; --d) { it[d] = lin % Q.P; lin /= Q.P; }
-                    double x[3];
-                    for (int d = 0; d < N; ++d) {
-                        const double wd = hi[(size_t)a * N + d] - lo[(size_t)a * N + d];
-                        const double rl = (lop[(size_t)i * N + d] - lo[(size_t)a * N + d]) / wd;
-                        const double rh = (hip[(size_t)i * N + d] - lo[(size_t)a * N + d]) / wd;
-                        x[d] = rl + (rh - rl) * Q.nodes[it[d]];
+                } else if (cnt[a] == 1) {
+                    for (int k = 0; k < bc * bc; ++k) Ei[k] = (k / bc == k % bc) ? 1.0 : 0.0;   // identity (:155-161)
+                } else {
+                    // box -> box (:164-199)
+                    for (int q = 0; q < bc; ++q) {
+                        int lin = q;
+                        for (int d = N - 1; d >= 0; --d) { it[d] = lin % Q.P; lin /= Q.P; }
+                        double x[3];
+                        for (int d = 0; d < N; ++d) {
+                            const double wd = hi[(size_t)a * N + d] - lo[(size_t)a * N + d];
+                            const double rl = (lop[(size_t)i * N + d] - lo[(size_t)a * N + d]) / wd;
+                            const double rh = (hip[(size_t)i * N + d] - lo[(size_t)a * N + d]) / wd;
+                            x[d] = rl + (rh - rl) * Q.nodes[it[d]];
+                        }
+                        Q.lagrangeN(x, Ei + (size_t)q * bc);
                     }
-                    Q.lagrangeN(x, Ei + (size_t)q * bc);
                 }
             }
+        });
+        // members of each agglomerate, in row order
+        std::vector<int> moff(nc + 1, 0), mem(nprev);
+        for (int v : part) moff[v + 1]++;
+        for (int c = 0; c < nc; ++c) moff[c + 1] += moff[c];
+        {
+            std::vector<int> pos(moff.begin(), moff.end() - 1);
+            for (int i = 0; i < nprev; ++i) mem[pos[part[i]]++] = i;
         }
-        // Gram matrices of the coarse box bases over the mesh
-        std::vector<double> G((size_t)nc * bc * bc, 0.0);
-        std::vector<double> Wf((size_t)rb * bc);
-        for (int i = 0; i < nprev; ++i) {
-            const double* Gi;
-            if (l == 0) {
-                const double sc = m.vol[i] / E.volume;
-                for (size_t k = 0; k < Me.size(); ++k) Me[k] = E.mass[k] * sc;
-                Gi = Me.data();
-            } else Gi = &Gp[(size_t)i * bc * bc];
-            const double* Ei = &Eb[(size_t)i * rb * bc];
-            mm(false, rb, bc, rb, Gi, Ei, Wf.data());
-            mm(true, bc, bc, rb, Ei, Wf.data(), &G[(size_t)part[i] * bc * bc], true);
-        }
-        std::vector<double> C((size_t)nc * bc * bc);
+        // Gram matrices of the coarse box bases over the mesh (sum over the
+        // finer rows of E^T G_fine E, G_fine = element mass on level 0) and
+        // their orthonormal bases
+        std::vector<double> G((size_t)nc * bc * bc), C((size_t)nc * bc * bc);
         rank[l].resize(nc);
-        for (int a = 0; a < nc; ++a) rank[l][a] = orth_basis(bc, &G[(size_t)a * bc * bc], &C[(size_t)a * bc * bc]);
+        parallel_for(nc, [&](long a0, long a1) {
+            std::vector<double> Wf((size_t)rb * bc), Me((size_t)bf * bf);
+            for (long a = a0; a < a1; ++a) {
+                double* Ga = &G[(size_t)a * bc * bc];
+                std::fill(Ga, Ga + (size_t)bc * bc, 0.0);
+                for (int t = moff[a]; t < moff[a + 1]; ++t) {
+                    const int i = mem[t];
+                    const double* Gi;
+                    if (l == 0) {
+                        const double sc = m.vol[i] / E.volume;
+                        for (size_t k = 0; k < Me.size(); ++k) Me[k] = E.mass[k] * sc;
+                        Gi = Me.data();
+                    } else Gi = &Gp[(size_t)i * bc * bc];
+                    const double* Ei = &Eb[(size_t)i * rb * bc];
+                    mm_nn(rb, bc, rb, Gi, Ei, Wf.data());
+                    mm_tn(bc, bc, rb, Ei, Wf.data(), Ga, true);
+                }
+                rank[l][a] = orth_basis(bc, Ga, &C[(size_t)a * bc * bc]);
+            }
+        });
         // rows in the orthonormal bases: E_i C_a from the mesh, (C_i^T G_i) E_i C_a above
         Interp& I = Iv[l];
         I.nf = nprev; I.nc = nc; I.bf = rb; I.bc = bc;
         I.parent.assign(part.begin(), part.end());
         I.type.resize(nprev);
-        I.pool.assign((size_t)nprev * rb * bc, 0.0);
+        I.pool.resize((size_t)nprev * rb * bc);
         I.ntypes = nprev;
-        for (int i = 0; i < nprev; ++i) {
-            const int a = part[i];
-            I.type[i] = i;
-            double* out = &I.pool[(size_t)i * rb * bc];
-            mm(false, rb, bc, bc, &Eb[(size_t)i * rb * bc], &C[(size_t)a * bc * bc], Wf.data());
-            if (l == 0) { std::copy(Wf.begin(), Wf.end(), out); continue; }
-            mm(true, bc, bc, bc, &Cp[(size_t)i * bc * bc], &Gp[(size_t)i * bc * bc], W2.data());
-            mm(false, bc, bc, bc, W2.data(), Wf.data(), out);
-        }
+        parallel_for(nprev, [&](long i0, long i1) {
+            std::vector<double> Wf((size_t)rb * bc), W2((size_t)bc * bc);
+            for (long i = i0; i < i1; ++i) {
+                const int a = part[i];
+                I.type[i] = (int)i;
+                double* out = &I.pool[(size_t)i * rb * bc];
+                if (l == 0) { mm_nn(rb, bc, bc, &Eb[(size_t)i * rb * bc], &C[(size_t)a * bc * bc], out); continue; }
+                mm_nn(rb, bc, bc, &Eb[(size_t)i * rb * bc], &C[(size_t)a * bc * bc], Wf.data());
+                mm_tn(bc, bc, bc, &Cp[(size_t)i * bc * bc], &Gp[(size_t)i * bc * bc], W2.data());
+                mm_nn(bc, bc, bc, W2.data(), Wf.data(), out);
+            }
+        });
         I.make_gather();
         Gp.swap(G); Cp.swap(C); lop.swap(lo); hip.swap(hi);
         for (int e = 0; e < m.ne; ++e) orig[e] = part[orig[e]];

@@ -399,6 +399,20 @@ def adjoint(mg, dLdu, tol=1e-12, maxit=500, criterion="true", flexible=False, pr
     return lam, dLdf, dLdg, (stats[0] if k == 1 else stats)
 
 
+def adjoint_maps(mg, lam):
+    """dL/df = M lam and dL/dg = B_g^T lam for adjoint states solved elsewhere
+    (device tensors [ndof(,k)]); returns (dLdf, dLdg)."""
+    import torch
+    k = 1 if lam.ndim == 1 else lam.shape[1]
+    lam = lam.contiguous()
+    dLdf = torch.empty_like(lam)
+    nq = mg.prob.nqb
+    dLdg = torch.empty((nq,) if k == 1 else (nq, k), dtype=torch.float64, device=lam.device)
+    torch.cuda.synchronize()
+    check(mg.ctx, lib().mgldg_adjoint_maps_device(mg.h, _ptr(lam), _ptr(dLdf), _ptr(dLdg), k))
+    return dLdf, dLdg
+
+
 def _make_autograd():
     import torch
 

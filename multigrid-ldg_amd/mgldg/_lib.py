@@ -66,6 +66,7 @@ SIGNATURES = {
     "mgldg_solve": (I, [V, PD, PD, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
     "mgldg_adjoint_device": (I, [V, V, V, V, V, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
     "mgldg_rhs_device": (I, [V, V, V, I, V]),
+    "mgldg_adjoint_maps_device": (I, [V, V, V, V, I]),
     "mgldg_spmv_device": (I, [V, I, I, V, V, I]),
     "mgldg_smooth_device": (I, [V, I, I, I, I, V, V, I]),
     "mgldg_vcycle_device": (I, [V, I, V, V, I]),

@@ -104,7 +104,7 @@ def test_autograd_gradients_against_finite_differences(gctx, oracle):
             assert abs(fd.item() - an.item()) <= bound * abs(an.item()), (eps, fd.item(), an.item())
 
 
-@pytest.mark.parametrize("case", ["quadtree_p3", "quadtree_p2", "simplex_p2"])
+@pytest.mark.parametrize("case", ["quadtree_p3", "quadtree_p2", "simplex_p2", "ball_p2"])
 def test_mfma_batched_kernels(gctx, oracle, case):
     """The MFMA block kernels (forced 1, 2, 4 tiles of 16 columns, and the
     autotuned plan) give the vector kernels' batched solves: block sizes 16
@@ -118,13 +118,19 @@ def test_mfma_batched_kernels(gctx, oracle, case):
         qt, pr, _, opr = build_pair(gctx, oracle, "adaptive", p, rapid=True)
         relax = "gs"
         mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, precision="mixed", coarse_dense_max=0)
-    else:
+    elif case == "simplex_p2":
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", "circle2.mesh")
         wf = mgldg.Wireframe.read(gctx, path)
         pr = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
         relax = "jacobi"
         mg = mgldg.Multigrid(pr, relax, omega=0.6, npre=2, npost=2, restriction="rat", precision="mixed",
                              coarse_dense_max=0)
+    else:   # tetrahedra: 10-wide fine blocks, 27-wide box blocks (two MFMA row tiles)
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", "ball.mesh")
+        wf = mgldg.Wireframe.read(gctx, path, N=3)
+        pr = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
+        relax = "gs"
+        mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, restriction="rat", precision="mixed", coarse_dense_max=0)
     rng = np.random.default_rng(31)
     k = 40                                         # a partial last tile
     B = torch.tensor(pr.rhs(rng.standard_normal((pr.ndof, k))), device="cuda")
@@ -139,7 +145,7 @@ def test_mfma_batched_kernels(gctx, oracle, case):
             res[(mode, prec)] = (X.cpu().numpy(), [s["iterations"] for s in st])
         if mode == "auto":
             plan = mg.batch_plan()
-            assert any(lv["apply"] > 0 or lv["smooth"] > 0 for lv in plan["fp32"]) or pr.npl not in (6, 8, 9, 10, 16)
+            assert any(lv["apply"] > 0 or lv["smooth"] > 0 for lv in plan["fp64"] + plan["fp32"])
     for prec in ("fp64", "mixed"):
         X0, it0 = res[("vector", prec)]
         for mode in (1, 2, 4, "auto"):
