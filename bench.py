@@ -61,6 +61,32 @@ def cpu_model():
     return platform.processor()
 
 
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X dense matrix-core peaks
+
+
+def mfma_use(mg, plan, k):
+    """Utilisation of the matrix cores by the fine-level batched operator
+    application (y = A x on k columns), from the autotuner's HIP-event timing
+    of the chosen kernel: useful FLOPs nnzb * 2 B^2 k (the 16 x 16 padding of
+    narrower blocks not counted) over time, against the dense MFMA peak; and
+    the algorithmic bytes (A once, x and y) over the same time."""
+    out = {}
+    nnzb, B, nb = mg.level_nnzb[0], mg.level_B[0], mg.level_rows[0]
+    for prec, size in (("fp64", 8), ("fp32", 4)):
+        lv = plan[prec][0]
+        if lv["apply"] == 0:
+            out[prec] = {"kernel": "vector (MFMA slower here)"}
+            continue
+        ms = lv["apply_ms"][{1: 1, 2: 2, 4: 3}[lv["apply"]]]
+        flops = nnzb * 2.0 * B * B * k
+        byts = nnzb * B * B * size + 4.0 * nnzb + 2.0 * nb * B * k * size
+        tf = flops / (ms * 1e-3) / 1e12
+        out[prec] = {"kernel": f"mf_apply<{'float' if size == 4 else 'double'},{B},{lv['apply']}>", "ms": round(ms, 4),
+                     "tflops": round(tf, 2), "peak_tflops": MFMA_PEAK_TFLOPS[prec],
+                     "frac": round(tf / MFMA_PEAK_TFLOPS[prec], 4), "gbs": round(byts / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def plan_summary(plan):
     """the autotuner's per-level choices: 0 = vector kernels, n = MFMA tiles of
     16 x n columns, with the fine level's timings (ms: vector, 1, 2, 4 tiles)"""
@@ -352,7 +378,7 @@ def ball_leg(mgldg, torch, nrhs=16, warm=1):
                                  "iterations": st["iterations"]}
         log(f"ball single {prec}: {dt * 1e3:.1f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {st['iterations']}")
     bkw = dict(kw, precision="mixed", flexible=True, use_graph=True)
-    mgldg.pcg(mg, Ball[:, :2].contiguous(), **bkw)            # plan the batch kernels on a small batch first
+    mgldg.pcg(mg, Ball, torch.zeros_like(Ball), **dict(bkw, maxit=2))   # batch buffers, kernel plan, graph: not timed
     torch.cuda.synchronize()
     t = time.perf_counter()
     Xall, sts = mgldg.pcg(mg, Ball, torch.zeros_like(Ball), **bkw)
@@ -384,7 +410,7 @@ def ball_leg(mgldg, torch, nrhs=16, warm=1):
         "mixed_batched_vs_fp64_single": round(out["batched_mixed"]["value"] / out["single_fp64"]["value"], 3),
         "mixed_batched_vs_fp64_single_rel_diff": diff,
         "adjoint_check_rel_err": adj_err,
-        "kernel_plan": plan_summary(mg.batch_plan()),
+        "kernel_plan": plan_summary(mg.batch_plan()), "mfma_fine_apply": mfma_use(mg, mg.batch_plan(), 2 * nrhs),
         "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
                     "hierarchy": round(t_mg, 3)},
     }
@@ -583,6 +609,7 @@ def main():
 
     unstructured = None if args.no_unstructured else unstructured_leg(mgldg, torch, k, args.warmup)
     adjoint = None if args.no_adjoint else adjoint_leg(mgldg, torch, warm=args.warmup)
+    ball = None if args.no_ball else ball_leg(mgldg, torch, warm=args.warmup)
 
     line = {
         "metric": "DoF-solves/s to 1e-10 residual, mixed vs FP64 MG-PCG; smoother HBM GB/s",
@@ -611,13 +638,14 @@ def main():
                     "mixed": {"value": round(vb32, 1), "ms_per_step": round(dtb32 / bsteps * 1e3, 3),
                               "max_iterations": max(s["iterations"] for s in stb32)},
                     "mixed_batched_vs_fp64_single": round(vb32 / v64, 3),
-                    "kernel_plan": plan_summary(plan)},
+                    "kernel_plan": plan_summary(plan), "mfma_fine_apply": mfma_use(mg, plan, k)},
         "error_vs_fp64_oracle": err,
         "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "unstructured": unstructured,
         "adjoint": adjoint,
+        "ball": ball,
     }
     print(json.dumps(line), flush=True)
 
