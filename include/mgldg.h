@@ -270,11 +270,18 @@ int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, 
  * `nrep` smoother sweeps (all colours) or SpMVs on level l with HIP events on
  * the context stream and reports the average duration of one kernel launch
  * (ms), the launches timed and the algorithmic HBM bytes one launch moves:
- *   GS colour launch: off-diagonal blocks + inverse diagonal blocks + column
- *   and row indices + b, x (read) and x (write) of its rows, per colour.
+ *   GS launch: off-diagonal blocks and column indices of its rows, their
+ *   inverse diagonal blocks and row descriptors, b, x (read) and x (write)
+ *   of its rows, and x of their distinct neighbours.
+ *   Jacobi sweep: all blocks + inverse diagonal blocks + indices + b, x, x_out.
  *   SpMV: all blocks + indices + x (read) + y (write). */
 int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double* ms_per_launch,
                         double* bytes_per_launch, int* launches);
+/* The same for one part of a multicolour GS sweep: part 0 = whole sweeps
+ * (as mgldg_profile_relax), 1 = the one-colour launches only, 2 = the launch
+ * that does the last two colours together (levels where it exists). */
+int mgldg_profile_relax_part(mgldg_mg mg, int level, int precision, int nrep, int part, double* ms_per_launch,
+                             double* bytes_per_launch, int* launches);
 int mgldg_profile_spmv(mgldg_mg mg, int level, int precision, int nrep, double* ms_per_launch,
                        double* bytes_per_launch);
 /* setup timings: t[0] host assembly, t[1] device assembly (A), t[2] hierarchy

@@ -556,6 +556,21 @@ int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double*
     });
 }
 
+int mgldg_profile_relax_part(mgldg_mg mg, int level, int precision, int nrep, int part, double* ms, double* bytes,
+                             int* launches)
+{
+    if (!mg || nrep < 1) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        ARG(part >= 0 && part <= 2, "part must be 0, 1 or 2");
+        if (precision == 1) ARG(mg->m.L[level].A32.p, "mixed precision not built");
+        ARG(part == 0 || mg->m.prm.relaxation == 1, "parts exist for multicolour Gauss-Seidel only");
+        ARG(part != 2 || mg->m.L[level].pair_nwg > 0, "level has no paired launch");
+        mg->m.profile_relax(level, precision, nrep, ms, bytes, launches, part);
+    });
+}
+
 int mgldg_profile_spmv(mgldg_mg mg, int level, int precision, int nrep, double* ms, double* bytes)
 {
     if (!mg || nrep < 1) return MGLDG_ERR_ARG;

@@ -247,91 +247,149 @@ void gather_blocks(hipStream_t s, int n, int bs, const int* idx, const double* i
 }
 
 // --------------------------------------------------------------------- solve kernels
-template <typename T, int B>
-__device__ __forceinline__ T row_dot(const T* __restrict__ Ab, const T* __restrict__ xj)
-{
-    constexpr int V = Map<T, B>::V;
-    T acc[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) acc[v] = 0;
-#pragma unroll
-    for (int cc = 0; cc < B / V; ++cc) {
-        VecT<T, V> a = ldv<T, V>(Ab + cc * B * V);
-        VecT<T, V> xv = ldv<T, V>(xj + cc * V);
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] += a.v[v] * xv.v[v];
-    }
-    T s = acc[0];
-#pragma unroll
-    for (int v = 1; v < V; ++v) s += acc[v];
-    return s;
-}
-
-// Blocks kept in flight per lane: loads of up to UNR blocks of a row are
-// issued before their FMAs (one HBM round trip per UNR blocks instead of one
-// per block).  Sized so that A and x operands stay near 32-64 VGPRs per block.
+// Blocks kept in flight per lane: the A operands of up to U blocks of a row
+// are loaded before any of them is used (one HBM round trip per U blocks).
+// Sized so that the A operands stay near 64 VGPRs.
 template <typename T, int B>
 __host__ __device__ constexpr int unroll_of()
 {
     return sizeof(T) == 4 ? (B <= 16 ? 4 : (B <= 27 ? 2 : 1)) : (B <= 10 ? 4 : (B <= 16 ? 2 : 1));
 }
 
-// Lane r of the row owning block row i returns sum_k A_k[r][:] . x_{col(k)}
-// over the blocks of the row (FILTER: without the diagonal block and, with
-// skip = 1 / 2, without columns whose colour comes after `cur` in sweep
-// order).  The row's column indices are loaded once (one per lane) and
-// broadcast with lane shuffles; the trip count is uniform across the
-// wavefront so every shuffle runs with all lanes active.
-template <typename T, int B, bool FILTER>
-__device__ __forceinline__ T row_sum(bool valid, int i, int k0, int k1, const int* __restrict__ cols,
-                                     const T* __restrict__ A, const T* __restrict__ x, int r, int lane0,
-                                     const int* __restrict__ color, int cur, int skip)
+// LDS exchange among the lanes of one wavefront: every lane that writes and
+// reads a staging area here belongs to the same wavefront, so ordering within
+// the wavefront is all that is needed (no workgroup barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// position of the m-th summed block of a row: k0 + m, stepping over the
+// diagonal block at dp (dp < 0: no block is stepped over)
+__device__ __forceinline__ int blk_at(int k0, int m, int dp)
+{
+    const int k = k0 + m;
+    return (dp >= 0 && k >= dp) ? k + 1 : k;
+}
+
+// Lane r of a block row returns sum_m A_k(m)[r][:] . x_col(k(m)) over the
+// row's blocks k0 .. k1-1 other than dp.  Latency is what bounds these
+// launches (one colour of a sweep is a few MB to a few hundred MB), so:
+//  * SKIP = 0: the A operands of a round of U blocks are loaded before the
+//    column indices arrive (their addresses follow from k0 and dp alone);
+//  * lane r loads entry r of x_j (one coalesced B-vector per block) and
+//    stages it in LDS (xs: U * B values per row), from where the B lanes of
+//    the row read it back as broadcast vectors: x costs one register per
+//    block in flight instead of B;
+//  * SKIP = 1 (2): first sweep from x = 0; x_j of colours after (before)
+//    `cur` in sweep order are still zero, and their blocks are not read.
+// The column indices of up to B blocks are loaded at once, one per lane, and
+// broadcast with lane shuffles; trip counts are uniform across the
+// wavefront, so every shuffle runs with all lanes active.
+template <typename T, int B, int SKIP>
+__device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const int* __restrict__ cols,
+                                     const T* __restrict__ A, const T* x, int r, int lane0, T* xs,
+                                     const int* __restrict__ color, int cur)
 {
     constexpr int V = Map<T, B>::V;
     constexpr int U = unroll_of<T, B>();
     T acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0;
-    const int len = valid ? k1 - k0 : 0;
-    int wl = len;
+    const int n = valid ? k1 - k0 - (dp >= 0 ? 1 : 0) : 0;
+    int wl = n;
     for (int o = 32; o > 0; o >>= 1) wl = max(wl, __shfl_xor(wl, o, 64));
-    for (int kc = 0; kc < wl; kc += B) {
-        const int mycol = (r < len - kc) ? cols[k0 + kc + r] : -1;
-        const int nc = min(B, wl - kc);
+    for (int m0 = 0; m0 < wl; m0 += B) {
+        const int mycol = (r < n - m0) ? cols[blk_at(k0, m0 + r, dp)] : -1;
+        const int nc = min(B, wl - m0);
         for (int u0 = 0; u0 < nc; u0 += U) {
-            VecT<T, V> a[U][B / V], xv[U][B / V];
+            VecT<T, V> a[U][B / V];
             bool use[U];
+            if (SKIP == 0) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int kk = u0 + u;
-                const int j = __shfl(mycol, lane0 + (kk < B ? kk : B - 1), 64);
-                bool ok = kk < nc && j >= 0;
-                if (FILTER && ok) {
-                    ok = j != i;
-                    if (ok && skip) { const int cj = color[j]; ok = (skip == 1) ? cj <= cur : cj >= cur; }
-                }
-                use[u] = ok;
-                if (ok) {
-                    const T* Ab = A + (size_t)(k0 + kc + kk) * (B * B) + r * V;
-                    const T* xj = x + (size_t)j * B;
+                for (int u = 0; u < U; ++u) {
+                    const int m = m0 + u0 + u;
+                    use[u] = u0 + u < nc && m < n;
+                    if (use[u]) {
+                        const T* Ab = A + (size_t)blk_at(k0, m, dp) * (B * B) + r * V;
 #pragma unroll
-                    for (int cc = 0; cc < B / V; ++cc) { a[u][cc] = ldv<T, V>(Ab + cc * B * V); xv[u][cc] = ldv<T, V>(xj + cc * V); }
+                        for (int cc = 0; cc < B / V; ++cc) a[u][cc] = ldv<T, V>(Ab + cc * B * V);
+                    }
                 }
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = __shfl(mycol, lane0 + min(u0 + u, B - 1), 64);
+                bool ok = valid && u0 + u < nc && j >= 0;
+                if (SKIP != 0) {
+                    if (ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
+                    use[u] = ok;
+                    if (ok) {
+                        const T* Ab = A + (size_t)blk_at(k0, m0 + u0 + u, dp) * (B * B) + r * V;
+#pragma unroll
+                        for (int cc = 0; cc < B / V; ++cc) a[u][cc] = ldv<T, V>(Ab + cc * B * V);
+                    }
+                }
+                xs[u * B + r] = ok ? x[(size_t)j * B + r] : (T)0;
+            }
+            wave_sync();
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (use[u]) {
 #pragma unroll
-                    for (int cc = 0; cc < B / V; ++cc)
+                    for (int cc = 0; cc < B / V; ++cc) {
+                        const VecT<T, V> xv = ldv<T, V>(xs + u * B + cc * V);
 #pragma unroll
-                        for (int v = 0; v < V; ++v) acc[v] += a[u][cc].v[v] * xv[u][cc].v[v];
+                        for (int v = 0; v < V; ++v) acc[v] += a[u][cc].v[v] * xv.v[v];
+                    }
                 }
+            wave_sync();
         }
     }
     T s = acc[0];
 #pragma unroll
     for (int v = 1; v < V; ++v) s += acc[v];
     return s;
+}
+
+// Per-wavefront LDS of the row kernels: RW row slots (the last one for the
+// lanes past 64/B rows, which hold no row) of U * B staged x values and B
+// values for the D^-1 product.
+template <typename T, int B>
+struct RowLds {
+    static constexpr int RW = Map<T, B>::RPW + 1;
+    static constexpr int XS = unroll_of<T, B>() * B;
+};
+
+// s_r = sum_c Dinv[r][c] t_c with lane r's row of Dinv in registers and t in LDS
+template <typename T, int B>
+__device__ __forceinline__ T dinv_row(const VecT<T, Map<T, B>::V>* dv, const T* t)
+{
+    constexpr int V = Map<T, B>::V;
+    T s[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) s[v] = 0;
+#pragma unroll
+    for (int cc = 0; cc < B / V; ++cc) {
+        const VecT<T, V> tv = ldv<T, V>(t + cc * V);
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[v] += dv[cc].v[v] * tv.v[v];
+    }
+    T out = s[0];
+#pragma unroll
+    for (int v = 1; v < V; ++v) out += s[v];
+    return out;
+}
+
+template <typename T, int B>
+__device__ __forceinline__ void load_dinv_row(const T* __restrict__ Dinv, int i, int r, VecT<T, Map<T, B>::V>* dv)
+{
+    constexpr int V = Map<T, B>::V;
+    const T* Db = Dinv + (size_t)i * (B * B) + r * V;
+#pragma unroll
+    for (int cc = 0; cc < B / V; ++cc) dv[cc] = ldv<T, V>(Db + cc * B * V);
 }
 
 // y = A x  or  y = b - A x ; optional deterministic reduction of y.dotv
@@ -341,47 +399,104 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
                                                T* __restrict__ y, const T* __restrict__ dotv, double* part)
 {
     using M = Map<T, B>;
+    using R = RowLds<T, B>;
+    __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, r = lane - slot * B;
     const int row = blockIdx.x * M::RPB + wave * M::RPW + slot;
     const bool valid = slot < M::RPW && row < nrows;
     const int k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
-    const T acc = row_sum<T, B, false>(valid, row, k0, k1, cols, A, x, r, slot * B, nullptr, 0, 0);
+    const size_t o = (size_t)row * B + r;
+    const T bo = (valid && b) ? b[o] : (T)0;
+    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, cols, A, x, r, slot * B,
+                                   xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
     double dv = 0;
     if (valid) {
-        const size_t o = (size_t)row * B + r;
-        T out = b ? (T)(b[o] - acc) : acc;
+        T out = b ? (T)(bo - acc) : acc;
         y[o] = out;
         if (dotv) dv = (double)out * (double)dotv[o];
     }
     if (part) block_partial(dv, part);
 }
 
-template <typename T, int B>
-__global__ __launch_bounds__(256) void k_gs(int nrows, const int* __restrict__ rows, const int* __restrict__ rowptr,
-                                            const int* __restrict__ cols, const T* __restrict__ A,
-                                            const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
-                                            const int* __restrict__ color, int cur, int skip)
+// Damped block GS on rows [beg, end) of a descriptor list, one block row per
+// B lanes.  desc[idx] = {row i, rowptr[i], rowptr[i+1], position of the
+// diagonal block}: the row's operands (b_i, x_i, its row of D^-1, its A
+// blocks) are all addressable from that one load.  The rows must be
+// independent (one colour).
+template <typename T, int B, int SKIP>
+__device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict__ desc, const int* __restrict__ cols,
+                                        const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                        T* x, T omega, const int* __restrict__ color, int cur, T* xs_all, T* tb)
 {
-    // skip = 1: first sweep from x = 0, forward: x_j of colours > cur (and x_i)
-    // are still zero; skip = 2: the same in reverse (colours < cur).  Their
-    // blocks contribute exactly 0 and are not read.
     using M = Map<T, B>;
-    __shared__ T tb[M::RPB * B];
+    using R = RowLds<T, B>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / B, r = lane - slot * B, lr = wave * M::RPW + slot;
-    const int idx = blockIdx.x * M::RPB + lr;
-    const bool valid = slot < M::RPW && idx < nrows;
-    const int i = valid ? rows[idx] : 0;
-    const int k0 = valid ? rowptr[i] : 0, k1 = valid ? rowptr[i + 1] : 0;
-    const T acc = row_sum<T, B, true>(valid, i, k0, k1, cols, A, x, r, slot * B, color, cur, skip);
-    if (valid) tb[lr * B + r] = b[(size_t)i * B + r] - acc;
-    __syncthreads();
+    const int slot = lane / B, r = lane - slot * B;
+    const int idx = beg + wave * M::RPW + slot;
+    const bool valid = slot < M::RPW && idx < end;
+    int4 d = make_int4(0, 0, 0, -1);
+    if (valid) d = desc[idx];
+    const size_t o = (size_t)d.x * B + r;
+    T bi = 0, xi = 0;
+    VecT<T, M::V> dv[B / M::V];
     if (valid) {
-        const T s = row_dot<T, B>(Dinv + (size_t)i * (B * B) + r * M::V, tb + lr * B);
-        const size_t o = (size_t)i * B + r;
-        const T xi = skip ? (T)0 : x[o];
+        bi = b[o];
+        if (SKIP == 0) xi = x[o];
+        load_dinv_row<T, B>(Dinv, d.x, r, dv);
+    }
+    const T acc = row_sum<T, B, SKIP>(valid, d.y, d.z, d.w, cols, A, x, r, slot * B,
+                                      xs_all + (wave * R::RW + slot) * R::XS, color, cur);
+    T* t = tb + (wave * R::RW + slot) * B;
+    t[r] = bi - acc;
+    wave_sync();
+    if (valid) {
+        const T s = dinv_row<T, B>(dv, t);
         x[o] = xi + omega * (s - xi);
+    }
+}
+
+// One colour of a damped block GS sweep (rows desc[0 .. nrows)).
+template <typename T, int B, int SKIP>
+__global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ desc, const int* __restrict__ cols,
+                                            const T* __restrict__ A, const T* __restrict__ Dinv,
+                                            const T* __restrict__ b, T* x, T omega,
+                                            const int* __restrict__ color, int cur)
+{
+    using R = RowLds<T, B>;
+    __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
+    __shared__ __align__(16) T tb[4 * R::RW * B];
+    const int beg = blockIdx.x * Map<T, B>::RPB;
+    gs_rows<T, B, SKIP>(beg, nrows, desc, cols, A, Dinv, b, x, omega, color, cur, xs_all, tb);
+}
+
+// The last two colours of a sweep in one launch.  Workgroup w owns the
+// descriptor ranges [s, m) (rows of the second-to-last colour) and [m, e)
+// (rows of the last colour), {s, m, e} = wg[3w .. 3w+2].  Every row of the
+// last colour shares a workgroup with all its neighbours of the
+// second-to-last colour (clusters of the coupling graph between the two
+// colours), so processing the first range, a workgroup barrier, then the
+// second (the reverse for a backward sweep) gives exactly the two
+// one-colour launches in sweep order.  Workgroups holding only rows with no
+// neighbour in the last colour have m = e.  cur0 / cur1: colour of the
+// first / second range processed.
+template <typename T, int B, int SKIP>
+__global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int reverse, const int4* __restrict__ desc,
+                                                 const int* __restrict__ cols, const T* __restrict__ A,
+                                                 const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
+                                                 const int* __restrict__ color, int cur0, int cur1)
+{
+    using M = Map<T, B>;
+    using R = RowLds<T, B>;
+    __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
+    __shared__ __align__(16) T tb[4 * R::RW * B];
+    const int s = wg[3 * blockIdx.x], m = wg[3 * blockIdx.x + 1], e = wg[3 * blockIdx.x + 2];
+    for (int ph = 0; ph < 2; ++ph) {
+        const bool lo = (ph == 0) != (reverse != 0);
+        const int beg = lo ? s : m, end = lo ? m : e;
+        if (ph == 1 && m != s && m != e) __syncthreads();   // (uniform: s, m, e are per workgroup)
+        for (int base = beg; base < end; base += M::RPB)
+            gs_rows<T, B, SKIP>(base, end, desc, cols, A, Dinv, b, x, omega, color, ph ? cur1 : cur0, xs_all, tb);
     }
 }
 
@@ -392,24 +507,28 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
                                                 T omega)
 {
     using M = Map<T, B>;
-    __shared__ T tb[M::RPB * B];
+    using R = RowLds<T, B>;
+    __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
+    __shared__ __align__(16) T tb[4 * R::RW * B];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / B, r = lane - slot * B, lr = wave * M::RPW + slot;
-    const int i = blockIdx.x * M::RPB + lr;
+    const int slot = lane / B, r = lane - slot * B;
+    const int i = blockIdx.x * M::RPB + wave * M::RPW + slot;
     const bool valid = slot < M::RPW && i < nrows;
+    const int k0 = valid ? rowptr[i] : 0, k1 = valid ? rowptr[i + 1] : 0;
+    const size_t o = (size_t)i * B + r;
+    T bi = 0, xi = 0;
+    VecT<T, M::V> dv[B / M::V];
     if (valid) {
-        T acc = 0;
-        const int k1 = rowptr[i + 1];
-        for (int k = rowptr[i]; k < k1; ++k)
-            acc += row_dot<T, B>(A + (size_t)k * (B * B) + r * M::V, x + (size_t)cols[k] * B);
-        tb[lr * B + r] = b[(size_t)i * B + r] - acc;
+        bi = b[o];
+        xi = x[o];
+        load_dinv_row<T, B>(Dinv, i, r, dv);
     }
-    __syncthreads();
-    if (valid) {
-        const T s = row_dot<T, B>(Dinv + (size_t)i * (B * B) + r * M::V, tb + lr * B);
-        const size_t o = (size_t)i * B + r;
-        xout[o] = x[o] + omega * s;
-    }
+    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, cols, A, x, r, slot * B,
+                                   xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
+    T* t = tb + (wave * R::RW + slot) * B;
+    t[r] = bi - acc;
+    wave_sync();
+    if (valid) xout[o] = xi + omega * dinv_row<T, B>(dv, t);
 }
 
 // Interpolation blocks are BF x BC row-major (fine simplex or box basis x
@@ -543,16 +662,50 @@ void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int
 }
 
 template <typename T>
-void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* rows, const int* rowptr, const int* cols, const T* A,
-                      const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
+void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* cols, const T* A, const T* Dinv,
+                      const T* b, T* x, T omega, const int* color, int cur, int skip)
 {
     if (nrows <= 0) return;
+    const int4* d = reinterpret_cast<const int4*>(desc);
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_gs<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rows, rowptr, cols, A, Dinv, b,
-                           x, omega, color, cur, skip);
+        const dim3 g(nblk(nrows, M::RPB));
+        if (skip == 1)
+            hipLaunchKernelGGL((k_gs<T, BB, 1>), g, dim3(256), 0, s, nrows, d, cols, A, Dinv, b, x, omega, color, cur);
+        else if (skip == 2)
+            hipLaunchKernelGGL((k_gs<T, BB, 2>), g, dim3(256), 0, s, nrows, d, cols, A, Dinv, b, x, omega, color, cur);
+        else
+            hipLaunchKernelGGL((k_gs<T, BB, 0>), g, dim3(256), 0, s, nrows, d, cols, A, Dinv, b, x, omega, color, cur);
     });
     check_launch("gs_color");
+}
+
+template <typename T>
+void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* cols,
+                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0, int cur1, int skip)
+{
+    if (nwg <= 0) return;
+    const int4* d = reinterpret_cast<const int4*>(desc);
+    MGLDG_DISPATCH(B, {
+        const dim3 g(nwg);
+        if (skip == 1)
+            hipLaunchKernelGGL((k_gs_pair<T, BB, 1>), g, dim3(256), 0, s, wg, reverse, d, cols, A, Dinv, b, x, omega, color,
+                               cur0, cur1);
+        else if (skip == 2)
+            hipLaunchKernelGGL((k_gs_pair<T, BB, 2>), g, dim3(256), 0, s, wg, reverse, d, cols, A, Dinv, b, x, omega, color,
+                               cur0, cur1);
+        else
+            hipLaunchKernelGGL((k_gs_pair<T, BB, 0>), g, dim3(256), 0, s, wg, reverse, d, cols, A, Dinv, b, x, omega, color,
+                               cur0, cur1);
+    });
+    check_launch("gs_pair");
+}
+
+int rows_per_workgroup(int B)
+{
+    int r = 0;
+    MGLDG_DISPATCH(B, { r = Map<float, BB>::RPB; });
+    return r;
 }
 
 template <typename T>

@@ -25,6 +25,9 @@ void to_panel_f32(hipStream_t s, long nblk, int b, const double* in, float* out)
 // gather diagonal blocks: out[i] = in[diag[i]]
 void gather_blocks(hipStream_t s, int n, int bs, const int* idx, const double* in, double* out);
 
+// block rows one 256-thread workgroup of the single-RHS row kernels handles per pass
+int rows_per_workgroup(int B);
+
 // ---------------------------------------------------------------- solve (T = float | double)
 template <typename T>
 struct Ops {
@@ -33,10 +36,19 @@ struct Ops {
     static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
                       const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
     // damped block Gauss-Seidel on the rows of colour `cur` (multigrid.h:240-249).
-    // skip = 1 (2): first forward (reverse) sweep from x = 0, where the
-    // colours after `cur` in sweep order are still zero and are not read.
-    static void gs_color(hipStream_t s, int B, int nrows, const int* rows, const int* rowptr, const int* cols,
-                         const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
+    // desc: 4 ints per row of the colour, {row, rowptr[row], rowptr[row+1],
+    // position of the diagonal block}.  skip = 1 (2): first forward (reverse)
+    // sweep from x = 0, where the colours after `cur` in sweep order are still
+    // zero and are not read.
+    static void gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* cols, const T* A, const T* Dinv,
+                         const T* b, T* x, T omega, const int* color, int cur, int skip);
+    // the last two colours of a sweep in one launch of nwg workgroups:
+    // workgroup w does desc ranges [s, m) then [m, e) ({s, m, e} = wg[3w..]),
+    // the reverse for a backward sweep, with a workgroup barrier between;
+    // cur0 / cur1 = colour of the first / second range processed
+    static void gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* cols,
+                        const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0, int cur1,
+                        int skip);
     // damped block Jacobi, x_out = x + omega Dinv (b - A x) (multigrid.h:223-229)
     static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);

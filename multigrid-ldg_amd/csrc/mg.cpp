@@ -379,6 +379,74 @@ static void coarsen(hipStream_t s, int N, Level& f, Level& c, int restriction)
 }
 
 // solve layouts, inverse diagonal blocks, colouring, vectors
+// The last two colours a = nc-2, b = nc-1 of a multicolour level in one
+// launch.  Rows of colour b are few (greedy colouring leaves the last colour
+// to the rows that see all the others); each is grouped, by union-find over
+// the a-b couplings, with all its colour-a neighbours into a cluster that one
+// workgroup processes in two ordered ranges.  Colour-a rows with no colour-b
+// neighbour need no ordering and fill plain workgroups.  Used only when every
+// cluster fits one pass of a workgroup per range; otherwise the level keeps
+// one launch per colour.
+static void pair_schedule(Level& lv, hipStream_t s)
+{
+    lv.pair_nwg = 0;
+    if (lv.ncolor < 3) return;
+    const int n = lv.nb, ca = lv.ncolor - 2, cb = lv.ncolor - 1;
+    const int rpb = dev::rows_per_workgroup(lv.B);
+    const HostBSR& P = lv.A.pat;
+    std::vector<int> par(n);
+    for (int i = 0; i < n; ++i) par[i] = i;
+    auto find = [&](int a) { while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; } return a; };
+    std::vector<char> in(n, 0);
+    for (int i = 0; i < n; ++i) {
+        if (lv.color[i] != cb) continue;
+        in[i] = 1;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+            const int j = P.cols[k];
+            if (j != i && lv.color[j] == ca) { in[j] = 1; const int ri = find(i), rj = find(j); if (ri != rj) par[std::max(ri, rj)] = std::min(ri, rj); }
+        }
+    }
+    // clusters in order of their smallest row, rows ascending in each range
+    std::vector<int> cid(n, -1);
+    std::vector<std::vector<int>> ra, rb;
+    std::vector<int> plain;
+    for (int i = 0; i < n; ++i) {
+        if (lv.color[i] != ca && lv.color[i] != cb) continue;
+        if (!in[i]) { plain.push_back(i); continue; }
+        const int r = find(i);
+        if (cid[r] < 0) { cid[r] = (int)ra.size(); ra.emplace_back(); rb.emplace_back(); }
+        (lv.color[i] == ca ? ra : rb)[cid[r]].push_back(i);
+    }
+    for (size_t c = 0; c < ra.size(); ++c)
+        if ((int)ra[c].size() > rpb || (int)rb[c].size() > rpb) return;
+    std::vector<int> order, wg;
+    order.reserve(plain.size() + 2 * ra.size());
+    for (size_t q = 0; q < plain.size(); q += rpb) {
+        const int a = (int)q, e = (int)std::min(plain.size(), q + rpb);
+        wg.insert(wg.end(), {a, e, e});
+    }
+    order = plain;
+    for (size_t c = 0; c < ra.size(); ++c) {
+        const int a = (int)order.size();
+        order.insert(order.end(), ra[c].begin(), ra[c].end());
+        const int m = (int)order.size();
+        order.insert(order.end(), rb[c].begin(), rb[c].end());
+        wg.insert(wg.end(), {a, m, (int)order.size()});
+    }
+    std::vector<int> desc(4 * order.size());
+    for (size_t q = 0; q < order.size(); ++q) {
+        const int i = order[q];
+        int dp = -1;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+            if (P.cols[k] == i) { dp = k; break; }
+        desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1]; desc[4 * q + 3] = dp;
+    }
+    lv.pair_desc.upload(desc.data(), desc.size(), s);
+    lv.pair_wg.upload(wg.data(), wg.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+    lv.pair_nwg = (int)(wg.size() / 3);
+}
+
 static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicographic)
 {
     const int b = lv.B;
@@ -397,14 +465,25 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         lv.D32.alloc((size_t)lv.nb * b * b);
         dev::to_panel_f32(s, lv.nb, b, lv.Dinv_rm, lv.D32);
     }
-    auto group = [&](const std::vector<int>& color, int nc, std::vector<int>& off, DBuf<int>& rows_d, DBuf<int>& col_d) {
+    auto group = [&](const std::vector<int>& color, int nc, std::vector<int>& off, DBuf<int>& rows_d, DBuf<int>& col_d,
+                     DBuf<int>& desc_d) {
         off.assign(nc + 1, 0);
         for (int c : color) off[c + 1]++;
         for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
         std::vector<int> rows(lv.nb), pos(off.begin(), off.end() - 1);
         for (int i = 0; i < lv.nb; ++i) rows[pos[color[i]]++] = i;
+        const HostBSR& P = lv.A.pat;
+        std::vector<int> desc((size_t)4 * lv.nb);
+        for (int q = 0; q < lv.nb; ++q) {
+            const int i = rows[q];
+            int dp = -1;
+            for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+                if (P.cols[k] == i) { dp = k; break; }
+            desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1]; desc[4 * q + 3] = dp;
+        }
         rows_d.upload(rows.data(), rows.size(), s);
         col_d.upload(color.data(), color.size(), s);
+        desc_d.upload(desc.data(), desc.size(), s);
         HIPCHK(hipStreamSynchronize(s));
     };
     if (lexicographic) {
@@ -413,11 +492,12 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         // wavefronts in order reproduces the sequential sweep exactly
         lv.ncolor = wavefronts(lv.A.pat, false, lv.color);
         lv.rncolor = wavefronts(lv.A.pat, true, lv.rcolor);
-        group(lv.rcolor, lv.rncolor, lv.rcolor_off, lv.rcolor_rows, lv.rcolor_dev);
+        group(lv.rcolor, lv.rncolor, lv.rcolor_off, lv.rcolor_rows, lv.rcolor_dev, lv.rcolor_desc);
     } else {
         lv.ncolor = greedy_colors(lv.A.pat, lv.color);
     }
-    group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev);
+    group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev, lv.color_desc);
+    if (!lexicographic) pair_schedule(lv, s);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
     if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
@@ -588,12 +668,13 @@ void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) con
     const int nc = use_r ? lv.rncolor : lv.ncolor;
     const std::vector<int>& off = use_r ? lv.rcolor_off : lv.color_off;
     const int* rows = use_r ? lv.rcolor_rows.p : lv.color_rows.p;
+    const int* dsc = use_r ? lv.rcolor_desc.p : lv.color_desc.p;
     const int* cdev = use_r ? lv.rcolor_dev.p : lv.color_dev.p;
     const bool desc = reverse && !lex;
     for (int cc = 0; cc < nc; ++cc) {
         const int c = desc ? nc - 1 - cc : cc;
         const int skip = first_from_zero ? (desc ? 2 : 1) : 0;
-        fn(rows + off[c], off[c + 1] - off[c], cdev, c, skip);
+        fn(rows + off[c], dsc + 4 * (size_t)off[c], off[c + 1] - off[c], cdev, c, skip);
     }
 }
 
@@ -607,11 +688,28 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
     hipStream_t s = ctx->stream;
     const T omega = (T)prm.omega;
     if (Aop<T>(lv) == nullptr) throw Error(-4, "precision not built for this hierarchy");
-    if (prm.relaxation >= 1) {
+    if (prm.relaxation == 1 && lv.pair_nwg > 0) {
+        // colours ascending (descending in reverse), the last two in one launch
+        const int nc = lv.ncolor;
+        for (int sw = 0; sw < n; ++sw) {
+            const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
+            auto one = [&](int c) {
+                const int o = lv.color_off[c];
+                dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o, lv.A.cols,
+                                      Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, lv.color_dev, c, skip);
+            };
+            auto pair = [&] {
+                dev::Ops<T>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, reverse ? 1 : 0, lv.pair_desc, lv.A.cols, Aop<T>(lv),
+                                     Dop<T>(lv), v.b, v.x, omega, lv.color_dev, reverse ? nc - 1 : nc - 2,
+                                     reverse ? nc - 2 : nc - 1, skip);
+            };
+            if (!reverse) { for (int c = 0; c < nc - 2; ++c) one(c); pair(); }
+            else { pair(); for (int c = nc - 3; c >= 0; --c) one(c); }
+        }
+    } else if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
-                dev::Ops<T>::gs_color(s, lv.B, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega,
-                                      cdev, c, skip);
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int*, const int* dsc, int nr, const int* cdev, int c, int skip) {
+                dev::Ops<T>::gs_color(s, lv.B, nr, dsc, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, cdev, c, skip);
             });
     } else {
         // Jacobi ping-pongs between x and xalt; the result ends in x
@@ -675,7 +773,7 @@ void MGH::mrelax(int l, int k, int nsw, bool reverse, bool zero_guess, double* x
     const double omega = prm.omega;
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < nsw; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, int nr, const int* cdev, int c, int skip) {
                 dev::MOps<double>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, lv.A64, lv.D64, b, x, omega, cdev, c,
                                             skip);
             });
@@ -976,7 +1074,7 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].smooth : 0;
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, int nr, const int* cdev, int c, int skip) {
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, int nr, const int* cdev, int c, int skip) {
                 if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
                                              bx<T>(v), omega, cdev, c, skip);
                 else dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
@@ -1221,32 +1319,97 @@ void MGH::vcycle(int precision, const double* b, double* x)
 }
 
 // ---------------------------------------------------------------------------- measurement
-void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches)
+// Algorithmic HBM bytes of one GS launch over the rows R (sweep not from
+// zero): off-diagonal blocks and column indices of R, its D^-1 blocks and
+// descriptors, b, x (read) and x (write) of R, and x of the distinct
+// off-diagonal neighbours of R (each read once).
+static double gs_launch_bytes(const Level& lv, const int* rows, int nr, int sa, std::vector<int>& mark, int stamp)
+{
+    const HostBSR& P = lv.A.pat;
+    const double b2 = (double)lv.B * lv.B;
+    double offd = 0, nbrs = 0;
+    for (int q = 0; q < nr; ++q) {
+        const int i = rows[q];
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+            const int j = P.cols[k];
+            if (j == i) continue;
+            offd += 1;
+            if (mark[j] != stamp) { mark[j] = stamp; nbrs += 1; }
+        }
+    }
+    return offd * (b2 * sa + 4.0) + nr * (b2 * sa + 16.0 + 3.0 * lv.B * sa) + nbrs * lv.B * sa;
+}
+
+void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches, int part)
 {
     Level& lv = L[l];
     hipStream_t s = ctx->stream;
     const long n = (long)lv.nb * lv.B;
-    const int sa = precision == 1 ? 4 : 8, sv = sa;
+    const int sa = precision == 1 ? 4 : 8;
     if (precision == 1) { dev::Ops<float>::fill(s, n, 1.f, lv.v32.b); dev::Ops<float>::fill(s, n, 0.f, lv.v32.x); }
     else { dev::Ops<double>::fill(s, n, 1.0, lv.v64.b); dev::Ops<double>::fill(s, n, 0.0, lv.v64.x); }
+    const bool gs = prm.relaxation >= 1, paired = prm.relaxation == 1 && lv.pair_nwg > 0;
+    if (part != 0 && !paired) part = part == 1 ? 0 : -1;
+    if (part < 0) throw Error(-2, "profile_relax: level has no paired launch");
+    // launches of one sweep and their bytes (forward order)
+    std::vector<int> mark(lv.nb, -1);
+    int per = 0;
+    double by = 0;
+    if (!gs) {
+        per = 1;
+        by = (double)lv.A.nnzb() * lv.B * lv.B * sa + (double)lv.nb * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() +
+             4.0 * (lv.nb + 1) + 3.0 * n * sa;
+    } else {
+        const int nc = lv.ncolor, nsingle = paired ? nc - 2 : nc;
+        std::vector<int> host_rows(lv.nb);
+        HIPCHK(hipMemcpy(host_rows.data(), lv.color_rows.p, sizeof(int) * lv.nb, hipMemcpyDeviceToHost));
+        if (part != 2)
+            for (int c = 0; c < nsingle; ++c) {
+                by += gs_launch_bytes(lv, host_rows.data() + lv.color_off[c], lv.color_off[c + 1] - lv.color_off[c], sa, mark, c);
+                ++per;
+            }
+        if (paired && part != 1) {
+            const int o = lv.color_off[nc - 2];
+            by += gs_launch_bytes(lv, host_rows.data() + o, lv.color_off[nc] - o, sa, mark, nc) + 12.0 * lv.pair_nwg;
+            ++per;
+        }
+    }
+    auto run = [&](int reps) {
+        if (part == 0) {
+            if (precision == 1) relax<float>(l, reps, false, false); else relax<double>(l, reps, false, false);
+            return;
+        }
+        const int nc = lv.ncolor;
+        for (int r = 0; r < reps; ++r) {
+            if (part == 1)
+                for (int c = 0; c < nc - 2; ++c) {
+                    const int o = lv.color_off[c], nr = lv.color_off[c + 1] - o;
+                    const int* d = lv.color_desc.p + 4 * (size_t)o;
+                    if (precision == 1)
+                        dev::Ops<float>::gs_color(s, lv.B, nr, d, lv.A.cols, lv.A32, lv.D32, lv.v32.b, lv.v32.x, (float)prm.omega,
+                                                  lv.color_dev, c, 0);
+                    else
+                        dev::Ops<double>::gs_color(s, lv.B, nr, d, lv.A.cols, lv.A64, lv.D64, lv.v64.b, lv.v64.x, prm.omega,
+                                                   lv.color_dev, c, 0);
+                }
+            else if (precision == 1)
+                dev::Ops<float>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.A.cols, lv.A32, lv.D32, lv.v32.b,
+                                         lv.v32.x, (float)prm.omega, lv.color_dev, nc - 2, nc - 1, 0);
+            else
+                dev::Ops<double>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.A.cols, lv.A64, lv.D64, lv.v64.b,
+                                          lv.v64.x, prm.omega, lv.color_dev, nc - 2, nc - 1, 0);
+        }
+    };
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
-    // one warm sweep, then the timed ones
-    if (precision == 1) relax<float>(l, 1, false, false); else relax<double>(l, 1, false, false);
+    run(1);   // warm
     HIPCHK(hipEventRecord(e0, s));
-    if (precision == 1) relax<float>(l, nrep, false, false); else relax<double>(l, nrep, false, false);
+    run(nrep);
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-    const int per = prm.relaxation >= 1 ? lv.ncolor : 1;
-    const double b2 = (double)lv.B * lv.B;
-    double by;
-    if (prm.relaxation >= 1)   // off-diagonal blocks + Dinv + cols + rowptr + colour rows + b, x in, x out
-        by = (double)(lv.A.nnzb() - lv.nb) * b2 * sa + lv.nb * b2 * sa + 4.0 * lv.A.nnzb() + 8.0 * lv.nb + 4.0 * lv.nb + 3.0 * n * sv;
-    else                       // all blocks + Dinv + indices + b, x in, x out
-        by = (double)lv.A.nnzb() * b2 * sa + lv.nb * b2 * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 3.0 * n * sv;
     *ms = t / (double)(nrep * per);
     *bytes = by / per;
     if (launches) *launches = nrep * per;

@@ -109,12 +109,18 @@ struct Level {
     // Gauss-Seidel schedule: rows grouped by colour (multicolour) or by
     // wavefront of the lexicographic order (forward; the reverse sweep has its
     // own wavefronts in rcolor*)
+    // (*_desc: per row in that order {row, rowptr[row], rowptr[row+1],
+    // position of the diagonal block}, read by the single-RHS GS kernel)
     std::vector<int> color, color_off;
     int ncolor = 0;
-    DBuf<int> color_rows, color_dev;
+    DBuf<int> color_rows, color_dev, color_desc;
     std::vector<int> rcolor, rcolor_off;
     int rncolor = 0;
-    DBuf<int> rcolor_rows, rcolor_dev;
+    DBuf<int> rcolor_rows, rcolor_dev, rcolor_desc;
+    // multicolour GS: the last two colours in one launch (pair_nwg > 0):
+    // per workgroup {s, m, e} into pair_desc, see dev::Ops::gs_pair
+    int pair_nwg = 0;
+    DBuf<int> pair_wg, pair_desc;
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     // interpolation to the next coarser level
@@ -195,7 +201,7 @@ struct MGH {
     void spmv(int l, int precision, const void* x, void* y);
     void smooth(int l, int precision, int n, bool reverse, const void* b, void* x);
     void vcycle(int precision, const double* b, double* x);
-    void profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches);
+    void profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches, int part = 0);
     void profile_spmv(int l, int precision, int nrep, double* ms, double* bytes);
 };
 

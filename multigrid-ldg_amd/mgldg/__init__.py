@@ -294,10 +294,14 @@ class Multigrid(_Handle):
     def vcycle(self, b, x, precision="fp64"):
         check(self.ctx, lib().mgldg_vcycle_device(self.h, _prec(precision), _ptr(b), _ptr(x), 1))
 
-    def profile_relax(self, level=0, precision="fp64", nrep=10):
-        """(ms per smoother launch, algorithmic bytes per launch, launches)"""
+    def profile_relax(self, level=0, precision="fp64", nrep=10, part="sweep"):
+        """(ms per smoother launch, algorithmic bytes per launch, launches);
+        part: 'sweep' (all launches of a sweep), 'single' (the one-colour GS
+        launches) or 'pair' (the launch doing the last two colours)"""
         ms, by, n = C.c_double(), C.c_double(), C.c_int()
-        check(self.ctx, lib().mgldg_profile_relax(self.h, level, _prec(precision), nrep, C.byref(ms), C.byref(by), C.byref(n)))
+        pk = {"sweep": 0, "single": 1, "pair": 2}[part]
+        check(self.ctx, lib().mgldg_profile_relax_part(self.h, level, _prec(precision), nrep, pk, C.byref(ms), C.byref(by),
+                                                      C.byref(n)))
         return ms.value, by.value, n.value
 
     def profile_spmv(self, level=0, precision="fp64", nrep=10):

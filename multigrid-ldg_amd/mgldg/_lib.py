@@ -71,6 +71,7 @@ SIGNATURES = {
     "mgldg_smooth_device": (I, [V, I, I, I, I, V, V, I]),
     "mgldg_vcycle_device": (I, [V, I, V, V, I]),
     "mgldg_profile_relax": (I, [V, I, I, I, PD, PD, PI]),
+    "mgldg_profile_relax_part": (I, [V, I, I, I, I, PD, PD, PI]),
     "mgldg_profile_spmv": (I, [V, I, I, I, PD, PD]),
     "mgldg_setup_times": (I, [V, PD]),
 }
