@@ -466,32 +466,34 @@ def main():
     log(f"ndof {prob.ndof} elements {prob.ne} levels {mg.nlevels} rows {mg.level_rows}")
     log(f"setup: quadtree {t_qt:.2f}s problem {t_prob:.2f}s hierarchy {t_mg:.2f}s {mg.setup_times()}")
 
-    # roofline leg: dominant kernel (fine-level GS colour sweep, FP32 V-cycle)
-    nrep = 50
-    # dominant kernel: the one-colour GS launches (k_gs); beside it whole
-    # sweeps (k_gs launches + the launch doing the last two colours, k_gs_pair)
-    ms32, by32, n32 = mg.profile_relax(0, "mixed", nrep, part="single")
-    ms64, by64, n64 = mg.profile_relax(0, "fp64", nrep, part="single")
-    sw32 = mg.profile_relax(0, "mixed", nrep, part="sweep")
-    sw64 = mg.profile_relax(0, "fp64", nrep, part="sweep")
-    mss, bys = mg.profile_spmv(0, "fp64", nrep)
-    gbs = lambda ms, by: by / (ms * 1e-3) / 1e9
-    gs32, gs64, spmv64 = gbs(ms32, by32), gbs(ms64, by64), gbs(mss, bys)
-    sweep = {}
-    for name, (ms, by, nl) in (("fp32", sw32), ("fp64", sw64)):
-        per = nl // nrep
-        sweep[name] = {"launches": per, "us_per_sweep": round(ms * per * 1e3, 2), "MB_per_sweep": round(by * per / 1e6, 2),
-                       "GBs": round(gbs(ms, by), 1)}
-    log(f"GS one-colour launches fp32: {ms32 * 1e3:.1f} us/launch, {by32 / 1e6:.2f} MB -> {gs32:.0f} GB/s; "
-        f"fp64: {ms64 * 1e3:.1f} us, {gs64:.0f} GB/s; SpMV fp64 {mss * 1e3:.1f} us {spmv64:.0f} GB/s; sweeps {sweep}")
-    traffic = None
-    tp = os.path.join(ROOT, "profiles", "k_gs_c2_fine_pmc_traffic.json")
-    if os.path.exists(tp):   # measured offline: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated
-        traffic = json.load(open(tp))["kernels"].get("void mgldg::dev::k_gs<float, 16, 0>", {}).get("traffic_bytes")
-    roofline = {"bound": "hbm", "kernel": "k_gs<float,16,0> (fine-level one-colour block GS launch)",
-                "achieved": round(gs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gs32 / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "bytes_per_launch": by32, "us_per_launch": round(ms32 * 1e3, 3),
-                "fp64_gs_GBs": round(gs64, 1), "fp64_spmv_GBs": round(spmv64, 1), "sweep": sweep}
+    roofline = None
+    if not (args.solve_only or args.batched_only):
+        # roofline leg.  Dominant kernel: the fine level's one-colour GS
+        # launches of the FP32 V-cycle (k_gs); beside it whole sweeps (those
+        # launches + the one doing the last two colours, k_gs_pair)
+        nrep = 50
+        ms32, by32, n32 = mg.profile_relax(0, "mixed", nrep, part="single")
+        ms64, by64, n64 = mg.profile_relax(0, "fp64", nrep, part="single")
+        sw32 = mg.profile_relax(0, "mixed", nrep, part="sweep")
+        sw64 = mg.profile_relax(0, "fp64", nrep, part="sweep")
+        mss, bys = mg.profile_spmv(0, "fp64", nrep)
+        gbs = lambda ms, by: by / (ms * 1e-3) / 1e9
+        gs32, gs64, spmv64 = gbs(ms32, by32), gbs(ms64, by64), gbs(mss, bys)
+        sweep = {}
+        for name, (ms, by, nl) in (("fp32", sw32), ("fp64", sw64)):
+            per = nl // nrep
+            sweep[name] = {"launches": per, "us_per_sweep": round(ms * per * 1e3, 2), "MB_per_sweep": round(by * per / 1e6, 2),
+                           "GBs": round(gbs(ms, by), 1)}
+        log(f"GS one-colour launches fp32: {ms32 * 1e3:.1f} us/launch, {by32 / 1e6:.2f} MB -> {gs32:.0f} GB/s; "
+            f"fp64: {ms64 * 1e3:.1f} us, {gs64:.0f} GB/s; SpMV fp64 {mss * 1e3:.1f} us {spmv64:.0f} GB/s; sweeps {sweep}")
+        traffic = None
+        tp = os.path.join(ROOT, "profiles", "k_gs_c2_fine_pmc_traffic.json")
+        if os.path.exists(tp):   # measured offline: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated
+            traffic = json.load(open(tp))["kernels"].get("void mgldg::dev::k_gs<float, 16, 0>", {}).get("traffic_bytes")
+        roofline = {"bound": "hbm", "kernel": "k_gs<float,16,0> (fine-level one-colour block GS launch)",
+                    "achieved": round(gs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gs32 / HBM_PEAK_GBS, 4),
+                    "traffic": traffic, "bytes_per_launch": by32, "us_per_launch": round(ms32 * 1e3, 3),
+                    "fp64_gs_GBs": round(gs64, 1), "fp64_spmv_GBs": round(spmv64, 1), "sweep": sweep}
     if args.roofline_only:
         # calibration for the PMC traffic counters: a plain device copy of known size
         src = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device="cuda")

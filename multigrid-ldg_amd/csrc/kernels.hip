@@ -285,11 +285,14 @@ __device__ __forceinline__ int blk_at(int k0, int m, int dp)
 //    block in flight instead of B;
 //  * SKIP = 1 (2): first sweep from x = 0; x_j of colours after (before)
 //    `cur` in sweep order are still zero, and their blocks are not read.
-// The column indices of up to B blocks are loaded at once, one per lane, and
-// broadcast with lane shuffles; trip counts are uniform across the
-// wavefront, so every shuffle runs with all lanes active.
+// The column indices come from a padded table (colp: the row's W entries,
+// column of the m-th summed block at colp[m], -1 past the row's blocks), so
+// their loads need neither the row's descriptor nor rowptr and go out in the
+// first round trip; up to B are loaded at once, one per lane, and broadcast
+// with lane shuffles.  Trip counts are uniform across the wavefront, so every
+// shuffle runs with all lanes active.
 template <typename T, int B, int SKIP>
-__device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const int* __restrict__ cols,
+__device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const int* __restrict__ colp, int W,
                                      const T* __restrict__ A, const T* x, int r, int lane0, T* xs,
                                      const int* __restrict__ color, int cur)
 {
@@ -298,11 +301,12 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
     T acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0;
+    int mycol = (valid && r < W) ? colp[r] : -1;
     const int n = valid ? k1 - k0 - (dp >= 0 ? 1 : 0) : 0;
     int wl = n;
     for (int o = 32; o > 0; o >>= 1) wl = max(wl, __shfl_xor(wl, o, 64));
     for (int m0 = 0; m0 < wl; m0 += B) {
-        const int mycol = (r < n - m0) ? cols[blk_at(k0, m0 + r, dp)] : -1;
+        if (m0 > 0) mycol = (valid && m0 + r < W) ? colp[m0 + r] : -1;
         const int nc = min(B, wl - m0);
         for (int u0 = 0; u0 < nc; u0 += U) {
             VecT<T, V> a[U][B / V];
@@ -394,9 +398,10 @@ __device__ __forceinline__ void load_dinv_row(const T* __restrict__ Dinv, int i,
 
 // y = A x  or  y = b - A x ; optional deterministic reduction of y.dotv
 template <typename T, int B>
-__global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                               const T* __restrict__ A, const T* __restrict__ x, const T* __restrict__ b,
-                                               T* __restrict__ y, const T* __restrict__ dotv, double* part)
+__global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict__ rowptr, const int* __restrict__ ell,
+                                               int W, const T* __restrict__ A, const T* __restrict__ x,
+                                               const T* __restrict__ b, T* __restrict__ y, const T* __restrict__ dotv,
+                                               double* part)
 {
     using M = Map<T, B>;
     using R = RowLds<T, B>;
@@ -408,7 +413,7 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
     const int k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
     const size_t o = (size_t)row * B + r;
     const T bo = (valid && b) ? b[o] : (T)0;
-    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, cols, A, x, r, slot * B,
+    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, ell + (size_t)row * W, W, A, x, r, slot * B,
                                    xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
     double dv = 0;
     if (valid) {
@@ -422,11 +427,12 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
 // Damped block GS on rows [beg, end) of a descriptor list, one block row per
 // B lanes.  desc[idx] = {row i, rowptr[i], rowptr[i+1], position of the
 // diagonal block}: the row's operands (b_i, x_i, its row of D^-1, its A
-// blocks) are all addressable from that one load.  The rows must be
+// blocks) are all addressable from that one load; gcols[idx * W + m] =
+// column of its m-th off-diagonal block (-1 past the last).  The rows must be
 // independent (one colour).
 template <typename T, int B, int SKIP>
-__device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict__ desc, const int* __restrict__ cols,
-                                        const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
+__device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict__ desc, const int* __restrict__ gcols,
+                                        int W, const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
                                         T* x, T omega, const int* __restrict__ color, int cur, T* xs_all, T* tb)
 {
     using M = Map<T, B>;
@@ -445,7 +451,7 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
         if (SKIP == 0) xi = x[o];
         load_dinv_row<T, B>(Dinv, d.x, r, dv);
     }
-    const T acc = row_sum<T, B, SKIP>(valid, d.y, d.z, d.w, cols, A, x, r, slot * B,
+    const T acc = row_sum<T, B, SKIP>(valid, d.y, d.z, d.w, gcols + (size_t)idx * W, W, A, x, r, slot * B,
                                       xs_all + (wave * R::RW + slot) * R::XS, color, cur);
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
@@ -458,8 +464,8 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
 
 // One colour of a damped block GS sweep (rows desc[0 .. nrows)).
 template <typename T, int B, int SKIP>
-__global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ desc, const int* __restrict__ cols,
-                                            const T* __restrict__ A, const T* __restrict__ Dinv,
+__global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ desc, const int* __restrict__ gcols,
+                                            int W, const T* __restrict__ A, const T* __restrict__ Dinv,
                                             const T* __restrict__ b, T* x, T omega,
                                             const int* __restrict__ color, int cur)
 {
@@ -467,7 +473,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ 
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
     __shared__ __align__(16) T tb[4 * R::RW * B];
     const int beg = blockIdx.x * Map<T, B>::RPB;
-    gs_rows<T, B, SKIP>(beg, nrows, desc, cols, A, Dinv, b, x, omega, color, cur, xs_all, tb);
+    gs_rows<T, B, SKIP>(beg, nrows, desc, gcols, W, A, Dinv, b, x, omega, color, cur, xs_all, tb);
 }
 
 // The last two colours of a sweep in one launch.  Workgroup w owns the
@@ -482,7 +488,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ 
 // first / second range processed.
 template <typename T, int B, int SKIP>
 __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int reverse, const int4* __restrict__ desc,
-                                                 const int* __restrict__ cols, const T* __restrict__ A,
+                                                 const int* __restrict__ gcols, int W, const T* __restrict__ A,
                                                  const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
                                                  const int* __restrict__ color, int cur0, int cur1)
 {
@@ -496,13 +502,13 @@ __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int
         const int beg = lo ? s : m, end = lo ? m : e;
         if (ph == 1 && m != s && m != e) __syncthreads();   // (uniform: s, m, e are per workgroup)
         for (int base = beg; base < end; base += M::RPB)
-            gs_rows<T, B, SKIP>(base, end, desc, cols, A, Dinv, b, x, omega, color, ph ? cur1 : cur0, xs_all, tb);
+            gs_rows<T, B, SKIP>(base, end, desc, gcols, W, A, Dinv, b, x, omega, color, ph ? cur1 : cur0, xs_all, tb);
     }
 }
 
 template <typename T, int B>
-__global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                                const T* __restrict__ A, const T* __restrict__ Dinv,
+__global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict__ rowptr, const int* __restrict__ ell,
+                                                int W, const T* __restrict__ A, const T* __restrict__ Dinv,
                                                 const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ xout,
                                                 T omega)
 {
@@ -523,7 +529,7 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
         xi = x[o];
         load_dinv_row<T, B>(Dinv, i, r, dv);
     }
-    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, cols, A, x, r, slot * B,
+    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, ell + (size_t)i * W, W, A, x, r, slot * B,
                                    xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
@@ -646,7 +652,7 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
     }
 
 template <typename T>
-void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
+void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A, const T* x,
                    const T* b, T* y, const T* dotv, double* dot_out, double* part)
 {
     if (nrows <= 0) return;
@@ -654,7 +660,7 @@ void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
         g = nblk(nrows, M::RPB);
-        hipLaunchKernelGGL((k_apply<T, BB>), dim3(g), dim3(256), 0, s, nrows, rowptr, cols, A, x, b, y,
+        hipLaunchKernelGGL((k_apply<T, BB>), dim3(g), dim3(256), 0, s, nrows, rowptr, ell, W, A, x, b, y,
                            dotv, dot_out ? part : nullptr);
     });
     check_launch("apply");
@@ -662,8 +668,8 @@ void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int
 }
 
 template <typename T>
-void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* cols, const T* A, const T* Dinv,
-                      const T* b, T* x, T omega, const int* color, int cur, int skip)
+void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* gcols, int W, const T* A,
+                      const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
 {
     if (nrows <= 0) return;
     const int4* d = reinterpret_cast<const int4*>(desc);
@@ -671,17 +677,17 @@ void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const in
         using M = Map<T, BB>;
         const dim3 g(nblk(nrows, M::RPB));
         if (skip == 1)
-            hipLaunchKernelGGL((k_gs<T, BB, 1>), g, dim3(256), 0, s, nrows, d, cols, A, Dinv, b, x, omega, color, cur);
+            hipLaunchKernelGGL((k_gs<T, BB, 1>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
         else if (skip == 2)
-            hipLaunchKernelGGL((k_gs<T, BB, 2>), g, dim3(256), 0, s, nrows, d, cols, A, Dinv, b, x, omega, color, cur);
+            hipLaunchKernelGGL((k_gs<T, BB, 2>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
         else
-            hipLaunchKernelGGL((k_gs<T, BB, 0>), g, dim3(256), 0, s, nrows, d, cols, A, Dinv, b, x, omega, color, cur);
+            hipLaunchKernelGGL((k_gs<T, BB, 0>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
     });
     check_launch("gs_color");
 }
 
 template <typename T>
-void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* cols,
+void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* gcols, int W,
                      const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0, int cur1, int skip)
 {
     if (nwg <= 0) return;
@@ -689,13 +695,13 @@ void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, 
     MGLDG_DISPATCH(B, {
         const dim3 g(nwg);
         if (skip == 1)
-            hipLaunchKernelGGL((k_gs_pair<T, BB, 1>), g, dim3(256), 0, s, wg, reverse, d, cols, A, Dinv, b, x, omega, color,
+            hipLaunchKernelGGL((k_gs_pair<T, BB, 1>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x, omega, color,
                                cur0, cur1);
         else if (skip == 2)
-            hipLaunchKernelGGL((k_gs_pair<T, BB, 2>), g, dim3(256), 0, s, wg, reverse, d, cols, A, Dinv, b, x, omega, color,
+            hipLaunchKernelGGL((k_gs_pair<T, BB, 2>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x, omega, color,
                                cur0, cur1);
         else
-            hipLaunchKernelGGL((k_gs_pair<T, BB, 0>), g, dim3(256), 0, s, wg, reverse, d, cols, A, Dinv, b, x, omega, color,
+            hipLaunchKernelGGL((k_gs_pair<T, BB, 0>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x, omega, color,
                                cur0, cur1);
     });
     check_launch("gs_pair");
@@ -709,14 +715,14 @@ int rows_per_workgroup(int B)
 }
 
 template <typename T>
-void Ops<T>::jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A, const T* Dinv,
-                    const T* b, const T* x, T* xout, T omega)
+void Ops<T>::jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
+                    const T* Dinv, const T* b, const T* x, T* xout, T omega)
 {
     if (nrows <= 0) return;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_jacobi<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, cols, A, Dinv, b, x,
-                           xout, omega);
+        hipLaunchKernelGGL((k_jacobi<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, ell, W, A, Dinv, b,
+                           x, xout, omega);
     });
     check_launch("jacobi");
 }
@@ -1099,26 +1105,32 @@ void dot(hipStream_t s, long n, const double* a, const double* b, double* out, d
     finalize(s, part, (int)g, 1, nullptr, 0, out);
 }
 
+// x += alpha d ; r -= alpha q ; v = (T) r, the V-cycle's level-0 right-hand
+// side (one pass instead of an update and a conversion)
+template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_xr(long n, double* S, const double* __restrict__ d, const double* __restrict__ q,
-                                                double* __restrict__ x, double* __restrict__ r, double* part)
+                                                double* __restrict__ x, double* __restrict__ r, T* __restrict__ v, double* part)
 {
     const double alpha = S[S_DELTA] / S[S_DQ];
     double s = 0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         x[i] += alpha * d[i];
         double ri = r[i] - alpha * q[i];
-        r[i] = ri; s += ri * ri;
+        r[i] = ri; v[i] = (T)ri; s += ri * ri;
     }
     block_partial(s, part);
 }
 
-void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, double* part)
+template <typename T>
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, T* v, double* part)
 {
     unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_pcg_xr, dim3(g), dim3(256), 0, s, n, S, d, q, x, r, part);
+    hipLaunchKernelGGL(k_pcg_xr<T>, dim3(g), dim3(256), 0, s, n, S, d, q, x, r, v, part);
     check_launch("pcg_xr");
     finalize(s, part, (int)g, 1, S, 0, S + S_RR);
 }
+template void pcg_xr<float>(hipStream_t, long, double*, const double*, const double*, double*, double*, float*, double*);
+template void pcg_xr<double>(hipStream_t, long, double*, const double*, const double*, double*, double*, double*, double*);
 
 __global__ void k_pcg_x(long n, const double* S, const double* __restrict__ d, double* __restrict__ x)
 {
@@ -1134,41 +1146,55 @@ void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x)
     check_launch("pcg_x");
 }
 
-// three reductions in one pass: r.z, r.zold, r.r ; then beta in the last block
-__global__ __launch_bounds__(256) void k_pcg_rz(long n, const double* __restrict__ r, const double* __restrict__ z,
-                                                const double* __restrict__ zold, double* part)
+// three reductions in one pass: r.z, r.zold, r.r (z = the V-cycle's level-0
+// result zs, precision T, converted exactly); zold <- zs in the same pass
+template <typename T>
+__global__ __launch_bounds__(256) void k_pcg_rz(long n, const double* __restrict__ r, const T* __restrict__ zs, T* zold,
+                                                int read_old, double* part)
 {
     double a = 0, b = 0, c = 0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        double ri = r[i];
-        a += ri * z[i]; c += ri * ri;
-        if (zold) b += ri * zold[i];
+        const double ri = r[i];
+        const T zi = zs[i];
+        a += ri * (double)zi; c += ri * ri;
+        if (zold) {
+            if (read_old) b += ri * (double)zold[i];
+            zold[i] = zi;
+        }
     }
     double ba = block_sum(a), bb = block_sum(b), bc = block_sum(c);
     if (threadIdx.x == 0) { part[3 * blockIdx.x] = ba; part[3 * blockIdx.x + 1] = bb; part[3 * blockIdx.x + 2] = bc; }
 }
 
-void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold, double* part)
+template <typename T>
+void pcg_rz(hipStream_t s, long n, double* S, const double* r, const T* zs, T* zold, bool read_old, double* part)
 {
     unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_pcg_rz, dim3(g), dim3(256), 0, s, n, r, z, zold, part);
+    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(g), dim3(256), 0, s, n, r, zs, zold, (int)(read_old && zold), part);
     check_launch("pcg_rz");
-    finalize(s, part, (int)g, 3, S, zold ? 2 : 1, nullptr);
+    finalize(s, part, (int)g, 3, S, (read_old && zold) ? 2 : 1, nullptr);
 }
+template void pcg_rz<float>(hipStream_t, long, double*, const double*, const float*, float*, bool, double*);
+template void pcg_rz<double>(hipStream_t, long, double*, const double*, const double*, double*, bool, double*);
 
-__global__ void k_pcg_d(long n, const double* S, const double* __restrict__ z, double* __restrict__ d)
+// d = z + beta d, or d = z (init), z = (double) zs
+template <typename T>
+__global__ void k_pcg_d(long n, const double* S, const T* __restrict__ zs, double* __restrict__ d, int init)
 {
-    const double beta = S[S_BETA];
     long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) d[i] = z[i] + beta * d[i];
+    if (i >= n) return;
+    d[i] = init ? (double)zs[i] : (double)zs[i] + S[S_BETA] * d[i];
 }
 
-void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d)
+template <typename T>
+void pcg_d(hipStream_t s, long n, const double* S, const T* zs, double* d, bool init)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_pcg_d, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, z, d);
+    hipLaunchKernelGGL(k_pcg_d<T>, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, zs, d, (int)init);
     check_launch("pcg_d");
 }
+template void pcg_d<float>(hipStream_t, long, const double*, const float*, double*, bool);
+template void pcg_d<double>(hipStream_t, long, const double*, const double*, double*, bool);
 
 // ---------------------------------------------------------------------- batched PCG
 // thread (group g, column c), c = tid % k, g = tid / k; k <= 256
@@ -1242,33 +1268,46 @@ void coldot(hipStream_t s, long n, int k, const double* a, const double* b, doub
     check_launch("colfinalize");
 }
 
+// per active column: x += alpha d ; r -= alpha q ; every column: v = (T) r,
+// the V-cycle's level-0 right-hand sides
+template <typename T>
 __global__ __launch_bounds__(256) void k_bpcg_xr(long n, int k, const double* S, const int* active, const double* __restrict__ d,
                                                  const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
-                                                 double* part)
+                                                 T* __restrict__ v, double* part)
 {
     const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
     double acc = 0;
-    if (g < groups && active[c]) {
-        const double alpha = S[(size_t)c * S_COUNT + S_DELTA] / S[(size_t)c * S_COUNT + S_DQ];
+    if (g < groups) {
+        const bool act = active[c] != 0;
+        const double alpha = act ? S[(size_t)c * S_COUNT + S_DELTA] / S[(size_t)c * S_COUNT + S_DQ] : 0.0;
         for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) {
             const long o = i * k + c;
-            x[o] += alpha * d[o];
-            const double ri = r[o] - alpha * q[o];
-            r[o] = ri; acc += ri * ri;
+            double ri = r[o];
+            if (act) {
+                x[o] += alpha * d[o];
+                ri = ri - alpha * q[o];
+                r[o] = ri; acc += ri * ri;
+            }
+            v[o] = (T)ri;
         }
     }
     col_partials(k, 1, &acc, part);
 }
 
+template <typename T>
 void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const double* d, const double* q, double* x, double* r,
-             double* part)
+             T* v, double* part)
 {
     unsigned g = red_grid(n, k);
-    hipLaunchKernelGGL(k_bpcg_xr, dim3(g), dim3(256), 0, s, n, k, S, active, d, q, x, r, part);
+    hipLaunchKernelGGL(k_bpcg_xr<T>, dim3(g), dim3(256), 0, s, n, k, S, active, d, q, x, r, v, part);
     check_launch("bpcg_xr");
     hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 1, S + S_RR, (int)S_COUNT, nullptr, nullptr, 0);
     check_launch("colfinalize");
 }
+template void bpcg_xr<float>(hipStream_t, long, int, double*, const int*, const double*, const double*, double*, double*, float*,
+                             double*);
+template void bpcg_xr<double>(hipStream_t, long, int, double*, const int*, const double*, const double*, double*, double*,
+                              double*, double*);
 
 __global__ void k_bpcg_x(long n, int k, const double* S, const int* active, const double* __restrict__ d, double* __restrict__ x)
 {
@@ -1285,8 +1324,10 @@ void bpcg_x(hipStream_t s, long n, int k, const double* S, const int* active, co
     check_launch("bpcg_x");
 }
 
-__global__ __launch_bounds__(256) void k_bpcg_rz(long n, int k, const double* __restrict__ r, const double* __restrict__ z,
-                                                 const double* __restrict__ zold, double* part)
+// per column r.z, r.zold, r.r with z = (double) zs; zold <- zs (every column)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bpcg_rz(long n, int k, const double* __restrict__ r, const T* __restrict__ zs, T* zold,
+                                                 int read_old, double* part)
 {
     const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
     double v[3] = {0, 0, 0};
@@ -1294,36 +1335,51 @@ __global__ __launch_bounds__(256) void k_bpcg_rz(long n, int k, const double* __
         for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) {
             const long o = i * k + c;
             const double ri = r[o];
-            v[0] += ri * z[o]; v[2] += ri * ri;
-            if (zold) v[1] += ri * zold[o];
+            const T zi = zs[o];
+            v[0] += ri * (double)zi; v[2] += ri * ri;
+            if (zold) {
+                if (read_old) v[1] += ri * (double)zold[o];
+                zold[o] = zi;
+            }
         }
     col_partials(k, 3, v, part);
 }
 
-void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const double* r, const double* z, const double* zold,
+template <typename T>
+void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const double* r, const T* zs, T* zold, bool read_old,
              double* part)
 {
     unsigned g = red_grid(n, k);
-    hipLaunchKernelGGL(k_bpcg_rz, dim3(g), dim3(256), 0, s, n, k, r, z, zold, part);
+    const int ro = read_old && zold;
+    hipLaunchKernelGGL(k_bpcg_rz<T>, dim3(g), dim3(256), 0, s, n, k, r, zs, zold, ro, part);
     check_launch("bpcg_rz");
-    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 3, nullptr, 0, S, active, zold ? 2 : 1);
+    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 3, nullptr, 0, S, active, ro ? 2 : 1);
     check_launch("colfinalize");
 }
+template void bpcg_rz<float>(hipStream_t, long, int, double*, const int*, const double*, const float*, float*, bool, double*);
+template void bpcg_rz<double>(hipStream_t, long, int, double*, const int*, const double*, const double*, double*, bool, double*);
 
-__global__ void k_bpcg_d(long n, int k, const double* S, const int* active, const double* __restrict__ z, double* __restrict__ d)
+// d = z + beta d for active columns, or d = z for every column (init); z = (double) zs
+template <typename T>
+__global__ void k_bpcg_d(long n, int k, const double* S, const int* active, const T* __restrict__ zs, double* __restrict__ d,
+                         int init)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= n * k) return;
     const int c = (int)(e % k);
+    if (init) { d[e] = (double)zs[e]; return; }
     if (!active[c]) return;
-    d[e] = z[e] + S[(size_t)c * S_COUNT + S_BETA] * d[e];
+    d[e] = (double)zs[e] + S[(size_t)c * S_COUNT + S_BETA] * d[e];
 }
 
-void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const double* z, double* d)
+template <typename T>
+void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const T* zs, double* d, bool init)
 {
-    hipLaunchKernelGGL(k_bpcg_d, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, S, active, z, d);
+    hipLaunchKernelGGL(k_bpcg_d<T>, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, S, active, zs, d, (int)init);
     check_launch("bpcg_d");
 }
+template void bpcg_d<float>(hipStream_t, long, int, const double*, const int*, const float*, double*, bool);
+template void bpcg_d<double>(hipStream_t, long, int, const double*, const int*, const double*, double*, bool);
 
 // ---------------------------------------------------------------------- boundary data map
 __global__ void k_bc_forward(int ne, int B, int nq, int k, int N, const int* __restrict__ bf_off, const int* __restrict__ bf_list,

@@ -33,24 +33,26 @@ template <typename T>
 struct Ops {
     // y = A x (b == nullptr) or y = b - A x ; if dot_out: y.dotv is reduced
     // into *dot_out (FP64, deterministic; part holds one partial per workgroup)
-    static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
+    // (ell: the rows' column indices at stride W, -1 past a row's blocks)
+    static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
                       const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
     // damped block Gauss-Seidel on the rows of colour `cur` (multigrid.h:240-249).
     // desc: 4 ints per row of the colour, {row, rowptr[row], rowptr[row+1],
-    // position of the diagonal block}.  skip = 1 (2): first forward (reverse)
-    // sweep from x = 0, where the colours after `cur` in sweep order are still
-    // zero and are not read.
-    static void gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* cols, const T* A, const T* Dinv,
-                         const T* b, T* x, T omega, const int* color, int cur, int skip);
+    // position of the diagonal block}; gcols: per row W columns of its
+    // off-diagonal blocks (-1 past the last).  skip = 1 (2): first forward
+    // (reverse) sweep from x = 0, where the colours after `cur` in sweep order
+    // are still zero and are not read.
+    static void gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* gcols, int W, const T* A,
+                         const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
     // the last two colours of a sweep in one launch of nwg workgroups:
     // workgroup w does desc ranges [s, m) then [m, e) ({s, m, e} = wg[3w..]),
     // the reverse for a backward sweep, with a workgroup barrier between;
     // cur0 / cur1 = colour of the first / second range processed
-    static void gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* cols,
-                        const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0, int cur1,
-                        int skip);
+    static void gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* gcols,
+                        int W, const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0,
+                        int cur1, int skip);
     // damped block Jacobi, x_out = x + omega Dinv (b - A x) (multigrid.h:223-229)
-    static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* cols, const T* A,
+    static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);
     // b_c = T^T r (multigrid.h:414-418) as a gather over children; T blocks
     // are BF x BC (fine x coarse basis)
@@ -128,26 +130,33 @@ void f32_to_f64(hipStream_t s, long n, const float* a, double* b);
 enum { S_DELTA = 0, S_DELTA_OLD, S_DQ, S_ALPHA, S_BETA, S_RR, S_RSOLD, S_DELTA0, S_BB, S_COUNT };
 // a.b reduced into *out
 void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part);
-// x += alpha d ; r -= alpha q ; S[S_RR] = r.r      (alpha = S[DELTA]/S[DQ])
-void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r,
-            double* part);
+// The V-cycle's level-0 vectors have the preconditioner's precision T: v
+// (its right-hand side) and zs (its result, z = (double) zs exactly).
+// x += alpha d ; r -= alpha q ; v = (T) r ; S[S_RR] = r.r      (alpha = S[DELTA]/S[DQ])
+template <typename T>
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, T* v, double* part);
 // x += alpha d only
 void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x);
-// S[DELTA_OLD]=S[DELTA]; S[DELTA] = r.z ; if flexible S[RSOLD] = r.zold ; S[RR] = r.r ; beta
-void pcg_rz(hipStream_t s, long n, double* S, const double* r, const double* z, const double* zold,
-            double* part);
-// d = z + beta d
-void pcg_d(hipStream_t s, long n, const double* S, const double* z, double* d);
+// S[DELTA_OLD]=S[DELTA]; S[DELTA] = r.z ; if read_old S[RSOLD] = r.zold ; S[RR] = r.r ; beta;
+// then zold = zs (zold may be null: not kept)
+template <typename T>
+void pcg_rz(hipStream_t s, long n, double* S, const double* r, const T* zs, T* zold, bool read_old, double* part);
+// d = z + beta d   (init: d = z)
+template <typename T>
+void pcg_d(hipStream_t s, long n, const double* S, const T* zs, double* d, bool init);
 // ---------------------------------------------------------------- batched PCG (k columns, [n][k])
 // Per-column scalars live at S[c * S_COUNT + slot]; active[c] = 0 freezes column c.
 // per-column a.b into out[c * ostride] (deterministic two-pass)
 void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part);
+template <typename T>
 void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const double* d, const double* q, double* x,
-             double* r, double* part);
+             double* r, T* v, double* part);
 void bpcg_x(hipStream_t s, long n, int k, const double* S, const int* active, const double* d, double* x);
-void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const double* r, const double* z,
-             const double* zold, double* part);
-void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const double* z, double* d);
+template <typename T>
+void bpcg_rz(hipStream_t s, long n, int k, double* S, const int* active, const double* r, const T* zs, T* zold,
+             bool read_old, double* part);
+template <typename T>
+void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const T* zs, double* d, bool init);
 // ---------------------------------------------------------------- boundary data map B_g (ldgpoisson.tpp:103-161)
 // Forward gather per element e (its boundary faces bf_off[e]..bf_off[e+1]):
 //   r_e  = sum_f w_f L_f g_f     (w_f = tauD for Dirichlet, 1 for Neumann)

@@ -83,6 +83,40 @@ void DevOp::upload_pattern(const HostBSR& h, hipStream_t s)
     cols.upload(h.cols.data(), std::max<size_t>(h.cols.size(), 0), s);
 }
 
+void DevOp::build_ell(hipStream_t s)
+{
+    const int n = pat.nbr;
+    ellw = 1;
+    for (int i = 0; i < n; ++i) ellw = std::max(ellw, pat.rowptr[i + 1] - pat.rowptr[i]);
+    std::vector<int> e((size_t)n * ellw, -1);
+    for (int i = 0; i < n; ++i)
+        for (int k = pat.rowptr[i]; k < pat.rowptr[i + 1]; ++k) e[(size_t)i * ellw + (k - pat.rowptr[i])] = pat.cols[k];
+    ell.upload(e.data(), e.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// GS row tables of the rows `rows` (in that order): descriptors {row,
+// rowptr[row], rowptr[row+1], diagonal position} and off-diagonal columns at
+// stride w (-1 past the last)
+static void gs_tables(const HostBSR& P, const std::vector<int>& rows, int w, hipStream_t s, DBuf<int>& desc_d,
+                      DBuf<int>& gcols_d)
+{
+    const size_t nr = rows.size();
+    std::vector<int> desc(4 * nr), gc(nr * w, -1);
+    for (size_t q = 0; q < nr; ++q) {
+        const int i = rows[q];
+        int dp = -1, m = 0;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+            if (P.cols[k] == i) { if (dp < 0) { dp = k; continue; } }
+            gc[q * w + m++] = P.cols[k];
+        }
+        desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1]; desc[4 * q + 3] = dp;
+    }
+    desc_d.upload(desc.data(), desc.size(), s);
+    gcols_d.upload(gc.data(), gc.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 // upload a term list and launch the triple-product kernel
 static void run_triple(hipStream_t s, const TermList& t, int p, int q, int m, int n, int transL, const double* Lp,
                        const double* Xp, const double* Rp, const double* addpool, const std::vector<int>* addslot,
@@ -433,15 +467,7 @@ static void pair_schedule(Level& lv, hipStream_t s)
         order.insert(order.end(), rb[c].begin(), rb[c].end());
         wg.insert(wg.end(), {a, m, (int)order.size()});
     }
-    std::vector<int> desc(4 * order.size());
-    for (size_t q = 0; q < order.size(); ++q) {
-        const int i = order[q];
-        int dp = -1;
-        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
-            if (P.cols[k] == i) { dp = k; break; }
-        desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1]; desc[4 * q + 3] = dp;
-    }
-    lv.pair_desc.upload(desc.data(), desc.size(), s);
+    gs_tables(P, order, lv.gw, s, lv.pair_desc, lv.pair_gcols);
     lv.pair_wg.upload(wg.data(), wg.size(), s);
     HIPCHK(hipStreamSynchronize(s));
     lv.pair_nwg = (int)(wg.size() / 3);
@@ -465,26 +491,23 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         lv.D32.alloc((size_t)lv.nb * b * b);
         dev::to_panel_f32(s, lv.nb, b, lv.Dinv_rm, lv.D32);
     }
+    lv.A.build_ell(s);
+    lv.gw = 1;   // most off-diagonal blocks in a row
+    for (int i = 0; i < lv.nb; ++i) {
+        int m = 0;
+        for (int k = lv.A.pat.rowptr[i]; k < lv.A.pat.rowptr[i + 1]; ++k) m += lv.A.pat.cols[k] != i;
+        lv.gw = std::max(lv.gw, m);
+    }
     auto group = [&](const std::vector<int>& color, int nc, std::vector<int>& off, DBuf<int>& rows_d, DBuf<int>& col_d,
-                     DBuf<int>& desc_d) {
+                     DBuf<int>& desc_d, DBuf<int>& gcols_d) {
         off.assign(nc + 1, 0);
         for (int c : color) off[c + 1]++;
         for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
         std::vector<int> rows(lv.nb), pos(off.begin(), off.end() - 1);
         for (int i = 0; i < lv.nb; ++i) rows[pos[color[i]]++] = i;
-        const HostBSR& P = lv.A.pat;
-        std::vector<int> desc((size_t)4 * lv.nb);
-        for (int q = 0; q < lv.nb; ++q) {
-            const int i = rows[q];
-            int dp = -1;
-            for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
-                if (P.cols[k] == i) { dp = k; break; }
-            desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1]; desc[4 * q + 3] = dp;
-        }
         rows_d.upload(rows.data(), rows.size(), s);
         col_d.upload(color.data(), color.size(), s);
-        desc_d.upload(desc.data(), desc.size(), s);
-        HIPCHK(hipStreamSynchronize(s));
+        gs_tables(lv.A.pat, rows, lv.gw, s, desc_d, gcols_d);
     };
     if (lexicographic) {
         // wavefronts of the lexicographic order: a row waits for its earlier
@@ -492,11 +515,11 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         // wavefronts in order reproduces the sequential sweep exactly
         lv.ncolor = wavefronts(lv.A.pat, false, lv.color);
         lv.rncolor = wavefronts(lv.A.pat, true, lv.rcolor);
-        group(lv.rcolor, lv.rncolor, lv.rcolor_off, lv.rcolor_rows, lv.rcolor_dev, lv.rcolor_desc);
+        group(lv.rcolor, lv.rncolor, lv.rcolor_off, lv.rcolor_rows, lv.rcolor_dev, lv.rcolor_desc, lv.rcolor_gcols);
     } else {
         lv.ncolor = greedy_colors(lv.A.pat, lv.color);
     }
-    group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev, lv.color_desc);
+    group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev, lv.color_desc, lv.color_gcols);
     if (!lexicographic) pair_schedule(lv, s);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
@@ -626,7 +649,7 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
     build_collapse();
     // PCG work space
     const size_t nd = (size_t)p->ndof;
-    r.alloc(nd); z.alloc(nd); zold.alloc(nd); d.alloc(nd); q.alloc(nd); xw.alloc(nd); bw.alloc(nd);
+    r.alloc(nd); zold.alloc(nd); d.alloc(nd); q.alloc(nd); xw.alloc(nd); bw.alloc(nd);
     S.alloc(16);
     part.alloc(std::max<size_t>(3 * 1024 + 64, (size_t)L[0].nb + 64));
     HIPCHK(hipMemsetAsync(S.p, 0, 16 * sizeof(double), s));
@@ -669,12 +692,13 @@ void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) con
     const std::vector<int>& off = use_r ? lv.rcolor_off : lv.color_off;
     const int* rows = use_r ? lv.rcolor_rows.p : lv.color_rows.p;
     const int* dsc = use_r ? lv.rcolor_desc.p : lv.color_desc.p;
+    const int* gcl = use_r ? lv.rcolor_gcols.p : lv.color_gcols.p;
     const int* cdev = use_r ? lv.rcolor_dev.p : lv.color_dev.p;
     const bool desc = reverse && !lex;
     for (int cc = 0; cc < nc; ++cc) {
         const int c = desc ? nc - 1 - cc : cc;
         const int skip = first_from_zero ? (desc ? 2 : 1) : 0;
-        fn(rows + off[c], dsc + 4 * (size_t)off[c], off[c + 1] - off[c], cdev, c, skip);
+        fn(rows + off[c], dsc + 4 * (size_t)off[c], gcl + (size_t)lv.gw * off[c], off[c + 1] - off[c], cdev, c, skip);
     }
 }
 
@@ -695,12 +719,13 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
             const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
             auto one = [&](int c) {
                 const int o = lv.color_off[c];
-                dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o, lv.A.cols,
-                                      Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, lv.color_dev, c, skip);
+                dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o,
+                                      lv.color_gcols.p + (size_t)lv.gw * o, lv.gw, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega,
+                                      lv.color_dev, c, skip);
             };
             auto pair = [&] {
-                dev::Ops<T>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, reverse ? 1 : 0, lv.pair_desc, lv.A.cols, Aop<T>(lv),
-                                     Dop<T>(lv), v.b, v.x, omega, lv.color_dev, reverse ? nc - 1 : nc - 2,
+                dev::Ops<T>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, reverse ? 1 : 0, lv.pair_desc, lv.pair_gcols, lv.gw,
+                                     Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, lv.color_dev, reverse ? nc - 1 : nc - 2,
                                      reverse ? nc - 2 : nc - 1, skip);
             };
             if (!reverse) { for (int c = 0; c < nc - 2; ++c) one(c); pair(); }
@@ -708,15 +733,17 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
         }
     } else if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int*, const int* dsc, int nr, const int* cdev, int c, int skip) {
-                dev::Ops<T>::gs_color(s, lv.B, nr, dsc, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, cdev, c, skip);
-            });
+            sweep(lv, reverse, zero_guess && sw == 0,
+                  [&](const int*, const int* dsc, const int* gcl, int nr, const int* cdev, int c, int skip) {
+                      dev::Ops<T>::gs_color(s, lv.B, nr, dsc, gcl, lv.gw, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, cdev, c,
+                                            skip);
+                  });
     } else {
         // Jacobi ping-pongs between x and xalt; the result ends in x
         if (zero_guess) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
         T* cur = v.x.p; T* alt = v.xalt.p;
         for (int sw = 0; sw < n; ++sw) {
-            dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), v.b, cur, alt, omega);
+            dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, Aop<T>(lv), Dop<T>(lv), v.b, cur, alt, omega);
             std::swap(cur, alt);
         }
         if (cur != v.x.p) dev::Ops<T>::copy(s, (long)lv.nb * lv.B, cur, v.x);
@@ -735,7 +762,7 @@ void MGH::vcycle_from(int l0)
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
         relax<T>(l, prm.npre, false, zero);
-        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr);
+        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr);
         LevelVecs<T>& vc = vecs<T>(l + 1);
         dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): implicit in the first sweep of a
@@ -773,7 +800,7 @@ void MGH::mrelax(int l, int k, int nsw, bool reverse, bool zero_guess, double* x
     const double omega = prm.omega;
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < nsw; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, int nr, const int* cdev, int c, int skip) {
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, const int*, int nr, const int* cdev, int c, int skip) {
                 dev::MOps<double>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, lv.A64, lv.D64, b, x, omega, cdev, c,
                                             skip);
             });
@@ -840,21 +867,19 @@ void MGH::build_collapse()
 }
 
 // z = V(r): enqueue the preconditioner on the stream
+// x_0 = V(b_0) on level 0's vectors of the given precision (v32 / v64): the
+// caller puts the right-hand side in L[0].v*.b and reads the result in L[0].v*.x
 void MGH::precon_enqueue(int precision)
 {
     hipStream_t s = ctx->stream;
     const long n = pr->ndof;
     if (precision == 1) {
         if (!L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
-        dev::f64_to_f32(s, n, r, L[0].v32.b);
         if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
         vcycle_from<float>(0);
-        dev::f32_to_f64(s, n, L[0].v32.x, z);
     } else {
-        dev::Ops<double>::copy(s, n, r, L[0].v64.b);
         if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
         vcycle_from<double>(0);
-        dev::Ops<double>::copy(s, n, L[0].v64.x, z);
     }
 }
 
@@ -882,6 +907,9 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     Level& f = L[0];
     const long n = pr->ndof;
     double* part = this->part.p;
+    const bool mixed = sp.precision == 1;
+    if (mixed && !f.A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
+    if (mixed && sp.flexible && !zold32.p) zold32.alloc(n);
     SolveStats st;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
@@ -891,16 +919,28 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
         HIPCHK(hipStreamSynchronize(s));
     };
     auto residual = [&](bool with_rr) {   // r = b - A x  (optionally r.r into S[RR])
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, x, b, r, with_rr ? r.p : nullptr,
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.ell, f.A.ellw, f.A64, x, b, r, with_rr ? r.p : nullptr,
                                 with_rr ? S.p + dev::S_RR : nullptr, part);
+    };
+    // the preconditioner reads r in its own precision from L[0].v*.b and
+    // leaves z in L[0].v*.x; the CG kernels convert on the fly
+    auto to_vin = [&] {
+        if (mixed) dev::f64_to_f32(s, n, r, f.v32.b); else dev::Ops<double>::copy(s, n, r, f.v64.b);
+    };
+    auto rz = [&](bool read_old) {   // S[DELTA] = r.z, S[RR] = r.r (+ r.zold, beta); zold = z
+        if (mixed) dev::pcg_rz<float>(s, n, S, r, f.v32.x, sp.flexible ? zold32.p : nullptr, read_old, part);
+        else dev::pcg_rz<double>(s, n, S, r, f.v64.x, sp.flexible ? zold.p : nullptr, read_old, part);
+    };
+    auto dupd = [&](bool init) {     // d = z + beta d  (init: d = z)
+        if (mixed) dev::pcg_d<float>(s, n, S, f.v32.x, d, init); else dev::pcg_d<double>(s, n, S, f.v64.x, d, init);
     };
     // ||b||
     dev::dot(s, n, b, b, S.p + dev::S_BB, part);
     residual(false);
+    to_vin();
     precon(sp.precision, sp.use_graph != 0);
-    dev::pcg_rz(s, n, S, r, z, nullptr, part);        // S[DELTA] = r.z, S[RR] = r.r
-    dev::Ops<double>::copy(s, n, z, d);
-    if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
+    rz(false);
+    dupd(true);
     fetchS();
     const double bnorm = std::sqrt(S_host[dev::S_BB]);
     const double delta0 = S_host[dev::S_DELTA];
@@ -914,10 +954,10 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
             residual(true);
             fetchS();
             if (std::sqrt(S_host[dev::S_RR]) > sp.tol * bnorm && i < sp.maxit) {
+                to_vin();
                 precon(sp.precision, sp.use_graph != 0);
-                dev::pcg_rz(s, n, S, r, z, nullptr, part);
-                dev::Ops<double>::copy(s, n, z, d);
-                if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
+                rz(false);
+                dupd(true);
                 fetchS();
                 delta = S_host[dev::S_DELTA]; rr = S_host[dev::S_RR];
                 go = true;
@@ -926,17 +966,19 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
         if (!go) { st.status = 0; break; }
         if (i >= sp.maxit) { st.status = 1; break; }
         // q = A d, S[DQ] = d.q
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.cols, f.A64, d, nullptr, q, d, S.p + dev::S_DQ, part);
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.ell, f.A.ellw, f.A64, d, nullptr, q, d, S.p + dev::S_DQ, part);
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
             dev::pcg_x(s, n, S, d, x);
             residual(true);
+            to_vin();
+        } else if (mixed) {
+            dev::pcg_xr<float>(s, n, S, d, q, x, r, f.v32.b, part);
         } else {
-            dev::pcg_xr(s, n, S, d, q, x, r, part);
+            dev::pcg_xr<double>(s, n, S, d, q, x, r, f.v64.b, part);
         }
         precon(sp.precision, sp.use_graph != 0);
-        dev::pcg_rz(s, n, S, r, z, sp.flexible ? zold.p : nullptr, part);
-        if (sp.flexible) dev::Ops<double>::copy(s, n, z, zold);
-        dev::pcg_d(s, n, S, z, d);
+        rz(sp.flexible != 0);
+        dupd(false);
         ++i;
         fetchS();
         if (!(S_host[dev::S_DQ] > 0)) { st.status = std::isfinite(S_host[dev::S_DQ]) ? 2 : 3; break; }
@@ -985,7 +1027,8 @@ void MGH::ensure_batch(int k, int precision)
         if (precision == 1) { BL[l].x32.alloc(n); BL[l].b32.alloc(n); BL[l].r32.alloc(n); BL[l].alt32.alloc(n); }
     }
     const size_t nd = (size_t)pr->ndof * k;
-    Br.alloc(nd); Bz.alloc(nd); Bzold.alloc(nd); Bd.alloc(nd); Bq.alloc(nd);
+    Br.alloc(nd); Bzold.alloc(nd); Bd.alloc(nd); Bq.alloc(nd);
+    if (precision == 1) Bzold32.alloc(nd);
     BS.alloc((size_t)k * dev::S_COUNT);
     Bpart.alloc((size_t)1024 * 3 * k + 64);
     Bactive.alloc(k);
@@ -1074,7 +1117,7 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].smooth : 0;
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, int nr, const int* cdev, int c, int skip) {
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, const int*, int nr, const int* cdev, int c, int skip) {
                 if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
                                              bx<T>(v), omega, cdev, c, skip);
                 else dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
@@ -1130,17 +1173,9 @@ void MGH::bvcycle(int k)
 void MGH::bprecon(int precision, int k, bool use_graph)
 {
     hipStream_t s = ctx->stream;
-    const long n = pr->ndof * (long)k;
+    // the V-cycle alone: right-hand sides in BL[0].b*, results in BL[0].x*
     auto enqueue = [&] {
-        if (precision == 1) {
-            dev::f64_to_f32(s, n, Br, BL[0].b32);
-            bvcycle<float>(k);
-            dev::f32_to_f64(s, n, BL[0].x32, Bz);
-        } else {
-            dev::Ops<double>::copy(s, n, Br, BL[0].b64);
-            bvcycle<double>(k);
-            dev::Ops<double>::copy(s, n, BL[0].x64, Bz);
-        }
+        if (precision == 1) bvcycle<float>(k); else bvcycle<double>(k);
     };
     if (!use_graph) { enqueue(); return; }
     hipGraphExec_t& g = bgraph[precision];
@@ -1176,14 +1211,28 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
     };
     auto push_active = [&] { HIPCHK(hipMemcpyAsync(Bactive.p, act.data(), k * sizeof(int), hipMemcpyHostToDevice, s)); };
     auto residual = [&] { bapply<double>(0, k, x, b, Br); };
+    // the preconditioner reads r in its own precision from BL[0].b* and
+    // leaves z in BL[0].x*; the CG kernels convert on the fly
+    const bool mixed = sp.precision == 1;
+    auto to_vin = [&] {
+        if (mixed) dev::f64_to_f32(s, n * k, Br, BL[0].b32); else dev::Ops<double>::copy(s, n * k, Br, BL[0].b64);
+    };
+    auto rz = [&](bool read_old) {
+        if (mixed) dev::bpcg_rz<float>(s, n, k, BS, Bactive, Br, BL[0].x32, sp.flexible ? Bzold32.p : nullptr, read_old, Bpart);
+        else dev::bpcg_rz<double>(s, n, k, BS, Bactive, Br, BL[0].x64, sp.flexible ? Bzold.p : nullptr, read_old, Bpart);
+    };
+    auto dupd = [&](bool init) {
+        if (mixed) dev::bpcg_d<float>(s, n, k, BS, Bactive, BL[0].x32, Bd, init);
+        else dev::bpcg_d<double>(s, n, k, BS, Bactive, BL[0].x64, Bd, init);
+    };
     HIPCHK(hipMemsetAsync(BS.p, 0, BS.n * sizeof(double), s));
     push_active();
     dev::coldot(s, n, k, b, b, BS.p + dev::S_BB, NS, Bpart);
     residual();
+    to_vin();
     bprecon(sp.precision, k, sp.use_graph != 0);
-    dev::bpcg_rz(s, n, k, BS, Bactive, Br, Bz, nullptr, Bpart);
-    dev::Ops<double>::copy(s, n * k, Bz, Bd);
-    if (sp.flexible) dev::Ops<double>::copy(s, n * k, Bz, Bzold);
+    rz(false);
+    dupd(true);
     fetch();
     std::vector<double> bnorm(k), delta0(k);
     for (int c = 0; c < k; ++c) { bnorm[c] = std::sqrt(Sh[(size_t)c * NS + dev::S_BB]); delta0[c] = Sh[(size_t)c * NS + dev::S_DELTA]; }
@@ -1220,13 +1269,13 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
             if (!redo.empty()) {
                 for (int c : redo) act[c] = 1;
                 push_active();
+                to_vin();
                 bprecon(sp.precision, k, sp.use_graph != 0);
-                dev::bpcg_rz(s, n, k, BS, Bactive, Br, Bz, nullptr, Bpart);
+                rz(false);
                 fetch();
                 for (int c : redo) Sh[(size_t)c * NS + dev::S_BETA] = 0.0;
                 HIPCHK(hipMemcpyAsync(BS.p, Sh.data(), Sh.size() * sizeof(double), hipMemcpyHostToDevice, s));
-                dev::bpcg_d(s, n, k, BS, Bactive, Bz, Bd);
-                if (sp.flexible) dev::Ops<double>::copy(s, n * k, Bz, Bzold);
+                dupd(false);
                 any = true;
             }
         }
@@ -1237,13 +1286,15 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
             dev::bpcg_x(s, n, k, BS, Bactive, Bd, x);
             residual();
+            to_vin();
+        } else if (mixed) {
+            dev::bpcg_xr<float>(s, n, k, BS, Bactive, Bd, Bq, x, Br, BL[0].b32.p, Bpart);
         } else {
-            dev::bpcg_xr(s, n, k, BS, Bactive, Bd, Bq, x, Br, Bpart);
+            dev::bpcg_xr<double>(s, n, k, BS, Bactive, Bd, Bq, x, Br, BL[0].b64.p, Bpart);
         }
         bprecon(sp.precision, k, sp.use_graph != 0);
-        dev::bpcg_rz(s, n, k, BS, Bactive, Br, Bz, sp.flexible ? Bzold.p : nullptr, Bpart);
-        if (sp.flexible) dev::Ops<double>::copy(s, n * k, Bz, Bzold);
-        dev::bpcg_d(s, n, k, BS, Bactive, Bz, Bd);
+        rz(sp.flexible != 0);
+        dupd(false);
         ++i;
         for (int c = 0; c < k; ++c) it[c] += act[c];
         fetch();
@@ -1289,8 +1340,8 @@ void MGH::spmv(int l, int precision, const void* x, void* y)
 {
     Level& lv = L[l];
     hipStream_t s = ctx->stream;
-    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr);
-    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr);
+    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A32, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr);
+    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A64, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr);
 }
 
 void MGH::smooth(int l, int precision, int n, bool reverse, const void* b, void* x)
@@ -1313,9 +1364,16 @@ void MGH::vcycle(int precision, const double* b, double* x)
 {
     hipStream_t s = ctx->stream;
     const long n = pr->ndof;
-    dev::Ops<double>::copy(s, n, b, r);
-    precon(precision, false);
-    dev::Ops<double>::copy(s, n, z, x);
+    if (precision == 1) {
+        if (!L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
+        dev::f64_to_f32(s, n, b, L[0].v32.b);
+        precon(precision, false);
+        dev::f32_to_f64(s, n, L[0].v32.x, x);
+    } else {
+        dev::Ops<double>::copy(s, n, b, L[0].v64.b);
+        precon(precision, false);
+        dev::Ops<double>::copy(s, n, L[0].v64.x, x);
+    }
 }
 
 // ---------------------------------------------------------------------------- measurement
@@ -1385,18 +1443,19 @@ void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* byte
                 for (int c = 0; c < nc - 2; ++c) {
                     const int o = lv.color_off[c], nr = lv.color_off[c + 1] - o;
                     const int* d = lv.color_desc.p + 4 * (size_t)o;
+                    const int* g = lv.color_gcols.p + (size_t)lv.gw * o;
                     if (precision == 1)
-                        dev::Ops<float>::gs_color(s, lv.B, nr, d, lv.A.cols, lv.A32, lv.D32, lv.v32.b, lv.v32.x, (float)prm.omega,
+                        dev::Ops<float>::gs_color(s, lv.B, nr, d, g, lv.gw, lv.A32, lv.D32, lv.v32.b, lv.v32.x, (float)prm.omega,
                                                   lv.color_dev, c, 0);
                     else
-                        dev::Ops<double>::gs_color(s, lv.B, nr, d, lv.A.cols, lv.A64, lv.D64, lv.v64.b, lv.v64.x, prm.omega,
+                        dev::Ops<double>::gs_color(s, lv.B, nr, d, g, lv.gw, lv.A64, lv.D64, lv.v64.b, lv.v64.x, prm.omega,
                                                    lv.color_dev, c, 0);
                 }
             else if (precision == 1)
-                dev::Ops<float>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.A.cols, lv.A32, lv.D32, lv.v32.b,
+                dev::Ops<float>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, lv.A32, lv.D32, lv.v32.b,
                                          lv.v32.x, (float)prm.omega, lv.color_dev, nc - 2, nc - 1, 0);
             else
-                dev::Ops<double>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.A.cols, lv.A64, lv.D64, lv.v64.b,
+                dev::Ops<double>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, lv.A64, lv.D64, lv.v64.b,
                                           lv.v64.x, prm.omega, lv.color_dev, nc - 2, nc - 1, 0);
         }
     };
@@ -1423,8 +1482,8 @@ void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes
     const int sa = precision == 1 ? 4 : 8;
     if (precision == 1) dev::Ops<float>::fill(s, n, 1.f, lv.v32.x); else dev::Ops<double>::fill(s, n, 1.0, lv.v64.x);
     auto once = [&] {
-        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A32, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr);
-        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.cols, lv.A64, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr);
+        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A32, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr);
+        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A64, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr);
     };
     once();
     hipEvent_t e0, e1;

@@ -48,7 +48,12 @@ struct DevOp {
     HostBSR pat;             // pattern (values empty)
     DBuf<int> rowptr, cols;
     DBuf<double> val;        // nnzb x br x bc row-major
+    // the rows' column indices at a fixed stride ellw (-1 past a row's
+    // blocks), for kernels that load them without rowptr (build_ell)
+    DBuf<int> ell;
+    int ellw = 0;
     void upload_pattern(const HostBSR& h, hipStream_t s);
+    void build_ell(hipStream_t s);
     long nnzb() const { return pat.nnzb(); }
 };
 
@@ -110,17 +115,19 @@ struct Level {
     // wavefront of the lexicographic order (forward; the reverse sweep has its
     // own wavefronts in rcolor*)
     // (*_desc: per row in that order {row, rowptr[row], rowptr[row+1],
-    // position of the diagonal block}, read by the single-RHS GS kernel)
+    // position of the diagonal block}; *_gcols: per row in that order the
+    // columns of its off-diagonal blocks at stride gw, -1 past the last; both
+    // read by the single-RHS GS kernels)
     std::vector<int> color, color_off;
-    int ncolor = 0;
-    DBuf<int> color_rows, color_dev, color_desc;
+    int ncolor = 0, gw = 1;
+    DBuf<int> color_rows, color_dev, color_desc, color_gcols;
     std::vector<int> rcolor, rcolor_off;
     int rncolor = 0;
-    DBuf<int> rcolor_rows, rcolor_dev, rcolor_desc;
+    DBuf<int> rcolor_rows, rcolor_dev, rcolor_desc, rcolor_gcols;
     // multicolour GS: the last two colours in one launch (pair_nwg > 0):
-    // per workgroup {s, m, e} into pair_desc, see dev::Ops::gs_pair
+    // per workgroup {s, m, e} into pair_desc / pair_gcols, see dev::Ops::gs_pair
     int pair_nwg = 0;
-    DBuf<int> pair_wg, pair_desc;
+    DBuf<int> pair_wg, pair_desc, pair_gcols;
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     // interpolation to the next coarser level
@@ -151,7 +158,8 @@ struct MGH {
     MGParams prm;
     std::vector<Level> L;
     // PCG work vectors (FP64) and scalars
-    DBuf<double> r, z, zold, d, q, S, xw, bw;
+    DBuf<double> r, zold, d, q, S, xw, bw;
+    DBuf<float> zold32;                // flexible CG, mixed form: previous z (exact in FP32)
     DBuf<double> part;                 // reduction partials (one per workgroup of the largest launch)
     double* S_host = nullptr;          // pinned
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -165,7 +173,8 @@ struct MGH {
     struct BatchLevel { DBuf<double> x64, b64, r64, alt64; DBuf<float> x32, b32, r32, alt32; };
     int bk = 0, bprec = -1;
     std::vector<BatchLevel> BL;
-    DBuf<double> Br, Bz, Bzold, Bd, Bq, BS, Bpart, Bgemm;
+    DBuf<double> Br, Bzold, Bd, Bq, BS, Bpart, Bgemm;
+    DBuf<float> Bzold32;
     DBuf<int> Bactive;
     hipGraphExec_t bgraph[2] = {nullptr, nullptr};
 
