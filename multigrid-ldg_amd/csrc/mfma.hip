@@ -48,28 +48,88 @@ template <> struct Mf<double> {
     static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
 };
 
-// acc[rt * NT + t] += A_blk X_j (rows padded to 16 RT, K to a multiple of 4)
+// Operands of one block product A_blk X_j for this lane: per K step s the A
+// element of each row tile and the X element of each column tile (rows padded
+// to 16 RT, K to a multiple of 4, with zeros).
 template <typename T, int B, int NT>
-__device__ __forceinline__ void block_mac(typename Mf<T>::v4* acc, const T* __restrict__ blk, const T* __restrict__ xj,
-                                          int k, int col0, int lane)
+struct BlkOps {
+    static constexpr int RT = (B + 15) / 16, S = (B + 3) / 4;
+    T a[S][RT];
+    T xb[S][NT];
+};
+
+// Blocks whose operands are loaded before any of their products (one memory
+// round trip per U blocks), sized to keep the operands near 128 VGPRs.
+template <typename T, int B, int NT>
+__host__ __device__ constexpr int mf_unroll()
 {
-    constexpr int RT = (B + 15) / 16;
+    constexpr int per = ((B + 3) / 4) * ((B + 15) / 16 + NT) * (int)(sizeof(T) / 4);
+    return per * 4 <= 128 ? 4 : (per * 3 <= 128 ? 3 : (per * 2 <= 128 ? 2 : 1));
+}
+
+template <typename T, int B, int NT>
+__device__ __forceinline__ void blk_load(BlkOps<T, B, NT>& o, const T* __restrict__ blk, const T* __restrict__ xj, int k,
+                                         int col0, int lane)
+{
     const int r16 = lane & 15, q = lane >> 4;
 #pragma unroll
-    for (int s = 0; s < (B + 3) / 4; ++s) {
+    for (int s = 0; s < BlkOps<T, B, NT>::S; ++s) {
         const int c = 4 * s + q;
-        T xb[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int col = col0 + 16 * t + r16;
-            xb[t] = (c < B && col < k) ? xj[(size_t)c * k + col] : (T)0;
+            o.xb[s][t] = (c < B && col < k) ? xj[(size_t)c * k + col] : (T)0;
         }
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
+        for (int rt = 0; rt < BlkOps<T, B, NT>::RT; ++rt) {
             const int r = 16 * rt + r16;
-            const T a = (r < B && c < B) ? panel_at<T, B>(blk, r, c) : (T)0;
+            o.a[s][rt] = (r < B && c < B) ? panel_at<T, B>(blk, r, c) : (T)0;
+        }
+    }
+}
+
+template <typename T, int B, int NT>
+__device__ __forceinline__ void blk_mac(typename Mf<T>::v4* acc, const BlkOps<T, B, NT>& o)
+{
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[rt * NT + t] = Mf<T>::mac(a, xb[t], acc[rt * NT + t]);
+    for (int s = 0; s < BlkOps<T, B, NT>::S; ++s)
+#pragma unroll
+        for (int rt = 0; rt < BlkOps<T, B, NT>::RT; ++rt)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[rt * NT + t] = Mf<T>::mac(o.a[s][rt], o.xb[s][t], acc[rt * NT + t]);
+}
+
+// acc += sum over the blocks k0 .. k1-1 of row i (one row per wavefront, so
+// every branch below is uniform): the row's column indices are loaded once,
+// one per lane, and broadcast with shuffles; the operands of U blocks are in
+// flight together.  SKIP: -1 = every block; 0 = not the diagonal block;
+// 1 (2) = also not the colours after (before) `cur` (first GS sweep from x = 0).
+template <typename T, int B, int NT, int SKIP>
+__device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, int k1, const int* __restrict__ cols,
+                                        const T* __restrict__ A, const T* __restrict__ x, int k, int col0, int lane,
+                                        const int* __restrict__ color, int cur)
+{
+    constexpr int U = mf_unroll<T, B, NT>();
+    const int deg = k1 - k0;
+    for (int m0 = 0; m0 < deg; m0 += 64) {
+        const int mycol = (m0 + lane < deg) ? cols[k0 + m0 + lane] : -1;
+        const int nc = min(64, deg - m0);
+        for (int u0 = 0; u0 < nc; u0 += U) {
+            BlkOps<T, B, NT> ops[U];
+            bool use[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int m = u0 + u;
+                const int j = __shfl(mycol, min(m, 63), 64);
+                bool ok = m < nc && j >= 0;
+                if (SKIP >= 0 && ok) ok = j != i;
+                if (SKIP > 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
+                use[u] = ok;
+                if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)(k0 + m0 + m) * B * B, x + (size_t)j * B * k, k, col0, lane);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (use[u]) blk_mac<T, B, NT>(acc, ops[u]);
         }
     }
 }
@@ -90,9 +150,7 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    const int k1 = rowptr[i + 1];
-    for (int kk = rowptr[i]; kk < k1; ++kk)
-        block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)cols[kk] * B * k, k, col0, lane);
+    row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0);
     const int col_l = lane & 15;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -148,11 +206,11 @@ __device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typena
 // damped block GS on the rows of one colour (multigrid.h:240-249):
 // x_i += omega (D_i^-1 (b_i - sum_{j != i} A_ij x_j) - x_i); skip as in the
 // vector kernel (first sweep from zero: later colours are not read)
-template <typename T, int B, int NT>
+template <typename T, int B, int NT, int SKIP>
 __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, const int* __restrict__ rows,
                                              const int* __restrict__ rowptr, const int* __restrict__ cols,
                                              const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
-                                             T* x, T omega, const int* __restrict__ color, int cur, int skip)
+                                             T* x, T omega, const int* __restrict__ color, int cur)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
@@ -167,15 +225,7 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid) {
-        const int k1 = rowptr[i + 1];
-        for (int kk = rowptr[i]; kk < k1; ++kk) {
-            const int j = cols[kk];
-            if (j == i) continue;
-            if (skip) { const int cj = color[j]; if (skip == 1 ? cj > cur : cj < cur) continue; }
-            block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)j * B * k, k, col0, lane);
-        }
-    }
+    if (valid) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur);
     V4 res[RT * NT];
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -202,7 +252,7 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
                 const int r = 16 * rr + Mf<T>::row(lane, e);
                 if (r >= B) continue;
                 const size_t o = ((size_t)i * B + r) * k + col;
-                const T xi = skip ? (T)0 : x[o];
+                const T xi = SKIP ? (T)0 : x[o];
                 x[o] = xi + omega * (out[rr * NT + t][e] - xi);
             }
         }
@@ -228,11 +278,7 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid) {
-        const int k1 = rowptr[i + 1];
-        for (int kk = rowptr[i]; kk < k1; ++kk)
-            block_mac<T, B, NT>(acc, A + (size_t)kk * B * B, x + (size_t)cols[kk] * B * k, k, col0, lane);
-    }
+    if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0);
     V4 res[RT * NT];
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -372,8 +418,17 @@ void MF<T>::gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* 
 {
     if (nrows <= 0 || k <= 0) return;
     const int ng = (k + 16 * nt - 1) / (16 * nt);
-    MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
-                                                nrows, ng, rows, rowptr, cols, A, Dinv, b, x, omega, color, cur, skip));
+    const dim3 g(nblocks((long)nrows * ng));
+    if (skip == 1) {
+        MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 1>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
+                                                    A, Dinv, b, x, omega, color, cur));
+    } else if (skip == 2) {
+        MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 2>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
+                                                    A, Dinv, b, x, omega, color, cur));
+    } else {
+        MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 0>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
+                                                    A, Dinv, b, x, omega, color, cur));
+    }
     check("mf_gs");
 }
 
