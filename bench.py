@@ -118,6 +118,7 @@ def unstructured_leg(mgldg, torch, nrhs, warm):
     nrhs right-hand sides batched in FP64 and in mixed precision."""
     t0 = time.perf_counter()
     pts, tri = disk_mesh()
+    tri = tri[mgldg.locality_order(pts, tri)]
     t_mesh = time.perf_counter() - t0
     ctx = mgldg.Context(0)
     t0 = time.perf_counter()
@@ -328,6 +329,7 @@ def ball_leg(mgldg, torch, nrhs=16, warm=1):
     FP64 single-column solve."""
     t0 = time.perf_counter()
     pts, tets = ball_mesh()
+    tets = tets[mgldg.locality_order(pts, tets)]
     t_mesh = time.perf_counter() - t0
     ctx = mgldg.Context(0)
     t0 = time.perf_counter()

@@ -214,6 +214,28 @@ class SimplexLDGPoisson(_Problem):
         self._sizes()
 
 
+def locality_order(points, simplices):
+    """Permutation of the simplices along a Morton (Z-order) curve of their
+    centroids.  Block rows follow element order, so the x_j gathers of the
+    block kernels stay in cache when neighbouring elements have nearby
+    indices; meshes from generators that emit elements in a scattered order
+    (qhull's Delaunay triangulations, lattice splits emitted pattern by
+    pattern) gain most.  Use as simplices = simplices[locality_order(points,
+    simplices)] before building the Wireframe."""
+    P = np.asarray(points, dtype=np.float64)
+    S = np.asarray(simplices)
+    c = P[S].mean(axis=1)
+    d = c.shape[1]
+    bits = min(21, 63 // d)
+    lo, hi = c.min(axis=0), c.max(axis=0)
+    q = ((c - lo) / np.maximum(hi - lo, 1e-300) * ((1 << bits) - 1)).astype(np.uint64)
+    code = np.zeros(len(c), dtype=np.uint64)
+    for b in range(bits):
+        for a in range(d):
+            code |= ((q[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(b * d + a)
+    return np.argsort(code, kind="stable")
+
+
 def agglomerate(prob, max_levels=10):
     """The library's Agglomeration<N,P>(mesh, max_levels) (agglomeration.h:21-34):
     list of partitions, level l elements -> level l+1 agglomerates."""
