@@ -951,6 +951,91 @@ __global__ __launch_bounds__(256) void mk_prolong(int k, int nf, const int* __re
     }
 }
 
+// The same two maps with the interpolation blocks staged in LDS: a workgroup
+// takes G = 256 / k block rows (k >= 16), copies the T blocks they use into
+// LDS once, and the k column threads of a row read them back as broadcasts
+// (the per-thread kernels above issue one cached global load per T entry and
+// column).  Sums run in the same order as above.
+template <typename T, int BF, int BC>
+__global__ __launch_bounds__(256) void mk_restrict_l(int k, int nc, const int* __restrict__ kid_off, const int* __restrict__ kids,
+                                                     const int* __restrict__ type, const T* __restrict__ pool,
+                                                     const T* __restrict__ r, T* __restrict__ bc)
+{
+    extern __shared__ __align__(16) unsigned char mk_lds[];
+    T* Ts = reinterpret_cast<T*>(mk_lds);
+    __shared__ int maxkids;
+    constexpr int TB = BF * BC;
+    const int G = 256 / k;
+    const int I0 = blockIdx.x * G, ng = min(G, nc - I0);
+    const int g = threadIdx.x / k, c = threadIdx.x - g * k;
+    const bool valid = g < ng;
+    const int t0 = valid ? kid_off[I0 + g] : 0, t1 = valid ? kid_off[I0 + g + 1] : 0;
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int q = 0; q < ng; ++q) m = max(m, kid_off[I0 + q + 1] - kid_off[I0 + q]);
+        maxkids = m;
+    }
+    __syncthreads();
+    const int mk = maxkids;
+    T acc[BC];
+#pragma unroll
+    for (int p = 0; p < BC; ++p) acc[p] = 0;
+    for (int t = 0; t < mk; ++t) {
+        for (int e = threadIdx.x; e < ng * TB; e += 256) {
+            const int q = e / TB, o = e - q * TB;
+            const int a = kid_off[I0 + q], b = kid_off[I0 + q + 1];
+            Ts[e] = (a + t < b) ? pool[(size_t)type[kids[a + t]] * TB + o] : (T)0;
+        }
+        __syncthreads();
+        if (valid && t0 + t < t1) {
+            const int f = kids[t0 + t];
+            const T* rf = r + (size_t)f * BF * k + c;
+            const T* Tb = Ts + g * TB;
+#pragma unroll
+            for (int q = 0; q < BF; ++q) {
+                const T rv = rf[(size_t)q * k];
+#pragma unroll
+                for (int p = 0; p < BC; ++p) acc[p] += Tb[q * BC + p] * rv;
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int p = 0; p < BC; ++p) bc[((size_t)(I0 + g) * BC + p) * k + c] = acc[p];
+}
+
+template <typename T, int BF, int BC>
+__global__ __launch_bounds__(256) void mk_prolong_l(int k, int nf, const int* __restrict__ parent, const int* __restrict__ type,
+                                                    const T* __restrict__ pool, const T* __restrict__ xc, T* __restrict__ xf)
+{
+    extern __shared__ __align__(16) unsigned char mk_lds[];
+    T* Ts = reinterpret_cast<T*>(mk_lds);
+    constexpr int TB = BF * BC;
+    const int G = 256 / k;
+    const int f0 = blockIdx.x * G, ng = min(G, nf - f0);
+    for (int e = threadIdx.x; e < ng * TB; e += 256) {
+        const int q = e / TB, o = e - q * TB;
+        Ts[e] = pool[(size_t)type[f0 + q] * TB + o];
+    }
+    __syncthreads();
+    const int g = threadIdx.x / k, c = threadIdx.x - g * k;
+    if (g >= ng) return;
+    const int f = f0 + g;
+    const T* xp = xc + (size_t)parent[f] * BC * k + c;
+    T xv[BC];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) xv[q] = xp[(size_t)q * k];
+    const T* Tb = Ts + g * TB;
+#pragma unroll
+    for (int rr = 0; rr < BF; ++rr) {
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < BC; ++q) acc += Tb[rr * BC + q] * xv[q];
+        xf[((size_t)f * BF + rr) * k + c] += acc;
+    }
+}
+
 // one workgroup per column
 template <typename T>
 __global__ __launch_bounds__(256) void mk_coarse(int n, int k, const double* __restrict__ Cinv, T* b, T* x, int subtract_mean)
@@ -1004,8 +1089,15 @@ void MOps<T>::restrict_(hipStream_t s, int BF, int BC, int k, int nc, const int*
                         const T* pool, const T* r, T* bc)
 {
     long n = (long)nc * k; if (n <= 0) return;
-    MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_restrict<T, FF, CC>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nc, kid_off,
-                                               kids, type, pool, r, bc));
+    const size_t lds = (size_t)(k >= 16 && k <= 256 ? 256 / k : 0) * BF * BC * sizeof(T);
+    if (lds > 0 && lds <= 65536) {
+        const int G = 256 / k;
+        MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_restrict_l<T, FF, CC>), dim3((nc + G - 1) / G), dim3(256), lds, s, k, nc,
+                                                   kid_off, kids, type, pool, r, bc));
+    } else {
+        MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_restrict<T, FF, CC>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nc, kid_off,
+                                                   kids, type, pool, r, bc));
+    }
     check_launch("mk_restrict");
 }
 template <typename T>
@@ -1013,8 +1105,15 @@ void MOps<T>::prolong(hipStream_t s, int BF, int BC, int k, int nf, const int* p
                       const T* xc, T* xf)
 {
     long n = (long)nf * k; if (n <= 0) return;
-    MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_prolong<T, FF, CC>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nf, parent,
-                                               type, pool, xc, xf));
+    const size_t lds = (size_t)(k >= 16 && k <= 256 ? 256 / k : 0) * BF * BC * sizeof(T);
+    if (lds > 0 && lds <= 65536) {
+        const int G = 256 / k;
+        MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_prolong_l<T, FF, CC>), dim3((nf + G - 1) / G), dim3(256), lds, s, k, nf,
+                                                   parent, type, pool, xc, xf));
+    } else {
+        MGLDG_DISPATCH2(BF, BC, hipLaunchKernelGGL((mk_prolong<T, FF, CC>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nf, parent,
+                                                   type, pool, xc, xf));
+    }
     check_launch("mk_prolong");
 }
 template <typename T>
