@@ -433,6 +433,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
     ap.add_argument("--no-ball", action="store_true", help="skip the 3-D ball workload")
     ap.add_argument("--ball-only", action="store_true", help="only the 3-D ball workload")
+    ap.add_argument("--unstructured-only", action="store_true", help="only the unit-disk workload")
     ap.add_argument("--batched-only", action="store_true",
                     help="setup + two mixed and one FP64 batched solves of --nrhs columns (profiling)")
     args = ap.parse_args()
@@ -452,6 +453,9 @@ def main():
         return
     if args.ball_only:
         print(json.dumps({"ball": ball_leg(mgldg, torch, warm=args.warmup)}), flush=True)
+        return
+    if args.unstructured_only:
+        print(json.dumps({"unstructured": unstructured_leg(mgldg, torch, args.nrhs, args.warmup)}), flush=True)
         return
 
     torch.cuda.init()
