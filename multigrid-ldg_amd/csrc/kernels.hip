@@ -904,6 +904,120 @@ __global__ __launch_bounds__(256) void mk_jacobi(int k, int nrows, const int* __
     }
 }
 
+// Batched row kernels for narrow blocks (B <= 10) and k = 16, 32 or 64
+// columns: the k lanes of a block row are the k columns, 64 / k rows per
+// wavefront.  Latency is what bounds the per-thread kernels above (one block
+// in flight per wavefront, rowptr -> cols -> x serial per block), so here:
+// the row's column indices are loaded once, one per lane, and shuffled; the A
+// blocks of MK_U consecutive positions (contiguous in storage) are staged in
+// LDS by the row's lanes together and read back as broadcasts; each lane
+// holds only its column's x_j values, so MK_U blocks are in flight per
+// wavefront.  Sums run in the same order as the per-thread kernels.
+// MODE 0: y = A x or b - A x; 1: block Jacobi; 2: block GS on one colour
+// (rows[]), SKIP as in mk_gs.
+constexpr int MK_U = 4;
+
+template <typename T, int B, int MODE, int SKIP>
+__global__ __launch_bounds__(256) void mk_row(int k, int nrows, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                              const int* __restrict__ cols, const T* __restrict__ A, const T* __restrict__ Dinv,
+                                              const T* __restrict__ b, const T* x, T* y, T omega,
+                                              const int* __restrict__ color, int cur)
+{
+    extern __shared__ __align__(16) unsigned char mk_row_lds[];
+    constexpr int B2 = B * B;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rpw = 64 / k, slot = lane / k, c = lane - slot * k, lane0 = slot * k;
+    T* La = reinterpret_cast<T*>(mk_row_lds) + (size_t)(wv * rpw + slot) * (MK_U * B2);
+    const int ri = (blockIdx.x * 4 + wv) * rpw + slot;
+    const bool valid = ri < nrows;
+    const int i = valid ? (MODE == 2 ? rows[ri] : ri) : 0;
+    const int k0 = valid ? rowptr[i] : 0, deg = valid ? rowptr[i + 1] - k0 : 0;
+    int wl = deg;
+    for (int o = 32; o > 0; o >>= 1) wl = max(wl, __shfl_xor(wl, o, 64));
+    T t[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) t[r] = (MODE != 0 && valid) ? b[((size_t)i * B + r) * k + c] : (T)0;
+    for (int m0 = 0; m0 < wl; m0 += k) {
+        const int mycol = (valid && m0 + c < deg) ? cols[k0 + m0 + c] : -1;
+        const int nc = min(k, wl - m0);
+        for (int u0 = 0; u0 < nc; u0 += MK_U) {
+            const int nb = valid ? max(0, min(MK_U, deg - (m0 + u0))) : 0;
+            const T* As = A + (size_t)(k0 + m0 + u0) * B2;
+            for (int e = c; e < nb * B2; e += k) La[e] = As[e];
+            T xv[MK_U][B];
+            bool use[MK_U];
+#pragma unroll
+            for (int u = 0; u < MK_U; ++u) {
+                const int m = u0 + u;
+                const int j = __shfl(mycol, lane0 + min(m, k - 1), 64);
+                bool ok = valid && m < nc && j >= 0;
+                if (MODE == 2 && ok) ok = j != i;
+                if (SKIP != 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
+                use[u] = ok;
+#pragma unroll
+                for (int q = 0; q < B; ++q) xv[u][q] = ok ? x[((size_t)j * B + q) * k + c] : (T)0;
+            }
+            wave_sync();
+#pragma unroll
+            for (int u = 0; u < MK_U; ++u)
+                if (use[u]) {
+                    const T* Ab = La + u * B2;
+#pragma unroll
+                    for (int q = 0; q < B; ++q)
+#pragma unroll
+                        for (int r = 0; r < B; ++r) {
+                            if (MODE == 0) t[r] += pan<T, B>(Ab, r, q) * xv[u][q];
+                            else t[r] -= pan<T, B>(Ab, r, q) * xv[u][q];
+                        }
+                }
+            wave_sync();
+        }
+    }
+    if (!valid) return;
+    if (MODE == 0) {
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const size_t o = ((size_t)i * B + r) * k + c;
+            y[o] = b ? (T)(b[o] - t[r]) : t[r];
+        }
+        return;
+    }
+    const T* D = Dinv + (size_t)i * B2;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        T sdot = 0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) sdot += pan<T, B>(D, r, q) * t[q];
+        const size_t o = ((size_t)i * B + r) * k + c;
+        if (MODE == 1) y[o] = x[o] + omega * sdot;
+        else {
+            const T xi = SKIP ? (T)0 : y[o];
+            y[o] = xi + omega * (sdot - xi);
+        }
+    }
+}
+
+// launch shape of mk_row, or 0 when the per-thread kernels are to be used
+template <typename T>
+static size_t mk_row_shape(int B, int k)
+{
+    if (B > 10 || !(k == 16 || k == 32 || k == 64)) return 0;
+    const size_t lds = (size_t)4 * (64 / k) * MK_U * B * B * sizeof(T);
+    return lds <= 32768 ? lds : 0;
+}
+
+#define MGLDG_DISPATCH_SMALL(B, ...)                                                 \
+    switch (B) {                                                                     \
+        case 1: { constexpr int BB = 1; __VA_ARGS__; } break;                        \
+        case 3: { constexpr int BB = 3; __VA_ARGS__; } break;                        \
+        case 4: { constexpr int BB = 4; __VA_ARGS__; } break;                        \
+        case 6: { constexpr int BB = 6; __VA_ARGS__; } break;                        \
+        case 8: { constexpr int BB = 8; __VA_ARGS__; } break;                        \
+        case 9: { constexpr int BB = 9; __VA_ARGS__; } break;                        \
+        case 10: { constexpr int BB = 10; __VA_ARGS__; } break;                      \
+        default: throw std::runtime_error("mk_row: block size " + std::to_string(B)); \
+    }
+
 template <typename T, int BF, int BC>
 __global__ __launch_bounds__(256) void mk_restrict(int k, int nc, const int* __restrict__ kid_off, const int* __restrict__ kids,
                                                    const int* __restrict__ type, const T* __restrict__ pool,
@@ -1063,6 +1177,13 @@ void MOps<T>::apply(hipStream_t s, int B, int k, int nrows, const int* rowptr, c
                     const T* b, T* y)
 {
     long n = (long)nrows * k; if (n <= 0) return;
+    if (const size_t lds = mk_row_shape<T>(B, k)) {
+        const dim3 g(nblk(nblk(nrows, 64 / k), 4));
+        MGLDG_DISPATCH_SMALL(B, hipLaunchKernelGGL((mk_row<T, BB, 0, 0>), g, dim3(256), lds, s, k, nrows, nullptr, rowptr, cols, A,
+                                                   nullptr, b, x, y, (T)0, nullptr, 0));
+        check_launch("mk_row");
+        return;
+    }
     MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_apply<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, rowptr, cols, A, x, b, y));
     check_launch("mk_apply");
 }
@@ -1071,6 +1192,21 @@ void MOps<T>::gs_color(hipStream_t s, int B, int k, int nrows, const int* rows, 
                        const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
 {
     long n = (long)nrows * k; if (n <= 0) return;
+    if (const size_t lds = mk_row_shape<T>(B, k)) {
+        const dim3 g(nblk(nblk(nrows, 64 / k), 4));
+        if (skip == 1) {
+            MGLDG_DISPATCH_SMALL(B, hipLaunchKernelGGL((mk_row<T, BB, 2, 1>), g, dim3(256), lds, s, k, nrows, rows, rowptr, cols,
+                                                       A, Dinv, b, x, x, omega, color, cur));
+        } else if (skip == 2) {
+            MGLDG_DISPATCH_SMALL(B, hipLaunchKernelGGL((mk_row<T, BB, 2, 2>), g, dim3(256), lds, s, k, nrows, rows, rowptr, cols,
+                                                       A, Dinv, b, x, x, omega, color, cur));
+        } else {
+            MGLDG_DISPATCH_SMALL(B, hipLaunchKernelGGL((mk_row<T, BB, 2, 0>), g, dim3(256), lds, s, k, nrows, rows, rowptr, cols,
+                                                       A, Dinv, b, x, x, omega, color, cur));
+        }
+        check_launch("mk_row");
+        return;
+    }
     MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_gs<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, rows, rowptr, cols, A,
                                          Dinv, b, x, omega, color, cur, skip));
     check_launch("mk_gs");
@@ -1080,6 +1216,13 @@ void MOps<T>::jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, 
                      const T* b, const T* x, T* xout, T omega)
 {
     long n = (long)nrows * k; if (n <= 0) return;
+    if (const size_t lds = mk_row_shape<T>(B, k)) {
+        const dim3 g(nblk(nblk(nrows, 64 / k), 4));
+        MGLDG_DISPATCH_SMALL(B, hipLaunchKernelGGL((mk_row<T, BB, 1, 0>), g, dim3(256), lds, s, k, nrows, nullptr, rowptr, cols, A,
+                                                   Dinv, b, x, xout, omega, nullptr, 0));
+        check_launch("mk_row");
+        return;
+    }
     MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_jacobi<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, rowptr, cols, A,
                                          Dinv, b, x, xout, omega));
     check_launch("mk_jacobi");
