@@ -22,6 +22,8 @@ MESHES = {
     "uniform8": (2, lambda x: 1 / 8),
     "adaptive": (2, h_adaptive(0.25)),
     "uniform3d": (3, lambda x: 1 / 4),
+    # refined towards (0.3, 0.7) to depth 7 (the benchmark's rule at a small size)
+    "point": (2, lambda x: max(2.0 ** -7, min(float(np.hypot(x[0] - 0.3, x[1] - 0.7)) / 28.8, 1 / 16))),
 }
 
 
@@ -113,6 +115,38 @@ def test_kernels_match_oracle(gctx, oracle, relax, collapse):
     assert rel(z.cpu().numpy(), zo) < 1e-12
     mg.vcycle(torch.tensor(b, device=dev), z, precision="mixed")
     assert rel(z.cpu().numpy(), zo) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_paired_last_colours_match_oracle(gctx, oracle, precision):
+    """Levels whose last two colours run in one launch (clusters of the
+    last colour's rows with their neighbours, one workgroup each): one and two
+    sweeps, forward and reverse, equal the oracle's colour-by-colour sweeps"""
+    import mgldg
+    import torch
+    qt, pr, oqt, opr = build_pair(gctx, oracle, "point", 2, rapid=True, balance=True)
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, precision="mixed", coarse_dense_max=0)
+    omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2)
+    paired = []
+    for l in range(mg.nlevels):
+        try:
+            mg.profile_relax(l, "fp64", 1, part="pair")
+            paired.append(l)
+        except mgldg.MGLDGError:
+            pass
+    assert 0 in paired, ("the fine level has no paired launch", paired)
+    rng = np.random.default_rng(11)
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for l in paired:
+        assert mg.colors(l)[1] >= 3
+        n = mg.level_rows[l] * pr.npl
+        x, b = rng.standard_normal(n), rng.standard_normal(n)
+        for reverse in (False, True):
+            for nsw in (1, 2):
+                xs = torch.tensor(x, dtype=dt, device="cuda")
+                mg.smooth(l, torch.tensor(b, dtype=dt, device="cuda"), xs, nsw, reverse, precision=precision)
+                assert rel(xs.double().cpu().numpy(), omg.relax(l, b, x, nsw, reverse)) < tol, (l, reverse, nsw)
 
 
 @pytest.mark.parametrize("collapse", [0, 4096])
