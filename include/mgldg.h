@@ -264,6 +264,20 @@ int mgldg_smooth_device(mgldg_mg mg, int level, int precision, int nsweeps, int 
                         const void* b, void* x, int nrhs);
 /* x = V(b) from x = 0 (one preconditioner application, dg_tests.cpp:300-305) */
 int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, int nrhs);
+/* The driver's stationary solver `--solver vcycles` (dg_tests.cpp:309-332):
+ * up to ncycle V-cycles (Multigrid::vcycle, multigrid.h:361-383) from the
+ * iterate in x (device, in/out).  rnorm (host, ncycle + 1 doubles, or NULL)
+ * receives ||b - A x|| before and after each cycle; *cycles the number run.
+ * tol > 0 stops once ||b - A x|| <= tol ||b|| (tol = 0: exactly ncycle, as
+ * the driver).  precision MGLDG_MIXED: x += V(b - A x) with an FP32 V-cycle
+ * and the residual and update in FP64. */
+int mgldg_vcycles_device(mgldg_mg mg, int precision, const double* b, double* x, int ncycle, double tol, double* rnorm,
+                         int* cycles);
+/* Function::meanZero (src/cartesian/dg/function.h:156-170) on the device:
+ * subtract from each of the nrhs columns of v ([n][nrhs]) its mean; the
+ * driver applies it to the right-hand side of Neumann and periodic problems
+ * (dg_tests.cpp:282-286). */
+int mgldg_mean_zero_device(mgldg_ctx ctx, long n, double* v, int nrhs);
 
 /* ======================= measurement hooks ======================= */
 /* No reference counterpart (the reference only prints a Timer).  Times

@@ -545,6 +545,31 @@ int mgldg_vcycle_device(mgldg_mg mg, int precision, const double* b, double* x, 
     });
 }
 
+int mgldg_vcycles_device(mgldg_mg mg, int precision, const double* b, double* x, int ncycle, double tol, double* rnorm,
+                         int* cycles)
+{
+    if (!mg || !b || !x || ncycle < 0) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        ARG(tol >= 0, "tol must be >= 0");
+        mg->m.vcycles(precision, b, x, ncycle, tol, rnorm, cycles);
+        HIPCHK(hipStreamSynchronize(mg->ctx->stream));
+    });
+}
+
+int mgldg_mean_zero_device(mgldg_ctx ctx, long n, double* v, int nrhs)
+{
+    if (!ctx || !v || n <= 0 || nrhs < 1) return MGLDG_ERR_ARG;
+    Ctx* c = &ctx->c;
+    return guard(c, [&] {
+        ARG(nrhs <= 256, "at most 256 right-hand sides per call");
+        const size_t need = dev::mean_zero_part_size(n, nrhs);
+        if (c->colpart.n < need) c->colpart.alloc(need);
+        dev::mean_zero(c->stream, n, nrhs, v, c->colpart.p);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
 int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double* ms, double* bytes, int* launches)
 {
     if (!mg || nrep < 1) return MGLDG_ERR_ARG;

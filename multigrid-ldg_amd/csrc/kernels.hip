@@ -1510,6 +1510,40 @@ void coldot(hipStream_t s, long n, int k, const double* a, const double* b, doub
     check_launch("colfinalize");
 }
 
+// per-column sums of a [n][k] array, then a -= sum / n per column
+// (Function::meanZero, src/cartesian/dg/function.h:156-170, for each column)
+__global__ __launch_bounds__(256) void k_colsum(long n, int k, const double* __restrict__ a, double* part)
+{
+    const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
+    double acc = 0;
+    if (g < groups)
+        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) acc += a[i * k + c];
+    col_partials(k, 1, &acc, part);
+}
+
+__global__ void k_colsub(long n, int k, const double* __restrict__ sums, double* __restrict__ a)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * k) return;
+    a[e] -= sums[e % k] / (double)n;
+}
+
+size_t mean_zero_part_size(long n, int k) { return (size_t)red_grid(n, k) * k + k; }
+
+void mean_zero(hipStream_t s, long n, int k, double* a, double* part)
+{
+    if (n <= 0 || k <= 0) return;
+    if (k > 256) throw std::runtime_error("mean_zero: more than 256 columns");
+    const unsigned g = red_grid(n, k);
+    double* sums = part + (size_t)g * k;
+    hipLaunchKernelGGL(k_colsum, dim3(g), dim3(256), 0, s, n, k, a, part);
+    check_launch("colsum");
+    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 1, sums, 1, nullptr, nullptr, 0);
+    check_launch("colfinalize");
+    hipLaunchKernelGGL(k_colsub, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, sums, a);
+    check_launch("colsub");
+}
+
 // per active column: x += alpha d ; r -= alpha q ; every column: v = (T) r,
 // the V-cycle's level-0 right-hand sides
 template <typename T>

@@ -148,6 +148,10 @@ void pcg_d(hipStream_t s, long n, const double* S, const T* zs, double* d, bool 
 // Per-column scalars live at S[c * S_COUNT + slot]; active[c] = 0 freezes column c.
 // per-column a.b into out[c * ostride] (deterministic two-pass)
 void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part);
+// a[:, c] -= mean(a[:, c]) for each of the k columns (deterministic); part
+// holds mean_zero_part_size(n, k) doubles
+size_t mean_zero_part_size(long n, int k);
+void mean_zero(hipStream_t s, long n, int k, double* a, double* part);
 template <typename T>
 void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const double* d, const double* q, double* x,
              double* r, T* v, double* part);

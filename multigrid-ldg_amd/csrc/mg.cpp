@@ -188,11 +188,17 @@ void ProblemH::upload()
     for (int d = 0; d < N; ++d) up(G[d], H.G[d]);
     build_laplacian(s, N, b, T, G, M, A);
     // boundary data for the device RHS / adjoint
+    // (at least one element each, so that a mesh without boundary faces -- a
+    // periodic quadtree -- still passes valid pointers)
     const int nbf = H.nbf;
-    Lb.upload(H.Lb.data(), std::max<size_t>(H.Lb.size(), 1), s);
-    bleft.upload(H.bleft.data(), std::max(nbf, 1), s);
-    btype.upload(H.btype.data(), std::max(nbf, 1), s);
-    bnormal.upload(H.bnormal.data(), std::max<size_t>(H.bnormal.size(), 1), s);
+    auto up1 = [&](auto& D, const auto& v) {
+        if (D.n < 1) D.alloc(1);
+        D.upload(v.data(), v.size(), s);
+    };
+    up1(Lb, H.Lb);
+    up1(bleft, H.bleft);
+    up1(btype, H.btype);
+    up1(bnormal, H.bnormal);
     std::vector<int> off(ne + 1, 0), lst(std::max(nbf, 1));
     for (int k = 0; k < nbf; ++k) off[H.bleft[k] + 1]++;
     for (int e = 0; e < ne; ++e) off[e + 1] += off[e];
@@ -316,17 +322,27 @@ static void build_interp(const Quadtree& qt, const Basis& B, int l, Interp& I)
 }
 
 // invert a dense SPD(-semidefinite) matrix on the host by Gauss-Jordan with
-// partial pivoting (zero pivots give zero rows, as LDLT::solve does)
+// partial pivoting.  Zero pivots give zero rows, as LDLT::solve does; a pivot
+// below 1e-13 of the matrix's largest entry counts as zero.  Without that, the
+// rounding-level pivot of a semidefinite matrix (the coarsest operator of a
+// Neumann or periodic problem, whose kernel is the constants) puts ~1e16
+// entries in the inverse, and their products with a mean-zero right-hand side
+// lose every digit, not only the kernel component.  With it, the result is a
+// generalised inverse whose solutions differ from the LDLT solve's only in
+// the kernel.
 static std::vector<double> dense_inverse(std::vector<double> A, int n)
 {
     std::vector<double> X((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) X[(size_t)i * n + i] = 1.0;
+    double amax = 0;
+    for (double v : A) amax = std::max(amax, std::fabs(v));
+    const double zero_pivot = std::max(1e-13 * amax, 2.2250738585072014e-308);
     for (int k = 0; k < n; ++k) {
         int p = k;
         for (int i = k + 1; i < n; ++i) if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)p * n + k])) p = i;
         if (p != k) for (int j = 0; j < n; ++j) { std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]); std::swap(X[(size_t)k * n + j], X[(size_t)p * n + j]); }
         double d = A[(size_t)k * n + k];
-        double inv = (std::fabs(d) > 2.2250738585072014e-308) ? 1.0 / d : 0.0;
+        double inv = (std::fabs(d) > zero_pivot) ? 1.0 / d : 0.0;
         for (int j = 0; j < n; ++j) { A[(size_t)k * n + j] *= inv; X[(size_t)k * n + j] *= inv; }
         for (int i = 0; i < n; ++i) {
             if (i == k) continue;
@@ -752,12 +768,12 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
 
 // Multigrid::vcycle (multigrid.h:361-383), enqueued without host syncs
 template <typename T>
-void MGH::vcycle_from(int l0)
+void MGH::vcycle_from(int l0, bool x0_zero)
 {
     hipStream_t s = ctx->stream;
     const int nl = (int)L.size();
     const int bottom = cut > l0 ? cut : nl - 1;
-    bool zero = true;   // x of the starting level is zero (preconditioner use)
+    bool zero = x0_zero;   // x of the starting level is zero (preconditioner use) or holds the iterate
     for (int l = l0; l < bottom; ++l) {
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
@@ -781,7 +797,7 @@ void MGH::vcycle_from(int l0)
     } else {
         // b -= mean(b) then relaxations from x = 0 (multigrid.h:184-187, 367)
         if (submean) dev::Ops<T>::submean(s, (long)c.nb * c.B, vc.b);
-        relax<T>(bottom, prm.npre, false, true);
+        relax<T>(bottom, prm.npre, false, bottom == l0 ? x0_zero : true);
         relax<T>(bottom, prm.npost, true, false);
     }
     for (int l = bottom - 1; l >= l0; --l) {
@@ -1358,6 +1374,55 @@ void MGH::smooth(int l, int precision, int n, bool reverse, const void* b, void*
         relax<double>(l, n, reverse, false);
         dev::Ops<double>::copy(s, m, lv.v64.x, (double*)x);
     }
+}
+
+// The reference driver's stationary solver (tests/cartesian/dg_tests.cpp:309-332):
+// V-cycles from the current solution, ||b - A x|| before and after each.
+// FP64: the reference's form, Multigrid::vcycle from x (multigrid.h:361-383).
+// Mixed: the correction form x += V(b - A x) with the V-cycle in FP32 from
+// zero and the residual and update in FP64 (the same iteration in exact
+// arithmetic: every step of a V-cycle is affine in (x, b)).  tol > 0 stops
+// once ||b - A x|| <= tol ||b||.
+void MGH::vcycles(int precision, const double* b, double* x, int ncycle, double tol, double* rnorm, int* cycles)
+{
+    hipStream_t s = ctx->stream;
+    Level& f = L[0];
+    const long n = pr->ndof;
+    double* part = this->part.p;
+    if (precision == 1 && !f.A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
+    auto fetch = [&] {
+        HIPCHK(hipMemcpyAsync(S_host, S.p, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    };
+    auto resid = [&]() -> double {   // r = b - A x, ||r||
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.ell, f.A.ellw, f.A64, x, b, r, r.p, S.p + dev::S_RR, part);
+        fetch();
+        return std::sqrt(S_host[dev::S_RR]);
+    };
+    dev::dot(s, n, b, b, S.p + dev::S_BB, part);
+    fetch();
+    const double bnorm = std::sqrt(S_host[dev::S_BB]);
+    double rn = resid();
+    if (rnorm) rnorm[0] = rn;
+    int i = 0;
+    for (; i < ncycle; ++i) {
+        if (tol > 0 && rn <= tol * bnorm) break;
+        if (precision == 0) {
+            dev::Ops<double>::copy(s, n, b, f.v64.b);
+            dev::Ops<double>::copy(s, n, x, f.v64.x);
+            vcycle_from<double>(0, false);
+            dev::Ops<double>::copy(s, n, f.v64.x, x);
+        } else {
+            dev::f64_to_f32(s, n, r, f.v32.b);
+            if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, f.v32.x);
+            vcycle_from<float>(0, true);
+            dev::f32_to_f64(s, n, f.v32.x, q);
+            dev::axpby(s, n, 1.0, q, 1.0, x, x);
+        }
+        rn = resid();
+        if (rnorm) rnorm[i + 1] = rn;
+    }
+    if (cycles) *cycles = i;
 }
 
 void MGH::vcycle(int precision, const double* b, double* x)

@@ -39,6 +39,7 @@ struct Ctx {
     std::string err;
     DBuf<double> part;       // reduction partials
     DBuf<unsigned> counter;  // reduction counter (kept at zero between kernels)
+    DBuf<double> colpart;    // per-column reduction partials and sums (mean_zero), grown on demand
     void init(int dev, void* stream);
     ~Ctx();
 };
@@ -202,7 +203,7 @@ struct MGH {
     template <typename T> void relax(int l, int n, bool reverse, bool zero_guess);
     // the ordered GS launches of one sweep: fn(rows, nrows, colour array, current colour, skip mode)
     template <typename F> void sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) const;
-    template <typename T> void vcycle_from(int l);
+    template <typename T> void vcycle_from(int l, bool x0_zero = true);
     template <typename T> LevelVecs<T>& vecs(int l);
     void precon(int precision, bool use_graph);           // z = V(r)
     void precon_enqueue(int precision);
@@ -210,6 +211,7 @@ struct MGH {
     void spmv(int l, int precision, const void* x, void* y);
     void smooth(int l, int precision, int n, bool reverse, const void* b, void* x);
     void vcycle(int precision, const double* b, double* x);
+    void vcycles(int precision, const double* b, double* x, int ncycle, double tol, double* rnorm, int* cycles);
     void profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches, int part = 0);
     void profile_spmv(int l, int precision, int nrep, double* ms, double* bytes);
 };

@@ -394,6 +394,43 @@ def pcg(mg, b, x0=None, tol=1e-8, maxit=200, criterion="reference", flexible=Fal
     return x, (stats[0] if nrhs == 1 else stats)
 
 
+def vcycles(mg, b, x0=None, ncycle=10, tol=0.0, precision="fp64"):
+    """The driver's stationary solver `--solver vcycles`
+    (tests/cartesian/dg_tests.cpp:309-332): up to ncycle V-cycles
+    (multigrid.h:361-383) from x0 (zero if None); tol > 0 stops once
+    ||b - A x|| <= tol ||b||.  precision "mixed": x += V(b - A x) with an
+    FP32 V-cycle, residual and update in FP64.
+
+    b: FP64 torch device tensor (x0, if given, updated in place) or numpy
+    array.  Returns (x, residual norms before and after each cycle run)."""
+    import torch
+    is_dev = hasattr(b, "data_ptr")
+    bd = b if is_dev else torch.tensor(np.ascontiguousarray(b, dtype=np.float64), device="cuda")
+    if x0 is None:
+        xd = torch.zeros_like(bd)
+    elif is_dev:
+        xd = x0
+    else:
+        xd = torch.tensor(np.asarray(x0, dtype=np.float64), device="cuda")
+    rn = np.zeros(int(ncycle) + 1)
+    cyc = C.c_int()
+    torch.cuda.synchronize()
+    check(mg.ctx, lib().mgldg_vcycles_device(mg.h, _prec(precision), _ptr(bd), _ptr(xd), int(ncycle), float(tol), _pd(rn),
+                                            C.byref(cyc)))
+    return (xd if is_dev else xd.cpu().numpy()), rn[:cyc.value + 1]
+
+
+def mean_zero(ctx, v):
+    """Function::meanZero (src/cartesian/dg/function.h:156-170) in place on an
+    FP64 device tensor [n] or [n, k] (each column); the driver applies it to
+    the right-hand side of Neumann and periodic problems (dg_tests.cpp:282-286)."""
+    import torch
+    nrhs = 1 if v.ndim == 1 else v.shape[1]
+    torch.cuda.synchronize()
+    check(ctx, lib().mgldg_mean_zero_device(ctx.h, v.shape[0], _ptr(v), nrhs))
+    return v
+
+
 def rhs_device(mg, f=None, g=None):
     """b = M f + B_g g on the GPU (ldgpoisson.tpp:103-176).  f [ndof(,k)] and g
     [nqb(,k)] are FP64 device tensors (either may be None)."""

@@ -70,6 +70,8 @@ SIGNATURES = {
     "mgldg_spmv_device": (I, [V, I, I, V, V, I]),
     "mgldg_smooth_device": (I, [V, I, I, I, I, V, V, I]),
     "mgldg_vcycle_device": (I, [V, I, V, V, I]),
+    "mgldg_vcycles_device": (I, [V, I, V, V, I, D, PD, PI]),
+    "mgldg_mean_zero_device": (I, [V, L, V, I]),
     "mgldg_profile_relax": (I, [V, I, I, I, PD, PD, PI]),
     "mgldg_profile_relax_part": (I, [V, I, I, I, I, PD, PD, PI]),
     "mgldg_profile_spmv": (I, [V, I, I, I, PD, PD]),

@@ -1350,6 +1350,25 @@ int orc_mg_vcycle(void* p, const double* b, double* x)
     return 0;
 }
 
+/* The stationary V-cycle solver of the reference driver
+ * (tests/cartesian/dg_tests.cpp:309-332): solution() = x, rhs() = b, then
+ * ncycle times vcycle() from the current solution (multigrid.h:361-383);
+ * rnorm[0] = ||b - A x0||, rnorm[i+1] = ||b - A x|| after cycle i. */
+int orc_mg_vcycles(void* p, const double* b, double* x, int ncycle, double* rnorm)
+{
+    omg* mg = (omg*)p; olevel* f = &mg->lev[0]; long n = (long)f->nb*f->b;
+    memcpy(f->x, x, sizeof(double)*n); memcpy(f->bv, b, sizeof(double)*n);
+    lv_residual(f);
+    if (rnorm) { double s = 0.0; for (long k = 0; k < n; ++k) s += f->r[k]*f->r[k]; rnorm[0] = sqrt(s); }
+    for (int i = 0; i < ncycle; ++i) {
+        mg_vcycle(mg, 0);
+        lv_residual(f);
+        if (rnorm) { double s = 0.0; for (long k = 0; k < n; ++k) s += f->r[k]*f->r[k]; rnorm[i+1] = sqrt(s); }
+    }
+    memcpy(x, f->x, sizeof(double)*n);
+    return 0;
+}
+
 int orc_mg_relax(void* p, int level, int nsweeps, int reverse, const double* b, double* x)
 {
     omg* mg = (omg*)p; if (level < 0 || level >= mg->L) return -1;

@@ -48,7 +48,7 @@ def lib():
         "orc_mg_new": (V, [V, P(MGParams)]), "orc_mg_free": (None, [V]),
         "orc_mg_nlevels": (I, [V]),
         "orc_mg_export": (C.c_long, [V, I, I, pi, pi, pd, pi, pi]),
-        "orc_mg_colors": (I, [V, I, pi]), "orc_mg_vcycle": (I, [V, pd, pd]),
+        "orc_mg_colors": (I, [V, I, pi]), "orc_mg_vcycle": (I, [V, pd, pd]), "orc_mg_vcycles": (I, [V, pd, pd, I, pd]),
         "orc_mg_relax": (I, [V, I, I, I, pd, pd]), "orc_mg_spmv": (I, [V, I, pd, pd]),
         "orc_pcg": (I, [V, V, pd, pd, D, I, I, I, pd, pd]),
         "orc_bsr_spmv": (I, [I, I, pi, pi, pd, pd, pd]),
@@ -289,6 +289,15 @@ class Multigrid:
         x = np.zeros_like(b)
         lib().orc_mg_vcycle(self.h, _pd(b), _pd(x))
         return x
+
+    def vcycles(self, b, x0, ncycle):
+        """the driver's stationary V-cycle solver (tests/cartesian/dg_tests.cpp:309-332):
+        ncycle V-cycles from x0; returns (x, residual norms before and after each cycle)"""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x0, dtype=np.float64)
+        rn = np.zeros(ncycle + 1)
+        lib().orc_mg_vcycles(self.h, _pd(b), _pd(x), int(ncycle), _pd(rn))
+        return x, rn
 
     def relax(self, level, b, x, nsweeps=1, reverse=False):
         b = np.ascontiguousarray(b, dtype=np.float64)
