@@ -288,6 +288,39 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1):
     }
 
 
+def periodic_leg(mgldg, torch, warm=1, n=256, p=3):
+    """Periodic unit square, uniform n x n quadtree, p = 3: the operator's
+    kernel is the constants, so the right-hand side is made mean-zero on the
+    device (Function::meanZero, as the reference driver does for periodic and
+    Neumann problems); MG-PCG in FP64 and mixed precision to 1e-10."""
+    t0 = time.perf_counter()
+    ctx = mgldg.Context(0)
+    qt = mgldg.Quadtree(ctx, 2, lambda x: 1.0 / n, periodic=True)
+    prob = mgldg.LDGPoisson(qt, p, tau0=0.0, tauD=1000.0)
+    mg = mgldg.Multigrid(prob, relaxation="gs", omega=0.8, npre=2, npost=2, restriction="ratrat", precision="mixed")
+    t_setup = time.perf_counter() - t0
+    X = prob.nodes()
+    f = np.sin(2 * np.pi * X[:, 0]) * np.cos(4 * np.pi * X[:, 1]) + np.random.default_rng(0).standard_normal(prob.ndof)
+    b = torch.tensor(prob.rhs(f), device="cuda")
+    mgldg.mean_zero(ctx, b)
+    out = {"workload": f"periodic uniform {n}x{n} quadtree, p={p}, f = sin(2 pi x) cos(4 pi y) + N(0,1), mean-zero b",
+           "ndof": prob.ndof, "levels": mg.nlevels, "setup_s": round(t_setup, 3)}
+    for prec in ("fp64", "mixed"):
+        kw = dict(tol=1e-10, maxit=500, criterion="true", precision=prec, flexible=prec == "mixed", use_graph=True)
+        for _ in range(warm):
+            mgldg.pcg(mg, b, torch.zeros_like(b), **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, st = mgldg.pcg(mg, b, torch.zeros_like(b), **kw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        out[prec] = {"value": round(prob.ndof / dt, 1), "ms_per_step": round(dt * 1e3, 3), "iterations": st["iterations"],
+                     "status": st["status"], "relres_true": st["relres_true"]}
+        log(f"periodic {prec}: {dt * 1e3:.2f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {st['iterations']} "
+            f"({st['status']}, relres {st['relres_true']:.2e})")
+    return out
+
+
 def ball_mesh(n=41, jitter=0.01, seed=0):
     """Unit ball: the cubic lattice of spacing 1/n, vertices jittered by
     jitter/n (seed), each lattice cube whose corners lie in r <= 1 split into
@@ -434,6 +467,7 @@ def main():
     ap.add_argument("--no-ball", action="store_true", help="skip the 3-D ball workload")
     ap.add_argument("--ball-only", action="store_true", help="only the 3-D ball workload")
     ap.add_argument("--unstructured-only", action="store_true", help="only the unit-disk workload")
+    ap.add_argument("--no-periodic", action="store_true", help="skip the periodic workload")
     ap.add_argument("--batched-only", action="store_true",
                     help="setup + two mixed and one FP64 batched solves of --nrhs columns (profiling)")
     args = ap.parse_args()
@@ -597,6 +631,22 @@ def main():
         f"mixed {dtb32 / bsteps * 1e3:.1f} ms ({vb32:.3g}); max iterations {max(s['iterations'] for s in stb32)}; "
         f"kernel plan {plan_summary(plan)}")
 
+    # the driver's stationary solver (--solver vcycles) on the same problem
+    vcyc = {}
+    for prec in ("fp64", "mixed"):
+        mgldg.vcycles(mg, b, torch.zeros_like(b), ncycle=2, precision=prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        xv, rn = mgldg.vcycles(mg, b, torch.zeros_like(b), ncycle=200, tol=1e-10, precision=prec)
+        torch.cuda.synchronize()
+        dtv = time.perf_counter() - t
+        ncyc = len(rn) - 1
+        vcyc[prec] = {"cycles": ncyc, "ms_per_solve": round(dtv * 1e3, 3), "value": round(ndof / dtv, 1),
+                      "mean_contraction": round(float((rn[-1] / rn[0]) ** (1.0 / max(ncyc, 1))), 4),
+                      "relres": float(rn[-1] / float(torch.linalg.norm(b)))}
+    log(f"vcycles (stationary solver) to 1e-10: {vcyc}")
+    periodic = None if args.no_periodic else periodic_leg(mgldg, torch, args.warmup)
+
     cpu = None
     err = {}
     if not args.no_cpu:
@@ -655,6 +705,8 @@ def main():
                               "max_iterations": max(s["iterations"] for s in stb32)},
                     "mixed_batched_vs_fp64_single": round(vb32 / v64, 3),
                     "kernel_plan": plan_summary(plan), "mfma_fine_apply": mfma_use(mg, plan, k)},
+        "vcycles": vcyc,
+        "periodic": periodic,
         "error_vs_fp64_oracle": err,
         "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
         "roofline": roofline,
