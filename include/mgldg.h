@@ -170,6 +170,16 @@ typedef struct {
 /* On a simplex problem: the library's own agglomeration (below) followed by
  * the agglomerated hierarchy. */
 int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out);
+/* Multigrid<N,P1,P2,...>(ops, InterpolationHierarchy<N,P1,P2,...>(qt), params)
+ * (src/cartesian/multigrid/multigrid.h:435-639,
+ * interpolationhierarchy.h:98-154): p-levels of the polynomial degrees
+ * degrees[0] > degrees[1] > ... (each below the problem's, >= 0; T on every
+ * element evaluates the coarser degree's Lagrange basis at the finer degree's
+ * GL nodes), then with hcoarsen = 1 the quadtree h-hierarchy at the last
+ * degree, with hcoarsen = 0 none (StopCoarsening: the last p-level is the
+ * coarsest).  Quadtree problems only. */
+int mgldg_mg_create_p(mgldg_problem pr, const mgldg_mg_params* prm, int ndeg, const int* degrees, int hcoarsen,
+                      mgldg_mg* out);
 /* Agglomeration<N,P>(mesh, maxAgglom) (src/unstructured/mesh/agglomeration.h:21-34)
  * with the library's MGridGen-style agglomerator (clusters of at most 2^N
  * agglomerates of the finer level, chosen for small perimeter^N/volume^(N-1)),

@@ -278,13 +278,27 @@ static MGParams mg_params(const mgldg_mg_params* prm)
     return m;
 }
 
-static void mg_build(mgldg_problem pr, const MGParams& m, const std::vector<std::vector<int>>* parts, mgldg_mg* out)
+static void mg_build(mgldg_problem pr, const MGParams& m, const std::vector<std::vector<int>>* parts, mgldg_mg* out,
+                     const std::vector<int>* pdeg = nullptr, bool hcoarsen = true)
 {
     auto* h = new mgldg_mg_s();
     h->ctx = pr->ctx;
-    try { h->m.build(&pr->p, m, parts); }
+    try { h->m.build(&pr->p, m, parts, pdeg, hcoarsen); }
     catch (...) { delete h; throw; }
     *out = h;
+}
+
+int mgldg_mg_create_p(mgldg_problem pr, const mgldg_mg_params* prm, int ndeg, const int* degrees, int hcoarsen,
+                      mgldg_mg* out)
+{
+    if (!pr || !prm || !out || ndeg < 1 || !degrees) return MGLDG_ERR_ARG;
+    *out = nullptr;
+    return guard(pr->ctx, [&] {
+        MGParams m = mg_params(prm);
+        ARG(!pr->p.simplex(), "p-multigrid hierarchies apply to quadtree problems only");
+        std::vector<int> deg(degrees, degrees + ndeg);
+        mg_build(pr, m, nullptr, out, &deg, hcoarsen != 0);
+    });
 }
 
 int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out)

@@ -101,7 +101,9 @@ static void finalize(hipStream_t s, const double* part, int n, int nval, double*
 }
 
 // --------------------------------------------------------------------- setup kernels
-#define TP_MAX 32
+// largest block of the setup kernels: (p+1)^N for 2-D p <= 5 and 3-D p <= 2
+#define TP_MAX 36
+#define TP_ACC ((TP_MAX * TP_MAX + 255) / 256)   // output entries per thread of k_triple
 __global__ __launch_bounds__(256) void k_triple(int p, int q, int m, int n, int transL,
                                                 const int* __restrict__ off, const int* __restrict__ li,
                                                 const int* __restrict__ xi, const int* __restrict__ ri,
@@ -112,9 +114,9 @@ __global__ __launch_bounds__(256) void k_triple(int p, int q, int m, int n, int 
     __shared__ double Ls[TP_MAX * TP_MAX], Xs[TP_MAX * TP_MAX], Rs[TP_MAX * TP_MAX], Ts[TP_MAX * TP_MAX];
     const int o = blockIdx.x, tid = threadIdx.x;
     const int pq = p * q;
-    double acc[4];
+    double acc[TP_ACC];
     const int as = addslot ? addslot[o] : -1;
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < TP_ACC; ++u) {
         int e = tid + u * 256;
         acc[u] = (e < pq && as >= 0) ? addpool[(size_t)as * pq + e] : 0.0;
     }
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(256) void k_triple(int p, int q, int m, int n, int 
             Ts[e] = s;
         }
         __syncthreads();
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < TP_ACC; ++u) {
             int e = tid + u * 256;
             if (e >= pq) break;
             int i = e / q, j = e % q; double s = 0;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(256) void k_triple(int p, int q, int m, int n, int 
         }
         __syncthreads();
     }
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < TP_ACC; ++u) {
         int e = tid + u * 256;
         if (e < pq) out[(size_t)o * pq + e] = acc[u];
     }
@@ -158,7 +160,7 @@ void triple(hipStream_t s, int nout, int p, int q, int m, int n, int transL, con
             const double* addpool, const int* addslot, double* out)
 {
     if (nout <= 0) return;
-    if (p > TP_MAX || q > TP_MAX || m > TP_MAX || n > TP_MAX) throw std::runtime_error("triple: block larger than 32");
+    if (p > TP_MAX || q > TP_MAX || m > TP_MAX || n > TP_MAX) throw std::runtime_error("triple: block larger than " + std::to_string(TP_MAX));
     hipLaunchKernelGGL(k_triple, dim3(nout), dim3(256), 0, s, p, q, m, n, transL, off, l, x, r, Lp, Xp, Rp, addpool, addslot, out);
     check_launch("triple");
 }
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(256) void k_inverse(int b, const double* __restrict
 void inverse(hipStream_t s, int nblk, int b, const double* in, const int* slot, double* out)
 {
     if (nblk <= 0) return;
-    if (b > TP_MAX) throw std::runtime_error("inverse: block larger than 32");
+    if (b > TP_MAX) throw std::runtime_error("inverse: block larger than " + std::to_string(TP_MAX));
     hipLaunchKernelGGL(k_inverse, dim3(nblk), dim3(256), 0, s, b, in, slot, out);
     check_launch("inverse");
 }
@@ -636,7 +638,8 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
     }
 
 // dispatch over the compiled (fine, coarse) interpolation block shapes: the
-// square sizes and P_p simplex -> Q_p box for 2-D p = 1..3 and 3-D p = 1, 2
+// square sizes, P_p simplex -> Q_p box for 2-D p = 1..3 and 3-D p = 1, 2, and
+// the p-coarsenings Q_p -> Q_q (q < p) for 2-D p <= 5 and 3-D p <= 2
 #define MGLDG_DISPATCH2(BF, BC, ...)                                                 \
     if ((BF) == (BC)) {                                                              \
         MGLDG_DISPATCH(BF, { constexpr int FF = BB, CC = BB; __VA_ARGS__; });        \
@@ -647,6 +650,24 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
             case 10016: { constexpr int FF = 10, CC = 16; __VA_ARGS__; } break;      \
             case 4008: { constexpr int FF = 4, CC = 8; __VA_ARGS__; } break;         \
             case 10027: { constexpr int FF = 10, CC = 27; __VA_ARGS__; } break;      \
+            case 36025: { constexpr int FF = 36, CC = 25; __VA_ARGS__; } break;      \
+            case 36016: { constexpr int FF = 36, CC = 16; __VA_ARGS__; } break;      \
+            case 36009: { constexpr int FF = 36, CC = 9; __VA_ARGS__; } break;      \
+            case 36004: { constexpr int FF = 36, CC = 4; __VA_ARGS__; } break;      \
+            case 36001: { constexpr int FF = 36, CC = 1; __VA_ARGS__; } break;      \
+            case 25016: { constexpr int FF = 25, CC = 16; __VA_ARGS__; } break;      \
+            case 25009: { constexpr int FF = 25, CC = 9; __VA_ARGS__; } break;      \
+            case 25004: { constexpr int FF = 25, CC = 4; __VA_ARGS__; } break;      \
+            case 25001: { constexpr int FF = 25, CC = 1; __VA_ARGS__; } break;      \
+            case 16009: { constexpr int FF = 16, CC = 9; __VA_ARGS__; } break;      \
+            case 16004: { constexpr int FF = 16, CC = 4; __VA_ARGS__; } break;      \
+            case 16001: { constexpr int FF = 16, CC = 1; __VA_ARGS__; } break;      \
+            case 9004: { constexpr int FF = 9, CC = 4; __VA_ARGS__; } break;      \
+            case 9001: { constexpr int FF = 9, CC = 1; __VA_ARGS__; } break;      \
+            case 4001: { constexpr int FF = 4, CC = 1; __VA_ARGS__; } break;      \
+            case 27008: { constexpr int FF = 27, CC = 8; __VA_ARGS__; } break;      \
+            case 27001: { constexpr int FF = 27, CC = 1; __VA_ARGS__; } break;      \
+            case 8001: { constexpr int FF = 8, CC = 1; __VA_ARGS__; } break;      \
             default: throw std::runtime_error("interpolation block " + std::to_string(BF) + "x" + std::to_string(BC) + " not compiled"); \
         }                                                                            \
     }

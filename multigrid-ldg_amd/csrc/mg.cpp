@@ -321,6 +321,27 @@ static void build_interp(const Quadtree& qt, const Basis& B, int l, Interp& I)
     I.make_gather();
 }
 
+// InterpolationHierarchy<N,P1,P2,...> (interpolationhierarchy.h:98-131): on
+// every element, the degree-P2 Lagrange basis evaluated at the degree-P1
+// Gauss-Lobatto nodes -- one block for all ne elements (parent = itself).
+static void build_pinterp(int ne, const Basis& B1, const Basis& B2, Interp& I)
+{
+    const int N = B1.N, b1 = B1.npl, b2 = B2.npl;
+    I.nf = ne; I.nc = ne; I.bf = b1; I.bc = b2;
+    I.parent.resize(ne); I.type.assign(ne, 0);
+    for (int e = 0; e < ne; ++e) I.parent[e] = e;
+    I.pool.assign((size_t)b1 * b2, 0.0); I.ntypes = 1;
+    int it[3];
+    for (int i = 0; i < b1; ++i) {
+        int lin = i;
+        for (int d = N - 1; d >= 0; --d) { it[d] = lin % B1.P; lin /= B1.P; }
+        double x[3];
+        for (int d = 0; d < N; ++d) x[d] = B1.nodes[it[d]];
+        B2.lagrangeN(x, &I.pool[(size_t)i * b2]);
+    }
+    I.make_gather();
+}
+
 // invert a dense SPD(-semidefinite) matrix on the host by Gauss-Jordan with
 // partial pivoting.  Zero pivots give zero rows, as LDLT::solve does; a pivot
 // below 1e-13 of the matrix's largest entry counts as zero.  Without that, the
@@ -574,7 +595,8 @@ static void pad_dropped(hipStream_t s, Level& c, const std::vector<int>& rank)
     HIPCHK(hipStreamSynchronize(s));
 }
 
-void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<int>>* parts)
+void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<int>>* parts, const std::vector<int>* pdeg,
+                bool hcoarsen)
 {
     pr = p; ctx = p->ctx; prm = pm;
     if (prm.relaxation < 0 || prm.relaxation > 2) throw Error(-1, "Unknown relaxation method.");   // multigrid.h:213
@@ -584,6 +606,7 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
     if (prm.npre < 0 || prm.npost < 0) throw Error(-1, "negative sweep count");
     const SimplexLdg* sx = p->simplex();
     int nl = 0;
+    np = 0;
     std::vector<Interp> AI;                 // agglomerated hierarchy: all interpolations
     std::vector<std::vector<int>> crank;    // kept coarse directions per agglomerate
     submean = sx ? 0 : 1;
@@ -605,9 +628,17 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
     } else {
         if (parts) throw Error(-1, "partitions apply to simplex problems only");
         const Quadtree& qt = *p->qt;
-        if (qt.num_levels < 2) throw Error(-1, "the quadtree needs at least two levels");  // interpolationhierarchy.h:33
-        nl = qt.num_levels;
+        np = pdeg ? (int)pdeg->size() : 0;
+        for (int i = 0; i < np; ++i) {
+            const int prev = i ? (*pdeg)[i - 1] : p->cart()->p;
+            if ((*pdeg)[i] < 0 || (*pdeg)[i] >= prev)
+                throw Error(-1, "p-multigrid degrees must decrease from the problem's and be >= 0");  // interpolationhierarchy.h:101
+        }
+        if (!pdeg) hcoarsen = true;
+        if (hcoarsen && qt.num_levels < 2) throw Error(-1, "the quadtree needs at least two levels");  // interpolationhierarchy.h:33
+        nl = 1 + np + (hcoarsen ? qt.num_levels - 1 : 0);
     }
+    if (sx && pdeg) throw Error(-1, "p-multigrid hierarchies apply to quadtree problems only");
     hipStream_t s = ctx->stream;
     const int N = p->N;
     L.clear(); L.resize(nl);
@@ -634,10 +665,14 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
         build_interps_agglo(sx->mesh, sx->E, Q, *parts, AI, crank);
         th += secs(th0);
     }
+    // p-levels (degrees pdeg), then h-levels at the last degree
+    std::vector<Basis> pb(np);
+    for (int i = 0; i < np; ++i) pb[i].init(N, (*pdeg)[i] + 1);
     for (int l = 0; l + 1 < nl; ++l) {
         auto th0 = Clock::now();
         if (sx) L[l].I = std::move(AI[l]);
-        else build_interp(*p->qt, p->cart()->basis, l, L[l].I);
+        else if (l < np) build_pinterp(p->ne, l ? pb[l - 1] : p->cart()->basis, pb[l], L[l].I);
+        else build_interp(*p->qt, np ? pb[np - 1] : p->cart()->basis, l - np, L[l].I);
         th += secs(th0);
         upload_interp(s, L[l], prm.precision);
         L[l + 1].nb = L[l].I.nc; L[l + 1].B = L[l].I.bc;

@@ -181,7 +181,12 @@ struct MGH {
 
     // parts: agglomeration partitions (level l -> l + 1) of a simplex problem;
     // null: the quadtree hierarchy of a quadtree problem
-    void build(ProblemH* p, const MGParams& prm, const std::vector<std::vector<int>>* parts = nullptr);
+    // parts: agglomerated hierarchy (simplex problems).  pdeg: p-levels of
+    // these degrees below the problem's (quadtree problems), followed by the
+    // h-levels at the last degree when hcoarsen.
+    void build(ProblemH* p, const MGParams& prm, const std::vector<std::vector<int>>* parts = nullptr,
+               const std::vector<int>* pdeg = nullptr, bool hcoarsen = true);
+    int np = 0;                        // p-levels below the fine level
     void build_collapse();
     void ensure_batch(int k, int precision);
     // kernel family of each batched operator: 0 = vector kernels (MOps),

@@ -260,10 +260,16 @@ class Multigrid(_Handle):
     On a simplex problem the hierarchy is agglomerated
     (src/unstructured/multigrid/interpolationhierarchy.h:88-204): by the
     library's own agglomeration, or from `partitions` (list of arrays, level l
-    elements -> level l+1 agglomerates)."""
+    elements -> level l+1 agglomerates).
+
+    pdegrees (quadtree problems): p-levels of these polynomial degrees below
+    the problem's, Multigrid<N,P1,P2,...> (multigrid.h:435-639), then the
+    h-levels at the last degree, or none if hcoarsen is False
+    (StopCoarsening)."""
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
-                 restriction="ratrat", precision="fp64", coarse_dense_max=4096, partitions=None):
+                 restriction="ratrat", precision="fp64", coarse_dense_max=4096, partitions=None, pdegrees=None,
+                 hcoarsen=True):
         self.prob, self.ctx = prob, prob.ctx
         self.params = MGParams({"jacobi": 0, "gs": 1, "gs-lex": 2}[relaxation], omega,
                                {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
@@ -276,6 +282,10 @@ class Multigrid(_Handle):
                 parts[l * prob.ne:l * prob.ne + len(q)] = q
             check(self.ctx, lib().mgldg_mg_create_agglomerated(prob.h, C.byref(self.params), len(partitions), _pi(parts),
                                                                C.byref(self.h)))
+        elif pdegrees is not None:
+            deg = np.ascontiguousarray(pdegrees, dtype=np.int32)
+            check(self.ctx, lib().mgldg_mg_create_p(prob.h, C.byref(self.params), len(deg), _pi(deg), int(hcoarsen),
+                                                    C.byref(self.h)))
         else:
             check(self.ctx, lib().mgldg_mg_create(prob.h, C.byref(self.params), C.byref(self.h)))
         self._own(lib().mgldg_mg_destroy)

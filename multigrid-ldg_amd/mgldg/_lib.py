@@ -53,6 +53,7 @@ SIGNATURES = {
     "mgldg_problem_export": (I, [V, I, PI, PI, PD, PL]),
     "mgldg_problem_destroy": (I, [V]),
     "mgldg_mg_create": (I, [V, C.POINTER(MGParams), C.POINTER(V)]),
+    "mgldg_mg_create_p": (I, [V, C.POINTER(MGParams), I, PI, I, C.POINTER(V)]),
     "mgldg_mg_info": (I, [V, PL, I]),
     "mgldg_mg_level": (I, [V, I, PI, PI, PL]),
     "mgldg_mg_batch_kernels": (I, [V, I]),

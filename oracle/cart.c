@@ -1061,12 +1061,14 @@ typedef struct {
     int submean;           /* coarsest b -= mean(b) (the quadtree Multigrid, multigrid.h:365-368) */
 } omg;
 
-/* InterpolationHierarchy<N,P>(qt) (interpolationhierarchy.h:30-82) */
-static bsm* build_interp_qt(const oproblem* pr, int* nlev)
+/* InterpolationHierarchy<N,P>(qt) (interpolationhierarchy.h:30-82), with
+ * the degree of basis B; Tl has room for `extra` operators in front (the
+ * p-levels of an hp-hierarchy) */
+static bsm* build_interp_qt_basis(const oproblem* pr, const basis_t* B, int extra, int* nlev)
 {
-    const qtree* qt = pr->mesh.qt; const basis_t* B = &pr->mesh.B;
+    const qtree* qt = pr->mesh.qt;
     int N = qt->N, P = B->P, b = B->npl, it[3];
-    bsm* Tl = (bsm*)xcalloc(qt->numLevels - 1, sizeof(bsm));
+    bsm* Tl = (bsm*)xcalloc(extra + qt->numLevels - 1, sizeof(bsm)) + extra;
     int* cg2l = (int*)xmalloc(sizeof(int)*qt->n);
     double* E = (double*)xmalloc(sizeof(double)*b*b);
     double* I = (double*)xcalloc((size_t)b*b, sizeof(double));
@@ -1097,7 +1099,26 @@ static bsm* build_interp_qt(const oproblem* pr, int* nlev)
     }
     free(cg2l); free(E); free(I);
     *nlev = qt->numLevels;
-    return Tl;
+    return Tl - extra;
+}
+
+static bsm* build_interp_qt(const oproblem* pr, int* nlev) { return build_interp_qt_basis(pr, &pr->mesh.B, 0, nlev); }
+
+/* InterpolationHierarchy<N,P1,P2,...> (interpolationhierarchy.h:98-131): on
+ * every element of the fine layer, the degree-P2 Lagrange basis evaluated at
+ * the degree-P1 Gauss-Lobatto nodes (a block-diagonal T of P1^N x P2^N blocks) */
+static void build_pinterp(bsm* T, int ne, int N, const basis_t* B1, const basis_t* B2)
+{
+    int b1 = B1->npl, b2 = B2->npl, it[3];
+    double* E = (double*)xmalloc(sizeof(double)*(size_t)b1*b2);
+    for (int i = 0; i < b1; ++i) {
+        unlin(i, N, B1->P, it); double x[3];
+        for (int k = 0; k < N; ++k) x[k] = B1->nodes[it[k]];
+        lagrange_eval_nd(B2, x, E + (size_t)i*b2);
+    }
+    bsm_init(T, ne, ne, b1, b2);
+    for (int e = 0; e < ne; ++e) bsm_add(T, e, e, 1.0, E);
+    free(E);
 }
 
 /* greedy colouring in block-row order: smallest colour unused by any
@@ -1220,6 +1241,30 @@ void* orc_mg_new(void* p, const orc_mg_params* prm)
     int L = 0;
     bsm* Tl = build_interp_qt(pr, &L);
     return mg_setup(&pr->ops, pr->mesh.N, Tl, L, prm, NULL, 1);
+}
+
+/* Multigrid<N,P1,P2,...> (multigrid.h:435-639) with InterpolationHierarchy<N,P1,P2,...>:
+ * p-levels of the polynomial degrees deg[0] > deg[1] > ... (each below the
+ * problem's), then, if hcoarsen, the quadtree h-hierarchy at degree
+ * deg[ndeg-1]; otherwise the last p-level is the coarsest
+ * (InterpolationHierarchy<N,P,StopCoarsening>) */
+void* orc_mg_new_hp(void* p, const orc_mg_params* prm, int ndeg, const int* deg, int hcoarsen)
+{
+    oproblem* pr = (oproblem*)p;
+    if (!pr || !params_ok(prm) || ndeg < 1 || !deg) return NULL;
+    int N = pr->mesh.N, Pf = pr->mesh.B.P;
+    for (int i = 0; i < ndeg; ++i) if (deg[i] < 0 || deg[i] + 1 >= (i ? deg[i-1] + 1 : Pf)) return NULL;
+    if (hcoarsen && pr->mesh.qt->numLevels < 2) return NULL;
+    basis_t* Bs = (basis_t*)xcalloc(ndeg, sizeof(basis_t));
+    for (int i = 0; i < ndeg; ++i) basis_init(&Bs[i], N, deg[i] + 1);
+    int nh = 1;
+    bsm* Tl;
+    if (hcoarsen) Tl = build_interp_qt_basis(pr, &Bs[ndeg-1], ndeg, &nh);
+    else Tl = (bsm*)xcalloc(ndeg, sizeof(bsm));
+    for (int i = 0; i < ndeg; ++i) build_pinterp(&Tl[i], pr->ops.ne, N, i ? &Bs[i-1] : &pr->mesh.B, &Bs[i]);
+    for (int i = 0; i < ndeg; ++i) basis_free(&Bs[i]);
+    free(Bs);
+    return mg_setup(&pr->ops, N, Tl, ndeg + nh, prm, NULL, 1);
 }
 
 void orc_mg_free(void* p)

@@ -84,6 +84,7 @@ int   orc_mg_colors(void* mg, int level, int* colors);        /* returns ncolors
 /* one preconditioner application: x = V(b) with x0 = 0 (dg_tests.cpp:300-305) */
 int   orc_mg_vcycle(void* mg, const double* b, double* x);
 int orc_mg_vcycles(void* mg, const double* b, double* x, int ncycle, double* rnorm);
+void* orc_mg_new_hp(void* problem, const orc_mg_params* prm, int ndeg, const int* deg, int hcoarsen);
 /* nsweeps relaxations on level l (multigrid.h:202-216) with given b, x in/out */
 int   orc_mg_relax(void* mg, int level, int nsweeps, int reverse, const double* b, double* x);
 /* y = A_l x */

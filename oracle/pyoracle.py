@@ -46,6 +46,7 @@ def lib():
         "orc_problem_rhs": (I, [V, pd, pd, pd]), "orc_problem_rhs_g_adjoint": (I, [V, pd, pd]),
         "orc_problem_export": (C.c_long, [V, I, pi, pi, pd]),
         "orc_mg_new": (V, [V, P(MGParams)]), "orc_mg_free": (None, [V]),
+        "orc_mg_new_hp": (V, [V, P(MGParams), I, pi, I]),
         "orc_mg_nlevels": (I, [V]),
         "orc_mg_export": (C.c_long, [V, I, I, pi, pi, pd, pi, pi]),
         "orc_mg_colors": (I, [V, I, pi]), "orc_mg_vcycle": (I, [V, pd, pd]), "orc_mg_vcycles": (I, [V, pd, pd, I, pd]),
@@ -236,16 +237,24 @@ class Multigrid:
     RELAX = {"jacobi": 0, "gs": 1, "mcgs": 2}
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
-                 restriction="ratrat"):
+                 restriction="ratrat", pdegrees=None, hcoarsen=True):
+        """pdegrees: polynomial degrees of p-levels below the problem's
+        (Multigrid<N,P1,P2,...>, multigrid.h:435-639); then the h-hierarchy at
+        the last degree, or none if hcoarsen is False (StopCoarsening)."""
         self.prob = prob
         self.params = MGParams(self.RELAX[relaxation], omega, {"cholesky": 0, "relaxation": 1}[coarse],
                                npre, npost, {"rat": 0, "ratrat": 1}[restriction])
+        self.pdegrees = None if pdegrees is None else np.ascontiguousarray(pdegrees, dtype=np.int32)
+        self.hcoarsen = hcoarsen
         self.h = self._create(prob)
         if not self.h:
             raise ValueError("oracle multigrid creation failed")
         self.nlevels = lib().orc_mg_nlevels(self.h)
 
     def _create(self, prob):
+        if self.pdegrees is not None:
+            return lib().orc_mg_new_hp(prob.h, C.byref(self.params), len(self.pdegrees), _pi(self.pdegrees),
+                                       int(self.hcoarsen))
         return lib().orc_mg_new(prob.h, C.byref(self.params))
 
     def __del__(self):
