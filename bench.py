@@ -645,6 +645,24 @@ def main():
                       "mean_contraction": round(float((rn[-1] / rn[0]) ** (1.0 / max(ncyc, 1))), 4),
                       "relres": float(rn[-1] / float(torch.linalg.norm(b)))}
     log(f"vcycles (stationary solver) to 1e-10: {vcyc}")
+
+    # hp-multigrid on the same problem: p = 3 -> 1, then the h-levels at p = 1
+    t0 = time.perf_counter()
+    mg_hp = mgldg.Multigrid(prob, relaxation="gs", omega=0.8, npre=2, npost=2, restriction="ratrat", precision="mixed",
+                            pdegrees=[1])
+    hp = {"pdegrees": [3, 1], "levels": mg_hp.nlevels, "level_B": mg_hp.level_B[:3], "setup_s": round(time.perf_counter() - t0, 3)}
+    for prec in ("fp64", "mixed"):
+        kw = dict(tol=1e-10, maxit=500, criterion="true", precision=prec, flexible=prec == "mixed", use_graph=True)
+        mgldg.pcg(mg_hp, b, torch.zeros_like(b), **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, sth = mgldg.pcg(mg_hp, b, torch.zeros_like(b), **kw)
+        torch.cuda.synchronize()
+        dth = time.perf_counter() - t
+        hp[prec] = {"value": round(ndof / dth, 1), "ms_per_step": round(dth * 1e3, 3), "iterations": sth["iterations"],
+                    "relres_true": sth["relres_true"]}
+    del mg_hp
+    log(f"hp-multigrid (p 3 -> 1, then h): {hp}")
     periodic = None if args.no_periodic else periodic_leg(mgldg, torch, args.warmup)
 
     cpu = None
@@ -706,6 +724,7 @@ def main():
                     "mixed_batched_vs_fp64_single": round(vb32 / v64, 3),
                     "kernel_plan": plan_summary(plan), "mfma_fine_apply": mfma_use(mg, plan, k)},
         "vcycles": vcyc,
+        "hp_multigrid": hp,
         "periodic": periodic,
         "error_vs_fp64_oracle": err,
         "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
