@@ -38,6 +38,7 @@ extern "C" {
 #define MGLDG_ERR_NOMEM     -3   /* host or device allocation failure */
 #define MGLDG_ERR_STATE     -4   /* handle in the wrong state */
 #define MGLDG_ERR_UNSUPPORTED -5 /* valid request this build does not implement */
+#define MGLDG_ERR_IO        -6   /* a file could not be opened or written */
 
 /* ---- solve status (pcg returns void in the reference: krylov.h:23) ---- */
 #define MGLDG_CONVERGED   0
@@ -288,6 +289,53 @@ int mgldg_vcycles_device(mgldg_mg mg, int precision, const double* b, double* x,
  * driver applies it to the right-hand side of Neumann and periodic problems
  * (dg_tests.cpp:282-286). */
 int mgldg_mean_zero_device(mgldg_ctx ctx, long n, double* v, int nrhs);
+
+/* ======================= interop file formats ======================= */
+/* Text formats of the reference's dumps and meshes.  Entries taking no handle
+ * report errors through mgldg_last_error(NULL).  Numbers are printed at 17
+ * significant digits (the reference's precision(max_digits10)), so every
+ * double round-trips. */
+
+/* SparseBlockMatrix::write (src/cartesian/util/sparseblockmatrix.tpp:145-164,
+ * src/unstructured/util/sparseblockmatrix.tpp:145-164): MatrixMarket
+ * "coordinate real general", header "rows cols nnz" with nnz = nnzb*br*bc,
+ * one 1-based line per entry of every stored block (explicit zeros
+ * included).  Blocks go in row order, columns ascending (the reference walks
+ * its hash map, whose order is unspecified).  blocks: nnzb row-major br x bc. */
+int mgldg_bsr_write_mm(const char* path, int nbrows, int nbcols, int br, int bc, const int* rowptr,
+                       const int* cols, const double* blocks);
+/* the same for a problem operator (which as mgldg_problem_export) */
+int mgldg_problem_write_mm(mgldg_problem pr, int which, const char* path);
+/* the same for a level operator or interpolation (which as mgldg_mg_export) */
+int mgldg_mg_write_mm(mgldg_mg mg, int level, int which, const char* path);
+/* Function::write (src/cartesian/dg/function.h:173-190,
+ * src/unstructured/dg/function.h:159-176): header "P N ne" (P = p + 1),
+ * then per element one line "x_1 .. x_N value" per node in storage order
+ * and an empty line.  xyz[ne*npl*N] node coordinates, u[ne*npl] values. */
+int mgldg_fun_write(const char* path, int P, int N, int ne, int npl, const double* xyz, const double* u);
+/* the same with the problem's nodes (mgldg_problem_nodes) and u[ndof] */
+int mgldg_problem_write_fun(mgldg_problem pr, const double* u, const char* path);
+/* LDGPoisson::dump (src/cartesian/dg/ldgpoisson.h:100-109, unstructured
+ * ldgpoisson.h:100-109): dir/A.mtx, M.mtx, T.mtx, G0.mtx .. G{N-1}.mtx and,
+ * when rhs (host, ndof) is not NULL, dir/rhs.fun.  dir must exist. */
+int mgldg_problem_dump(mgldg_problem pr, const double* rhs, const char* dir);
+/* The .mesh layout (lib/distmesh/writemesh.m:17-28, read back by
+ * mgldg_wireframe_read and Wireframe<N>(file), wireframe.h:22-68): "np nt",
+ * an empty line, np lines of N coordinates, an empty line, nt lines of N+1
+ * 1-based vertex ids.  digits <= 0: coordinates as "%f" (writemesh.m);
+ * digits in 1..17: that many significant digits.  simplices are 0-based. */
+int mgldg_mesh_write(const char* path, int N, int np, const double* points, int nt, const int* simplices,
+                     int digits);
+int mgldg_wireframe_write(mgldg_wireframe wf, const char* path, int digits);
+/* Multigrid::vcycle_matrix (src/cartesian/multigrid/multigrid.h:386-400,
+ * :523-537): V[:, i] = one V-cycle from zero applied to the unit vector e_i,
+ * for every fine dof i; V host, row-major ndof x ndof.  The columns run
+ * through the batched V-cycle 32 at a time. */
+int mgldg_mg_vcycle_matrix(mgldg_mg mg, int precision, double* V);
+/* A dense row-major matrix as the driver writes V.dat (dg_tests.cpp:337-347,
+ * the Eigen stream operator under precision 17): one line per row, every
+ * coefficient right-aligned to the widest, single spaces between them. */
+int mgldg_dense_write(const char* path, int rows, int cols, const double* a);
 
 /* ======================= measurement hooks ======================= */
 /* No reference counterpart (the reference only prints a Timer).  Times

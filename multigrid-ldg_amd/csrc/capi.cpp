@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "../../include/mgldg.h"
+#include "io.hpp"
 #include "kernels.hpp"
 #include "mg.hpp"
 
@@ -582,6 +583,129 @@ int mgldg_mean_zero_device(mgldg_ctx ctx, long n, double* v, int nrhs)
         dev::mean_zero(c->stream, n, nrhs, v, c->colpart.p);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
+}
+
+// ---------------------------------------------------------------- interop file formats
+int mgldg_bsr_write_mm(const char* path, int nbrows, int nbcols, int br, int bc, const int* rowptr, const int* cols,
+                       const double* blocks)
+{
+    if (!path) return MGLDG_ERR_ARG;
+    return guard(nullptr, [&] { io::write_mm(path, nbrows, nbcols, br, bc, rowptr, cols, blocks); });
+}
+
+static void write_op_mm(Ctx* ctx, const DevOp& op, const char* path)
+{
+    std::vector<double> blk((size_t)op.nnzb() * op.pat.br * op.pat.bc);
+    export_op(ctx, op, nullptr, nullptr, blk.data(), nullptr);
+    io::write_mm(path, op.pat.nbr, op.pat.nbc, op.pat.br, op.pat.bc, op.pat.rowptr.data(), op.pat.cols.data(),
+                 blk.data());
+}
+
+static const DevOp* problem_op(ProblemH& p, int which)
+{
+    return which == 0 ? &p.A : which == 1 ? &p.M : which == 2 ? &p.T
+         : (which >= 3 && which - 3 < p.N) ? &p.G[which - 3] : nullptr;
+}
+
+int mgldg_problem_write_mm(mgldg_problem pr, int which, const char* path)
+{
+    if (!pr || !path) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] {
+        const DevOp* op = problem_op(pr->p, which);
+        ARG(op, "unknown operator");
+        write_op_mm(pr->ctx, *op, path);
+    });
+}
+
+int mgldg_mg_write_mm(mgldg_mg mg, int level, int which, const char* path)
+{
+    if (!mg || !path) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        MGH& m = mg->m;
+        ARG(level >= 0 && level < (int)m.L.size(), "level out of range");
+        Level& lv = m.L[level];
+        if (which == 6) {
+            ARG(level + 1 < (int)m.L.size(), "coarsest level has no interpolation");
+            const Interp& I = lv.I;
+            const size_t bs = (size_t)I.bf * I.bc;
+            std::vector<int> rp(I.nf + 1), cols(I.parent.begin(), I.parent.begin() + I.nf);
+            std::vector<double> blk((size_t)I.nf * bs);
+            for (int i = 0; i <= I.nf; ++i) rp[i] = i;
+            for (int i = 0; i < I.nf; ++i) std::memcpy(blk.data() + i * bs, I.pool.data() + (size_t)I.type[i] * bs, bs * sizeof(double));
+            io::write_mm(path, I.nf, I.nc, I.bf, I.bc, rp.data(), cols.data(), blk.data());
+            return;
+        }
+        const DevOp* op = which == 0 ? &lv.A : nullptr;
+        if (which >= 1 && which <= 5) {
+            ARG(lv.has_ops, "operator only kept on RATRAT levels");
+            op = which == 1 ? &lv.M : which == 2 ? &lv.Tp : (which - 3 < m.pr->N) ? &lv.G[which - 3] : nullptr;
+        }
+        ARG(op, "unknown operator");
+        write_op_mm(mg->ctx, *op, path);
+    });
+}
+
+int mgldg_fun_write(const char* path, int P, int N, int ne, int npl, const double* xyz, const double* u)
+{
+    if (!path) return MGLDG_ERR_ARG;
+    return guard(nullptr, [&] { io::write_fun(path, P, N, ne, npl, xyz, u); });
+}
+
+static void problem_fun(ProblemH& p, const double* u, const std::string& path)
+{
+    std::vector<double> xyz((size_t)p.ndof * p.N);
+    p.host->nodes_xyz(xyz.data());
+    io::write_fun(path, p.host->p + 1, p.N, p.ne, p.b, xyz.data(), u);
+}
+
+int mgldg_problem_write_fun(mgldg_problem pr, const double* u, const char* path)
+{
+    if (!pr || !u || !path) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] { problem_fun(pr->p, u, path); });
+}
+
+int mgldg_problem_dump(mgldg_problem pr, const double* rhs, const char* dir)
+{
+    if (!pr || !dir) return MGLDG_ERR_ARG;
+    return guard(pr->ctx, [&] {
+        ProblemH& p = pr->p;
+        const std::string d = std::string(dir) + "/";
+        write_op_mm(pr->ctx, p.A, (d + "A.mtx").c_str());
+        write_op_mm(pr->ctx, p.M, (d + "M.mtx").c_str());
+        write_op_mm(pr->ctx, p.T, (d + "T.mtx").c_str());
+        for (int i = 0; i < p.N; ++i) write_op_mm(pr->ctx, p.G[i], (d + "G" + std::to_string(i) + ".mtx").c_str());
+        if (rhs) problem_fun(p, rhs, d + "rhs.fun");
+    });
+}
+
+int mgldg_mesh_write(const char* path, int N, int np, const double* points, int nt, const int* simplices, int digits)
+{
+    if (!path) return MGLDG_ERR_ARG;
+    return guard(nullptr, [&] { io::write_mesh(path, N, np, points, nt, simplices, digits); });
+}
+
+int mgldg_wireframe_write(mgldg_wireframe wf, const char* path, int digits)
+{
+    if (!wf || !path) return MGLDG_ERR_ARG;
+    return guard(wf->ctx, [&] {
+        const int np = (int)(wf->p.size() / wf->N), nt = (int)(wf->t.size() / (wf->N + 1));
+        io::write_mesh(path, wf->N, np, wf->p.data(), nt, wf->t.data(), digits);
+    });
+}
+
+int mgldg_mg_vcycle_matrix(mgldg_mg mg, int precision, double* V)
+{
+    if (!mg || !V) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        mg->m.vcycle_matrix(precision, V);
+    });
+}
+
+int mgldg_dense_write(const char* path, int rows, int cols, const double* a)
+{
+    if (!path) return MGLDG_ERR_ARG;
+    return guard(nullptr, [&] { io::write_dense(path, rows, cols, a); });
 }
 
 int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double* ms, double* bytes, int* launches)

@@ -1476,6 +1476,33 @@ void MGH::vcycle(int precision, const double* b, double* x)
     }
 }
 
+// Multigrid::vcycle_matrix (multigrid.h:386-400): the V-cycle from a zero
+// iterate applied to every unit vector.  The unit vectors go through the
+// batched V-cycle k at a time (columns c0 .. c0+k-1 of the identity; a last
+// partial chunk carries zero columns, whose V-cycle is zero).
+void MGH::vcycle_matrix(int precision, double* V)
+{
+    hipStream_t s = ctx->stream;
+    const long n = pr->ndof;
+    const int k = (int)std::min<long>(n, 32);
+    ensure_batch(k, precision);
+    std::vector<double> E((size_t)n * k), X((size_t)n * k);
+    BatchLevel& b0 = BL[0];
+    for (long c0 = 0; c0 < n; c0 += k) {
+        const int kc = (int)std::min<long>(k, n - c0);
+        std::fill(E.begin(), E.end(), 0.0);
+        for (int j = 0; j < kc; ++j) E[(size_t)(c0 + j) * k + j] = 1.0;
+        b0.b64.upload(E.data(), E.size(), s);
+        if (precision == 1) dev::f64_to_f32(s, n * k, b0.b64, b0.b32);
+        bprecon(precision, k, false);
+        if (precision == 1) dev::f32_to_f64(s, n * k, b0.x32, b0.x64);
+        b0.x64.download(X.data(), X.size(), s);
+        HIPCHK(hipStreamSynchronize(s));
+        for (long i = 0; i < n; ++i)
+            for (int j = 0; j < kc; ++j) V[i * n + c0 + j] = X[(size_t)i * k + j];
+    }
+}
+
 // ---------------------------------------------------------------------------- measurement
 // Algorithmic HBM bytes of one GS launch over the rows R (sweep not from
 // zero): off-diagonal blocks and column indices of R, its D^-1 blocks and

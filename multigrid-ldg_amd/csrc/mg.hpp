@@ -217,6 +217,8 @@ struct MGH {
     void smooth(int l, int precision, int n, bool reverse, const void* b, void* x);
     void vcycle(int precision, const double* b, double* x);
     void vcycles(int precision, const double* b, double* x, int ncycle, double tol, double* rnorm, int* cycles);
+    // V[:, i] = V(e_i) from zero for every fine dof (host, row-major ndof x ndof)
+    void vcycle_matrix(int precision, double* V);
     void profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches, int part = 0);
     void profile_spmv(int l, int precision, int nrep, double* ms, double* bytes);
 };

@@ -136,6 +136,20 @@ class _Problem(_Handle):
         check(self.ctx, lib().mgldg_problem_export(self.h, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz)))
         return rp, cols, blk
 
+    def write_mm(self, which, path):
+        """SparseBlockMatrix::write (sparseblockmatrix.tpp:145-164) of A, M, T or G_d"""
+        check(self.ctx, lib().mgldg_problem_write_mm(self.h, self.OPS[which], os.fsencode(path)))
+
+    def write_fun(self, u, path):
+        """Function::write (function.h:173-190): u [ndof] at the problem's nodes"""
+        ua = _vec(u, self.ndof)
+        check(self.ctx, lib().mgldg_problem_write_fun(self.h, _pd(ua), os.fsencode(path)))
+
+    def dump(self, directory, rhs=None):
+        """LDGPoisson::dump (ldgpoisson.h:100-109): A, M, T, G_d .mtx and rhs.fun into directory"""
+        ra = None if rhs is None else _vec(rhs, self.ndof)
+        check(self.ctx, lib().mgldg_problem_dump(self.h, _pd(ra), os.fsencode(directory)))
+
 
 class LDGPoisson(_Problem):
     """Mesh<N,P>(qt) + BoundaryConditions + LDGPoisson<N,P>(mesh, bcs, tau0, tauD)
@@ -184,6 +198,11 @@ class Wireframe(_Handle):
         t = np.zeros((self.nt, self.N + 1), dtype=np.int32)
         check(self.ctx, lib().mgldg_wireframe_get(self.h, None, _pd(p), _pi(t)))
         return p, t
+
+    def write(self, path, digits=0):
+        """the .mesh layout (writemesh.m:17-28); digits 0 = writemesh.m's "%f",
+        1..17 significant digits (17 round-trips every double)"""
+        check(self.ctx, lib().mgldg_wireframe_write(self.h, os.fsencode(path), digits))
 
 
 class SimplexLDGPoisson(_Problem):
@@ -310,6 +329,19 @@ class Multigrid(_Handle):
         check(self.ctx, lib().mgldg_mg_export(self.h, level, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz), None, None))
         return rp, cols, blk, nbr.value, nbc.value
 
+    def write_mm(self, level, which, path):
+        """MatrixMarket dump of a level operator or interpolation (which as export)"""
+        w = {"A": 0, "M": 1, "T": 2, "G0": 3, "G1": 4, "G2": 5, "interp": 6}[which]
+        check(self.ctx, lib().mgldg_mg_write_mm(self.h, level, w, os.fsencode(path)))
+
+    def vcycle_matrix(self, precision="fp64"):
+        """Multigrid::vcycle_matrix (multigrid.h:386-400): column i = one V-cycle
+        from zero applied to e_i, [ndof, ndof]"""
+        n = self.prob.ndof
+        V = np.zeros((n, n))
+        check(self.ctx, lib().mgldg_mg_vcycle_matrix(self.h, _prec(precision), _pd(V)))
+        return V
+
     def colors(self, level):
         nc = C.c_int()
         c = np.zeros(self.level_rows[level], dtype=np.int32)
@@ -366,6 +398,70 @@ class Multigrid(_Handle):
         t = np.zeros(4)
         check(self.ctx, lib().mgldg_setup_times(self.h, _pd(t)))
         return dict(assembly_host=t[0], assembly_device=t[1], hierarchy_host=t[2], hierarchy_device=t[3])
+
+
+def _vec(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if a.size != n:
+        raise MGLDGError(-1, f"expected {n} values, got {a.size}")
+    return a
+
+
+# ---------------------------------------------------------------- interop file formats
+def write_mm(path, rowptr, cols, blocks, nbcols=None):
+    """SparseBlockMatrix::write (sparseblockmatrix.tpp:145-164) of a block CSR
+    matrix: blocks [nnzb, br, bc] row-major; nbcols defaults to the row count"""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    cl = np.ascontiguousarray(cols, dtype=np.int32)
+    bl = np.ascontiguousarray(blocks, dtype=np.float64)
+    nbr = len(rp) - 1
+    nbc = nbr if nbcols is None else nbcols
+    if bl.ndim != 3 or bl.shape[0] != len(cl) or rp[-1] != len(cl):
+        raise MGLDGError(-1, "blocks must be [nnzb, br, bc] with nnzb = rowptr[-1] = len(cols)")
+    check(None, lib().mgldg_bsr_write_mm(os.fsencode(path), nbr, nbc, bl.shape[1], bl.shape[2], _pi(rp), _pi(cl), _pd(bl)))
+
+
+def write_fun(path, P, xyz, u, npl):
+    """Function::write (function.h:173-190): header "P N ne", per element its
+    npl nodes as "x .. value" lines and an empty line; xyz [ndof, N], u [ndof]"""
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    ua = _vec(u, x.shape[0])
+    if x.shape[0] % npl:
+        raise MGLDGError(-1, "ndof is not a multiple of npl")
+    check(None, lib().mgldg_fun_write(os.fsencode(path), P, x.shape[1], x.shape[0] // npl, npl, _pd(x), _pd(ua)))
+
+
+def read_fun(path):
+    """Read a .fun file as src/DGTools.m:71-83 imports it: the header
+    (P, N, ne) and the per-element node lines as data [ne, npl, N + 1]
+    (coordinates, then the coefficient)."""
+    with open(path) as f:
+        head, _, body = f.read().partition("\n")
+    P, N, ne = (int(v) for v in head.split())
+    elems = [e for e in body.split("\n\n") if e.strip()]
+    data = np.array([[[float(v) for v in ln.split()] for ln in e.strip("\n").split("\n")] for e in elems])
+    if ne == 0 and data.size == 0:
+        return (P, N, ne), np.zeros((0, 0, N + 1))
+    if data.ndim != 3 or data.shape[0] != ne or data.shape[2] != N + 1:
+        raise MGLDGError(-1, f"{path}: {data.shape} does not match the header {(P, N, ne)}")
+    return (P, N, ne), data.reshape(ne, -1, N + 1)
+
+
+def write_mesh(path, points, simplices, digits=0):
+    """The .mesh layout (writemesh.m:17-28, read by Wireframe.read); simplices
+    0-based; digits 0 = "%f" as writemesh.m, 1..17 significant digits"""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    tri = np.ascontiguousarray(simplices, dtype=np.int32)
+    check(None, lib().mgldg_mesh_write(os.fsencode(path), pts.shape[1], pts.shape[0], _pd(pts), tri.shape[0],
+                                       _pi(tri), digits))
+
+
+def write_dense(path, a):
+    """A dense matrix as the driver writes V.dat (dg_tests.cpp:337-347)"""
+    A = np.ascontiguousarray(a, dtype=np.float64)
+    if A.ndim != 2:
+        raise MGLDGError(-1, "expected a matrix")
+    check(None, lib().mgldg_dense_write(os.fsencode(path), A.shape[0], A.shape[1], _pd(A)))
 
 
 def _prec(p):

@@ -73,6 +73,16 @@ SIGNATURES = {
     "mgldg_vcycle_device": (I, [V, I, V, V, I]),
     "mgldg_vcycles_device": (I, [V, I, V, V, I, D, PD, PI]),
     "mgldg_mean_zero_device": (I, [V, L, V, I]),
+    "mgldg_bsr_write_mm": (I, [C.c_char_p, I, I, I, I, PI, PI, PD]),
+    "mgldg_problem_write_mm": (I, [V, I, C.c_char_p]),
+    "mgldg_mg_write_mm": (I, [V, I, I, C.c_char_p]),
+    "mgldg_fun_write": (I, [C.c_char_p, I, I, I, I, PD, PD]),
+    "mgldg_problem_write_fun": (I, [V, PD, C.c_char_p]),
+    "mgldg_problem_dump": (I, [V, PD, C.c_char_p]),
+    "mgldg_mesh_write": (I, [C.c_char_p, I, I, PD, I, PI, I]),
+    "mgldg_wireframe_write": (I, [V, C.c_char_p, I]),
+    "mgldg_mg_vcycle_matrix": (I, [V, I, PD]),
+    "mgldg_dense_write": (I, [C.c_char_p, I, I, PD]),
     "mgldg_profile_relax": (I, [V, I, I, I, PD, PD, PI]),
     "mgldg_profile_relax_part": (I, [V, I, I, I, I, PD, PD, PI]),
     "mgldg_profile_spmv": (I, [V, I, I, I, PD, PD]),
