@@ -241,7 +241,18 @@ int mgldg_problem_bc_qpoints(mgldg_problem pr, double* xyz, int* tag)
 int mgldg_rhs(mgldg_problem pr, const double* f, const double* g, int nrhs, double* b)
 {
     if (!pr || !b || nrhs < 1) return MGLDG_ERR_ARG;
-    return guard(pr->ctx, [&] { pr->p.host->rhs(f, g, nrhs, b); });
+    return guard(pr->ctx, [&] {
+        // the device path (ProblemH::rhs_device) on staged copies
+        ProblemH& p = pr->p;
+        hipStream_t st = pr->ctx->stream;
+        DBuf<double> fd, gd, bd;
+        if (f) fd.upload(f, (size_t)p.ndof * nrhs, st);
+        if (g && p.nqb > 0) gd.upload(g, (size_t)p.nqb * nrhs, st);
+        bd.alloc((size_t)p.ndof * nrhs);
+        p.rhs_device(f ? fd.p : nullptr, (g && p.nqb > 0) ? gd.p : nullptr, nrhs, bd);
+        bd.download(b, (size_t)p.ndof * nrhs, st);
+        HIPCHK(hipStreamSynchronize(st));
+    });
 }
 
 static void export_op(Ctx* ctx, const DevOp& op, int* rowptr, int* cols, double* blocks, long* nnzb)

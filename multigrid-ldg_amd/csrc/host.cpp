@@ -320,7 +320,7 @@ int HostBSR::find(int i, int j) const
     return (it != e && *it == j) ? (int)(it - cols.begin()) : -1;
 }
 
-void PatternBuilder::finish(HostBSR& out, int br, int bc)
+void PatternBuilder::finish(HostBSR& out, int br, int bc, bool values)
 {
     out.nbr = nbr; out.nbc = nbc; out.br = br; out.bc = bc;
     // counting sort of the pairs by row, then sort and deduplicate each row
@@ -344,7 +344,8 @@ void PatternBuilder::finish(HostBSR& out, int br, int bc)
     }
     tmp.resize(w);
     out.cols.swap(tmp);
-    out.val.assign((size_t)out.cols.size() * br * bc, 0.0);
+    if (values) out.val.assign((size_t)out.cols.size() * br * bc, 0.0);
+    else out.val.clear();
 }
 
 // ======================================================================= mesh
@@ -428,87 +429,47 @@ void Mesh::build(const Quadtree& qt, const Basis& B)
     nbf = (int)bfaces.size();
 }
 
-// Mesh::lift (mesh.h:282-352)
-void lift(const Mesh& m, const Basis& B, const Face& f, double* L, double* LL, double* RR, double* RL)
-{
-    const int N = m.N, P = B.P, n = B.npl, d = f.dim;
-    double area = 1; for (int k = 0; k < N; ++k) if (k != d) area *= f.hi[k] - f.lo[k];
-    if (f.canonical && f.right >= 0) {
-        // slice the face nodes out of each element: face mass on matching nodes
-        int it[3], jt[3];
-        std::vector<int> sl(B.nfl), sr(B.nfl);
-        for (int i = 0; i < B.nfl; ++i) {
-            split_index(i, N - 1, P, it);
-            for (int k = 0, kk = 0; k < N; ++k) if (k != d) jt[k] = it[kk++];
-            jt[d] = f.sign > 0 ? P - 1 : 0; sl[i] = join_index(jt, N, P);
-            jt[d] = f.sign > 0 ? 0 : P - 1; sr[i] = join_index(jt, N, P);
-        }
-        for (double* X : {LL, RR, RL}) if (X) std::fill(X, X + (size_t)n * n, 0.0);
-        for (int i = 0; i < B.nfl; ++i)
-            for (int j = 0; j < B.nfl; ++j) {
-                double v = B.fmass[(size_t)i * B.nfl + j] * area;
-                if (LL) LL[(size_t)sl[i] * n + sl[j]] = v;
-                if (RR) RR[(size_t)sr[i] * n + sr[j]] = v;
-                if (RL) RL[(size_t)sr[i] * n + sl[j]] = v;
-            }
-        return;
-    }
-    const int nq = (N == 2) ? kQuad : kQuad * kQuad;
-    std::vector<double> pl((size_t)nq * n), pr((size_t)nq * n), w(nq);
-    int it[3];
-    for (int q = 0; q < nq; ++q) {
-        double ql[3], qr[3];
-        split_index(q, N - 1, kQuad, it);
-        if (f.canonical) {
-            for (int k = 0; k < N - 1; ++k) { int kk = k + (k >= d); ql[kk] = qr[kk] = B.qx[it[k]]; }
-            ql[d] = f.sign > 0 ? 1.0 : 0.0; qr[d] = f.sign > 0 ? 0.0 : 1.0;
-        } else {
-            const double* xl = m.face_q.data() + f.qoff; const double* xr = xl + (size_t)nq * N;
-            for (int k = 0; k < N; ++k) { ql[k] = xl[q * N + k]; qr[k] = xr[q * N + k]; }
-        }
-        B.lagrangeN(ql, &pl[(size_t)q * n]);
-        B.lagrangeN(qr, &pr[(size_t)q * n]);
-        double wq = 1; for (int k = 0; k < N - 1; ++k) wq *= B.qw[it[k]];
-        w[q] = wq;
-    }
-    for (int q = 0; q < nq; ++q) w[q] *= area;
-    if (L) for (int i = 0; i < n; ++i) for (int q = 0; q < nq; ++q) L[(size_t)i * nq + q] = pl[(size_t)q * n + i] * w[q];
-    auto quad = [&](const std::vector<double>& a, const std::vector<double>& b, double* out) {
-        for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) {
-            double s = 0;
-            for (int q = 0; q < nq; ++q) s += (a[(size_t)q * n + i] * w[q]) * b[(size_t)q * n + j];
-            out[(size_t)i * n + j] = s;
-        }
-    };
-    if (LL) quad(pl, pl, LL);
-    if (RR) quad(pr, pr, RR);
-    if (RL) quad(pr, pl, RL);
-}
-
 // ======================================================================= LDG
-static void gemm_nn(int n, double alpha, const double* A, const double* B, double* C)
+// Rows of the device assembly from the patterns: per element its T / G_d
+// diagonal blocks and, per listed face, the block (e, other side) of T and
+// (e, left) of G, with first-term flags (two faces may join the same pair
+// of elements on a periodic quadtree of width two).
+void AsmPlan::finish_rows(int ne, const HostBSR& T, const HostBSR* G)
 {
-    for (int i = 0; i < n; ++i) {
-        double* c = C + (size_t)i * n;
-        for (int j = 0; j < n; ++j) c[j] = 0;
-        for (int k = 0; k < n; ++k) {
-            double a = alpha * A[(size_t)i * n + k];
-            const double* b = B + (size_t)k * n;
-            for (int j = 0; j < n; ++j) c[j] += a * b[j];
+    tdiag.assign(ne, -1); gdiag.assign((size_t)ne * N, -1);
+    ef_tslot.assign(ef.size(), -1); ef_gslot.assign(ef.size(), -1); ef_first.assign(ef.size(), 0);
+    std::vector<int> gdim(ef.size(), 0);      // which G_d pattern ef_gslot indexes
+    for (int e = 0; e < ne; ++e) {
+        tdiag[e] = T.find(e, e);
+        for (int d = 0; d < N; ++d) gdiag[(size_t)e * N + d] = G[d].find(e, e);
+        for (int k = ef_off[e]; k < ef_off[e + 1]; ++k) {
+            const int f = ef[k] >> 1, side = ef[k] & 1;
+            if (fkind[f] == kDirichlet || fkind[f] == kNone) continue;
+            const int other = side ? fl[f] : fr[f];
+            ef_tslot[k] = T.find(e, other);
+            if (ef_tslot[k] < 0) throw Error(-4, "assembly: missing block in symbolic pattern");
+            if (side) {
+                int gd = 0;
+                if (!gshared) while (gd < N && fnormal[(size_t)f * N + gd] == 0.0) ++gd;
+                ef_gslot[k] = G[gd].find(e, other);
+                gdim[k] = gd;
+                if (ef_gslot[k] < 0) throw Error(-4, "assembly: missing block in symbolic pattern");
+            }
+            int first = 3;
+            for (int q = ef_off[e]; q < k; ++q) {
+                if (ef_tslot[q] == ef_tslot[k]) first &= ~1;
+                if (ef_gslot[k] >= 0 && ef_gslot[q] == ef_gslot[k] && gdim[q] == gdim[k]) first &= ~2;
+            }
+            ef_first[k] = first;
         }
     }
 }
 
-static void add_block(HostBSR& A, int i, int j, double alpha, const double* X)
-{
-    int k = A.find(i, j);
-    if (k < 0) throw Error(-4, "assembly: missing block in symbolic pattern");
-    double* p = A.block(k);
-    const size_t n = (size_t)A.br * A.bc;
-    for (size_t t = 0; t < n; ++t) p[t] += alpha * X[t];
-}
-
-// LDGPoisson::discretize without the Laplacian (ldgpoisson.tpp:14-100)
+// LDGPoisson::discretize without the Laplacian (ldgpoisson.tpp:14-100), the
+// symbolic half: the mesh, the patterns of M, T and G_d (addToBlock creates
+// structural blocks, sparseblockmatrix.tpp:92-100) and the plan from which
+// the device forms every block (mass, broken gradient with its volume and
+// lifting terms, penalty) and the boundary lifts.
 void LdgHost::assemble(const Quadtree& qt, int p_, const int* bct, double t0, double tD)
 {
     N = qt.N; p = p_; tau0 = t0; tauD = tD;
@@ -519,10 +480,11 @@ void LdgHost::assemble(const Quadtree& qt, int p_, const int* bct, double t0, do
     basis.init(N, p + 1);
     mesh.build(qt, basis);
     npl = basis.npl; nq = (N == 2) ? kQuad : kQuad * kQuad;
-    const int ne = mesh.ne, b = npl;
+    const int ne = mesh.ne, b = npl, P = basis.P, nfl = basis.nfl;
+    this->ne = ne; nf = mesh.nf; nbf = mesh.nbf;
     auto dirichlet = [&](const Face& f) { return f.right < 0 && bctype[-f.right - 1] == 0; };
 
-    // symbolic patterns (addToBlock creates structural blocks, sparseblockmatrix.tpp:92-100)
+    // patterns
     {
         PatternBuilder pm(ne, ne), pt(ne, ne);
         std::vector<PatternBuilder> pg(N, PatternBuilder(ne, ne));
@@ -535,96 +497,117 @@ void LdgHost::assemble(const Quadtree& qt, int p_, const int* bct, double t0, do
                 pg[f.dim].add(f.left, f.left); pt.add(f.left, f.left);
             }
         }
-        pm.finish(M, b, b); pt.finish(T, b, b);
-        for (int d = 0; d < N; ++d) pg[d].finish(G[d], b, b);
+        pm.finish(M, b, b, false); pt.finish(T, b, b, false);
+        for (int d = 0; d < N; ++d) pg[d].finish(G[d], b, b, false);
     }
-    // numeric
-    for (int e = 0; e < ne; ++e) add_block(M, e, e, mesh.volume(e), basis.mass.data());
-    for (int d = 0; d < N; ++d)
-        for (int e = 0; e < ne; ++e) add_block(G[d], e, e, 1.0 / mesh.width(e, d), basis.diff[d].data());
-    std::vector<double> LL((size_t)b * b), RR((size_t)b * b), RL((size_t)b * b), MI((size_t)b * b), W((size_t)b * b), RLt((size_t)b * b);
-    for (const Face& f : mesh.faces) {
-        if (f.right < 0 && !dirichlet(f)) continue;
-        lift(mesh, basis, f, nullptr, LL.data(), RR.data(), RL.data());
-        const double n = f.sign;
-        if (f.right >= 0) {
-            const double iv = 1.0 / mesh.volume(f.right);
-            for (size_t k = 0; k < MI.size(); ++k) MI[k] = basis.invmass[k] * iv;
-            gemm_nn(b, n, MI.data(), RR.data(), W.data());  add_block(G[f.dim], f.right, f.right, 1.0, W.data());
-            gemm_nn(b, -n, MI.data(), RL.data(), W.data()); add_block(G[f.dim], f.right, f.left, 1.0, W.data());
-            add_block(T, f.right, f.right, tau0, RR.data());
-            add_block(T, f.left, f.left, tau0, LL.data());
-            add_block(T, f.right, f.left, -tau0, RL.data());
-            for (int i = 0; i < b; ++i) for (int j = 0; j < b; ++j) RLt[(size_t)i * b + j] = RL[(size_t)j * b + i];
-            add_block(T, f.left, f.right, -tau0, RLt.data());
-        } else {
-            const double iv = 1.0 / mesh.volume(f.left);
-            for (size_t k = 0; k < MI.size(); ++k) MI[k] = basis.invmass[k] * iv;
-            gemm_nn(b, -n, MI.data(), LL.data(), W.data()); add_block(G[f.dim], f.left, f.left, 1.0, W.data());
-            add_block(T, f.left, f.left, tauD, LL.data());
+
+    // the plan.  Code 2d + s: the face normal to dimension d on the element
+    // side where that index is 0 (s = 0) or P - 1 (s = 1).
+    AsmPlan& A = plan;
+    A = AsmPlan();
+    A.N = N; A.b = b; A.nfl = nfl; A.ncode = 2 * N; A.nq = nq; A.gshared = 0;
+    A.emass = basis.mass; A.einv = basis.invmass; A.fmass = basis.fmass;
+    for (int d = 0; d < N; ++d) A.ediff.insert(A.ediff.end(), basis.diff[d].begin(), basis.diff[d].end());
+    A.slice.assign((size_t)A.ncode * nfl, 0); A.islice.assign((size_t)A.ncode * b, -1);
+    for (int code = 0; code < A.ncode; ++code) {
+        const int d = code / 2;
+        int it[3], jt[3];
+        for (int a = 0; a < nfl; ++a) {
+            split_index(a, N - 1, P, it);
+            for (int k = 0, kk = 0; k < N; ++k) if (k != d) jt[k] = it[kk++];
+            jt[d] = (code & 1) ? P - 1 : 0;
+            const int i = join_index(jt, N, P);
+            A.slice[(size_t)code * nfl + a] = i;
+            A.islice[(size_t)code * b + i] = a;
         }
     }
-    // boundary lifting blocks and face data for the RHS and its adjoint
-    this->ne = ne; nf = mesh.nf; nbf = mesh.nbf;
-    Lb.assign((size_t)nbf * b * nq, 0.0);
+    // reference boundary blocks per code: LL = sum_q w_q phi phi^T and the
+    // lift phi_i(q) w_q at the face rule (Mesh::lift, mesh.h:282-352, on a
+    // face of unit area)
+    std::vector<double> wq(nq), phi((size_t)nq * b);
+    A.pool.assign((size_t)A.ncode * b * b, 0.0); A.npool_ref = A.ncode;
+    A.lbref.assign((size_t)A.ncode * b * nq, 0.0);
+    for (int code = 0; code < A.ncode; ++code) {
+        const int d = code / 2;
+        int it[3];
+        for (int q = 0; q < nq; ++q) {
+            double x[3];
+            split_index(q, N - 1, kQuad, it);
+            for (int k = 0; k < N - 1; ++k) x[k + (k >= d)] = basis.qx[it[k]];
+            x[d] = (code & 1) ? 1.0 : 0.0;
+            basis.lagrangeN(x, &phi[(size_t)q * b]);
+            double w = 1; for (int k = 0; k < N - 1; ++k) w *= basis.qw[it[k]];
+            wq[q] = w;
+        }
+        double* LL = &A.pool[(size_t)code * b * b];
+        for (int i = 0; i < b; ++i)
+            for (int j = 0; j < b; ++j) {
+                double s = 0;
+                for (int q = 0; q < nq; ++q) s += (phi[(size_t)q * b + i] * wq[q]) * phi[(size_t)q * b + j];
+                LL[(size_t)i * b + j] = s;
+            }
+        double* Lr = &A.lbref[(size_t)code * b * nq];
+        for (int i = 0; i < b; ++i) for (int q = 0; q < nq; ++q) Lr[(size_t)i * nq + q] = phi[(size_t)q * b + i] * wq[q];
+    }
+    A.lnodes.assign(basis.nodes, basis.nodes + P); A.ldenom.assign(basis.denom, basis.denom + P);
+    // elements: M_e = |e| mass, M_e^-1 = invmass / |e|, G_d volume term diff_d / h_d
+    A.mscale.resize(ne); A.iscale.resize(ne); A.C.assign((size_t)ne * N * N, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        A.mscale[e] = mesh.volume(e); A.iscale[e] = 1.0 / mesh.volume(e);
+        for (int d = 0; d < N; ++d) A.C[((size_t)e * N + d) * N + d] = 1.0 / mesh.width(e, d);
+    }
+    // faces
+    const int nfc = mesh.nf;
+    A.fl.resize(nfc); A.fr.resize(nfc); A.fkind.resize(nfc); A.fca.assign(nfc, 0); A.fcb.assign(nfc, 0);
+    A.fscale.assign(nfc, 0.0); A.fnormal.assign((size_t)nfc * N, 0.0);
+    std::vector<int> cnt(ne + 1, 0);
+    for (int fi = 0; fi < nfc; ++fi) {
+        const Face& f = mesh.faces[fi];
+        double area = 1; for (int k = 0; k < N; ++k) if (k != f.dim) area *= f.hi[k] - f.lo[k];
+        A.fl[fi] = f.left; A.fr[fi] = f.right; A.fnormal[(size_t)fi * N + f.dim] = f.sign;
+        if (f.right >= 0) {
+            if (f.canonical) {
+                A.fkind[fi] = AsmPlan::kSlice; A.fca[fi] = 2 * f.dim + 1; A.fcb[fi] = 2 * f.dim; A.fscale[fi] = area;
+            } else {
+                // points from the mesh (mesh.h:44-77), weights with the area
+                A.fkind[fi] = AsmPlan::kDense;
+                A.fca[fi] = A.npool_ref + 3 * (int)A.qface.size();
+                A.qface.push_back(fi);
+                const double* xq = mesh.face_q.data() + f.qoff;
+                A.qpts.insert(A.qpts.end(), xq, xq + 2 * (size_t)nq * N);
+                for (int q = 0; q < nq; ++q) {
+                    int it[3]; split_index(q, N - 1, kQuad, it);
+                    double w = 1; for (int k = 0; k < N - 1; ++k) w *= basis.qw[it[k]];
+                    A.qwt.push_back(w * area);
+                }
+            }
+            cnt[f.left + 1]++; cnt[f.right + 1]++;
+        } else if (dirichlet(f)) {
+            A.fkind[fi] = AsmPlan::kDirichlet; A.fca[fi] = 2 * f.dim + (f.sign > 0 ? 1 : 0); A.fscale[fi] = area;
+            cnt[f.left + 1]++;
+        } else {
+            A.fkind[fi] = AsmPlan::kNone;
+        }
+    }
+    for (int e = 0; e < ne; ++e) cnt[e + 1] += cnt[e];
+    A.ef_off = cnt; A.ef.resize(cnt[ne]);
+    for (int fi = 0; fi < nfc; ++fi) {
+        const int k = A.fkind[fi];
+        if (k == AsmPlan::kNone) continue;
+        A.ef[cnt[A.fl[fi]]++] = fi << 1;
+        if (k != AsmPlan::kDirichlet) A.ef[cnt[A.fr[fi]]++] = (fi << 1) | 1;
+    }
+    A.finish_rows(ne, T, G);
+
+    // boundary faces: data for the right-hand side and its adjoint
     bleft.resize(nbf); btype.resize(nbf); btag.resize(nbf); bnormal.assign((size_t)nbf * N, 0.0);
+    A.bcode.resize(nbf); A.bscale.resize(nbf);
     for (int k = 0; k < nbf; ++k) {
         const Face& f = mesh.faces[mesh.bfaces[k]];
-        lift(mesh, basis, f, &Lb[(size_t)k * b * nq], nullptr, nullptr, nullptr);
+        double area = 1; for (int q = 0; q < N; ++q) if (q != f.dim) area *= f.hi[q] - f.lo[q];
+        A.bcode[k] = 2 * f.dim + (f.sign > 0 ? 1 : 0); A.bscale[k] = area;
         bleft[k] = f.left; btype[k] = bctype[-f.right - 1]; btag[k] = f.right;
         bnormal[(size_t)k * N + f.dim] = f.sign;
-    }
-}
-
-// add_source_terms + computeRHS (ldgpoisson.tpp:103-176), host, [ndof][nrhs]
-void LdgOps::rhs(const double* f, const double* g, int nrhs, double* out) const
-{
-    const int b = npl;
-    const size_t ndof = (size_t)ne * b;
-    std::vector<double> r(ndof), J[3], Lg(b);
-    for (int c = 0; c < nrhs; ++c) {
-        std::fill(r.begin(), r.end(), 0.0);
-        for (int d = 0; d < N; ++d) J[d].assign(ndof, 0.0);
-        if (g) {
-            for (int k = 0; k < nbf; ++k) {
-                const double* L = &Lb[(size_t)k * b * nq];
-                for (int i = 0; i < b; ++i) {
-                    double s = 0;
-                    for (int q = 0; q < nq; ++q) s += L[(size_t)i * nq + q] * g[((size_t)k * nq + q) * nrhs + c];
-                    Lg[i] = s;
-                }
-                const size_t off = (size_t)bleft[k] * b;
-                if (btype[k] == 0) {
-                    for (int d = 0; d < N; ++d) {
-                        const double n = bnormal[(size_t)k * N + d];
-                        if (n != 0.0) for (int i = 0; i < b; ++i) J[d][off + i] += n * Lg[i];
-                    }
-                    for (int i = 0; i < b; ++i) r[off + i] += tauD * Lg[i];
-                } else
-                    for (int i = 0; i < b; ++i) r[off + i] += Lg[i];
-            }
-            // r_e -= sum_d sum_i G_d(i,e)^T J_d(i)
-            for (int d = 0; d < N; ++d) {
-                const HostBSR& Gd = G[d];
-                for (int i = 0; i < ne; ++i)
-                    for (int k = Gd.rowptr[i]; k < Gd.rowptr[i + 1]; ++k) {
-                        const double* blk = Gd.block(k); const int e = Gd.cols[k];
-                        for (int cc = 0; cc < b; ++cc) {
-                            double s = 0;
-                            for (int rr = 0; rr < b; ++rr) s += blk[(size_t)rr * b + cc] * J[d][(size_t)i * b + rr];
-                            r[(size_t)e * b + cc] -= s;
-                        }
-                    }
-            }
-        }
-        for (int e = 0; e < ne; ++e) {
-            const double* Me = M.block(M.find(e, e));
-            for (int i = 0; i < b; ++i) {
-                double s = 0;
-                if (f) for (int j = 0; j < b; ++j) s += Me[(size_t)i * b + j] * f[((size_t)e * b + j) * nrhs + c];
-                out[((size_t)e * b + i) * nrhs + c] = r[(size_t)e * b + i] + s;
-            }
-        }
     }
 }
 

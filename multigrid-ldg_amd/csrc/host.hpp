@@ -83,7 +83,8 @@ struct PatternBuilder {
     std::vector<int> ri, ci;
     PatternBuilder(int m, int n) : nbr(m), nbc(n) {}
     void add(int i, int j) { ri.push_back(i); ci.push_back(j); }
-    void finish(HostBSR& out, int br, int bc);     // allocates zero values
+    // values = false: the pattern only (val stays empty)
+    void finish(HostBSR& out, int br, int bc, bool values = true);     // allocates zero values
 };
 
 // ---------------------------------------------------------------- mesh
@@ -108,8 +109,49 @@ struct Mesh {
     double width(int e, int d) const { return hi[e * N + d] - lo[e * N + d]; }
 };
 
-// Mesh::lift (mesh.h:282-352).  L (npl x nq) may be null; LL/RR/RL npl x npl.
-void lift(const Mesh& m, const Basis& B, const Face& f, double* L, double* LL, double* RR, double* RL);
+// ---------------------------------------------------------------- device assembly plan
+// The symbolic half of LDGPoisson::discretize (ldgpoisson.tpp:14-100) in
+// the form the device assembly reads.  Every face carries its face
+// matrices LL, RR, RL (the lifting terms, ldgpoisson.tpp:60-99) in one of
+// four forms:
+//   kSlice     conforming interior face: X = fscale * F_mass scattered
+//              through the slices of the two sides' codes;
+//   kDense     non-conforming interior face: LL, RR, RL at pool blocks
+//              fca, fca+1, fca+2, formed on the device by quadrature at the
+//              face's points (qface);
+//   kDirichlet Dirichlet boundary face: LL = fscale * pool block fca;
+//   kNone      Neumann boundary face: no operator term.
+// A code names how a face's nodes sit in an element: slice[code] maps face
+// node a to element node, islice[code] the element nodes back (-1 off the
+// face).  Rows are gathered per element: an element lists its faces in face
+// order with its side (ef = face << 1 | side, side 1 = right), the T block
+// of (e, other side) and the G block of (e, left) its terms land in, and
+// bit 0 / bit 1 of ef_first when it is the first term of that T / G block.
+struct AsmPlan {
+    enum { kSlice = 0, kDense = 1, kDirichlet = 2, kNone = 3 };
+    int N = 2, b = 0, nfl = 0, ncode = 0, nq = 0;
+    int gshared = 1;                          // 1: the G_d share one pattern (normals in every direction)
+    std::vector<double> mscale, iscale, C;    // per element: M_e = mscale mass, M_e^-1 = iscale invmass,
+                                              // G_d(e,e) volume term = sum_j C[e][j][d] diff_j (C: ne x N x N)
+    std::vector<int> ef_off, ef, ef_tslot, ef_gslot, ef_first;
+    std::vector<int> tdiag, gdiag;            // per element: T block (e,e) or -1; G_d block (e,e) (ne x N)
+    std::vector<int> fl, fr, fkind, fca, fcb; // per face
+    std::vector<double> fscale, fnormal;      // per face; normals nf x N
+    std::vector<double> emass, einv, ediff, fmass;   // reference element: mass, inverse, N diff (b x b); face mass (nfl x nfl)
+    std::vector<int> slice, islice;           // ncode x nfl, ncode x b
+    std::vector<double> pool;                 // reference b x b blocks (npool_ref of them; the device appends 3 per qface)
+    int npool_ref = 0;
+    // non-conforming faces: quadrature points (nq x N in the left element,
+    // then in the right), weights (area included) and the 1-D Lagrange
+    // nodes and denominators of the tensor basis
+    std::vector<int> qface;
+    std::vector<double> qpts, qwt, lnodes, ldenom;
+    // boundary lifts: Lb(boundary face k) = bscale[k] * lbref[bcode[k]] (b x nq)
+    std::vector<double> lbref, bscale;
+    std::vector<int> bcode;
+    // slots, first-term flags, diagonals and entry list from the patterns
+    void finish_rows(int ne, const HostBSR& T, const HostBSR* G);
+};
 
 // ---------------------------------------------------------------- LDG
 // The assembled discretisation handed to the device, whatever the mesh:
@@ -118,13 +160,11 @@ void lift(const Mesh& m, const Basis& B, const Face& f, double* L, double* LL, d
 struct LdgOps {
     int N = 2, p = 1, npl = 0, nq = 0, ne = 0, nf = 0, nbf = 0;
     double tau0 = 0, tauD = 0;
-    HostBSR M, T, G[3];
-    std::vector<double> Lb;                // boundary lift blocks, nbf x (npl x nq)
+    HostBSR M, T, G[3];                    // patterns (values are formed on the device)
+    AsmPlan plan;
     std::vector<int> bleft, btype, btag;   // per boundary face: element, 0 Dirichlet / 1 Neumann, boundary key
     std::vector<double> bnormal;           // per boundary face: outward unit normal (N)
     virtual ~LdgOps() = default;
-    // b = M f + B_g g (ldgpoisson.tpp:103-176), host, [ndof][nrhs]
-    void rhs(const double* f, const double* g, int nrhs, double* b) const;
     // boundary quadrature points (nbf*nq x N) and their boundary keys
     virtual void bc_qpoints(double* xyz, int* tag) const = 0;
     virtual void nodes_xyz(double* xyz) const = 0;

@@ -25,6 +25,34 @@ void to_panel_f32(hipStream_t s, long nblk, int b, const double* in, float* out)
 // gather diagonal blocks: out[i] = in[diag[i]]
 void gather_blocks(hipStream_t s, int n, int bs, const int* idx, const double* in, double* out);
 
+// ---------------------------------------------------------------- LDG assembly (assemble.hip)
+// The device half of LDGPoisson::discretize: one thread per entry of the
+// block rows of one element, walking the element's faces (the host plan,
+// AsmPlan in host.hpp, describes every array).  Writes M, T and G_d.
+struct AsmRows {
+    int ne = 0, N = 2, b = 0, nfl = 0, gshared = 1;
+    double tau0 = 0, tauD = 0;
+    const double *mscale = nullptr, *iscale = nullptr, *C = nullptr;
+    const int *ef_off = nullptr, *ef = nullptr, *ef_tslot = nullptr, *ef_gslot = nullptr, *ef_first = nullptr;
+    const int *tdiag = nullptr, *gdiag = nullptr;
+    const int *fkind = nullptr, *fca = nullptr, *fcb = nullptr;
+    const double *fscale = nullptr, *fnormal = nullptr;
+    const double *emass = nullptr, *einv = nullptr, *ediff = nullptr, *fmass = nullptr, *pool = nullptr;
+    const int *slice = nullptr, *islice = nullptr;
+    double *M = nullptr, *T = nullptr, *G0 = nullptr, *G1 = nullptr, *G2 = nullptr;
+};
+void asm_rows(hipStream_t s, const AsmRows& a);
+// LL, RR, RL (3 b x b blocks per face, into pool) of nface non-conforming
+// tensor faces by quadrature: pts 2 x nq x N reference points per face (left
+// element, then right), wts nq weights per face, 1-D Lagrange nodes and
+// barycentric denominators of the P-point basis
+void asm_quad(hipStream_t s, int nface, int N, int P, int nq, int b, const double* pts, const double* wts,
+              const double* lnodes, const double* ldenom, double* pool);
+// boundary lifts Lb(k) = bscale[k] lbref[bcode[k]] (b x nq per boundary face)
+void asm_lift(hipStream_t s, int nbf, int b, int nq, const int* bcode, const double* bscale, const double* lbref, double* Lb);
+// out block o = transpose of in block src[o] (row-major b x b)
+void transpose_blocks(hipStream_t s, long nblk, int b, const int* src, const double* in, double* out);
+
 // block rows one 256-thread workgroup of the single-RHS row kernels handles per pass
 int rows_per_workgroup(int B);
 

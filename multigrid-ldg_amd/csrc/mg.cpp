@@ -177,25 +177,58 @@ void ProblemH::build_simplex(Ctx* c, int N_, int np, const double* pts, int nt, 
     upload();
 }
 
+// LDGPoisson::discretize, device half: the patterns go up, the plan's
+// tables and face lists go up, and every block of M, T and G_d, the boundary
+// lifts and A = T + sum_d G_d^T M G_d (construct_laplacian,
+// ldgpoisson.tpp:178-190) are formed on the device.
 void ProblemH::upload()
 {
-    const LdgOps& H = *host;
+    LdgOps& H = *host;
+    const AsmPlan& P = H.plan;
     N = H.N; b = H.npl; ne = H.ne; ndof = (long)ne * b; nqb = H.nbf * H.nq;
     hipStream_t s = ctx->stream;
     auto t1 = Clock::now();
-    auto up = [&](DevOp& D, const HostBSR& X) { D.upload_pattern(X, s); D.val.upload(X.val.data(), X.val.size(), s); };
-    up(M, H.M); up(T, H.T);
-    for (int d = 0; d < N; ++d) up(G[d], H.G[d]);
-    build_laplacian(s, N, b, T, G, M, A);
-    // boundary data for the device RHS / adjoint
-    // (at least one element each, so that a mesh without boundary faces -- a
-    // periodic quadtree -- still passes valid pointers)
-    const int nbf = H.nbf;
+    const size_t bb = (size_t)b * b;
+    M.upload_pattern(H.M, s); T.upload_pattern(H.T, s);
+    for (int d = 0; d < N; ++d) G[d].upload_pattern(H.G[d], s);
+    M.val.alloc(std::max<size_t>((size_t)M.nnzb() * bb, 1));
+    T.val.alloc(std::max<size_t>((size_t)T.nnzb() * bb, 1));
+    for (int d = 0; d < N; ++d) G[d].val.alloc(std::max<size_t>((size_t)G[d].nnzb() * bb, 1));
+    // (at least one element each, so that empty lists -- a mesh without
+    // boundary faces, a periodic quadtree -- still pass valid pointers)
     auto up1 = [&](auto& D, const auto& v) {
-        if (D.n < 1) D.alloc(1);
+        if (D.n < std::max<size_t>(v.size(), 1)) D.alloc(std::max<size_t>(v.size(), 1));
         D.upload(v.data(), v.size(), s);
     };
-    up1(Lb, H.Lb);
+    DBuf<double> mscale, iscale, C, fscale, fnormal, emass, einv, ediff, fmass, pool, qpts, qwt, lnodes, ldenom, lbref, bscale;
+    DBuf<int> ef_off, ef, ef_tslot, ef_gslot, ef_first, tdiag, gdiag, fkind, fca, fcb, slice, islice, bcode;
+    up1(mscale, P.mscale); up1(iscale, P.iscale); up1(C, P.C); up1(fscale, P.fscale); up1(fnormal, P.fnormal);
+    up1(emass, P.emass); up1(einv, P.einv); up1(ediff, P.ediff); up1(fmass, P.fmass);
+    up1(ef_off, P.ef_off); up1(ef, P.ef); up1(ef_tslot, P.ef_tslot); up1(ef_gslot, P.ef_gslot); up1(ef_first, P.ef_first);
+    up1(tdiag, P.tdiag); up1(gdiag, P.gdiag); up1(fkind, P.fkind); up1(fca, P.fca); up1(fcb, P.fcb);
+    up1(slice, P.slice); up1(islice, P.islice);
+    const int nqf = (int)P.qface.size();
+    pool.alloc(((size_t)P.npool_ref + 3 * (size_t)nqf) * bb + 1);
+    pool.upload(P.pool.data(), P.pool.size(), s);
+    if (nqf) {
+        up1(qpts, P.qpts); up1(qwt, P.qwt); up1(lnodes, P.lnodes); up1(ldenom, P.ldenom);
+        dev::asm_quad(s, nqf, N, (int)P.lnodes.size(), P.nq, b, qpts, qwt, lnodes, ldenom, pool.p + (size_t)P.npool_ref * bb);
+    }
+    dev::AsmRows a;
+    a.ne = ne; a.N = N; a.b = b; a.nfl = P.nfl; a.gshared = P.gshared; a.tau0 = H.tau0; a.tauD = H.tauD;
+    a.mscale = mscale; a.iscale = iscale; a.C = C;
+    a.ef_off = ef_off; a.ef = ef; a.ef_tslot = ef_tslot; a.ef_gslot = ef_gslot; a.ef_first = ef_first;
+    a.tdiag = tdiag; a.gdiag = gdiag; a.fkind = fkind; a.fca = fca; a.fcb = fcb;
+    a.fscale = fscale; a.fnormal = fnormal;
+    a.emass = emass; a.einv = einv; a.ediff = ediff; a.fmass = fmass; a.pool = pool;
+    a.slice = slice; a.islice = islice;
+    a.M = M.val; a.T = T.val; a.G0 = G[0].val; a.G1 = N > 1 ? G[1].val.p : nullptr; a.G2 = N > 2 ? G[2].val.p : nullptr;
+    dev::asm_rows(s, a);
+    // boundary data for the device RHS / adjoint
+    const int nbf = H.nbf;
+    up1(lbref, P.lbref); up1(bscale, P.bscale); up1(bcode, P.bcode);
+    Lb.alloc(std::max<size_t>((size_t)nbf * b * H.nq, 1));
+    dev::asm_lift(s, nbf, b, H.nq, bcode, bscale, lbref, Lb);
     up1(bleft, H.bleft);
     up1(btype, H.btype);
     up1(bnormal, H.bnormal);
@@ -207,27 +240,37 @@ void ProblemH::upload()
         for (int k = 0; k < nbf; ++k) lst[pos[H.bleft[k]]++] = k;
     }
     bf_off.upload(off.data(), off.size(), s); bf_list.upload(lst.data(), lst.size(), s);
+    build_laplacian(s, N, b, T, G, M, A);
     // G_d (panel) and G_d^T (pattern transpose, transposed blocks, panel) for B_g and its adjoint
     for (int d = 0; d < N; ++d) {
         Gpan[d].alloc(G[d].val.n);
         dev::to_panel_f64(s, G[d].nnzb(), b, G[d].val, Gpan[d]);
         const HostBSR& Gh = H.G[d];
-        PatternBuilder pb(ne, ne);
-        for (int i = 0; i < ne; ++i) for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) pb.add(Gh.cols[t], i);
-        HostBSR Tr; pb.finish(Tr, b, b);
-        for (int i = 0; i < ne; ++i)
-            for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) {
-                double* dst = Tr.block(Tr.find(Gh.cols[t], i));
-                const double* src = Gh.block(t);
-                for (int r = 0; r < b; ++r) for (int q = 0; q < b; ++q) dst[(size_t)q * b + r] = src[(size_t)r * b + q];
-            }
+        HostBSR Tr;
+        Tr.nbr = ne; Tr.nbc = ne; Tr.br = b; Tr.bc = b;
+        Tr.rowptr.assign(ne + 1, 0);
+        for (int c : Gh.cols) Tr.rowptr[c + 1]++;
+        for (int i = 0; i < ne; ++i) Tr.rowptr[i + 1] += Tr.rowptr[i];
+        Tr.cols.resize(Gh.cols.size());
+        std::vector<int> src(std::max<size_t>(Gh.cols.size(), 1));
+        {
+            std::vector<int> pos(Tr.rowptr.begin(), Tr.rowptr.end() - 1);
+            for (int i = 0; i < ne; ++i)                     // rows ascending: each transposed row comes out sorted
+                for (int t = Gh.rowptr[i]; t < Gh.rowptr[i + 1]; ++t) {
+                    const int q = pos[Gh.cols[t]]++;
+                    Tr.cols[q] = i; src[q] = t;
+                }
+        }
         GT[d].upload_pattern(Tr, s);
-        DBuf<double> rm; rm.upload(Tr.val.data(), Tr.val.size(), s);
-        GT[d].val.alloc(Tr.val.size());
+        DBuf<int> srcd; srcd.upload(src.data(), src.size(), s);
+        DBuf<double> rm; rm.alloc(std::max<size_t>((size_t)Tr.nnzb() * bb, 1));
+        dev::transpose_blocks(s, Tr.nnzb(), b, srcd, G[d].val, rm);
+        GT[d].val.alloc(std::max<size_t>((size_t)Tr.nnzb() * bb, 1));
         dev::to_panel_f64(s, Tr.nnzb(), b, rm, GT[d].val);
         HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    H.plan = AsmPlan();                  // only the device needed it
     setup_dev_s = secs(t1);
 }
 

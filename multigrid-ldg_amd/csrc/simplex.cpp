@@ -328,24 +328,6 @@ static void face_rule(int N, std::vector<double>& x, std::vector<double>& w)
     put(a6, b6, w6); put(c6, a6, w6); put(b6, c6, w6); put(a6, c6, w6); put(c6, b6, w6); put(b6, a6, w6);
 }
 
-static void gemm_sq(int n, double alpha, const double* A, const double* B, double* C)
-{
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-            double s = 0;
-            for (int k = 0; k < n; ++k) s += A[i * n + k] * B[k * n + j];
-            C[i * n + j] = alpha * s;
-        }
-}
-
-static void add_blk(HostBSR& A, int i, int j, double alpha, const double* X)
-{
-    int k = A.find(i, j);
-    double* p = A.block(k);
-    const size_t n = (size_t)A.br * A.bc;
-    for (size_t t = 0; t < n; ++t) p[t] += alpha * X[t];
-}
-
 void SimplexLdg::assemble(int N_, int np, const double* pts, int nt, const int* tris, int nbnd, const int* vmask, int p_,
                           const int* bct, double t0, double tD)
 {
@@ -365,28 +347,82 @@ void SimplexLdg::assemble(int N_, int np, const double* pts, int nt, const int* 
     ne = mesh.ne; nf = mesh.nf; nbf = mesh.nbf;
     const int b = npl, nfl = F.npl;
     auto dirichlet = [&](const SimplexFace& f) { return f.right < 0 && bctype[-f.right - 1] == 0; };
-    // slices of interior faces: face node -> element node, by position (mesh.h:32-66)
-    std::vector<int> sl((size_t)nf * nfl, 0), sr((size_t)nf * nfl, 0);
-    for (int fi = 0; fi < nf; ++fi) {
-        const SimplexFace& f = mesh.faces[fi];
-        if (f.right < 0) continue;
-        for (int i = 0; i < nfl; ++i) {
-            if (E.P == 1) { sl[(size_t)fi * nfl] = 0; sr[(size_t)fi * nfl] = 0; break; }
-            double x[3], loc[3];
-            mesh.face_point(f, &F.nodes[(size_t)i * (N - 1)], x);
-            for (int side = 0; side < 2; ++side) {
-                mesh.to_local(side ? f.right : f.left, x, loc);
-                int m = -1;
-                for (int j = 0; j < b && m < 0; ++j) {
-                    bool ok = true;
-                    for (int d = 0; d < N; ++d) ok &= std::fabs(loc[d] - E.nodes[(size_t)j * N + d]) < 1e-6;
-                    if (ok) m = j;
-                }
-                if (m < 0) throw Error(-1, "face nodes do not match element nodes (non-conforming mesh)");
-                (side ? sr : sl)[(size_t)fi * nfl + i] = m;
-            }
+    // codes: face vertex k (ascending ids) is element vertex m_k; code =
+    // sum_k m_k (N+1)^k.  A face node with barycentric weights lambda over the
+    // face vertices sits at sum_k lambda_k V(m_k) in the element's reference
+    // coordinates (V(0) = 0, V(m) = e_{m-1}), so every code fixes a slice
+    // (face node -> element node, mesh.h:32-66) and the element basis at the
+    // face rule (mesh.h:67-76, 336-347).
+    AsmPlan& A = plan;
+    A = AsmPlan();
+    int ncode = 1;
+    for (int k = 0; k < N; ++k) ncode *= N + 1;
+    A.N = N; A.b = b; A.nfl = nfl; A.ncode = ncode; A.nq = nq; A.gshared = 1;
+    A.emass = E.mass; A.einv = E.invmass; A.fmass = F.mass;
+    for (int d = 0; d < N; ++d) A.ediff.insert(A.ediff.end(), E.diff[d].begin(), E.diff[d].end());
+    A.slice.assign((size_t)ncode * nfl, 0); A.islice.assign((size_t)ncode * b, -1);
+    A.pool.assign((size_t)ncode * b * b, 0.0); A.npool_ref = ncode;
+    A.lbref.assign((size_t)ncode * b * nq, 0.0);
+    auto decode = [&](int code, int* m) {
+        for (int k = 0; k < N; ++k) { m[k] = code % (N + 1); code /= N + 1; }
+        for (int k = 0; k < N; ++k) for (int q = 0; q < k; ++q) if (m[k] == m[q]) return false;
+        return true;
+    };
+    auto to_elem = [&](const int* m, const double* r, double* y) {
+        double l0 = 1;
+        for (int k = 0; k < N - 1; ++k) l0 -= r[k];
+        for (int d = 0; d < N; ++d) y[d] = 0;
+        for (int k = 0; k < N; ++k) {
+            const double lam = k ? r[k - 1] : l0;
+            if (m[k] > 0) y[m[k] - 1] += lam;
         }
+    };
+    std::vector<double> phi((size_t)nq * b);
+    for (int code = 0; code < ncode; ++code) {
+        int m[3];
+        if (!decode(code, m)) continue;
+        for (int a = 0; a < nfl; ++a) {
+            int hit = 0;
+            if (E.P > 1) {
+                double y[3];
+                to_elem(m, &F.nodes[(size_t)a * (N - 1)], y);
+                hit = -1;
+                for (int j = 0; j < b && hit < 0; ++j) {
+                    bool ok = true;
+                    for (int d = 0; d < N; ++d) ok &= std::fabs(y[d] - E.nodes[(size_t)j * N + d]) < 1e-9;
+                    if (ok) hit = j;
+                }
+                if (hit < 0) throw Error(-1, "face nodes do not match element nodes");
+            }
+            A.slice[(size_t)code * nfl + a] = hit;
+            A.islice[(size_t)code * b + hit] = a;
+        }
+        for (int q = 0; q < nq; ++q) {
+            double y[3];
+            to_elem(m, &qx[(size_t)q * (N - 1)], y);
+            E.eval(y, &phi[(size_t)q * b]);
+        }
+        double* LL = &A.pool[(size_t)code * b * b];
+        for (int i = 0; i < b; ++i)
+            for (int j = 0; j < b; ++j) {
+                double s = 0;
+                for (int q = 0; q < nq; ++q) s += phi[(size_t)q * b + i] * qw[q] * phi[(size_t)q * b + j];
+                LL[(size_t)i * b + j] = s;
+            }
+        double* Lr = &A.lbref[(size_t)code * b * nq];
+        for (int i = 0; i < b; ++i) for (int q = 0; q < nq; ++q) Lr[(size_t)i * nq + q] = phi[(size_t)q * b + i] * qw[q];
     }
+    auto code_of = [&](const SimplexFace& f, int e) {
+        const int* t = &mesh.tris[(size_t)e * (N + 1)];
+        int code = 0, pw = 1;
+        for (int k = 0; k < N; ++k) {
+            int mk = 0;
+            while (mk <= N && t[mk] != f.v[k]) ++mk;
+            if (mk > N) throw Error(-1, "face vertex not in its element");
+            code += mk * pw; pw *= N + 1;
+        }
+        return code;
+    };
     // patterns (addToBlock creates structural blocks for every normal
     // component, so the N gradient patterns are one)
     {
@@ -400,111 +436,53 @@ void SimplexLdg::assemble(int N_, int np, const double* pts, int nt, const int* 
                 pt.add(f.left, f.left);
             }
         }
-        pm.finish(M, b, b); pt.finish(T, b, b);
-        pg.finish(G[0], b, b);
+        pm.finish(M, b, b, false); pt.finish(T, b, b, false);
+        pg.finish(G[0], b, b, false);
         for (int d = 1; d < N; ++d) G[d] = G[0];
     }
-    // mass and broken gradient (ldgpoisson.tpp:26-58)
-    std::vector<double> RLt((size_t)b * b);
-    for (int e = 0; e < ne; ++e) add_blk(M, e, e, mesh.vol[e] / E.volume, E.mass.data());
-    for (int e = 0; e < ne; ++e) {
-        const double* I = &mesh.Jinv[(size_t)e * N * N];
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) add_blk(G[i], e, e, I[j * N + i], E.diff[j].data());
-    }
-    // boundary evaluation of the left element's basis at the face rule (mesh.h:67-76, 336-347)
-    auto boundary_phi = [&](const SimplexFace& f, std::vector<double>& phi, std::vector<double>& w) {
-        phi.resize((size_t)nq * b); w.resize(nq);
-        for (int q = 0; q < nq; ++q) {
-            double x[3], loc[3];
-            mesh.face_point(f, &qx[(size_t)q * (N - 1)], x);
-            mesh.to_local(f.left, x, loc);
-            E.eval(loc, &phi[(size_t)q * b]);
-            w[q] = qw[q] * f.area * F.volume;   // as written in mesh.h:343-344
-        }
-    };
-    // lifting and penalty terms (ldgpoisson.tpp:60-99).  Per chunk of faces
-    // the face blocks LL, RR, RL and the products M_e^{-1} RR, M_e^{-1} RL
-    // (interior) or M_e^{-1} LL (Dirichlet) are formed in parallel, then added
-    // into the operators in face order.
-    const size_t bb = (size_t)b * b;
-    const long chunk = 32768;
-    std::vector<double> fbuf;
-    for (long c0 = 0; c0 < nf; c0 += chunk) {
-        const long c1 = std::min<long>(nf, c0 + chunk);
-        fbuf.resize((size_t)(c1 - c0) * 5 * bb);
-        parallel_for(c1 - c0, [&](long i0, long i1) {
-            std::vector<double> MIt(bb), ph, wq;
-            for (long t = i0; t < i1; ++t) {
-                const int fi = (int)(c0 + t);
-                const SimplexFace& f = mesh.faces[fi];
-                double* LLb = &fbuf[(size_t)t * 5 * bb];
-                double *RRb = LLb + bb, *RLb = LLb + 2 * bb, *W1 = LLb + 3 * bb, *W2 = LLb + 4 * bb;
-                if (f.right >= 0) {
-                    std::fill(LLb, LLb + 3 * bb, 0.0);
-                    const double jac = f.area / F.volume;
-                    const int* a = &sl[(size_t)fi * nfl]; const int* c = &sr[(size_t)fi * nfl];
-                    for (int i = 0; i < nfl; ++i)
-                        for (int j = 0; j < nfl; ++j) {
-                            const double v = F.mass[(size_t)i * nfl + j] * jac;
-                            LLb[(size_t)a[i] * b + a[j]] += v;
-                            RRb[(size_t)c[i] * b + c[j]] += v;
-                            RLb[(size_t)c[i] * b + a[j]] += v;
-                        }
-                    const double iv = E.volume / mesh.vol[f.right];
-                    for (size_t k = 0; k < bb; ++k) MIt[k] = E.invmass[k] * iv;
-                    gemm_sq(b, 1.0, MIt.data(), RRb, W1);
-                    gemm_sq(b, 1.0, MIt.data(), RLb, W2);
-                } else if (dirichlet(f)) {
-                    boundary_phi(f, ph, wq);
-                    for (int i = 0; i < b; ++i)
-                        for (int j = 0; j < b; ++j) {
-                            double sv = 0;
-                            for (int q = 0; q < nq; ++q) sv += ph[(size_t)q * b + i] * wq[q] * ph[(size_t)q * b + j];
-                            LLb[(size_t)i * b + j] = sv;
-                        }
-                    const double iv = E.volume / mesh.vol[f.left];
-                    for (size_t k = 0; k < bb; ++k) MIt[k] = E.invmass[k] * iv;
-                    gemm_sq(b, 1.0, MIt.data(), LLb, W1);
-                }
-            }
-        });
-        for (long t = 0; t < c1 - c0; ++t) {
-            const SimplexFace& f = mesh.faces[c0 + t];
-            const double* LLb = &fbuf[(size_t)t * 5 * bb];
-            const double *RRb = LLb + bb, *RLb = LLb + 2 * bb, *W1 = LLb + 3 * bb, *W2 = LLb + 4 * bb;
-            if (f.right >= 0) {
-                for (int d = 0; d < N; ++d) {
-                    const double n = f.normal[d];
-                    add_blk(G[d], f.right, f.right, n, W1);
-                    add_blk(G[d], f.right, f.left, -n, W2);
-                }
-                add_blk(T, f.right, f.right, tau0, RRb);
-                add_blk(T, f.left, f.left, tau0, LLb);
-                add_blk(T, f.right, f.left, -tau0, RLb);
-                for (int i = 0; i < b; ++i) for (int j = 0; j < b; ++j) RLt[(size_t)i * b + j] = RLb[(size_t)j * b + i];
-                add_blk(T, f.left, f.right, -tau0, RLt.data());
-            } else if (dirichlet(f)) {
-                for (int d = 0; d < N; ++d) add_blk(G[d], f.left, f.left, -f.normal[d], W1);
-                add_blk(T, f.left, f.left, tauD, LLb);
-            }
+    // elements: M_e = (|e| / |ref|) mass, M_e^-1 its inverse, G_i volume
+    // term sum_j Jinv(j, i) diff_j (ldgpoisson.tpp:26-58)
+    A.mscale.resize(ne); A.iscale.resize(ne);
+    A.C.assign(mesh.Jinv.begin(), mesh.Jinv.end());
+    for (int e = 0; e < ne; ++e) { A.mscale[e] = mesh.vol[e] / E.volume; A.iscale[e] = E.volume / mesh.vol[e]; }
+    // faces: interior by slices with jacobian |f| / |ref face|, Dirichlet by
+    // the reference blocks with |f| |ref face| (mesh.h:343-344 as written)
+    A.fl.resize(nf); A.fr.resize(nf); A.fkind.resize(nf); A.fca.assign(nf, 0); A.fcb.assign(nf, 0);
+    A.fscale.assign(nf, 0.0); A.fnormal.assign((size_t)nf * N, 0.0);
+    std::vector<int> cnt(ne + 1, 0);
+    for (int fi = 0; fi < nf; ++fi) {
+        const SimplexFace& f = mesh.faces[fi];
+        A.fl[fi] = f.left; A.fr[fi] = f.right;
+        for (int d = 0; d < N; ++d) A.fnormal[(size_t)fi * N + d] = f.normal[d];
+        if (f.right >= 0) {
+            A.fkind[fi] = AsmPlan::kSlice; A.fca[fi] = code_of(f, f.left); A.fcb[fi] = code_of(f, f.right);
+            A.fscale[fi] = f.area / F.volume;
+            cnt[f.left + 1]++; cnt[f.right + 1]++;
+        } else if (dirichlet(f)) {
+            A.fkind[fi] = AsmPlan::kDirichlet; A.fca[fi] = code_of(f, f.left); A.fscale[fi] = f.area * F.volume;
+            cnt[f.left + 1]++;
+        } else {
+            A.fkind[fi] = AsmPlan::kNone;
         }
     }
-    // boundary lifting blocks and face data for the RHS and its adjoint
-    Lb.assign((size_t)nbf * b * nq, 0.0);
+    for (int e = 0; e < ne; ++e) cnt[e + 1] += cnt[e];
+    A.ef_off = cnt; A.ef.resize(cnt[ne]);
+    for (int fi = 0; fi < nf; ++fi) {
+        const int k = A.fkind[fi];
+        if (k == AsmPlan::kNone) continue;
+        A.ef[cnt[A.fl[fi]]++] = fi << 1;
+        if (k != AsmPlan::kDirichlet) A.ef[cnt[A.fr[fi]]++] = (fi << 1) | 1;
+    }
+    A.finish_rows(ne, T, G);
+    // boundary faces: data for the right-hand side and its adjoint
     bleft.resize(nbf); btype.resize(nbf); btag.resize(nbf); bnormal.assign((size_t)nbf * N, 0.0);
-    parallel_for(nbf, [&](long k0, long k1) {
-        std::vector<double> ph, wq;
-        for (long k = k0; k < k1; ++k) {
-            const SimplexFace& f = mesh.faces[mesh.bfaces[k]];
-            boundary_phi(f, ph, wq);
-            double* L = &Lb[(size_t)k * b * nq];
-            for (int i = 0; i < b; ++i)
-                for (int q = 0; q < nq; ++q) L[(size_t)i * nq + q] = ph[(size_t)q * b + i] * wq[q];
-            bleft[k] = f.left; btag[k] = -f.right; btype[k] = bctype[-f.right - 1];
-            for (int d = 0; d < N; ++d) bnormal[(size_t)k * N + d] = f.normal[d];
-        }
-    });
+    A.bcode.resize(nbf); A.bscale.resize(nbf);
+    for (int k = 0; k < nbf; ++k) {
+        const SimplexFace& f = mesh.faces[mesh.bfaces[k]];
+        A.bcode[k] = code_of(f, f.left); A.bscale[k] = f.area * F.volume;
+        bleft[k] = f.left; btag[k] = -f.right; btype[k] = bctype[-f.right - 1];
+        for (int d = 0; d < N; ++d) bnormal[(size_t)k * N + d] = f.normal[d];
+    }
 }
 
 void SimplexLdg::bc_qpoints(double* xyz, int* tag) const
