@@ -175,8 +175,20 @@ def unstructured_leg(mgldg, torch, nrhs, warm):
         "mixed_batched_vs_fp64_single": round(out["batched_mixed"]["value"] / out["single_fp64"]["value"], 3),
         "mixed_vs_fp64_batched_rel_diff": diff,
         "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
-                    "hierarchy": round(t_mg, 3)},
+                    "hierarchy": round(t_mg, 3), "assembly": assembly_info(prob)},
     }
+
+
+def assembly_info(prob):
+    """The device half of LDGPoisson::discretize: host symbolic and device
+    phases (s), and the assembly kernel against the HBM roofline (its
+    algorithmic bytes -- every block of M, T, G_d written once, the plan read
+    once -- over its duration, HIP events)"""
+    t = prob.setup_times()
+    gbs = t["assembly_bytes"] / (t["assembly_kernel_ms"] * 1e-3) / 1e9
+    return {"host_symbolic_s": round(t["host_s"], 3), "device_s": round(t["device_s"], 3), "kernel": "k_asm_rows",
+            "kernel_ms": round(t["assembly_kernel_ms"], 3), "bytes": int(t["assembly_bytes"]),
+            "achieved_gbs": round(gbs, 1), "peak_gbs": 8000.0, "frac": round(gbs / 8000.0, 4)}
 
 
 def bumps(X, seed, n=16):
@@ -447,7 +459,7 @@ def ball_leg(mgldg, torch, nrhs=16, warm=1):
         "adjoint_check_rel_err": adj_err,
         "kernel_plan": plan_summary(mg.batch_plan()), "mfma_fine_apply": mfma_use(mg, mg.batch_plan(), 2 * nrhs),
         "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
-                    "hierarchy": round(t_mg, 3)},
+                    "hierarchy": round(t_mg, 3), "assembly": assembly_info(prob)},
     }
 
 
@@ -727,7 +739,8 @@ def main():
         "hp_multigrid": hp,
         "periodic": periodic,
         "error_vs_fp64_oracle": err,
-        "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3)},
+        "setup_s": {"quadtree": round(t_qt, 3), "problem": round(t_prob, 3), "hierarchy": round(t_mg, 3),
+                    "assembly": assembly_info(prob)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "unstructured": unstructured,

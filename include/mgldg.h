@@ -356,6 +356,12 @@ int mgldg_profile_relax_part(mgldg_mg mg, int level, int precision, int nrep, in
                              double* bytes_per_launch, int* launches);
 int mgldg_profile_spmv(mgldg_mg mg, int level, int precision, int nrep, double* ms_per_launch,
                        double* bytes_per_launch);
+/* problem setup: t[0] host symbolic phase (mesh, patterns, assembly plan)
+ * and t[1] device phase (upload, assembly, A = T + sum G^T M G, G_d^T) in
+ * seconds; t[2] the assembly kernel's duration (ms, HIP events) and t[3] its
+ * algorithmic HBM bytes (every block of M, T, G_d written once, plan data
+ * read once) */
+int mgldg_problem_setup_times(mgldg_problem pr, double* t);
 /* setup timings: t[0] host assembly, t[1] device assembly (A), t[2] hierarchy
  * host symbolic, t[3] hierarchy device numeric (seconds) */
 int mgldg_setup_times(mgldg_mg mg, double* t);

@@ -756,6 +756,14 @@ int mgldg_profile_spmv(mgldg_mg mg, int level, int precision, int nrep, double* 
     });
 }
 
+int mgldg_problem_setup_times(mgldg_problem pr, double* t)
+{
+    if (!pr || !t) return MGLDG_ERR_ARG;
+    const ProblemH& p = pr->p;
+    t[0] = p.setup_host_s; t[1] = p.setup_dev_s; t[2] = p.asm_ms; t[3] = p.asm_bytes;
+    return MGLDG_OK;
+}
+
 int mgldg_setup_times(mgldg_mg mg, double* t)
 {
     if (!mg || !t) return MGLDG_ERR_ARG;

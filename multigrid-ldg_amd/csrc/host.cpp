@@ -646,8 +646,7 @@ void laplacian_symbolic(const HostBSR& T, const HostBSR* G, int N, const HostBSR
         for (int k = 0; k < n; ++k)
             for (int a = G[d].rowptr[k]; a < G[d].rowptr[k + 1]; ++a)
                 for (int c = G[d].rowptr[k]; c < G[d].rowptr[k + 1]; ++c) pb.add(G[d].cols[a], G[d].cols[c]);
-    pb.finish(out.pattern, T.br, T.bc);
-    out.pattern.val.clear(); out.pattern.val.shrink_to_fit();
+    pb.finish(out.pattern, T.br, T.bc, false);
     const HostBSR& A = out.pattern;
     out.tslot.assign(A.nnzb(), -1);
     for (int i = 0; i < n; ++i) for (int k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) out.tslot[k] = T.find(i, A.cols[k]);
@@ -688,8 +687,7 @@ void rap_symbolic(const Interp& T, const HostBSR& X, HostBSR& C, TermList& terms
     PatternBuilder pb(T.nc, T.nc);
     for (int k = 0; k < X.nbr; ++k)
         for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) pb.add(T.parent[k], T.parent[X.cols[s]]);
-    pb.finish(C, T.bc, T.bc);
-    C.val.clear(); C.val.shrink_to_fit();
+    pb.finish(C, T.bc, T.bc, false);
     std::vector<int> cnt(C.nnzb() + 1, 0), slot(X.nnzb());
     for (int k = 0; k < X.nbr; ++k)
         for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) { slot[s] = C.find(T.parent[k], T.parent[X.cols[s]]); cnt[slot[s] + 1]++; }

@@ -223,7 +223,21 @@ void ProblemH::upload()
     a.emass = emass; a.einv = einv; a.ediff = ediff; a.fmass = fmass; a.pool = pool;
     a.slice = slice; a.islice = islice;
     a.M = M.val; a.T = T.val; a.G0 = G[0].val; a.G1 = N > 1 ? G[1].val.p : nullptr; a.G2 = N > 2 ? G[2].val.p : nullptr;
+    hipEvent_t ea, eb;
+    HIPCHK(hipEventCreate(&ea)); HIPCHK(hipEventCreate(&eb));
+    HIPCHK(hipEventRecord(ea, s));
     dev::asm_rows(s, a);
+    HIPCHK(hipEventRecord(eb, s));
+    // algorithmic bytes: every block written once, the per-element, per-entry
+    // and per-face plan data read once (reference tables are cache-resident)
+    {
+        long blocks = M.nnzb() + T.nnzb();
+        for (int d = 0; d < N; ++d) blocks += G[d].nnzb();
+        asm_bytes = (double)blocks * bb * 8
+                  + (double)ne * (8.0 * (2 + N * N) + 4.0 * (2 + N))
+                  + (double)P.ef.size() * 16.0
+                  + (double)P.fkind.size() * (12.0 + 8.0 + 8.0 * N);
+    }
     // boundary data for the device RHS / adjoint
     const int nbf = H.nbf;
     up1(lbref, P.lbref); up1(bscale, P.bscale); up1(bcode, P.bcode);
@@ -270,6 +284,12 @@ void ProblemH::upload()
         HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ea, eb));
+        asm_ms = ms;
+        (void)hipEventDestroy(ea); (void)hipEventDestroy(eb);
+    }
     H.plan = AsmPlan();                  // only the device needed it
     setup_dev_s = secs(t1);
 }

@@ -77,6 +77,7 @@ struct ProblemH {
     int N = 2, b = 0, ne = 0, nqb = 0;
     long ndof = 0;
     double setup_host_s = 0, setup_dev_s = 0;
+    double asm_ms = 0, asm_bytes = 0;   // the assembly kernel: duration and algorithmic HBM bytes
     LdgHost* cart() const { return dynamic_cast<LdgHost*>(host.get()); }
     SimplexLdg* simplex() const { return dynamic_cast<SimplexLdg*>(host.get()); }
     void build(Ctx* c, const Quadtree* q, int p, const int* bct, double tau0, double tauD);

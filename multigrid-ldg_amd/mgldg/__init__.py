@@ -136,6 +136,12 @@ class _Problem(_Handle):
         check(self.ctx, lib().mgldg_problem_export(self.h, w, _pi(rp), _pi(cols), _pd(blk), C.byref(nz)))
         return rp, cols, blk
 
+    def setup_times(self):
+        """host symbolic and device phases (s), the assembly kernel (ms) and its algorithmic bytes"""
+        t = np.zeros(4)
+        check(self.ctx, lib().mgldg_problem_setup_times(self.h, _pd(t)))
+        return dict(host_s=t[0], device_s=t[1], assembly_kernel_ms=t[2], assembly_bytes=t[3])
+
     def write_mm(self, which, path):
         """SparseBlockMatrix::write (sparseblockmatrix.tpp:145-164) of A, M, T or G_d"""
         check(self.ctx, lib().mgldg_problem_write_mm(self.h, self.OPS[which], os.fsencode(path)))

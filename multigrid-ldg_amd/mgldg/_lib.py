@@ -87,6 +87,7 @@ SIGNATURES = {
     "mgldg_profile_relax_part": (I, [V, I, I, I, I, PD, PD, PI]),
     "mgldg_profile_spmv": (I, [V, I, I, I, PD, PD]),
     "mgldg_setup_times": (I, [V, PD]),
+    "mgldg_problem_setup_times": (I, [V, PD]),
 }
 
 _lib = None
