@@ -161,6 +161,11 @@ typedef struct {
                              reference's Multigrid::vcycle_matrix, multigrid.h:
                              386-400, on that level) and applied as one GEMV:
                              the same linear map in one launch. <= 8192. */
+    int pool_off;      /* 0: where at most a quarter of a level's blocks are
+                          distinct (quadtree levels), the single-RHS kernels
+                          read them from a pool of the distinct blocks (the
+                          same bits, so the same results, with a fraction of
+                          the HBM traffic); 1: always the full block arrays */
 } mgldg_mg_params;
 
 /* InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
@@ -220,6 +225,9 @@ int mgldg_mg_batch_plan(mgldg_mg mg, int* plan, float* ms);
  * Exported patterns are the host symbolic phase; values come from the GPU. */
 int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols,
                     double* blocks, long* nnzb, int* nbrows, int* nbcols);
+/* no reference counterpart: the pool of level l (npool distinct A blocks,
+ * ndpool distinct D^-1 blocks; 0, 0 when the level is not pooled) */
+int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool);
 /* colouring used by the multicolour Gauss-Seidel on level l */
 int mgldg_mg_colors(mgldg_mg mg, int level, int* colors, int* ncolors);
 int mgldg_mg_destroy(mgldg_mg mg);

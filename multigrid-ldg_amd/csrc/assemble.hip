@@ -188,9 +188,51 @@ __global__ void k_transpose_blocks(long nblk, int b, const int* __restrict__ src
     out[t] = in[(size_t)src[o] * bb + (size_t)j * b + i];
 }
 
+// 64-bit hash of the bit pattern of each row-major FP64 block (bs words)
+__global__ void k_block_hash(long nblk, int bs, const double* __restrict__ in, unsigned long long* __restrict__ out)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    const unsigned long long* w = reinterpret_cast<const unsigned long long*>(in + k * bs);
+    unsigned long long h = 0x243F6A8885A308D3ull ^ (unsigned long long)bs;
+    for (int i = 0; i < bs; ++i) {
+        h ^= w[i];
+        h *= 0x9E3779B97F4A7C15ull;
+        h ^= h >> 29;
+    }
+    out[k] = h;
+}
+
+// bad[k] = 1 unless block k equals block rep[pidx[k]] bit for bit
+__global__ void k_block_verify(long nblk, int bs, const double* __restrict__ in, const int* __restrict__ pidx,
+                               const int* __restrict__ rep, int* __restrict__ bad)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    const unsigned long long* a = reinterpret_cast<const unsigned long long*>(in + k * bs);
+    const unsigned long long* c = reinterpret_cast<const unsigned long long*>(in + (long)rep[pidx[k]] * bs);
+    int d = 0;
+    for (int i = 0; i < bs; ++i) d |= a[i] != c[i];
+    bad[k] = d;
+}
+
 unsigned grid_of(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+void block_hash(hipStream_t s, long nblk, int bs, const double* in, unsigned long long* out)
+{
+    if (nblk <= 0) return;
+    hipLaunchKernelGGL(k_block_hash, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, bs, in, out);
+    check_asm("block_hash");
+}
+
+void block_verify(hipStream_t s, long nblk, int bs, const double* in, const int* pidx, const int* rep, int* bad)
+{
+    if (nblk <= 0) return;
+    hipLaunchKernelGGL(k_block_verify, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, bs, in, pidx, rep, bad);
+    check_asm("block_verify");
+}
 
 void asm_quad(hipStream_t s, int nface, int N, int P, int nq, int b, const double* pts, const double* wts,
               const double* lnodes, const double* ldenom, double* pool)

@@ -286,6 +286,7 @@ static MGParams mg_params(const mgldg_mg_params* prm)
     m.relaxation = prm->relaxation; m.omega = prm->omega; m.coarse = prm->coarse;
     m.npre = prm->npre; m.npost = prm->npost; m.restriction = prm->restriction; m.precision = prm->precision;
     m.coarse_dense_max = prm->coarse_dense_max;
+    m.pool_off = prm->pool_off != 0;
     ARG(m.coarse_dense_max >= 0 && m.coarse_dense_max <= 8192, "coarse_dense_max must be in [0, 8192]");
     return m;
 }
@@ -444,6 +445,16 @@ int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols, d
         if (nbrows) *nbrows = op->pat.nbr;
         if (nbcols) *nbcols = op->pat.nbc;
         export_op(mg->ctx, *op, rowptr, cols, blocks, nnzb);
+    });
+}
+
+int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        if (npool) *npool = mg->m.L[level].npool;
+        if (ndpool) *ndpool = mg->m.L[level].ndpool;
     });
 }
 

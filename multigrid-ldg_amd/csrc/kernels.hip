@@ -293,7 +293,11 @@ __device__ __forceinline__ int blk_at(int k0, int m, int dp)
 // first round trip; up to B are loaded at once, one per lane, and broadcast
 // with lane shuffles.  Trip counts are uniform across the wavefront, so every
 // shuffle runs with all lanes active.
-template <typename T, int B, int SKIP>
+// POOL: the operator is a pool of distinct blocks (quadtree operators repeat
+// a few hundred blocks over the whole mesh); colp holds (column, pool index)
+// pairs and a block is read at A + index * B^2, from cache once the pool is
+// resident, so a launch moves its vectors and index tables only.
+template <typename T, int B, int SKIP, bool POOL = false>
 __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const int* __restrict__ colp, int W,
                                      const T* __restrict__ A, const T* x, int r, int lane0, T* xs,
                                      const int* __restrict__ color, int cur)
@@ -303,16 +307,42 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
     T acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0;
-    int mycol = (valid && r < W) ? colp[r] : -1;
+    int mycol = -1, mypid = 0;
+    auto fetch = [&](int m) {
+        if (POOL) {
+            const int2 e = (valid && m < W) ? reinterpret_cast<const int2*>(colp)[m] : make_int2(-1, 0);
+            mycol = e.x; mypid = e.y;
+        } else {
+            mycol = (valid && m < W) ? colp[m] : -1;
+        }
+    };
+    fetch(r);
     const int n = valid ? k1 - k0 - (dp >= 0 ? 1 : 0) : 0;
     int wl = n;
     for (int o = 32; o > 0; o >>= 1) wl = max(wl, __shfl_xor(wl, o, 64));
     for (int m0 = 0; m0 < wl; m0 += B) {
-        if (m0 > 0) mycol = (valid && m0 + r < W) ? colp[m0 + r] : -1;
+        if (m0 > 0) fetch(m0 + r);
         const int nc = min(B, wl - m0);
         for (int u0 = 0; u0 < nc; u0 += U) {
             VecT<T, V> a[U][B / V];
             bool use[U];
+            if (POOL) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int src = lane0 + min(u0 + u, B - 1);
+                    const int j = __shfl(mycol, src, 64);
+                    const int pid = __shfl(mypid, src, 64);
+                    bool ok = valid && u0 + u < nc && j >= 0;
+                    if (SKIP != 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
+                    use[u] = ok;
+                    if (ok) {
+                        const T* Ab = A + (size_t)pid * (B * B) + r * V;
+#pragma unroll
+                        for (int cc = 0; cc < B / V; ++cc) a[u][cc] = ldv<T, V>(Ab + cc * B * V);
+                    }
+                    xs[u * B + r] = ok ? x[(size_t)j * B + r] : (T)0;
+                }
+            } else {
             if (SKIP == 0) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -339,6 +369,7 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
                     }
                 }
                 xs[u * B + r] = ok ? x[(size_t)j * B + r] : (T)0;
+            }
             }
             wave_sync();
 #pragma unroll
@@ -399,7 +430,7 @@ __device__ __forceinline__ void load_dinv_row(const T* __restrict__ Dinv, int i,
 }
 
 // y = A x  or  y = b - A x ; optional deterministic reduction of y.dotv
-template <typename T, int B>
+template <typename T, int B, bool POOL = false>
 __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict__ rowptr, const int* __restrict__ ell,
                                                int W, const T* __restrict__ A, const T* __restrict__ x,
                                                const T* __restrict__ b, T* __restrict__ y, const T* __restrict__ dotv,
@@ -415,8 +446,8 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
     const int k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
     const size_t o = (size_t)row * B + r;
     const T bo = (valid && b) ? b[o] : (T)0;
-    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, ell + (size_t)row * W, W, A, x, r, slot * B,
-                                   xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
+    const T acc = row_sum<T, B, 0, POOL>(valid, k0, k1, -1, ell + (size_t)row * W * (POOL ? 2 : 1), W, A, x, r, slot * B,
+                                         xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
     double dv = 0;
     if (valid) {
         T out = b ? (T)(bo - acc) : acc;
@@ -432,7 +463,9 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
 // blocks) are all addressable from that one load; gcols[idx * W + m] =
 // column of its m-th off-diagonal block (-1 past the last).  The rows must be
 // independent (one colour).
-template <typename T, int B, int SKIP>
+// POOL: desc.w is the pool index of the row's D^-1 block and gcols holds
+// (column, pool index) pairs (row_sum).
+template <typename T, int B, int SKIP, bool POOL = false>
 __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict__ desc, const int* __restrict__ gcols,
                                         int W, const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
                                         T* x, T omega, const int* __restrict__ color, int cur, T* xs_all, T* tb)
@@ -451,10 +484,10 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
     if (valid) {
         bi = b[o];
         if (SKIP == 0) xi = x[o];
-        load_dinv_row<T, B>(Dinv, d.x, r, dv);
+        load_dinv_row<T, B>(Dinv, POOL ? d.w : d.x, r, dv);
     }
-    const T acc = row_sum<T, B, SKIP>(valid, d.y, d.z, d.w, gcols + (size_t)idx * W, W, A, x, r, slot * B,
-                                      xs_all + (wave * R::RW + slot) * R::XS, color, cur);
+    const T acc = row_sum<T, B, SKIP, POOL>(valid, d.y, d.z, POOL ? 0 : d.w, gcols + (size_t)idx * W * (POOL ? 2 : 1), W, A,
+                                            x, r, slot * B, xs_all + (wave * R::RW + slot) * R::XS, color, cur);
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
     wave_sync();
@@ -465,7 +498,7 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
 }
 
 // One colour of a damped block GS sweep (rows desc[0 .. nrows)).
-template <typename T, int B, int SKIP>
+template <typename T, int B, int SKIP, bool POOL = false>
 __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ desc, const int* __restrict__ gcols,
                                             int W, const T* __restrict__ A, const T* __restrict__ Dinv,
                                             const T* __restrict__ b, T* x, T omega,
@@ -475,7 +508,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ 
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
     __shared__ __align__(16) T tb[4 * R::RW * B];
     const int beg = blockIdx.x * Map<T, B>::RPB;
-    gs_rows<T, B, SKIP>(beg, nrows, desc, gcols, W, A, Dinv, b, x, omega, color, cur, xs_all, tb);
+    gs_rows<T, B, SKIP, POOL>(beg, nrows, desc, gcols, W, A, Dinv, b, x, omega, color, cur, xs_all, tb);
 }
 
 // The last two colours of a sweep in one launch.  Workgroup w owns the
@@ -488,7 +521,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ 
 // one-colour launches in sweep order.  Workgroups holding only rows with no
 // neighbour in the last colour have m = e.  cur0 / cur1: colour of the
 // first / second range processed.
-template <typename T, int B, int SKIP>
+template <typename T, int B, int SKIP, bool POOL = false>
 __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int reverse, const int4* __restrict__ desc,
                                                  const int* __restrict__ gcols, int W, const T* __restrict__ A,
                                                  const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
@@ -504,15 +537,16 @@ __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int
         const int beg = lo ? s : m, end = lo ? m : e;
         if (ph == 1 && m != s && m != e) __syncthreads();   // (uniform: s, m, e are per workgroup)
         for (int base = beg; base < end; base += M::RPB)
-            gs_rows<T, B, SKIP>(base, end, desc, gcols, W, A, Dinv, b, x, omega, color, ph ? cur1 : cur0, xs_all, tb);
+            gs_rows<T, B, SKIP, POOL>(base, end, desc, gcols, W, A, Dinv, b, x, omega, color, ph ? cur1 : cur0, xs_all, tb);
     }
 }
 
-template <typename T, int B>
+// POOL: ell holds (column, pool index) pairs, didx the rows' D^-1 pool indices
+template <typename T, int B, bool POOL = false>
 __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict__ rowptr, const int* __restrict__ ell,
                                                 int W, const T* __restrict__ A, const T* __restrict__ Dinv,
-                                                const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ xout,
-                                                T omega)
+                                                const int* __restrict__ didx, const T* __restrict__ b,
+                                                const T* __restrict__ x, T* __restrict__ xout, T omega)
 {
     using M = Map<T, B>;
     using R = RowLds<T, B>;
@@ -529,10 +563,10 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     if (valid) {
         bi = b[o];
         xi = x[o];
-        load_dinv_row<T, B>(Dinv, i, r, dv);
+        load_dinv_row<T, B>(Dinv, POOL ? didx[i] : i, r, dv);
     }
-    const T acc = row_sum<T, B, 0>(valid, k0, k1, -1, ell + (size_t)i * W, W, A, x, r, slot * B,
-                                   xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
+    const T acc = row_sum<T, B, 0, POOL>(valid, k0, k1, -1, ell + (size_t)i * W * (POOL ? 2 : 1), W, A, x, r, slot * B,
+                                         xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
     wave_sync();
@@ -674,15 +708,19 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
 
 template <typename T>
 void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A, const T* x,
-                   const T* b, T* y, const T* dotv, double* dot_out, double* part)
+                   const T* b, T* y, const T* dotv, double* dot_out, double* part, int pool)
 {
     if (nrows <= 0) return;
     unsigned g = 0;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
         g = nblk(nrows, M::RPB);
-        hipLaunchKernelGGL((k_apply<T, BB>), dim3(g), dim3(256), 0, s, nrows, rowptr, ell, W, A, x, b, y,
-                           dotv, dot_out ? part : nullptr);
+        if (pool)
+            hipLaunchKernelGGL((k_apply<T, BB, true>), dim3(g), dim3(256), 0, s, nrows, rowptr, ell, W, A, x, b, y,
+                               dotv, dot_out ? part : nullptr);
+        else
+            hipLaunchKernelGGL((k_apply<T, BB>), dim3(g), dim3(256), 0, s, nrows, rowptr, ell, W, A, x, b, y,
+                               dotv, dot_out ? part : nullptr);
     });
     check_launch("apply");
     if (dot_out) finalize(s, part, (int)g, 1, nullptr, 0, dot_out);
@@ -690,14 +728,21 @@ void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int
 
 template <typename T>
 void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* gcols, int W, const T* A,
-                      const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
+                      const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip, int pool)
 {
     if (nrows <= 0) return;
     const int4* d = reinterpret_cast<const int4*>(desc);
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
         const dim3 g(nblk(nrows, M::RPB));
-        if (skip == 1)
+        if (pool) {
+            if (skip == 1)
+                hipLaunchKernelGGL((k_gs<T, BB, 1, true>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+            else if (skip == 2)
+                hipLaunchKernelGGL((k_gs<T, BB, 2, true>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+            else
+                hipLaunchKernelGGL((k_gs<T, BB, 0, true>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+        } else if (skip == 1)
             hipLaunchKernelGGL((k_gs<T, BB, 1>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
         else if (skip == 2)
             hipLaunchKernelGGL((k_gs<T, BB, 2>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
@@ -709,13 +754,24 @@ void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const in
 
 template <typename T>
 void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* gcols, int W,
-                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0, int cur1, int skip)
+                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0, int cur1, int skip,
+                     int pool)
 {
     if (nwg <= 0) return;
     const int4* d = reinterpret_cast<const int4*>(desc);
     MGLDG_DISPATCH(B, {
         const dim3 g(nwg);
-        if (skip == 1)
+        if (pool) {
+            if (skip == 1)
+                hipLaunchKernelGGL((k_gs_pair<T, BB, 1, true>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
+                                   omega, color, cur0, cur1);
+            else if (skip == 2)
+                hipLaunchKernelGGL((k_gs_pair<T, BB, 2, true>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
+                                   omega, color, cur0, cur1);
+            else
+                hipLaunchKernelGGL((k_gs_pair<T, BB, 0, true>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
+                                   omega, color, cur0, cur1);
+        } else if (skip == 1)
             hipLaunchKernelGGL((k_gs_pair<T, BB, 1>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x, omega, color,
                                cur0, cur1);
         else if (skip == 2)
@@ -737,13 +793,17 @@ int rows_per_workgroup(int B)
 
 template <typename T>
 void Ops<T>::jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
-                    const T* Dinv, const T* b, const T* x, T* xout, T omega)
+                    const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* didx)
 {
     if (nrows <= 0) return;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_jacobi<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, ell, W, A, Dinv, b,
-                           x, xout, omega);
+        if (didx)
+            hipLaunchKernelGGL((k_jacobi<T, BB, true>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, ell, W, A,
+                               Dinv, didx, b, x, xout, omega);
+        else
+            hipLaunchKernelGGL((k_jacobi<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, rowptr, ell, W, A, Dinv,
+                               nullptr, b, x, xout, omega);
     });
     check_launch("jacobi");
 }
@@ -1312,6 +1372,31 @@ void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y)
 }
 template void gemv_f64<float>(hipStream_t, int, const double*, const float*, float*);
 template void gemv_f64<double>(hipStream_t, int, const double*, const double*, double*);
+
+// y = W x with W in FP32 (the collapsed tail of the FP32 V-cycle): one
+// wavefront per row, 16-byte loads of W (n a multiple of 4), FP64 sums
+__global__ __launch_bounds__(256) void k_gemv_w32(int n, const float* __restrict__ W, const float* __restrict__ x,
+                                                  float* __restrict__ y)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float4* w = reinterpret_cast<const float4*>(W + (size_t)row * n);
+    const float4* xv = reinterpret_cast<const float4*>(x);
+    double acc = 0;
+    for (int j = lane; j < n / 4; j += 64) {
+        const float4 a = w[j], c = xv[j];
+        acc += (double)a.x * c.x + (double)a.y * c.y + (double)a.z * c.z + (double)a.w * c.w;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = (float)acc;
+}
+void gemv_w32(hipStream_t s, int n, const float* W, const float* x, float* y)
+{
+    if (n <= 0) return;
+    if (n % 4) throw std::runtime_error("gemv_w32: n must be a multiple of 4");
+    hipLaunchKernelGGL(k_gemv_w32, dim3(nblk(n, 4)), dim3(256), 0, s, n, W, x, y);
+    check_launch("gemv_w32");
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_f64(int n, int k, const double* __restrict__ W, const T* __restrict__ x,

@@ -52,6 +52,11 @@ void asm_quad(hipStream_t s, int nface, int N, int P, int nq, int b, const doubl
 void asm_lift(hipStream_t s, int nbf, int b, int nq, const int* bcode, const double* bscale, const double* lbref, double* Lb);
 // out block o = transpose of in block src[o] (row-major b x b)
 void transpose_blocks(hipStream_t s, long nblk, int b, const int* src, const double* in, double* out);
+// pooling of repeated blocks: a 64-bit hash of each row-major FP64 block's
+// bit pattern, and bad[k] = 1 unless block k equals block rep[pidx[k]]
+// bit for bit
+void block_hash(hipStream_t s, long nblk, int bs, const double* in, unsigned long long* out);
+void block_verify(hipStream_t s, long nblk, int bs, const double* in, const int* pidx, const int* rep, int* bad);
 
 // block rows one 256-thread workgroup of the single-RHS row kernels handles per pass
 int rows_per_workgroup(int B);
@@ -62,26 +67,30 @@ struct Ops {
     // y = A x (b == nullptr) or y = b - A x ; if dot_out: y.dotv is reduced
     // into *dot_out (FP64, deterministic; part holds one partial per workgroup)
     // (ell: the rows' column indices at stride W, -1 past a row's blocks)
+    // pool != 0: A is a pool of distinct blocks and ell holds (column, pool
+    // index) pairs at stride 2W (the other launchers below alike)
     static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
-                      const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
+                      const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part, int pool = 0);
     // damped block Gauss-Seidel on the rows of colour `cur` (multigrid.h:240-249).
     // desc: 4 ints per row of the colour, {row, rowptr[row], rowptr[row+1],
     // position of the diagonal block}; gcols: per row W columns of its
     // off-diagonal blocks (-1 past the last).  skip = 1 (2): first forward
     // (reverse) sweep from x = 0, where the colours after `cur` in sweep order
-    // are still zero and are not read.
+    // are still zero and are not read.  pool != 0: A and Dinv are pools,
+    // gcols holds (column, pool index) pairs and desc.w the D^-1 pool index.
     static void gs_color(hipStream_t s, int B, int nrows, const int* desc, const int* gcols, int W, const T* A,
-                         const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
+                         const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip, int pool = 0);
     // the last two colours of a sweep in one launch of nwg workgroups:
     // workgroup w does desc ranges [s, m) then [m, e) ({s, m, e} = wg[3w..]),
     // the reverse for a backward sweep, with a workgroup barrier between;
     // cur0 / cur1 = colour of the first / second range processed
     static void gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, const int* desc, const int* gcols,
                         int W, const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur0,
-                        int cur1, int skip);
+                        int cur1, int skip, int pool = 0);
     // damped block Jacobi, x_out = x + omega Dinv (b - A x) (multigrid.h:223-229)
+    // (didx != nullptr: A and Dinv are pools, ell pairs, didx the rows' D^-1 pool indices)
     static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
-                       const T* Dinv, const T* b, const T* x, T* xout, T omega);
+                       const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* didx = nullptr);
     // b_c = T^T r (multigrid.h:414-418) as a gather over children; T blocks
     // are BF x BC (fine x coarse basis)
     static void restrict_(hipStream_t s, int BF, int BC, int nc, const int* kid_off, const int* kids, const int* type,
@@ -145,6 +154,8 @@ void gemm_f64(hipStream_t s, int n, int k, const double* W, const T* x, T* y);
 // y = W x with W FP64 n x n row-major (the collapsed coarse V-cycle); x, y of type T
 template <typename T>
 void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y);
+// y = W x, W FP32 row-major n x n (n a multiple of 4), FP64 sums
+void gemv_w32(hipStream_t s, int n, const float* W, const float* x, float* y);
 // identity columns: X[i][c] = (i == c), n x n
 void eye(hipStream_t s, int n, double* X);
 

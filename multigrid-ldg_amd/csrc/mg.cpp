@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 
 #include "kernels.hpp"
 
@@ -98,19 +99,26 @@ void DevOp::build_ell(hipStream_t s)
 // GS row tables of the rows `rows` (in that order): descriptors {row,
 // rowptr[row], rowptr[row+1], diagonal position} and off-diagonal columns at
 // stride w (-1 past the last)
+// pidx / didx (pooled operator): per block slot its A pool index, per row its
+// D^-1 pool index; the table then holds (column, pool index) pairs and
+// desc.w the D^-1 pool index
 static void gs_tables(const HostBSR& P, const std::vector<int>& rows, int w, hipStream_t s, DBuf<int>& desc_d,
-                      DBuf<int>& gcols_d)
+                      DBuf<int>& gcols_d, const std::vector<int>* pidx = nullptr, const std::vector<int>* didx = nullptr)
 {
     const size_t nr = rows.size();
-    std::vector<int> desc(4 * nr), gc(nr * w, -1);
+    const int ew = pidx ? 2 : 1;
+    std::vector<int> desc(4 * nr), gc(nr * w * ew, -1);
+    if (pidx) for (size_t t = 1; t < gc.size(); t += 2) gc[t] = 0;
     for (size_t q = 0; q < nr; ++q) {
         const int i = rows[q];
         int dp = -1, m = 0;
         for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
             if (P.cols[k] == i) { if (dp < 0) { dp = k; continue; } }
-            gc[q * w + m++] = P.cols[k];
+            if (pidx) { gc[(q * w + m) * 2] = P.cols[k]; gc[(q * w + m) * 2 + 1] = (*pidx)[k]; ++m; }
+            else gc[q * w + m++] = P.cols[k];
         }
-        desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1]; desc[4 * q + 3] = dp;
+        desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1];
+        desc[4 * q + 3] = didx ? (*didx)[i] : dp;
     }
     desc_d.upload(desc.data(), desc.size(), s);
     gcols_d.upload(gc.data(), gc.size(), s);
@@ -521,7 +529,7 @@ static void coarsen(hipStream_t s, int N, Level& f, Level& c, int restriction)
 // neighbour need no ordering and fill plain workgroups.  Used only when every
 // cluster fits one pass of a workgroup per range; otherwise the level keeps
 // one launch per colour.
-static void pair_schedule(Level& lv, hipStream_t s)
+static void pair_schedule(Level& lv, hipStream_t s, const std::vector<int>* pidx, const std::vector<int>* didx)
 {
     lv.pair_nwg = 0;
     if (lv.ncolor < 3) return;
@@ -567,13 +575,55 @@ static void pair_schedule(Level& lv, hipStream_t s)
         order.insert(order.end(), rb[c].begin(), rb[c].end());
         wg.insert(wg.end(), {a, m, (int)order.size()});
     }
-    gs_tables(P, order, lv.gw, s, lv.pair_desc, lv.pair_gcols);
+    gs_tables(P, order, lv.gw, s, lv.pair_desc, lv.pair_gcols, pidx, didx);
     lv.pair_wg.upload(wg.data(), wg.size(), s);
     HIPCHK(hipStreamSynchronize(s));
     lv.pair_nwg = (int)(wg.size() / 3);
 }
 
-static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicographic)
+// Distinct blocks of a device array of nblk row-major FP64 blocks (bs
+// doubles each): pidx[k] = pool index of block k, rep[p] = first block of
+// pool entry p.  Blocks are grouped by a 64-bit hash of their bits and every
+// block is then compared with its entry bit for bit on the device, so the
+// pool is exact.  false (no pool) when there are more than nblk / 4 distinct
+// blocks -- the pool would not cut the traffic -- or a comparison fails.
+static bool pool_blocks(hipStream_t s, long nblk, int bs, const double* blocks, std::vector<int>& pidx,
+                        std::vector<int>& rep)
+{
+    pidx.clear(); rep.clear();
+    if (nblk < 64) return false;
+    DBuf<unsigned long long> hd;
+    hd.alloc(nblk);
+    dev::block_hash(s, nblk, bs, blocks, hd);
+    std::vector<unsigned long long> h(nblk);
+    hd.download(h.data(), h.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+    std::unordered_map<unsigned long long, int> first;
+    first.reserve(1024);
+    pidx.resize(nblk);
+    for (long k = 0; k < nblk; ++k) {
+        auto it = first.find(h[k]);
+        if (it == first.end()) {
+            if ((long)rep.size() >= nblk / 4) { pidx.clear(); rep.clear(); return false; }
+            first.emplace(h[k], (int)rep.size());
+            pidx[k] = (int)rep.size();
+            rep.push_back((int)k);
+        } else {
+            pidx[k] = it->second;
+        }
+    }
+    DBuf<int> pd, rd, bad;
+    pd.upload(pidx.data(), pidx.size(), s); rd.upload(rep.data(), rep.size(), s);
+    bad.alloc(nblk);
+    dev::block_verify(s, nblk, bs, blocks, pd, rd, bad);
+    std::vector<int> hb(nblk);
+    bad.download(hb.data(), hb.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+    for (int v : hb) if (v) { pidx.clear(); rep.clear(); return false; }
+    return true;
+}
+
+static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicographic, bool pool_off)
 {
     const int b = lv.B;
     const long nnzb = lv.A.nnzb();
@@ -592,6 +642,45 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         dev::to_panel_f32(s, lv.nb, b, lv.Dinv_rm, lv.D32);
     }
     lv.A.build_ell(s);
+    // pooled operator for the single-RHS kernels when A repeats its blocks
+    std::vector<int> pidx, prep, didx, drep;
+    const bool pooled = !pool_off && pool_blocks(s, nnzb, b * b, lv.A.val, pidx, prep);
+    lv.npool = lv.ndpool = 0;
+    if (pooled) {
+        if (!pool_blocks(s, lv.nb, b * b, lv.Dinv_rm, didx, drep)) {
+            didx.resize(lv.nb); drep.resize(lv.nb);
+            for (int i = 0; i < lv.nb; ++i) didx[i] = drep[i] = i;
+        }
+        auto make_pool = [&](const std::vector<int>& rep, const double* src, DBuf<double>& p64, DBuf<float>& p32) {
+            const long np = (long)rep.size();
+            DBuf<int> rd; rd.upload(rep.data(), rep.size(), s);
+            DBuf<double> rm; rm.alloc((size_t)np * b * b);
+            dev::gather_blocks(s, (int)np, b * b, rd, src, rm);
+            p64.alloc((size_t)np * b * b);
+            dev::to_panel_f64(s, np, b, rm, p64);
+            if (precision == 1) { p32.alloc((size_t)np * b * b); dev::to_panel_f32(s, np, b, rm, p32); }
+            HIPCHK(hipStreamSynchronize(s));
+        };
+        make_pool(prep, lv.A.val, lv.P64, lv.P32);
+        make_pool(drep, lv.Dinv_rm, lv.DP64, lv.DP32);
+        const int W = lv.A.ellw;
+        std::vector<int> pe((size_t)lv.nb * W * 2, 0);
+        for (int i = 0; i < lv.nb; ++i) {
+            int m = 0;
+            for (int k = lv.A.pat.rowptr[i]; k < lv.A.pat.rowptr[i + 1]; ++k, ++m) {
+                pe[((size_t)i * W + m) * 2] = lv.A.pat.cols[k];
+                pe[((size_t)i * W + m) * 2 + 1] = pidx[k];
+            }
+            for (; m < W; ++m) pe[((size_t)i * W + m) * 2] = -1;
+        }
+        lv.pell.upload(pe.data(), pe.size(), s);
+        lv.pdidx.upload(didx.data(), didx.size(), s);
+        HIPCHK(hipStreamSynchronize(s));
+        lv.npool = (int)prep.size(); lv.ndpool = (int)drep.size();
+        lv.hpidx = pidx; lv.hdidx = didx;
+    }
+    const std::vector<int>* pp = pooled ? &pidx : nullptr;
+    const std::vector<int>* dp = pooled ? &didx : nullptr;
     lv.gw = 1;   // most off-diagonal blocks in a row
     for (int i = 0; i < lv.nb; ++i) {
         int m = 0;
@@ -607,7 +696,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         for (int i = 0; i < lv.nb; ++i) rows[pos[color[i]]++] = i;
         rows_d.upload(rows.data(), rows.size(), s);
         col_d.upload(color.data(), color.size(), s);
-        gs_tables(lv.A.pat, rows, lv.gw, s, desc_d, gcols_d);
+        gs_tables(lv.A.pat, rows, lv.gw, s, desc_d, gcols_d, pp, dp);
     };
     if (lexicographic) {
         // wavefronts of the lexicographic order: a row waits for its earlier
@@ -620,7 +709,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         lv.ncolor = greedy_colors(lv.A.pat, lv.color);
     }
     group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev, lv.color_desc, lv.color_gcols);
-    if (!lexicographic) pair_schedule(lv, s);
+    if (!lexicographic) pair_schedule(lv, s, pp, dp);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
     if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
@@ -742,7 +831,7 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
         coarsen(s, N, L[l], L[l + 1], prm.restriction);
         if (sx) pad_dropped(s, L[l + 1], crank[l]);
     }
-    for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision, prm.relaxation == 2);
+    for (int l = 0; l < nl; ++l) finalize_level(s, L[l], prm.precision, prm.relaxation == 2, prm.pool_off != 0);
     // coarsest level: dense inverse of the assembled operator (FP64)
     Level& c = L[nl - 1];
     if (prm.coarse == 0) {
@@ -789,6 +878,19 @@ template <> const float* Aop<float>(const Level& lv) { return lv.A32.p; }
 template <typename T> static const T* Dop(const Level& lv);
 template <> const double* Dop<double>(const Level& lv) { return lv.D64.p; }
 template <> const float* Dop<float>(const Level& lv) { return lv.D32.p; }
+// the operator as the single-RHS kernels read it: the block pool when the
+// level is pooled (with the pair tables and D^-1 indices), else the full
+// panel arrays
+template <typename T> static const T* Sop(const Level& lv);
+template <> const double* Sop<double>(const Level& lv) { return lv.npool ? lv.P64.p : lv.A64.p; }
+template <> const float* Sop<float>(const Level& lv) { return lv.npool ? lv.P32.p : lv.A32.p; }
+template <typename T> static const T* SDop(const Level& lv);
+template <> const double* SDop<double>(const Level& lv) { return lv.npool ? lv.DP64.p : lv.D64.p; }
+template <> const float* SDop<float>(const Level& lv) { return lv.npool ? lv.DP32.p : lv.D32.p; }
+static const int* Sell(const Level& lv) { return lv.npool ? lv.pell.p : lv.A.ell.p; }
+static const int* Sdidx(const Level& lv) { return lv.npool ? lv.pdidx.p : nullptr; }
+static int Spool(const Level& lv) { return lv.npool ? 1 : 0; }
+static size_t Sgw(const Level& lv) { return (size_t)lv.gw * (lv.npool ? 2 : 1); }   // ints per row of a GS column table
 template <typename T> static const T* Pool(const Level& lv);
 template <> const double* Pool<double>(const Level& lv) { return lv.pool64.p; }
 template <> const float* Pool<float>(const Level& lv) { return lv.pool32.p; }
@@ -812,7 +914,7 @@ void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) con
     for (int cc = 0; cc < nc; ++cc) {
         const int c = desc ? nc - 1 - cc : cc;
         const int skip = first_from_zero ? (desc ? 2 : 1) : 0;
-        fn(rows + off[c], dsc + 4 * (size_t)off[c], gcl + (size_t)lv.gw * off[c], off[c + 1] - off[c], cdev, c, skip);
+        fn(rows + off[c], dsc + 4 * (size_t)off[c], gcl + Sgw(lv) * off[c], off[c + 1] - off[c], cdev, c, skip);
     }
 }
 
@@ -834,13 +936,13 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
             auto one = [&](int c) {
                 const int o = lv.color_off[c];
                 dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o,
-                                      lv.color_gcols.p + (size_t)lv.gw * o, lv.gw, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega,
-                                      lv.color_dev, c, skip);
+                                      lv.color_gcols.p + Sgw(lv) * o, lv.gw, Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega,
+                                      lv.color_dev, c, skip, Spool(lv));
             };
             auto pair = [&] {
                 dev::Ops<T>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, reverse ? 1 : 0, lv.pair_desc, lv.pair_gcols, lv.gw,
-                                     Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, lv.color_dev, reverse ? nc - 1 : nc - 2,
-                                     reverse ? nc - 2 : nc - 1, skip);
+                                     Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega, lv.color_dev, reverse ? nc - 1 : nc - 2,
+                                     reverse ? nc - 2 : nc - 1, skip, Spool(lv));
             };
             if (!reverse) { for (int c = 0; c < nc - 2; ++c) one(c); pair(); }
             else { pair(); for (int c = nc - 3; c >= 0; --c) one(c); }
@@ -849,15 +951,16 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
         for (int sw = 0; sw < n; ++sw)
             sweep(lv, reverse, zero_guess && sw == 0,
                   [&](const int*, const int* dsc, const int* gcl, int nr, const int* cdev, int c, int skip) {
-                      dev::Ops<T>::gs_color(s, lv.B, nr, dsc, gcl, lv.gw, Aop<T>(lv), Dop<T>(lv), v.b, v.x, omega, cdev, c,
-                                            skip);
+                      dev::Ops<T>::gs_color(s, lv.B, nr, dsc, gcl, lv.gw, Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega, cdev, c,
+                                            skip, Spool(lv));
                   });
     } else {
         // Jacobi ping-pongs between x and xalt; the result ends in x
         if (zero_guess) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
         T* cur = v.x.p; T* alt = v.xalt.p;
         for (int sw = 0; sw < n; ++sw) {
-            dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, Aop<T>(lv), Dop<T>(lv), v.b, cur, alt, omega);
+            dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), SDop<T>(lv), v.b, cur, alt, omega,
+                                Sdidx(lv));
             std::swap(cur, alt);
         }
         if (cur != v.x.p) dev::Ops<T>::copy(s, (long)lv.nb * lv.B, cur, v.x);
@@ -876,7 +979,8 @@ void MGH::vcycle_from(int l0, bool x0_zero)
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
         relax<T>(l, prm.npre, false, zero);
-        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, Aop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr);
+        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr,
+                           Spool(lv));
         LevelVecs<T>& vc = vecs<T>(l + 1);
         dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): implicit in the first sweep of a
@@ -888,8 +992,10 @@ void MGH::vcycle_from(int l0, bool x0_zero)
     Level& c = L[bottom];
     LevelVecs<T>& vc = vecs<T>(bottom);
     if (bottom == cut) {
-        // the V-cycle below `cut` as one precomputed linear map (FP64)
-        dev::gemv_f64<T>(s, c.nb * c.B, W, vc.b, vc.x);
+        // the V-cycle below `cut` as one precomputed linear map (FP64; the
+        // FP32 V-cycle reads its FP32 copy, half the bytes)
+        if (sizeof(T) == 4 && W32.p) dev::gemv_w32(s, c.nb * c.B, W32.p, (const float*)vc.b.p, (float*)vc.x.p);
+        else dev::gemv_f64<T>(s, c.nb * c.B, W, vc.b, vc.x);
     } else if (prm.coarse == 0) {
         dev::Ops<T>::coarse(s, c.nb * c.B, c.Cinv, vc.b, vc.x, submean);
     } else {
@@ -975,6 +1081,10 @@ void MGH::build_collapse()
     }
     HIPCHK(hipStreamSynchronize(s));
     W = std::move(X[cut]);
+    if (prm.precision == 1 && k % 4 == 0) {
+        W32.alloc((size_t)k * k);
+        dev::f64_to_f32(s, (long)k * k, W, W32);
+    }
     Wt.alloc((size_t)dev::gemm_tiles_size(k));
     dev::gemm_tiles(s, k, W, Wt);
     HIPCHK(hipStreamSynchronize(s));
@@ -1025,16 +1135,28 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     if (mixed && !f.A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
     if (mixed && sp.flexible && !zold32.p) zold32.alloc(n);
     SolveStats st;
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, es;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreateWithFlags(&es, hipEventDisableTiming));
     HIPCHK(hipEventRecord(e0, s));
-    auto fetchS = [&]() {
+    // the scalars go to pinned host memory behind an event; the host waits
+    // for that copy only, so work enqueued after it keeps the GPU busy
+    auto copyS = [&]() {
         HIPCHK(hipMemcpyAsync(S_host, S.p, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipEventRecord(es, s));
     };
+    auto waitS = [&]() { HIPCHK(hipEventSynchronize(es)); };
+    auto fetchS = [&]() { copyS(); waitS(); };
     auto residual = [&](bool with_rr) {   // r = b - A x  (optionally r.r into S[RR])
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.ell, f.A.ellw, f.A64, x, b, r, with_rr ? r.p : nullptr,
-                                with_rr ? S.p + dev::S_RR : nullptr, part);
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, Sell(f), f.A.ellw, Sop<double>(f), x, b, r, with_rr ? r.p : nullptr,
+                                with_rr ? S.p + dev::S_RR : nullptr, part, Spool(f));
+    };
+    // q = A d, S[DQ] = d.q: the first kernel of an iteration.  It writes only
+    // q and S[DQ], so it is enqueued before the host has tested the previous
+    // iteration's scalars and is simply not used when the loop ends there.
+    auto product = [&]() {
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, Sell(f), f.A.ellw, Sop<double>(f), d, nullptr, q, d, S.p + dev::S_DQ, part,
+                                Spool(f));
     };
     // the preconditioner reads r in its own precision from L[0].v*.b and
     // leaves z in L[0].v*.x; the CG kernels convert on the fly
@@ -1055,7 +1177,9 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     precon(sp.precision, sp.use_graph != 0);
     rz(false);
     dupd(true);
-    fetchS();
+    copyS();
+    product();
+    waitS();
     const double bnorm = std::sqrt(S_host[dev::S_BB]);
     const double delta0 = S_host[dev::S_DELTA];
     double delta = delta0, rr = S_host[dev::S_RR];
@@ -1072,15 +1196,16 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
                 precon(sp.precision, sp.use_graph != 0);
                 rz(false);
                 dupd(true);
-                fetchS();
+                copyS();
+                product();                 // the product of the restarted direction
+                waitS();
                 delta = S_host[dev::S_DELTA]; rr = S_host[dev::S_RR];
                 go = true;
             }
         }
         if (!go) { st.status = 0; break; }
         if (i >= sp.maxit) { st.status = 1; break; }
-        // q = A d, S[DQ] = d.q
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.ell, f.A.ellw, f.A64, d, nullptr, q, d, S.p + dev::S_DQ, part);
+        // q = A d and S[DQ] are in flight (product() above)
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
             dev::pcg_x(s, n, S, d, x);
             residual(true);
@@ -1094,7 +1219,9 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
         rz(sp.flexible != 0);
         dupd(false);
         ++i;
-        fetchS();
+        copyS();
+        product();
+        waitS();
         if (!(S_host[dev::S_DQ] > 0)) { st.status = std::isfinite(S_host[dev::S_DQ]) ? 2 : 3; break; }
         delta = S_host[dev::S_DELTA]; rr = S_host[dev::S_RR];
     }
@@ -1102,9 +1229,10 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     residual(true);
     HIPCHK(hipEventRecord(e1, s));
     fetchS();
+    HIPCHK(hipStreamSynchronize(s));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(es);
     st.iterations = i;
     st.relres_true = bnorm > 0 ? std::sqrt(S_host[dev::S_RR]) / bnorm : 0.0;
     st.relres_prec = delta0 > 0 ? std::sqrt(std::fabs(delta / delta0)) : 0.0;
@@ -1454,8 +1582,8 @@ void MGH::spmv(int l, int precision, const void* x, void* y)
 {
     Level& lv = L[l];
     hipStream_t s = ctx->stream;
-    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A32, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr);
-    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A64, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr);
+    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<float>(lv), (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr, Spool(lv));
+    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<double>(lv), (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr, Spool(lv));
 }
 
 void MGH::smooth(int l, int precision, int n, bool reverse, const void* b, void* x)
@@ -1493,7 +1621,8 @@ void MGH::vcycles(int precision, const double* b, double* x, int ncycle, double 
         HIPCHK(hipStreamSynchronize(s));
     };
     auto resid = [&]() -> double {   // r = b - A x, ||r||
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, f.A.ell, f.A.ellw, f.A64, x, b, r, r.p, S.p + dev::S_RR, part);
+        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, Sell(f), f.A.ellw, Sop<double>(f), x, b, r, r.p, S.p + dev::S_RR, part,
+                                Spool(f));
         fetch();
         return std::sqrt(S_host[dev::S_RR]);
     };
@@ -1571,20 +1700,28 @@ void MGH::vcycle_matrix(int precision, double* V)
 // zero): off-diagonal blocks and column indices of R, its D^-1 blocks and
 // descriptors, b, x (read) and x (write) of R, and x of the distinct
 // off-diagonal neighbours of R (each read once).
+// Pooled level: the (column, pool index) pair of every off-diagonal block,
+// the distinct A and D^-1 pool blocks the launch touches (each once), the
+// descriptors, b, x (read) and x (write) of R, and x of its neighbours.
 static double gs_launch_bytes(const Level& lv, const int* rows, int nr, int sa, std::vector<int>& mark, int stamp)
 {
     const HostBSR& P = lv.A.pat;
     const double b2 = (double)lv.B * lv.B;
-    double offd = 0, nbrs = 0;
+    double offd = 0, nbrs = 0, apool = 0, dpool = 0;
+    std::vector<char> seenA(lv.npool, 0), seenD(lv.ndpool, 0);
     for (int q = 0; q < nr; ++q) {
         const int i = rows[q];
+        if (lv.npool && !seenD[lv.hdidx[i]]) { seenD[lv.hdidx[i]] = 1; dpool += 1; }
         for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
             const int j = P.cols[k];
             if (j == i) continue;
             offd += 1;
+            if (lv.npool && !seenA[lv.hpidx[k]]) { seenA[lv.hpidx[k]] = 1; apool += 1; }
             if (mark[j] != stamp) { mark[j] = stamp; nbrs += 1; }
         }
     }
+    if (lv.npool)
+        return offd * 8.0 + (apool + dpool) * b2 * sa + nr * (16.0 + 3.0 * lv.B * sa) + nbrs * lv.B * sa;
     return offd * (b2 * sa + 4.0) + nr * (b2 * sa + 16.0 + 3.0 * lv.B * sa) + nbrs * lv.B * sa;
 }
 
@@ -1605,7 +1742,9 @@ void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* byte
     double by = 0;
     if (!gs) {
         per = 1;
-        by = (double)lv.A.nnzb() * lv.B * lv.B * sa + (double)lv.nb * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() +
+        by = lv.npool
+           ? (double)(lv.npool + lv.ndpool) * lv.B * lv.B * sa + 8.0 * lv.A.nnzb() + 4.0 * lv.nb + 4.0 * (lv.nb + 1) + 3.0 * n * sa
+           : (double)lv.A.nnzb() * lv.B * lv.B * sa + (double)lv.nb * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() +
              4.0 * (lv.nb + 1) + 3.0 * n * sa;
     } else {
         const int nc = lv.ncolor, nsingle = paired ? nc - 2 : nc;
@@ -1633,20 +1772,22 @@ void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* byte
                 for (int c = 0; c < nc - 2; ++c) {
                     const int o = lv.color_off[c], nr = lv.color_off[c + 1] - o;
                     const int* d = lv.color_desc.p + 4 * (size_t)o;
-                    const int* g = lv.color_gcols.p + (size_t)lv.gw * o;
+                    const int* g = lv.color_gcols.p + Sgw(lv) * o;
                     if (precision == 1)
-                        dev::Ops<float>::gs_color(s, lv.B, nr, d, g, lv.gw, lv.A32, lv.D32, lv.v32.b, lv.v32.x, (float)prm.omega,
-                                                  lv.color_dev, c, 0);
+                        dev::Ops<float>::gs_color(s, lv.B, nr, d, g, lv.gw, Sop<float>(lv), SDop<float>(lv), lv.v32.b, lv.v32.x,
+                                                  (float)prm.omega, lv.color_dev, c, 0, Spool(lv));
                     else
-                        dev::Ops<double>::gs_color(s, lv.B, nr, d, g, lv.gw, lv.A64, lv.D64, lv.v64.b, lv.v64.x, prm.omega,
-                                                   lv.color_dev, c, 0);
+                        dev::Ops<double>::gs_color(s, lv.B, nr, d, g, lv.gw, Sop<double>(lv), SDop<double>(lv), lv.v64.b, lv.v64.x,
+                                                   prm.omega, lv.color_dev, c, 0, Spool(lv));
                 }
             else if (precision == 1)
-                dev::Ops<float>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, lv.A32, lv.D32, lv.v32.b,
-                                         lv.v32.x, (float)prm.omega, lv.color_dev, nc - 2, nc - 1, 0);
+                dev::Ops<float>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, Sop<float>(lv),
+                                         SDop<float>(lv), lv.v32.b, lv.v32.x, (float)prm.omega, lv.color_dev, nc - 2, nc - 1, 0,
+                                         Spool(lv));
             else
-                dev::Ops<double>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, lv.A64, lv.D64, lv.v64.b,
-                                          lv.v64.x, prm.omega, lv.color_dev, nc - 2, nc - 1, 0);
+                dev::Ops<double>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, Sop<double>(lv),
+                                          SDop<double>(lv), lv.v64.b, lv.v64.x, prm.omega, lv.color_dev, nc - 2, nc - 1, 0,
+                                          Spool(lv));
         }
     };
     hipEvent_t e0, e1;
@@ -1672,8 +1813,8 @@ void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes
     const int sa = precision == 1 ? 4 : 8;
     if (precision == 1) dev::Ops<float>::fill(s, n, 1.f, lv.v32.x); else dev::Ops<double>::fill(s, n, 1.0, lv.v64.x);
     auto once = [&] {
-        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A32, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr);
-        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, lv.A.ell, lv.A.ellw, lv.A64, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr);
+        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<float>(lv), lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr, Spool(lv));
+        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<double>(lv), lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr, Spool(lv));
     };
     once();
     hipEvent_t e0, e1;
@@ -1686,7 +1827,8 @@ void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     *ms = t / nrep;
-    *bytes = (double)lv.A.nnzb() * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 2.0 * n * sa;
+    *bytes = lv.npool ? (double)lv.npool * lv.B * lv.B * sa + 8.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 2.0 * n * sa
+                      : (double)lv.A.nnzb() * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 2.0 * n * sa;
 }
 
 }  // namespace mgldg

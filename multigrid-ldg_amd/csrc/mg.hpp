@@ -98,6 +98,7 @@ struct MGParams {
     int restriction = 1;
     int precision = 0;   // 0: FP64 only, 1: also FP32 copies for the mixed V-cycle
     int coarse_dense_max = 0;   // collapse the V-cycle below the first level with at most this many dofs (0: off)
+    int pool_off = 0;           // 1: never pool repeated blocks for the single-RHS kernels
 };
 
 template <typename T>
@@ -130,6 +131,16 @@ struct Level {
     // per workgroup {s, m, e} into pair_desc / pair_gcols, see dev::Ops::gs_pair
     int pair_nwg = 0;
     DBuf<int> pair_wg, pair_desc, pair_gcols;
+    // pooled operator of the single-RHS kernels (npool > 0): the distinct
+    // blocks of A and of D^-1 (quadtree operators repeat a few hundred
+    // blocks), panel layouts; pell: A.ell as (column, pool index) pairs at
+    // stride 2 * A.ellw; pdidx: each row's D^-1 pool index.  The GS tables
+    // above then hold pairs and desc.w = D^-1 pool index (Ops::gs_color).
+    int npool = 0, ndpool = 0;
+    DBuf<double> P64, DP64;
+    DBuf<float> P32, DP32;
+    DBuf<int> pell, pdidx;
+    std::vector<int> hpidx, hdidx;      // host copies of the pool indices (traffic accounting)
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     // interpolation to the next coarser level
@@ -169,6 +180,7 @@ struct MGH {
     int cut = -1;                      // first collapsed level (-1: none)
     int submean = 1;                   // coarsest b -= mean(b) (the quadtree Multigrid, multigrid.h:365-368)
     DBuf<double> W;                    // collapsed V-cycle below `cut`, FP64 dense n x n
+    DBuf<float> W32;                   // the same in FP32 for the FP32 V-cycle (mixed hierarchies, n % 4 == 0)
     DBuf<double> Wt;                   // the same in the MFMA tile layout (batched form)
 
     // batched (k right-hand sides) state

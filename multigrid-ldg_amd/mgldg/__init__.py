@@ -290,16 +290,20 @@ class Multigrid(_Handle):
     pdegrees (quadtree problems): p-levels of these polynomial degrees below
     the problem's, Multigrid<N,P1,P2,...> (multigrid.h:435-639), then the
     h-levels at the last degree, or none if hcoarsen is False
-    (StopCoarsening)."""
+    (StopCoarsening).
+
+    pool (default True): levels whose operator repeats its blocks (quadtree
+    levels) are read by the single-RHS kernels from a pool of the distinct
+    blocks -- the same bits, a fraction of the HBM traffic."""
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
-                 restriction="ratrat", precision="fp64", coarse_dense_max=4096, partitions=None, pdegrees=None,
-                 hcoarsen=True):
+                 restriction="ratrat", precision="fp64", coarse_dense_max=8192, partitions=None, pdegrees=None,
+                 hcoarsen=True, pool=True):
         self.prob, self.ctx = prob, prob.ctx
         self.params = MGParams({"jacobi": 0, "gs": 1, "gs-lex": 2}[relaxation], omega,
                                {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
                                {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision],
-                               coarse_dense_max)
+                               coarse_dense_max, 0 if pool else 1)
         self.h = C.c_void_p()
         if partitions is not None:
             parts = np.zeros(len(partitions) * prob.ne, dtype=np.int32)
@@ -347,6 +351,12 @@ class Multigrid(_Handle):
         V = np.zeros((n, n))
         check(self.ctx, lib().mgldg_mg_vcycle_matrix(self.h, _prec(precision), _pd(V)))
         return V
+
+    def level_pool(self, level):
+        """(distinct A blocks, distinct D^-1 blocks) of a pooled level, (0, 0) otherwise"""
+        a, d = C.c_int(), C.c_int()
+        check(self.ctx, lib().mgldg_mg_level_pool(self.h, level, C.byref(a), C.byref(d)))
+        return a.value, d.value
 
     def colors(self, level):
         nc = C.c_int()

@@ -9,7 +9,8 @@ SIZEFN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_voi
 
 class MGParams(C.Structure):
     _fields_ = [("relaxation", C.c_int), ("omega", C.c_double), ("coarse", C.c_int), ("npre", C.c_int),
-                ("npost", C.c_int), ("restriction", C.c_int), ("precision", C.c_int), ("coarse_dense_max", C.c_int)]
+                ("npost", C.c_int), ("restriction", C.c_int), ("precision", C.c_int), ("coarse_dense_max", C.c_int),
+                ("pool_off", C.c_int)]
 
 
 class SolveParams(C.Structure):
@@ -62,6 +63,7 @@ SIGNATURES = {
     "mgldg_mg_create_agglomerated": (I, [V, C.POINTER(MGParams), I, PI, C.POINTER(V)]),
     "mgldg_mg_export": (I, [V, I, I, PI, PI, PD, PL, PI, PI]),
     "mgldg_mg_colors": (I, [V, I, PI, PI]),
+    "mgldg_mg_level_pool": (I, [V, I, PI, PI]),
     "mgldg_mg_destroy": (I, [V]),
     "mgldg_solve_device": (I, [V, V, V, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
     "mgldg_solve": (I, [V, PD, PD, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),

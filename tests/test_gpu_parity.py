@@ -245,16 +245,21 @@ def test_collapsed_vcycle_equals_levelwise(gctx, oracle, coarse):
     import torch
     qt, pr, oqt, opr = build_pair(gctx, oracle, "adaptive", 2, rapid=True)
     b = torch.tensor(np.random.default_rng(2).standard_normal(pr.ndof), device="cuda")
-    outs = []
-    for cdm in (0, 64, 4096):
-        mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, coarse=coarse, coarse_dense_max=cdm)
+    outs, outs32 = [], []
+    for cdm in (0, 64, 4096, 8192):
+        mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, coarse=coarse, coarse_dense_max=cdm, precision="mixed")
         z = torch.zeros_like(b)
         mg.vcycle(b, z)
         outs.append(z.cpu().numpy())
+        z32 = torch.zeros_like(b)
+        mg.vcycle(b, z32, "mixed")          # FP32 V-cycle (FP32 copy of the collapsed tail)
+        outs32.append(z32.cpu().numpy())
     omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2, coarse=coarse)
     zo = omg.vcycle(b.cpu().numpy())
     for z in outs:
         assert rel(z, zo) < 1e-12
+    for z in outs32:
+        assert rel(z, zo) < 1e-5
 
 
 def test_solve_statuses(gctx, oracle):
