@@ -538,13 +538,30 @@ def main():
                            "GBs": round(gbs(ms, by), 1)}
         log(f"GS one-colour launches fp32: {ms32 * 1e3:.1f} us/launch, {by32 / 1e6:.2f} MB -> {gs32:.0f} GB/s; "
             f"fp64: {ms64 * 1e3:.1f} us, {gs64:.0f} GB/s; SpMV fp64 {mss * 1e3:.1f} us {spmv64:.0f} GB/s; sweeps {sweep}")
-        traffic = None
+        # the same launch on the unpooled operator (every block streamed from
+        # HBM): the kernel's streaming efficiency, beside the pooled launch
+        # the solve uses (the quadtree operator's distinct blocks from cache)
+        mg_np = mgldg.Multigrid(prob, relaxation="gs", omega=0.8, npre=2, npost=2, restriction="ratrat",
+                                precision="mixed", pool=False)
+        msu, byu, _ = mg_np.profile_relax(0, "mixed", nrep, part="single")
+        del mg_np
+        gsu = gbs(msu, byu)
+        log(f"unpooled GS one-colour launch fp32: {msu * 1e3:.1f} us/launch, {byu / 1e6:.2f} MB -> {gsu:.0f} GB/s")
+        traffic = traffic_u = None
         tp = os.path.join(ROOT, "profiles", "k_gs_c2_fine_pmc_traffic.json")
         if os.path.exists(tp):   # measured offline: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated
-            traffic = json.load(open(tp))["kernels"].get("void mgldg::dev::k_gs<float, 16, 0>", {}).get("traffic_bytes")
-        roofline = {"bound": "hbm", "kernel": "k_gs<float,16,0> (fine-level one-colour block GS launch)",
+            kk = json.load(open(tp))["kernels"]
+            traffic = kk.get("void mgldg::dev::k_gs<float, 16, 0, true>", {}).get("traffic_bytes")
+            traffic_u = kk.get("void mgldg::dev::k_gs<float, 16, 0, false>", {}).get("traffic_bytes")
+        roofline = {"bound": "hbm", "kernel": "k_gs<float,16,0,true> (fine-level one-colour block GS launch, pooled operator)",
                     "achieved": round(gs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gs32 / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "bytes_per_launch": by32, "us_per_launch": round(ms32 * 1e3, 3),
+                    "limiter": "dependent-load latency: descriptor -> (column, pool) pairs -> x gathers and pool blocks "
+                               "from cache -> store; the pool removed 94% of the launch's HBM bytes",
+                    "unpooled": {"kernel": "k_gs<float,16,0,false>", "us_per_launch": round(msu * 1e3, 3),
+                                 "bytes_per_launch": byu, "achieved": round(gsu, 1), "frac": round(gsu / HBM_PEAK_GBS, 4),
+                                 "traffic": traffic_u},
+                    "speedup_vs_unpooled": round(msu / ms32, 2),
                     "fp64_gs_GBs": round(gs64, 1), "fp64_spmv_GBs": round(spmv64, 1), "sweep": sweep}
     if args.roofline_only:
         # calibration for the PMC traffic counters: a plain device copy of known size
