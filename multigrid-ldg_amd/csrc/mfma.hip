@@ -104,15 +104,18 @@ __device__ __forceinline__ void blk_mac(typename Mf<T>::v4* acc, const BlkOps<T,
 // one per lane, and broadcast with shuffles; the operands of U blocks are in
 // flight together.  SKIP: -1 = every block; 0 = not the diagonal block;
 // 1 (2) = also not the colours after (before) `cur` (first GS sweep from x = 0).
+// pidx (pooled operator): per block slot its index in the pool A; null: A
+// holds every block.
 template <typename T, int B, int NT, int SKIP>
 __device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, int k1, const int* __restrict__ cols,
                                         const T* __restrict__ A, const T* __restrict__ x, int k, int col0, int lane,
-                                        const int* __restrict__ color, int cur)
+                                        const int* __restrict__ color, int cur, const int* __restrict__ pidx)
 {
     constexpr int U = mf_unroll<T, B, NT>();
     const int deg = k1 - k0;
     for (int m0 = 0; m0 < deg; m0 += 64) {
         const int mycol = (m0 + lane < deg) ? cols[k0 + m0 + lane] : -1;
+        const int mypid = (pidx && m0 + lane < deg) ? pidx[k0 + m0 + lane] : 0;
         const int nc = min(64, deg - m0);
         for (int u0 = 0; u0 < nc; u0 += U) {
             BlkOps<T, B, NT> ops[U];
@@ -121,11 +124,12 @@ __device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, 
             for (int u = 0; u < U; ++u) {
                 const int m = u0 + u;
                 const int j = __shfl(mycol, min(m, 63), 64);
+                const int blk = pidx ? __shfl(mypid, min(m, 63), 64) : k0 + m0 + m;   // (pidx: uniform)
                 bool ok = m < nc && j >= 0;
                 if (SKIP >= 0 && ok) ok = j != i;
                 if (SKIP > 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
                 use[u] = ok;
-                if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)(k0 + m0 + m) * B * B, x + (size_t)j * B * k, k, col0, lane);
+                if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)blk * B * B, x + (size_t)j * B * k, k, col0, lane);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -138,7 +142,8 @@ __device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, 
 template <typename T, int B, int NT>
 __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
                                                 const int* __restrict__ cols, const T* __restrict__ A,
-                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y)
+                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
+                                                const int* __restrict__ pidx)
 {
     using V4 = typename Mf<T>::v4;
     const int lane = threadIdx.x & 63;
@@ -150,7 +155,7 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0);
+    row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
     const int col_l = lane & 15;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -210,7 +215,8 @@ template <typename T, int B, int NT, int SKIP>
 __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, const int* __restrict__ rows,
                                              const int* __restrict__ rowptr, const int* __restrict__ cols,
                                              const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
-                                             T* x, T omega, const int* __restrict__ color, int cur)
+                                             T* x, T omega, const int* __restrict__ color, int cur,
+                                             const int* __restrict__ pidx, const int* __restrict__ didx)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
@@ -225,7 +231,7 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur);
+    if (valid) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur, pidx);
     V4 res[RT * NT];
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -239,7 +245,7 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
             }
         }
     V4 out[RT * NT];
-    dinv_times<T, B, NT>(out, res, Dinv + (size_t)i * B * B, lds, lane);
+    dinv_times<T, B, NT>(out, res, Dinv + (size_t)(didx ? didx[i] : i) * B * B, lds, lane);
     if (!valid) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -263,7 +269,8 @@ template <typename T, int B, int NT>
 __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
                                                  const int* __restrict__ cols, const T* __restrict__ A,
                                                  const T* __restrict__ Dinv, const T* __restrict__ b,
-                                                 const T* __restrict__ x, T* __restrict__ xout, T omega)
+                                                 const T* __restrict__ x, T* __restrict__ xout, T omega,
+                                                 const int* __restrict__ pidx, const int* __restrict__ didx)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
@@ -278,7 +285,7 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0);
+    if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
     V4 res[RT * NT];
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
             }
         }
     V4 out[RT * NT];
-    dinv_times<T, B, NT>(out, res, Dinv + (size_t)i * B * B, lds, lane);
+    dinv_times<T, B, NT>(out, res, Dinv + (size_t)(didx ? didx[i] : i) * B * B, lds, lane);
     if (!valid) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -403,43 +410,44 @@ int mf_max_tiles(int B) { return B > 16 ? 2 : 4; }
 
 template <typename T>
 void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
-                  const T* b, T* y)
+                  const T* b, T* y, const int* pidx)
 {
     if (nrows <= 0 || k <= 0) return;
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
-                                                nrows, ng, rowptr, cols, A, x, b, y));
+                                                nrows, ng, rowptr, cols, A, x, b, y, pidx));
     check("mf_apply");
 }
 
 template <typename T>
 void MF<T>::gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* rows, const int* rowptr, const int* cols,
-                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip)
+                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip,
+                     const int* pidx, const int* didx)
 {
     if (nrows <= 0 || k <= 0) return;
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     const dim3 g(nblocks((long)nrows * ng));
     if (skip == 1) {
         MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 1>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
-                                                    A, Dinv, b, x, omega, color, cur));
+                                                    A, Dinv, b, x, omega, color, cur, pidx, didx));
     } else if (skip == 2) {
         MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 2>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
-                                                    A, Dinv, b, x, omega, color, cur));
+                                                    A, Dinv, b, x, omega, color, cur, pidx, didx));
     } else {
         MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 0>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
-                                                    A, Dinv, b, x, omega, color, cur));
+                                                    A, Dinv, b, x, omega, color, cur, pidx, didx));
     }
     check("mf_gs");
 }
 
 template <typename T>
 void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
-                   const T* Dinv, const T* b, const T* x, T* xout, T omega)
+                   const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* pidx, const int* didx)
 {
     if (nrows <= 0 || k <= 0) return;
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_jacobi<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
-                                                nrows, ng, rowptr, cols, A, Dinv, b, x, xout, omega));
+                                                nrows, ng, rowptr, cols, A, Dinv, b, x, xout, omega, pidx, didx));
     check("mf_jacobi");
 }
 

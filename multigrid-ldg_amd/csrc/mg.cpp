@@ -675,6 +675,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         }
         lv.pell.upload(pe.data(), pe.size(), s);
         lv.pdidx.upload(didx.data(), didx.size(), s);
+        lv.ppidx.upload(pidx.data(), pidx.size(), s);
         HIPCHK(hipStreamSynchronize(s));
         lv.npool = (int)prep.size(); lv.ndpool = (int)drep.size();
         lv.hpidx = pidx; lv.hdidx = didx;
@@ -890,6 +891,7 @@ template <> const float* SDop<float>(const Level& lv) { return lv.npool ? lv.DP3
 static const int* Sell(const Level& lv) { return lv.npool ? lv.pell.p : lv.A.ell.p; }
 static const int* Sdidx(const Level& lv) { return lv.npool ? lv.pdidx.p : nullptr; }
 static int Spool(const Level& lv) { return lv.npool ? 1 : 0; }
+static const int* Spidx(const Level& lv) { return lv.npool ? lv.ppidx.p : nullptr; }
 static size_t Sgw(const Level& lv) { return (size_t)lv.gw * (lv.npool ? 2 : 1); }   // ints per row of a GS column table
 template <typename T> static const T* Pool(const Level& lv);
 template <> const double* Pool<double>(const Level& lv) { return lv.pool64.p; }
@@ -1284,7 +1286,7 @@ void MGH::bapply(int l, int k, const T* x, const T* b, T* y)
 {
     Level& lv = L[l];
     const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].apply : 0;
-    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), x, b, y);
+    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Sop<T>(lv), x, b, y, Spidx(lv));
     else dev::MOps<T>::apply(ctx->stream, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), x, b, y);
 }
 
@@ -1360,8 +1362,8 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
             sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, const int*, int nr, const int* cdev, int c, int skip) {
-                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
-                                             bx<T>(v), omega, cdev, c, skip);
+                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Sop<T>(lv), SDop<T>(lv), bb<T>(v),
+                                             bx<T>(v), omega, cdev, c, skip, Spidx(lv), Sdidx(lv));
                 else dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
                                             bx<T>(v), omega, cdev, c, skip);
             });
@@ -1370,8 +1372,8 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
         if (zero_guess) dev::Ops<T>::fill(s, m, (T)0, bx<T>(v));
         T* cur = bx<T>(v); T* nxt = balt<T>(v);
         for (int sw = 0; sw < n; ++sw) {
-            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt,
-                                       omega);
+            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Sop<T>(lv), SDop<T>(lv), bb<T>(v), cur, nxt,
+                                       omega, Spidx(lv), Sdidx(lv));
             else dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
             std::swap(cur, nxt);
         }

@@ -140,6 +140,7 @@ struct Level {
     DBuf<double> P64, DP64;
     DBuf<float> P32, DP32;
     DBuf<int> pell, pdidx;
+    DBuf<int> ppidx;                    // per block slot its pool index (batched MFMA kernels)
     std::vector<int> hpidx, hdidx;      // host copies of the pool indices (traffic accounting)
     LevelVecs<double> v64;
     LevelVecs<float> v32;

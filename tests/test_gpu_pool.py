@@ -84,3 +84,21 @@ def test_simplex_levels_are_not_pooled(gctx, oracle):
     pr = mgldg.SimplexLDGPoisson(mgldg.Wireframe(gctx, pts, tris), 2, tau0=100.0)
     mg = mgldg.Multigrid(pr, "jacobi", omega=0.6, restriction="rat")
     assert mg.level_pool(0) == (0, 0)
+
+
+@pytest.mark.parametrize("relax", ["gs", "jacobi"])
+def test_pooled_batched_mfma_bitwise(gctx, relax):
+    """the batched MFMA kernels read the pool too (same tile family forced on both)"""
+    import mgldg
+    import torch
+    pr, mg, mg0 = pair(gctx, relax, 3)
+    rng = np.random.default_rng(4)
+    B = torch.tensor(pr.rhs(rng.standard_normal((pr.ndof, 32))), device="cuda")
+    for m in (mg, mg0):
+        m.batch_kernels(1)                  # one 16-column MFMA tile per wavefront, every level
+    for prec in ["fp64", "mixed"]:
+        kw = dict(tol=1e-10, maxit=200, criterion="true", precision=prec, flexible=prec == "mixed")
+        X1, s1 = mgldg.pcg(mg, B, torch.zeros_like(B), **kw)
+        X0, s0 = mgldg.pcg(mg0, B, torch.zeros_like(B), **kw)
+        assert [s["iterations"] for s in s1] == [s["iterations"] for s in s0]
+        assert torch.equal(X1, X0), prec
