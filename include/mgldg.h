@@ -354,7 +354,9 @@ int mgldg_dense_write(const char* path, int rows, int cols, const double* a);
  *   inverse diagonal blocks and row descriptors, b, x (read) and x (write)
  *   of its rows, and x of their distinct neighbours.
  *   Jacobi sweep: all blocks + inverse diagonal blocks + indices + b, x, x_out.
- *   SpMV: all blocks + indices + x (read) + y (write). */
+ *   SpMV: all blocks + indices + x (read) + y (write).
+ * On a pooled level (mgldg_mg_level_pool) the blocks count once per distinct
+ * pool block a launch touches and the indices as (column, pool index) pairs. */
 int mgldg_profile_relax(mgldg_mg mg, int level, int precision, int nrep, double* ms_per_launch,
                         double* bytes_per_launch, int* launches);
 /* The same for one part of a multicolour GS sweep: part 0 = whole sweeps

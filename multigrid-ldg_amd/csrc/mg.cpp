@@ -598,6 +598,16 @@ static bool pool_blocks(hipStream_t s, long nblk, int bs, const double* blocks, 
     std::vector<unsigned long long> h(nblk);
     hd.download(h.data(), h.size(), s);
     HIPCHK(hipStreamSynchronize(s));
+    {
+        // a strided sample first: an operator whose sample is mostly distinct
+        // (unstructured meshes) is not worth the full grouping
+        std::vector<unsigned long long> smp;
+        const long step = std::max<long>(1, nblk / 4096);
+        for (long k = 0; k < nblk; k += step) smp.push_back(h[k]);
+        std::sort(smp.begin(), smp.end());
+        const long nd = (long)(std::unique(smp.begin(), smp.end()) - smp.begin());
+        if (nd * 2 > (long)smp.size()) return false;
+    }
     std::unordered_map<unsigned long long, int> first;
     first.reserve(1024);
     pidx.resize(nblk);
