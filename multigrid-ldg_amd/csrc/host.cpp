@@ -682,25 +682,69 @@ void Interp::make_gather()
     for (int i = 0; i < nf; ++i) kids[pos[parent[i]]++] = i;
 }
 
+// Coarse rows are independent: row I's pattern, its blocks' slots and their
+// terms come from the kids of I only, visited in ascending order (kids
+// ascending, their blocks ascending), which is the order of a serial sweep
+// over the fine rows -- so the rows are formed on up to 16 threads with the
+// same pattern and term order.
 void rap_symbolic(const Interp& T, const HostBSR& X, HostBSR& C, TermList& terms)
 {
-    PatternBuilder pb(T.nc, T.nc);
-    for (int k = 0; k < X.nbr; ++k)
-        for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) pb.add(T.parent[k], T.parent[X.cols[s]]);
-    pb.finish(C, T.bc, T.bc, false);
+    const int nc = T.nc;
+    std::vector<int> koff(nc + 1, 0), kid(X.nbr);
+    for (int k = 0; k < X.nbr; ++k) koff[T.parent[k] + 1]++;
+    for (int c = 0; c < nc; ++c) koff[c + 1] += koff[c];
+    {
+        std::vector<int> pos(koff.begin(), koff.end() - 1);
+        for (int k = 0; k < X.nbr; ++k) kid[pos[T.parent[k]]++] = k;
+    }
+    std::vector<std::vector<int>> rows(nc);
+    parallel_rows(nc, [&](long a, long b) {
+        std::vector<int> tmp;
+        for (long I = a; I < b; ++I) {
+            tmp.clear();
+            for (int q = koff[I]; q < koff[I + 1]; ++q) {
+                const int k = kid[q];
+                for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) tmp.push_back(T.parent[X.cols[s]]);
+            }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            rows[I] = tmp;
+        }
+    });
+    C.nbr = C.nbc = nc; C.br = C.bc = T.bc; C.val.clear();
+    C.rowptr.assign(nc + 1, 0);
+    for (int I = 0; I < nc; ++I) C.rowptr[I + 1] = C.rowptr[I] + (int)rows[I].size();
+    C.cols.resize(C.rowptr[nc]);
     std::vector<int> cnt(C.nnzb() + 1, 0), slot(X.nnzb());
-    for (int k = 0; k < X.nbr; ++k)
-        for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) { slot[s] = C.find(T.parent[k], T.parent[X.cols[s]]); cnt[slot[s] + 1]++; }
+    parallel_rows(nc, [&](long a, long b) {
+        for (long I = a; I < b; ++I) {
+            std::copy(rows[I].begin(), rows[I].end(), C.cols.begin() + C.rowptr[I]);
+            const auto rb = C.cols.begin() + C.rowptr[I], re = C.cols.begin() + C.rowptr[I + 1];
+            for (int q = koff[I]; q < koff[I + 1]; ++q) {
+                const int k = kid[q];
+                for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) {
+                    slot[s] = (int)(std::lower_bound(rb, re, T.parent[X.cols[s]]) - C.cols.begin());
+                    cnt[slot[s] + 1]++;          // (slots of row I: this thread's alone)
+                }
+            }
+        }
+    });
+    rows.clear(); rows.shrink_to_fit();
     terms.off.assign(C.nnzb() + 1, 0);
     for (long s = 0; s < C.nnzb(); ++s) terms.off[s + 1] = terms.off[s] + cnt[s + 1];
-    long nt = terms.off.back();
+    const long nt = terms.off.back();
     terms.l.resize(nt); terms.x.resize(nt); terms.r.resize(nt);
     std::vector<int> pos(terms.off.begin(), terms.off.end() - 1);
-    for (int k = 0; k < X.nbr; ++k)
-        for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) {
-            int p = pos[slot[s]]++;
-            terms.l[p] = T.type[k]; terms.x[p] = s; terms.r[p] = T.type[X.cols[s]];
-        }
+    parallel_rows(nc, [&](long a, long b) {
+        for (long I = a; I < b; ++I)
+            for (int q = koff[I]; q < koff[I + 1]; ++q) {
+                const int k = kid[q];
+                for (int s = X.rowptr[k]; s < X.rowptr[k + 1]; ++s) {
+                    const int p = pos[slot[s]]++;
+                    terms.l[p] = T.type[k]; terms.x[p] = s; terms.r[p] = T.type[X.cols[s]];
+                }
+            }
+    });
 }
 
 int greedy_colors(const HostBSR& A, std::vector<int>& color)

@@ -4,9 +4,11 @@
 // (kernels.hip, mg.cpp).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace mgldg {
@@ -76,6 +78,22 @@ struct HostBSR {
     double* block(long k) { return val.data() + k * (long)br * bc; }
     const double* block(long k) const { return val.data() + k * (long)br * bc; }
 };
+
+// fn(lo, hi) over [0, n) on up to 16 threads (disjoint index ranges)
+template <typename F>
+inline void parallel_rows(long n, F&& fn)
+{
+    const long nthr = std::min<long>({16L, (long)std::max(1u, std::thread::hardware_concurrency()), std::max(1L, n / 256)});
+    if (nthr <= 1) { fn(0L, n); return; }
+    std::vector<std::thread> th;
+    const long chunk = (n + nthr - 1) / nthr;
+    for (long t = 0; t < nthr; ++t) {
+        const long lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+    }
+    for (auto& x : th) x.join();
+}
 
 // Pattern from (row, col) pairs: sorted unique per row.
 struct PatternBuilder {
