@@ -330,9 +330,13 @@ __global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, do
     Wt[e] = (r < n && c < n) ? W[(size_t)r * n + c] : 0.0;
 }
 
-// partial products of Y = W X over K slices: one wavefront per (16 x 16 output
-// tile, K slice); part[(slice * nRpad + row) * k + col]
-template <typename T>
+// partial products of Y = W X over K slices, part[(slice * nRpad + row) * k + col].
+// A wavefront owns one 16-row tile of W, NCW consecutive 16-column tiles of
+// Y and one K slice: each W operand is loaded once for its NCW column tiles,
+// and four K steps of operands are loaded before their products, so
+// several loads are in flight per wavefront.  Products accumulate in K
+// order within the slice.
+template <typename T, int NCW>
 __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int nslices, int kpers,
                                                const double* __restrict__ Wt, const T* __restrict__ x,
                                                double* __restrict__ part)
@@ -340,25 +344,48 @@ __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int
     using V4 = Mf<double>::v4;
     const int lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nC = (k + 15) / 16;
+    const int nC = (k + 15) / 16, ncg = (nC + NCW - 1) / NCW;
     const int tile = w / nslices, sl = w - tile * nslices;
-    if (tile >= nR * nC) return;
-    const int R = tile / nC, Cc = tile - R * nC;
-    const int col = 16 * Cc + (lane & 15), q = lane >> 4;
+    if (tile >= nR * ncg) return;
+    const int R = tile / ncg, G = tile - R * ncg;
+    const int q = lane >> 4, cl = lane & 15;
     const int K0 = sl * kpers, K1 = min(nK, K0 + kpers);
-    V4 acc = V4{0, 0, 0, 0};
-    const double* wt = Wt + (size_t)R * nK * 64 + lane;
-    for (int K = K0; K < K1; ++K) {
-        const int c = 4 * K + q;
-        const double a = wt[(size_t)K * 64];
-        const double b = (c < n && col < k) ? (double)x[(size_t)c * k + col] : 0.0;
-        acc = Mf<double>::mac(a, b, acc);
-    }
-    if (col >= k) return;
+    V4 acc[NCW];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int r = 16 * R + Mf<double>::row(lane, e);
-        part[((size_t)sl * (16 * nR) + r) * k + col] = acc[e];
+    for (int t = 0; t < NCW; ++t) acc[t] = V4{0, 0, 0, 0};
+    const double* wt = Wt + (size_t)R * nK * 64 + lane;
+    auto xval = [&](int K, int t) {
+        const int c = 4 * K + q, col = 16 * (G * NCW + t) + cl;
+        return (c < n && col < k) ? (double)x[(size_t)c * k + col] : 0.0;
+    };
+    int K = K0;
+    for (; K + 4 <= K1; K += 4) {
+        double a[4], b[4][NCW];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = wt[(size_t)(K + u) * 64];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < NCW; ++t) b[u][t] = xval(K + u, t);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < NCW; ++t) acc[t] = Mf<double>::mac(a[u], b[u][t], acc[t]);
+    }
+    for (; K < K1; ++K) {
+        const double a = wt[(size_t)K * 64];
+#pragma unroll
+        for (int t = 0; t < NCW; ++t) acc[t] = Mf<double>::mac(a, xval(K, t), acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < NCW; ++t) {
+        const int col = 16 * (G * NCW + t) + cl;
+        if (col >= k) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * R + Mf<double>::row(lane, e);
+            part[((size_t)sl * (16 * nR) + r) * k + col] = acc[t][e];
+        }
     }
 }
 
@@ -462,9 +489,17 @@ void gemm_tiles(hipStream_t s, int n, const double* W, double* Wt)
     check("tile_w");
 }
 
+// column tiles per wavefront of mf_gemm
+static int gemm_ncw(int k)
+{
+    const int nC = (k + 15) / 16;
+    return nC >= 4 ? 4 : (nC >= 2 ? 2 : 1);
+}
+
 int gemm_slices(int n, int k)
 {
-    const int tiles = ((n + 15) / 16) * ((k + 15) / 16);
+    const int ncw = gemm_ncw(k);
+    const int tiles = ((n + 15) / 16) * (((k + 15) / 16 + ncw - 1) / ncw);
     int sl = (2048 + tiles - 1) / tiles;          // about 2048 wavefronts
     const int nK = (n + 3) / 4;
     if (sl > 32) sl = 32;
@@ -480,7 +515,11 @@ void gemm_mf(hipStream_t s, int n, int k, const double* Wt, const T* x, T* y, do
     if (n <= 0 || k <= 0) return;
     const int nR = (n + 15) / 16, nK = (n + 3) / 4, nC = (k + 15) / 16;
     const int sl = gemm_slices(n, k), kpers = (nK + sl - 1) / sl;
-    hipLaunchKernelGGL(mf_gemm<T>, dim3(nblocks((long)nR * nC * sl)), dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    const int ncw = gemm_ncw(k), ncg = (nC + ncw - 1) / ncw;
+    const dim3 g(nblocks((long)nR * ncg * sl));
+    if (ncw == 4) hipLaunchKernelGGL((mf_gemm<T, 4>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    else if (ncw == 2) hipLaunchKernelGGL((mf_gemm<T, 2>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    else hipLaunchKernelGGL((mf_gemm<T, 1>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
     check("mf_gemm");
     const long t = (long)n * k;
     hipLaunchKernelGGL(k_gemm_sum<T>, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, k, nR, sl, part, y);
