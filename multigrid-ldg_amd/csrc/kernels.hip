@@ -573,6 +573,28 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     if (valid) xout[o] = xi + omega * dinv_row<T, B>(dv, t);
 }
 
+// The first Jacobi sweep from x = 0: xout = omega D^-1 b (A x = 0 is not
+// read), the value k_jacobi gives for x = 0
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_jacobi0(int nrows, const T* __restrict__ Dinv, const int* __restrict__ didx,
+                                                 const T* __restrict__ b, T* __restrict__ xout, T omega)
+{
+    using M = Map<T, B>;
+    using R = RowLds<T, B>;
+    __shared__ __align__(16) T tb[4 * R::RW * B];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B;
+    const int i = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    const bool valid = slot < M::RPW && i < nrows;
+    const size_t o = (size_t)i * B + r;
+    VecT<T, M::V> dv[B / M::V];
+    T* t = tb + (wave * R::RW + slot) * B;
+    t[r] = valid ? b[o] : (T)0;
+    if (valid) load_dinv_row<T, B>(Dinv, didx ? didx[i] : i, r, dv);
+    wave_sync();
+    if (valid) xout[o] = omega * dinv_row<T, B>(dv, t);
+}
+
 // Interpolation blocks are BF x BC row-major (fine simplex or box basis x
 // coarse box basis); BF == BC on quadtree hierarchies.
 template <typename T, int BF, int BC>
@@ -806,6 +828,17 @@ void Ops<T>::jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const in
                                nullptr, b, x, xout, omega);
     });
     check_launch("jacobi");
+}
+
+template <typename T>
+void Ops<T>::jacobi0(hipStream_t s, int B, int nrows, const T* Dinv, const T* b, T* xout, T omega, const int* didx)
+{
+    if (nrows <= 0) return;
+    MGLDG_DISPATCH(B, {
+        using M = Map<T, BB>;
+        hipLaunchKernelGGL((k_jacobi0<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, Dinv, didx, b, xout, omega);
+    });
+    check_launch("jacobi0");
 }
 
 template <typename T>
@@ -1292,6 +1325,38 @@ void MOps<T>::gs_color(hipStream_t s, int B, int k, int nrows, const int* rows, 
                                          Dinv, b, x, omega, color, cur, skip));
     check_launch("mk_gs");
 }
+// the first batched Jacobi sweep from x = 0: xout = omega D^-1 b, one thread
+// per (block row, column)
+template <typename T, int B>
+__global__ __launch_bounds__(256) void mk_jacobi0(int k, int nrows, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                                  T* __restrict__ xout, T omega)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)nrows * k) return;
+    const int i = (int)(e / k), c = (int)(e % k);
+    T t[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) t[r] = b[((size_t)i * B + r) * k + c];
+    const T* D = Dinv + (size_t)i * B * B;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        T sdot = 0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) sdot += pan<T, B>(D, r, q) * t[q];
+        xout[((size_t)i * B + r) * k + c] = omega * sdot;
+    }
+}
+
+template <typename T>
+void MOps<T>::jacobi0(hipStream_t s, int B, int k, int nrows, const T* Dinv, const T* b, T* xout, T omega)
+{
+    const long n = (long)nrows * k;
+    if (n <= 0) return;
+    MGLDG_DISPATCH(B, hipLaunchKernelGGL((mk_jacobi0<T, BB>), dim3(nblk(n, 256)), dim3(256), 0, s, k, nrows, Dinv, b, xout,
+                                         omega));
+    check_launch("mk_jacobi0");
+}
+
 template <typename T>
 void MOps<T>::jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* Dinv,
                      const T* b, const T* x, T* xout, T omega)

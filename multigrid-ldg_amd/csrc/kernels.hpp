@@ -91,6 +91,9 @@ struct Ops {
     // (didx != nullptr: A and Dinv are pools, ell pairs, didx the rows' D^-1 pool indices)
     static void jacobi(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* didx = nullptr);
+    // the first Jacobi sweep from x = 0: x_out = omega Dinv b (didx as in jacobi)
+    static void jacobi0(hipStream_t s, int B, int nrows, const T* Dinv, const T* b, T* xout, T omega,
+                        const int* didx = nullptr);
     // b_c = T^T r (multigrid.h:414-418) as a gather over children; T blocks
     // are BF x BC (fine x coarse basis)
     static void restrict_(hipStream_t s, int BF, int BC, int nc, const int* kid_off, const int* kids, const int* type,
@@ -116,6 +119,8 @@ struct MOps {
                          const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
     static void jacobi(hipStream_t s, int B, int k, int nrows, const int* rowptr, const int* cols, const T* A,
                        const T* Dinv, const T* b, const T* x, T* xout, T omega);
+    // the first sweep from x = 0: xout = omega Dinv b (Dinv: one block per row)
+    static void jacobi0(hipStream_t s, int B, int k, int nrows, const T* Dinv, const T* b, T* xout, T omega);
     static void restrict_(hipStream_t s, int BF, int BC, int k, int nc, const int* kid_off, const int* kids,
                           const int* type, const T* pool, const T* r, T* bc);
     static void prolong(hipStream_t s, int BF, int BC, int k, int nf, const int* parent, const int* type,

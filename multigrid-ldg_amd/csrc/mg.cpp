@@ -967,10 +967,15 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
                                             skip, Spool(lv));
                   });
     } else {
-        // Jacobi ping-pongs between x and xalt; the result ends in x
-        if (zero_guess) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
+        // Jacobi ping-pongs between x and xalt; the result ends in x.  From
+        // x = 0 the first sweep is omega D^-1 b (no A x, no clearing of x).
         T* cur = v.x.p; T* alt = v.xalt.p;
-        for (int sw = 0; sw < n; ++sw) {
+        int sw0 = 0;
+        if (zero_guess) {
+            if (n == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
+            else { dev::Ops<T>::jacobi0(s, lv.B, lv.nb, SDop<T>(lv), v.b, alt, omega, Sdidx(lv)); std::swap(cur, alt); sw0 = 1; }
+        }
+        for (int sw = sw0; sw < n; ++sw) {
             dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), SDop<T>(lv), v.b, cur, alt, omega,
                                 Sdidx(lv));
             std::swap(cur, alt);
@@ -1379,9 +1384,13 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
             });
     } else {
         const long m = (long)lv.nb * lv.B * k;
-        if (zero_guess) dev::Ops<T>::fill(s, m, (T)0, bx<T>(v));
         T* cur = bx<T>(v); T* nxt = balt<T>(v);
-        for (int sw = 0; sw < n; ++sw) {
+        int sw0 = 0;
+        if (zero_guess) {   // from x = 0 the first sweep is omega D^-1 b
+            if (n == 0) dev::Ops<T>::fill(s, m, (T)0, bx<T>(v));
+            else { dev::MOps<T>::jacobi0(s, lv.B, k, lv.nb, Dop<T>(lv), bb<T>(v), nxt, omega); std::swap(cur, nxt); sw0 = 1; }
+        }
+        for (int sw = sw0; sw < n; ++sw) {
             if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Sop<T>(lv), SDop<T>(lv), bb<T>(v), cur, nxt,
                                        omega, Spidx(lv), Sdidx(lv));
             else dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
