@@ -478,6 +478,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
     ap.add_argument("--no-ball", action="store_true", help="skip the 3-D ball workload")
     ap.add_argument("--ball-only", action="store_true", help="only the 3-D ball workload")
+    ap.add_argument("--setup-only", action="store_true",
+                    help="the device assembly: the adaptive quadtree problem and hierarchy and the ball problem (profiling)")
     ap.add_argument("--unstructured-only", action="store_true", help="only the unit-disk workload")
     ap.add_argument("--no-periodic", action="store_true", help="skip the periodic workload")
     ap.add_argument("--batched-only", action="store_true",
@@ -502,6 +504,22 @@ def main():
         return
     if args.unstructured_only:
         print(json.dumps({"unstructured": unstructured_leg(mgldg, torch, args.nrhs, args.warmup)}), flush=True)
+        return
+    if args.setup_only:
+        ctx = mgldg.Context(0)
+        t0 = time.perf_counter()
+        qt = mgldg.Quadtree(ctx, 2, c2_size, coarsening="rapid", balance=True)
+        prob = mgldg.LDGPoisson(qt, 3, bc="dirichlet", tau0=0.0, tauD=1000.0)
+        mgldg.Multigrid(prob, relaxation="gs", omega=0.8, npre=2, npost=2, restriction="ratrat", precision="mixed")
+        quad = {"setup_s": round(time.perf_counter() - t0, 3), "assembly": assembly_info(prob)}
+        log(f"quadtree problem: {quad}")
+        pts, tets = ball_mesh()
+        tets = tets[mgldg.locality_order(pts, tets)]
+        t0 = time.perf_counter()
+        bprob = mgldg.SimplexLDGPoisson(mgldg.Wireframe(ctx, pts, tets), 2, tau0=100.0, tauD=1000.0)
+        ball = {"problem_s": round(time.perf_counter() - t0, 3), "assembly": assembly_info(bprob)}
+        log(f"ball problem: {ball}")
+        print(json.dumps({"setup": {"quadtree": quad, "ball": ball}}), flush=True)
         return
 
     torch.cuda.init()
