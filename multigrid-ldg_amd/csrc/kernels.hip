@@ -13,6 +13,7 @@
 // Reductions (PCG dot products) are deterministic: per-workgroup partials are
 // summed in index order by the last workgroup to finish.
 #include "kernels.hpp"
+#include "devutil.hpp"
 
 #include <stdexcept>
 #include <string>
@@ -69,11 +70,12 @@ __device__ double block_sum(double v)
 // a one-workgroup kernel adds the partials in index order.  (A fused
 // "last workgroup finishes" scheme needs agent-scope fences, which write back
 // the L2 of every XCD and cost more than the extra launch.)
-__device__ __forceinline__ void block_partial(double v, double* part)
+__device__ __forceinline__ void block_partial(double v, double* part, int slot)
 {
     double bs = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+    if (threadIdx.x == 0) part[slot] = bs;
 }
+__device__ __forceinline__ void block_partial(double v, double* part) { block_partial(v, part, blockIdx.x); }
 
 // sums nval interleaved partial streams (part[i * nval + k], i < n)
 __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ part, int n, int nval, double* S, int mode,
@@ -441,7 +443,8 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, r = lane - slot * B;
-    const int row = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    const int lb = xcd_block();
+    const int row = lb * M::RPB + wave * M::RPW + slot;
     const bool valid = slot < M::RPW && row < nrows;
     const int k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
     const size_t o = (size_t)row * B + r;
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
         y[o] = out;
         if (dotv) dv = (double)out * (double)dotv[o];
     }
-    if (part) block_partial(dv, part);
+    if (part) block_partial(dv, part, lb);   // partials in row order, whatever the placement
 }
 
 // Damped block GS on rows [beg, end) of a descriptor list, one block row per
@@ -486,7 +489,8 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
         if (SKIP == 0) xi = x[o];
         load_dinv_row<T, B>(Dinv, POOL ? d.w : d.x, r, dv);
     }
-    const T acc = row_sum<T, B, SKIP, POOL>(valid, d.y, d.z, POOL ? 0 : d.w, gcols + (size_t)idx * W * (POOL ? 2 : 1), W, A,
+    const T acc = reads_none<SKIP>(cur) ? (T)0
+                : row_sum<T, B, SKIP, POOL>(valid, d.y, d.z, POOL ? 0 : d.w, gcols + (size_t)idx * W * (POOL ? 2 : 1), W, A,
                                             x, r, slot * B, xs_all + (wave * R::RW + slot) * R::XS, color, cur);
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ 
     using R = RowLds<T, B>;
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
     __shared__ __align__(16) T tb[4 * R::RW * B];
-    const int beg = blockIdx.x * Map<T, B>::RPB;
+    const int beg = xcd_block() * Map<T, B>::RPB;
     gs_rows<T, B, SKIP, POOL>(beg, nrows, desc, gcols, W, A, Dinv, b, x, omega, color, cur, xs_all, tb);
 }
 
@@ -531,7 +535,8 @@ __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int
     using R = RowLds<T, B>;
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
     __shared__ __align__(16) T tb[4 * R::RW * B];
-    const int s = wg[3 * blockIdx.x], m = wg[3 * blockIdx.x + 1], e = wg[3 * blockIdx.x + 2];
+    const int lb = xcd_block();
+    const int s = wg[3 * lb], m = wg[3 * lb + 1], e = wg[3 * lb + 2];
     for (int ph = 0; ph < 2; ++ph) {
         const bool lo = (ph == 0) != (reverse != 0);
         const int beg = lo ? s : m, end = lo ? m : e;
@@ -554,7 +559,7 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     __shared__ __align__(16) T tb[4 * R::RW * B];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, r = lane - slot * B;
-    const int i = blockIdx.x * M::RPB + wave * M::RPW + slot;
+    const int i = xcd_block() * M::RPB + wave * M::RPW + slot;
     const bool valid = slot < M::RPW && i < nrows;
     const int k0 = valid ? rowptr[i] : 0, k1 = valid ? rowptr[i + 1] : 0;
     const size_t o = (size_t)i * B + r;
@@ -1042,7 +1047,7 @@ __global__ __launch_bounds__(256) void mk_row(int k, int nrows, const int* __res
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int rpw = 64 / k, slot = lane / k, c = lane - slot * k, lane0 = slot * k;
     T* La = reinterpret_cast<T*>(mk_row_lds) + (size_t)(wv * rpw + slot) * (MK_U * B2);
-    const int ri = (blockIdx.x * 4 + wv) * rpw + slot;
+    const int ri = (xcd_block() * 4 + wv) * rpw + slot;
     const bool valid = ri < nrows;
     const int i = valid ? (MODE == 2 ? rows[ri] : ri) : 0;
     const int k0 = valid ? rowptr[i] : 0, deg = valid ? rowptr[i + 1] - k0 : 0;
@@ -1051,6 +1056,7 @@ __global__ __launch_bounds__(256) void mk_row(int k, int nrows, const int* __res
     T t[B];
 #pragma unroll
     for (int r = 0; r < B; ++r) t[r] = (MODE != 0 && valid) ? b[((size_t)i * B + r) * k + c] : (T)0;
+    if (reads_none<SKIP>(cur)) wl = 0;
     for (int m0 = 0; m0 < wl; m0 += k) {
         const int mycol = (valid && m0 + c < deg) ? cols[k0 + m0 + c] : -1;
         const int nc = min(k, wl - m0);

@@ -8,6 +8,12 @@
 namespace mgldg {
 namespace dev {
 
+// `cur` of the first launch of a Gauss-Seidel sweep from x = 0 (skip 1:
+// forward, skip 2: reverse): no neighbour of its rows has been written yet,
+// and the colour tests of the skip modes read none of them
+constexpr int kFirstFwd = -1;
+constexpr int kFirstRev = 1 << 30;
+
 // ---------------------------------------------------------------- setup
 // out[o] = sum_{t in [off[o], off[o+1])} opL(L[l[t]]) * X[x[t]] * R[r[t]]
 // (row-major FP64 blocks; l/r = -1 means identity; opL = transpose if transL).

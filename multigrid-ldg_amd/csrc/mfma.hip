@@ -3,12 +3,16 @@
 // For a block row i the batched operators are small dense products
 //   Y_i = sum_j A_ij X_j     (A_ij: B x B, X_j: B x k, k columns contiguous)
 // One wavefront owns one block row and NT tiles of 16 columns.  Each block is
-// a product of RT = ceil(B / 16) row tiles of 16 x 16 x 4-step MFMAs
-// (v_mfma_f32_16x16x4f32 / _f64_16x16x4f64), blocks of 6..32 rows padded
-// with zeros in registers: for row tile rt lane l supplies
-// A[16 rt + l % 16][4s + l / 16] and X[4s + l / 16][l % 16] for K step s.  The
-// accumulator lane layout is the instruction's (f32: register i of lane l
-// holds row 4 (l / 16) + i; f64: row 4 i + l / 16; column l % 16).
+// a product of RT = ceil(B / 16) row tiles of S = ceil(B / 4) K steps of
+// 16 x 16 x 4 MFMAs (v_mfma_f32_16x16x4f32 / _f64_16x16x4f64), blocks of
+// 6..32 rows padded with zeros in registers.  Lane l = 16 q + n (q = l / 16,
+// n = l % 16) supplies, for K step s and row tile rt, A[16 rt + n][S q + s]
+// and, for column tile t, X[S q + s][col0 + NT n + t]: block column S q + s
+// (not 4 s + q) so that a lane's S columns of an A row are contiguous in the
+// panel layout, and output column col0 + NT n + t (not col0 + 16 t + n) so
+// that a lane's NT columns of X, b, x and y are contiguous; both come as
+// vector loads.  The accumulator lane layout is the instruction's (f32:
+// register e of lane l holds row 4 q + e; f64: row 4 e + q; tile column n).
 // The smoothers write the residual tile to LDS and read it back in the
 // B-operand layout for the D^-1 product.
 #include <hip/hip_runtime.h>
@@ -17,6 +21,7 @@
 #include <string>
 
 #include "kernels.hpp"
+#include "devutil.hpp"
 
 namespace mgldg {
 namespace dev {
@@ -48,9 +53,36 @@ template <> struct Mf<double> {
     static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
 };
 
+// N contiguous elements at p (N * sizeof(T)-aligned) into o[0 .. N)
+template <typename T, int N>
+__device__ __forceinline__ void mf_ld(T* o, const T* __restrict__ p)
+{
+    if constexpr (N == 1) {
+        o[0] = p[0];
+    } else {
+        typedef T VN __attribute__((ext_vector_type(N)));
+        const VN v = *reinterpret_cast<const VN*>(p);
+#pragma unroll
+        for (int e = 0; e < N; ++e) o[e] = v[e];
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void mf_st(T* p, const T* v)
+{
+    if constexpr (N == 1) {
+        p[0] = v[0];
+    } else {
+        typedef T VN __attribute__((ext_vector_type(N)));
+        VN w;
+#pragma unroll
+        for (int e = 0; e < N; ++e) w[e] = v[e];
+        *reinterpret_cast<VN*>(p) = w;
+    }
+}
+
 // Operands of one block product A_blk X_j for this lane: per K step s the A
 // element of each row tile and the X element of each column tile (rows padded
-// to 16 RT, K to a multiple of 4, with zeros).
+// to 16 RT, K to 4 S, with zeros).
 template <typename T, int B, int NT>
 struct BlkOps {
     static constexpr int RT = (B + 15) / 16, S = (B + 3) / 4;
@@ -67,23 +99,66 @@ __host__ __device__ constexpr int mf_unroll()
     return per * 4 <= 128 ? 4 : (per * 3 <= 128 ? 3 : (per * 2 <= 128 ? 2 : 1));
 }
 
+// The lane-dependent, block-independent parts of the operand offsets
+// (elements).  X[S q + s][gc .. gc + NT) of a block column sits at xl + s k,
+// gc = col0 + NT n: a group wholly at or past k (k is a multiple of NT) is
+// read from columns k - NT .. k instead, in bounds, and its products only
+// reach output columns that are never stored.  When V divides S, the panel
+// element (16 rt + n, S q + s) of an A block sits at
+// al + (s / V) B V + 16 rt V + s % V (the lane's V columns s, s+1, .. of a
+// group are contiguous); otherwise it is addressed element by element.  The
+// block bases themselves are wave-uniform (scalar registers).
+template <typename T, int B, int NT>
+struct LaneOff {
+    static constexpr int S = (B + 3) / 4, V = pv(sizeof(T), B);
+    unsigned xl, al;
+    int q, n, gc;
+    __device__ __forceinline__ LaneOff(int lane, int k, int col0)
+    {
+        n = lane & 15;
+        q = lane >> 4;
+        gc = col0 + NT * n;
+        xl = (unsigned)(S * q * k + min(gc, k - NT));
+        al = (unsigned)((S % V == 0 ? (S * q / V) * B * V : 0) + n * V);
+    }
+    // lane's K index S q + s inside the block?  (compile-time true but where q = 3 can pass B)
+    __device__ __forceinline__ bool kin(int s) const { return 3 * S + s < B || S * q + s < B; }
+};
+
 template <typename T, int B, int NT>
 __device__ __forceinline__ void blk_load(BlkOps<T, B, NT>& o, const T* __restrict__ blk, const T* __restrict__ xj, int k,
-                                         int col0, int lane)
+                                         const LaneOff<T, B, NT>& lo)
 {
-    const int r16 = lane & 15, q = lane >> 4;
+    constexpr int S = LaneOff<T, B, NT>::S, V = LaneOff<T, B, NT>::V, RT = BlkOps<T, B, NT>::RT;
 #pragma unroll
-    for (int s = 0; s < BlkOps<T, B, NT>::S; ++s) {
-        const int c = 4 * s + q;
+    for (int s = 0; s < S; ++s) {
+        if (lo.kin(s)) mf_ld<T, NT>(o.xb[s], xj + (size_t)s * k + lo.xl);
+        else {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = col0 + 16 * t + r16;
-            o.xb[s][t] = (c < B && col < k) ? xj[(size_t)c * k + col] : (T)0;
+            for (int t = 0; t < NT; ++t) o.xb[s][t] = (T)0;
         }
+    }
 #pragma unroll
-        for (int rt = 0; rt < BlkOps<T, B, NT>::RT; ++rt) {
-            const int r = 16 * rt + r16;
-            o.a[s][rt] = (r < B && c < B) ? panel_at<T, B>(blk, r, c) : (T)0;
+    for (int rt = 0; rt < RT; ++rt) {
+        const bool rin = 16 * rt + 15 < B || 16 * rt + lo.n < B;
+        if constexpr (S % V == 0) {
+#pragma unroll
+            for (int s0 = 0; s0 < S; s0 += V) {
+                T v[V];
+                if (rin && lo.kin(s0)) mf_ld<T, V>(v, blk + (s0 / V) * B * V + 16 * rt * V + lo.al);   // (B % V == 0: whole group in or out)
+                else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) v[e] = (T)0;
+                }
+#pragma unroll
+                for (int e = 0; e < V; ++e) o.a[s0 + e][rt] = v[e];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int c = S * lo.q + s, r = 16 * rt + lo.n;
+                o.a[s][rt] = (rin && lo.kin(s)) ? panel_at<T, B>(blk, r, c) : (T)0;
+            }
         }
     }
 }
@@ -101,17 +176,18 @@ __device__ __forceinline__ void blk_mac(typename Mf<T>::v4* acc, const BlkOps<T,
 
 // acc += sum over the blocks k0 .. k1-1 of row i (one row per wavefront, so
 // every branch below is uniform): the row's column indices are loaded once,
-// one per lane, and broadcast with shuffles; the operands of U blocks are in
-// flight together.  SKIP: -1 = every block; 0 = not the diagonal block;
-// 1 (2) = also not the colours after (before) `cur` (first GS sweep from x = 0).
-// pidx (pooled operator): per block slot its index in the pool A; null: A
-// holds every block.
+// one per lane, and read out with readlane into scalar registers; the
+// operands of U blocks are in flight together.  SKIP: -1 = every block; 0 =
+// not the diagonal block; 1 (2) = also not the colours after (before) `cur`
+// (first GS sweep from x = 0).  pidx (pooled operator): per block slot its
+// index in the pool A; null: A holds every block.
 template <typename T, int B, int NT, int SKIP>
 __device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, int k1, const int* __restrict__ cols,
                                         const T* __restrict__ A, const T* __restrict__ x, int k, int col0, int lane,
                                         const int* __restrict__ color, int cur, const int* __restrict__ pidx)
 {
     constexpr int U = mf_unroll<T, B, NT>();
+    const LaneOff<T, B, NT> lo(lane, k, col0);
     const int deg = k1 - k0;
     for (int m0 = 0; m0 < deg; m0 += 64) {
         const int mycol = (m0 + lane < deg) ? cols[k0 + m0 + lane] : -1;
@@ -122,14 +198,14 @@ __device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, 
             bool use[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int m = u0 + u;
-                const int j = __shfl(mycol, min(m, 63), 64);
-                const int blk = pidx ? __shfl(mypid, min(m, 63), 64) : k0 + m0 + m;   // (pidx: uniform)
-                bool ok = m < nc && j >= 0;
+                const int m = min(u0 + u, 63);
+                const int j = __builtin_amdgcn_readlane(mycol, m);
+                const int blk = pidx ? __builtin_amdgcn_readlane(mypid, m) : k0 + m0 + m;
+                bool ok = u0 + u < nc && j >= 0;
                 if (SKIP >= 0 && ok) ok = j != i;
                 if (SKIP > 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
                 use[u] = ok;
-                if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)blk * B * B, x + (size_t)j * B * k, k, col0, lane);
+                if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)blk * (B * B), x + (size_t)j * B * k, k, lo);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -147,7 +223,7 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
 {
     using V4 = typename Mf<T>::v4;
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));   // wave-uniform
     const int i = w / ngroups, g = w - i * ngroups;
     if (i >= nrows) return;
     constexpr int RT = (B + 15) / 16;
@@ -156,20 +232,45 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
-    const int col_l = lane & 15;
+    const int gc = col0 + NT * (lane & 15);   // this lane's NT output columns (k is a multiple of NT)
+    if (gc >= k) return;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = col0 + 16 * t + col_l;
-            if (col >= k) continue;
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * rt + Mf<T>::row(lane, e);
+            if (r >= B) continue;
+            const size_t o = ((size_t)i * B + r) * k + gc;
+            T v[NT];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 16 * rt + Mf<T>::row(lane, e);
-                if (r >= B) continue;
-                const size_t o = ((size_t)i * B + r) * k + col;
-                y[o] = b ? (T)(b[o] - acc[rt * NT + t][e]) : acc[rt * NT + t][e];
+            for (int t = 0; t < NT; ++t) v[t] = acc[rt * NT + t][e];
+            if (b) {
+                T bv[NT];
+                mf_ld<T, NT>(bv, b + o);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) v[t] = bv[t] - v[t];
             }
+            mf_st<T, NT>(y + o, v);
+        }
+}
+
+// res = b_i - acc on this lane's rows and NT columns gc .. gc + NT (zero
+// outside the block, past k, or for an invalid wave)
+template <typename T, int B, int NT>
+__device__ __forceinline__ void residual_tile(typename Mf<T>::v4* res, const typename Mf<T>::v4* acc, const T* __restrict__ b,
+                                              int i, int k, int gc, bool valid, int lane)
+{
+    constexpr int RT = (B + 15) / 16;
+#pragma unroll
+    for (int rr = 0; rr < RT; ++rr)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * rr + Mf<T>::row(lane, e);
+            const bool in = valid && r < B && gc < k;
+            T bv[NT];
+            if (in) mf_ld<T, NT>(bv, b + ((size_t)i * B + r) * k + gc);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) res[rr * NT + t][e] = in ? (T)(bv[t] - acc[rr * NT + t][e]) : (T)0;
         }
 }
 
@@ -223,7 +324,7 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     __shared__ T lds_all[4 * 16 * RT * 16 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     T* lds = lds_all + wv * 16 * RT * 16 * NT;
-    const int w = blockIdx.x * 4 + wv;
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + wv);   // wave-uniform
     const int ri = w / ngroups, g = w - ri * ngroups;
     const bool valid = ri < nrows;          // every wave reaches the barrier in dinv_times
     const int i = valid ? rows[ri] : 0;
@@ -231,36 +332,30 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur, pidx);
+    if (valid && !reads_none<SKIP>(cur)) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur, pidx);
+    const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
     V4 res[RT * NT];
-#pragma unroll
-    for (int rr = 0; rr < RT; ++rr)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = col0 + 16 * t + col_l;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 16 * rr + Mf<T>::row(lane, e);
-                res[rr * NT + t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[rr * NT + t][e]) : (T)0;
-            }
-        }
+    residual_tile<T, B, NT>(res, acc, b, i, k, gc, valid, lane);
     V4 out[RT * NT];
     dinv_times<T, B, NT>(out, res, Dinv + (size_t)(didx ? didx[i] : i) * B * B, lds, lane);
-    if (!valid) return;
+    if (!valid || gc >= k) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = col0 + 16 * t + col_l;
-            if (col >= k) continue;
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * rr + Mf<T>::row(lane, e);
+            if (r >= B) continue;
+            const size_t o = ((size_t)i * B + r) * k + gc;
+            T xv[NT], nv[NT];
+            if (SKIP) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 16 * rr + Mf<T>::row(lane, e);
-                if (r >= B) continue;
-                const size_t o = ((size_t)i * B + r) * k + col;
-                const T xi = SKIP ? (T)0 : x[o];
-                x[o] = xi + omega * (out[rr * NT + t][e] - xi);
+                for (int t = 0; t < NT; ++t) xv[t] = (T)0;
+            } else {
+                mf_ld<T, NT>(xv, x + o);
             }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) nv[t] = xv[t] + omega * (out[rr * NT + t][e] - xv[t]);
+            mf_st<T, NT>(x + o, nv);
         }
 }
 
@@ -277,7 +372,7 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     __shared__ T lds_all[4 * 16 * RT * 16 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     T* lds = lds_all + wv * 16 * RT * 16 * NT;
-    const int w = blockIdx.x * 4 + wv;
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + wv);   // wave-uniform
     const int i0 = w / ngroups, g = w - i0 * ngroups;
     const bool valid = i0 < nrows;
     const int i = valid ? i0 : 0;
@@ -286,34 +381,24 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
+    const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
     V4 res[RT * NT];
-#pragma unroll
-    for (int rr = 0; rr < RT; ++rr)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = col0 + 16 * t + col_l;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 16 * rr + Mf<T>::row(lane, e);
-                res[rr * NT + t][e] = (valid && r < B && col < k) ? (T)(b[((size_t)i * B + r) * k + col] - acc[rr * NT + t][e]) : (T)0;
-            }
-        }
+    residual_tile<T, B, NT>(res, acc, b, i, k, gc, valid, lane);
     V4 out[RT * NT];
     dinv_times<T, B, NT>(out, res, Dinv + (size_t)(didx ? didx[i] : i) * B * B, lds, lane);
-    if (!valid) return;
+    if (!valid || gc >= k) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = col0 + 16 * t + col_l;
-            if (col >= k) continue;
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * rr + Mf<T>::row(lane, e);
+            if (r >= B) continue;
+            const size_t o = ((size_t)i * B + r) * k + gc;
+            T xv[NT], nv[NT];
+            mf_ld<T, NT>(xv, x + o);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 16 * rr + Mf<T>::row(lane, e);
-                if (r >= B) continue;
-                const size_t o = ((size_t)i * B + r) * k + col;
-                xout[o] = x[o] + omega * out[rr * NT + t][e];
-            }
+            for (int t = 0; t < NT; ++t) nv[t] = xv[t] + omega * out[rr * NT + t][e];
+            mf_st<T, NT>(xout + o, nv);
         }
 }
 
@@ -433,6 +518,14 @@ void check(const char* what)
     }
 
 bool mf_supported(int B) { return B == 6 || B == 8 || B == 9 || B == 10 || B == 16 || B == 27; }
+
+// a wave's NT tiles read and write NT contiguous columns as one vector, so NT
+// must divide k: the largest of nt, nt / 2, .., 1 that does
+static int fit_tiles(int nt, int k)
+{
+    while (nt > 1 && k % nt) nt >>= 1;
+    return nt;
+}
 int mf_max_tiles(int B) { return B > 16 ? 2 : 4; }
 
 template <typename T>
@@ -440,6 +533,7 @@ void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* row
                   const T* b, T* y, const int* pidx)
 {
     if (nrows <= 0 || k <= 0) return;
+    nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
                                                 nrows, ng, rowptr, cols, A, x, b, y, pidx));
@@ -452,6 +546,7 @@ void MF<T>::gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* 
                      const int* pidx, const int* didx)
 {
     if (nrows <= 0 || k <= 0) return;
+    nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     const dim3 g(nblocks((long)nrows * ng));
     if (skip == 1) {
@@ -472,6 +567,7 @@ void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* ro
                    const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* pidx, const int* didx)
 {
     if (nrows <= 0 || k <= 0) return;
+    nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_jacobi<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
                                                 nrows, ng, rowptr, cols, A, Dinv, b, x, xout, omega, pidx, didx));

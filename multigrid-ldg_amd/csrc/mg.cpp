@@ -926,7 +926,8 @@ void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) con
     for (int cc = 0; cc < nc; ++cc) {
         const int c = desc ? nc - 1 - cc : cc;
         const int skip = first_from_zero ? (desc ? 2 : 1) : 0;
-        fn(rows + off[c], dsc + 4 * (size_t)off[c], gcl + Sgw(lv) * off[c], off[c + 1] - off[c], cdev, c, skip);
+        const int cur = (skip && cc == 0) ? (desc ? dev::kFirstRev : dev::kFirstFwd) : c;
+        fn(rows + off[c], dsc + 4 * (size_t)off[c], gcl + Sgw(lv) * off[c], off[c + 1] - off[c], cdev, cur, skip);
     }
 }
 
@@ -947,13 +948,17 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
             const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
             auto one = [&](int c) {
                 const int o = lv.color_off[c];
+                const int cur = (skip && c == 0) ? dev::kFirstFwd : c;   // (forward: colour 0 goes first)
                 dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o,
                                       lv.color_gcols.p + Sgw(lv) * o, lv.gw, Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega,
-                                      lv.color_dev, c, skip, Spool(lv));
+                                      lv.color_dev, cur, skip, Spool(lv));
             };
             auto pair = [&] {
+                // the pair goes first in reverse, and forward when there are only two colours
+                const int c0 = reverse ? nc - 1 : nc - 2;
+                const int cur0 = (skip && (reverse || nc == 2)) ? (reverse ? dev::kFirstRev : dev::kFirstFwd) : c0;
                 dev::Ops<T>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, reverse ? 1 : 0, lv.pair_desc, lv.pair_gcols, lv.gw,
-                                     Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega, lv.color_dev, reverse ? nc - 1 : nc - 2,
+                                     Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega, lv.color_dev, cur0,
                                      reverse ? nc - 2 : nc - 1, skip, Spool(lv));
             };
             if (!reverse) { for (int c = 0; c < nc - 2; ++c) one(c); pair(); }
