@@ -254,11 +254,52 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
         }
 }
 
-// res = b_i - acc on this lane's rows and NT columns gc .. gc + NT (zero
-// outside the block, past k, or for an invalid wave)
+// The operands of a smoother's row that do not depend on its neighbour sum:
+// b_i and x_i on this lane's rows and NT columns gc .. gc + NT (zero outside
+// the block, past k, or for an invalid wave), and the row's D^-1 entries in
+// the A-operand layout.  They are loaded before the neighbour sum, so that
+// they are in flight with it instead of after it.
 template <typename T, int B, int NT>
-__device__ __forceinline__ void residual_tile(typename Mf<T>::v4* res, const typename Mf<T>::v4* acc, const T* __restrict__ b,
-                                              int i, int k, int gc, bool valid, int lane)
+struct RowTile {
+    static constexpr int RT = (B + 15) / 16, S = (B + 3) / 4;
+    T bt[RT][4][NT], xt[RT][4][NT];
+    T dv[S][RT];
+};
+
+template <typename T, int B, int NT, bool LOADX>
+__device__ __forceinline__ void row_tile_load(RowTile<T, B, NT>& o, const T* __restrict__ b, const T* x,
+                                              const T* __restrict__ D, int i, int k, int gc, bool valid, int lane)
+{
+    constexpr int RT = RowTile<T, B, NT>::RT, S = RowTile<T, B, NT>::S;
+#pragma unroll
+    for (int rr = 0; rr < RT; ++rr)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * rr + Mf<T>::row(lane, e);
+            const bool in = valid && r < B && gc < k;
+            const size_t off = ((size_t)i * B + r) * k + gc;
+            if (in) mf_ld<T, NT>(o.bt[rr][e], b + off);
+            if (LOADX && in) mf_ld<T, NT>(o.xt[rr][e], x + off);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                if (!in) o.bt[rr][e][t] = (T)0;
+                if (!LOADX || !in) o.xt[rr][e][t] = (T)0;
+            }
+        }
+    const int r16 = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int c = 4 * s + q, r = 16 * rt + r16;
+            o.dv[s][rt] = (r < B && c < B) ? panel_at<T, B>(D, r, c) : (T)0;
+        }
+}
+
+// res = b_i - acc (acc layout; zero where row_tile_load left b zero)
+template <typename T, int B, int NT>
+__device__ __forceinline__ void residual_tile(typename Mf<T>::v4* res, const typename Mf<T>::v4* acc,
+                                              const RowTile<T, B, NT>& o, int k, int gc, bool valid, int lane)
 {
     constexpr int RT = (B + 15) / 16;
 #pragma unroll
@@ -267,17 +308,15 @@ __device__ __forceinline__ void residual_tile(typename Mf<T>::v4* res, const typ
         for (int e = 0; e < 4; ++e) {
             const int r = 16 * rr + Mf<T>::row(lane, e);
             const bool in = valid && r < B && gc < k;
-            T bv[NT];
-            if (in) mf_ld<T, NT>(bv, b + ((size_t)i * B + r) * k + gc);
 #pragma unroll
-            for (int t = 0; t < NT; ++t) res[rr * NT + t][e] = in ? (T)(bv[t] - acc[rr * NT + t][e]) : (T)0;
+            for (int t = 0; t < NT; ++t) res[rr * NT + t][e] = in ? (T)(o.bt[rr][e][t] - acc[rr * NT + t][e]) : (T)0;
         }
 }
 
 // the D^-1 product of a smoother: r (acc layout, 16 RT x 16 NT) -> LDS -> D_i r
 template <typename T, int B, int NT>
-__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* rt_in, const T* __restrict__ D,
-                                           T* lds, int lane)
+__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* rt_in,
+                                           const RowTile<T, B, NT>& o, T* lds, int lane)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
@@ -300,12 +339,9 @@ __device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typena
 #pragma unroll
         for (int t = 0; t < NT; ++t) xb[t] = (c < B) ? lds[c * (16 * NT) + 16 * t + r16] : (T)0;
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-            const int r = 16 * rt + r16;
-            const T a = (r < B && c < B) ? panel_at<T, B>(D, r, c) : (T)0;
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) out[rt * NT + t] = Mf<T>::mac(a, xb[t], out[rt * NT + t]);
-        }
+            for (int t = 0; t < NT; ++t) out[rt * NT + t] = Mf<T>::mac(o.dv[s][rt], xb[t], out[rt * NT + t]);
     }
 }
 
@@ -332,12 +368,14 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid && !reads_none<SKIP>(cur)) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur, pidx);
     const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
+    RowTile<T, B, NT> rt;
+    row_tile_load<T, B, NT, SKIP == 0>(rt, b, x, Dinv + (size_t)(didx ? didx[i] : i) * B * B, i, k, gc, valid, lane);
+    if (valid && !reads_none<SKIP>(cur)) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur, pidx);
     V4 res[RT * NT];
-    residual_tile<T, B, NT>(res, acc, b, i, k, gc, valid, lane);
+    residual_tile<T, B, NT>(res, acc, rt, k, gc, valid, lane);
     V4 out[RT * NT];
-    dinv_times<T, B, NT>(out, res, Dinv + (size_t)(didx ? didx[i] : i) * B * B, lds, lane);
+    dinv_times<T, B, NT>(out, res, rt, lds, lane);
     if (!valid || gc >= k) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -346,15 +384,9 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
             const int r = 16 * rr + Mf<T>::row(lane, e);
             if (r >= B) continue;
             const size_t o = ((size_t)i * B + r) * k + gc;
-            T xv[NT], nv[NT];
-            if (SKIP) {
+            T nv[NT];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) xv[t] = (T)0;
-            } else {
-                mf_ld<T, NT>(xv, x + o);
-            }
-#pragma unroll
-            for (int t = 0; t < NT; ++t) nv[t] = xv[t] + omega * (out[rr * NT + t][e] - xv[t]);
+            for (int t = 0; t < NT; ++t) nv[t] = rt.xt[rr][e][t] + omega * (out[rr * NT + t][e] - rt.xt[rr][e][t]);   // (SKIP: x_i = 0)
             mf_st<T, NT>(x + o, nv);
         }
 }
@@ -380,12 +412,14 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
     const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
+    RowTile<T, B, NT> rt;
+    row_tile_load<T, B, NT, true>(rt, b, x, Dinv + (size_t)(didx ? didx[i] : i) * B * B, i, k, gc, valid, lane);
+    if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
     V4 res[RT * NT];
-    residual_tile<T, B, NT>(res, acc, b, i, k, gc, valid, lane);
+    residual_tile<T, B, NT>(res, acc, rt, k, gc, valid, lane);
     V4 out[RT * NT];
-    dinv_times<T, B, NT>(out, res, Dinv + (size_t)(didx ? didx[i] : i) * B * B, lds, lane);
+    dinv_times<T, B, NT>(out, res, rt, lds, lane);
     if (!valid || gc >= k) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
@@ -394,10 +428,9 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
             const int r = 16 * rr + Mf<T>::row(lane, e);
             if (r >= B) continue;
             const size_t o = ((size_t)i * B + r) * k + gc;
-            T xv[NT], nv[NT];
-            mf_ld<T, NT>(xv, x + o);
+            T nv[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) nv[t] = xv[t] + omega * out[rr * NT + t][e];
+            for (int t = 0; t < NT; ++t) nv[t] = rt.xt[rr][e][t] + omega * out[rr * NT + t][e];
             mf_st<T, NT>(xout + o, nv);
         }
 }
