@@ -231,8 +231,21 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
-    row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
     const int gc = col0 + NT * (lane & 15);   // this lane's NT output columns (k is a multiple of NT)
+    // b_i of the residual form, in flight with the neighbour sum
+    T bt[RT][4][NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = 16 * rt + Mf<T>::row(lane, e);
+            if (b && r < B && gc < k) mf_ld<T, NT>(bt[rt][e], b + ((size_t)i * B + r) * k + gc);
+            else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bt[rt][e][t] = (T)0;
+            }
+        }
+    row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
     if (gc >= k) return;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -243,13 +256,7 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
             const size_t o = ((size_t)i * B + r) * k + gc;
             T v[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) v[t] = acc[rt * NT + t][e];
-            if (b) {
-                T bv[NT];
-                mf_ld<T, NT>(bv, b + o);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) v[t] = bv[t] - v[t];
-            }
+            for (int t = 0; t < NT; ++t) v[t] = b ? (T)(bt[rt][e][t] - acc[rt * NT + t][e]) : acc[rt * NT + t][e];
             mf_st<T, NT>(y + o, v);
         }
 }
