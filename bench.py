@@ -165,13 +165,25 @@ def unstructured_leg(mgldg, torch, nrhs, warm):
                 f"iterations {out[f'{form}_{prec}']['iterations']}")
     plan = plan_summary(mg.batch_plan())
     log(f"unstructured batched kernel plan: {plan}")
+    # the single solve's fine-level launches against the HBM roofline (HIP
+    # events, 50 launches each; bytes: every block of the operator once, the
+    # D^-1 blocks, indices and the vector streams)
+    roof = {}
+    for name, fn in (("k_jacobi<double,6>", lambda: mg.profile_relax(0, "fp64", 50, part="sweep")[:2]),
+                     ("k_jacobi<float,6>", lambda: mg.profile_relax(0, "mixed", 50, part="sweep")[:2]),
+                     ("k_apply<double,6>", lambda: mg.profile_spmv(0, "fp64", 50))):
+        ms, by = fn()
+        roof[name] = {"us_per_launch": round(ms * 1e3, 2), "bytes_per_launch": int(by),
+                      "achieved": round(by / (ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                      "frac": round(by / (ms * 1e-3) / 1e9 / 8000.0, 4)}
+    log(f"unstructured fine-level launches: {roof}")
     x64, x32 = xs[("batched", "fp64")], xs[("batched", "mixed")]
     diff = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
     return {
         "workload": "unit disk, Delaunay of 200k random interior + 1571 boundary points, p=2, tau0=100, tauD=1000",
         "elements": prob.ne, "ndof": prob.ndof, "levels": mg.nlevels, "level_rows": mg.level_rows,
         "smoother": "V(2,2) block Jacobi, omega 0.6", "coarsening": "own agglomeration (<= 4 per level), RAT",
-        "nrhs": nrhs, **out, "kernel_plan": plan,
+        "nrhs": nrhs, **out, "kernel_plan": plan, "fine_level_launches": roof,
         "mixed_batched_vs_fp64_single": round(out["batched_mixed"]["value"] / out["single_fp64"]["value"], 3),
         "mixed_vs_fp64_batched_rel_diff": diff,
         "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
