@@ -1338,7 +1338,12 @@ void MGH::tune_batch(int k, int precision)
             Level& lv = L[l];
             BTune& t = btune[p][l];
             if (!dev::mf_supported(lv.B)) continue;
-            if (bforce >= 0) { t.apply = t.smooth = std::min(bforce, dev::mf_max_tiles(lv.B)); continue; }
+            if (bforce >= 0) {
+                int nt = std::min(bforce, dev::mf_max_tiles(lv.B));
+                while (nt > 1 && k % nt) nt >>= 1;                    // (as the launchers do)
+                t.apply = t.smooth = nt;
+                continue;
+            }
             const long m = (long)lv.nb * lv.B * k;
             auto run = [&](auto tag) {
                 using T = decltype(tag);
@@ -1350,6 +1355,7 @@ void MGH::tune_batch(int k, int precision)
                     const int nt = cand[ci];
                     if (nt > 1 && 16 * nt / 2 >= k) continue;      // more tiles than columns
                     if (nt > dev::mf_max_tiles(lv.B)) continue;
+                    if (nt > 1 && k % nt) continue;                 // a wave's nt columns are one vector access
                     t.apply = nt; t.smooth = nt;
                     const float ma = timed([&] { bapply<T>(l, k, bx<T>(v), bb<T>(v), br<T>(v)); });
                     const float msm = timed([&] { brelax<T>(l, k, 1, false, false); });
