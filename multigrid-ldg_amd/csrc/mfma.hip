@@ -91,12 +91,15 @@ struct BlkOps {
 };
 
 // Blocks whose operands are loaded before any of their products (one memory
-// round trip per U blocks), sized to keep the operands near 128 VGPRs.
+// round trip per U blocks), sized to keep the operands within 64 VGPRs: the
+// wavefronts are latency-bound, and more of them per SIMD beat more blocks in
+// flight per wavefront (a 128-register budget was 5% slower in FP64, and up
+// to 8 blocks in flight 2-7% slower everywhere).
 template <typename T, int B, int NT>
 __host__ __device__ constexpr int mf_unroll()
 {
     constexpr int per = ((B + 3) / 4) * ((B + 15) / 16 + NT) * (int)(sizeof(T) / 4);
-    return per * 4 <= 128 ? 4 : (per * 3 <= 128 ? 3 : (per * 2 <= 128 ? 2 : 1));
+    return per * 4 <= 64 ? 4 : (per * 3 <= 64 ? 3 : (per * 2 <= 64 ? 2 : 1));
 }
 
 // The lane-dependent, block-independent parts of the operand offsets
