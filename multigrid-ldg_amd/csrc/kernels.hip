@@ -305,7 +305,9 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
                                      const int* __restrict__ color, int cur)
 {
     constexpr int V = Map<T, B>::V;
-    constexpr int U = unroll_of<T, B>();
+    // pooled blocks come from cache: half as many in flight, and more
+    // wavefronts per SIMD (k_gs<float,16> 124 -> 80 VGPRs with the late D^-1)
+    constexpr int U = POOL ? (unroll_of<T, B>() + 1) / 2 : unroll_of<T, B>();
     T acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0;
@@ -487,11 +489,14 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
     if (valid) {
         bi = b[o];
         if (SKIP == 0) xi = x[o];
-        load_dinv_row<T, B>(Dinv, POOL ? d.w : d.x, r, dv);
+        if (!POOL) load_dinv_row<T, B>(Dinv, d.x, r, dv);
     }
     const T acc = reads_none<SKIP>(cur) ? (T)0
                 : row_sum<T, B, SKIP, POOL>(valid, d.y, d.z, POOL ? 0 : d.w, gcols + (size_t)idx * W * (POOL ? 2 : 1), W, A,
                                             x, r, slot * B, xs_all + (wave * R::RW + slot) * R::XS, color, cur);
+    // a pooled D^-1 block is read from cache: loaded after the sum, it holds
+    // no registers across it (more wavefronts per SIMD)
+    if (POOL && valid) load_dinv_row<T, B>(Dinv, d.w, r, dv);
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
     wave_sync();
@@ -568,10 +573,11 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     if (valid) {
         bi = b[o];
         xi = x[o];
-        load_dinv_row<T, B>(Dinv, POOL ? didx[i] : i, r, dv);
+        if (!POOL) load_dinv_row<T, B>(Dinv, i, r, dv);
     }
     const T acc = row_sum<T, B, 0, POOL>(valid, k0, k1, -1, ell + (size_t)i * W * (POOL ? 2 : 1), W, A, x, r, slot * B,
                                          xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
+    if (POOL && valid) load_dinv_row<T, B>(Dinv, didx[i], r, dv);   // (as in gs_rows)
     T* t = tb + (wave * R::RW + slot) * B;
     t[r] = bi - acc;
     wave_sync();
