@@ -212,7 +212,10 @@ int mgldg_mg_level(mgldg_mg mg, int level, int* nb, int* B, long* nnzb);
  * applications and smoothers: mode -1 (default) = picked per level by the
  * library's autotuner when a batch size is first used; 0 = vector kernels;
  * 1, 2, 4 = MFMA tiles of 16, 32, 64 columns on levels whose block size has
- * them (6, 8, 9, 10, 16).  Takes effect at the next batch-size change. */
+ * them (6, 8, 9, 10, 16; up to 2 tiles for 27).  A wavefront reads and writes
+ * its tile columns as one vector per row, so a tile count that does not
+ * divide the batch size runs as the largest of 2, 1 that does.  Takes effect
+ * at the next batch-size change. */
 int mgldg_mg_batch_kernels(mgldg_mg mg, int mode);
 /* the choices in use for the current batch: plan[2*l] (operator
  * application) and plan[2*l+1] (smoother) per level, FP64 then FP32 (levels
