@@ -305,9 +305,10 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
                                      const int* __restrict__ color, int cur)
 {
     constexpr int V = Map<T, B>::V;
-    // pooled blocks come from cache: half as many in flight, and more
-    // wavefronts per SIMD (k_gs<float,16> 124 -> 80 VGPRs with the late D^-1)
-    constexpr int U = POOL ? (unroll_of<T, B>() + 1) / 2 : unroll_of<T, B>();
+    // pooled blocks come from cache: one block in flight, and more
+    // wavefronts per SIMD instead (k_gs<float,16> 124 -> 64 VGPRs with the
+    // late D^-1: 8 wavefronts per SIMD, a fine colour's rows all resident)
+    constexpr int U = POOL ? 1 : unroll_of<T, B>();
     T acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0;
