@@ -910,7 +910,9 @@ template <> const float* Pool<float>(const Level& lv) { return lv.pool32.p; }
 // The GS launches of one sweep in order.  Multicolour: colours ascending
 // (forward) or descending (reverse).  Lexicographic: forward wavefronts
 // ascending, or reverse-order wavefronts ascending.  skip marks the first
-// sweep from x = 0 (rows after the current launch in sweep order are zero).
+// sweep from x = 0 (rows after the current launch in sweep order are zero);
+// fn receives the colour to pass to the kernels as `cur`, which for the first
+// launch of such a sweep is dev::kFirstFwd / kFirstRev (it reads no neighbour).
 template <typename F>
 void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) const
 {
