@@ -943,6 +943,11 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
     hipStream_t s = ctx->stream;
     const T omega = (T)prm.omega;
     if (Aop<T>(lv) == nullptr) throw Error(-4, "precision not built for this hierarchy");
+    // zero_guess means x starts at zero whatever n is: Gauss-Seidel's first
+    // sweep and Jacobi's first sweep read no x, but with no sweep at all x
+    // must be cleared here (the reference zeroes x before relaxing,
+    // multigrid.h:376)
+    if (zero_guess && n == 0) { dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x); return; }
     if (prm.relaxation == 1 && lv.pair_nwg > 0) {
         // colours ascending (descending in reverse), the last two in one launch
         const int nc = lv.ncolor;
@@ -978,10 +983,7 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
         // x = 0 the first sweep is omega D^-1 b (no A x, no clearing of x).
         T* cur = v.x.p; T* alt = v.xalt.p;
         int sw0 = 0;
-        if (zero_guess) {
-            if (n == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B, (T)0, v.x);
-            else { dev::Ops<T>::jacobi0(s, lv.B, lv.nb, SDop<T>(lv), v.b, alt, omega, Sdidx(lv)); std::swap(cur, alt); sw0 = 1; }
-        }
+        if (zero_guess) { dev::Ops<T>::jacobi0(s, lv.B, lv.nb, SDop<T>(lv), v.b, alt, omega, Sdidx(lv)); std::swap(cur, alt); sw0 = 1; }
         for (int sw = sw0; sw < n; ++sw) {
             dev::Ops<T>::jacobi(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), SDop<T>(lv), v.b, cur, alt, omega,
                                 Sdidx(lv));
@@ -1007,10 +1009,8 @@ void MGH::vcycle_from(int l0, bool x0_zero)
                            Spool(lv));
         LevelVecs<T>& vc = vecs<T>(l + 1);
         dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
-        // x_{l+1} = 0 (multigrid.h:376): implicit in the first sweep of a
-        // Gauss-Seidel pre-smoother, explicit otherwise
-        if (l + 1 < bottom && !(prm.relaxation >= 1 && prm.npre > 0))
-            dev::Ops<T>::fill(s, (long)L[l + 1].nb * L[l + 1].B, (T)0, vc.x);
+        // x_{l+1} = 0 (multigrid.h:376): relax() from a zero guess starts
+        // there (its first sweep reads no x; with no pre-sweep it clears x)
         zero = true;
     }
     Level& c = L[bottom];
@@ -1119,14 +1119,10 @@ void MGH::build_collapse()
 // caller puts the right-hand side in L[0].v*.b and reads the result in L[0].v*.x
 void MGH::precon_enqueue(int precision)
 {
-    hipStream_t s = ctx->stream;
-    const long n = pr->ndof;
     if (precision == 1) {
         if (!L[0].A32.p) throw Error(-4, "mixed precision was not built (mg params precision)");
-        if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, L[0].v32.x);
         vcycle_from<float>(0);
     } else {
-        if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<double>::fill(s, n, 0.0, L[0].v64.x);
         vcycle_from<double>(0);
     }
 }
@@ -1675,7 +1671,6 @@ void MGH::vcycles(int precision, const double* b, double* x, int ncycle, double 
             dev::Ops<double>::copy(s, n, f.v64.x, x);
         } else {
             dev::f64_to_f32(s, n, r, f.v32.b);
-            if (!(prm.relaxation >= 1 && prm.npre > 0)) dev::Ops<float>::fill(s, n, 0.f, f.v32.x);
             vcycle_from<float>(0, true);
             dev::f32_to_f64(s, n, f.v32.x, q);
             dev::axpby(s, n, 1.0, q, 1.0, x, x);

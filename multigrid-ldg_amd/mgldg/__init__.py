@@ -365,14 +365,25 @@ class Multigrid(_Handle):
         return c, nc.value
 
     # unit-parity hooks on device tensors (torch) -----------------------------
+    def _level_n(self, level):
+        if not 0 <= level < self.nlevels:
+            raise MGLDGError(-1, f"level {level} out of range (0..{self.nlevels - 1})")
+        return self.level_rows[level] * self.level_B[level]
+
     def spmv(self, level, x, y, precision="fp64"):
-        check(self.ctx, lib().mgldg_spmv_device(self.h, level, _prec(precision), _ptr(x), _ptr(y), 1))
+        """y = A_l x; x, y of the level's length, float64 (fp64) or float32 (mixed)"""
+        n, dt = self._level_n(level), _vdtype(precision)
+        check(self.ctx, lib().mgldg_spmv_device(self.h, level, _prec(precision), _ptr(x, n, dt, "x"), _ptr(y, n, dt, "y"), 1))
 
     def smooth(self, level, b, x, nsweeps=1, reverse=False, precision="fp64"):
-        check(self.ctx, lib().mgldg_smooth_device(self.h, level, _prec(precision), nsweeps, int(reverse), _ptr(b), _ptr(x), 1))
+        n, dt = self._level_n(level), _vdtype(precision)
+        check(self.ctx, lib().mgldg_smooth_device(self.h, level, _prec(precision), nsweeps, int(reverse), _ptr(b, n, dt, "b"),
+                                                  _ptr(x, n, dt, "x"), 1))
 
     def vcycle(self, b, x, precision="fp64"):
-        check(self.ctx, lib().mgldg_vcycle_device(self.h, _prec(precision), _ptr(b), _ptr(x), 1))
+        """x = V(b) from zero; b and x are float64 of the fine length in either precision"""
+        n = self.prob.ndof
+        check(self.ctx, lib().mgldg_vcycle_device(self.h, _prec(precision), _ptr(b, n, name="b"), _ptr(x, n, name="x"), 1))
 
     def profile_relax(self, level=0, precision="fp64", nrep=10, part="sweep"):
         """(ms per smoother launch, algorithmic bytes per launch, launches);
@@ -484,11 +495,34 @@ def _prec(p):
     return {"fp64": 0, "mixed": 1, "fp32": 1}[p]
 
 
-def _ptr(t):
-    """device pointer of a torch tensor (contiguous, on the GPU)"""
-    if not hasattr(t, "data_ptr") or not t.is_cuda or not t.is_contiguous():
-        raise MGLDGError(-1, "expected a contiguous device tensor")
+def _vdtype(p):
+    """element type of the level vectors of the unit hooks in a precision"""
+    return "float64" if _prec(p) == 0 else "float32"
+
+
+def _ptr(t, numel=None, dtype="float64", name="tensor"):
+    """Device pointer of a torch tensor, checked before anything reaches the
+    library: the entry points read and write numel elements of dtype
+    ("float64" or "float32") at that address, so a tensor of another type or
+    length would be an out-of-bounds device access."""
+    if not hasattr(t, "data_ptr"):
+        raise MGLDGError(-1, f"{name}: expected a torch tensor")
+    if str(t.dtype) != "torch." + dtype:
+        raise MGLDGError(-1, f"{name}: expected dtype {dtype}, got {t.dtype}")
+    if numel is not None and t.numel() != numel:
+        raise MGLDGError(-1, f"{name}: expected {numel} elements, got {t.numel()} (shape {tuple(t.shape)})")
+    if not t.is_contiguous():
+        raise MGLDGError(-1, f"{name}: expected a contiguous tensor")
+    if not t.is_cuda:
+        raise MGLDGError(-1, f"{name}: expected a device tensor")
     return C.c_void_p(t.data_ptr())
+
+
+def _cols(t, rows, name):
+    """number of right-hand sides of a [rows] or [rows, k] tensor"""
+    if t.ndim not in (1, 2) or t.shape[0] != rows:
+        raise MGLDGError(-1, f"{name}: expected shape ({rows},) or ({rows}, k), got {tuple(t.shape)}")
+    return 1 if t.ndim == 1 else int(t.shape[1])
 
 
 def pcg(mg, b, x0=None, tol=1e-8, maxit=200, criterion="reference", flexible=False, precision="fp64",
@@ -500,16 +534,24 @@ def pcg(mg, b, x0=None, tol=1e-8, maxit=200, criterion="reference", flexible=Fal
     sp = SolveParams(tol, maxit, {"reference": 0, "true": 1}[criterion], int(flexible), _prec(precision),
                      int(use_graph))
     is_dev = hasattr(b, "data_ptr")
-    nrhs = 1 if b.ndim == 1 else b.shape[1]
-    st = (SolveStats * nrhs)()
+    n = mg.prob.ndof
     if is_dev:
         import torch
+        nrhs = _cols(b, n, "b")
         x = torch.zeros_like(b) if x0 is None else x0
+        pb, px = _ptr(b, n * nrhs, name="b"), _ptr(x, n * nrhs, name="x0")
+        st = (SolveStats * nrhs)()
         torch.cuda.synchronize()
-        check(mg.ctx, lib().mgldg_solve_device(mg.h, _ptr(b), _ptr(x), nrhs, C.byref(sp), st))
+        check(mg.ctx, lib().mgldg_solve_device(mg.h, pb, px, nrhs, C.byref(sp), st))
     else:
         bb = np.ascontiguousarray(b, dtype=np.float64)
+        if bb.ndim not in (1, 2) or bb.shape[0] != n:
+            raise MGLDGError(-1, f"b: expected shape ({n},) or ({n}, k), got {bb.shape}")
+        nrhs = 1 if bb.ndim == 1 else bb.shape[1]
         x = np.zeros_like(bb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        if x.shape != bb.shape:
+            raise MGLDGError(-1, f"x0: expected shape {bb.shape}, got {x.shape}")
+        st = (SolveStats * nrhs)()
         check(mg.ctx, lib().mgldg_solve(mg.h, _pd(bb), _pd(x), nrhs, C.byref(sp), st))
     stats = [dict(iterations=s.iterations, status=STATUS.get(s.status, s.status), relres_true=s.relres_true,
                   relres_prec=s.relres_prec, seconds=s.seconds) for s in st]
@@ -527,6 +569,9 @@ def vcycles(mg, b, x0=None, ncycle=10, tol=0.0, precision="fp64"):
     array.  Returns (x, residual norms before and after each cycle run)."""
     import torch
     is_dev = hasattr(b, "data_ptr")
+    n = mg.prob.ndof
+    if is_dev:
+        _ptr(b, n, name="b")
     bd = b if is_dev else torch.tensor(np.ascontiguousarray(b, dtype=np.float64), device="cuda")
     if x0 is None:
         xd = torch.zeros_like(bd)
@@ -534,10 +579,11 @@ def vcycles(mg, b, x0=None, ncycle=10, tol=0.0, precision="fp64"):
         xd = x0
     else:
         xd = torch.tensor(np.asarray(x0, dtype=np.float64), device="cuda")
+    pb, px = _ptr(bd, n, name="b"), _ptr(xd, n, name="x0")
     rn = np.zeros(int(ncycle) + 1)
     cyc = C.c_int()
     torch.cuda.synchronize()
-    check(mg.ctx, lib().mgldg_vcycles_device(mg.h, _prec(precision), _ptr(bd), _ptr(xd), int(ncycle), float(tol), _pd(rn),
+    check(mg.ctx, lib().mgldg_vcycles_device(mg.h, _prec(precision), pb, px, int(ncycle), float(tol), _pd(rn),
                                             C.byref(cyc)))
     return (xd if is_dev else xd.cpu().numpy()), rn[:cyc.value + 1]
 
@@ -547,9 +593,10 @@ def mean_zero(ctx, v):
     FP64 device tensor [n] or [n, k] (each column); the driver applies it to
     the right-hand side of Neumann and periodic problems (dg_tests.cpp:282-286)."""
     import torch
-    nrhs = 1 if v.ndim == 1 else v.shape[1]
+    nrhs = _cols(v, v.shape[0] if v.ndim else -1, "v")
+    pv = _ptr(v, v.shape[0] * nrhs, name="v")
     torch.cuda.synchronize()
-    check(ctx, lib().mgldg_mean_zero_device(ctx.h, v.shape[0], _ptr(v), nrhs))
+    check(ctx, lib().mgldg_mean_zero_device(ctx.h, v.shape[0], pv, nrhs))
     return v
 
 
@@ -557,12 +604,16 @@ def rhs_device(mg, f=None, g=None):
     """b = M f + B_g g on the GPU (ldgpoisson.tpp:103-176).  f [ndof(,k)] and g
     [nqb(,k)] are FP64 device tensors (either may be None)."""
     import torch
+    if f is None and g is None:
+        raise MGLDGError(-1, "rhs_device: f and g are both None")
+    n, nq = mg.prob.ndof, mg.prob.nqb
+    k = _cols(f, n, "f") if f is not None else _cols(g, nq, "g")
+    pf = None if f is None else _ptr(f, n * k, name="f")
+    pg = None if g is None else _ptr(g, nq * k, name="g")
     ref = f if f is not None else g
-    k = 1 if ref.ndim == 1 else ref.shape[1]
-    n = mg.prob.ndof
     b = torch.empty((n,) if k == 1 else (n, k), dtype=torch.float64, device=ref.device)
     torch.cuda.synchronize()
-    check(mg.ctx, lib().mgldg_rhs_device(mg.h, None if f is None else _ptr(f), None if g is None else _ptr(g), k, _ptr(b)))
+    check(mg.ctx, lib().mgldg_rhs_device(mg.h, pf, pg, k, _ptr(b, n * k, name="b")))
     return b
 
 
@@ -570,7 +621,10 @@ def adjoint(mg, dLdu, tol=1e-12, maxit=500, criterion="true", flexible=False, pr
     """Adjoint solve through the same MG-PCG: lambda = A^{-1} dL/du, then
     dL/df = M lambda and dL/dg = B_g^T lambda.  Returns (lambda, dLdf, dLdg, stats)."""
     import torch
-    k = 1 if dLdu.ndim == 1 else dLdu.shape[1]
+    n = mg.prob.ndof
+    k = _cols(dLdu, n, "dLdu")
+    dLdu = dLdu.contiguous()
+    _ptr(dLdu, n * k, name="dLdu")
     sp = SolveParams(tol, maxit, {"reference": 0, "true": 1}[criterion], int(flexible), _prec(precision), int(use_graph))
     st = (SolveStats * k)()
     lam = torch.empty_like(dLdu)
@@ -588,8 +642,10 @@ def adjoint_maps(mg, lam):
     """dL/df = M lam and dL/dg = B_g^T lam for adjoint states solved elsewhere
     (device tensors [ndof(,k)]); returns (dLdf, dLdg)."""
     import torch
-    k = 1 if lam.ndim == 1 else lam.shape[1]
+    n = mg.prob.ndof
+    k = _cols(lam, n, "lam")
     lam = lam.contiguous()
+    _ptr(lam, n * k, name="lam")
     dLdf = torch.empty_like(lam)
     nq = mg.prob.nqb
     dLdg = torch.empty((nq,) if k == 1 else (nq, k), dtype=torch.float64, device=lam.device)
@@ -611,16 +667,28 @@ def _make_autograd():
         def forward(ctx, f, g, mg, opts):
             b = rhs_device(mg, f, g)
             u = torch.zeros_like(b)
-            pcg(mg, b, u, **opts)
+            _, st = pcg(mg, b, u, **opts)
+            _require_converged("forward solve", st)
             ctx.mg, ctx.opts = mg, opts
+            ctx.has_f, ctx.has_g = f is not None, g is not None
             return u
 
         @staticmethod
         def backward(ctx, gu):
-            _, dLdf, dLdg, _ = adjoint(ctx.mg, gu.contiguous(), **ctx.opts)
-            return dLdf, dLdg, None, None
+            _, dLdf, dLdg, st = adjoint(ctx.mg, gu.contiguous(), **ctx.opts)
+            _require_converged("adjoint solve", st)
+            return (dLdf if ctx.has_f else None), (dLdg if ctx.has_g else None), None, None
 
     return LDGSolve
+
+
+def _require_converged(what, st):
+    """a solve that stopped at maxit, broke down or met NaN/Inf must not feed
+    autograd a wrong u or gradient"""
+    for c, d in enumerate(st if isinstance(st, list) else [st]):
+        if d["status"] != "converged":
+            raise MGLDGError(-7, f"{what}: column {c} {d['status']} after {d['iterations']} iterations "
+                                 f"(relative residual {d['relres_true']:.3e})")
 
 
 _ldg_solve_cls = None

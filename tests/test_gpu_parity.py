@@ -262,6 +262,27 @@ def test_collapsed_vcycle_equals_levelwise(gctx, oracle, coarse):
         assert rel(z, zo) < 1e-5
 
 
+@pytest.mark.parametrize("relaxation,orelax", [("gs", "mcgs"), ("jacobi", "jacobi")])
+def test_vcycle_without_presmoothing_is_stateless(gctx, oracle, relaxation, orelax):
+    """V(0,2) with relaxation on the coarsest level: every level's x starts at
+    zero (multigrid.h:376) although no pre-sweep clears it, so two V-cycles
+    in a row (the second after one on another right-hand side) both equal
+    the oracle's"""
+    import mgldg
+    import torch
+    qt, pr, oqt, opr = build_pair(gctx, oracle, "adaptive", 2)
+    mg = mgldg.Multigrid(pr, relaxation, npre=0, npost=2, coarse="relaxation", coarse_dense_max=0, precision="mixed")
+    omg = oracle.Multigrid(opr, orelax, npre=0, npost=2, coarse="relaxation")
+    rng = np.random.default_rng(11)
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+        for _ in range(2):
+            bh = rng.standard_normal(pr.ndof)
+            b = torch.tensor(bh, device="cuda")
+            z = torch.zeros_like(b)
+            mg.vcycle(b, z, prec)
+            assert rel(z.cpu().numpy(), omg.vcycle(bh)) < tol, prec
+
+
 def test_solve_statuses(gctx, oracle):
     import mgldg
     qt, pr, _, _ = build_pair(gctx, oracle, "uniform8", 1)
