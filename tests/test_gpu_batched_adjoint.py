@@ -107,35 +107,52 @@ def test_autograd_gradients_against_finite_differences(gctx, oracle):
 @pytest.mark.parametrize("case", ["quadtree_p3", "quadtree_p2", "simplex_p2", "ball_p2"])
 def test_mfma_batched_kernels(gctx, oracle, case):
     """The MFMA block kernels (forced 1, 2, 4 tiles of 16 columns, and the
-    autotuned plan) give the vector kernels' batched solves: block sizes 16
-    (p = 3 quadtree), 9 (p = 2 quadtree) and 6 / 9 (p = 2 triangles with box
-    coarse levels), GS and Jacobi, FP64 and FP32 V-cycles."""
+    autotuned plan) and the vector kernels against the oracle's PCG
+    (krylov.h:22-68) column by column under the same colouring and
+    hierarchy: block sizes 16 (p = 3 quadtree), 9 (p = 2 quadtree), 6 / 9
+    (p = 2 triangles with box coarse levels) and 10 / 27 (p = 2 tetrahedra,
+    two MFMA row tiles), GS and Jacobi, FP64 and FP32 V-cycles.  FP64
+    quadtree solves take the oracle's iteration count exactly; mixed solves
+    and agglomerated hierarchies (whose coarse bases are chosen
+    independently, so iterates agree only up to rounding) within one."""
     import os
     import mgldg
     import torch
+    here = os.path.dirname(os.path.abspath(__file__))
     if case.startswith("quadtree"):
         p = 3 if case.endswith("p3") else 2
         qt, pr, _, opr = build_pair(gctx, oracle, "adaptive", p, rapid=True)
         relax = "gs"
         mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, precision="mixed", coarse_dense_max=0)
+        omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2, restriction="ratrat")
     elif case == "simplex_p2":
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", "circle2.mesh")
+        path = os.path.join(here, "golden", "meshes", "circle2.mesh")
         wf = mgldg.Wireframe.read(gctx, path)
         pr = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
         relax = "jacobi"
+        parts = mgldg.agglomerate(pr)
         mg = mgldg.Multigrid(pr, relax, omega=0.6, npre=2, npost=2, restriction="rat", precision="mixed",
-                             coarse_dense_max=0)
+                             coarse_dense_max=0, partitions=parts)
+        pts, tris = oracle.read_mesh(path)
+        opr = oracle.SxProblem(pts, tris, 2, tau0=100.0, tauD=1000.0)
+        omg = oracle.SxMultigrid(opr, parts, relaxation="jacobi", omega=0.6, npre=2, npost=2, restriction="rat")
     else:   # tetrahedra: 10-wide fine blocks, 27-wide box blocks (two MFMA row tiles)
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", "ball.mesh")
+        path = os.path.join(here, "golden", "meshes", "ball.mesh")
         wf = mgldg.Wireframe.read(gctx, path, N=3)
         pr = mgldg.SimplexLDGPoisson(wf, 2, tau0=100.0, tauD=1000.0)
         relax = "gs"
-        mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, restriction="rat", precision="mixed", coarse_dense_max=0)
+        parts = mgldg.agglomerate(pr)
+        mg = mgldg.Multigrid(pr, relax, npre=2, npost=2, restriction="rat", precision="mixed", coarse_dense_max=0,
+                             partitions=parts)
+        pts, tets = oracle.read_mesh(path)
+        opr = oracle.SxProblem(pts, tets, 2, tau0=100.0, tauD=1000.0)
+        omg = oracle.SxMultigrid(opr, parts, relaxation="mcgs", npre=2, npost=2, restriction="rat")
     rng = np.random.default_rng(31)
     # a partial last tile; on the p = 3 quadtree also a k that 4 does not
     # divide (a wave's NT columns are one vector access: 4 tiles run as 2)
     k = 22 if case == "quadtree_p3" else 40
-    B = torch.tensor(pr.rhs(rng.standard_normal((pr.ndof, k))), device="cuda")
+    Bh = pr.rhs(rng.standard_normal((pr.ndof, k)))
+    B = torch.tensor(Bh, device="cuda")
     res = {}
     for mode in ("vector", 1, 2, 4, "auto"):
         mg.batch_kernels(mode)
@@ -154,3 +171,16 @@ def test_mfma_batched_kernels(gctx, oracle, case):
             X1, it1 = res[(mode, prec)]
             assert max(abs(a - b) for a, b in zip(it0, it1)) <= 1, (mode, prec)
             assert rel(X1, X0) < 1e-8, (mode, prec)
+    # each column against the oracle: the first and last column of the first
+    # tile, the first of the second, the last of the batch (all of them on
+    # the quadtrees, where the oracle is fast)
+    cols = range(k) if case.startswith("quadtree") else (0, 15, 16, k - 1)
+    exact = case.startswith("quadtree")
+    for c in cols:
+        xo, ito, _, _ = oracle.pcg(opr, omg, Bh[:, c].copy(), tol=1e-10, maxit=300, criterion="true")
+        for mode in ("vector", 1, 2, 4, "auto"):
+            for prec in ("fp64", "mixed"):
+                X, its = res[(mode, prec)]
+                slack = 0 if (exact and prec == "fp64") else 1
+                assert abs(its[c] - ito) <= slack, (c, mode, prec, its[c], ito)
+                assert rel(X[:, c], xo) < (1e-9 if prec == "fp64" and exact else 1e-8), (c, mode, prec)

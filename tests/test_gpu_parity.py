@@ -172,6 +172,53 @@ def test_pcg_matches_oracle(gctx, oracle, mesh, p, criterion, collapse):
     assert rel(xm, xo) < 1e-8
 
 
+_C2_ORACLE = {}
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed", "mixed-graph", "mixed-batched"])
+def test_adaptive_point_rule_pcg_matches_oracle(gctx, oracle, form):
+    """The adaptive benchmark's configuration at depth 7 below the unit square
+    (distance refinement towards (0.3, 0.7), 2:1 balanced, rapid coarsening,
+    p = 3: 10,192 elements, 163,072 dofs) with everything the benchmark turns
+    on: pooled operators, the collapsed coarse tail (8192), V(2,2)
+    multicolour GS, RATRAT.  f ~ N(0,1) per dof.  MG-PCG to 1e-10 on the true
+    residual against the oracle's PCG: FP64 iterations equal, mixed (FP32
+    V-cycle, flexible beta; eager, graph-replayed and as 4 batched columns on
+    the autotuned kernels) within one; solutions to 1e-9 / 1e-8."""
+    import mgldg
+    import torch
+    qt, pr, oqt, opr = build_pair(gctx, oracle, "point", 3, rapid=True, balance=True)
+    assert pr.ndof == 163072
+    mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, precision="mixed")
+    assert mg.level_pool(0) != (0, 0)
+    rng = np.random.default_rng(0)
+    k = 4 if form == "mixed-batched" else 1
+    F = rng.standard_normal((pr.ndof, 4))[:, :k]     # column 0 is the same in every form
+    for c in range(k):
+        if c not in _C2_ORACLE:
+            omg = oracle.Multigrid(opr, "mcgs", npre=2, npost=2, restriction="ratrat")
+            _C2_ORACLE[c] = oracle.pcg(opr, omg, opr.rhs(F[:, c], np.zeros(opr.nqb)), tol=1e-10, maxit=200,
+                                       criterion="true")[:2]
+    prec = "fp64" if form == "fp64" else "mixed"
+    opts = dict(tol=1e-10, maxit=200, criterion="true", precision=prec, flexible=prec == "mixed",
+                use_graph=form == "mixed-graph")
+    if k == 1:
+        x, st = mgldg.pcg(mg, pr.rhs(F[:, 0]), **opts)
+        X, stats = x[:, None], [st]
+    else:
+        Bt = torch.tensor(pr.rhs(F), device="cuda")
+        Xt = torch.zeros_like(Bt)
+        _, stats = mgldg.pcg(mg, Bt, Xt, **opts)
+        X = Xt.cpu().numpy()
+    for c in range(k):
+        xo, ito = _C2_ORACLE[c]
+        assert stats[c]["status"] == "converged" and stats[c]["relres_true"] <= 1e-10
+        if form == "fp64":
+            assert stats[c]["iterations"] == ito and rel(X[:, c], xo) < 1e-9
+        else:
+            assert abs(stats[c]["iterations"] - ito) <= 1 and rel(X[:, c], xo) < 1e-8, (c, stats[c], ito)
+
+
 @pytest.mark.parametrize("criterion", ["reference", "true"])
 def test_config1_iteration_parity(gctx, oracle, criterion):
     """C1: 32x32 p=1, V(2,2) block GS, MG-PCG to 1e-10.  The lexicographic GS
