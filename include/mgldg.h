@@ -187,13 +187,30 @@ int mgldg_mg_create(mgldg_problem pr, const mgldg_mg_params* prm, mgldg_mg* out)
 int mgldg_mg_create_p(mgldg_problem pr, const mgldg_mg_params* prm, int ndeg, const int* degrees, int hcoarsen,
                       mgldg_mg* out);
 /* Agglomeration<N,P>(mesh, maxAgglom) (src/unstructured/mesh/agglomeration.h:21-34)
- * with the library's MGridGen-style agglomerator (clusters of at most 2^N
- * agglomerates of the finer level, chosen for small perimeter^N/volume^(N-1)),
+ * with the library's multilevel agglomerator on each level's dual graph
+ * (mgldg_mgridgen below, minsize 1, maxsize 2^N, as agglomeration.h:96-116),
  * stopping at one agglomerate, max_levels partitions, or a level that no
  * longer shrinks.  Outputs: *nlevels partitions; nparts[l] agglomerates made
  * by partition l; parts[l*ne + i] = agglomerate of level-l element i
  * (capacity ne * max_levels; may be NULL). */
 int mgldg_agglomerate(mgldg_problem pr, int max_levels, int* nlevels, int* nparts, int* parts);
+/* MGridGen(nvtxs, xadj, vvol, vsurf, adjncy, adjwgt, minsize, maxsize,
+ * options, &nmoves, &nparts, part) (lib/ParMGridGen-1.0/MGridGen/mgridgen.h:5-6,
+ * Lib/mgridgen.c:18-49), the call of src/unstructured/mesh/agglomeration.h:
+ * 103-116 and 196-209, by the library's own multilevel agglomerator: rounds
+ * of aspect-ratio matching, then multi-objective k-way refinement with
+ * component splitting, merging and attraction of clusters under minsize.
+ * Same arguments and meaning: the dual graph as CSR (xadj[nvtxs+1],
+ * adjncy / adjwgt = neighbour and shared face area), vvol = volume and
+ * vsurf = boundary-face area per vertex; out: *nparts clusters, part[v].
+ * options = {CType, RType, dbglvl, dim}; only the reference's CType 4 and
+ * RType 6 are built (others: MGLDG_ERR_UNSUPPORTED).  Deterministic (own
+ * random orders, not drand48's), so partitions are of MGridGen's kind and
+ * quality but not its bits.  *nmoves (may be NULL) is set to 0.  Host only;
+ * errors through mgldg_last_error(NULL). */
+int mgldg_mgridgen(int nvtxs, const int* xadj, const double* vvol, const double* vsurf, const int* adjncy,
+                   const double* adjwgt, int minsize, int maxsize, const int* options, int* nmoves, int* nparts,
+                   int* part);
 /* InterpolationHierarchy<N,P>(agglomeration) + multigrid for a simplex problem
  * (src/unstructured/multigrid/interpolationhierarchy.h:88-204; the reference
  * has no unstructured Multigrid and its constructor does not compile, :96,150,217).

@@ -347,6 +347,23 @@ int mgldg_agglomerate(mgldg_problem pr, int max_levels, int* nlevels, int* npart
     });
 }
 
+int mgldg_mgridgen(int nvtxs, const int* xadj, const double* vvol, const double* vsurf, const int* adjncy,
+                   const double* adjwgt, int minsize, int maxsize, const int* options, int* nmoves, int* nparts, int* part)
+{
+    if (nvtxs < 0 || !xadj || !vvol || !vsurf || !options || !nparts || (nvtxs > 0 && !part)) return MGLDG_ERR_ARG;
+    return guard(nullptr, [&] {
+        if (options[0] != 4 || options[1] != 6)
+            throw Error(MGLDG_ERR_UNSUPPORTED, "mgridgen: only CType 4 (aspect-ratio matching) with RType 6 "
+                                               "(multi-objective refinement), the reference's options, are built");
+        ARG(options[3] == 2 || options[3] == 3, "mgridgen: options[3] (dim) must be 2 or 3");
+        ARG(nvtxs == 0 || xadj[nvtxs] == 0 || (adjncy && adjwgt), "mgridgen: adjacency arrays are NULL");
+        std::vector<int> q;
+        *nparts = mgridgen_partition(nvtxs, xadj, vvol, vsurf, adjncy, adjwgt, options[3], minsize, maxsize, q);
+        if (nvtxs > 0) std::memcpy(part, q.data(), q.size() * sizeof(int));
+        if (nmoves) *nmoves = 0;
+    });
+}
+
 int mgldg_mg_create_agglomerated(mgldg_problem pr, const mgldg_mg_params* prm, int nlev, const int* parts, mgldg_mg* out)
 {
     if (!pr || !prm || !out || !parts || nlev < 1) return MGLDG_ERR_ARG;

@@ -551,87 +551,6 @@ void agglomeration_graph(const SimplexMesh& m, const int* orig, int nparts, AggG
     }
 }
 
-int agglomerate_graph(const AggGraph& g, int N, int maxsize, std::vector<int>& part)
-{
-    const int nv = (int)g.vvol.size();
-    // cluster state: members counted in graph vertices
-    std::vector<int> cl(nv);
-    std::iota(cl.begin(), cl.end(), 0);
-    std::vector<int> size(nv, 1);
-    std::vector<double> vol(g.vvol), per(nv);
-    for (int i = 0; i < nv; ++i) {
-        double s = g.vsurf[i];
-        for (int t = g.xadj[i]; t < g.xadj[i + 1]; ++t) s += g.w[t];
-        per[i] = s;
-    }
-    auto ratio = [&](double P, double V) { return std::pow(P, N) / std::pow(V, N - 1); };
-    // current cluster graph (CSR), starts as g
-    std::vector<int> xadj(g.xadj), adj(g.adj);
-    std::vector<double> w(g.w);
-    int nc = nv;
-    for (int round = 0; round < 64; ++round) {
-        // visit the worst-shaped clusters first
-        std::vector<int> order(nc);
-        std::iota(order.begin(), order.end(), 0);
-        std::vector<double> ar(nc);
-        for (int c = 0; c < nc; ++c) ar[c] = ratio(per[c], vol[c]);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ar[a] > ar[b]; });
-        std::vector<int> mate(nc, -1);
-        int merges = 0;
-        for (int u : order) {
-            if (mate[u] >= 0) continue;
-            int best = -1; double bestv = 0;
-            for (int t = xadj[u]; t < xadj[u + 1]; ++t) {
-                const int v = adj[t];
-                if (v == u || mate[v] >= 0 || size[u] + size[v] > maxsize) continue;
-                const double val = ratio(per[u] + per[v] - 2.0 * w[t], vol[u] + vol[v]);
-                if (best < 0 || val < bestv || (val == bestv && v < best)) { best = v; bestv = val; }
-            }
-            if (best >= 0) { mate[u] = best; mate[best] = u; ++merges; }
-        }
-        if (!merges) break;
-        // contract: new ids in order of the smaller member
-        std::vector<int> nid(nc, -1);
-        int k = 0;
-        for (int c = 0; c < nc; ++c) {
-            if (nid[c] >= 0) continue;
-            nid[c] = k;
-            if (mate[c] >= 0) nid[mate[c]] = k;
-            ++k;
-        }
-        std::vector<int> nsize(k, 0);
-        std::vector<double> nvol(k, 0.0), nper(k, 0.0);
-        for (int c = 0; c < nc; ++c) { nsize[nid[c]] += size[c]; nvol[nid[c]] += vol[c]; nper[nid[c]] += per[c]; }
-        for (int c = 0; c < nc; ++c)
-            if (mate[c] > c)
-                for (int t = xadj[c]; t < xadj[c + 1]; ++t) if (adj[t] == mate[c]) nper[nid[c]] -= 2.0 * w[t];
-        // contracted adjacency with summed weights
-        std::vector<std::vector<int>> mem(k);
-        for (int c = 0; c < nc; ++c) mem[nid[c]].push_back(c);
-        std::vector<int> nx(k + 1, 0), na;
-        std::vector<double> nw;
-        std::vector<int> slot(k, -1);
-        for (int a = 0; a < k; ++a) {
-            for (int c : mem[a])
-                for (int t = xadj[c]; t < xadj[c + 1]; ++t) {
-                    const int b = nid[adj[t]];
-                    if (b == a) continue;
-                    if (slot[b] >= 0) { nw[slot[b]] += w[t]; continue; }
-                    slot[b] = (int)na.size();
-                    na.push_back(b); nw.push_back(w[t]);
-                }
-            nx[a + 1] = (int)na.size();
-            for (int t = nx[a]; t < nx[a + 1]; ++t) slot[na[t]] = -1;
-        }
-        for (int i = 0; i < nv; ++i) cl[i] = nid[cl[i]];
-        xadj.swap(nx); adj.swap(na); w.swap(nw);
-        size.swap(nsize); vol.swap(nvol); per.swap(nper);
-        nc = k;
-    }
-    part = cl;
-    return nc;
-}
-
 std::vector<std::vector<int>> agglomerate(const SimplexMesh& m, int max_levels, int maxsize)
 {
     std::vector<std::vector<int>> parts;
@@ -642,7 +561,9 @@ std::vector<std::vector<int>> agglomerate(const SimplexMesh& m, int max_levels, 
         AggGraph g;
         agglomeration_graph(m, l == 0 ? nullptr : orig.data(), n, g);
         std::vector<int> part;
-        const int nc = agglomerate_graph(g, m.N, maxsize, part);
+        // MGridGen's call of agglomeration.h:96-116: minsize 1, maxsize 2^N
+        const int nc = mgridgen_partition(n, g.xadj.data(), g.vvol.data(), g.vsurf.data(), g.adj.data(), g.w.data(), m.N,
+                                          1, maxsize, part);
         if (nc >= n) break;
         for (int e = 0; e < m.ne; ++e) orig[e] = part[orig[e]];
         parts.push_back(std::move(part));

@@ -72,11 +72,18 @@ struct AggGraph {
 };
 void agglomeration_graph(const SimplexMesh& m, const int* orig, int nparts, AggGraph& g);
 
-// MGridGen-style agglomeration of one graph: clusters of at most maxsize
-// vertices grown by rounds of matching that pick, for each cluster, the
-// neighbour giving the smallest aspect ratio perimeter^N / volume^(N-1).
-// Returns the number of agglomerates; part[v] in [0, nparts).
-int agglomerate_graph(const AggGraph& g, int N, int maxsize, std::vector<int>& part);
+// Multilevel agglomeration of one graph (agglomerate.cpp): MGridGen's
+// algorithm (lib/ParMGridGen-1.0/MGridGen/Lib/mgridgen.c:18-110) of
+// aspect-ratio matching, then k-way multi-objective refinement with
+// component splitting, merging and attraction of clusters under minsize.
+// Input as MGridGen's: CSR adjacency with face-area weights, vertex volumes
+// and boundary-face areas.  Returns the number of clusters; part[v] in
+// [0, nparts), each cluster connected and of minsize..maxsize vertices
+// where the graph allows.
+// seed: the random visiting orders (the default is what every entry point uses)
+int mgridgen_partition(int nv, const int* xadj, const double* vvol, const double* vsurf, const int* adjncy,
+                       const double* adjwgt, int N, int minsize, int maxsize, std::vector<int>& part,
+                       uint64_t seed = 0x6D67726964676E31ull);
 
 // Agglomeration<N,P>(mesh, maxAgglom) (agglomeration.h:21-34): successive
 // partitions, level l elements -> level l+1 agglomerates, until one

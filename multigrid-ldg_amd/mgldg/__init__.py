@@ -275,6 +275,26 @@ def agglomerate(prob, max_levels=10):
     return out
 
 
+def mgridgen(xadj, vvol, vsurf, adjncy, adjwgt, minsize, maxsize, dim):
+    """MGridGen(nvtxs, xadj, vvol, vsurf, adjncy, adjwgt, minsize, maxsize,
+    {4, 6, 1, dim}, ...) (lib/ParMGridGen-1.0/MGridGen/Lib/mgridgen.c:18-49) by
+    the library's multilevel agglomerator (host only).  Returns (nparts, part)."""
+    xa = np.ascontiguousarray(xadj, dtype=np.int32)
+    n = len(xa) - 1
+    adj = np.ascontiguousarray(adjncy, dtype=np.int32)
+    w = np.ascontiguousarray(adjwgt, dtype=np.float64)
+    vv = np.ascontiguousarray(vvol, dtype=np.float64)
+    vs = np.ascontiguousarray(vsurf, dtype=np.float64)
+    if len(vv) != n or len(vs) != n or len(adj) != xa[-1] or len(w) != xa[-1]:
+        raise MGLDGError(-1, "mgridgen: array lengths do not match xadj")
+    opts = np.array([4, 6, 1, dim], dtype=np.int32)
+    part = np.zeros(max(n, 1), dtype=np.int32)
+    nparts, nmoves = C.c_int(), C.c_int()
+    check(None, lib().mgldg_mgridgen(n, _pi(xa), _pd(vv), _pd(vs), _pi(adj), _pd(w), int(minsize), int(maxsize), _pi(opts),
+                                     C.byref(nmoves), C.byref(nparts), _pi(part)))
+    return nparts.value, part[:n]
+
+
 class Multigrid(_Handle):
     """InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
     (src/cartesian/multigrid/interpolationhierarchy.h:30-82, multigrid.h:291-301).

@@ -60,6 +60,7 @@ SIGNATURES = {
     "mgldg_mg_batch_kernels": (I, [V, I]),
     "mgldg_mg_batch_plan": (I, [V, PI, C.POINTER(C.c_float)]),
     "mgldg_agglomerate": (I, [V, I, PI, PI, PI]),
+    "mgldg_mgridgen": (I, [I, PI, PD, PD, PI, PD, I, I, PI, PI, PI, PI]),
     "mgldg_mg_create_agglomerated": (I, [V, C.POINTER(MGParams), I, PI, C.POINTER(V)]),
     "mgldg_mg_export": (I, [V, I, I, PI, PI, PD, PL, PI, PI]),
     "mgldg_mg_colors": (I, [V, I, PI, PI]),
