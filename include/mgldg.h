@@ -317,6 +317,17 @@ int mgldg_vcycles_device(mgldg_mg mg, int precision, const double* b, double* x,
  * driver applies it to the right-hand side of Neumann and periodic problems
  * (dg_tests.cpp:282-286). */
 int mgldg_mean_zero_device(mgldg_ctx ctx, long n, double* v, int nrhs);
+/* No counterpart in the reference.  mgldg_mean_zero_device for callers that
+ * cannot allocate device memory (code between graph captures, serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and results, with the
+ * reduction's partial sums in a static device array of the library (2.1 MB,
+ * held by each device from the first context on it to the end of the
+ * process) instead of a buffer grown on demand.  Calls no hipMalloc / hipFree.
+ * Limit: nrhs <= 256 (MGLDG_ERR_ARG beyond).  Every call on one device shares
+ * that array: calls that overlap on one device (other host threads or other
+ * streams) race; calls in sequence are independent (nothing is carried from
+ * one call to the next).  Returns after the work is complete. */
+int mgldg_mean_zero_device_noalloc(mgldg_ctx ctx, long n, double* v, int nrhs);
 
 /* ======================= interop file formats ======================= */
 /* Text formats of the reference's dumps and meshes.  Entries taking no handle

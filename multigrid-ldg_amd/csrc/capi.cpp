@@ -624,6 +624,19 @@ int mgldg_mean_zero_device(mgldg_ctx ctx, long n, double* v, int nrhs)
     });
 }
 
+int mgldg_mean_zero_device_noalloc(mgldg_ctx ctx, long n, double* v, int nrhs)
+{
+    if (!ctx || !v || n <= 0 || nrhs < 1) return MGLDG_ERR_ARG;
+    Ctx* c = &ctx->c;
+    return guard(c, [&] {
+        ARG(nrhs <= 256, "mean_zero_noalloc: at most 256 right-hand sides per call (the static scratch's limit)");
+        ARG(dev::mean_zero_part_size(n, nrhs) <= dev::kScratchDoubles, "mean_zero_noalloc: scratch too small");
+        ARG(c->scratch, "mean_zero_noalloc: scratch address not resolved");
+        dev::mean_zero(c->stream, n, nrhs, v, c->scratch);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
 // ---------------------------------------------------------------- interop file formats
 int mgldg_bsr_write_mm(const char* path, int nbrows, int nbcols, int br, int bc, const int* rowptr, const int* cols,
                        const double* blocks)

@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "devutil.hpp"
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -1713,6 +1714,28 @@ __global__ void k_colsub(long n, int k, const double* __restrict__ sums, double*
 }
 
 size_t mean_zero_part_size(long n, int k) { return (size_t)red_grid(n, k) * k + k; }
+
+static_assert(kScratchDoubles >= (size_t)kRedBlocks * 256 + 256, "scratch holds mean_zero's partials for 256 columns");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+double* scratch_address(int device)
+{
+    static std::mutex mu;
+    static double* addr[256] = {};
+    if (device < 0 || device >= 256) throw std::runtime_error("scratch_address: device id out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess || cur != device) throw std::runtime_error("scratch_address: the device must be current");
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[device] = (double*)p;
+    }
+    return addr[device];
+}
 
 void mean_zero(hipStream_t s, long n, int k, double* a, double* part)
 {

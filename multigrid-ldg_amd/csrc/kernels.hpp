@@ -205,6 +205,15 @@ void coldot(hipStream_t s, long n, int k, const double* a, const double* b, doub
 // holds mean_zero_part_size(n, k) doubles
 size_t mean_zero_part_size(long n, int k);
 void mean_zero(hipStream_t s, long n, int k, double* a, double* part);
+// Scratch of the allocation-free mean_zero entry point: a file-scope device
+// array in kernels.hip of kScratchDoubles doubles (the per-column partials of
+// the reduction's at most 1024 workgroups and the column sums, for up to 256
+// columns; 2.1 MB), which each device holds from the library's load to the
+// end of the process.  scratch_address(device) returns that device's copy,
+// resolved once per device with hipGetSymbolAddress while the device is
+// current (the array's name is not a device address in host code).
+constexpr size_t kScratchDoubles = (size_t)1024 * 256 + 256;
+double* scratch_address(int device);
 template <typename T>
 void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const double* d, const double* q, double* x,
              double* r, T* v, double* part);

@@ -63,6 +63,7 @@ void Ctx::init(int dev, void* s)
     HIPCHK(hipSetDevice(dev));
     if (s) { stream = (hipStream_t)s; own_stream = false; }
     else { HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); own_stream = true; }
+    scratch = dev::scratch_address(dev);   // with dev current
     part.alloc(3 * 1024 + 64);
     counter.alloc(16);
     HIPCHK(hipMemsetAsync(counter.p, 0, 16 * sizeof(unsigned), stream));

@@ -40,6 +40,7 @@ struct Ctx {
     DBuf<double> part;       // reduction partials
     DBuf<unsigned> counter;  // reduction counter (kept at zero between kernels)
     DBuf<double> colpart;    // per-column reduction partials and sums (mean_zero), grown on demand
+    double* scratch = nullptr;  // this device's copy of the library's static scratch array (dev::scratch_address)
     void init(int dev, void* stream);
     ~Ctx();
 };

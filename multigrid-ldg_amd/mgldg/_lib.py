@@ -76,6 +76,7 @@ SIGNATURES = {
     "mgldg_vcycle_device": (I, [V, I, V, V, I]),
     "mgldg_vcycles_device": (I, [V, I, V, V, I, D, PD, PI]),
     "mgldg_mean_zero_device": (I, [V, L, V, I]),
+    "mgldg_mean_zero_device_noalloc": (I, [V, L, V, I]),
     "mgldg_bsr_write_mm": (I, [C.c_char_p, I, I, I, I, PI, PI, PD]),
     "mgldg_problem_write_mm": (I, [V, I, C.c_char_p]),
     "mgldg_mg_write_mm": (I, [V, I, I, C.c_char_p]),
