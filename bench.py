@@ -215,7 +215,7 @@ def bumps(X, seed, n=16):
     return f
 
 
-def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1):
+def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1, fd=True):
     """The adjoint workload: uniform 1024 x 1024 quadtree, p = 2.  u* solves
     f* (16 Gaussian bumps, seed 0); u solves f (16 bumps, seed 1) with
     g = sin(2 pi (x + y)) on the boundary; L = 0.5 ||u - u*||^2.  dL/df and
@@ -268,6 +268,8 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1):
         if prec == "fp64":
             grads = (dLdf, dLdg)
         log(f"adjoint {prec}: {dt * 1e3:.1f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {ast['iterations']}")
+    if not fd:   # profiling mode: the adjoint solves alone
+        return {"ndof": prob.ndof, "levels": mg.nlevels, "adjoint": out}
     dLdf, dLdg = grads
     # central differences along random smooth directions (4 Gaussian bumps in f,
     # low Fourier modes of g), 32 perturbed solves per batch.  L is quadratic in
@@ -488,6 +490,8 @@ def main():
     ap.add_argument("--no-unstructured", action="store_true", help="skip the unit-disk workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
+    ap.add_argument("--adjoint-solve-only", action="store_true",
+                    help="the adjoint workload's setup and adjoint solves, no finite differences (profiling)")
     ap.add_argument("--no-ball", action="store_true", help="skip the 3-D ball workload")
     ap.add_argument("--ball-only", action="store_true", help="only the 3-D ball workload")
     ap.add_argument("--setup-only", action="store_true",
@@ -508,8 +512,9 @@ def main():
     if not os.path.exists(mgldg.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
-    if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_leg(mgldg, torch, warm=args.warmup)}), flush=True)
+    if args.adjoint_only or args.adjoint_solve_only:
+        print(json.dumps({"adjoint": adjoint_leg(mgldg, torch, warm=args.warmup, fd=not args.adjoint_solve_only)}),
+              flush=True)
         return
     if args.ball_only:
         print(json.dumps({"ball": ball_leg(mgldg, torch, warm=args.warmup)}), flush=True)

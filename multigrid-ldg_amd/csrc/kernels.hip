@@ -31,6 +31,19 @@ __host__ __device__ constexpr int panel_v(int tsize, int B)
 template <typename T, int V>
 struct alignas(sizeof(T) * V) VecT { T v[V]; };
 
+// Pooled blocks (A and D^-1 of a pooled level, read from cache by the
+// single-RHS kernels) are in the panel layout with the columns padded with
+// zeros to pad_of<T, B>(), a multiple of the 16-byte vector VP = vp_of<T>():
+// element (r, c) at ((c / VP) * B + r) * VP + c % VP, c < pad_of.  Lane r
+// loads its row as pad_of / VP aligned 16-byte vectors whatever B is (9 x 9
+// blocks: 5 FP64 / 3 FP32 loads per block row instead of 9), and the B lanes
+// of a row read consecutive vectors (one contiguous run per load).  Where
+// B * sizeof(T) is a multiple of 16 this is the panel layout itself.
+template <typename T>
+__host__ __device__ constexpr int vp_of() { return 16 / (int)sizeof(T); }
+template <typename T, int B>
+__host__ __device__ constexpr int pad_of() { return (B + vp_of<T>() - 1) / vp_of<T>() * vp_of<T>(); }
+
 template <typename T, int V>
 __device__ __forceinline__ VecT<T, V> ldv(const T* p) { return *reinterpret_cast<const VecT<T, V>*>(p); }
 
@@ -224,6 +237,41 @@ __global__ void k_to_panel(long nblk, int B, int V, const double* __restrict__ i
     out[k * bs + ((c / V) * B + r) * V + (c % V)] = (T)in[e];
 }
 
+// row-major FP64 blocks -> the padded panel layout of T (pad_of): element
+// (r, c) of block k at k * B * bp + ((c / vp) * B + r) * vp + c % vp, zero
+// for c >= B
+template <typename T>
+__global__ void k_to_rowpad(long nblk, int B, int bp, const double* __restrict__ in, T* __restrict__ out)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nblk * B * bp) return;
+    constexpr int vp = vp_of<T>();
+    const long k = e / ((long)B * bp);
+    const int q = (int)(e - k * B * bp);              // ((c / vp) * B + r) * vp + c % vp
+    const int cm = q % vp, rest = q / vp, r = rest % B, c = (rest / B) * vp + cm;
+    out[e] = c < B ? (T)in[(k * B + r) * B + c] : (T)0;
+}
+
+int pool_pad(int tsize, int b) { return tsize == 8 ? (b + 1) / 2 * 2 : (b + 3) / 4 * 4; }
+static_assert(pad_of<double, 9>() == 10 && pad_of<float, 9>() == 12 && pad_of<float, 16>() == 16, "pool_pad and pad_of agree");
+
+void to_rowpad_f64(hipStream_t s, long nblk, int b, const double* in, double* out)
+{
+    const int bp = pool_pad(8, b);
+    const long n = nblk * b * bp;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_rowpad<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nblk, b, bp, in, out);
+    check_launch("to_rowpad");
+}
+void to_rowpad_f32(hipStream_t s, long nblk, int b, const double* in, float* out)
+{
+    const int bp = pool_pad(4, b);
+    const long n = nblk * b * bp;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_rowpad<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nblk, b, bp, in, out);
+    check_launch("to_rowpad");
+}
+
 void to_panel_f64(hipStream_t s, long nblk, int b, const double* in, double* out)
 {
     long n = nblk * b * b; if (n <= 0) return;
@@ -306,9 +354,6 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
                                      const int* __restrict__ color, int cur)
 {
     constexpr int V = Map<T, B>::V;
-    // pooled blocks come from cache: one block in flight, and more
-    // wavefronts per SIMD instead (k_gs<float,16> 124 -> 64 VGPRs with the
-    // late D^-1: 8 wavefronts per SIMD, a fine colour's rows all resident)
     constexpr int U = POOL ? 1 : unroll_of<T, B>();
     T acc[V];
 #pragma unroll
@@ -326,6 +371,66 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
     const int n = valid ? k1 - k0 - (dp >= 0 ? 1 : 0) : 0;
     int wl = n;
     for (int o = 32; o > 0; o >>= 1) wl = max(wl, __shfl_xor(wl, o, 64));
+    if constexpr (POOL) {
+        T pacc[vp_of<T>()];
+#pragma unroll
+        for (int v = 0; v < vp_of<T>(); ++v) pacc[v] = 0;
+        // Pooled blocks come from cache; the x_j gathers are the row's HBM
+        // round trips.  All UX gathers of a round (and, on a first sweep,
+        // the neighbours' colours) are issued together, so a row waits for
+        // one round trip per UX blocks; the A operands follow one block at a
+        // time from cache, which keeps the registers low (8 wavefronts per
+        // SIMD on the fine levels).  A gathered x_j of a colour not yet
+        // updated in this sweep is replaced by zero, never used.
+        constexpr int UX = unroll_of<T, B>();
+        constexpr int VP = vp_of<T>(), BP = pad_of<T, B>();
+        // the pad entries of the staged x stay zero (A's pad columns are
+        // zero too, and 0 * x must not meet a stale non-finite value)
+        if (r < BP - B) {
+#pragma unroll
+            for (int u = 0; u < UX; ++u) xs[u * BP + B + r] = (T)0;
+        }
+        for (int m0 = 0; m0 < wl; m0 += B) {
+            if (m0 > 0) fetch(m0 + r);
+            const int nc = min(B, wl - m0);
+            for (int u0 = 0; u0 < nc; u0 += UX) {
+                int pid[UX];
+                bool use[UX];
+                T xv[UX];
+#pragma unroll
+                for (int u = 0; u < UX; ++u) {
+                    const int src = lane0 + min(u0 + u, B - 1);
+                    const int j = __shfl(mycol, src, 64);
+                    pid[u] = __shfl(mypid, src, 64);
+                    const bool live = valid && u0 + u < nc && j >= 0;
+                    xv[u] = live ? x[(size_t)j * B + r] : (T)0;
+                    bool ok = live;
+                    if (SKIP != 0 && live) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
+                    use[u] = ok;
+                }
+#pragma unroll
+                for (int u = 0; u < UX; ++u) xs[u * BP + r] = use[u] ? xv[u] : (T)0;
+                wave_sync();
+#pragma unroll
+                for (int u = 0; u < UX; ++u)
+                    if (use[u]) {
+                        const T* Ab = A + (size_t)pid[u] * (B * BP) + r * VP;
+#pragma unroll
+                        for (int cc = 0; cc < BP / VP; ++cc) {
+                            const VecT<T, VP> av = ldv<T, VP>(Ab + cc * B * VP);
+                            const VecT<T, VP> xq = ldv<T, VP>(xs + u * BP + cc * VP);
+#pragma unroll
+                            for (int v = 0; v < VP; ++v) pacc[v] += av.v[v] * xq.v[v];
+                        }
+                    }
+                wave_sync();
+            }
+        }
+        T s = pacc[0];
+#pragma unroll
+        for (int v = 1; v < VP; ++v) s += pacc[v];
+        return s;
+    }
     for (int m0 = 0; m0 < wl; m0 += B) {
         if (m0 > 0) fetch(m0 + r);
         const int nc = min(B, wl - m0);
@@ -403,7 +508,8 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
 template <typename T, int B>
 struct RowLds {
     static constexpr int RW = Map<T, B>::RPW + 1;
-    static constexpr int XS = unroll_of<T, B>() * B;
+    static constexpr int XS = unroll_of<T, B>() * pad_of<T, B>();
+    static constexpr int TB = pad_of<T, B>();   // D^-1 operand staging per row
 };
 
 // s_r = sum_c Dinv[r][c] t_c with lane r's row of Dinv in registers and t in LDS
@@ -433,6 +539,39 @@ __device__ __forceinline__ void load_dinv_row(const T* __restrict__ Dinv, int i,
     const T* Db = Dinv + (size_t)i * (B * B) + r * V;
 #pragma unroll
     for (int cc = 0; cc < B / V; ++cc) dv[cc] = ldv<T, V>(Db + cc * B * V);
+}
+
+// the same for a pooled D^-1 block (padded panel layout)
+template <typename T, int B>
+struct PadRow { VecT<T, vp_of<T>()> v[pad_of<T, B>() / vp_of<T>()]; };
+
+template <typename T, int B>
+__device__ __forceinline__ void load_dinv_row_pad(const T* __restrict__ Dinv, int i, int r, PadRow<T, B>& dv)
+{
+    constexpr int VP = vp_of<T>(), BP = pad_of<T, B>();
+    const T* Db = Dinv + (size_t)i * (B * BP) + r * VP;
+#pragma unroll
+    for (int cc = 0; cc < BP / VP; ++cc) dv.v[cc] = ldv<T, VP>(Db + cc * B * VP);
+}
+
+// s_r = sum_c Dinv[r][c] t_c, t in LDS with its pad entries zero
+template <typename T, int B>
+__device__ __forceinline__ T dinv_row_pad(const PadRow<T, B>& dv, const T* t)
+{
+    constexpr int VP = vp_of<T>(), BP = pad_of<T, B>();
+    T s[VP];
+#pragma unroll
+    for (int v = 0; v < VP; ++v) s[v] = 0;
+#pragma unroll
+    for (int cc = 0; cc < BP / VP; ++cc) {
+        const VecT<T, VP> tv = ldv<T, VP>(t + cc * VP);
+#pragma unroll
+        for (int v = 0; v < VP; ++v) s[v] += dv.v[cc].v[v] * tv.v[v];
+    }
+    T out = s[0];
+#pragma unroll
+    for (int v = 1; v < VP; ++v) out += s[v];
+    return out;
 }
 
 // y = A x  or  y = b - A x ; optional deterministic reduction of y.dotv
@@ -498,12 +637,14 @@ __device__ __forceinline__ void gs_rows(int beg, int end, const int4* __restrict
                                             x, r, slot * B, xs_all + (wave * R::RW + slot) * R::XS, color, cur);
     // a pooled D^-1 block is read from cache: loaded after the sum, it holds
     // no registers across it (more wavefronts per SIMD)
-    if (POOL && valid) load_dinv_row<T, B>(Dinv, d.w, r, dv);
-    T* t = tb + (wave * R::RW + slot) * B;
+    PadRow<T, B> dp;
+    if (POOL && valid) load_dinv_row_pad<T, B>(Dinv, d.w, r, dp);
+    T* t = tb + (wave * R::RW + slot) * R::TB;
     t[r] = bi - acc;
+    if (POOL && r < R::TB - B) t[B + r] = (T)0;
     wave_sync();
     if (valid) {
-        const T s = dinv_row<T, B>(dv, t);
+        const T s = POOL ? dinv_row_pad<T, B>(dp, t) : dinv_row<T, B>(dv, t);
         x[o] = xi + omega * (s - xi);
     }
 }
@@ -517,7 +658,7 @@ __global__ __launch_bounds__(256) void k_gs(int nrows, const int4* __restrict__ 
 {
     using R = RowLds<T, B>;
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
-    __shared__ __align__(16) T tb[4 * R::RW * B];
+    __shared__ __align__(16) T tb[4 * R::RW * R::TB];
     const int beg = xcd_block() * Map<T, B>::RPB;
     gs_rows<T, B, SKIP, POOL>(beg, nrows, desc, gcols, W, A, Dinv, b, x, omega, color, cur, xs_all, tb);
 }
@@ -541,7 +682,7 @@ __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int
     using M = Map<T, B>;
     using R = RowLds<T, B>;
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
-    __shared__ __align__(16) T tb[4 * R::RW * B];
+    __shared__ __align__(16) T tb[4 * R::RW * R::TB];
     const int lb = xcd_block();
     const int s = wg[3 * lb], m = wg[3 * lb + 1], e = wg[3 * lb + 2];
     for (int ph = 0; ph < 2; ++ph) {
@@ -563,7 +704,7 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     using M = Map<T, B>;
     using R = RowLds<T, B>;
     __shared__ __align__(16) T xs_all[4 * R::RW * R::XS];
-    __shared__ __align__(16) T tb[4 * R::RW * B];
+    __shared__ __align__(16) T tb[4 * R::RW * R::TB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, r = lane - slot * B;
     const int i = xcd_block() * M::RPB + wave * M::RPW + slot;
@@ -579,33 +720,41 @@ __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict
     }
     const T acc = row_sum<T, B, 0, POOL>(valid, k0, k1, -1, ell + (size_t)i * W * (POOL ? 2 : 1), W, A, x, r, slot * B,
                                          xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
-    if (POOL && valid) load_dinv_row<T, B>(Dinv, didx[i], r, dv);   // (as in gs_rows)
-    T* t = tb + (wave * R::RW + slot) * B;
+    PadRow<T, B> dp;
+    if (POOL && valid) load_dinv_row_pad<T, B>(Dinv, didx[i], r, dp);   // (as in gs_rows)
+    T* t = tb + (wave * R::RW + slot) * R::TB;
     t[r] = bi - acc;
+    if (POOL && r < R::TB - B) t[B + r] = (T)0;
     wave_sync();
-    if (valid) xout[o] = xi + omega * dinv_row<T, B>(dv, t);
+    if (valid) xout[o] = xi + omega * (POOL ? dinv_row_pad<T, B>(dp, t) : dinv_row<T, B>(dv, t));
 }
 
 // The first Jacobi sweep from x = 0: xout = omega D^-1 b (A x = 0 is not
 // read), the value k_jacobi gives for x = 0
-template <typename T, int B>
+// POOL: Dinv is a pool of padded row-major blocks indexed by didx
+template <typename T, int B, bool POOL = false>
 __global__ __launch_bounds__(256) void k_jacobi0(int nrows, const T* __restrict__ Dinv, const int* __restrict__ didx,
                                                  const T* __restrict__ b, T* __restrict__ xout, T omega)
 {
     using M = Map<T, B>;
     using R = RowLds<T, B>;
-    __shared__ __align__(16) T tb[4 * R::RW * B];
+    __shared__ __align__(16) T tb[4 * R::RW * R::TB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, r = lane - slot * B;
     const int i = blockIdx.x * M::RPB + wave * M::RPW + slot;
     const bool valid = slot < M::RPW && i < nrows;
     const size_t o = (size_t)i * B + r;
     VecT<T, M::V> dv[B / M::V];
-    T* t = tb + (wave * R::RW + slot) * B;
+    PadRow<T, B> dp;
+    T* t = tb + (wave * R::RW + slot) * R::TB;
     t[r] = valid ? b[o] : (T)0;
-    if (valid) load_dinv_row<T, B>(Dinv, didx ? didx[i] : i, r, dv);
+    if (POOL && r < R::TB - B) t[B + r] = (T)0;
+    if (valid) {
+        if (POOL) load_dinv_row_pad<T, B>(Dinv, didx[i], r, dp);
+        else load_dinv_row<T, B>(Dinv, i, r, dv);
+    }
     wave_sync();
-    if (valid) xout[o] = omega * dinv_row<T, B>(dv, t);
+    if (valid) xout[o] = omega * (POOL ? dinv_row_pad<T, B>(dp, t) : dinv_row<T, B>(dv, t));
 }
 
 // Interpolation blocks are BF x BC row-major (fine simplex or box basis x
@@ -849,7 +998,11 @@ void Ops<T>::jacobi0(hipStream_t s, int B, int nrows, const T* Dinv, const T* b,
     if (nrows <= 0) return;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
-        hipLaunchKernelGGL((k_jacobi0<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, Dinv, didx, b, xout, omega);
+        if (didx)
+            hipLaunchKernelGGL((k_jacobi0<T, BB, true>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, Dinv, didx, b, xout,
+                               omega);
+        else
+            hipLaunchKernelGGL((k_jacobi0<T, BB>), dim3(nblk(nrows, M::RPB)), dim3(256), 0, s, nrows, Dinv, didx, b, xout, omega);
     });
     check_launch("jacobi0");
 }

@@ -26,6 +26,11 @@ void triple(hipStream_t s, int nout, int p, int q, int m, int n, int transL,
 // batched inverse of nblk row-major b x b FP64 blocks (Gauss-Jordan, partial pivoting)
 void inverse(hipStream_t s, int nblk, int b, const double* in, const int* slot, double* out);
 // row-major FP64 blocks -> panel layout of float or double (see kernels.hip)
+// pooled blocks for the single-RHS kernels: the panel layout with the columns
+// padded with zeros to pool_pad(sizeof(T), b) (a multiple of 16 bytes)
+int pool_pad(int tsize, int b);
+void to_rowpad_f64(hipStream_t s, long nblk, int b, const double* in, double* out);
+void to_rowpad_f32(hipStream_t s, long nblk, int b, const double* in, float* out);
 void to_panel_f64(hipStream_t s, long nblk, int b, const double* in, double* out);
 void to_panel_f32(hipStream_t s, long nblk, int b, const double* in, float* out);
 // gather diagonal blocks: out[i] = in[diag[i]]

@@ -662,18 +662,24 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
             didx.resize(lv.nb); drep.resize(lv.nb);
             for (int i = 0; i < lv.nb; ++i) didx[i] = drep[i] = i;
         }
-        auto make_pool = [&](const std::vector<int>& rep, const double* src, DBuf<double>& p64, DBuf<float>& p32) {
+        auto make_pool = [&](const std::vector<int>& rep, const double* src, DBuf<double>& p64, DBuf<float>& p32,
+                             DBuf<double>& r64, DBuf<float>& r32) {
             const long np = (long)rep.size();
             DBuf<int> rd; rd.upload(rep.data(), rep.size(), s);
             DBuf<double> rm; rm.alloc((size_t)np * b * b);
             dev::gather_blocks(s, (int)np, b * b, rd, src, rm);
             p64.alloc((size_t)np * b * b);
             dev::to_panel_f64(s, np, b, rm, p64);
-            if (precision == 1) { p32.alloc((size_t)np * b * b); dev::to_panel_f32(s, np, b, rm, p32); }
+            r64.alloc((size_t)np * b * dev::pool_pad(8, b));
+            dev::to_rowpad_f64(s, np, b, rm, r64);
+            if (precision == 1) {
+                p32.alloc((size_t)np * b * b); dev::to_panel_f32(s, np, b, rm, p32);
+                r32.alloc((size_t)np * b * dev::pool_pad(4, b)); dev::to_rowpad_f32(s, np, b, rm, r32);
+            }
             HIPCHK(hipStreamSynchronize(s));
         };
-        make_pool(prep, lv.A.val, lv.P64, lv.P32);
-        make_pool(drep, lv.Dinv_rm, lv.DP64, lv.DP32);
+        make_pool(prep, lv.A.val, lv.P64, lv.P32, lv.R64, lv.R32);
+        make_pool(drep, lv.Dinv_rm, lv.DP64, lv.DP32, lv.DR64, lv.DR32);
         const int W = lv.A.ellw;
         std::vector<int> pe((size_t)lv.nb * W * 2, 0);
         for (int i = 0; i < lv.nb; ++i) {
@@ -894,11 +900,18 @@ template <> const float* Dop<float>(const Level& lv) { return lv.D32.p; }
 // level is pooled (with the pair tables and D^-1 indices), else the full
 // panel arrays
 template <typename T> static const T* Sop(const Level& lv);
-template <> const double* Sop<double>(const Level& lv) { return lv.npool ? lv.P64.p : lv.A64.p; }
-template <> const float* Sop<float>(const Level& lv) { return lv.npool ? lv.P32.p : lv.A32.p; }
+template <> const double* Sop<double>(const Level& lv) { return lv.npool ? lv.R64.p : lv.A64.p; }
+template <> const float* Sop<float>(const Level& lv) { return lv.npool ? lv.R32.p : lv.A32.p; }
 template <typename T> static const T* SDop(const Level& lv);
-template <> const double* SDop<double>(const Level& lv) { return lv.npool ? lv.DP64.p : lv.D64.p; }
-template <> const float* SDop<float>(const Level& lv) { return lv.npool ? lv.DP32.p : lv.D32.p; }
+template <> const double* SDop<double>(const Level& lv) { return lv.npool ? lv.DR64.p : lv.D64.p; }
+template <> const float* SDop<float>(const Level& lv) { return lv.npool ? lv.DR32.p : lv.D32.p; }
+// the batched MFMA kernels read the pools in the panel layout
+template <typename T> static const T* Mop(const Level& lv);
+template <> const double* Mop<double>(const Level& lv) { return lv.npool ? lv.P64.p : lv.A64.p; }
+template <> const float* Mop<float>(const Level& lv) { return lv.npool ? lv.P32.p : lv.A32.p; }
+template <typename T> static const T* MDop(const Level& lv);
+template <> const double* MDop<double>(const Level& lv) { return lv.npool ? lv.DP64.p : lv.D64.p; }
+template <> const float* MDop<float>(const Level& lv) { return lv.npool ? lv.DP32.p : lv.D32.p; }
 static const int* Sell(const Level& lv) { return lv.npool ? lv.pell.p : lv.A.ell.p; }
 static const int* Sdidx(const Level& lv) { return lv.npool ? lv.pdidx.p : nullptr; }
 static int Spool(const Level& lv) { return lv.npool ? 1 : 0; }
@@ -1305,7 +1318,7 @@ void MGH::bapply(int l, int k, const T* x, const T* b, T* y)
 {
     Level& lv = L[l];
     const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].apply : 0;
-    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Sop<T>(lv), x, b, y, Spidx(lv));
+    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Mop<T>(lv), x, b, y, Spidx(lv));
     else dev::MOps<T>::apply(ctx->stream, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), x, b, y);
 }
 
@@ -1387,7 +1400,7 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
             sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, const int*, int nr, const int* cdev, int c, int skip) {
-                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Sop<T>(lv), SDop<T>(lv), bb<T>(v),
+                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Mop<T>(lv), MDop<T>(lv), bb<T>(v),
                                              bx<T>(v), omega, cdev, c, skip, Spidx(lv), Sdidx(lv));
                 else dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
                                             bx<T>(v), omega, cdev, c, skip);
@@ -1401,7 +1414,7 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
             else { dev::MOps<T>::jacobi0(s, lv.B, k, lv.nb, Dop<T>(lv), bb<T>(v), nxt, omega); std::swap(cur, nxt); sw0 = 1; }
         }
         for (int sw = sw0; sw < n; ++sw) {
-            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Sop<T>(lv), SDop<T>(lv), bb<T>(v), cur, nxt,
+            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Mop<T>(lv), MDop<T>(lv), bb<T>(v), cur, nxt,
                                        omega, Spidx(lv), Sdidx(lv));
             else dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
             std::swap(cur, nxt);

@@ -140,6 +140,10 @@ struct Level {
     int npool = 0, ndpool = 0;
     DBuf<double> P64, DP64;
     DBuf<float> P32, DP32;
+    // the same pools with columns padded to dev::pool_pad, read by the
+    // single-RHS kernels (one 16-byte vector per load whatever B is)
+    DBuf<double> R64, DR64;
+    DBuf<float> R32, DR32;
     DBuf<int> pell, pdidx;
     DBuf<int> ppidx;                    // per block slot its pool index (batched MFMA kernels)
     std::vector<int> hpidx, hdidx;      // host copies of the pool indices (traffic accounting)
