@@ -19,8 +19,15 @@ mixed-precision form (FP32 V-cycle inside FP64 CG); the plain FP64 GPU solve
 is timed beside it.  The roofline object describes the dominant kernel, the
 fine-level block Gauss-Seidel colour sweep of the FP32 V-cycle, timed with HIP
 events on the library's stream.  The CPU baseline is the FP64 oracle (a C
-restatement of the reference algorithm, single thread) solving the same
-problem on the host.
+restatement of the reference algorithm, one thread per solve as the
+reference) solving the same problem on the host: one core, and as many
+concurrent solves as the process has cores (all-core throughput); beside it
+the plain method's 32 x 32 p = 1 configuration on one and all cores and, for
+the disk, adjoint and ball workloads (too large for the time budget), the
+oracle's cost per iteration on stated smaller instances.  The ball workload
+is measured on a Kuhn split of the jittered lattice; the Delaunay mesh the
+configuration names is run too and its status recorded (its slivers stall
+MG-PCG).
 """
 import argparse
 import json
@@ -189,6 +196,150 @@ def unstructured_leg(mgldg, torch, nrhs, warm):
         "setup_s": {"mesh": round(t_mesh, 3), "problem": round(t_prob, 3), "agglomeration": round(t_agg, 3),
                     "hierarchy": round(t_mg, 3), "assembly": assembly_info(prob)},
     }
+
+
+def c1_gpu_leg(mgldg, torch, warm=3, steps=20):
+    """The plain method's configuration (uniform 32 x 32, p = 1, tau0 = 0,
+    tauD = 1000, f = 2 pi^2 sin(pi x) sin(pi y), g = 0, V(2,2) block GS, RATRAT,
+    MG-PCG to 1e-10) as the FP64 GPU solve with the V-cycle replayed from its
+    hipGraph: latency-bound (4,096 dofs, 6 levels), so reported as time per
+    solve.  Gauss-Seidel in the reference's lexicographic order (wavefronts)
+    and multicolour."""
+    ctx = mgldg.Context(0)
+    qt = mgldg.Quadtree(ctx, 2, lambda x: 1.0 / 32)
+    prob = mgldg.LDGPoisson(qt, 1, tau0=0.0, tauD=1000.0)
+    X = prob.nodes()
+    b = torch.tensor(prob.rhs(2 * np.pi ** 2 * np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1])), device="cuda")
+    out = {"ndof": prob.ndof}
+    for relax in ("gs-lex", "gs"):
+        mg = mgldg.Multigrid(prob, relax, npre=2, npost=2)
+        kw = dict(tol=1e-10, maxit=200, criterion="true", precision="fp64", use_graph=True)
+        for _ in range(warm):
+            mgldg.pcg(mg, b, torch.zeros_like(b), **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            _, st = mgldg.pcg(mg, b, torch.zeros_like(b), **kw)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        out[relax] = {"ms_per_solve": round(dt * 1e3, 3), "value": round(prob.ndof / dt, 1),
+                      "iterations": st["iterations"], "relres_true": st["relres_true"]}
+    log(f"C1 (32x32 p=1) FP64 GPU solves: {out}")
+    return out
+
+
+# ---------------------------------------------------------------- CPU baseline (FP64 oracle)
+def _oracle_job(job):
+    """One FP64 oracle solve in a worker process (spawned, no GPU): builds the
+    job's problem and hierarchy (not timed), then times PCG to 1e-10 or for
+    job['maxit'] iterations.  Returns (setup_s, solve_s, iterations, relres,
+    ndof)."""
+    import oracle as O
+    kind = job["kind"]
+    t0 = time.perf_counter()
+    if kind == "c1":
+        qt = O.Quadtree(2, lambda x: 1.0 / 32)
+        pr = O.Problem(qt, 1, tau0=0.0, tauD=1000.0)
+        mg = O.Multigrid(pr, job["relax"], npre=2, npost=2)
+        X = pr.nodes()
+        b = pr.rhs(2 * np.pi ** 2 * np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]), np.zeros(pr.nqb))
+    elif kind in ("c2", "c4"):
+        if kind == "c2":
+            qt = O.Quadtree(2, c2_size, rapid=True, balance=True)
+            pr = O.Problem(qt, 3)
+        else:
+            qt = O.Quadtree(2, lambda x: 1.0 / job["n"])
+            pr = O.Problem(qt, 2, tau0=0.0, tauD=1000.0)
+        mg = O.Multigrid(pr, "mcgs", npre=2, npost=2)
+        b = pr.rhs(np.random.default_rng(0).standard_normal(pr.ndof), np.zeros(pr.nqb))
+    else:   # c3 / c5: agglomerated hierarchies from the reference's MGridGen (oracle/_ref)
+        if kind == "c3":
+            pts, cells = disk_mesh(job["n_int"], job["h"])
+        else:
+            pts, cells = ball_mesh(job["n"])
+        pr = O.SxProblem(pts, cells, 2, tau0=100.0, tauD=1000.0)
+        parts = O.agglomerate(pr)
+        mg = O.SxMultigrid(pr, parts, relaxation="jacobi" if kind == "c3" else "mcgs",
+                           omega=0.6 if kind == "c3" else 0.8, npre=2, npost=2, restriction="rat")
+        b = pr.rhs(np.random.default_rng(0).standard_normal(pr.ndof), np.zeros(pr.nqb))
+    t_setup = time.perf_counter() - t0
+    if job.get("barrier") is not None:
+        job["barrier"].wait()
+    t0 = time.perf_counter()
+    _, it, rr, _ = O.pcg(pr, mg, b, tol=1e-10, maxit=job.get("maxit", 500), criterion="true")
+    return t_setup, time.perf_counter() - t0, it, rr, pr.ndof
+
+
+def _cpu_pool_run(jobs, concurrent):
+    """run jobs in spawned processes; concurrent ones start their timed solves together"""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    if not concurrent or len(jobs) == 1:
+        with ctx.Pool(1) as pool:
+            return [pool.apply(_oracle_job, (j,)) for j in jobs]
+    mgr = ctx.Manager()
+    bar = mgr.Barrier(len(jobs))
+    with ctx.Pool(len(jobs)) as pool:
+        return pool.map(_oracle_job, [dict(j, barrier=bar) for j in jobs])
+
+
+def cpu_cores():
+    """host threads the CPU baseline uses: this process's CPU share (the GPU
+    box grants 16 per GPU)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_legs(gpu_iters, c2_one_core=None):
+    """The reference's CPU path (the FP64 oracle, a C restatement, one thread
+    per solve as the reference: Makefile THREADS=1, no OpenMP in src/) on this
+    host.  C1 and C2: full solves on 1 core and `cores` concurrent solves on
+    `cores` cores (all-core throughput).  C3 / C4 / C5 do not fit the time
+    budget at full size (oracle setup 21 s / 56 s / minutes and 2.4 s / 9 s /
+    ~20 s per PCG iteration on one core): each is timed on a stated smaller
+    instance of its configuration for 3 PCG iterations, and its figure is the
+    DoF-solves/s the oracle would reach on the full problem with a cost per
+    iteration linear in the dofs and the full problem's iteration count
+    (from the GPU solve): ndof_sub / (seconds per iteration on the subset *
+    iterations)."""
+    cores = cpu_cores()
+    out = {"host_cpu": cpu_model(), "host_nproc": os.cpu_count(), "cores_used": cores}
+    for name, job, what in (("c1", {"kind": "c1", "relax": "gs"},
+                             "uniform 32x32 p=1, V(2,2) lexicographic block GS (the reference's order)"),
+                            ("c2", {"kind": "c2"}, "the adaptive p=3 workload, V(2,2) multicolour block GS")):
+        if name == "c2" and c2_one_core is not None:
+            t_setup, t1, it, rr, ndof = c2_one_core      # measured in the parent (its solution checks the GPU's)
+        else:
+            t_setup, t1, it, rr, ndof = _cpu_pool_run([job], False)[0]
+        res = _cpu_pool_run([job] * cores, True)
+        tmax = max(r[1] for r in res)
+        out[name] = {"what": what, "ndof": ndof, "iterations": it, "relres_true": rr, "oracle_setup_s": round(t_setup, 2),
+                     "one_core": {"solve_s": round(t1, 4), "value": round(ndof / t1, 1)},
+                     "all_cores": {"cores": cores, "solves": cores, "wall_s": round(tmax, 4),
+                                   "value": round(cores * ndof / tmax, 1)}}
+        log(f"cpu {name}: {out[name]}")
+    subs = [("c3", {"kind": "c3", "n_int": 25_000, "h": 0.0113, "maxit": 3},
+             "disk: 25k random interior points (1/8 of the workload), p=2, MGridGen hierarchy, block Jacobi"),
+            ("c4", {"kind": "c4", "n": 256, "maxit": 3}, "uniform 256x256 quadtree, p=2 (1/16 of the workload)"),
+            ("c5", {"kind": "c5", "n": 17, "maxit": 3},
+             "ball: Kuhn split of the jittered lattice of spacing 1/17 (1/15 of the workload), p=2, MGridGen "
+             "hierarchy, multicolour GS")]
+    res = _cpu_pool_run([j for _, j, _ in subs], True)
+    for (name, job, what), (t_setup, ts, it, rr, ndof) in zip(subs, res):
+        per_it = ts / max(it, 1)
+        iters = gpu_iters.get(name)
+        o = {"subset": what, "ndof_subset": ndof, "oracle_setup_s": round(t_setup, 2), "iterations_timed": it,
+             "s_per_iteration": round(per_it, 4), "dof_iterations_per_s": round(ndof / per_it, 1),
+             "fits_budget_at_full_size": False}
+        if iters:
+            o.update({"full_iterations_from_gpu": iters, "value": round(ndof / (per_it * iters), 1),
+                      "unit": "DoF-solves/s (1 core, extrapolated)"})
+        out[name] = o
+        log(f"cpu {name}: {o}")
+    return out
 
 
 def assembly_info(prob):
@@ -377,6 +528,51 @@ def ball_mesh(n=41, jitter=0.01, seed=0):
     return P, inv.reshape(T.shape).astype(np.int32)
 
 
+def ball_delaunay_status(mgldg, torch, maxit=300):
+    """The ball workload as the configuration names it: the Delaunay
+    tetrahedralisation (scipy / qhull) of the same jittered lattice points
+    inside r <= 1, not the Kuhn split the measured leg uses.  Jittered cube
+    corners are nearly cospherical, so the Delaunay mesh holds slivers; the
+    FP64 MG-PCG is run for at most maxit iterations and its status and final
+    residual are recorded."""
+    from scipy.spatial import Delaunay
+    t0 = time.perf_counter()
+    n, jitter = 41, 0.01
+    s = 1.0 / n
+    rng = np.random.default_rng(0)
+    g = np.arange(-n, n + 1)
+    P = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * s
+    P = P[np.linalg.norm(P, axis=1) <= 1.0]
+    P = P + jitter * s * rng.uniform(-1, 1, P.shape)
+    T = Delaunay(P).simplices.astype(np.int32)
+    T = T[mgldg.locality_order(P, T)]
+    e = [P[T[:, k]] - P[T[:, 0]] for k in (1, 2, 3)]
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(e[0], e[1]), e[2])) / 6
+    med = float(np.median(vol))
+    t_mesh = time.perf_counter() - t0
+    ctx = mgldg.Context(0)
+    t0 = time.perf_counter()
+    prob = mgldg.SimplexLDGPoisson(mgldg.Wireframe(ctx, P, T), 2, tau0=100.0, tauD=1000.0)
+    mg = mgldg.Multigrid(prob, "gs", omega=0.8, npre=2, npost=2, restriction="rat", precision="fp64")
+    t_setup = time.perf_counter() - t0
+    b = mgldg.rhs_device(mg, torch.tensor(np.random.default_rng(0).standard_normal(prob.ndof), device="cuda"))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = mgldg.pcg(mg, b, torch.zeros_like(b), tol=1e-10, maxit=maxit, criterion="true", precision="fp64",
+                      use_graph=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    out = {"mesh": "Delaunay (qhull) of the 1%-jittered lattice points (spacing 1/41) in r <= 1, p=2",
+           "elements": prob.ne, "ndof": prob.ndof, "levels": mg.nlevels,
+           "slivers_below_1e-3_median_volume": int((vol < 1e-3 * med).sum()),
+           "min_volume_over_median": float(vol.min() / med),
+           "fp64": {"status": st["status"], "iterations": st["iterations"], "relres_true": st["relres_true"],
+                    "ms": round(dt * 1e3, 1)},
+           "setup_s": {"mesh": round(t_mesh, 2), "problem_and_hierarchy": round(t_setup, 2)}}
+    log(f"ball, Delaunay mesh (not the measured leg): {out}")
+    return out
+
+
 def ball_leg(mgldg, torch, nrhs=16, warm=1):
     """The 3-D workload: unit ball of tetrahedra (ball_mesh), p = 2, tau0 =
     100, tauD = 1000, the library's agglomeration (at most 8 per agglomerate),
@@ -493,6 +689,7 @@ def main():
     ap.add_argument("--adjoint-solve-only", action="store_true",
                     help="the adjoint workload's setup and adjoint solves, no finite differences (profiling)")
     ap.add_argument("--no-ball", action="store_true", help="skip the 3-D ball workload")
+    ap.add_argument("--no-delaunay", action="store_true", help="skip the Delaunay-mesh status run of the ball workload")
     ap.add_argument("--ball-only", action="store_true", help="only the 3-D ball workload")
     ap.add_argument("--setup-only", action="store_true",
                     help="the device assembly: the adaptive quadtree problem and hierarchy and the ball problem (profiling)")
@@ -748,6 +945,7 @@ def main():
         cpu_rate = ndof / (per_it * full)
         sample = (f"same C2 problem, oracle setup {t_setup:.1f}s (not timed), {ito} PCG iterations in {t_solve:.1f}s"
                   + ("" if rro <= 1e-10 else f", extrapolated to {full} iterations"))
+        c2_one = (t_setup, t_solve, ito, rro, ndof) if rro <= 1e-10 else None
         cpu = {"value": round(cpu_rate, 1), "unit": "DoF-solves/s", "cores": 1, "kind": "port", "sample": sample,
                "host_cpu": cpu_model(), "host_nproc": os.cpu_count(), "iterations": ito, "relres_true": rro}
         nx = np.linalg.norm(xo)
@@ -755,9 +953,20 @@ def main():
                "oracle_iterations": ito}
         log(f"cpu: {cpu_rate:.0f} DoF-solves/s, iterations {ito}, errors {err}")
 
+    c1 = c1_gpu_leg(mgldg, torch)
     unstructured = None if args.no_unstructured else unstructured_leg(mgldg, torch, k, args.warmup)
     adjoint = None if args.no_adjoint else adjoint_leg(mgldg, torch, warm=args.warmup)
     ball = None if args.no_ball else ball_leg(mgldg, torch, warm=args.warmup)
+    if ball is not None and not args.no_delaunay:
+        ball["delaunay_mesh_status"] = ball_delaunay_status(mgldg, torch)
+    if cpu is not None:
+        gpu_iters = {"c3": unstructured and unstructured["single_fp64"]["iterations"],
+                     "c4": adjoint and adjoint["adjoint"]["fp64"]["iterations"],
+                     "c5": ball and ball["single_fp64"]["iterations"]}
+        legs = cpu_legs(gpu_iters, c2_one)
+        cpu["all_cores"] = {"value": legs["c2"]["all_cores"]["value"], "cores": legs["c2"]["all_cores"]["cores"],
+                            "sample": f"{legs['c2']['all_cores']['solves']} concurrent full C2 oracle solves, one per core"}
+        cpu["legs"] = legs
 
     line = {
         "metric": "DoF-solves/s to 1e-10 residual, mixed vs FP64 MG-PCG; smoother HBM GB/s",
@@ -795,6 +1004,7 @@ def main():
                     "assembly": assembly_info(prob)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "c1": c1,
         "unstructured": unstructured,
         "adjoint": adjoint,
         "ball": ball,
