@@ -116,6 +116,8 @@ static void finalize(hipStream_t s, const double* part, int n, int nval, double*
     check_launch("finalize");
 }
 
+void finalize_sum(hipStream_t s, const double* part, int n, double* out) { finalize(s, part, n, 1, nullptr, 0, out); }
+
 // --------------------------------------------------------------------- setup kernels
 // largest block of the setup kernels: (p+1)^N for 2-D p <= 5 and 3-D p <= 2
 #define TP_MAX 36

@@ -208,6 +208,20 @@ void pcg_d(hipStream_t s, long n, const double* S, const T* zs, double* d, bool 
 void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part);
 // a[:, c] -= mean(a[:, c]) for each of the k columns (deterministic); part
 // holds mean_zero_part_size(n, k) doubles
+// *out = sum of part[0 .. n) (fixed order)
+void finalize_sum(hipStream_t s, const double* part, int n, double* out);
+// ---------------------------------------------------------------- row tiles of pooled levels (tiles.hip)
+// Rows grouped by block signature into tiles of 16 (one wavefront each, MFMA):
+// trow[16 t + n] row of tile row n (-1: none), tcol[(t W + u) 16 + n] its
+// u-th block's column, tpid[t W + u] that block's pool index, tmeta[2 t] =
+// {blocks, D^-1 pool index}.  A / Dinv: the panel-layout pools.
+bool tile_block_supported(int B);
+template <typename T>
+void tile_gs(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta, int W,
+             const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
+template <typename T>
+void tile_apply(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta,
+                int W, const T* A, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
 size_t mean_zero_part_size(long n, int k);
 void mean_zero(hipStream_t s, long n, int k, double* a, double* part);
 // Scratch of the allocation-free mean_zero entry point: a file-scope device

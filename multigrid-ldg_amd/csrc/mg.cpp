@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <unordered_map>
 
 #include "kernels.hpp"
@@ -634,6 +635,91 @@ static bool pool_blocks(hipStream_t s, long nblk, int bs, const double* blocks, 
     return true;
 }
 
+// Rows grouped by block signature (D^-1 pool index, blocks in order of pool
+// index then column) into tiles of 16 rows, signatures in order of first
+// appearance and rows in their given order within a signature.  blocks(i,
+// out): the (pool index, column) pairs of row i's summed blocks.
+struct TileHost {
+    std::vector<int> row, col, pid, meta;
+    int n = 0;
+};
+template <typename F>
+static void tile_rows(const std::vector<int>& rows, int W, const std::vector<int>& dkey, F&& blocks, TileHost& th)
+{
+    std::map<std::vector<int>, int> sig;
+    std::vector<std::vector<int>> members;
+    std::vector<std::pair<int, int>> bl;
+    for (int i : rows) {
+        blocks(i, bl);
+        std::sort(bl.begin(), bl.end());
+        std::vector<int> key;
+        key.reserve(bl.size() + 2);
+        key.push_back(dkey.empty() ? 0 : dkey[i]);
+        key.push_back((int)bl.size());
+        for (const auto& e : bl) key.push_back(e.first);
+        auto it = sig.find(key);
+        int id;
+        if (it == sig.end()) { id = (int)members.size(); sig.emplace(std::move(key), id); members.emplace_back(); }
+        else id = it->second;
+        members[id].push_back(i);
+    }
+    for (const std::vector<int>& m : members)
+        for (size_t c0 = 0; c0 < m.size(); c0 += 16) {
+            const int t = th.n++;
+            th.row.resize((size_t)th.n * 16, -1);
+            th.col.resize((size_t)th.n * W * 16, -1);
+            th.pid.resize((size_t)th.n * W, -1);
+            th.meta.resize((size_t)th.n * 2, 0);
+            for (size_t r = 0; r < 16 && c0 + r < m.size(); ++r) {
+                const int i = m[c0 + r];
+                blocks(i, bl);
+                std::sort(bl.begin(), bl.end());
+                th.row[(size_t)t * 16 + r] = i;
+                for (size_t u = 0; u < bl.size(); ++u) {
+                    th.col[((size_t)t * W + u) * 16 + r] = bl[u].second;
+                    if (r == 0) th.pid[(size_t)t * W + u] = bl[u].first;
+                }
+                if (r == 0) { th.meta[2 * t] = (int)bl.size(); th.meta[2 * t + 1] = dkey.empty() ? 0 : dkey[i]; }
+            }
+        }
+}
+
+// row tiles of a pooled level: GS per colour, application over all rows
+static void build_tiles(hipStream_t s, Level& lv, const std::vector<int>& pidx, const std::vector<int>& didx)
+{
+    const HostBSR& P = lv.A.pat;
+    TileHost g, a;
+    lv.tile_off.assign(lv.ncolor + 1, 0);
+    auto offd = [&](int i, std::vector<std::pair<int, int>>& bl) {
+        bl.clear();
+        bool diag = false;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+            if (P.cols[k] == i && !diag) { diag = true; continue; }
+            bl.push_back({pidx[k], P.cols[k]});
+        }
+    };
+    auto all = [&](int i, std::vector<std::pair<int, int>>& bl) {
+        bl.clear();
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) bl.push_back({pidx[k], P.cols[k]});
+    };
+    for (int c = 0; c < lv.ncolor; ++c) {
+        std::vector<int> rows;
+        for (int i = 0; i < lv.nb; ++i) if (lv.color[i] == c) rows.push_back(i);
+        tile_rows(rows, lv.gw, didx, offd, g);
+        lv.tile_off[c + 1] = g.n;
+    }
+    std::vector<int> rows(lv.nb);
+    std::iota(rows.begin(), rows.end(), 0);
+    lv.a_W = lv.A.ellw;
+    tile_rows(rows, lv.a_W, {}, all, a);
+    lv.a_ntiles = a.n;
+    lv.t_row.upload(g.row.data(), g.row.size(), s); lv.t_col.upload(g.col.data(), g.col.size(), s);
+    lv.t_pid.upload(g.pid.data(), g.pid.size(), s); lv.t_meta.upload(g.meta.data(), g.meta.size(), s);
+    lv.a_row.upload(a.row.data(), a.row.size(), s); lv.a_col.upload(a.col.data(), a.col.size(), s);
+    lv.a_pid.upload(a.pid.data(), a.pid.size(), s); lv.a_meta.upload(a.meta.data(), a.meta.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicographic, bool pool_off)
 {
     const int b = lv.B;
@@ -728,6 +814,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
     }
     group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev, lv.color_desc, lv.color_gcols);
     if (!lexicographic) pair_schedule(lv, s, pp, dp);
+    if (pooled && !lexicographic && dev::tile_block_supported(b)) build_tiles(s, lv, pidx, didx);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
     if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
@@ -876,6 +963,7 @@ void MGH::build(ProblemH* p, const MGParams& pm, const std::vector<std::vector<i
     HIPCHK(hipMemsetAsync(S.p, 0, 16 * sizeof(double), s));
     HIPCHK(hipHostMalloc((void**)&S_host, 16 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipStreamSynchronize(s));
+    select_tiles();
     setup_dev_s = secs(t0) - th;
     setup_host_s = th;
 }
@@ -947,6 +1035,53 @@ void MGH::sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) con
     }
 }
 
+template <typename T>
+void MGH::gs_colour(Level& lv, int c, int cur, int skip, const T* b, T* x)
+{
+    hipStream_t s = ctx->stream;
+    const T omega = (T)prm.omega;
+    if (lv.use_tiles[sizeof(T) == 4] & 1) {
+        const int t0 = lv.tile_off[c], nt = lv.tile_off[c + 1] - t0;
+        dev::tile_gs<T>(s, lv.B, nt, lv.t_row.p + 16 * (size_t)t0, lv.t_col.p + 16 * (size_t)lv.gw * t0,
+                        lv.t_pid.p + (size_t)lv.gw * t0, lv.t_meta.p + 2 * (size_t)t0, lv.gw, Mop<T>(lv), MDop<T>(lv), b, x,
+                        omega, lv.color_dev, cur, skip);
+        return;
+    }
+    const int o = lv.color_off[c];
+    dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o, lv.color_gcols.p + Sgw(lv) * o,
+                          lv.gw, Sop<T>(lv), SDop<T>(lv), b, x, omega, lv.color_dev, cur, skip, Spool(lv));
+}
+
+template <typename T>
+void MGH::apply_level(const Level& lv, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part)
+{
+    hipStream_t s = ctx->stream;
+    if (lv.use_tiles[sizeof(T) == 4] & 2)
+        dev::tile_apply<T>(s, lv.B, lv.a_ntiles, lv.a_row, lv.a_col, lv.a_pid, lv.a_meta, lv.a_W, Mop<T>(lv), x, b, y, dotv,
+                           dot_out, part);
+    else
+        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), x, b, y, dotv, dot_out, part,
+                           Spool(lv));
+}
+
+// Row tiles replace the vector kernels on pooled levels with at least
+// kTileMin tiles (about 32k block rows), in both precisions and for both the
+// GS sweep and the operator application.  Below that the tile launch has too
+// few workgroups to hide the MFMA chain's latency: on the 1024^2 p = 2
+// quadtree (nine levels) and the adaptive p = 3 hierarchy, timed one sweep
+// and one application at a time, the tiles take 0.6-0.9x the vector time on
+// the levels of 65k rows and more and up to 1.25x on the smaller ones.  A
+// fixed rule rather than a timed choice keeps the summation order, and so
+// the iterates, the same from run to run.
+void MGH::select_tiles()
+{
+    constexpr int kTileMin = 2048;
+    for (Level& lv : L) {
+        const int on = lv.a_ntiles >= kTileMin ? 3 : 0;
+        lv.use_tiles[0] = lv.use_tiles[1] = on;
+    }
+}
+
 // Level::relax (multigrid.h:202-262); Gauss-Seidel is multicolour on the GPU:
 // colours in increasing order forward, decreasing order in reverse.
 template <typename T>
@@ -968,11 +1103,8 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
         for (int sw = 0; sw < n; ++sw) {
             const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
             auto one = [&](int c) {
-                const int o = lv.color_off[c];
                 const int cur = (skip && c == 0) ? dev::kFirstFwd : c;   // (forward: colour 0 goes first)
-                dev::Ops<T>::gs_color(s, lv.B, lv.color_off[c + 1] - o, lv.color_desc.p + 4 * (size_t)o,
-                                      lv.color_gcols.p + Sgw(lv) * o, lv.gw, Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega,
-                                      lv.color_dev, cur, skip, Spool(lv));
+                gs_colour<T>(lv, c, cur, skip, v.b, v.x);
             };
             auto pair = [&] {
                 // the pair goes first in reverse, and forward when there are only two colours
@@ -984,6 +1116,17 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
             };
             if (!reverse) { for (int c = 0; c < nc - 2; ++c) one(c); pair(); }
             else { pair(); for (int c = nc - 3; c >= 0; --c) one(c); }
+        }
+    } else if (prm.relaxation == 1) {
+        // multicolour, one launch per colour: ascending (descending in reverse)
+        const int nc = lv.ncolor;
+        for (int sw = 0; sw < n; ++sw) {
+            const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
+            for (int cc = 0; cc < nc; ++cc) {
+                const int c = reverse ? nc - 1 - cc : cc;
+                const int cur = (skip && cc == 0) ? (reverse ? dev::kFirstRev : dev::kFirstFwd) : c;
+                gs_colour<T>(lv, c, cur, skip, v.b, v.x);
+            }
         }
     } else if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
@@ -1019,8 +1162,7 @@ void MGH::vcycle_from(int l0, bool x0_zero)
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
         relax<T>(l, prm.npre, false, zero);
-        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), v.x, v.b, v.r, nullptr, nullptr, nullptr,
-                           Spool(lv));
+        apply_level<T>(lv, v.x, v.b, v.r, nullptr, nullptr, nullptr);
         LevelVecs<T>& vc = vecs<T>(l + 1);
         dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): relax() from a zero guess starts
@@ -1182,15 +1324,13 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     auto waitS = [&]() { HIPCHK(hipEventSynchronize(es)); };
     auto fetchS = [&]() { copyS(); waitS(); };
     auto residual = [&](bool with_rr) {   // r = b - A x  (optionally r.r into S[RR])
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, Sell(f), f.A.ellw, Sop<double>(f), x, b, r, with_rr ? r.p : nullptr,
-                                with_rr ? S.p + dev::S_RR : nullptr, part, Spool(f));
+        apply_level<double>(f, x, b, r, with_rr ? r.p : nullptr, with_rr ? S.p + dev::S_RR : nullptr, part);
     };
     // q = A d, S[DQ] = d.q: the first kernel of an iteration.  It writes only
     // q and S[DQ], so it is enqueued before the host has tested the previous
     // iteration's scalars and is simply not used when the loop ends there.
     auto product = [&]() {
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, Sell(f), f.A.ellw, Sop<double>(f), d, nullptr, q, d, S.p + dev::S_DQ, part,
-                                Spool(f));
+        apply_level<double>(f, d, nullptr, q, d, S.p + dev::S_DQ, part);
     };
     // the preconditioner reads r in its own precision from L[0].v*.b and
     // leaves z in L[0].v*.x; the CG kernels convert on the fly
@@ -1625,9 +1765,8 @@ std::vector<SolveStats> MGH::adjoint(const double* dLdu, double* lambda, double*
 void MGH::spmv(int l, int precision, const void* x, void* y)
 {
     Level& lv = L[l];
-    hipStream_t s = ctx->stream;
-    if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<float>(lv), (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr, Spool(lv));
-    else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<double>(lv), (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr, Spool(lv));
+    if (precision == 1) apply_level<float>(lv, (const float*)x, nullptr, (float*)y, nullptr, nullptr, nullptr);
+    else apply_level<double>(lv, (const double*)x, nullptr, (double*)y, nullptr, nullptr, nullptr);
 }
 
 void MGH::smooth(int l, int precision, int n, bool reverse, const void* b, void* x)
@@ -1665,8 +1804,7 @@ void MGH::vcycles(int precision, const double* b, double* x, int ncycle, double 
         HIPCHK(hipStreamSynchronize(s));
     };
     auto resid = [&]() -> double {   // r = b - A x, ||r||
-        dev::Ops<double>::apply(s, f.B, f.nb, f.A.rowptr, Sell(f), f.A.ellw, Sop<double>(f), x, b, r, r.p, S.p + dev::S_RR, part,
-                                Spool(f));
+        apply_level<double>(f, x, b, r, r.p, S.p + dev::S_RR, part);
         fetch();
         return std::sqrt(S_host[dev::S_RR]);
     };
@@ -1813,15 +1951,8 @@ void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* byte
         for (int r = 0; r < reps; ++r) {
             if (part == 1)
                 for (int c = 0; c < nc - 2; ++c) {
-                    const int o = lv.color_off[c], nr = lv.color_off[c + 1] - o;
-                    const int* d = lv.color_desc.p + 4 * (size_t)o;
-                    const int* g = lv.color_gcols.p + Sgw(lv) * o;
-                    if (precision == 1)
-                        dev::Ops<float>::gs_color(s, lv.B, nr, d, g, lv.gw, Sop<float>(lv), SDop<float>(lv), lv.v32.b, lv.v32.x,
-                                                  (float)prm.omega, lv.color_dev, c, 0, Spool(lv));
-                    else
-                        dev::Ops<double>::gs_color(s, lv.B, nr, d, g, lv.gw, Sop<double>(lv), SDop<double>(lv), lv.v64.b, lv.v64.x,
-                                                   prm.omega, lv.color_dev, c, 0, Spool(lv));
+                    if (precision == 1) gs_colour<float>(lv, c, c, 0, lv.v32.b, lv.v32.x);
+                    else gs_colour<double>(lv, c, c, 0, lv.v64.b, lv.v64.x);
                 }
             else if (precision == 1)
                 dev::Ops<float>::gs_pair(s, lv.B, lv.pair_nwg, lv.pair_wg, 0, lv.pair_desc, lv.pair_gcols, lv.gw, Sop<float>(lv),
@@ -1856,8 +1987,8 @@ void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes
     const int sa = precision == 1 ? 4 : 8;
     if (precision == 1) dev::Ops<float>::fill(s, n, 1.f, lv.v32.x); else dev::Ops<double>::fill(s, n, 1.0, lv.v64.x);
     auto once = [&] {
-        if (precision == 1) dev::Ops<float>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<float>(lv), lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr, Spool(lv));
-        else dev::Ops<double>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<double>(lv), lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr, Spool(lv));
+        if (precision == 1) apply_level<float>(lv, lv.v32.x, nullptr, lv.v32.r, nullptr, nullptr, nullptr);
+        else apply_level<double>(lv, lv.v64.x, nullptr, lv.v64.r, nullptr, nullptr, nullptr);
     };
     once();
     hipEvent_t e0, e1;

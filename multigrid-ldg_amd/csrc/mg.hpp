@@ -147,6 +147,16 @@ struct Level {
     DBuf<int> pell, pdidx;
     DBuf<int> ppidx;                    // per block slot its pool index (batched MFMA kernels)
     std::vector<int> hpidx, hdidx;      // host copies of the pool indices (traffic accounting)
+    // row tiles of a pooled level (dev::tile_gs / tile_apply): rows with the
+    // same block signature in tiles of 16, per colour for GS (tile_off[c] ..
+    // tile_off[c + 1]) and over all rows for the operator application;
+    // use_tiles[precision] (0 FP64, 1 FP32) bit 0: GS, bit 1: application,
+    // set at setup from the level's tile count (MGH::select_tiles)
+    std::vector<int> tile_off;
+    DBuf<int> t_row, t_col, t_pid, t_meta;
+    int a_ntiles = 0, a_W = 0;
+    DBuf<int> a_row, a_col, a_pid, a_meta;
+    int use_tiles[2] = {0, 0};
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     // interpolation to the next coarser level
@@ -225,6 +235,12 @@ struct MGH {
     ~MGH();
 
     template <typename T> void relax(int l, int n, bool reverse, bool zero_guess);
+    // one colour c of a single-RHS GS sweep on level l (row tiles where chosen)
+    template <typename T> void gs_colour(Level& lv, int c, int cur, int skip, const T* b, T* x);
+    // y = A_l x or b - A_l x (+ y . dotv into *dot_out) on a level (row tiles where chosen)
+    template <typename T>
+    void apply_level(const Level& lv, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
+    void select_tiles();               // per pooled level: row tiles or vector kernels
     // the ordered GS launches of one sweep: fn(rows, nrows, colour array, current colour, skip mode)
     template <typename F> void sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) const;
     template <typename T> void vcycle_from(int l, bool x0_zero = true);
