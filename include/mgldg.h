@@ -166,6 +166,12 @@ typedef struct {
                           read them from a pool of the distinct blocks (the
                           same bits, so the same results, with a fraction of
                           the HBM traffic); 1: always the full block arrays */
+    int row_tiles;     /* pooled levels (no counterpart in the reference): the
+                          single-RHS GS and operator kernels on rows grouped
+                          in tiles of 16 with the same blocks, on the matrix
+                          cores.  0: on levels with at least 32k block rows;
+                          1: on every pooled level; 2: never.  The same
+                          operator; summation order differs in the last bits */
 } mgldg_mg_params;
 
 /* InterpolationHierarchy<N,P>(qt) + Multigrid<N,P>(ops, hierarchy, params)
@@ -248,6 +254,11 @@ int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols,
 /* no reference counterpart: the pool of level l (npool distinct A blocks,
  * ndpool distinct D^-1 blocks; 0, 0 when the level is not pooled) */
 int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool);
+/* no reference counterpart: the row tiles of level l (mgldg_mg_params
+ * row_tiles): *ntiles tiles of 16 rows over the level (0: none built),
+ * *width their table width, *used 1 when the single-RHS GS and operator
+ * kernels of the level run on them */
+int mgldg_mg_level_tiles(mgldg_mg mg, int level, int* ntiles, int* width, int* used);
 /* colouring used by the multicolour Gauss-Seidel on level l */
 int mgldg_mg_colors(mgldg_mg mg, int level, int* colors, int* ncolors);
 int mgldg_mg_destroy(mgldg_mg mg);

@@ -287,6 +287,8 @@ static MGParams mg_params(const mgldg_mg_params* prm)
     m.npre = prm->npre; m.npost = prm->npost; m.restriction = prm->restriction; m.precision = prm->precision;
     m.coarse_dense_max = prm->coarse_dense_max;
     m.pool_off = prm->pool_off != 0;
+    m.row_tiles = prm->row_tiles;
+    ARG(m.row_tiles >= 0 && m.row_tiles <= 2, "row_tiles must be 0, 1 or 2");
     ARG(m.coarse_dense_max >= 0 && m.coarse_dense_max <= 8192, "coarse_dense_max must be in [0, 8192]");
     return m;
 }
@@ -472,6 +474,18 @@ int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool)
         ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
         if (npool) *npool = mg->m.L[level].npool;
         if (ndpool) *ndpool = mg->m.L[level].ndpool;
+    });
+}
+
+int mgldg_mg_level_tiles(mgldg_mg mg, int level, int* ntiles, int* width, int* used)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(level >= 0 && level < (int)mg->m.L.size(), "level out of range");
+        const Level& lv = mg->m.L[level];
+        if (ntiles) *ntiles = lv.a_ntiles;
+        if (width) *width = lv.t_W;
+        if (used) *used = lv.use_tiles[0] != 0;
     });
 }
 

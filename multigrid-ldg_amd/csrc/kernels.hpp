@@ -212,16 +212,23 @@ void coldot(hipStream_t s, long n, int k, const double* a, const double* b, doub
 void finalize_sum(hipStream_t s, const double* part, int n, double* out);
 // ---------------------------------------------------------------- row tiles of pooled levels (tiles.hip)
 // Rows grouped by block signature into tiles of 16 (one wavefront each, MFMA):
-// trow[16 t + n] row of tile row n (-1: none), tcol[(t W + u) 16 + n] its
-// u-th block's column, tpid[t W + u] that block's pool index, tmeta[2 t] =
-// {blocks, D^-1 pool index}.  A / Dinv: the panel-layout pools.
-bool tile_block_supported(int B);
+// trow[16 t + n] row of tile row n (-1: none), tcol[(16 t + n) W + u] its
+// u-th off-diagonal block's column, tpid[t W + u] that block's pool index,
+// tmeta[2 t] = {blocks, D^-1 (GS) or diagonal (application) pool index}; W
+// = tile_width(most off-diagonal blocks of a row).  At / Dt: the pools in
+// the tile layout (tile_pool, tile_block_elems() elements per block).
+bool tile_supported(int B, int W);
+int tile_width(int W);
+int tile_block_elems();
+// mode 0: A pool (the A-operand layout), 1: D^-1 pool (the GS kernels' layout)
+template <typename T>
+void tile_pool(hipStream_t s, long np, int B, const double* src, T* dst, int mode);
 template <typename T>
 void tile_gs(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta, int W,
-             const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip);
+             const T* At, const T* Dt, const T* b, T* x, T omega, const int* color, int cur, int skip);
 template <typename T>
 void tile_apply(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta,
-                int W, const T* A, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
+                int W, const T* At, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
 size_t mean_zero_part_size(long n, int k);
 void mean_zero(hipStream_t s, long n, int k, double* a, double* part);
 // Scratch of the allocation-free mean_zero entry point: a file-scope device

@@ -635,10 +635,12 @@ static bool pool_blocks(hipStream_t s, long nblk, int bs, const double* blocks, 
     return true;
 }
 
-// Rows grouped by block signature (D^-1 pool index, blocks in order of pool
-// index then column) into tiles of 16 rows, signatures in order of first
-// appearance and rows in their given order within a signature.  blocks(i,
-// out): the (pool index, column) pairs of row i's summed blocks.
+// Rows grouped by block signature (D^-1 or diagonal pool index, blocks in
+// order of pool index then column) into tiles of 16 rows, signatures in
+// order of first appearance and rows in their given order within a
+// signature.  blocks(i, out): the (pool index, column) pairs of row i's
+// summed blocks.  Tables of width W: col[(16 t + r) W + u], pid[t W + u],
+// meta[2 t] = {blocks, key}.
 struct TileHost {
     std::vector<int> row, col, pid, meta;
     int n = 0;
@@ -676,7 +678,7 @@ static void tile_rows(const std::vector<int>& rows, int W, const std::vector<int
                 std::sort(bl.begin(), bl.end());
                 th.row[(size_t)t * 16 + r] = i;
                 for (size_t u = 0; u < bl.size(); ++u) {
-                    th.col[((size_t)t * W + u) * 16 + r] = bl[u].second;
+                    th.col[((size_t)t * 16 + r) * W + u] = bl[u].second;
                     if (r == 0) th.pid[(size_t)t * W + u] = bl[u].first;
                 }
                 if (r == 0) { th.meta[2 * t] = (int)bl.size(); th.meta[2 * t + 1] = dkey.empty() ? 0 : dkey[i]; }
@@ -684,11 +686,22 @@ static void tile_rows(const std::vector<int>& rows, int W, const std::vector<int
         }
 }
 
+static bool every_row_has_diagonal(const HostBSR& P)
+{
+    for (int i = 0; i + 1 < (int)P.rowptr.size(); ++i) {
+        bool d = false;
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1] && !d; ++k) d = P.cols[k] == i;
+        if (!d) return false;
+    }
+    return true;
+}
+
 // row tiles of a pooled level: GS per colour, application over all rows
 static void build_tiles(hipStream_t s, Level& lv, const std::vector<int>& pidx, const std::vector<int>& didx)
 {
     const HostBSR& P = lv.A.pat;
     TileHost g, a;
+    lv.t_W = dev::tile_width(lv.gw);
     lv.tile_off.assign(lv.ncolor + 1, 0);
     auto offd = [&](int i, std::vector<std::pair<int, int>>& bl) {
         bl.clear();
@@ -698,21 +711,36 @@ static void build_tiles(hipStream_t s, Level& lv, const std::vector<int>& pidx, 
             bl.push_back({pidx[k], P.cols[k]});
         }
     };
-    auto all = [&](int i, std::vector<std::pair<int, int>>& bl) {
-        bl.clear();
-        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) bl.push_back({pidx[k], P.cols[k]});
-    };
     for (int c = 0; c < lv.ncolor; ++c) {
         std::vector<int> rows;
         for (int i = 0; i < lv.nb; ++i) if (lv.color[i] == c) rows.push_back(i);
-        tile_rows(rows, lv.gw, didx, offd, g);
+        tile_rows(rows, lv.t_W, didx, offd, g);
         lv.tile_off[c + 1] = g.n;
     }
-    std::vector<int> rows(lv.nb);
+    // the application's tiles: the diagonal block's pool index as the key,
+    // the off-diagonal blocks in the tables
+    std::vector<int> rows(lv.nb), adiag(lv.nb, -1);
     std::iota(rows.begin(), rows.end(), 0);
-    lv.a_W = lv.A.ellw;
-    tile_rows(rows, lv.a_W, {}, all, a);
+    for (int i = 0; i < lv.nb; ++i)
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+            if (P.cols[k] == i) { adiag[i] = pidx[k]; break; }
+    tile_rows(rows, lv.t_W, adiag, offd, a);
     lv.a_ntiles = a.n;
+    // what the kernels index with: rows and columns in [-1, nb), pool
+    // indices in [-1, npool), keys in [0, ndpool) (GS) / [0, npool)
+    auto check = [&](const TileHost& th, int nkey) {
+        const int W = lv.t_W;
+        bool ok = th.row.size() == (size_t)th.n * 16 && th.col.size() == (size_t)th.n * 16 * W &&
+                  th.pid.size() == (size_t)th.n * W && th.meta.size() == (size_t)th.n * 2;
+        for (int v : th.row) ok = ok && v >= -1 && v < lv.nb;
+        for (int v : th.col) ok = ok && v >= -1 && v < lv.nb;
+        for (int v : th.pid) ok = ok && v >= -1 && v < lv.npool;
+        for (int t = 0; t < th.n; ++t)
+            ok = ok && th.meta[2 * t] >= 0 && th.meta[2 * t] <= W && th.meta[2 * t + 1] >= 0 && th.meta[2 * t + 1] < nkey;
+        if (!ok) throw Error(-5, "row tiles: table out of range");
+    };
+    check(g, lv.ndpool);
+    check(a, lv.npool);
     lv.t_row.upload(g.row.data(), g.row.size(), s); lv.t_col.upload(g.col.data(), g.col.size(), s);
     lv.t_pid.upload(g.pid.data(), g.pid.size(), s); lv.t_meta.upload(g.meta.data(), g.meta.size(), s);
     lv.a_row.upload(a.row.data(), a.row.size(), s); lv.a_col.upload(a.col.data(), a.col.size(), s);
@@ -749,7 +777,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
             for (int i = 0; i < lv.nb; ++i) didx[i] = drep[i] = i;
         }
         auto make_pool = [&](const std::vector<int>& rep, const double* src, DBuf<double>& p64, DBuf<float>& p32,
-                             DBuf<double>& r64, DBuf<float>& r32) {
+                             DBuf<double>& r64, DBuf<float>& r32, DBuf<double>& t64, DBuf<float>& t32, int tmode) {
             const long np = (long)rep.size();
             DBuf<int> rd; rd.upload(rep.data(), rep.size(), s);
             DBuf<double> rm; rm.alloc((size_t)np * b * b);
@@ -758,14 +786,18 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
             dev::to_panel_f64(s, np, b, rm, p64);
             r64.alloc((size_t)np * b * dev::pool_pad(8, b));
             dev::to_rowpad_f64(s, np, b, rm, r64);
+            t64.alloc((size_t)np * dev::tile_block_elems());
+            dev::tile_pool<double>(s, np, b, rm, t64, tmode);
             if (precision == 1) {
                 p32.alloc((size_t)np * b * b); dev::to_panel_f32(s, np, b, rm, p32);
                 r32.alloc((size_t)np * b * dev::pool_pad(4, b)); dev::to_rowpad_f32(s, np, b, rm, r32);
+                t32.alloc((size_t)np * dev::tile_block_elems());
+                dev::tile_pool<float>(s, np, b, rm, t32, tmode);
             }
             HIPCHK(hipStreamSynchronize(s));
         };
-        make_pool(prep, lv.A.val, lv.P64, lv.P32, lv.R64, lv.R32);
-        make_pool(drep, lv.Dinv_rm, lv.DP64, lv.DP32, lv.DR64, lv.DR32);
+        make_pool(prep, lv.A.val, lv.P64, lv.P32, lv.R64, lv.R32, lv.TA64, lv.TA32, 0);
+        make_pool(drep, lv.Dinv_rm, lv.DP64, lv.DP32, lv.DR64, lv.DR32, lv.TD64, lv.TD32, 1);
         const int W = lv.A.ellw;
         std::vector<int> pe((size_t)lv.nb * W * 2, 0);
         for (int i = 0; i < lv.nb; ++i) {
@@ -814,7 +846,10 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
     }
     group(lv.color, lv.ncolor, lv.color_off, lv.color_rows, lv.color_dev, lv.color_desc, lv.color_gcols);
     if (!lexicographic) pair_schedule(lv, s, pp, dp);
-    if (pooled && !lexicographic && dev::tile_block_supported(b)) build_tiles(s, lv, pidx, didx);
+    // row tiles where the D^-1 blocks repeat too (a tile's rows share one)
+    if (pooled && !lexicographic && 4 * (long)lv.ndpool <= lv.nb && dev::tile_supported(b, lv.gw) &&
+        every_row_has_diagonal(lv.A.pat))
+        build_tiles(s, lv, pidx, didx);
     const size_t n = (size_t)lv.nb * b;
     lv.v64.x.alloc(n); lv.v64.b.alloc(n); lv.v64.r.alloc(n); lv.v64.xalt.alloc(n);
     if (precision == 1) { lv.v32.x.alloc(n); lv.v32.b.alloc(n); lv.v32.r.alloc(n); lv.v32.xalt.alloc(n); }
@@ -1000,6 +1035,13 @@ template <> const float* Mop<float>(const Level& lv) { return lv.npool ? lv.P32.
 template <typename T> static const T* MDop(const Level& lv);
 template <> const double* MDop<double>(const Level& lv) { return lv.npool ? lv.DP64.p : lv.D64.p; }
 template <> const float* MDop<float>(const Level& lv) { return lv.npool ? lv.DP32.p : lv.D32.p; }
+// the pools in the row tiles' layout
+template <typename T> static const T* TAop(const Level& lv);
+template <> const double* TAop<double>(const Level& lv) { return lv.TA64.p; }
+template <> const float* TAop<float>(const Level& lv) { return lv.TA32.p; }
+template <typename T> static const T* TDop(const Level& lv);
+template <> const double* TDop<double>(const Level& lv) { return lv.TD64.p; }
+template <> const float* TDop<float>(const Level& lv) { return lv.TD32.p; }
 static const int* Sell(const Level& lv) { return lv.npool ? lv.pell.p : lv.A.ell.p; }
 static const int* Sdidx(const Level& lv) { return lv.npool ? lv.pdidx.p : nullptr; }
 static int Spool(const Level& lv) { return lv.npool ? 1 : 0; }
@@ -1042,9 +1084,9 @@ void MGH::gs_colour(Level& lv, int c, int cur, int skip, const T* b, T* x)
     const T omega = (T)prm.omega;
     if (lv.use_tiles[sizeof(T) == 4] & 1) {
         const int t0 = lv.tile_off[c], nt = lv.tile_off[c + 1] - t0;
-        dev::tile_gs<T>(s, lv.B, nt, lv.t_row.p + 16 * (size_t)t0, lv.t_col.p + 16 * (size_t)lv.gw * t0,
-                        lv.t_pid.p + (size_t)lv.gw * t0, lv.t_meta.p + 2 * (size_t)t0, lv.gw, Mop<T>(lv), MDop<T>(lv), b, x,
-                        omega, lv.color_dev, cur, skip);
+        dev::tile_gs<T>(s, lv.B, nt, lv.t_row.p + 16 * (size_t)t0, lv.t_col.p + 16 * (size_t)lv.t_W * t0,
+                        lv.t_pid.p + (size_t)lv.t_W * t0, lv.t_meta.p + 2 * (size_t)t0, lv.t_W, TAop<T>(lv), TDop<T>(lv), b,
+                        x, omega, lv.color_dev, cur, skip);
         return;
     }
     const int o = lv.color_off[c];
@@ -1057,8 +1099,8 @@ void MGH::apply_level(const Level& lv, const T* x, const T* b, T* y, const T* do
 {
     hipStream_t s = ctx->stream;
     if (lv.use_tiles[sizeof(T) == 4] & 2)
-        dev::tile_apply<T>(s, lv.B, lv.a_ntiles, lv.a_row, lv.a_col, lv.a_pid, lv.a_meta, lv.a_W, Mop<T>(lv), x, b, y, dotv,
-                           dot_out, part);
+        dev::tile_apply<T>(s, lv.B, lv.a_ntiles, lv.a_row, lv.a_col, lv.a_pid, lv.a_meta, lv.t_W, TAop<T>(lv), x, b, y,
+                           dotv, dot_out, part);
     else
         dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), x, b, y, dotv, dot_out, part,
                            Spool(lv));
@@ -1077,8 +1119,8 @@ void MGH::select_tiles()
 {
     constexpr int kTileMin = 2048;
     for (Level& lv : L) {
-        const int on = lv.a_ntiles >= kTileMin ? 3 : 0;
-        lv.use_tiles[0] = lv.use_tiles[1] = on;
+        const bool on = lv.a_ntiles > 0 && prm.row_tiles != 2 && (prm.row_tiles == 1 || lv.a_ntiles >= kTileMin);
+        lv.use_tiles[0] = lv.use_tiles[1] = on ? 3 : 0;
     }
 }
 

@@ -100,6 +100,7 @@ struct MGParams {
     int precision = 0;   // 0: FP64 only, 1: also FP32 copies for the mixed V-cycle
     int coarse_dense_max = 0;   // collapse the V-cycle below the first level with at most this many dofs (0: off)
     int pool_off = 0;           // 1: never pool repeated blocks for the single-RHS kernels
+    int row_tiles = 0;          // row-tile kernels on pooled levels: 0 by size, 1 every level, 2 never
 };
 
 template <typename T>
@@ -154,7 +155,9 @@ struct Level {
     // set at setup from the level's tile count (MGH::select_tiles)
     std::vector<int> tile_off;
     DBuf<int> t_row, t_col, t_pid, t_meta;
-    int a_ntiles = 0, a_W = 0;
+    int a_ntiles = 0, t_W = 0;            // application tiles; table width (4, 8 or 16)
+    DBuf<double> TA64, TD64;              // A and D^-1 pools in the tile layout
+    DBuf<float> TA32, TD32;
     DBuf<int> a_row, a_col, a_pid, a_meta;
     int use_tiles[2] = {0, 0};
     LevelVecs<double> v64;

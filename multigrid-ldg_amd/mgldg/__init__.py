@@ -314,16 +314,21 @@ class Multigrid(_Handle):
 
     pool (default True): levels whose operator repeats its blocks (quadtree
     levels) are read by the single-RHS kernels from a pool of the distinct
-    blocks -- the same bits, a fraction of the HBM traffic."""
+    blocks -- the same bits, a fraction of the HBM traffic.
+    row_tiles ("auto" | "on" | "off"): on pooled levels, the GS and operator
+    kernels that take rows with the same blocks 16 at a time on the matrix
+    cores -- "auto" on levels of at least 32k block rows.  Same operator,
+    summation order differing in the last bits."""
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
                  restriction="ratrat", precision="fp64", coarse_dense_max=8192, partitions=None, pdegrees=None,
-                 hcoarsen=True, pool=True):
+                 hcoarsen=True, pool=True, row_tiles="auto"):
         self.prob, self.ctx = prob, prob.ctx
         self.params = MGParams({"jacobi": 0, "gs": 1, "gs-lex": 2}[relaxation], omega,
                                {"cholesky": 0, "relaxation": 1}[coarse], npre, npost,
                                {"rat": 0, "ratrat": 1}[restriction], {"fp64": 0, "mixed": 1}[precision],
-                               coarse_dense_max, 0 if pool else 1)
+                               coarse_dense_max, 0 if pool else 1,
+                               {"auto": 0, "on": 1, "off": 2}[row_tiles])
         self.h = C.c_void_p()
         if partitions is not None:
             parts = np.zeros(len(partitions) * prob.ne, dtype=np.int32)
@@ -377,6 +382,12 @@ class Multigrid(_Handle):
         a, d = C.c_int(), C.c_int()
         check(self.ctx, lib().mgldg_mg_level_pool(self.h, level, C.byref(a), C.byref(d)))
         return a.value, d.value
+
+    def level_tiles(self, level):
+        """(tiles of 16 rows, table width, used by the single-RHS kernels) of a level"""
+        t, w, u = C.c_int(), C.c_int(), C.c_int()
+        check(self.ctx, lib().mgldg_mg_level_tiles(self.h, level, C.byref(t), C.byref(w), C.byref(u)))
+        return t.value, w.value, bool(u.value)
 
     def colors(self, level):
         nc = C.c_int()
@@ -524,7 +535,10 @@ def _ptr(t, numel=None, dtype="float64", name="tensor"):
     """Device pointer of a torch tensor, checked before anything reaches the
     library: the entry points read and write numel elements of dtype
     ("float64" or "float32") at that address, so a tensor of another type or
-    length would be an out-of-bounds device access."""
+    length would be an out-of-bounds device access.  The library works on
+    its own stream and returns once that stream is idle, so torch's work
+    still queued on its current stream (the kernel that filled or wrote the
+    tensor) is waited for here, before the library reads or writes it."""
     if not hasattr(t, "data_ptr"):
         raise MGLDGError(-1, f"{name}: expected a torch tensor")
     if str(t.dtype) != "torch." + dtype:
@@ -535,6 +549,8 @@ def _ptr(t, numel=None, dtype="float64", name="tensor"):
         raise MGLDGError(-1, f"{name}: expected a contiguous tensor")
     if not t.is_cuda:
         raise MGLDGError(-1, f"{name}: expected a device tensor")
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
     return C.c_void_p(t.data_ptr())
 
 

@@ -10,7 +10,7 @@ SIZEFN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_voi
 class MGParams(C.Structure):
     _fields_ = [("relaxation", C.c_int), ("omega", C.c_double), ("coarse", C.c_int), ("npre", C.c_int),
                 ("npost", C.c_int), ("restriction", C.c_int), ("precision", C.c_int), ("coarse_dense_max", C.c_int),
-                ("pool_off", C.c_int)]
+                ("pool_off", C.c_int), ("row_tiles", C.c_int)]
 
 
 class SolveParams(C.Structure):
@@ -65,6 +65,7 @@ SIGNATURES = {
     "mgldg_mg_export": (I, [V, I, I, PI, PI, PD, PL, PI, PI]),
     "mgldg_mg_colors": (I, [V, I, PI, PI]),
     "mgldg_mg_level_pool": (I, [V, I, PI, PI]),
+    "mgldg_mg_level_tiles": (I, [V, I, PI, PI, PI]),
     "mgldg_mg_destroy": (I, [V]),
     "mgldg_solve_device": (I, [V, V, V, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
     "mgldg_solve": (I, [V, PD, PD, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
