@@ -17,8 +17,10 @@ and mixed precision and a batch of 32 right-hand sides solved together.
 One step = one full solve from x = 0 with b resident in HBM.  `value` is the
 mixed-precision form (FP32 V-cycle inside FP64 CG); the plain FP64 GPU solve
 is timed beside it.  The roofline object describes the dominant kernel, the
-fine-level block Gauss-Seidel colour sweep of the FP32 V-cycle, timed with HIP
-events on the library's stream.  The CPU baseline is the FP64 oracle (a C
+fine-level one-colour block Gauss-Seidel launch of the FP32 V-cycle, timed
+with HIP events on the library's stream, and beside it the whole CG iteration
+(the library's cost model of its algorithmic bytes over the measured time per
+iteration, with the launches per V-cycle).  The CPU baseline is the FP64 oracle (a C
 restatement of the reference algorithm, one thread per solve as the
 reference) solving the same problem on the host: one core, and as many
 concurrent solves as the process has cores (all-core throughput); beside it
@@ -779,17 +781,23 @@ def main():
         del mg_np
         gsu = gbs(msu, byu)
         log(f"unpooled GS one-colour launch fp32: {msu * 1e3:.1f} us/launch, {byu / 1e6:.2f} MB -> {gsu:.0f} GB/s")
+        # the fine level's one-colour launches run on the row tiles when the
+        # level has enough of them (MFMA, rows with the same blocks 16 at a time)
+        ntile, twidth, tiled = mg.level_tiles(0)
+        kname = f"k_gs_tile<float, 16, 0, {twidth}>" if tiled else "k_gs<float, 16, 0, true>"
         traffic = traffic_u = None
         tp = os.path.join(ROOT, "profiles", "k_gs_c2_fine_pmc_traffic.json")
         if os.path.exists(tp):   # measured offline: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated
             kk = json.load(open(tp))["kernels"]
-            traffic = kk.get("void mgldg::dev::k_gs<float, 16, 0, true>", {}).get("traffic_bytes")
+            traffic = kk.get("void mgldg::dev::" + kname, {}).get("traffic_bytes")
             traffic_u = kk.get("void mgldg::dev::k_gs<float, 16, 0, false>", {}).get("traffic_bytes")
-        roofline = {"bound": "hbm", "kernel": "k_gs<float,16,0,true> (fine-level one-colour block GS launch, pooled operator)",
+        roofline = {"bound": "hbm", "kernel": kname.replace(" ", "") + " (fine-level one-colour block GS launch, pooled "
+                    "operator" + (f", row tiles: {ntile} tiles of 16 rows over the level" if tiled else "") + ")",
                     "achieved": round(gs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gs32 / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "bytes_per_launch": by32, "us_per_launch": round(ms32 * 1e3, 3),
-                    "limiter": "dependent-load latency: descriptor -> (column, pool) pairs -> x gathers and pool blocks "
-                               "from cache -> store; the pool removed 94% of the launch's HBM bytes",
+                    "limiter": "latency: about a thousand tiles per launch, one wavefront each (one per SIMD), each a "
+                               "chain of table loads -> x gathers -> 32 MFMAs -> D^-1 MFMAs -> store; the pool removed 94% "
+                               "of the launch's HBM bytes",
                     "unpooled": {"kernel": "k_gs<float,16,0,false>", "us_per_launch": round(msu * 1e3, 3),
                                  "bytes_per_launch": byu, "achieved": round(gsu, 1), "frac": round(gsu / HBM_PEAK_GBS, 4),
                                  "traffic": traffic_u},
@@ -859,6 +867,22 @@ def main():
     ndof = prob.ndof
     v64 = ndof * args.steps / dt64
     v32 = ndof * args.steps / dt32
+    # whole-iteration roofline: the cost model's algorithmic bytes of one CG
+    # iteration (V-cycle + product + updates + reductions) over the measured
+    # time per iteration; launches per V-cycle beside the captured graph's nodes
+    iteration = {}
+    for prec, dt, sts in (("mixed", dt32, st32), ("fp64", dt64, st64)):
+        c = mg.solve_cost(prec)
+        its = sts[-1]["iterations"] + 1          # the initial V-cycle and product besides the iterations
+        t_it = dt / args.steps / its
+        iteration[prec] = {"vcycle_launches": c["vcycle_launches"], "vcycle_graph_nodes": c["graph_nodes"],
+                           "iteration_launches": c["iteration_launches"], "MB_per_iteration": round(c["iteration_bytes"] / 1e6, 2),
+                           "us_per_iteration": round(t_it * 1e6, 1),
+                           "achieved": round(c["iteration_bytes"] / t_it / 1e9, 1),
+                           "frac": round(c["iteration_bytes"] / t_it / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"whole-iteration cost: {iteration}")
+    if roofline is not None:
+        roofline["iteration"] = iteration
 
     # batched form: k right-hand sides share every operator application
     k = args.nrhs

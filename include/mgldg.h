@@ -259,6 +259,15 @@ int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool);
  * *width their table width, *used 1 when the single-RHS GS and operator
  * kernels of the level run on them */
 int mgldg_mg_level_tiles(mgldg_mg mg, int level, int* ntiles, int* width, int* used);
+/* no reference counterpart: the cost model of the single-RHS MG-PCG in the
+ * given precision (flexible: the mixed form's r.z_old term) -- one
+ * preconditioner V-cycle's kernel launches and algorithmic HBM bytes (every
+ * stream counted once, as the launch reads it), the same for one whole CG
+ * iteration (V-cycle, operator product, vector updates and reductions), and
+ * the kernel nodes of the V-cycle's captured graph (-1 before the first
+ * graph solve) */
+int mgldg_mg_solve_cost(mgldg_mg mg, int precision, int flexible, int* vc_launches, double* vc_bytes,
+                        int* it_launches, double* it_bytes, int* graph_nodes);
 /* colouring used by the multicolour Gauss-Seidel on level l */
 int mgldg_mg_colors(mgldg_mg mg, int level, int* colors, int* ncolors);
 int mgldg_mg_destroy(mgldg_mg mg);

@@ -477,6 +477,25 @@ int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool)
     });
 }
 
+int mgldg_mg_solve_cost(mgldg_mg mg, int precision, int flexible, int* vc_launches, double* vc_bytes,
+                        int* it_launches, double* it_bytes, int* graph_nodes)
+{
+    if (!mg) return MGLDG_ERR_ARG;
+    return guard(mg->ctx, [&] {
+        ARG(precision == 0 || precision == 1, "unknown precision");
+        if (precision == 1) ARG(mg->m.L[0].A32.p, "mixed precision not built");
+        int n = 0, ni = 0;
+        double b = 0, bi = 0;
+        mg->m.vcycle_cost(precision, &n, &b);
+        mg->m.iteration_cost(precision, flexible != 0, &ni, &bi);
+        if (vc_launches) *vc_launches = n;
+        if (vc_bytes) *vc_bytes = b;
+        if (it_launches) *it_launches = n + ni;
+        if (it_bytes) *it_bytes = b + bi;
+        if (graph_nodes) *graph_nodes = mg->m.graph_nodes[precision];
+    });
+}
+
 int mgldg_mg_level_tiles(mgldg_mg mg, int level, int* ntiles, int* width, int* used)
 {
     if (!mg) return MGLDG_ERR_ARG;

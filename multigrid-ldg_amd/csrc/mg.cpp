@@ -1336,6 +1336,8 @@ void MGH::precon(int precision, bool use_graph)
         try { precon_enqueue(precision); }
         catch (...) { hipGraph_t junk; (void)hipStreamEndCapture(s, &junk); if (junk) (void)hipGraphDestroy(junk); throw; }
         HIPCHK(hipStreamEndCapture(s, &gr));
+        size_t nodes = 0;
+        if (hipGraphGetNodes(gr, nullptr, &nodes) == hipSuccess) graph_nodes[precision] = (int)nodes;
         HIPCHK(hipGraphInstantiate(&g, gr, nullptr, nullptr, 0));
         (void)hipGraphDestroy(gr);
     }
@@ -1653,6 +1655,8 @@ void MGH::bprecon(int precision, int k, bool use_graph)
         try { enqueue(); }
         catch (...) { hipGraph_t junk; (void)hipStreamEndCapture(s, &junk); if (junk) (void)hipGraphDestroy(junk); throw; }
         HIPCHK(hipStreamEndCapture(s, &gr));
+        size_t nodes = 0;
+        if (hipGraphGetNodes(gr, nullptr, &nodes) == hipSuccess) graph_nodes[precision] = (int)nodes;
         HIPCHK(hipGraphInstantiate(&g, gr, nullptr, nullptr, 0));
         (void)hipGraphDestroy(gr);
     }
@@ -1919,17 +1923,25 @@ void MGH::vcycle_matrix(int precision, double* V)
 }
 
 // ---------------------------------------------------------------------------- measurement
-// Algorithmic HBM bytes of one GS launch over the rows R (sweep not from
-// zero): off-diagonal blocks and column indices of R, its D^-1 blocks and
-// descriptors, b, x (read) and x (write) of R, and x of the distinct
-// off-diagonal neighbours of R (each read once).
-// Pooled level: the (column, pool index) pair of every off-diagonal block,
-// the distinct A and D^-1 pool blocks the launch touches (each once), the
-// descriptors, b, x (read) and x (write) of R, and x of its neighbours.
-static double gs_launch_bytes(const Level& lv, const int* rows, int nr, int sa, std::vector<int>& mark, int stamp)
+// Algorithmic HBM bytes of one GS launch over the rows R of one colour (or
+// of the pair launch's two colours), each stream counted once: b and x of R
+// (x not read in the first sweep from zero), x written, x of the distinct
+// neighbours the launch reads (skip / cur as the kernels: in the first sweep
+// from zero only neighbours already updated in sweep order, none for the
+// sweep's first launch; their colours are read), the operator's blocks and
+// the launch's tables.  Unpooled: every off-diagonal block with its column,
+// the D^-1 blocks and the 16-byte row descriptors.  Pooled: the (column,
+// pool index) pair of each off-diagonal block, the distinct A and D^-1 pool
+// blocks touched once each, the descriptors.  Row tiles (ntiles > 0): the
+// tiles' tables (rows, columns and pool indices at width W, two ints) and
+// the pool blocks in the tile layout.
+static double gs_launch_bytes(const Level& lv, const int* rows, int nr, int sa, std::vector<int>& mark, int stamp,
+                              int skip = 0, int cur = 0, int ntiles = 0, const std::vector<int>* colour = nullptr)
 {
+    const std::vector<int>& col = colour ? *colour : lv.color;
     const HostBSR& P = lv.A.pat;
     const double b2 = (double)lv.B * lv.B;
+    const bool none = (skip == 1 && cur <= dev::kFirstFwd) || (skip == 2 && cur >= dev::kFirstRev);
     double offd = 0, nbrs = 0, apool = 0, dpool = 0;
     std::vector<char> seenA(lv.npool, 0), seenD(lv.ndpool, 0);
     for (int q = 0; q < nr; ++q) {
@@ -1939,13 +1951,163 @@ static double gs_launch_bytes(const Level& lv, const int* rows, int nr, int sa, 
             const int j = P.cols[k];
             if (j == i) continue;
             offd += 1;
+            if (none) continue;
+            if (skip == 1 && col[j] > cur) continue;
+            if (skip == 2 && col[j] < cur) continue;
             if (lv.npool && !seenA[lv.hpidx[k]]) { seenA[lv.hpidx[k]] = 1; apool += 1; }
             if (mark[j] != stamp) { mark[j] = stamp; nbrs += 1; }
         }
     }
-    if (lv.npool)
-        return offd * 8.0 + (apool + dpool) * b2 * sa + nr * (16.0 + 3.0 * lv.B * sa) + nbrs * lv.B * sa;
-    return offd * (b2 * sa + 4.0) + nr * (b2 * sa + 16.0 + 3.0 * lv.B * sa) + nbrs * lv.B * sa;
+    const double vec = nr * ((skip ? 2.0 : 3.0) * lv.B * sa) + nbrs * lv.B * sa + ((skip && !none) ? 4.0 * offd : 0.0);
+    if (ntiles > 0)
+        return ntiles * (64.0 + 68.0 * lv.t_W + 8.0) + (apool + dpool) * dev::tile_block_elems() * sa + vec;
+    if (lv.npool) return (none ? 0.0 : offd * 8.0) + (apool + dpool) * b2 * sa + nr * 16.0 + vec;
+    return (none ? 0.0 : offd * (b2 * sa + 4.0)) + nr * (b2 * sa + 16.0) + vec;
+}
+
+// Algorithmic bytes of one operator application y = A x (or b - A x) on a
+// level: x once, y written (b read), the blocks as above (every block with
+// its column index and the row pointers; pooled: the distinct blocks and the
+// (column, pool index) pairs; row tiles: the tables and the tile-layout
+// pool).
+static double apply_bytes(const Level& lv, int sa, bool tiles, bool with_b)
+{
+    const double n = (double)lv.nb * lv.B, b2 = (double)lv.B * lv.B;
+    const double vec = (with_b ? 3.0 : 2.0) * n * sa;
+    if (tiles) return lv.a_ntiles * (64.0 + 68.0 * lv.t_W + 8.0) + (double)lv.npool * dev::tile_block_elems() * sa + vec;
+    if (lv.npool) return (double)lv.npool * b2 * sa + 8.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + vec;
+    return (double)lv.A.nnzb() * (b2 * sa + 4.0) + 4.0 * (lv.nb + 1) + vec;
+}
+
+// The launches of one V-cycle (the preconditioner of the single-RHS solve,
+// from x = 0, in the given precision) and their algorithmic bytes, walked in
+// the order precon_enqueue issues them: relaxations (as relax()), residuals,
+// restrictions, the coarse solve (dense inverse, collapsed map or
+// relaxations) and prolongations.
+void MGH::vcycle_cost(int precision, int* launches, double* bytes)
+{
+    const int sa = precision == 1 ? 4 : 8;
+    const int nl = (int)L.size();
+    const int bottom = cut > 0 ? cut : nl - 1;
+    long nlaunch = 0;
+    double by = 0;
+    auto relax_cost = [&](int l, int n, bool reverse, bool zero) {
+        const Level& lv = L[l];
+        const double nv = (double)lv.nb * lv.B;
+        if (zero && n == 0) { ++nlaunch; by += nv * sa; return; }
+        if (prm.relaxation == 0) {
+            // Jacobi ping-pongs between x and its partner, the first sweep
+            // from zero reading only b and D^-1; an odd count ends with a copy
+            const double blocks = lv.npool ? (double)(lv.npool + lv.ndpool) * lv.B * lv.B * sa + 8.0 * lv.A.nnzb() + 4.0 * lv.nb
+                                           : (double)(lv.A.nnzb() + lv.nb) * lv.B * lv.B * sa + 4.0 * lv.A.nnzb();
+            for (int sw = 0; sw < n; ++sw) {
+                ++nlaunch;
+                by += (zero && sw == 0) ? (double)(lv.npool ? lv.ndpool : lv.nb) * lv.B * lv.B * sa + 2.0 * nv * sa
+                                        : blocks + 4.0 * (lv.nb + 1) + 3.0 * nv * sa;
+            }
+            if (n % 2 == 1) { ++nlaunch; by += 2.0 * nv * sa; }
+            return;
+        }
+        std::vector<int> mark(lv.nb, -1);
+        int stamp = 0;
+        if (prm.relaxation == 2) {
+            // lexicographic: the wavefronts of the sweep's direction, in
+            // increasing order, the first from zero reading earlier ones
+            const std::vector<int>& col = reverse ? lv.rcolor : lv.color;
+            const int nw = reverse ? lv.rncolor : lv.ncolor;
+            std::vector<std::vector<int>> wf(nw);
+            for (int i = 0; i < lv.nb; ++i) wf[col[i]].push_back(i);
+            for (int sw = 0; sw < n; ++sw) {
+                const int skip = (zero && sw == 0) ? 1 : 0;
+                for (int c = 0; c < nw; ++c) {
+                    ++nlaunch;
+                    by += gs_launch_bytes(lv, wf[c].data(), (int)wf[c].size(), sa, mark, ++stamp, skip,
+                                          (skip && c == 0) ? dev::kFirstFwd : c, 0, &col);
+                }
+            }
+            return;
+        }
+        const int nc = lv.ncolor;
+        const bool tiles = lv.use_tiles[precision == 1] & 1;
+        const bool paired = prm.relaxation == 1 && lv.pair_nwg > 0;
+        std::vector<std::vector<int>> by_colour(nc);
+        for (int i = 0; i < lv.nb; ++i) by_colour[lv.color[i]].push_back(i);
+        for (int sw = 0; sw < n; ++sw) {
+            const int skip = (zero && sw == 0) ? (reverse ? 2 : 1) : 0;
+            auto single = [&](int c, int cur) {
+                ++nlaunch;
+                const int nt = tiles ? lv.tile_off[c + 1] - lv.tile_off[c] : 0;
+                by += gs_launch_bytes(lv, by_colour[c].data(), (int)by_colour[c].size(), sa, mark, ++stamp, skip, cur, nt);
+            };
+            if (paired) {
+                auto pair = [&](int cur0) {
+                    ++nlaunch;
+                    std::vector<int> r2 = by_colour[nc - 2];
+                    r2.insert(r2.end(), by_colour[nc - 1].begin(), by_colour[nc - 1].end());
+                    by += gs_launch_bytes(lv, r2.data(), (int)r2.size(), sa, mark, ++stamp, skip, cur0) + 12.0 * lv.pair_nwg;
+                };
+                if (!reverse) {
+                    for (int c = 0; c < nc - 2; ++c) single(c, (skip && c == 0) ? dev::kFirstFwd : c);
+                    pair((skip && nc == 2) ? dev::kFirstFwd : nc - 2);
+                } else {
+                    pair(skip ? dev::kFirstRev : nc - 1);
+                    for (int c = nc - 3; c >= 0; --c) single(c, c);
+                }
+            } else {
+                for (int cc = 0; cc < nc; ++cc) {
+                    const int c = reverse ? nc - 1 - cc : cc;
+                    single(c, (skip && cc == 0) ? (reverse ? dev::kFirstRev : dev::kFirstFwd) : c);
+                }
+            }
+        }
+    };
+    for (int l = 0; l < bottom; ++l) {
+        const Level& lv = L[l];
+        const Level& lc = L[l + 1];
+        relax_cost(l, prm.npre, false, true);
+        ++nlaunch; by += apply_bytes(lv, sa, lv.use_tiles[precision == 1] & 2, true);
+        // restriction: r of the fine level, b of the coarse, the child lists
+        // and the interpolation pool
+        ++nlaunch;
+        by += ((double)lv.nb * lv.B + (double)lc.nb * lc.B) * sa + 4.0 * (lc.nb + 1) + 8.0 * lv.nb +
+              (double)lv.I.pool.size() * sa;
+    }
+    const Level& c = L[bottom];
+    const double k = (double)c.nb * c.B;
+    if (bottom == cut) { ++nlaunch; by += k * k * ((precision == 1 && W32.p) ? 4.0 : 8.0) + 2.0 * k * sa; }
+    else if (prm.coarse == 0) { ++nlaunch; by += k * k * 8.0 + 2.0 * k * sa; }
+    else { relax_cost(bottom, prm.npre, false, true); relax_cost(bottom, prm.npost, true, false); }
+    for (int l = bottom - 1; l >= 0; --l) {
+        const Level& lv = L[l];
+        const Level& lc = L[l + 1];
+        // prolongation: x of the coarse level, x of the fine read and written,
+        // parents and types, the interpolation pool
+        ++nlaunch;
+        by += ((double)lc.nb * lc.B + 2.0 * lv.nb * lv.B) * sa + 8.0 * lv.nb + (double)lv.I.pool.size() * sa;
+        relax_cost(l, prm.npost, true, false);
+    }
+    *launches = (int)nlaunch;
+    *bytes = by;
+}
+
+// The CG work of one iteration besides the V-cycle (solve()'s loop): q = A d
+// with d.q (FP64, the fine level), x += alpha d and r -= alpha q with r.r and
+// r written in the V-cycle's precision, r.z (r.z_old, z_old written, in the
+// flexible form) with r.r, d = z + beta d; each reduction's finalize launch
+// and the 128-byte copy of the scalars.
+void MGH::iteration_cost(int precision, bool flexible, int* launches, double* bytes)
+{
+    const Level& f = L[0];
+    const int sv = precision == 1 ? 4 : 8;
+    const double n = (double)f.nb * f.B;
+    const double part = 8.0 * 1024 * 3;
+    double by = apply_bytes(f, 8, f.use_tiles[0] & 2, false);             // d read (also for d.q), q written
+    by += 8.0 * n * 4 + 8.0 * n * 2 + sv * n;                             // xr: x, d, r, q read; x, r written; v
+    by += 8.0 * n + sv * n + (flexible ? 2.0 * sv * n : 0.0);              // rz
+    by += sv * n + 16.0 * n;                                               // d
+    by += 3 * part + 128.0;
+    *launches = 4 + 3;
+    *bytes = by;
 }
 
 void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches, int part)
@@ -1973,9 +2135,11 @@ void MGH::profile_relax(int l, int precision, int nrep, double* ms, double* byte
         const int nc = lv.ncolor, nsingle = paired ? nc - 2 : nc;
         std::vector<int> host_rows(lv.nb);
         HIPCHK(hipMemcpy(host_rows.data(), lv.color_rows.p, sizeof(int) * lv.nb, hipMemcpyDeviceToHost));
+        const bool tiles = lv.use_tiles[precision == 1] & 1;
         if (part != 2)
             for (int c = 0; c < nsingle; ++c) {
-                by += gs_launch_bytes(lv, host_rows.data() + lv.color_off[c], lv.color_off[c + 1] - lv.color_off[c], sa, mark, c);
+                by += gs_launch_bytes(lv, host_rows.data() + lv.color_off[c], lv.color_off[c + 1] - lv.color_off[c], sa, mark, c,
+                                      0, 0, tiles ? lv.tile_off[c + 1] - lv.tile_off[c] : 0);
                 ++per;
             }
         if (paired && part != 1) {
@@ -2043,8 +2207,7 @@ void MGH::profile_spmv(int l, int precision, int nrep, double* ms, double* bytes
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     *ms = t / nrep;
-    *bytes = lv.npool ? (double)lv.npool * lv.B * lv.B * sa + 8.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 2.0 * n * sa
-                      : (double)lv.A.nnzb() * lv.B * lv.B * sa + 4.0 * lv.A.nnzb() + 4.0 * (lv.nb + 1) + 2.0 * n * sa;
+    *bytes = apply_bytes(lv, sa, lv.use_tiles[precision == 1] & 2, false);
 }
 
 }  // namespace mgldg

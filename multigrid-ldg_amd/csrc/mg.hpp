@@ -195,6 +195,7 @@ struct MGH {
     DBuf<double> part;                 // reduction partials (one per workgroup of the largest launch)
     double* S_host = nullptr;          // pinned
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_nodes[2] = {-1, -1};             // nodes of the captured preconditioner graphs
     double setup_host_s = 0, setup_dev_s = 0;
     int cut = -1;                      // first collapsed level (-1: none)
     int submean = 1;                   // coarsest b -= mean(b) (the quadtree Multigrid, multigrid.h:365-368)
@@ -258,6 +259,10 @@ struct MGH {
     // V[:, i] = V(e_i) from zero for every fine dof (host, row-major ndof x ndof)
     void vcycle_matrix(int precision, double* V);
     void profile_relax(int l, int precision, int nrep, double* ms, double* bytes, int* launches, int part = 0);
+    // launches and algorithmic bytes of one preconditioner V-cycle (precision 0 FP64, 1 FP32),
+    // and of the rest of one CG iteration (operator product, updates, reductions)
+    void vcycle_cost(int precision, int* launches, double* bytes);
+    void iteration_cost(int precision, bool flexible, int* launches, double* bytes);
     void profile_spmv(int l, int precision, int nrep, double* ms, double* bytes);
 };
 

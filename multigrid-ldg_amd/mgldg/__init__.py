@@ -383,6 +383,18 @@ class Multigrid(_Handle):
         check(self.ctx, lib().mgldg_mg_level_pool(self.h, level, C.byref(a), C.byref(d)))
         return a.value, d.value
 
+    def solve_cost(self, precision="fp64", flexible=None):
+        """cost model of the single-RHS MG-PCG: kernel launches and
+        algorithmic HBM bytes of one preconditioner V-cycle and of one whole
+        CG iteration, and the node count of the V-cycle's captured graph (-1
+        before a graph solve).  flexible defaults to the mixed form's use."""
+        fl = precision == "mixed" if flexible is None else flexible
+        n, b, ni, bi, g = C.c_int(), C.c_double(), C.c_int(), C.c_double(), C.c_int()
+        check(self.ctx, lib().mgldg_mg_solve_cost(self.h, _prec(precision), int(fl), C.byref(n), C.byref(b), C.byref(ni),
+                                                  C.byref(bi), C.byref(g)))
+        return {"vcycle_launches": n.value, "vcycle_bytes": b.value, "iteration_launches": ni.value,
+                "iteration_bytes": bi.value, "graph_nodes": g.value}
+
     def level_tiles(self, level):
         """(tiles of 16 rows, table width, used by the single-RHS kernels) of a level"""
         t, w, u = C.c_int(), C.c_int(), C.c_int()

@@ -66,6 +66,7 @@ SIGNATURES = {
     "mgldg_mg_colors": (I, [V, I, PI, PI]),
     "mgldg_mg_level_pool": (I, [V, I, PI, PI]),
     "mgldg_mg_level_tiles": (I, [V, I, PI, PI, PI]),
+    "mgldg_mg_solve_cost": (I, [V, I, I, PI, PD, PI, PD, PI]),
     "mgldg_mg_destroy": (I, [V]),
     "mgldg_solve_device": (I, [V, V, V, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
     "mgldg_solve": (I, [V, PD, PD, I, C.POINTER(SolveParams), C.POINTER(SolveStats)]),
