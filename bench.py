@@ -783,7 +783,7 @@ def main():
         log(f"unpooled GS one-colour launch fp32: {msu * 1e3:.1f} us/launch, {byu / 1e6:.2f} MB -> {gsu:.0f} GB/s")
         # the fine level's one-colour launches run on the row tiles when the
         # level has enough of them (MFMA, rows with the same blocks 16 at a time)
-        ntile, twidth, tiled = mg.level_tiles(0)
+        ntile, twidth, _, tiled = mg.level_tiles(0)
         kname = f"k_gs_tile<float, 16, 0, {twidth}>" if tiled else "k_gs<float, 16, 0, true>"
         traffic = traffic_u = None
         tp = os.path.join(ROOT, "profiles", "k_gs_c2_fine_pmc_traffic.json")

@@ -169,7 +169,8 @@ typedef struct {
     int row_tiles;     /* pooled levels (no counterpart in the reference): the
                           single-RHS GS and operator kernels on rows grouped
                           in tiles of 16 with the same blocks, on the matrix
-                          cores.  0: on levels with at least 32k block rows;
+                          cores.  0: on levels with at least 2048 tiles (32k
+                          block rows) in FP64, 256 tiles (4k rows) in FP32;
                           1: on every pooled level; 2: never.  The same
                           operator; summation order differs in the last bits */
 } mgldg_mg_params;
@@ -256,7 +257,8 @@ int mgldg_mg_export(mgldg_mg mg, int level, int which, int* rowptr, int* cols,
 int mgldg_mg_level_pool(mgldg_mg mg, int level, int* npool, int* ndpool);
 /* no reference counterpart: the row tiles of level l (mgldg_mg_params
  * row_tiles): *ntiles tiles of 16 rows over the level (0: none built),
- * *width their table width, *used 1 when the single-RHS GS and operator
+ * *width their table width, *used bit 0 (FP64) / bit 1 (FP32) set when the
+ * single-RHS GS and operator
  * kernels of the level run on them */
 int mgldg_mg_level_tiles(mgldg_mg mg, int level, int* ntiles, int* width, int* used);
 /* no reference counterpart: the cost model of the single-RHS MG-PCG in the

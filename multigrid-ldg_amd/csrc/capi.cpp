@@ -504,7 +504,7 @@ int mgldg_mg_level_tiles(mgldg_mg mg, int level, int* ntiles, int* width, int* u
         const Level& lv = mg->m.L[level];
         if (ntiles) *ntiles = lv.a_ntiles;
         if (width) *width = lv.t_W;
-        if (used) *used = lv.use_tiles[0] != 0;
+        if (used) *used = (lv.use_tiles[0] != 0 ? 1 : 0) | (lv.use_tiles[1] != 0 ? 2 : 0);
     });
 }
 

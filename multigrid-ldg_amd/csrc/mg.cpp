@@ -1107,21 +1107,23 @@ void MGH::apply_level(const Level& lv, const T* x, const T* b, T* y, const T* do
 }
 
 // Row tiles replace the vector kernels on pooled levels with at least
-// kTileMin tiles (about 32k block rows), in both precisions and for both the
-// GS sweep and the operator application.  Below that the tile launch has too
-// few workgroups to hide the MFMA chain's latency: on the 1024^2 p = 2
-// quadtree (nine levels) and the adaptive p = 3 hierarchy, timed one sweep
-// and one application at a time, the tiles take 0.6-0.9x the vector time on
-// the levels of 65k rows and more and up to 1.25x on the smaller ones.  A
-// fixed rule rather than a timed choice keeps the summation order, and so
-// the iterates, the same from run to run.
+// 256 tiles (4k block rows) in FP32 and 2048 tiles (32k rows) in FP64, for
+// both the GS sweep and the operator application.  Below that the tile
+// launch has too few wavefronts to hide its chain of loads and MFMAs (FP64
+// MFMAs take twice as long): timed one sweep and one application at a time
+// on the adaptive p = 3 hierarchy (levels of 65k, 17k, 4.5k and 1.3k rows),
+// the FP32 tiles take 0.75-0.9x the vector time from 4.5k rows up and the
+// FP64 tiles 1.0-1.3x below 65k rows.  A fixed rule rather than a timed
+// choice keeps the summation order, and so the iterates, the same from run
+// to run.
 void MGH::select_tiles()
 {
-    constexpr int kTileMin = 2048;
-    for (Level& lv : L) {
-        const bool on = lv.a_ntiles > 0 && prm.row_tiles != 2 && (prm.row_tiles == 1 || lv.a_ntiles >= kTileMin);
-        lv.use_tiles[0] = lv.use_tiles[1] = on ? 3 : 0;
-    }
+    const int kTileMin[2] = {2048, 256};
+    for (Level& lv : L)
+        for (int p = 0; p < 2; ++p) {
+            const bool on = lv.a_ntiles > 0 && prm.row_tiles != 2 && (prm.row_tiles == 1 || lv.a_ntiles >= kTileMin[p]);
+            lv.use_tiles[p] = on ? 3 : 0;
+        }
 }
 
 // Level::relax (multigrid.h:202-262); Gauss-Seidel is multicolour on the GPU:

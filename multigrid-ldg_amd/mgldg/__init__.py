@@ -317,7 +317,8 @@ class Multigrid(_Handle):
     blocks -- the same bits, a fraction of the HBM traffic.
     row_tiles ("auto" | "on" | "off"): on pooled levels, the GS and operator
     kernels that take rows with the same blocks 16 at a time on the matrix
-    cores -- "auto" on levels of at least 32k block rows.  Same operator,
+    cores -- "auto" from 2048 tiles (32k block rows) in FP64 and 256 tiles
+    (4k rows) in FP32.  Same operator,
     summation order differing in the last bits."""
 
     def __init__(self, prob, relaxation="gs", omega=0.8, coarse="cholesky", npre=3, npost=3,
@@ -396,10 +397,11 @@ class Multigrid(_Handle):
                 "iteration_bytes": bi.value, "graph_nodes": g.value}
 
     def level_tiles(self, level):
-        """(tiles of 16 rows, table width, used by the single-RHS kernels) of a level"""
+        """(tiles of 16 rows, table width, used by the FP64 single-RHS
+        kernels, used by the FP32 ones) of a level"""
         t, w, u = C.c_int(), C.c_int(), C.c_int()
         check(self.ctx, lib().mgldg_mg_level_tiles(self.h, level, C.byref(t), C.byref(w), C.byref(u)))
-        return t.value, w.value, bool(u.value)
+        return t.value, w.value, bool(u.value & 1), bool(u.value & 2)
 
     def colors(self, level):
         nc = C.c_int()

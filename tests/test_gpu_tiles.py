@@ -1,7 +1,7 @@
 """GPU: the row-tile kernels of pooled levels (rows with the same blocks, 16 at
 a time on the matrix cores) against the CPU oracle, with the tiles forced
-onto every pooled level (row_tiles="on"; by default only levels of 32k rows
-and more use them).
+onto every pooled level (row_tiles="on"; by default only levels of 2048
+tiles and more in FP64, 256 in FP32).
 
 Bars as the other parity tests: SpMV, Gauss-Seidel sweeps (forward and
 reverse, multicolour as the oracle's "mcgs") and the V-cycle within 1e-12
@@ -95,13 +95,18 @@ def test_tile_pcg_matches_oracle(gctx, oracle, mesh, p):
 
 def test_tiles_off_auto_on(gctx):
     """row_tiles: "off" builds the tables but never uses them, "auto" uses them
-    only from 2048 tiles (32k rows), "on" on every level that has them"""
+    from 2048 tiles (32k rows) in FP64 and 256 tiles in FP32, "on" on every
+    level that has them"""
     import mgldg
     qt = mgldg.Quadtree(gctx, 2, lambda x: 1 / 32)
     pr = mgldg.LDGPoisson(qt, 2)
     for mode, want in (("off", False), ("auto", False), ("on", True)):
-        mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, row_tiles=mode)
-        t, w, used = mg.level_tiles(0)
-        assert t >= 64 and w == 4 and used == want
+        mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, row_tiles=mode, precision="mixed")
+        t, w, u64, u32 = mg.level_tiles(0)
+        assert 64 <= t < 256 and w == 4 and u64 == want and u32 == want
+    qt = mgldg.Quadtree(gctx, 2, lambda x: 1 / 128)          # 16,384 rows: about 1,100 tiles
+    mg = mgldg.Multigrid(mgldg.LDGPoisson(qt, 2), "gs", npre=2, npost=2, precision="mixed")
+    t, w, u64, u32 = mg.level_tiles(0)
+    assert 256 <= t < 2048 and not u64 and u32
     with pytest.raises(KeyError):
         mgldg.Multigrid(pr, "gs", row_tiles="always")
