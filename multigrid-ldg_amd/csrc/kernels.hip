@@ -696,6 +696,192 @@ __global__ __launch_bounds__(256) void k_gs_pair(const int* __restrict__ wg, int
     }
 }
 
+// ------------------------------------------------- pooled GS, latency form
+// A pooled GS launch of a few thousand rows is bound by the chain of
+// dependent loads of each row, not by bytes: descriptor and pairs -> x_j
+// gathers (rounds of UX blocks) -> the pool rows of those blocks (from cache,
+// after the gathers arrive) -> D^-1 (after the sum) -> store.  The latency
+// form issues everything a row needs -- b_i, x_i, its D^-1 pool row, every
+// x_j (table width W <= kLatW, one pair per lane) with its colour, and the
+// pool rows of the first UA blocks -- at once as soon as the row's
+// descriptor and pairs have arrived, so a row waits for two round trips.
+// The products and the order of every sum are those of the pooled row_sum /
+// gs_rows (table order, the same VP-wide partial sums, the same D^-1
+// product), so the results are the same bits.  It holds more registers and
+// is launched only where the launch is too small to need occupancy (and for
+// the two-colour launch, whose rows are sparse per workgroup).
+constexpr int kLatW = 8;
+
+template <typename T, int B>
+struct LatCfg {
+    static constexpr int VP = vp_of<T>(), BP = pad_of<T, B>();
+    static constexpr int NV = BP / VP;   // vector loads per pool row
+    // pool rows kept in registers: at most 128 VGPRs
+    static constexpr int UA0 = (128 * 4) / (BP * (int)sizeof(T));
+    static constexpr int UA = UA0 < kLatW ? UA0 : kLatW;
+    static constexpr int XS = kLatW * BP;   // staged x_j per row slot
+};
+
+template <typename T, int B, int SKIP>
+struct LatRow {
+    using C = LatCfg<T, B>;
+    bool valid = false;
+    int4 d = make_int4(0, 0, 0, 0);
+    int2 e = make_int2(-1, 0);
+    T bi = 0, xi = 0;
+    PadRow<T, B> dp;
+    T xv[kLatW];
+    int pid[kLatW];
+    bool use[kLatW];
+    VecT<T, C::VP> a[C::UA][C::NV];
+
+    // first round trip: the descriptor and the (column, pool index) pairs,
+    // pair r on lane r of the row (W <= B)
+    __device__ __forceinline__ void tables(int idx, bool v, const int4* __restrict__ desc, const int* __restrict__ gcols,
+                                           int W, int r)
+    {
+        valid = v;
+        if (v) d = desc[idx];
+        if (v && r < W) e = reinterpret_cast<const int2*>(gcols)[(size_t)idx * W + r];
+    }
+    // the row's own operands (x_i only when the sweep does not start from zero)
+    __device__ __forceinline__ void own(const T* __restrict__ b, const T* x, const T* __restrict__ Dinv, int r)
+    {
+        if (!valid) return;
+        const size_t o = (size_t)d.x * B + r;
+        bi = b[o];
+        if (SKIP == 0) xi = x[o];
+        load_dinv_row_pad<T, B>(Dinv, d.w, r, dp);
+    }
+    // second round trip: every x_j with its colour (first sweep) and the
+    // pool rows of the first UA blocks.  Uniform trip counts: every shuffle
+    // runs with all lanes active.
+    __device__ __forceinline__ void nbrs(const T* __restrict__ A, const T* x, const int* __restrict__ color, int cur, int W,
+                                         int r, int lane0)
+    {
+        const bool none = reads_none<SKIP>(cur);
+#pragma unroll
+        for (int u = 0; u < kLatW; ++u) {
+            const int src = lane0 + (u < B ? u : B - 1);
+            const int j = __shfl(e.x, src, 64);
+            pid[u] = __shfl(e.y, src, 64);
+            const bool live = !none && valid && u < W && j >= 0;
+            xv[u] = live ? x[(size_t)j * B + r] : (T)0;
+            bool ok = live;
+            if (SKIP != 0 && live) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
+            use[u] = ok;
+            if (u < C::UA && live) {
+                const T* Ab = A + (size_t)pid[u] * (B * C::BP) + r * C::VP;
+#pragma unroll
+                for (int cc = 0; cc < C::NV; ++cc) a[u < C::UA ? u : 0][cc] = ldv<T, C::VP>(Ab + cc * B * C::VP);
+            }
+        }
+    }
+    // the row sum, the D^-1 product and the damped update of x_i
+    __device__ __forceinline__ void finish(const T* __restrict__ A, T* x, T omega, int r, T* xs, T* t)
+    {
+        constexpr int VP = C::VP, BP = C::BP;
+        wave_sync();   // the slot's previous readers (an earlier row of this wavefront) are done
+#pragma unroll
+        for (int u = 0; u < kLatW; ++u) xs[u * BP + r] = use[u] ? xv[u] : (T)0;
+        if (r < BP - B) {
+#pragma unroll
+            for (int u = 0; u < kLatW; ++u) xs[u * BP + B + r] = (T)0;
+        }
+        wave_sync();
+        T pacc[VP];
+#pragma unroll
+        for (int v = 0; v < VP; ++v) pacc[v] = 0;
+#pragma unroll
+        for (int u = 0; u < kLatW; ++u)
+            if (use[u]) {
+                const T* Ab = A + (size_t)pid[u] * (B * BP) + r * VP;
+#pragma unroll
+                for (int cc = 0; cc < C::NV; ++cc) {
+                    const VecT<T, VP> av = u < C::UA ? a[u < C::UA ? u : 0][cc] : ldv<T, VP>(Ab + cc * B * VP);
+                    const VecT<T, VP> xq = ldv<T, VP>(xs + u * BP + cc * VP);
+#pragma unroll
+                    for (int v = 0; v < VP; ++v) pacc[v] += av.v[v] * xq.v[v];
+                }
+            }
+        T acc = pacc[0];
+#pragma unroll
+        for (int v = 1; v < VP; ++v) acc += pacc[v];
+        t[r] = bi - acc;
+        if (r < RowLds<T, B>::TB - B) t[B + r] = (T)0;
+        wave_sync();
+        if (valid) {
+            const T s = dinv_row_pad<T, B>(dp, t);
+            x[(size_t)d.x * B + r] = xi + omega * (s - xi);
+        }
+    }
+};
+
+// k_gs<T, B, SKIP, true> in the latency form
+template <typename T, int B, int SKIP>
+__global__ __launch_bounds__(256) void k_gs_lat(int nrows, const int4* __restrict__ desc, const int* __restrict__ gcols,
+                                                int W, const T* __restrict__ A, const T* __restrict__ Dinv,
+                                                const T* __restrict__ b, T* x, T omega, const int* __restrict__ color,
+                                                int cur)
+{
+    using M = Map<T, B>;
+    using R = RowLds<T, B>;
+    using C = LatCfg<T, B>;
+    __shared__ __align__(16) T xs_all[4 * R::RW * C::XS];
+    __shared__ __align__(16) T tb[4 * R::RW * R::TB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B;
+    const int idx = xcd_block() * M::RPB + wave * M::RPW + slot;
+    LatRow<T, B, SKIP> q;
+    q.tables(idx, slot < M::RPW && idx < nrows, desc, gcols, W, r);
+    q.own(b, x, Dinv, r);
+    q.nbrs(A, x, color, cur, W, r, slot * B);
+    q.finish(A, x, omega, r, xs_all + (wave * R::RW + slot) * C::XS, tb + (wave * R::RW + slot) * R::TB);
+}
+
+// k_gs_pair<T, B, SKIP, true> in the latency form.  Each of a workgroup's
+// two ranges holds at most one workgroup of rows (pair_schedule builds them
+// so), so one row per row slot in each phase: the second range's tables and
+// own operands (rows of the other colour, which the first phase does not
+// write) are loaded with the first's, and after the barrier only its x_j
+// gathers and pool rows remain.
+template <typename T, int B, int SKIP>
+__global__ __launch_bounds__(256) void k_gs_pair_lat(const int* __restrict__ wg, int reverse, const int4* __restrict__ desc,
+                                                     const int* __restrict__ gcols, int W, const T* __restrict__ A,
+                                                     const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
+                                                     const int* __restrict__ color, int cur0, int cur1)
+{
+    using M = Map<T, B>;
+    using R = RowLds<T, B>;
+    using C = LatCfg<T, B>;
+    __shared__ __align__(16) T xs_all[4 * R::RW * C::XS];
+    __shared__ __align__(16) T tb[4 * R::RW * R::TB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B;
+    const int lb = xcd_block();
+    const int s = wg[3 * lb], m = wg[3 * lb + 1], e = wg[3 * lb + 2];
+    const int b0 = reverse ? m : s, e0 = reverse ? e : m;   // first range processed
+    const int b1 = reverse ? s : m, e1 = reverse ? m : e;   // second
+    const int off = wave * M::RPW + slot;
+    const bool inw = slot < M::RPW;
+    LatRow<T, B, SKIP> p, q;
+    p.tables(b0 + off, inw && b0 + off < e0, desc, gcols, W, r);
+    q.tables(b1 + off, inw && b1 + off < e1, desc, gcols, W, r);
+    p.own(b, x, Dinv, r);
+    q.own(b, x, Dinv, r);
+    T* xs = xs_all + (wave * R::RW + slot) * C::XS;
+    T* t = tb + (wave * R::RW + slot) * R::TB;
+    p.nbrs(A, x, color, cur0, W, r, slot * B);
+    p.finish(A, x, omega, r, xs, t);
+    if (m != s && m != e) __syncthreads();   // (uniform: s, m, e are per workgroup)
+    q.nbrs(A, x, color, cur1, W, r, slot * B);
+    q.finish(A, x, omega, r, xs, t);
+}
+
+// block sizes with a latency form (the 2-D quadtree blocks, p = 1, 2, 3)
+constexpr bool lat_block(int B) { return B == 4 || B == 9 || B == 16; }
+constexpr long kSimds = 1024;   // 256 CUs x 4 SIMDs
+
 // POOL: ell holds (column, pool index) pairs, didx the rows' D^-1 pool indices
 template <typename T, int B, bool POOL = false>
 __global__ __launch_bounds__(256) void k_jacobi(int nrows, const int* __restrict__ rowptr, const int* __restrict__ ell,
@@ -921,6 +1107,19 @@ void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const in
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
         const dim3 g(nblk(nrows, M::RPB));
+        // the latency form for launches of at most two wavefronts per SIMD
+        if constexpr (lat_block(BB)) {
+            if (pool && W <= std::min(BB, kLatW) && (long)g.x * 4 <= 2L * kSimds) {
+                if (skip == 1)
+                    hipLaunchKernelGGL((k_gs_lat<T, BB, 1>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+                else if (skip == 2)
+                    hipLaunchKernelGGL((k_gs_lat<T, BB, 2>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+                else
+                    hipLaunchKernelGGL((k_gs_lat<T, BB, 0>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+                check_launch("gs_color");
+                return;
+            }
+        }
         if (pool) {
             if (skip == 1)
                 hipLaunchKernelGGL((k_gs<T, BB, 1, true>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
@@ -947,6 +1146,21 @@ void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, 
     const int4* d = reinterpret_cast<const int4*>(desc);
     MGLDG_DISPATCH(B, {
         const dim3 g(nwg);
+        if constexpr (lat_block(BB)) {
+            if (pool && W <= std::min(BB, kLatW)) {
+                if (skip == 1)
+                    hipLaunchKernelGGL((k_gs_pair_lat<T, BB, 1>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
+                                       omega, color, cur0, cur1);
+                else if (skip == 2)
+                    hipLaunchKernelGGL((k_gs_pair_lat<T, BB, 2>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
+                                       omega, color, cur0, cur1);
+                else
+                    hipLaunchKernelGGL((k_gs_pair_lat<T, BB, 0>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
+                                       omega, color, cur0, cur1);
+                check_launch("gs_pair");
+                return;
+            }
+        }
         if (pool) {
             if (skip == 1)
                 hipLaunchKernelGGL((k_gs_pair<T, BB, 1, true>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,

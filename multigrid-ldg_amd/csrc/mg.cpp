@@ -1442,6 +1442,9 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
         copyS();
         product();
         waitS();
+        // S was copied before product() was enqueued: S_host[DQ] is d.q of
+        // the direction this iteration used (as in the batched loop), not of
+        // the next one, so a converged z = 0 never reads as a breakdown here
         if (!(S_host[dev::S_DQ] > 0)) { st.status = std::isfinite(S_host[dev::S_DQ]) ? 2 : 3; break; }
         delta = S_host[dev::S_DELTA]; rr = S_host[dev::S_RR];
     }
