@@ -594,13 +594,17 @@ __global__ __launch_bounds__(256) void k_apply(int nrows, const int* __restrict_
     const int k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
     const size_t o = (size_t)row * B + r;
     const T bo = (valid && b) ? b[o] : (T)0;
+    // the dot's other vector: y itself (its value is `out`), or an input
+    // loaded with the row's other operands
+    const bool dot_self = dotv == y;
+    const T dw = (valid && dotv && !dot_self) ? dotv[o] : (T)0;
     const T acc = row_sum<T, B, 0, POOL>(valid, k0, k1, -1, ell + (size_t)row * W * (POOL ? 2 : 1), W, A, x, r, slot * B,
                                          xs_all + (wave * R::RW + slot) * R::XS, nullptr, 0);
     double dv = 0;
     if (valid) {
         T out = b ? (T)(bo - acc) : acc;
         y[o] = out;
-        if (dotv) dv = (double)out * (double)dotv[o];
+        if (dotv) dv = (double)out * (double)(dot_self ? out : dw);
     }
     if (part) block_partial(dv, part, lb);   // partials in row order, whatever the placement
 }
@@ -985,6 +989,116 @@ __global__ __launch_bounds__(256) void k_prolong(int nf, const int* __restrict__
     xf[(size_t)k * BF + r] += acc;
 }
 
+// Quadtree interpolation in the latency form (BF == BC == B, at most kKidW
+// children per coarse element, a pool of a few blocks).  The loops above
+// walk a coarse row's children one dependent load after another (kid_off ->
+// kids -> type -> blocks and r), and their lanes read block rows and columns
+// at a stride of B elements, one cache line per lane and load.  Here a
+// coarse row's table entry comes in one load and every operand in the next
+// round trip: kpair[I kKidW + s] = (child, block) of coarse row I in kids[]
+// order (-1: none).  The pools are in the padded panel layout of the A
+// pools (row r of a block as pad_of / VP aligned 16-byte vectors, the B
+// lanes of a row reading consecutive vectors): P holds the blocks, PT their
+// transposes (lane c reads column c).  Vectors pass through LDS, one
+// coalesced load per row.  Both walk coarse rows: the prolongation updates
+// the coarse row's children, so a parent's x_c is read once.  Products and
+// sums in the same order as k_restrict / k_prolong (the pad columns are
+// never summed): the same bits.
+constexpr int kKidW = kKidWidth;
+
+template <typename T, int B>
+struct InterpLat {
+    static constexpr int VP = vp_of<T>(), BP = pad_of<T, B>(), NV = BP / VP;
+};
+
+// row r of panel block t: o[0 .. BP)
+template <typename T, int B>
+__device__ __forceinline__ void panel_row(T* o, const T* __restrict__ P, int t, int r)
+{
+    using C = InterpLat<T, B>;
+    const T* p = P + (size_t)t * (B * C::BP) + r * C::VP;
+#pragma unroll
+    for (int cc = 0; cc < C::NV; ++cc) {
+        const VecT<T, C::VP> v = ldv<T, C::VP>(p + cc * B * C::VP);
+#pragma unroll
+        for (int e = 0; e < C::VP; ++e) o[cc * C::VP + e] = v.v[e];
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_restrict_lat(int nc, const int2* __restrict__ kpair, const T* __restrict__ PT,
+                                                      const T* __restrict__ r, T* __restrict__ bc)
+{
+    using M = Map<T, B>;
+    using C = InterpLat<T, B>;
+    __shared__ __align__(16) T rs_all[4 * (M::RPW + 1) * kKidW * B];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, c = lane - slot * B;
+    const int I = xcd_block() * M::RPB + wave * M::RPW + slot;
+    const bool valid = slot < M::RPW && I < nc;
+    int2 e = make_int2(-1, 0);
+    if (valid && c < kKidW) e = kpair[(size_t)I * kKidW + c];
+    T rv[kKidW], tc[kKidW][C::BP];
+    bool live[kKidW];
+#pragma unroll
+    for (int s = 0; s < kKidW; ++s) {
+        const int src = slot * B + (s < B ? s : B - 1);
+        const int k = __shfl(e.x, src, 64), ty = __shfl(e.y, src, 64);
+        live[s] = valid && k >= 0;
+        rv[s] = live[s] ? r[(size_t)k * B + c] : (T)0;
+        if (live[s]) panel_row<T, B>(tc[s], PT, ty, c);
+    }
+    T* rs = rs_all + (wave * (M::RPW + 1) + slot) * (kKidW * B);
+#pragma unroll
+    for (int s = 0; s < kKidW; ++s) rs[s * B + c] = rv[s];
+    wave_sync();
+    T acc = 0;
+#pragma unroll
+    for (int s = 0; s < kKidW; ++s)
+        if (live[s]) {
+#pragma unroll
+            for (int q = 0; q < B; ++q) acc += tc[s][q] * rs[s * B + q];
+        }
+    if (valid) bc[(size_t)I * B + c] = acc;
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_prolong_lat(int nc, const int2* __restrict__ kpair, const T* __restrict__ P,
+                                                     const T* __restrict__ xc, T* __restrict__ xf)
+{
+    using M = Map<T, B>;
+    using C = InterpLat<T, B>;
+    __shared__ __align__(16) T xs_all[4 * (M::RPW + 1) * B];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / B, r = lane - slot * B;
+    const int I = xcd_block() * M::RPB + wave * M::RPW + slot;
+    const bool valid = slot < M::RPW && I < nc;
+    int2 e = make_int2(-1, 0);
+    if (valid && r < kKidW) e = kpair[(size_t)I * kKidW + r];
+    const T xv = valid ? xc[(size_t)I * B + r] : (T)0;
+    T xo[kKidW], tr[kKidW][C::BP];
+    int ks[kKidW];
+#pragma unroll
+    for (int s = 0; s < kKidW; ++s) {
+        const int src = slot * B + (s < B ? s : B - 1);
+        const int k = __shfl(e.x, src, 64), ty = __shfl(e.y, src, 64);
+        ks[s] = valid ? k : -1;
+        xo[s] = ks[s] >= 0 ? xf[(size_t)k * B + r] : (T)0;
+        if (ks[s] >= 0) panel_row<T, B>(tr[s], P, ty, r);
+    }
+    T* xs = xs_all + (wave * (M::RPW + 1) + slot) * B;
+    xs[r] = xv;
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < kKidW; ++s)
+        if (ks[s] >= 0) {
+            T acc = 0;
+#pragma unroll
+            for (int c = 0; c < B; ++c) acc += tr[s][c] * xs[c];
+            xf[(size_t)ks[s] * B + r] = xo[s] + acc;
+        }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse(int n, const double* __restrict__ Cinv, const T* __restrict__ b,
                                                 T* __restrict__ x, int subtract_mean)
@@ -1246,6 +1360,40 @@ void Ops<T>::prolong(hipStream_t s, int BF, int BC, int nf, const int* parent, c
     });
     check_launch("prolong");
 }
+
+template <typename T>
+void Ops<T>::restrict_lat(hipStream_t s, int B, int nc, const int* kpair, const T* PT, const T* r, T* bc)
+{
+    if (nc <= 0) return;
+    MGLDG_DISPATCH(B, {
+        if constexpr (lat_block(BB)) {
+            using M = Map<T, BB>;
+            hipLaunchKernelGGL((k_restrict_lat<T, BB>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc,
+                               reinterpret_cast<const int2*>(kpair), PT, r, bc);
+        } else {
+            throw std::runtime_error("restrict_lat: block size not compiled");
+        }
+    });
+    check_launch("restrict_lat");
+}
+
+template <typename T>
+void Ops<T>::prolong_lat(hipStream_t s, int B, int nc, const int* kpair, const T* P, const T* xc, T* xf)
+{
+    if (nc <= 0) return;
+    MGLDG_DISPATCH(B, {
+        if constexpr (lat_block(BB)) {
+            using M = Map<T, BB>;
+            hipLaunchKernelGGL((k_prolong_lat<T, BB>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc,
+                               reinterpret_cast<const int2*>(kpair), P, xc, xf);
+        } else {
+            throw std::runtime_error("prolong_lat: block size not compiled");
+        }
+    });
+    check_launch("prolong_lat");
+}
+
+bool interp_lat(int B, int max_kids) { return lat_block(B) && max_kids <= kKidW; }
 
 template <typename T>
 void Ops<T>::coarse(hipStream_t s, int n, const double* Cinv, const T* b, T* x, int subtract_mean)

@@ -71,6 +71,10 @@ void block_verify(hipStream_t s, long nblk, int bs, const double* in, const int*
 
 // block rows one 256-thread workgroup of the single-RHS row kernels handles per pass
 int rows_per_workgroup(int B);
+// restriction / prolongation with square B x B blocks and at most max_kids
+// children per coarse row have the table-driven forms (Ops::restrict_lat)
+bool interp_lat(int B, int max_kids);
+constexpr int kKidWidth = 4;   // kpair entries per coarse row
 
 // ---------------------------------------------------------------- solve (T = float | double)
 template <typename T>
@@ -112,6 +116,12 @@ struct Ops {
     // x_f += T x_c (multigrid.h:421-425)
     static void prolong(hipStream_t s, int BF, int BC, int nf, const int* parent, const int* type, const T* pool,
                         const T* xc, T* xf);
+    // the same two on quadtree levels (interp_lat), both over the nc coarse
+    // rows: kpair = (child, block) pairs, kKidWidth per coarse row, -1
+    // padded; P / PT = the pool's blocks / their transposes in the padded
+    // panel layout of T (panel_pool)
+    static void restrict_lat(hipStream_t s, int B, int nc, const int* kpair, const T* PT, const T* r, T* bc);
+    static void prolong_lat(hipStream_t s, int B, int nc, const int* kpair, const T* P, const T* xc, T* xf);
     // coarsest level: b -= mean(b); x = Cinv b, Cinv FP64 dense n x n (multigrid.h:365-368)
     static void coarse(hipStream_t s, int n, const double* Cinv, const T* b, T* x, int subtract_mean);
     static void submean(hipStream_t s, long n, T* b);   // b -= mean(b), one workgroup

@@ -869,6 +869,38 @@ static void upload_interp(hipStream_t s, Level& lv, int precision)
         lv.pool32.upload(p32.data(), p32.size(), s);
         HIPCHK(hipStreamSynchronize(s));
     }
+    // packed tables for the quadtree form: square blocks, few children per
+    // coarse row and a small pool (a transposed copy of it is kept)
+    int maxk = 0;
+    for (int c = 0; c < I.nc; ++c) maxk = std::max(maxk, I.kid_off[c + 1] - I.kid_off[c]);
+    lv.ilat = I.bf == I.bc && I.ntypes <= 64 && dev::interp_lat(I.bf, maxk);
+    if (!lv.ilat) return;
+    const int B = I.bf, KW = dev::kKidWidth;
+    std::vector<int> kp((size_t)I.nc * KW * 2, -1);
+    for (int c = 0; c < I.nc; ++c)
+        for (int kk = I.kid_off[c], sl = 0; kk < I.kid_off[c + 1]; ++kk, ++sl) {
+            const int k = I.kids[kk];
+            kp[((size_t)c * KW + sl) * 2] = k;
+            kp[((size_t)c * KW + sl) * 2 + 1] = I.type[k];
+        }
+    std::vector<double> pt(I.pool.size());
+    for (int t = 0; t < I.ntypes; ++t)
+        for (int c = 0; c < B; ++c)
+            for (int q = 0; q < B; ++q) pt[((size_t)t * B + c) * B + q] = I.pool[((size_t)t * B + q) * B + c];
+    lv.kpair.upload(kp.data(), kp.size(), s);
+    DBuf<double> tr;
+    tr.upload(pt.data(), pt.size(), s);
+    lv.ipool64.alloc((size_t)I.ntypes * B * dev::pool_pad(8, B));
+    lv.ipoolT64.alloc((size_t)I.ntypes * B * dev::pool_pad(8, B));
+    dev::to_rowpad_f64(s, I.ntypes, B, lv.pool64, lv.ipool64);
+    dev::to_rowpad_f64(s, I.ntypes, B, tr, lv.ipoolT64);
+    if (precision == 1) {
+        lv.ipool32.alloc((size_t)I.ntypes * B * dev::pool_pad(4, B));
+        lv.ipoolT32.alloc((size_t)I.ntypes * B * dev::pool_pad(4, B));
+        dev::to_rowpad_f32(s, I.ntypes, B, lv.pool64, lv.ipool32);
+        dev::to_rowpad_f32(s, I.ntypes, B, tr, lv.ipoolT32);
+    }
+    HIPCHK(hipStreamSynchronize(s));
 }
 
 // directions of an agglomerate's coarse space beyond its rank have zero
@@ -1050,6 +1082,12 @@ static size_t Sgw(const Level& lv) { return (size_t)lv.gw * (lv.npool ? 2 : 1); 
 template <typename T> static const T* Pool(const Level& lv);
 template <> const double* Pool<double>(const Level& lv) { return lv.pool64.p; }
 template <> const float* Pool<float>(const Level& lv) { return lv.pool32.p; }
+template <typename T> static const T* IPool(const Level& lv);
+template <> const double* IPool<double>(const Level& lv) { return lv.ipool64.p; }
+template <> const float* IPool<float>(const Level& lv) { return lv.ipool32.p; }
+template <typename T> static const T* IPoolT(const Level& lv);
+template <> const double* IPoolT<double>(const Level& lv) { return lv.ipoolT64.p; }
+template <> const float* IPoolT<float>(const Level& lv) { return lv.ipoolT32.p; }
 
 // The GS launches of one sweep in order.  Multicolour: colours ascending
 // (forward) or descending (reverse).  Lexicographic: forward wavefronts
@@ -1208,7 +1246,8 @@ void MGH::vcycle_from(int l0, bool x0_zero)
         relax<T>(l, prm.npre, false, zero);
         apply_level<T>(lv, v.x, v.b, v.r, nullptr, nullptr, nullptr);
         LevelVecs<T>& vc = vecs<T>(l + 1);
-        dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
+        if (lv.ilat) dev::Ops<T>::restrict_lat(s, lv.B, lv.I.nc, lv.kpair, IPoolT<T>(lv), v.r, vc.b);
+        else dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): relax() from a zero guess starts
         // there (its first sweep reads no x; with no pre-sweep it clears x)
         zero = true;
@@ -1231,7 +1270,8 @@ void MGH::vcycle_from(int l0, bool x0_zero)
     for (int l = bottom - 1; l >= l0; --l) {
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
-        dev::Ops<T>::prolong(s, lv.B, L[l + 1].B, lv.nb, lv.parent, lv.type, Pool<T>(lv), vecs<T>(l + 1).x, v.x);
+        if (lv.ilat) dev::Ops<T>::prolong_lat(s, lv.B, lv.I.nc, lv.kpair, IPool<T>(lv), vecs<T>(l + 1).x, v.x);
+        else dev::Ops<T>::prolong(s, lv.B, L[l + 1].B, lv.nb, lv.parent, lv.type, Pool<T>(lv), vecs<T>(l + 1).x, v.x);
         relax<T>(l, prm.npost, true, false);
     }
 }

@@ -167,6 +167,14 @@ struct Level {
     DBuf<int> parent, type, kid_off, kids;
     DBuf<double> pool64;
     DBuf<float> pool32;
+    // quadtree interpolation (dev::interp_lat): (child, block) pairs per
+    // coarse row and the pool and its transposes in the padded panel layout
+    // (dev::to_rowpad_*), read by Ops::restrict_lat / prolong_lat instead of
+    // the gathers above
+    bool ilat = false;
+    DBuf<int> kpair;
+    DBuf<double> ipool64, ipoolT64;
+    DBuf<float> ipool32, ipoolT32;
     DBuf<double> Cinv;                  // coarsest: dense inverse (n x n)
 };
 

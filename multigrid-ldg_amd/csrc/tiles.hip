@@ -257,6 +257,11 @@ __global__ __launch_bounds__(256) void k_apply_tile(int ntiles, const int* __res
     const int i0 = i >= 0 ? i : 0;
     T bv[4];
     acc_load<T, B>(bv, b ? b : x, i0, q);             // (x: an in-range source when there is no b)
+    // the dot's other vector: y itself (its values are `out`), or an input
+    // loaded with the row's other operands
+    const bool dot_self = dotv == y;
+    T dw[4] = {0, 0, 0, 0};
+    if (dotv && !dot_self) acc_load<T, B>(dw, dotv, i0, q);
     typename Tm<T>::v4 acc = {0, 0, 0, 0};
     tile_blocks<T, B, 0, WM, true>(acc, ts, n, q, i, uni(tmeta[tl].y), At, x, nullptr, 0);
     double dv = 0;
@@ -265,11 +270,9 @@ __global__ __launch_bounds__(256) void k_apply_tile(int ntiles, const int* __res
 #pragma unroll
         for (int e = 0; e < 4; ++e) out[e] = b ? (T)(bv[e] - acc[e]) : acc[e];
         acc_store<T, B>(y, out, i, q);
-        if (dotv) {                                   // after the store: dotv may be y itself
-            T dw[4];
-            acc_load<T, B>(dw, dotv, i, q);
+        if (dotv) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) dv += (double)out[e] * (double)dw[e];
+            for (int e = 0; e < 4; ++e) dv += (double)out[e] * (double)(dot_self ? out[e] : dw[e]);
         }
     }
     if (part) {
