@@ -1025,19 +1025,35 @@ __device__ __forceinline__ void panel_row(T* o, const T* __restrict__ P, int t, 
     }
 }
 
-template <typename T, int B>
+// FUSE: the coarse level's first Gauss-Seidel launch of a sweep from x = 0
+// as well.  That launch (colour 0, reading no neighbour) sets x_I = omega
+// D^-1 b_I on the rows of colour 0 and nothing else, so it is done here
+// from b_I as it is formed: colour[I], the row's D^-1 pool index didx[I]
+// and its padded D^-1 row come with the other operands, and the update is
+// the one of the latency GS form (LatRow::finish with no neighbour).
+template <typename T, int B, bool FUSE>
 __global__ __launch_bounds__(256) void k_restrict_lat(int nc, const int2* __restrict__ kpair, const T* __restrict__ PT,
-                                                      const T* __restrict__ r, T* __restrict__ bc)
+                                                      const T* __restrict__ r, T* __restrict__ bc,
+                                                      const int* __restrict__ color, const int* __restrict__ didx,
+                                                      const T* __restrict__ Dinv, T* __restrict__ x, T omega)
 {
     using M = Map<T, B>;
     using C = InterpLat<T, B>;
+    using R = RowLds<T, B>;
     __shared__ __align__(16) T rs_all[4 * (M::RPW + 1) * kKidW * B];
+    __shared__ __align__(16) T tb[FUSE ? 4 * R::RW * R::TB : 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / B, c = lane - slot * B;
     const int I = xcd_block() * M::RPB + wave * M::RPW + slot;
     const bool valid = slot < M::RPW && I < nc;
     int2 e = make_int2(-1, 0);
     if (valid && c < kKidW) e = kpair[(size_t)I * kKidW + c];
+    bool gs = false;
+    PadRow<T, B> dp;
+    if constexpr (FUSE) {
+        gs = valid && color[I] == 0;
+        if (gs) load_dinv_row_pad<T, B>(Dinv, didx[I], c, dp);
+    }
     T rv[kKidW], tc[kKidW][C::BP];
     bool live[kKidW];
 #pragma unroll
@@ -1060,6 +1076,16 @@ __global__ __launch_bounds__(256) void k_restrict_lat(int nc, const int2* __rest
             for (int q = 0; q < B; ++q) acc += tc[s][q] * rs[s * B + q];
         }
     if (valid) bc[(size_t)I * B + c] = acc;
+    if constexpr (FUSE) {
+        T* t = tb + (wave * R::RW + slot) * R::TB;
+        t[c] = acc;
+        if (c < R::TB - B) t[B + c] = (T)0;
+        wave_sync();
+        if (gs) {
+            const T xi = 0, sv = dinv_row_pad<T, B>(dp, t);
+            x[(size_t)I * B + c] = xi + omega * (sv - xi);
+        }
+    }
 }
 
 template <typename T, int B>
@@ -1362,14 +1388,20 @@ void Ops<T>::prolong(hipStream_t s, int BF, int BC, int nf, const int* parent, c
 }
 
 template <typename T>
-void Ops<T>::restrict_lat(hipStream_t s, int B, int nc, const int* kpair, const T* PT, const T* r, T* bc)
+void Ops<T>::restrict_lat(hipStream_t s, int B, int nc, const int* kpair, const T* PT, const T* r, T* bc,
+                          const int* color, const int* didx, const T* Dinv, T* x, T omega)
 {
     if (nc <= 0) return;
     MGLDG_DISPATCH(B, {
         if constexpr (lat_block(BB)) {
             using M = Map<T, BB>;
-            hipLaunchKernelGGL((k_restrict_lat<T, BB>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc,
-                               reinterpret_cast<const int2*>(kpair), PT, r, bc);
+            const int2* kp = reinterpret_cast<const int2*>(kpair);
+            if (color)
+                hipLaunchKernelGGL((k_restrict_lat<T, BB, true>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc, kp, PT, r,
+                                   bc, color, didx, Dinv, x, omega);
+            else
+                hipLaunchKernelGGL((k_restrict_lat<T, BB, false>), dim3(nblk(nc, M::RPB)), dim3(256), 0, s, nc, kp, PT, r,
+                                   bc, nullptr, nullptr, nullptr, nullptr, (T)0);
         } else {
             throw std::runtime_error("restrict_lat: block size not compiled");
         }

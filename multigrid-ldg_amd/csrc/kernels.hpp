@@ -120,7 +120,12 @@ struct Ops {
     // rows: kpair = (child, block) pairs, kKidWidth per coarse row, -1
     // padded; P / PT = the pool's blocks / their transposes in the padded
     // panel layout of T (panel_pool)
-    static void restrict_lat(hipStream_t s, int B, int nc, const int* kpair, const T* PT, const T* r, T* bc);
+    // With color != nullptr it also does the coarse level's first GS launch
+    // of a sweep from zero: x_I = omega D^-1 b_I on the rows of colour 0
+    // (didx: the rows' D^-1 pool indices, Dinv: the padded D^-1 pool)
+    static void restrict_lat(hipStream_t s, int B, int nc, const int* kpair, const T* PT, const T* r, T* bc,
+                             const int* color = nullptr, const int* didx = nullptr, const T* Dinv = nullptr,
+                             T* x = nullptr, T omega = 0);
     static void prolong_lat(hipStream_t s, int B, int nc, const int* kpair, const T* P, const T* xc, T* xf);
     // coarsest level: b -= mean(b); x = Cinv b, Cinv FP64 dense n x n (multigrid.h:365-368)
     static void coarse(hipStream_t s, int n, const double* Cinv, const T* b, T* x, int subtract_mean);

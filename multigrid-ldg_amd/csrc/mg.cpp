@@ -1166,8 +1166,21 @@ void MGH::select_tiles()
 
 // Level::relax (multigrid.h:202-262); Gauss-Seidel is multicolour on the GPU:
 // colours in increasing order forward, decreasing order in reverse.
+// The first launch of a forward multicolour GS sweep from x = 0 (colour 0,
+// reading no neighbour) can be done by the restriction that forms the
+// level's right-hand side (Ops::restrict_lat with the colour arguments):
+// quadtree interpolation into a pooled level whose colour 0 has a launch of
+// its own.
+bool MGH::fused_first(int l) const
+{
+    if (l < 1 || prm.relaxation != 1 || prm.npre < 1) return false;
+    const Level& f = L[l - 1];
+    const Level& c = L[l];
+    return f.ilat && c.npool > 0 && c.pdidx.p && (c.pair_nwg == 0 || c.ncolor >= 3);
+}
+
 template <typename T>
-void MGH::relax(int l, int n, bool reverse, bool zero_guess)
+void MGH::relax(int l, int n, bool reverse, bool zero_guess, bool first_done)
 {
     Level& lv = L[l];
     LevelVecs<T>& v = vecs<T>(l);
@@ -1196,7 +1209,7 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
                                      Sop<T>(lv), SDop<T>(lv), v.b, v.x, omega, lv.color_dev, cur0,
                                      reverse ? nc - 2 : nc - 1, skip, Spool(lv));
             };
-            if (!reverse) { for (int c = 0; c < nc - 2; ++c) one(c); pair(); }
+            if (!reverse) { for (int c = (first_done && sw == 0) ? 1 : 0; c < nc - 2; ++c) one(c); pair(); }
             else { pair(); for (int c = nc - 3; c >= 0; --c) one(c); }
         }
     } else if (prm.relaxation == 1) {
@@ -1204,7 +1217,7 @@ void MGH::relax(int l, int n, bool reverse, bool zero_guess)
         const int nc = lv.ncolor;
         for (int sw = 0; sw < n; ++sw) {
             const int skip = (zero_guess && sw == 0) ? (reverse ? 2 : 1) : 0;
-            for (int cc = 0; cc < nc; ++cc) {
+            for (int cc = (first_done && sw == 0) ? 1 : 0; cc < nc; ++cc) {
                 const int c = reverse ? nc - 1 - cc : cc;
                 const int cur = (skip && cc == 0) ? (reverse ? dev::kFirstRev : dev::kFirstFwd) : c;
                 gs_colour<T>(lv, c, cur, skip, v.b, v.x);
@@ -1240,14 +1253,21 @@ void MGH::vcycle_from(int l0, bool x0_zero)
     const int nl = (int)L.size();
     const int bottom = cut > l0 ? cut : nl - 1;
     bool zero = x0_zero;   // x of the starting level is zero (preconditioner use) or holds the iterate
+    bool first_done = false;
     for (int l = l0; l < bottom; ++l) {
         Level& lv = L[l];
         LevelVecs<T>& v = vecs<T>(l);
-        relax<T>(l, prm.npre, false, zero);
+        relax<T>(l, prm.npre, false, zero, first_done);
         apply_level<T>(lv, v.x, v.b, v.r, nullptr, nullptr, nullptr);
+        Level& lc = L[l + 1];
         LevelVecs<T>& vc = vecs<T>(l + 1);
-        if (lv.ilat) dev::Ops<T>::restrict_lat(s, lv.B, lv.I.nc, lv.kpair, IPoolT<T>(lv), v.r, vc.b);
-        else dev::Ops<T>::restrict_(s, lv.B, L[l + 1].B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
+        // the restriction also does level l+1's first GS launch where it can
+        first_done = l + 1 < bottom && fused_first(l + 1);
+        if (first_done)
+            dev::Ops<T>::restrict_lat(s, lv.B, lv.I.nc, lv.kpair, IPoolT<T>(lv), v.r, vc.b, lc.color_dev, lc.pdidx,
+                                      SDop<T>(lc), vc.x, (T)prm.omega);
+        else if (lv.ilat) dev::Ops<T>::restrict_lat(s, lv.B, lv.I.nc, lv.kpair, IPoolT<T>(lv), v.r, vc.b);
+        else dev::Ops<T>::restrict_(s, lv.B, lc.B, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), v.r, vc.b);
         // x_{l+1} = 0 (multigrid.h:376): relax() from a zero guess starts
         // there (its first sweep reads no x; with no pre-sweep it clears x)
         zero = true;
@@ -2036,7 +2056,7 @@ void MGH::vcycle_cost(int precision, int* launches, double* bytes)
     const int bottom = cut > 0 ? cut : nl - 1;
     long nlaunch = 0;
     double by = 0;
-    auto relax_cost = [&](int l, int n, bool reverse, bool zero) {
+    auto relax_cost = [&](int l, int n, bool reverse, bool zero, bool first_done) {
         const Level& lv = L[l];
         const double nv = (double)lv.nb * lv.B;
         if (zero && n == 0) { ++nlaunch; by += nv * sa; return; }
@@ -2080,6 +2100,9 @@ void MGH::vcycle_cost(int precision, int* launches, double* bytes)
         for (int sw = 0; sw < n; ++sw) {
             const int skip = (zero && sw == 0) ? (reverse ? 2 : 1) : 0;
             auto single = [&](int c, int cur) {
+                // (colour 0 of the first sweep done by the restriction: its
+                // launch is gone, its b, D^-1 and x traffic counted there)
+                if (first_done && sw == 0 && c == 0 && !reverse) return;
                 ++nlaunch;
                 const int nt = tiles ? lv.tile_off[c + 1] - lv.tile_off[c] : 0;
                 by += gs_launch_bytes(lv, by_colour[c].data(), (int)by_colour[c].size(), sa, mark, ++stamp, skip, cur, nt);
@@ -2109,19 +2132,24 @@ void MGH::vcycle_cost(int precision, int* launches, double* bytes)
     for (int l = 0; l < bottom; ++l) {
         const Level& lv = L[l];
         const Level& lc = L[l + 1];
-        relax_cost(l, prm.npre, false, true);
+        relax_cost(l, prm.npre, false, true, l > 0 && fused_first(l));
         ++nlaunch; by += apply_bytes(lv, sa, lv.use_tiles[precision == 1] & 2, true);
         // restriction: r of the fine level, b of the coarse, the child lists
-        // and the interpolation pool
+        // and the interpolation pool; fused with the coarse level's first GS
+        // launch, also the colours, D^-1 indices and pool and x of colour 0
         ++nlaunch;
         by += ((double)lv.nb * lv.B + (double)lc.nb * lc.B) * sa + 4.0 * (lc.nb + 1) + 8.0 * lv.nb +
               (double)lv.I.pool.size() * sa;
+        if (l + 1 < bottom && fused_first(l + 1)) {
+            const double n0 = (double)(lc.color_off[1] - lc.color_off[0]);
+            by += 8.0 * lc.nb + (double)lc.ndpool * lc.B * lc.B * sa + n0 * lc.B * sa;
+        }
     }
     const Level& c = L[bottom];
     const double k = (double)c.nb * c.B;
     if (bottom == cut) { ++nlaunch; by += k * k * ((precision == 1 && W32.p) ? 4.0 : 8.0) + 2.0 * k * sa; }
     else if (prm.coarse == 0) { ++nlaunch; by += k * k * 8.0 + 2.0 * k * sa; }
-    else { relax_cost(bottom, prm.npre, false, true); relax_cost(bottom, prm.npost, true, false); }
+    else { relax_cost(bottom, prm.npre, false, true, false); relax_cost(bottom, prm.npost, true, false, false); }
     for (int l = bottom - 1; l >= 0; --l) {
         const Level& lv = L[l];
         const Level& lc = L[l + 1];
@@ -2129,7 +2157,7 @@ void MGH::vcycle_cost(int precision, int* launches, double* bytes)
         // parents and types, the interpolation pool
         ++nlaunch;
         by += ((double)lc.nb * lc.B + 2.0 * lv.nb * lv.B) * sa + 8.0 * lv.nb + (double)lv.I.pool.size() * sa;
-        relax_cost(l, prm.npost, true, false);
+        relax_cost(l, prm.npost, true, false, false);
     }
     *launches = (int)nlaunch;
     *bytes = by;

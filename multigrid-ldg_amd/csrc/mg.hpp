@@ -246,7 +246,10 @@ struct MGH {
     void mrelax(int l, int k, int n, bool reverse, bool zero_guess, double* x, const double* b, double* alt);
     ~MGH();
 
-    template <typename T> void relax(int l, int n, bool reverse, bool zero_guess);
+    // first_done: the sweep's first launch (colour 0 from zero) was done by
+    // the restriction into this level (fused_first)
+    template <typename T> void relax(int l, int n, bool reverse, bool zero_guess, bool first_done = false);
+    bool fused_first(int l) const;
     // one colour c of a single-RHS GS sweep on level l (row tiles where chosen)
     template <typename T> void gs_colour(Level& lv, int c, int cur, int skip, const T* b, T* x);
     // y = A_l x or b - A_l x (+ y . dotv into *dot_out) on a level (row tiles where chosen)

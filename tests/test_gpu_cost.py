@@ -4,7 +4,8 @@ launch and byte counts the benchmark's whole-iteration roofline uses.
 Its V-cycle walk mirrors the enqueue path, so its launch count must equal
 the kernel nodes of the captured V-cycle graph, for every smoother, with and
 without the collapsed coarse tail, pooled or not, with the row tiles on or
-off.  Bytes: positive, FP32 V-cycle below FP64, pooled below unpooled, the
+off (pooled GS levels below the finest: the restriction into the level also
+does its first GS launch).  Bytes: positive, FP32 V-cycle below FP64, pooled below unpooled, the
 iteration above its V-cycle."""
 import numpy as np
 import pytest
@@ -45,5 +46,9 @@ def test_pooled_bytes_below_unpooled(gctx):
     pr = problem(gctx, 3)
     a = mgldg.Multigrid(pr, "gs", npre=2, npost=2, pool=True, coarse_dense_max=0).solve_cost()
     b = mgldg.Multigrid(pr, "gs", npre=2, npost=2, pool=False, coarse_dense_max=0).solve_cost()
-    assert a["vcycle_launches"] == b["vcycle_launches"]
+    # on pooled levels the restriction also does the coarse level's first GS
+    # launch: at most one launch fewer per level between the finest and the
+    # coarsest
+    nl = mgldg.Multigrid(pr, "gs", npre=2, npost=2, coarse_dense_max=0).nlevels
+    assert b["vcycle_launches"] - (nl - 2) <= a["vcycle_launches"] < b["vcycle_launches"]
     assert a["vcycle_bytes"] < 0.5 * b["vcycle_bytes"]
