@@ -1219,23 +1219,25 @@ static inline unsigned nblk(long n, int per) { return (unsigned)((n + per - 1) /
     }
 
 template <typename T>
-void Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A, const T* x,
-                   const T* b, T* y, const T* dotv, double* dot_out, double* part, int pool)
+int Ops<T>::apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A, const T* x,
+                  const T* b, T* y, const T* dotv, double* dot_out, double* part, int pool)
 {
-    if (nrows <= 0) return;
+    if (nrows <= 0) return 0;
     unsigned g = 0;
+    double* pw = (dotv && part) ? part : nullptr;
     MGLDG_DISPATCH(B, {
         using M = Map<T, BB>;
         g = nblk(nrows, M::RPB);
         if (pool)
             hipLaunchKernelGGL((k_apply<T, BB, true>), dim3(g), dim3(256), 0, s, nrows, rowptr, ell, W, A, x, b, y,
-                               dotv, dot_out ? part : nullptr);
+                               dotv, pw);
         else
             hipLaunchKernelGGL((k_apply<T, BB>), dim3(g), dim3(256), 0, s, nrows, rowptr, ell, W, A, x, b, y,
-                               dotv, dot_out ? part : nullptr);
+                               dotv, pw);
     });
     check_launch("apply");
     if (dot_out) finalize(s, part, (int)g, 1, nullptr, 0, dot_out);
+    return (int)g;
 }
 
 template <typename T>
@@ -2081,53 +2083,91 @@ void dot(hipStream_t s, long n, const double* a, const double* b, double* out, d
     finalize(s, part, (int)g, 1, nullptr, 0, out);
 }
 
+// The second pass of the CG's reductions is done by the kernel that needs
+// the result: each of its workgroups sums the previous launch's partials
+// (part[i * NV + k], i < n) in the same fixed order -- the same value in
+// every workgroup -- and workgroup 0 stores it in S for the host.  No launch
+// is spent on it, and the partials are read from cache.  Every thread gets
+// out[0 .. NV).
+template <int NV>
+__device__ __forceinline__ void wg_sum_partials(const double* __restrict__ part, int n, double* out)
+{
+    __shared__ double res[NV];
+    double t[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) t[k] = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) t[k] += part[(size_t)i * NV + k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double v = block_sum(t[k]);
+        if (threadIdx.x == 0) res[k] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = res[k];
+}
+
 // x += alpha d ; r -= alpha q ; v = (T) r, the V-cycle's level-0 right-hand
-// side (one pass instead of an update and a conversion)
+// side (one pass instead of an update and a conversion).  alpha = S[DELTA] /
+// d.q, d.q summed here from the product's partials (dqp, ndq) and stored in
+// S[DQ]
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_xr(long n, double* S, const double* __restrict__ d, const double* __restrict__ q,
-                                                double* __restrict__ x, double* __restrict__ r, T* __restrict__ v, double* part)
+                                                double* __restrict__ x, double* __restrict__ r, T* __restrict__ v,
+                                                const double* __restrict__ dqp, int ndq)
 {
-    const double alpha = S[S_DELTA] / S[S_DQ];
-    double s = 0;
+    double dq;
+    wg_sum_partials<1>(dqp, ndq, &dq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_DQ] = dq;
+    const double alpha = S[S_DELTA] / dq;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         x[i] += alpha * d[i];
         double ri = r[i] - alpha * q[i];
-        r[i] = ri; v[i] = (T)ri; s += ri * ri;
+        r[i] = ri; v[i] = (T)ri;
     }
-    block_partial(s, part);
 }
+
+static unsigned red_grid(long n) { return (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256)); }
 
 template <typename T>
-void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, T* v, double* part)
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, T* v,
+            const double* dqp, int ndq)
 {
-    unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_pcg_xr<T>, dim3(g), dim3(256), 0, s, n, S, d, q, x, r, v, part);
+    hipLaunchKernelGGL(k_pcg_xr<T>, dim3(red_grid(n)), dim3(256), 0, s, n, S, d, q, x, r, v, dqp, ndq);
     check_launch("pcg_xr");
-    finalize(s, part, (int)g, 1, S, 0, S + S_RR);
 }
-template void pcg_xr<float>(hipStream_t, long, double*, const double*, const double*, double*, double*, float*, double*);
-template void pcg_xr<double>(hipStream_t, long, double*, const double*, const double*, double*, double*, double*, double*);
+template void pcg_xr<float>(hipStream_t, long, double*, const double*, const double*, double*, double*, float*,
+                            const double*, int);
+template void pcg_xr<double>(hipStream_t, long, double*, const double*, const double*, double*, double*, double*,
+                             const double*, int);
 
-__global__ void k_pcg_x(long n, const double* S, const double* __restrict__ d, double* __restrict__ x)
+__global__ __launch_bounds__(256) void k_pcg_x(long n, double* S, const double* __restrict__ d, double* __restrict__ x,
+                                               const double* __restrict__ dqp, int ndq)
 {
-    const double alpha = S[S_DELTA] / S[S_DQ];
-    long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) x[i] += alpha * d[i];
+    double dq;
+    wg_sum_partials<1>(dqp, ndq, &dq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_DQ] = dq;
+    const double alpha = S[S_DELTA] / dq;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) x[i] += alpha * d[i];
 }
 
-void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x)
+void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x, const double* dqp, int ndq)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_pcg_x, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, d, x);
+    hipLaunchKernelGGL(k_pcg_x, dim3(red_grid(n)), dim3(256), 0, s, n, S, d, x, dqp, ndq);
     check_launch("pcg_x");
 }
 
 // three reductions in one pass: r.z, r.zold, r.r (z = the V-cycle's level-0
-// result zs, precision T, converted exactly); zold <- zs in the same pass
+// result zs, precision T, converted exactly); zold <- zs in the same pass.
+// Partials part[3 i + {0, 1, 2}]; S[DELTA_OLD] = S[DELTA] (pcg_d reads it)
 template <typename T>
-__global__ __launch_bounds__(256) void k_pcg_rz(long n, const double* __restrict__ r, const T* __restrict__ zs, T* zold,
-                                                int read_old, double* part)
+__global__ __launch_bounds__(256) void k_pcg_rz(long n, double* S, const double* __restrict__ r, const T* __restrict__ zs,
+                                                T* zold, int read_old, double* part)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_DELTA_OLD] = S[S_DELTA];
     double a = 0, b = 0, c = 0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         const double ri = r[i];
@@ -2143,34 +2183,44 @@ __global__ __launch_bounds__(256) void k_pcg_rz(long n, const double* __restrict
 }
 
 template <typename T>
-void pcg_rz(hipStream_t s, long n, double* S, const double* r, const T* zs, T* zold, bool read_old, double* part)
+int pcg_rz(hipStream_t s, long n, double* S, const double* r, const T* zs, T* zold, bool read_old, double* part)
 {
-    unsigned g = (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(g), dim3(256), 0, s, n, r, zs, zold, (int)(read_old && zold), part);
+    const unsigned g = red_grid(n);
+    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(g), dim3(256), 0, s, n, S, r, zs, zold, (int)(read_old && zold), part);
     check_launch("pcg_rz");
-    finalize(s, part, (int)g, 3, S, (read_old && zold) ? 2 : 1, nullptr);
+    return (int)g;
 }
-template void pcg_rz<float>(hipStream_t, long, double*, const double*, const float*, float*, bool, double*);
-template void pcg_rz<double>(hipStream_t, long, double*, const double*, const double*, double*, bool, double*);
+template int pcg_rz<float>(hipStream_t, long, double*, const double*, const float*, float*, bool, double*);
+template int pcg_rz<double>(hipStream_t, long, double*, const double*, const double*, double*, bool, double*);
 
-// d = z + beta d, or d = z (init), z = (double) zs
+// d = z + beta d, or d = z (init), z = (double) zs.  The r.z, r.zold, r.r
+// partials of pcg_rz are summed here: S[DELTA] = r.z, S[RSOLD] = r.zold,
+// S[RR] = r.r, S[BETA] = beta = r.z / S[DELTA_OLD] (flexible: (r.z - r.zold)
+// / S[DELTA_OLD]; krylov.h:54-57), 0 for init
 template <typename T>
-__global__ void k_pcg_d(long n, const double* S, const T* __restrict__ zs, double* __restrict__ d, int init)
+__global__ __launch_bounds__(256) void k_pcg_d(long n, double* S, const T* __restrict__ zs, double* __restrict__ d, int init,
+                                               const double* __restrict__ rzp, int nrz, int flexible)
 {
-    long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    d[i] = init ? (double)zs[i] : (double)zs[i] + S[S_BETA] * d[i];
+    double t[3];
+    wg_sum_partials<3>(rzp, nrz, t);
+    const double old = S[S_DELTA_OLD];
+    const double beta = init ? 0.0 : (flexible ? (t[0] - t[1]) / old : t[0] / old);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S[S_DELTA] = t[0]; S[S_RSOLD] = t[1]; S[S_RR] = t[2]; S[S_BETA] = beta;
+    }
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        d[i] = init ? (double)zs[i] : (double)zs[i] + beta * d[i];
 }
 
 template <typename T>
-void pcg_d(hipStream_t s, long n, const double* S, const T* zs, double* d, bool init)
+void pcg_d(hipStream_t s, long n, double* S, const T* zs, double* d, bool init, const double* rzp, int nrz, bool flexible)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_pcg_d<T>, dim3(nblk(n, 256)), dim3(256), 0, s, n, S, zs, d, (int)init);
+    hipLaunchKernelGGL(k_pcg_d<T>, dim3(red_grid(n)), dim3(256), 0, s, n, S, zs, d, (int)init, rzp, nrz, (int)flexible);
     check_launch("pcg_d");
 }
-template void pcg_d<float>(hipStream_t, long, const double*, const float*, double*, bool);
-template void pcg_d<double>(hipStream_t, long, const double*, const double*, double*, bool);
+template void pcg_d<float>(hipStream_t, long, double*, const float*, double*, bool, const double*, int, bool);
+template void pcg_d<double>(hipStream_t, long, double*, const double*, double*, bool, const double*, int, bool);
 
 // ---------------------------------------------------------------------- batched PCG
 // thread (group g, column c), c = tid % k, g = tid / k; k <= 256

@@ -84,8 +84,11 @@ struct Ops {
     // (ell: the rows' column indices at stride W, -1 past a row's blocks)
     // pool != 0: A is a pool of distinct blocks and ell holds (column, pool
     // index) pairs at stride 2W (the other launchers below alike)
-    static void apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
-                      const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part, int pool = 0);
+    // (with dotv and part, the y.dotv partials go to part, one per workgroup;
+    // with dot_out they are also summed into *dot_out.  Returns the number of
+    // workgroups, i.e. of partials.)
+    static int apply(hipStream_t s, int B, int nrows, const int* rowptr, const int* ell, int W, const T* A,
+                     const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part, int pool = 0);
     // damped block Gauss-Seidel on the rows of colour `cur` (multigrid.h:240-249).
     // desc: 4 ints per row of the colour, {row, rowptr[row], rowptr[row+1],
     // position of the diagonal block}; gcols: per row W columns of its
@@ -205,18 +208,23 @@ enum { S_DELTA = 0, S_DELTA_OLD, S_DQ, S_ALPHA, S_BETA, S_RR, S_RSOLD, S_DELTA0,
 void dot(hipStream_t s, long n, const double* a, const double* b, double* out, double* part);
 // The V-cycle's level-0 vectors have the preconditioner's precision T: v
 // (its right-hand side) and zs (its result, z = (double) zs exactly).
-// x += alpha d ; r -= alpha q ; v = (T) r ; S[S_RR] = r.r      (alpha = S[DELTA]/S[DQ])
+// The second pass of these reductions is done by the consumer: pcg_xr /
+// pcg_x sum the product's d.q partials (dqp, ndq: Ops::apply / tile_apply
+// with dotv = d and no dot_out), pcg_d sums pcg_rz's.
+// S[DQ] = d.q ; x += alpha d ; r -= alpha q ; v = (T) r      (alpha = S[DELTA]/S[DQ])
 template <typename T>
-void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, T* v, double* part);
-// x += alpha d only
-void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x);
-// S[DELTA_OLD]=S[DELTA]; S[DELTA] = r.z ; if read_old S[RSOLD] = r.zold ; S[RR] = r.r ; beta;
-// then zold = zs (zold may be null: not kept)
+void pcg_xr(hipStream_t s, long n, double* S, const double* d, const double* q, double* x, double* r, T* v,
+            const double* dqp, int ndq);
+// S[DQ] = d.q ; x += alpha d only
+void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x, const double* dqp, int ndq);
+// S[DELTA_OLD] = S[DELTA]; r.z, r.zold (read_old), r.r partials into part
+// (returns their count); then zold = zs (zold may be null: not kept)
 template <typename T>
-void pcg_rz(hipStream_t s, long n, double* S, const double* r, const T* zs, T* zold, bool read_old, double* part);
-// d = z + beta d   (init: d = z)
+int pcg_rz(hipStream_t s, long n, double* S, const double* r, const T* zs, T* zold, bool read_old, double* part);
+// S[DELTA] = r.z, S[RSOLD] = r.zold, S[RR] = r.r, beta (flexible: with r.zold)
+// from pcg_rz's nrz partials rzp; d = z + beta d   (init: d = z)
 template <typename T>
-void pcg_d(hipStream_t s, long n, const double* S, const T* zs, double* d, bool init);
+void pcg_d(hipStream_t s, long n, double* S, const T* zs, double* d, bool init, const double* rzp, int nrz, bool flexible);
 // ---------------------------------------------------------------- batched PCG (k columns, [n][k])
 // Per-column scalars live at S[c * S_COUNT + slot]; active[c] = 0 freezes column c.
 // per-column a.b into out[c * ostride] (deterministic two-pass)
@@ -242,8 +250,8 @@ template <typename T>
 void tile_gs(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta, int W,
              const T* At, const T* Dt, const T* b, T* x, T omega, const int* color, int cur, int skip);
 template <typename T>
-void tile_apply(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta,
-                int W, const T* At, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
+int tile_apply(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta,
+               int W, const T* At, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
 size_t mean_zero_part_size(long n, int k);
 void mean_zero(hipStream_t s, long n, int k, double* a, double* part);
 // Scratch of the allocation-free mean_zero entry point: a file-scope device

@@ -1133,15 +1133,14 @@ void MGH::gs_colour(Level& lv, int c, int cur, int skip, const T* b, T* x)
 }
 
 template <typename T>
-void MGH::apply_level(const Level& lv, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part)
+int MGH::apply_level(const Level& lv, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part)
 {
     hipStream_t s = ctx->stream;
     if (lv.use_tiles[sizeof(T) == 4] & 2)
-        dev::tile_apply<T>(s, lv.B, lv.a_ntiles, lv.a_row, lv.a_col, lv.a_pid, lv.a_meta, lv.t_W, TAop<T>(lv), x, b, y,
-                           dotv, dot_out, part);
-    else
-        dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), x, b, y, dotv, dot_out, part,
-                           Spool(lv));
+        return dev::tile_apply<T>(s, lv.B, lv.a_ntiles, lv.a_row, lv.a_col, lv.a_pid, lv.a_meta, lv.t_W, TAop<T>(lv), x, b,
+                                  y, dotv, dot_out, part);
+    return dev::Ops<T>::apply(s, lv.B, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Sop<T>(lv), x, b, y, dotv, dot_out, part,
+                              Spool(lv));
 }
 
 // Row tiles replace the vector kernels on pooled levels with at least
@@ -1432,23 +1431,28 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
     auto residual = [&](bool with_rr) {   // r = b - A x  (optionally r.r into S[RR])
         apply_level<double>(f, x, b, r, with_rr ? r.p : nullptr, with_rr ? S.p + dev::S_RR : nullptr, part);
     };
-    // q = A d, S[DQ] = d.q: the first kernel of an iteration.  It writes only
-    // q and S[DQ], so it is enqueued before the host has tested the previous
-    // iteration's scalars and is simply not used when the loop ends there.
+    // q = A d and the d.q partials: the first kernel of an iteration.  It
+    // writes only q and the partials, so it is enqueued before the host has
+    // tested the previous iteration's scalars and is simply not used when
+    // the loop ends there.  pcg_xr / pcg_x sum the partials into S[DQ].
+    int ndq = 0, nrz = 0;
     auto product = [&]() {
-        apply_level<double>(f, d, nullptr, q, d, S.p + dev::S_DQ, part);
+        ndq = apply_level<double>(f, d, nullptr, q, d, nullptr, part);
     };
     // the preconditioner reads r in its own precision from L[0].v*.b and
     // leaves z in L[0].v*.x; the CG kernels convert on the fly
     auto to_vin = [&] {
         if (mixed) dev::f64_to_f32(s, n, r, f.v32.b); else dev::Ops<double>::copy(s, n, r, f.v64.b);
     };
-    auto rz = [&](bool read_old) {   // S[DELTA] = r.z, S[RR] = r.r (+ r.zold, beta); zold = z
-        if (mixed) dev::pcg_rz<float>(s, n, S, r, f.v32.x, sp.flexible ? zold32.p : nullptr, read_old, part);
-        else dev::pcg_rz<double>(s, n, S, r, f.v64.x, sp.flexible ? zold.p : nullptr, read_old, part);
+    bool rz_old = false;
+    auto rz = [&](bool read_old) {   // r.z, r.r (+ r.zold) partials; zold = z
+        rz_old = read_old && sp.flexible;
+        if (mixed) nrz = dev::pcg_rz<float>(s, n, S, r, f.v32.x, sp.flexible ? zold32.p : nullptr, read_old, part);
+        else nrz = dev::pcg_rz<double>(s, n, S, r, f.v64.x, sp.flexible ? zold.p : nullptr, read_old, part);
     };
-    auto dupd = [&](bool init) {     // d = z + beta d  (init: d = z)
-        if (mixed) dev::pcg_d<float>(s, n, S, f.v32.x, d, init); else dev::pcg_d<double>(s, n, S, f.v64.x, d, init);
+    auto dupd = [&](bool init) {     // S[DELTA] = r.z, S[RR] = r.r, beta; d = z + beta d  (init: d = z)
+        if (mixed) dev::pcg_d<float>(s, n, S, f.v32.x, d, init, part, nrz, rz_old);
+        else dev::pcg_d<double>(s, n, S, f.v64.x, d, init, part, nrz, rz_old);
     };
     // ||b||
     dev::dot(s, n, b, b, S.p + dev::S_BB, part);
@@ -1487,13 +1491,13 @@ SolveStats MGH::solve(const double* b, double* x, const SolveParams& sp)
         if (i >= sp.maxit) { st.status = 1; break; }
         // q = A d and S[DQ] are in flight (product() above)
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
-            dev::pcg_x(s, n, S, d, x);
+            dev::pcg_x(s, n, S, d, x, part, ndq);
             residual(true);
             to_vin();
         } else if (mixed) {
-            dev::pcg_xr<float>(s, n, S, d, q, x, r, f.v32.b, part);
+            dev::pcg_xr<float>(s, n, S, d, q, x, r, f.v32.b, part, ndq);
         } else {
-            dev::pcg_xr<double>(s, n, S, d, q, x, r, f.v64.b, part);
+            dev::pcg_xr<double>(s, n, S, d, q, x, r, f.v64.b, part, ndq);
         }
         precon(sp.precision, sp.use_graph != 0);
         rz(sp.flexible != 0);
@@ -2179,7 +2183,8 @@ void MGH::iteration_cost(int precision, bool flexible, int* launches, double* by
     by += 8.0 * n + sv * n + (flexible ? 2.0 * sv * n : 0.0);              // rz
     by += sv * n + 16.0 * n;                                               // d
     by += 3 * part + 128.0;
-    *launches = 4 + 3;
+    // product, xr, rz and d; the reductions' second passes run inside xr and d
+    *launches = 4;
     *bytes = by;
 }
 

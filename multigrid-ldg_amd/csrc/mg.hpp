@@ -254,7 +254,7 @@ struct MGH {
     template <typename T> void gs_colour(Level& lv, int c, int cur, int skip, const T* b, T* x);
     // y = A_l x or b - A_l x (+ y . dotv into *dot_out) on a level (row tiles where chosen)
     template <typename T>
-    void apply_level(const Level& lv, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
+    int apply_level(const Level& lv, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part);
     void select_tiles();               // per pooled level: row tiles or vector kernels
     // the ordered GS launches of one sweep: fn(rows, nrows, colour array, current colour, skip mode)
     template <typename F> void sweep(const Level& lv, bool reverse, bool first_from_zero, F&& fn) const;

@@ -360,18 +360,20 @@ void tile_gs(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol,
 }
 
 template <typename T>
-void tile_apply(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta,
-                int W, const T* At, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part)
+int tile_apply(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta,
+               int W, const T* At, const T* x, const T* b, T* y, const T* dotv, double* dot_out, double* part)
 {
-    if (ntiles <= 0) return;
+    if (ntiles <= 0) return 0;
     const int2* m = reinterpret_cast<const int2*>(tmeta);
     const unsigned g = (unsigned)((ntiles + 3) / 4);
+    double* pw = (dotv && part) ? part : nullptr;
     MGLDG_TILE_DISPATCH(B, MGLDG_TILE_WIDTH(W, {
         hipLaunchKernelGGL((k_apply_tile<T, BB, WW>), dim3(g), dim3(256), 0, s, ntiles, trow, tcol, tpid, m, At, x, b, y,
-                           dotv, dot_out ? part : nullptr);
+                           dotv, pw);
     }));
     tile_check("apply_tile");
     if (dot_out) finalize_sum(s, part, (int)g, dot_out);
+    return (int)g;
 }
 
 template void tile_pool<float>(hipStream_t, long, int, const double*, float*, int);
@@ -380,10 +382,10 @@ template void tile_gs<float>(hipStream_t, int, int, const int*, const int*, cons
                              const float*, const float*, float*, float, const int*, int, int);
 template void tile_gs<double>(hipStream_t, int, int, const int*, const int*, const int*, const int*, int, const double*,
                               const double*, const double*, double*, double, const int*, int, int);
-template void tile_apply<float>(hipStream_t, int, int, const int*, const int*, const int*, const int*, int, const float*,
-                                const float*, const float*, float*, const float*, double*, double*);
-template void tile_apply<double>(hipStream_t, int, int, const int*, const int*, const int*, const int*, int,
-                                 const double*, const double*, const double*, double*, const double*, double*, double*);
+template int tile_apply<float>(hipStream_t, int, int, const int*, const int*, const int*, const int*, int, const float*,
+                               const float*, const float*, float*, const float*, double*, double*);
+template int tile_apply<double>(hipStream_t, int, int, const int*, const int*, const int*, const int*, int,
+                                const double*, const double*, const double*, double*, const double*, double*, double*);
 
 }  // namespace dev
 }  // namespace mgldg
