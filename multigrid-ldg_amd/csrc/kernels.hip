@@ -364,7 +364,7 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
     auto fetch = [&](int m) {
         if (POOL) {
             const int2 e = (valid && m < W) ? reinterpret_cast<const int2*>(colp)[m] : make_int2(-1, 0);
-            mycol = e.x; mypid = e.y;
+            mycol = e.x; mypid = e.y;   // (pool index | kPidLower on GS tables)
         } else {
             mycol = (valid && m < W) ? colp[m] : -1;
         }
@@ -403,12 +403,13 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
                 for (int u = 0; u < UX; ++u) {
                     const int src = lane0 + min(u0 + u, B - 1);
                     const int j = __shfl(mycol, src, 64);
-                    pid[u] = __shfl(mypid, src, 64);
+                    const int pe = __shfl(mypid, src, 64);
+                    pid[u] = pe & kPidMask;
                     const bool live = valid && u0 + u < nc && j >= 0;
                     xv[u] = live ? x[(size_t)j * B + r] : (T)0;
-                    bool ok = live;
-                    if (SKIP != 0 && live) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
-                    use[u] = ok;
+                    // first sweep: a neighbour of a preceding colour is written (forward) or not (reverse)
+                    const bool lower = (pe & kPidLower) != 0;
+                    use[u] = live && (SKIP == 0 || (SKIP == 1 ? lower : !lower));
                 }
 #pragma unroll
                 for (int u = 0; u < UX; ++u) xs[u * BP + r] = use[u] ? xv[u] : (T)0;
@@ -439,23 +440,6 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
         for (int u0 = 0; u0 < nc; u0 += U) {
             VecT<T, V> a[U][B / V];
             bool use[U];
-            if (POOL) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int src = lane0 + min(u0 + u, B - 1);
-                    const int j = __shfl(mycol, src, 64);
-                    const int pid = __shfl(mypid, src, 64);
-                    bool ok = valid && u0 + u < nc && j >= 0;
-                    if (SKIP != 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
-                    use[u] = ok;
-                    if (ok) {
-                        const T* Ab = A + (size_t)pid * (B * B) + r * V;
-#pragma unroll
-                        for (int cc = 0; cc < B / V; ++cc) a[u][cc] = ldv<T, V>(Ab + cc * B * V);
-                    }
-                    xs[u * B + r] = ok ? x[(size_t)j * B + r] : (T)0;
-                }
-            } else {
             if (SKIP == 0) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -482,7 +466,6 @@ __device__ __forceinline__ T row_sum(bool valid, int k0, int k1, int dp, const i
                     }
                 }
                 xs[u * B + r] = ok ? x[(size_t)j * B + r] : (T)0;
-            }
             }
             wave_sync();
 #pragma unroll
@@ -768,12 +751,13 @@ struct LatRow {
         for (int u = 0; u < kLatW; ++u) {
             const int src = lane0 + (u < B ? u : B - 1);
             const int j = __shfl(e.x, src, 64);
-            pid[u] = __shfl(e.y, src, 64);
+            const int pe = __shfl(e.y, src, 64);
+            pid[u] = pe & kPidMask;
             const bool live = !none && valid && u < W && j >= 0;
             xv[u] = live ? x[(size_t)j * B + r] : (T)0;
-            bool ok = live;
-            if (SKIP != 0 && live) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
-            use[u] = ok;
+            // first sweep: a neighbour of a preceding colour is written (forward) or not (reverse)
+            const bool lower = (pe & kPidLower) != 0;
+            use[u] = live && (SKIP == 0 || (SKIP == 1 ? lower : !lower));
             if (u < C::UA && live) {
                 const T* Ab = A + (size_t)pid[u] * (B * C::BP) + r * C::VP;
 #pragma unroll

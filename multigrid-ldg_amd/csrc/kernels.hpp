@@ -12,6 +12,10 @@ namespace dev {
 // forward, skip 2: reverse): no neighbour of its rows has been written yet,
 // and the colour tests of the skip modes read none of them
 constexpr int kFirstFwd = -1;
+// pooled GS tables: (column, pool index) pairs whose pool index carries
+// kPidLower when the column's colour precedes the row's in the sweep order
+constexpr int kPidMask = (1 << 30) - 1;
+constexpr int kPidLower = 1 << 30;
 constexpr int kFirstRev = 1 << 30;
 
 // ---------------------------------------------------------------- setup

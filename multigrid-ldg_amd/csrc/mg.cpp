@@ -104,8 +104,13 @@ void DevOp::build_ell(hipStream_t s)
 // pidx / didx (pooled operator): per block slot its A pool index, per row its
 // D^-1 pool index; the table then holds (column, pool index) pairs and
 // desc.w the D^-1 pool index
+// colour (pooled tables): the order of the sweep the tables serve (colours
+// or wavefronts); a pair's pool index then carries dev::kPidLower when the
+// column's colour precedes the row's, so that the first sweep from zero
+// knows which neighbours are already written without reading colours
 static void gs_tables(const HostBSR& P, const std::vector<int>& rows, int w, hipStream_t s, DBuf<int>& desc_d,
-                      DBuf<int>& gcols_d, const std::vector<int>* pidx = nullptr, const std::vector<int>* didx = nullptr)
+                      DBuf<int>& gcols_d, const std::vector<int>* pidx = nullptr, const std::vector<int>* didx = nullptr,
+                      const std::vector<int>* colour = nullptr)
 {
     const size_t nr = rows.size();
     const int ew = pidx ? 2 : 1;
@@ -116,7 +121,12 @@ static void gs_tables(const HostBSR& P, const std::vector<int>& rows, int w, hip
         int dp = -1, m = 0;
         for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
             if (P.cols[k] == i) { if (dp < 0) { dp = k; continue; } }
-            if (pidx) { gc[(q * w + m) * 2] = P.cols[k]; gc[(q * w + m) * 2 + 1] = (*pidx)[k]; ++m; }
+            if (pidx) {
+                const int pv = (*pidx)[k];
+                if (pv < 0 || pv > dev::kPidMask) throw Error(-5, "pool index out of range");
+                const bool lower = colour && (*colour)[P.cols[k]] < (*colour)[i];
+                gc[(q * w + m) * 2] = P.cols[k]; gc[(q * w + m) * 2 + 1] = pv | (lower ? dev::kPidLower : 0); ++m;
+            }
             else gc[q * w + m++] = P.cols[k];
         }
         desc[4 * q] = i; desc[4 * q + 1] = P.rowptr[i]; desc[4 * q + 2] = P.rowptr[i + 1];
@@ -577,7 +587,7 @@ static void pair_schedule(Level& lv, hipStream_t s, const std::vector<int>* pidx
         order.insert(order.end(), rb[c].begin(), rb[c].end());
         wg.insert(wg.end(), {a, m, (int)order.size()});
     }
-    gs_tables(P, order, lv.gw, s, lv.pair_desc, lv.pair_gcols, pidx, didx);
+    gs_tables(P, order, lv.gw, s, lv.pair_desc, lv.pair_gcols, pidx, didx, &lv.color);
     lv.pair_wg.upload(wg.data(), wg.size(), s);
     HIPCHK(hipStreamSynchronize(s));
     lv.pair_nwg = (int)(wg.size() / 3);
@@ -832,7 +842,7 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         for (int i = 0; i < lv.nb; ++i) rows[pos[color[i]]++] = i;
         rows_d.upload(rows.data(), rows.size(), s);
         col_d.upload(color.data(), color.size(), s);
-        gs_tables(lv.A.pat, rows, lv.gw, s, desc_d, gcols_d, pp, dp);
+        gs_tables(lv.A.pat, rows, lv.gw, s, desc_d, gcols_d, pp, dp, &color);
     };
     if (lexicographic) {
         // wavefronts of the lexicographic order: a row waits for its earlier
