@@ -656,7 +656,10 @@ struct TileHost {
     int n = 0;
 };
 template <typename F>
-static void tile_rows(const std::vector<int>& rows, int W, const std::vector<int>& dkey, F&& blocks, TileHost& th)
+// colour (GS tiles): a column whose colour precedes the row's carries
+// dev::kPidLower (the first sweep from zero reads those neighbours only)
+static void tile_rows(const std::vector<int>& rows, int W, const std::vector<int>& dkey, F&& blocks, TileHost& th,
+                      const std::vector<int>* colour = nullptr)
 {
     std::map<std::vector<int>, int> sig;
     std::vector<std::vector<int>> members;
@@ -688,7 +691,8 @@ static void tile_rows(const std::vector<int>& rows, int W, const std::vector<int
                 std::sort(bl.begin(), bl.end());
                 th.row[(size_t)t * 16 + r] = i;
                 for (size_t u = 0; u < bl.size(); ++u) {
-                    th.col[((size_t)t * 16 + r) * W + u] = bl[u].second;
+                    const int j = bl[u].second;
+                    th.col[((size_t)t * 16 + r) * W + u] = j | ((colour && (*colour)[j] < (*colour)[i]) ? dev::kPidLower : 0);
                     if (r == 0) th.pid[(size_t)t * W + u] = bl[u].first;
                 }
                 if (r == 0) { th.meta[2 * t] = (int)bl.size(); th.meta[2 * t + 1] = dkey.empty() ? 0 : dkey[i]; }
@@ -710,6 +714,7 @@ static bool every_row_has_diagonal(const HostBSR& P)
 static void build_tiles(hipStream_t s, Level& lv, const std::vector<int>& pidx, const std::vector<int>& didx)
 {
     const HostBSR& P = lv.A.pat;
+    if (lv.nb > dev::kPidMask) return;   // (columns share their word with kPidLower)
     TileHost g, a;
     lv.t_W = dev::tile_width(lv.gw);
     lv.tile_off.assign(lv.ncolor + 1, 0);
@@ -724,7 +729,7 @@ static void build_tiles(hipStream_t s, Level& lv, const std::vector<int>& pidx, 
     for (int c = 0; c < lv.ncolor; ++c) {
         std::vector<int> rows;
         for (int i = 0; i < lv.nb; ++i) if (lv.color[i] == c) rows.push_back(i);
-        tile_rows(rows, lv.t_W, didx, offd, g);
+        tile_rows(rows, lv.t_W, didx, offd, g, &lv.color);
         lv.tile_off[c + 1] = g.n;
     }
     // the application's tiles: the diagonal block's pool index as the key,
@@ -743,7 +748,7 @@ static void build_tiles(hipStream_t s, Level& lv, const std::vector<int>& pidx, 
         bool ok = th.row.size() == (size_t)th.n * 16 && th.col.size() == (size_t)th.n * 16 * W &&
                   th.pid.size() == (size_t)th.n * W && th.meta.size() == (size_t)th.n * 2;
         for (int v : th.row) ok = ok && v >= -1 && v < lv.nb;
-        for (int v : th.col) ok = ok && v >= -1 && v < lv.nb;
+        for (int v : th.col) ok = ok && (v == -1 || (v >= 0 && (v & dev::kPidMask) < lv.nb));
         for (int v : th.pid) ok = ok && v >= -1 && v < lv.npool;
         for (int t = 0; t < th.n; ++t)
             ok = ok && th.meta[2 * t] >= 0 && th.meta[2 * t] <= W && th.meta[2 * t + 1] >= 0 && th.meta[2 * t + 1] < nkey;

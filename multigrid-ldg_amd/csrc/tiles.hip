@@ -134,10 +134,13 @@ __device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<W
     if constexpr (OWN) { jj[0] = iown; pp[0] = pown; }
     const bool kin = S * q < B;                       // this lane's K range is inside the block
     const int koff = kin ? S * q : B - S;
-    int cj[NS];
-    if constexpr (SKIP != 0) {
+    // GS tables: a column carries kPidLower when its colour precedes the
+    // row's (the neighbours a first sweep from zero has already written)
+    bool low[NS];
 #pragma unroll
-        for (int u = O; u < NS; ++u) cj[u] = color[jj[u] >= 0 ? jj[u] : 0];
+    for (int u = 0; u < NS; ++u) {
+        low[u] = (jj[u] & kPidLower) != 0;
+        jj[u] = jj[u] >= 0 ? (jj[u] & kPidMask) : -1;
     }
     T xb[NS][S];
 #pragma unroll
@@ -151,7 +154,7 @@ __device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<W
     for (int u = 0; u < NS; ++u) {
         if (u + 1 < NS) tile_ld4<T>(a[(u + 1) & 1], At + (size_t)pp[u + 1] * kTileBlock, n, q);
         bool use = jj[u] >= 0 && kin;
-        if constexpr (SKIP != 0) if (u >= O) use = use && (SKIP == 1 ? cj[u] <= cur : cj[u] >= cur);
+        if constexpr (SKIP != 0) if (u >= O) use = use && (SKIP == 1 ? low[u] : !low[u]);
         const T* au = a[u & 1];
 #pragma unroll
         for (int e = 0; e < S; ++e) acc = Tm<T>::mac(au[e], use ? xb[u][e] : (T)0, acc);
