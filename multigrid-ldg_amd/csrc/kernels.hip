@@ -2051,11 +2051,31 @@ void eye(hipStream_t s, int n, double* X)
 // --------------------------------------------------------------------- PCG kernels
 constexpr int kRedBlocks = 1024;   // grid of the reduction kernels (grid-stride)
 
+// The grid-stride loops of the CG passes take kUnroll elements of a thread
+// at a time, their loads issued together (a grid of kRedBlocks workgroups
+// holds a few wavefronts per SIMD: one element per round trip would leave
+// HBM idle).  Sums run in ascending i as in the plain loop.
+constexpr int kUnroll = 4;
+#define MGLDG_STRIDED(n, BODY_LOAD, BODY_USE)                                               \
+    {                                                                                       \
+        const long stride_ = (long)gridDim.x * 256;                                         \
+        for (long i0_ = (long)blockIdx.x * 256 + threadIdx.x; i0_ < (n); i0_ += kUnroll * stride_) { \
+            _Pragma("unroll") for (int u = 0; u < kUnroll; ++u) {                           \
+                const long i = i0_ + u * stride_;                                           \
+                if (i < (n)) { BODY_LOAD; }                                                 \
+            }                                                                               \
+            _Pragma("unroll") for (int u = 0; u < kUnroll; ++u) {                           \
+                const long i = i0_ + u * stride_;                                           \
+                if (i < (n)) { BODY_USE; }                                                  \
+            }                                                                               \
+        }                                                                                   \
+    }
+
 __global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
                                              double* part)
 {
-    double s = 0;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) s += a[i] * b[i];
+    double s = 0, av[kUnroll], bv[kUnroll];
+    MGLDG_STRIDED(n, (av[u] = a[i], bv[u] = b[i]), (s += av[u] * bv[u]));
     block_partial(s, part);
 }
 
@@ -2080,9 +2100,22 @@ __device__ __forceinline__ void wg_sum_partials(const double* __restrict__ part,
     double t[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) t[k] = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    // a thread's partials i = tid, tid + blockDim, ... are loaded four at a
+    // time (one round trip) and added in that order
+    for (int base = 0; base < n; base += 4 * (int)blockDim.x) {
+        double v[4][NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) t[k] += part[(size_t)i * NV + k];
+        for (int q = 0; q < 4; ++q) {
+            const int i = base + q * (int)blockDim.x + (int)threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[q][k] = i < n ? part[(size_t)i * NV + k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                if (base + q * (int)blockDim.x + (int)threadIdx.x < n) t[k] += v[q][k];
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const double v = block_sum(t[k]);
@@ -2106,11 +2139,12 @@ __global__ __launch_bounds__(256) void k_pcg_xr(long n, double* S, const double*
     wg_sum_partials<1>(dqp, ndq, &dq);
     if (blockIdx.x == 0 && threadIdx.x == 0) S[S_DQ] = dq;
     const double alpha = S[S_DELTA] / dq;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        x[i] += alpha * d[i];
-        double ri = r[i] - alpha * q[i];
+    double dv[kUnroll], qv[kUnroll], xv[kUnroll], rv[kUnroll];
+    MGLDG_STRIDED(n, (dv[u] = d[i], qv[u] = q[i], xv[u] = x[i], rv[u] = r[i]), {
+        x[i] = xv[u] + alpha * dv[u];
+        const double ri = rv[u] - alpha * qv[u];
         r[i] = ri; v[i] = (T)ri;
-    }
+    });
 }
 
 static unsigned red_grid(long n) { return (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + 255) / 256)); }
@@ -2134,7 +2168,8 @@ __global__ __launch_bounds__(256) void k_pcg_x(long n, double* S, const double* 
     wg_sum_partials<1>(dqp, ndq, &dq);
     if (blockIdx.x == 0 && threadIdx.x == 0) S[S_DQ] = dq;
     const double alpha = S[S_DELTA] / dq;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) x[i] += alpha * d[i];
+    double dv[kUnroll], xv[kUnroll];
+    MGLDG_STRIDED(n, (dv[u] = d[i], xv[u] = x[i]), (x[i] = xv[u] + alpha * dv[u]));
 }
 
 void pcg_x(hipStream_t s, long n, double* S, const double* d, double* x, const double* dqp, int ndq)
@@ -2153,15 +2188,16 @@ __global__ __launch_bounds__(256) void k_pcg_rz(long n, double* S, const double*
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) S[S_DELTA_OLD] = S[S_DELTA];
     double a = 0, b = 0, c = 0;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        const double ri = r[i];
-        const T zi = zs[i];
-        a += ri * (double)zi; c += ri * ri;
+    double rv[kUnroll];
+    T zv[kUnroll], ov[kUnroll];
+    MGLDG_STRIDED(n, (rv[u] = r[i], zv[u] = zs[i], ov[u] = (zold && read_old) ? zold[i] : (T)0), {
+        const double ri = rv[u];
+        a += ri * (double)zv[u]; c += ri * ri;
         if (zold) {
-            if (read_old) b += ri * (double)zold[i];
-            zold[i] = zi;
+            if (read_old) b += ri * (double)ov[u];
+            zold[i] = zv[u];
         }
-    }
+    });
     double ba = block_sum(a), bb = block_sum(b), bc = block_sum(c);
     if (threadIdx.x == 0) { part[3 * blockIdx.x] = ba; part[3 * blockIdx.x + 1] = bb; part[3 * blockIdx.x + 2] = bc; }
 }
@@ -2192,8 +2228,9 @@ __global__ __launch_bounds__(256) void k_pcg_d(long n, double* S, const T* __res
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S[S_DELTA] = t[0]; S[S_RSOLD] = t[1]; S[S_RR] = t[2]; S[S_BETA] = beta;
     }
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-        d[i] = init ? (double)zs[i] : (double)zs[i] + beta * d[i];
+    T zv[kUnroll];
+    double dv[kUnroll];
+    MGLDG_STRIDED(n, (zv[u] = zs[i], dv[u] = init ? 0.0 : d[i]), (d[i] = init ? (double)zv[u] : (double)zv[u] + beta * dv[u]));
 }
 
 template <typename T>
