@@ -184,11 +184,13 @@ struct MF {
 // tile layout made by gemm_tiles (gemm_tiles_size(n) doubles), the K range
 // split over slices summed in order (part: gemm_part_size(n, k) doubles)
 int gemm_tiles_size(int n);
-void gemm_tiles(hipStream_t s, int n, const double* W, double* Wt);
+template <typename TW>
+void gemm_tiles(hipStream_t s, int n, const double* W, TW* Wt);
 int gemm_slices(int n, int k);
 long gemm_part_size(int n, int k);
-template <typename T>
-void gemm_mf(hipStream_t s, int n, int k, const double* Wt, const T* x, T* y, double* part);
+// (Wt of type TW: FP64 tiles, or FP32 tiles on FP32 MFMAs for T = float)
+template <typename T, typename TW>
+void gemm_mf(hipStream_t s, int n, int k, const TW* Wt, const T* x, T* y, double* part);
 // Y = W X for k columns ([n][k] layout), W FP64 n x n row-major
 template <typename T>
 void gemm_f64(hipStream_t s, int n, int k, const double* W, const T* x, T* y);

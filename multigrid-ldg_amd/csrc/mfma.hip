@@ -447,7 +447,9 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
 
 // W (n x n, row-major FP64) -> 16 x 4 tiles in the A-operand lane order:
 // Wt[(R * nK + K) * 64 + l] = W[16 R + l % 16][4 K + l / 16], zero-padded
-__global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, double* __restrict__ Wt)
+// (FP64, or rounded to FP32 for the FP32 V-cycle)
+template <typename TW>
+__global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, TW* __restrict__ Wt)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long)nR * nK * 64) return;
@@ -455,7 +457,7 @@ __global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, do
     const long t = e / 64;
     const int K = (int)(t % nK), R = (int)(t / nK);
     const int r = 16 * R + (l & 15), c = 4 * K + (l >> 4);
-    Wt[e] = (r < n && c < n) ? W[(size_t)r * n + c] : 0.0;
+    Wt[e] = (r < n && c < n) ? (TW)W[(size_t)r * n + c] : (TW)0;
 }
 
 // partial products of Y = W X over K slices, part[(slice * nRpad + row) * k + col].
@@ -463,13 +465,15 @@ __global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, do
 // Y and one K slice: each W operand is loaded once for its NCW column tiles,
 // and four K steps of operands are loaded before their products, so
 // several loads are in flight per wavefront.  Products accumulate in K
-// order within the slice.
-template <typename T, int NCW>
+// order within the slice, on the MFMAs of W's type TW (the FP32 V-cycle
+// reads an FP32 copy of W: half the bytes, FP32 matrix-core rate); the
+// slices are summed in FP64.
+template <typename T, typename TW, int NCW>
 __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int nslices, int kpers,
-                                               const double* __restrict__ Wt, const T* __restrict__ x,
+                                               const TW* __restrict__ Wt, const T* __restrict__ x,
                                                double* __restrict__ part)
 {
-    using V4 = Mf<double>::v4;
+    using V4 = typename Mf<TW>::v4;
     const int lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int nC = (k + 15) / 16, ncg = (nC + NCW - 1) / NCW;
@@ -481,14 +485,14 @@ __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int
     V4 acc[NCW];
 #pragma unroll
     for (int t = 0; t < NCW; ++t) acc[t] = V4{0, 0, 0, 0};
-    const double* wt = Wt + (size_t)R * nK * 64 + lane;
+    const TW* wt = Wt + (size_t)R * nK * 64 + lane;
     auto xval = [&](int K, int t) {
         const int c = 4 * K + q, col = 16 * (G * NCW + t) + cl;
-        return (c < n && col < k) ? (double)x[(size_t)c * k + col] : 0.0;
+        return (c < n && col < k) ? (TW)x[(size_t)c * k + col] : (TW)0;
     };
     int K = K0;
     for (; K + 4 <= K1; K += 4) {
-        double a[4], b[4][NCW];
+        TW a[4], b[4][NCW];
 #pragma unroll
         for (int u = 0; u < 4; ++u) a[u] = wt[(size_t)(K + u) * 64];
 #pragma unroll
@@ -498,12 +502,12 @@ __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int t = 0; t < NCW; ++t) acc[t] = Mf<double>::mac(a[u], b[u][t], acc[t]);
+            for (int t = 0; t < NCW; ++t) acc[t] = Mf<TW>::mac(a[u], b[u][t], acc[t]);
     }
     for (; K < K1; ++K) {
-        const double a = wt[(size_t)K * 64];
+        const TW a = wt[(size_t)K * 64];
 #pragma unroll
-        for (int t = 0; t < NCW; ++t) acc[t] = Mf<double>::mac(a, xval(K, t), acc[t]);
+        for (int t = 0; t < NCW; ++t) acc[t] = Mf<TW>::mac(a, xval(K, t), acc[t]);
     }
 #pragma unroll
     for (int t = 0; t < NCW; ++t) {
@@ -511,8 +515,8 @@ __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int
         if (col >= k) continue;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int r = 16 * R + Mf<double>::row(lane, e);
-            part[((size_t)sl * (16 * nR) + r) * k + col] = acc[t][e];
+            const int r = 16 * R + Mf<TW>::row(lane, e);
+            part[((size_t)sl * (16 * nR) + r) * k + col] = (double)acc[t][e];
         }
     }
 }
@@ -619,14 +623,17 @@ void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* ro
 
 int gemm_tiles_size(int n) { return ((n + 15) / 16) * ((n + 3) / 4) * 64; }
 
-void gemm_tiles(hipStream_t s, int n, const double* W, double* Wt)
+template <typename TW>
+void gemm_tiles(hipStream_t s, int n, const double* W, TW* Wt)
 {
     const int nR = (n + 15) / 16, nK = (n + 3) / 4;
     const long t = (long)nR * nK * 64;
     if (t <= 0) return;
-    hipLaunchKernelGGL(k_tile_w, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, nR, nK, W, Wt);
+    hipLaunchKernelGGL(k_tile_w<TW>, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, nR, nK, W, Wt);
     check("tile_w");
 }
+template void gemm_tiles<double>(hipStream_t, int, const double*, double*);
+template void gemm_tiles<float>(hipStream_t, int, const double*, float*);
 
 // column tiles per wavefront of mf_gemm
 static int gemm_ncw(int k)
@@ -648,24 +655,25 @@ int gemm_slices(int n, int k)
 
 long gemm_part_size(int n, int k) { return (long)gemm_slices(n, k) * 16 * ((n + 15) / 16) * k; }
 
-template <typename T>
-void gemm_mf(hipStream_t s, int n, int k, const double* Wt, const T* x, T* y, double* part)
+template <typename T, typename TW>
+void gemm_mf(hipStream_t s, int n, int k, const TW* Wt, const T* x, T* y, double* part)
 {
     if (n <= 0 || k <= 0) return;
     const int nR = (n + 15) / 16, nK = (n + 3) / 4, nC = (k + 15) / 16;
     const int sl = gemm_slices(n, k), kpers = (nK + sl - 1) / sl;
     const int ncw = gemm_ncw(k), ncg = (nC + ncw - 1) / ncw;
     const dim3 g(nblocks((long)nR * ncg * sl));
-    if (ncw == 4) hipLaunchKernelGGL((mf_gemm<T, 4>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
-    else if (ncw == 2) hipLaunchKernelGGL((mf_gemm<T, 2>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
-    else hipLaunchKernelGGL((mf_gemm<T, 1>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    if (ncw == 4) hipLaunchKernelGGL((mf_gemm<T, TW, 4>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    else if (ncw == 2) hipLaunchKernelGGL((mf_gemm<T, TW, 2>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    else hipLaunchKernelGGL((mf_gemm<T, TW, 1>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
     check("mf_gemm");
     const long t = (long)n * k;
     hipLaunchKernelGGL(k_gemm_sum<T>, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, k, nR, sl, part, y);
     check("gemm_sum");
 }
-template void gemm_mf<float>(hipStream_t, int, int, const double*, const float*, float*, double*);
-template void gemm_mf<double>(hipStream_t, int, int, const double*, const double*, double*, double*);
+template void gemm_mf<float, double>(hipStream_t, int, int, const double*, const float*, float*, double*);
+template void gemm_mf<float, float>(hipStream_t, int, int, const float*, const float*, float*, double*);
+template void gemm_mf<double, double>(hipStream_t, int, int, const double*, const double*, double*, double*);
 
 template struct MF<float>;
 template struct MF<double>;

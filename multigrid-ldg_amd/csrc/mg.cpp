@@ -1384,7 +1384,11 @@ void MGH::build_collapse()
         dev::f64_to_f32(s, (long)k * k, W, W32);
     }
     Wt.alloc((size_t)dev::gemm_tiles_size(k));
-    dev::gemm_tiles(s, k, W, Wt);
+    dev::gemm_tiles<double>(s, k, W, Wt);
+    if (prm.precision == 1) {
+        Wt32.alloc((size_t)dev::gemm_tiles_size(k));
+        dev::gemm_tiles<float>(s, k, W, Wt32);
+    }
     HIPCHK(hipStreamSynchronize(s));
 }
 
@@ -1708,7 +1712,10 @@ void MGH::bvcycle(int k)
     Level& c = L[bottom];
     const int n = c.nb * c.B;
     if (bottom == cut) {
-        dev::gemm_mf<T>(s, n, k, Wt, bb<T>(BL[bottom]), bx<T>(BL[bottom]), Bgemm);
+        if (sizeof(T) == 4 && Wt32.p)
+            dev::gemm_mf<float, float>(s, n, k, Wt32.p, (const float*)bb<T>(BL[bottom]), (float*)bx<T>(BL[bottom]), Bgemm);
+        else
+            dev::gemm_mf<T, double>(s, n, k, Wt.p, bb<T>(BL[bottom]), bx<T>(BL[bottom]), Bgemm);
     } else if (prm.coarse == 0) {
         dev::MOps<T>::coarse(s, n, k, c.Cinv, bb<T>(BL[bottom]), bx<T>(BL[bottom]), submean);
     } else {

@@ -210,6 +210,7 @@ struct MGH {
     DBuf<double> W;                    // collapsed V-cycle below `cut`, FP64 dense n x n
     DBuf<float> W32;                   // the same in FP32 for the FP32 V-cycle (mixed hierarchies, n % 4 == 0)
     DBuf<double> Wt;                   // the same in the MFMA tile layout (batched form)
+    DBuf<float> Wt32;                  // and rounded to FP32 (the batched FP32 V-cycle, mixed hierarchies)
 
     // batched (k right-hand sides) state
     struct BatchLevel { DBuf<double> x64, b64, r64, alt64; DBuf<float> x32, b32, r32, alt32; };
