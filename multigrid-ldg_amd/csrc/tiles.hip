@@ -115,9 +115,8 @@ struct TileSlots {
 // Y += sum over the tile's block positions of A_{p_u} X_u; SKIP as the
 // vector GS kernels (1 / 2: first sweep from zero, neighbours of colours not
 // yet updated in sweep order are zero).  With OWN the row's own block comes
-// first (x_i with the pool block pown).  Order of issue: every x_j gather
-// (and the neighbours' colours), then the pool blocks one position ahead of
-// the MFMAs that use them.  Loads use in-range addresses (x_0 for an absent
+// first (x_i with the pool block pown).  Order of issue: every x_j gather,
+// then the pool blocks of up to PD positions, then the MFMAs.  Loads use in-range addresses (x_0 for an absent
 // neighbour, x_j's last S elements for a lane whose K range lies past B) and
 // the values are zeroed afterwards; positions past the tile's blocks have
 // column -1 and add zero.  At: the pool in the tile layout (zero padding).
@@ -145,19 +144,25 @@ __device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<W
     T xb[NS][S];
 #pragma unroll
     for (int u = 0; u < NS; ++u) uld<T, S>(xb[u], x + (size_t)(jj[u] >= 0 ? jj[u] : 0) * B + koff);
-    T a[2][4];
-    tile_ld4<T>(a[0], At + (size_t)pp[0] * kTileBlock, n, q);
+    // The pool blocks of PD positions are in flight together: a position's
+    // four MFMAs take far less than a load from cache, so with one block
+    // ahead every position waited for its own block (a chain of NS round
+    // trips).  FP32 holds them all (4 VGPRs each); FP64 eight at a time.
+    constexpr int PD = sizeof(T) == 4 ? NS : (NS < 8 ? NS : 8);
+    T a[PD][4];
+#pragma unroll
+    for (int u = 0; u < PD; ++u) tile_ld4<T>(a[u], At + (size_t)pp[u] * kTileBlock, n, q);
     // every gather is in flight before the first MFMA waits on one (left to
     // itself the scheduler issues each position's gathers next to its MFMAs)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-        if (u + 1 < NS) tile_ld4<T>(a[(u + 1) & 1], At + (size_t)pp[u + 1] * kTileBlock, n, q);
         bool use = jj[u] >= 0 && kin;
         if constexpr (SKIP != 0) if (u >= O) use = use && (SKIP == 1 ? low[u] : !low[u]);
-        const T* au = a[u & 1];
+        T* au = a[u % PD];
 #pragma unroll
         for (int e = 0; e < S; ++e) acc = Tm<T>::mac(au[e], use ? xb[u][e] : (T)0, acc);
+        if (u + PD < NS) tile_ld4<T>(au, At + (size_t)pp[u + PD] * kTileBlock, n, q);
     }
 }
 
