@@ -421,8 +421,22 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1, fd=True):
         if prec == "fp64":
             grads = (dLdf, dLdg)
         log(f"adjoint {prec}: {dt * 1e3:.1f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {ast['iterations']}")
+    # the fine level's one-colour GS launch (row tiles where the level has
+    # them), HIP-event timed, against the HBM peak: algorithmic bytes of the
+    # library's model (tables, b and x of the colour's rows, x of their
+    # neighbours once, the pool blocks once, x written)
+    ntile, twidth, t64, t32 = mg.level_tiles(0)
+    fine_gs = {}
+    for prec, name, tiled in (("mixed", "fp32", t32), ("fp64", "fp64", t64)):
+        ms, by, _ = mg.profile_relax(0, prec, 50, part="single")
+        T = "float" if name == "fp32" else "double"
+        fine_gs[name] = {"kernel": f"k_gs_tile<{T},9,0,{twidth}>" if tiled else f"k_gs<{T},9,0,true>",
+                         "us_per_launch": round(ms * 1e3, 3), "bytes_per_launch": by,
+                         "achieved": round(by / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"adjoint fine GS {name}: {ms * 1e3:.2f} us, {by / 1e6:.2f} MB -> {fine_gs[name]['achieved']:.0f} GB/s")
     if not fd:   # profiling mode: the adjoint solves alone
-        return {"ndof": prob.ndof, "levels": mg.nlevels, "adjoint": out}
+        return {"ndof": prob.ndof, "levels": mg.nlevels, "adjoint": out, "fine_gs_launch": fine_gs}
     dLdf, dLdg = grads
     # central differences along random smooth directions (4 Gaussian bumps in f,
     # low Fourier modes of g), 32 perturbed solves per batch.  L is quadratic in
@@ -461,7 +475,9 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1, fd=True):
     return {
         "workload": "uniform 1024x1024 quadtree, p=2, L = 0.5||u - u*||^2, f* / f = 16 Gaussian bumps (seeds 0 / 1), "
                     "g = sin(2 pi (x+y))",
-        "ndof": prob.ndof, "levels": mg.nlevels, "setup_s": round(t_setup, 3), "adjoint": out, "loss": L0,
+        "ndof": prob.ndof, "levels": mg.nlevels, "setup_s": round(t_setup, 3), "adjoint": out,
+        "mixed_vs_fp64": round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3),
+        "fine_gs_launch": fine_gs, "loss": L0,
         "fd_check": {"directions": ndir, "kind": "smooth random (4 bumps in f, 3 Fourier modes in g)",
                      "by_eps": fd_out, "solve_tol": 1e-10, "relres_floor_accepted": floor, "seconds": round(t_fd, 2)},
     }
@@ -1019,6 +1035,7 @@ def main():
                     "mixed": {"value": round(vb32, 1), "ms_per_step": round(dtb32 / bsteps * 1e3, 3),
                               "max_iterations": max(s["iterations"] for s in stb32)},
                     "mixed_batched_vs_fp64_single": round(vb32 / v64, 3),
+                    "mixed_vs_fp64_batched": round(vb32 / vb64, 3),
                     "kernel_plan": plan_summary(plan), "mfma_fine_apply": mfma_use(mg, plan, k)},
         "vcycles": vcyc,
         "hp_multigrid": hp,
