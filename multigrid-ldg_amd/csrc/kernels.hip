@@ -2259,13 +2259,33 @@ __device__ __forceinline__ void col_partials(int k, int nval, const double* vals
         }
 }
 
+// The batched passes walk rows i = g, g + G, ... (G = gridDim * groups) of
+// one column per thread, kUnroll rows at a time with their loads issued
+// together; sums in ascending i as in the plain loop.
+#define MGLDG_COLSTRIDED(BODY_LOAD, BODY_USE)                                               \
+    {                                                                                       \
+        const long gs_ = (long)gridDim.x * groups;                                          \
+        for (long i0_ = (long)blockIdx.x * groups + g; i0_ < n; i0_ += kUnroll * gs_) {     \
+            _Pragma("unroll") for (int u = 0; u < kUnroll; ++u) {                           \
+                const long o = (i0_ + u * gs_) * k + c;                                     \
+                if (i0_ + u * gs_ < n) { BODY_LOAD; }                                       \
+            }                                                                               \
+            _Pragma("unroll") for (int u = 0; u < kUnroll; ++u) {                           \
+                const long o = (i0_ + u * gs_) * k + c;                                     \
+                if (i0_ + u * gs_ < n) { BODY_USE; }                                        \
+            }                                                                               \
+        }                                                                                   \
+    }
+
 __global__ __launch_bounds__(256) void k_coldot(long n, int k, const double* __restrict__ a, const double* __restrict__ b,
                                                 double* part)
 {
     const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
     double acc = 0;
-    if (g < groups)
-        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) acc += a[i * k + c] * b[i * k + c];
+    if (g < groups) {
+        double av[kUnroll], bv[kUnroll];
+        MGLDG_COLSTRIDED((av[u] = a[o], bv[u] = b[o]), (acc += av[u] * bv[u]));
+    }
     col_partials(k, 1, &acc, part);
 }
 
@@ -2372,29 +2392,27 @@ void mean_zero(hipStream_t s, long n, int k, double* a, double* part)
 }
 
 // per active column: x += alpha d ; r -= alpha q ; every column: v = (T) r,
-// the V-cycle's level-0 right-hand sides
+// the V-cycle's level-0 right-hand sides (no r.r here: pcg_rz forms it
+// before anyone reads it)
 template <typename T>
 __global__ __launch_bounds__(256) void k_bpcg_xr(long n, int k, const double* S, const int* active, const double* __restrict__ d,
                                                  const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
-                                                 T* __restrict__ v, double* part)
+                                                 T* __restrict__ v)
 {
     const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
-    double acc = 0;
-    if (g < groups) {
-        const bool act = active[c] != 0;
-        const double alpha = act ? S[(size_t)c * S_COUNT + S_DELTA] / S[(size_t)c * S_COUNT + S_DQ] : 0.0;
-        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) {
-            const long o = i * k + c;
-            double ri = r[o];
-            if (act) {
-                x[o] += alpha * d[o];
-                ri = ri - alpha * q[o];
-                r[o] = ri; acc += ri * ri;
-            }
-            v[o] = (T)ri;
+    if (g >= groups) return;
+    const bool act = active[c] != 0;
+    const double alpha = act ? S[(size_t)c * S_COUNT + S_DELTA] / S[(size_t)c * S_COUNT + S_DQ] : 0.0;
+    double rv[kUnroll], dv[kUnroll], qv[kUnroll], xv[kUnroll];
+    MGLDG_COLSTRIDED((rv[u] = r[o], dv[u] = act ? d[o] : 0.0, qv[u] = act ? q[o] : 0.0, xv[u] = act ? x[o] : 0.0), {
+        double ri = rv[u];
+        if (act) {
+            x[o] = xv[u] + alpha * dv[u];
+            ri = ri - alpha * qv[u];
+            r[o] = ri;
         }
-    }
-    col_partials(k, 1, &acc, part);
+        v[o] = (T)ri;
+    });
 }
 
 template <typename T>
@@ -2402,10 +2420,8 @@ void bpcg_xr(hipStream_t s, long n, int k, double* S, const int* active, const d
              T* v, double* part)
 {
     unsigned g = red_grid(n, k);
-    hipLaunchKernelGGL(k_bpcg_xr<T>, dim3(g), dim3(256), 0, s, n, k, S, active, d, q, x, r, v, part);
+    hipLaunchKernelGGL(k_bpcg_xr<T>, dim3(g), dim3(256), 0, s, n, k, S, active, d, q, x, r, v);
     check_launch("bpcg_xr");
-    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, (int)g, k, 1, S + S_RR, (int)S_COUNT, nullptr, nullptr, 0);
-    check_launch("colfinalize");
 }
 template void bpcg_xr<float>(hipStream_t, long, int, double*, const int*, const double*, const double*, double*, double*, float*,
                              double*);
@@ -2434,17 +2450,18 @@ __global__ __launch_bounds__(256) void k_bpcg_rz(long n, int k, const double* __
 {
     const int groups = 256 / k, g = threadIdx.x / k, c = threadIdx.x % k;
     double v[3] = {0, 0, 0};
-    if (g < groups)
-        for (long i = (long)blockIdx.x * groups + g; i < n; i += (long)gridDim.x * groups) {
-            const long o = i * k + c;
-            const double ri = r[o];
-            const T zi = zs[o];
-            v[0] += ri * (double)zi; v[2] += ri * ri;
+    if (g < groups) {
+        double rv[kUnroll];
+        T zv[kUnroll], ov[kUnroll];
+        MGLDG_COLSTRIDED((rv[u] = r[o], zv[u] = zs[o], ov[u] = (zold && read_old) ? zold[o] : (T)0), {
+            const double ri = rv[u];
+            v[0] += ri * (double)zv[u]; v[2] += ri * ri;
             if (zold) {
-                if (read_old) v[1] += ri * (double)zold[o];
-                zold[o] = zi;
+                if (read_old) v[1] += ri * (double)ov[u];
+                zold[o] = zv[u];
             }
-        }
+        });
+    }
     col_partials(k, 3, v, part);
 }
 
