@@ -420,6 +420,10 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1, fd=True):
                      "status": ast["status"], "relres_true": ast["relres_true"]}
         if prec == "fp64":
             grads = (dLdf, dLdg)
+        else:
+            # the mixed form's gradients against the FP64 adjoint's, full size
+            rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+            out["mixed_vs_fp64_gradients"] = {"dLdf_rel_diff": rel(dLdf, grads[0]), "dLdg_rel_diff": rel(dLdg, grads[1])}
         log(f"adjoint {prec}: {dt * 1e3:.1f} ms, {prob.ndof / dt:.3g} DoF-solves/s, iterations {ast['iterations']}")
     # the fine level's one-colour GS launch (row tiles where the level has
     # them), HIP-event timed, against the HBM peak: algorithmic bytes of the
@@ -456,7 +460,9 @@ def adjoint_leg(mgldg, torch, ndir=64, eps=1e-6, warm=1, fd=True):
             cols.append(("g", v, float(dLdg @ v)))
     fd_out = {}
     t_fd = time.perf_counter()
-    for e in (eps, 1e-3):
+    # (L is quadratic in (f, g): every eps is exact but for rounding, which
+    # the larger ones divide by more)
+    for e in (eps, 1e-3, 1e-2):
         errs = []
         for c0 in range(0, ndir, 16):
             chunk = cols[c0:c0 + 16]
