@@ -743,8 +743,7 @@ struct LatRow {
     // second round trip: every x_j with its colour (first sweep) and the
     // pool rows of the first UA blocks.  Uniform trip counts: every shuffle
     // runs with all lanes active.
-    __device__ __forceinline__ void nbrs(const T* __restrict__ A, const T* x, const int* __restrict__ color, int cur, int W,
-                                         int r, int lane0)
+    __device__ __forceinline__ void nbrs(const T* __restrict__ A, const T* x, int cur, int W, int r, int lane0)
     {
         const bool none = reads_none<SKIP>(cur);
 #pragma unroll
@@ -809,8 +808,7 @@ struct LatRow {
 template <typename T, int B, int SKIP>
 __global__ __launch_bounds__(256) void k_gs_lat(int nrows, const int4* __restrict__ desc, const int* __restrict__ gcols,
                                                 int W, const T* __restrict__ A, const T* __restrict__ Dinv,
-                                                const T* __restrict__ b, T* x, T omega, const int* __restrict__ color,
-                                                int cur)
+                                                const T* __restrict__ b, T* x, T omega, int cur)
 {
     using M = Map<T, B>;
     using R = RowLds<T, B>;
@@ -823,7 +821,7 @@ __global__ __launch_bounds__(256) void k_gs_lat(int nrows, const int4* __restric
     LatRow<T, B, SKIP> q;
     q.tables(idx, slot < M::RPW && idx < nrows, desc, gcols, W, r);
     q.own(b, x, Dinv, r);
-    q.nbrs(A, x, color, cur, W, r, slot * B);
+    q.nbrs(A, x, cur, W, r, slot * B);
     q.finish(A, x, omega, r, xs_all + (wave * R::RW + slot) * C::XS, tb + (wave * R::RW + slot) * R::TB);
 }
 
@@ -837,7 +835,7 @@ template <typename T, int B, int SKIP>
 __global__ __launch_bounds__(256) void k_gs_pair_lat(const int* __restrict__ wg, int reverse, const int4* __restrict__ desc,
                                                      const int* __restrict__ gcols, int W, const T* __restrict__ A,
                                                      const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
-                                                     const int* __restrict__ color, int cur0, int cur1)
+                                                     int cur0, int cur1)
 {
     using M = Map<T, B>;
     using R = RowLds<T, B>;
@@ -859,10 +857,10 @@ __global__ __launch_bounds__(256) void k_gs_pair_lat(const int* __restrict__ wg,
     q.own(b, x, Dinv, r);
     T* xs = xs_all + (wave * R::RW + slot) * C::XS;
     T* t = tb + (wave * R::RW + slot) * R::TB;
-    p.nbrs(A, x, color, cur0, W, r, slot * B);
+    p.nbrs(A, x, cur0, W, r, slot * B);
     p.finish(A, x, omega, r, xs, t);
     if (m != s && m != e) __syncthreads();   // (uniform: s, m, e are per workgroup)
-    q.nbrs(A, x, color, cur1, W, r, slot * B);
+    q.nbrs(A, x, cur1, W, r, slot * B);
     q.finish(A, x, omega, r, xs, t);
 }
 
@@ -1237,11 +1235,11 @@ void Ops<T>::gs_color(hipStream_t s, int B, int nrows, const int* desc, const in
         if constexpr (lat_block(BB)) {
             if (pool && W <= std::min(BB, kLatW) && (long)g.x * 4 <= 2L * kSimds) {
                 if (skip == 1)
-                    hipLaunchKernelGGL((k_gs_lat<T, BB, 1>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+                    hipLaunchKernelGGL((k_gs_lat<T, BB, 1>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, cur);
                 else if (skip == 2)
-                    hipLaunchKernelGGL((k_gs_lat<T, BB, 2>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+                    hipLaunchKernelGGL((k_gs_lat<T, BB, 2>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, cur);
                 else
-                    hipLaunchKernelGGL((k_gs_lat<T, BB, 0>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, color, cur);
+                    hipLaunchKernelGGL((k_gs_lat<T, BB, 0>), g, dim3(256), 0, s, nrows, d, gcols, W, A, Dinv, b, x, omega, cur);
                 check_launch("gs_color");
                 return;
             }
@@ -1276,13 +1274,13 @@ void Ops<T>::gs_pair(hipStream_t s, int B, int nwg, const int* wg, int reverse, 
             if (pool && W <= std::min(BB, kLatW)) {
                 if (skip == 1)
                     hipLaunchKernelGGL((k_gs_pair_lat<T, BB, 1>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
-                                       omega, color, cur0, cur1);
+                                       omega, cur0, cur1);
                 else if (skip == 2)
                     hipLaunchKernelGGL((k_gs_pair_lat<T, BB, 2>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
-                                       omega, color, cur0, cur1);
+                                       omega, cur0, cur1);
                 else
                     hipLaunchKernelGGL((k_gs_pair_lat<T, BB, 0>), g, dim3(256), 0, s, wg, reverse, d, gcols, W, A, Dinv, b, x,
-                                       omega, color, cur0, cur1);
+                                       omega, cur0, cur1);
                 check_launch("gs_pair");
                 return;
             }
@@ -2272,6 +2270,7 @@ __device__ __forceinline__ void col_partials(int k, int nval, const double* vals
             }                                                                               \
             _Pragma("unroll") for (int u = 0; u < kUnroll; ++u) {                           \
                 const long o = (i0_ + u * gs_) * k + c;                                     \
+                (void)o;                                                                    \
                 if (i0_ + u * gs_ < n) { BODY_USE; }                                        \
             }                                                                               \
         }                                                                                   \

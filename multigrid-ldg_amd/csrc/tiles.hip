@@ -122,8 +122,7 @@ struct TileSlots {
 // column -1 and add zero.  At: the pool in the tile layout (zero padding).
 template <typename T, int B, int SKIP, int WM, bool OWN>
 __device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<WM, OWN ? 1 : 0>& ts, int n, int q,
-                                            int iown, int pown, const T* __restrict__ At, const T* x,
-                                            const int* __restrict__ color, int cur)
+                                            int iown, int pown, const T* __restrict__ At, const T* x)
 {
     constexpr int S = (B + 3) / 4;
     static_assert(B % S == 0, "a lane's K range lies wholly inside or wholly past the block");
@@ -210,8 +209,7 @@ template <typename T, int B, int SKIP, int WM>
 __global__ __launch_bounds__(256) void k_gs_tile(int ntiles, const int* __restrict__ trow, const int* __restrict__ tcol,
                                                  const int* __restrict__ tpid, const int2* __restrict__ tmeta,
                                                  const T* __restrict__ At, const T* __restrict__ Dt,
-                                                 const T* __restrict__ b, T* x, T omega, const int* __restrict__ color,
-                                                 int cur)
+                                                 const T* __restrict__ b, T* x, T omega, int cur)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = uni(xcd_block() * 4 + wave);
@@ -228,7 +226,7 @@ __global__ __launch_bounds__(256) void k_gs_tile(int ntiles, const int* __restri
     if constexpr (SKIP == 0) acc_load<T, B>(xo, x, i0, q);
     tile_ld4<T>(dd, Dt + (size_t)dp * kTileBlock, n, q);
     typename Tm<T>::v4 acc = {0, 0, 0, 0};
-    if (!reads_none<SKIP>(cur)) tile_blocks<T, B, SKIP, WM, false>(acc, ts, n, q, -1, 0, At, x, color, cur);
+    if (!reads_none<SKIP>(cur)) tile_blocks<T, B, SKIP, WM, false>(acc, ts, n, q, -1, 0, At, x);
     // s = b - Y in the accumulator layout is the B operand of the D^-1
     // product: K index row(l, e) at step e (Dt is zero past B)
     typename Tm<T>::v4 y = {0, 0, 0, 0};
@@ -271,7 +269,7 @@ __global__ __launch_bounds__(256) void k_apply_tile(int ntiles, const int* __res
     T dw[4] = {0, 0, 0, 0};
     if (dotv && !dot_self) acc_load<T, B>(dw, dotv, i0, q);
     typename Tm<T>::v4 acc = {0, 0, 0, 0};
-    tile_blocks<T, B, 0, WM, true>(acc, ts, n, q, i, uni(tmeta[tl].y), At, x, nullptr, 0);
+    tile_blocks<T, B, 0, WM, true>(acc, ts, n, q, i, uni(tmeta[tl].y), At, x);
     double dv = 0;
     if (i >= 0) {
         T out[4];
@@ -348,7 +346,8 @@ void tile_pool(hipStream_t s, long np, int B, const double* src, T* dst, int mod
 
 template <typename T>
 void tile_gs(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol, const int* tpid, const int* tmeta, int W,
-             const T* At, const T* Dt, const T* b, T* x, T omega, const int* color, int cur, int skip)
+             const T* At, const T* Dt, const T* b, T* x, T omega, const int* /*color: the tables carry kPidLower*/, int cur,
+             int skip)
 {
     if (ntiles <= 0) return;
     const int2* m = reinterpret_cast<const int2*>(tmeta);
@@ -356,13 +355,13 @@ void tile_gs(hipStream_t s, int B, int ntiles, const int* trow, const int* tcol,
     MGLDG_TILE_DISPATCH(B, MGLDG_TILE_WIDTH(W, {
         if (skip == 1)
             hipLaunchKernelGGL((k_gs_tile<T, BB, 1, WW>), g, dim3(256), 0, s, ntiles, trow, tcol, tpid, m, At, Dt, b, x,
-                               omega, color, cur);
+                               omega, cur);
         else if (skip == 2)
             hipLaunchKernelGGL((k_gs_tile<T, BB, 2, WW>), g, dim3(256), 0, s, ntiles, trow, tcol, tpid, m, At, Dt, b, x,
-                               omega, color, cur);
+                               omega, cur);
         else
             hipLaunchKernelGGL((k_gs_tile<T, BB, 0, WW>), g, dim3(256), 0, s, ntiles, trow, tcol, tpid, m, At, Dt, b, x,
-                               omega, color, cur);
+                               omega, cur);
     }));
     tile_check("gs_tile");
 }
