@@ -120,9 +120,13 @@ struct TileSlots {
 // neighbour, x_j's last S elements for a lane whose K range lies past B) and
 // the values are zeroed afterwards; positions past the tile's blocks have
 // column -1 and add zero.  At: the pool in the tile layout (zero padding).
-template <typename T, int B, int SKIP, int WM, bool OWN>
-__device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<WM, OWN ? 1 : 0>& ts, int n, int q,
-                                            int iown, int pown, const T* __restrict__ At, const T* x)
+// nblk (wave-uniform): the tile's off-diagonal blocks, positions O ..
+// O + nblk - 1 of the table (a tile's rows share their block signature, so
+// the table is -1 past them for every row); the positions past them are
+// neither loaded nor multiplied.
+template <typename T, int B, int SKIP, int WM, bool OWN, bool ALL>
+__device__ __forceinline__ void tile_blocks_n(typename Tm<T>::v4& acc, TileSlots<WM, OWN ? 1 : 0>& ts, int n, int q,
+                                              int iown, int pown, const T* __restrict__ At, const T* x, int nblk)
 {
     constexpr int S = (B + 3) / 4;
     static_assert(B % S == 0, "a lane's K range lies wholly inside or wholly past the block");
@@ -140,9 +144,11 @@ __device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<W
         low[u] = (jj[u] & kPidLower) != 0;
         jj[u] = jj[u] >= 0 ? (jj[u] & kPidMask) : -1;
     }
+    const int nv = ALL ? NS : O + nblk;               // positions in use (uniform)
     T xb[NS][S];
 #pragma unroll
-    for (int u = 0; u < NS; ++u) uld<T, S>(xb[u], x + (size_t)(jj[u] >= 0 ? jj[u] : 0) * B + koff);
+    for (int u = 0; u < NS; ++u)
+        if (ALL || u < nv) uld<T, S>(xb[u], x + (size_t)(jj[u] >= 0 ? jj[u] : 0) * B + koff);
     // The pool blocks of PD positions are in flight together: a position's
     // four MFMAs take far less than a load from cache, so with one block
     // ahead every position waited for its own block (a chain of NS round
@@ -150,19 +156,32 @@ __device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<W
     constexpr int PD = sizeof(T) == 4 ? NS : (NS < 8 ? NS : 8);
     T a[PD][4];
 #pragma unroll
-    for (int u = 0; u < PD; ++u) tile_ld4<T>(a[u], At + (size_t)pp[u] * kTileBlock, n, q);
+    for (int u = 0; u < PD; ++u)
+        if (ALL || u < nv) tile_ld4<T>(a[u], At + (size_t)pp[u] * kTileBlock, n, q);
     // every gather is in flight before the first MFMA waits on one (left to
     // itself the scheduler issues each position's gathers next to its MFMAs)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-        bool use = jj[u] >= 0 && kin;
-        if constexpr (SKIP != 0) if (u >= O) use = use && (SKIP == 1 ? low[u] : !low[u]);
-        T* au = a[u % PD];
+        if (ALL || u < nv) {
+            bool use = jj[u] >= 0 && kin;
+            if constexpr (SKIP != 0) if (u >= O) use = use && (SKIP == 1 ? low[u] : !low[u]);
+            T* au = a[u % PD];
 #pragma unroll
-        for (int e = 0; e < S; ++e) acc = Tm<T>::mac(au[e], use ? xb[u][e] : (T)0, acc);
-        if (u + PD < NS) tile_ld4<T>(au, At + (size_t)pp[u + PD] * kTileBlock, n, q);
+            for (int e = 0; e < S; ++e) acc = Tm<T>::mac(au[e], use ? xb[u][e] : (T)0, acc);
+            if (u + PD < (ALL ? NS : nv)) tile_ld4<T>(au, At + (size_t)pp[u + PD] * kTileBlock, n, q);
+        }
     }
+}
+
+// the table's full width in use (every tile of a uniform grid): no per-
+// position tests at all; otherwise the positions past nblk are skipped
+template <typename T, int B, int SKIP, int WM, bool OWN>
+__device__ __forceinline__ void tile_blocks(typename Tm<T>::v4& acc, TileSlots<WM, OWN ? 1 : 0>& ts, int n, int q,
+                                            int iown, int pown, const T* __restrict__ At, const T* x, int nblk)
+{
+    if (nblk >= WM) tile_blocks_n<T, B, SKIP, WM, OWN, true>(acc, ts, n, q, iown, pown, At, x, nblk);
+    else tile_blocks_n<T, B, SKIP, WM, OWN, false>(acc, ts, n, q, iown, pown, At, x, nblk);
 }
 
 // a lane's four accumulator-layout elements of a row vector v (block row i,
@@ -218,7 +237,8 @@ __global__ __launch_bounds__(256) void k_gs_tile(int ntiles, const int* __restri
     TileSlots<WM, 0> ts;
     ts.load(t, n, tcol, tpid);
     const int i = trow[(size_t)t * 16 + n];
-    const int dp = uni(tmeta[t].y);
+    const int2 meta = tmeta[t];
+    const int dp = uni(meta.y), nblk = uni(meta.x);
     // the row's own operands (b, x, its D^-1 block), in-range addresses
     const int i0 = i >= 0 ? i : 0;
     T bv[4], xo[4], dd[4];
@@ -226,7 +246,7 @@ __global__ __launch_bounds__(256) void k_gs_tile(int ntiles, const int* __restri
     if constexpr (SKIP == 0) acc_load<T, B>(xo, x, i0, q);
     tile_ld4<T>(dd, Dt + (size_t)dp * kTileBlock, n, q);
     typename Tm<T>::v4 acc = {0, 0, 0, 0};
-    if (!reads_none<SKIP>(cur)) tile_blocks<T, B, SKIP, WM, false>(acc, ts, n, q, -1, 0, At, x);
+    if (!reads_none<SKIP>(cur)) tile_blocks<T, B, SKIP, WM, false>(acc, ts, n, q, -1, 0, At, x, nblk);
     // s = b - Y in the accumulator layout is the B operand of the D^-1
     // product: K index row(l, e) at step e (Dt is zero past B)
     typename Tm<T>::v4 y = {0, 0, 0, 0};
@@ -269,7 +289,7 @@ __global__ __launch_bounds__(256) void k_apply_tile(int ntiles, const int* __res
     T dw[4] = {0, 0, 0, 0};
     if (dotv && !dot_self) acc_load<T, B>(dw, dotv, i0, q);
     typename Tm<T>::v4 acc = {0, 0, 0, 0};
-    tile_blocks<T, B, 0, WM, true>(acc, ts, n, q, i, uni(tmeta[tl].y), At, x);
+    tile_blocks<T, B, 0, WM, true>(acc, ts, n, q, i, uni(tmeta[tl].y), At, x, uni(tmeta[tl].x));
     double dv = 0;
     if (i >= 0) {
         T out[4];
