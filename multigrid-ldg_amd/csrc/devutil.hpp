@@ -27,6 +27,14 @@ __device__ __forceinline__ int xcd_block()
     return x * q + min(x, r) + (b >> 3);
 }
 
+// position of the m-th summed block of a row: k0 + m, stepping over the
+// diagonal block at dp (dp < 0: no block is stepped over)
+__device__ __forceinline__ int blk_at(int k0, int m, int dp)
+{
+    const int k = k0 + m;
+    return (dp >= 0 && k >= dp) ? k + 1 : k;
+}
+
 // Gauss-Seidel launches of the first sweep from x = 0 read only neighbours
 // already written in sweep order: SKIP 1 (forward) those of colour <= cur,
 // SKIP 2 (reverse) those of colour >= cur.  The sweep's first launch reads

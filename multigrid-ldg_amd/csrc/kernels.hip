@@ -321,14 +321,6 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// position of the m-th summed block of a row: k0 + m, stepping over the
-// diagonal block at dp (dp < 0: no block is stepped over)
-__device__ __forceinline__ int blk_at(int k0, int m, int dp)
-{
-    const int k = k0 + m;
-    return (dp >= 0 && k >= dp) ? k + 1 : k;
-}
-
 // Lane r of a block row returns sum_m A_k(m)[r][:] . x_col(k(m)) over the
 // row's blocks k0 .. k1-1 other than dp.  Latency is what bounds these
 // launches (one colour of a sweep is a few MB to a few hundred MB), so:

@@ -169,16 +169,20 @@ bool mf_supported(int B);
 int mf_max_tiles(int B);
 template <typename T>
 struct MF {
-    // pidx / didx (pooled operator): per block slot its pool index in A, per
-    // row its D^-1 pool index in Dinv; null: A and Dinv hold every block
-    static void apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
-                      const T* x, const T* b, T* y, const int* pidx = nullptr);
-    static void gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* rows, const int* rowptr,
-                         const int* cols, const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color,
-                         int cur, int skip, const int* pidx = nullptr, const int* didx = nullptr);
-    static void jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
-                       const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* pidx = nullptr,
-                       const int* didx = nullptr);
+    // The single-RHS kernels' row tables: ell (every block of a row at
+    // stride W, -1 past the last) with rowptr for the operator and Jacobi;
+    // per GS launch the colour's descriptors {row, rowptr[row],
+    // rowptr[row + 1], diagonal position} and off-diagonal columns at stride
+    // gw.  pool (pooled operator): the tables hold (column, pool index) pairs
+    // into the pool A (GS: with kPidLower), descriptors and didx the D^-1
+    // pool index into Dinv; otherwise A and Dinv hold every block.
+    static void apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W,
+                      const T* A, const T* x, const T* b, T* y, bool pool);
+    static void gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* desc, const int* gcols, int gw,
+                         const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip,
+                         bool pool);
+    static void jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W,
+                       const T* A, const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* didx);
 };
 // Y = W X on the matrix cores (FP64 16x16x4) for k columns: W in the 16 x 4
 // tile layout made by gemm_tiles (gemm_tiles_size(n) doubles), the K range

@@ -177,52 +177,97 @@ __device__ __forceinline__ void blk_mac(typename Mf<T>::v4* acc, const BlkOps<T,
             for (int t = 0; t < NT; ++t) acc[rt * NT + t] = Mf<T>::mac(o.a[s][rt], o.xb[s][t], acc[rt * NT + t]);
 }
 
-// acc += sum over the blocks k0 .. k1-1 of row i (one row per wavefront, so
-// every branch below is uniform): the row's column indices are loaded once,
-// one per lane, and read out with readlane into scalar registers; the
-// operands of U blocks are in flight together.  SKIP: -1 = every block; 0 =
-// not the diagonal block; 1 (2) = also not the colours after (before) `cur`
-// (first GS sweep from x = 0).  pidx (pooled operator): per block slot its
-// index in the pool A; null: A holds every block.
-template <typename T, int B, int NT, int SKIP>
-__device__ __forceinline__ void row_mac(typename Mf<T>::v4* acc, int i, int k0, int k1, const int* __restrict__ cols,
-                                        const T* __restrict__ A, const T* __restrict__ x, int k, int col0, int lane,
-                                        const int* __restrict__ color, int cur, const int* __restrict__ pidx)
+// A row's block positions, one per lane: lane m holds position m0 + m's
+// column j (-1: none) and its block's index in A (pooled operator: the pool
+// index, on Gauss-Seidel tables with kPidLower when j's colour precedes the
+// row's).  The tables are read in the same memory round trip as the row's own
+// operands, so the neighbour operands follow one round trip later.
+struct RowTab {
+    int col, blk;
+};
+
+// lane m's entry, position m0 + m of nblk: Gauss-Seidel tables (d = {row,
+// rowptr[row], rowptr[row + 1], diagonal position or, pooled, D^-1 pool
+// index}; gr: the row's off-diagonal columns, pairs (column, pool index) when
+// pooled) ...
+template <bool POOL>
+__device__ __forceinline__ RowTab gs_tab(const int* __restrict__ gr, int m0, int lane, int nblk, int4 d)
+{
+    RowTab e{-1, 0};
+    const int m = m0 + lane;
+    if (m < nblk) {
+        if (POOL) {
+            const int2 pe = reinterpret_cast<const int2*>(gr)[m];
+            e.col = pe.x; e.blk = pe.y;
+        } else {
+            e.col = gr[m];
+            e.blk = blk_at(d.y, m, d.w);
+        }
+    }
+    return e;
+}
+// ... and the operator's ELL row er (every block of the row in pattern
+// order; pairs when pooled), k0 = rowptr[row]
+template <bool POOL>
+__device__ __forceinline__ RowTab ell_tab(const int* __restrict__ er, int m0, int lane, int nblk, int k0)
+{
+    RowTab e{-1, 0};
+    const int m = m0 + lane;
+    if (m < nblk) {
+        if (POOL) {
+            const int2 pe = reinterpret_cast<const int2*>(er)[m];
+            e.col = pe.x; e.blk = pe.y;
+        } else {
+            e.col = er[m];
+            e.blk = k0 + m;
+        }
+    }
+    return e;
+}
+
+// acc += sum over the nc positions held by the lanes (one row per wavefront,
+// so every branch below is uniform): columns and block indices are read out
+// with readlane into scalar registers; the operands of U blocks are in flight
+// together.  SKIP: -1 / 0 = every position held; 1 (2) = only neighbours of
+// a colour before (after) the current one (first GS sweep from x = 0), from
+// the pooled tables' kPidLower bit or, unpooled, from color[j].
+template <typename T, int B, int NT, int SKIP, bool POOL>
+__device__ __forceinline__ void tab_mac(typename Mf<T>::v4* acc, RowTab e, int nc, const T* __restrict__ A,
+                                        const T* __restrict__ x, int k, const LaneOff<T, B, NT>& lo,
+                                        const int* __restrict__ color, int cur)
 {
     constexpr int U = mf_unroll<T, B, NT>();
-    const LaneOff<T, B, NT> lo(lane, k, col0);
-    const int deg = k1 - k0;
-    for (int m0 = 0; m0 < deg; m0 += 64) {
-        const int mycol = (m0 + lane < deg) ? cols[k0 + m0 + lane] : -1;
-        const int mypid = (pidx && m0 + lane < deg) ? pidx[k0 + m0 + lane] : 0;
-        const int nc = min(64, deg - m0);
-        for (int u0 = 0; u0 < nc; u0 += U) {
-            BlkOps<T, B, NT> ops[U];
-            bool use[U];
+    for (int u0 = 0; u0 < nc; u0 += U) {
+        BlkOps<T, B, NT> ops[U];
+        bool use[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int m = min(u0 + u, 63);
-                const int j = __builtin_amdgcn_readlane(mycol, m);
-                const int blk = pidx ? __builtin_amdgcn_readlane(mypid, m) : k0 + m0 + m;
-                bool ok = u0 + u < nc && j >= 0;
-                if (SKIP >= 0 && ok) ok = j != i;
-                if (SKIP > 0 && ok) { const int cj = color[j]; ok = SKIP == 1 ? cj <= cur : cj >= cur; }
-                use[u] = ok;
-                if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)blk * (B * B), x + (size_t)j * B * k, k, lo);
+        for (int u = 0; u < U; ++u) {
+            const int m = min(u0 + u, 63);
+            const int j = __builtin_amdgcn_readlane(e.col, m);
+            int blk = __builtin_amdgcn_readlane(e.blk, m);
+            bool ok = u0 + u < nc && j >= 0;
+            if (POOL) {
+                if (SKIP > 0) { const bool lower = (blk & kPidLower) != 0; ok = ok && (SKIP == 1 ? lower : !lower); }
+                blk &= kPidMask;
+            } else if (SKIP > 0 && ok) {
+                const int cj = color[j];
+                ok = SKIP == 1 ? cj <= cur : cj >= cur;
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (use[u]) blk_mac<T, B, NT>(acc, ops[u]);
+            use[u] = ok;
+            if (ok) blk_load<T, B, NT>(ops[u], A + (size_t)blk * (B * B), x + (size_t)j * B * k, k, lo);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (use[u]) blk_mac<T, B, NT>(acc, ops[u]);
     }
 }
 
-// y = A x (b null) or y = b - A x
-template <typename T, int B, int NT>
+// y = A x (b null) or y = b - A x; the row's ELL entries, rowptr and b_i in
+// one round trip, then the blocks
+template <typename T, int B, int NT, bool POOL>
 __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
-                                                const int* __restrict__ cols, const T* __restrict__ A,
-                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
-                                                const int* __restrict__ pidx)
+                                                const int* __restrict__ ell, int W, const T* __restrict__ A,
+                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y)
 {
     using V4 = typename Mf<T>::v4;
     const int lane = threadIdx.x & 63;
@@ -231,35 +276,42 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
     if (i >= nrows) return;
     constexpr int RT = (B + 15) / 16;
     const int col0 = g * 16 * NT;
+    const int k0 = rowptr[i], nblk = rowptr[i + 1] - k0;
+    const int* er = ell + (size_t)i * W * (POOL ? 2 : 1);
+    RowTab e = ell_tab<POOL>(er, 0, lane, nblk, k0);
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int gc = col0 + NT * (lane & 15);   // this lane's NT output columns (k is a multiple of NT)
-    // b_i of the residual form, in flight with the neighbour sum
+    // b_i of the residual form, in flight with the tables
     T bt[RT][4][NT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = 16 * rt + Mf<T>::row(lane, e);
-            if (b && r < B && gc < k) mf_ld<T, NT>(bt[rt][e], b + ((size_t)i * B + r) * k + gc);
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * rt + Mf<T>::row(lane, q);
+            if (b && r < B && gc < k) mf_ld<T, NT>(bt[rt][q], b + ((size_t)i * B + r) * k + gc);
             else {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) bt[rt][e][t] = (T)0;
+                for (int t = 0; t < NT; ++t) bt[rt][q][t] = (T)0;
             }
         }
-    row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
+    const LaneOff<T, B, NT> lo(lane, k, col0);
+    for (int m0 = 0; m0 < nblk; m0 += 64) {
+        if (m0 > 0) e = ell_tab<POOL>(er, m0, lane, nblk, k0);
+        tab_mac<T, B, NT, -1, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, nullptr, 0);
+    }
     if (gc >= k) return;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = 16 * rt + Mf<T>::row(lane, e);
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * rt + Mf<T>::row(lane, q);
             if (r >= B) continue;
             const size_t o = ((size_t)i * B + r) * k + gc;
             T v[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) v[t] = b ? (T)(bt[rt][e][t] - acc[rt * NT + t][e]) : acc[rt * NT + t][e];
+            for (int t = 0; t < NT; ++t) v[t] = b ? (T)(bt[rt][q][t] - acc[rt * NT + t][q]) : acc[rt * NT + t][q];
             mf_st<T, NT>(y + o, v);
         }
 }
@@ -357,13 +409,14 @@ __device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typena
 
 // damped block GS on the rows of one colour (multigrid.h:240-249):
 // x_i += omega (D_i^-1 (b_i - sum_{j != i} A_ij x_j) - x_i); skip as in the
-// vector kernel (first sweep from zero: later colours are not read)
-template <typename T, int B, int NT, int SKIP>
-__global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, const int* __restrict__ rows,
-                                             const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                             const T* __restrict__ A, const T* __restrict__ Dinv, const T* __restrict__ b,
-                                             T* x, T omega, const int* __restrict__ color, int cur,
-                                             const int* __restrict__ pidx, const int* __restrict__ didx)
+// vector kernel (first sweep from zero: later colours are not read).  Wave
+// (ri, g) reads the colour's descriptor ri and column table, and b_i, x_i in
+// one round trip, D^-1 and the neighbour blocks in the next.
+template <typename T, int B, int NT, int SKIP, bool POOL>
+__global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, const int4* __restrict__ desc,
+                                             const int* __restrict__ gcols, int gw, const T* __restrict__ A,
+                                             const T* __restrict__ Dinv, const T* __restrict__ b, T* x, T omega,
+                                             const int* __restrict__ color, int cur)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
@@ -373,15 +426,25 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + wv);   // wave-uniform
     const int ri = w / ngroups, g = w - ri * ngroups;
     const bool valid = ri < nrows;          // every wave reaches the barrier in dinv_times
-    const int i = valid ? rows[ri] : 0;
+    const int4 d = valid ? desc[ri] : make_int4(0, 0, 0, 0);
+    const int i = d.x;
+    const bool reads = valid && !reads_none<SKIP>(cur);
+    // off-diagonal blocks of the row (every row has its diagonal block)
+    const int nblk = reads ? d.z - d.y - (POOL || d.w >= 0 ? 1 : 0) : 0;
+    const int* gr = gcols + (size_t)ri * gw * (POOL ? 2 : 1);
+    RowTab e = gs_tab<POOL>(gr, 0, lane, nblk, d);
     const int col0 = g * 16 * NT, col_l = lane & 15;
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
     RowTile<T, B, NT> rt;
-    row_tile_load<T, B, NT, SKIP == 0>(rt, b, x, Dinv + (size_t)(didx ? didx[i] : i) * B * B, i, k, gc, valid, lane);
-    if (valid && !reads_none<SKIP>(cur)) row_mac<T, B, NT, SKIP>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, color, cur, pidx);
+    row_tile_load<T, B, NT, SKIP == 0>(rt, b, x, Dinv + (size_t)(POOL ? d.w : i) * B * B, i, k, gc, valid, lane);
+    const LaneOff<T, B, NT> lo(lane, k, col0);
+    for (int m0 = 0; m0 < nblk; m0 += 64) {
+        if (m0 > 0) e = gs_tab<POOL>(gr, m0, lane, nblk, d);
+        tab_mac<T, B, NT, SKIP, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, color, cur);
+    }
     V4 res[RT * NT];
     residual_tile<T, B, NT>(res, acc, rt, k, gc, valid, lane);
     V4 out[RT * NT];
@@ -390,24 +453,26 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = 16 * rr + Mf<T>::row(lane, e);
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * rr + Mf<T>::row(lane, q);
             if (r >= B) continue;
             const size_t o = ((size_t)i * B + r) * k + gc;
             T nv[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) nv[t] = rt.xt[rr][e][t] + omega * (out[rr * NT + t][e] - rt.xt[rr][e][t]);   // (SKIP: x_i = 0)
+            for (int t = 0; t < NT; ++t) nv[t] = rt.xt[rr][q][t] + omega * (out[rr * NT + t][q] - rt.xt[rr][q][t]);   // (SKIP: x_i = 0)
             mf_st<T, NT>(x + o, nv);
         }
 }
 
-// damped block Jacobi (multigrid.h:223-229): xout = x + omega D^-1 (b - A x)
-template <typename T, int B, int NT>
+// damped block Jacobi (multigrid.h:223-229): xout = x + omega D^-1 (b - A x);
+// the row's ELL entries, rowptr, D^-1 pool index, b_i and x_i in one round
+// trip, D^-1 and the blocks in the next
+template <typename T, int B, int NT, bool POOL>
 __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
-                                                 const int* __restrict__ cols, const T* __restrict__ A,
+                                                 const int* __restrict__ ell, int W, const T* __restrict__ A,
                                                  const T* __restrict__ Dinv, const T* __restrict__ b,
                                                  const T* __restrict__ x, T* __restrict__ xout, T omega,
-                                                 const int* __restrict__ pidx, const int* __restrict__ didx)
+                                                 const int* __restrict__ didx)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
@@ -418,14 +483,21 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     const int i0 = w / ngroups, g = w - i0 * ngroups;
     const bool valid = i0 < nrows;
     const int i = valid ? i0 : 0;
+    const int k0 = valid ? rowptr[i] : 0, nblk = valid ? rowptr[i + 1] - k0 : 0;
+    const int* er = ell + (size_t)i * W * (POOL ? 2 : 1);
+    RowTab e = ell_tab<POOL>(er, 0, lane, nblk, k0);
     const int col0 = g * 16 * NT, col_l = lane & 15;
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
     RowTile<T, B, NT> rt;
-    row_tile_load<T, B, NT, true>(rt, b, x, Dinv + (size_t)(didx ? didx[i] : i) * B * B, i, k, gc, valid, lane);
-    if (valid) row_mac<T, B, NT, -1>(acc, i, rowptr[i], rowptr[i + 1], cols, A, x, k, col0, lane, nullptr, 0, pidx);
+    row_tile_load<T, B, NT, true>(rt, b, x, Dinv + (size_t)(POOL ? didx[i] : i) * B * B, i, k, gc, valid, lane);
+    const LaneOff<T, B, NT> lo(lane, k, col0);
+    for (int m0 = 0; m0 < nblk; m0 += 64) {
+        if (m0 > 0) e = ell_tab<POOL>(er, m0, lane, nblk, k0);
+        tab_mac<T, B, NT, -1, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, nullptr, 0);
+    }
     V4 res[RT * NT];
     residual_tile<T, B, NT>(res, acc, rt, k, gc, valid, lane);
     V4 out[RT * NT];
@@ -434,13 +506,13 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = 16 * rr + Mf<T>::row(lane, e);
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * rr + Mf<T>::row(lane, q);
             if (r >= B) continue;
             const size_t o = ((size_t)i * B + r) * k + gc;
             T nv[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) nv[t] = rt.xt[rr][e][t] + omega * out[rr * NT + t][e];
+            for (int t = 0; t < NT; ++t) nv[t] = rt.xt[rr][q][t] + omega * out[rr * NT + t][q];
             mf_st<T, NT>(xout + o, nv);
         }
 }
@@ -575,49 +647,59 @@ static int fit_tiles(int nt, int k)
 }
 int mf_max_tiles(int B) { return B > 16 ? 2 : 4; }
 
-template <typename T>
-void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A, const T* x,
-                  const T* b, T* y, const int* pidx)
-{
-    if (nrows <= 0 || k <= 0) return;
-    nt = fit_tiles(nt, k);
-    const int ng = (k + 16 * nt - 1) / (16 * nt);
-    MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
-                                                nrows, ng, rowptr, cols, A, x, b, y, pidx));
-    check("mf_apply");
-}
+// POOL as a compile-time constant PP
+#define MGLDG_MF_POOL(pool, ...)                        \
+    do {                                                \
+        if (pool) { constexpr bool PP = true; __VA_ARGS__; } \
+        else { constexpr bool PP = false; __VA_ARGS__; }     \
+    } while (0)
 
 template <typename T>
-void MF<T>::gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* rows, const int* rowptr, const int* cols,
-                     const T* A, const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip,
-                     const int* pidx, const int* didx)
+void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W, const T* A,
+                  const T* x, const T* b, T* y, bool pool)
 {
     if (nrows <= 0 || k <= 0) return;
     nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     const dim3 g(nblocks((long)nrows * ng));
+    MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT, PP>), g, dim3(256), 0, s, k, nrows,
+                                                                    ng, rowptr, ell, W, A, x, b, y)));
+    check("mf_apply");
+}
+
+template <typename T>
+void MF<T>::gs_color(hipStream_t s, int B, int nt, int k, int nrows, const int* desc, const int* gcols, int gw, const T* A,
+                     const T* Dinv, const T* b, T* x, T omega, const int* color, int cur, int skip, bool pool)
+{
+    if (nrows <= 0 || k <= 0) return;
+    nt = fit_tiles(nt, k);
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    const dim3 g(nblocks((long)nrows * ng));
+    const int4* d4 = reinterpret_cast<const int4*>(desc);
     if (skip == 1) {
-        MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 1>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
-                                                    A, Dinv, b, x, omega, color, cur, pidx, didx));
+        MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 1, PP>), g, dim3(256), 0, s, k, nrows, ng,
+                                                                        d4, gcols, gw, A, Dinv, b, x, omega, color, cur)));
     } else if (skip == 2) {
-        MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 2>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
-                                                    A, Dinv, b, x, omega, color, cur, pidx, didx));
+        MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 2, PP>), g, dim3(256), 0, s, k, nrows, ng,
+                                                                        d4, gcols, gw, A, Dinv, b, x, omega, color, cur)));
     } else {
-        MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 0>), g, dim3(256), 0, s, k, nrows, ng, rows, rowptr, cols,
-                                                    A, Dinv, b, x, omega, color, cur, pidx, didx));
+        MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_gs<T, BB, TT, 0, PP>), g, dim3(256), 0, s, k, nrows, ng,
+                                                                        d4, gcols, gw, A, Dinv, b, x, omega, color, cur)));
     }
     check("mf_gs");
 }
 
 template <typename T>
-void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* cols, const T* A,
-                   const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* pidx, const int* didx)
+void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W, const T* A,
+                   const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* didx)
 {
     if (nrows <= 0 || k <= 0) return;
     nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
-    MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_jacobi<T, BB, TT>), dim3(nblocks((long)nrows * ng)), dim3(256), 0, s, k,
-                                                nrows, ng, rowptr, cols, A, Dinv, b, x, xout, omega, pidx, didx));
+    const dim3 g(nblocks((long)nrows * ng));
+    MGLDG_MF_POOL(didx != nullptr, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_jacobi<T, BB, TT, PP>), g, dim3(256), 0, s, k,
+                                                                               nrows, ng, rowptr, ell, W, A, Dinv, b, x, xout,
+                                                                               omega, didx)));
     check("mf_jacobi");
 }
 

@@ -825,7 +825,6 @@ static void finalize_level(hipStream_t s, Level& lv, int precision, bool lexicog
         }
         lv.pell.upload(pe.data(), pe.size(), s);
         lv.pdidx.upload(didx.data(), didx.size(), s);
-        lv.ppidx.upload(pidx.data(), pidx.size(), s);
         HIPCHK(hipStreamSynchronize(s));
         lv.npool = (int)prep.size(); lv.ndpool = (int)drep.size();
         lv.hpidx = pidx; lv.hdidx = didx;
@@ -1092,7 +1091,6 @@ template <> const float* TDop<float>(const Level& lv) { return lv.TD32.p; }
 static const int* Sell(const Level& lv) { return lv.npool ? lv.pell.p : lv.A.ell.p; }
 static const int* Sdidx(const Level& lv) { return lv.npool ? lv.pdidx.p : nullptr; }
 static int Spool(const Level& lv) { return lv.npool ? 1 : 0; }
-static const int* Spidx(const Level& lv) { return lv.npool ? lv.ppidx.p : nullptr; }
 static size_t Sgw(const Level& lv) { return (size_t)lv.gw * (lv.npool ? 2 : 1); }   // ints per row of a GS column table
 template <typename T> static const T* Pool(const Level& lv);
 template <> const double* Pool<double>(const Level& lv) { return lv.pool64.p; }
@@ -1590,7 +1588,7 @@ void MGH::bapply(int l, int k, const T* x, const T* b, T* y)
 {
     Level& lv = L[l];
     const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].apply : 0;
-    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Mop<T>(lv), x, b, y, Spidx(lv));
+    if (nt) dev::MF<T>::apply(ctx->stream, lv.B, nt, k, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Mop<T>(lv), x, b, y, lv.npool > 0);
     else dev::MOps<T>::apply(ctx->stream, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), x, b, y);
 }
 
@@ -1671,9 +1669,9 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     const int nt = (int)btune[sizeof(T) == 4].size() > l ? btune[sizeof(T) == 4][l].smooth : 0;
     if (prm.relaxation >= 1) {
         for (int sw = 0; sw < n; ++sw)
-            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int*, const int*, int nr, const int* cdev, int c, int skip) {
-                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, rows, lv.A.rowptr, lv.A.cols, Mop<T>(lv), MDop<T>(lv), bb<T>(v),
-                                             bx<T>(v), omega, cdev, c, skip, Spidx(lv), Sdidx(lv));
+            sweep(lv, reverse, zero_guess && sw == 0, [&](const int* rows, const int* dsc, const int* gcl, int nr, const int* cdev, int c, int skip) {
+                if (nt) dev::MF<T>::gs_color(s, lv.B, nt, k, nr, dsc, gcl, lv.gw, Mop<T>(lv), MDop<T>(lv), bb<T>(v), bx<T>(v), omega,
+                                             cdev, c, skip, lv.npool > 0);
                 else dev::MOps<T>::gs_color(s, lv.B, k, nr, rows, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v),
                                             bx<T>(v), omega, cdev, c, skip);
             });
@@ -1686,8 +1684,8 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
             else { dev::MOps<T>::jacobi0(s, lv.B, k, lv.nb, Dop<T>(lv), bb<T>(v), nxt, omega); std::swap(cur, nxt); sw0 = 1; }
         }
         for (int sw = sw0; sw < n; ++sw) {
-            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, lv.A.cols, Mop<T>(lv), MDop<T>(lv), bb<T>(v), cur, nxt,
-                                       omega, Spidx(lv), Sdidx(lv));
+            if (nt) dev::MF<T>::jacobi(s, lv.B, nt, k, lv.nb, lv.A.rowptr, Sell(lv), lv.A.ellw, Mop<T>(lv), MDop<T>(lv), bb<T>(v),
+                                       cur, nxt, omega, Sdidx(lv));
             else dev::MOps<T>::jacobi(s, lv.B, k, lv.nb, lv.A.rowptr, lv.A.cols, Aop<T>(lv), Dop<T>(lv), bb<T>(v), cur, nxt, omega);
             std::swap(cur, nxt);
         }

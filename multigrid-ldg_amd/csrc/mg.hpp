@@ -133,7 +133,7 @@ struct Level {
     // per workgroup {s, m, e} into pair_desc / pair_gcols, see dev::Ops::gs_pair
     int pair_nwg = 0;
     DBuf<int> pair_wg, pair_desc, pair_gcols;
-    // pooled operator of the single-RHS kernels (npool > 0): the distinct
+    // pooled operator of the single-RHS and batched kernels (npool > 0): the distinct
     // blocks of A and of D^-1 (quadtree operators repeat a few hundred
     // blocks), panel layouts; pell: A.ell as (column, pool index) pairs at
     // stride 2 * A.ellw; pdidx: each row's D^-1 pool index.  The GS tables
@@ -146,7 +146,6 @@ struct Level {
     DBuf<double> R64, DR64;
     DBuf<float> R32, DR32;
     DBuf<int> pell, pdidx;
-    DBuf<int> ppidx;                    // per block slot its pool index (batched MFMA kernels)
     std::vector<int> hpidx, hdidx;      // host copies of the pool indices (traffic accounting)
     // row tiles of a pooled level (dev::tile_gs / tile_apply): rows with the
     // same block signature in tiles of 16, per colour for GS (tile_off[c] ..
