@@ -2316,6 +2316,12 @@ static unsigned red_grid(long n, int k)
     return (unsigned)std::min<long>(kRedBlocks, std::max<long>(1, (n + groups - 1) / groups));
 }
 
+void colfinalize(hipStream_t s, const double* part, int nb, int k, double* out, int ostride)
+{
+    hipLaunchKernelGGL(k_colfinalize, dim3(k), dim3(256), 0, s, part, nb, k, 1, out, ostride, nullptr, nullptr, 0);
+    check_launch("colfinalize");
+}
+
 void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part)
 {
     if (k > 256) throw std::runtime_error("coldot: more than 256 columns");

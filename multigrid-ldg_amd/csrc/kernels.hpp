@@ -239,6 +239,15 @@ void pcg_d(hipStream_t s, long n, double* S, const T* zs, double* d, bool init, 
 // Per-column scalars live at S[c * S_COUNT + slot]; active[c] = 0 freezes column c.
 // per-column a.b into out[c * ostride] (deterministic two-pass)
 void coldot(hipStream_t s, long n, int k, const double* a, const double* b, double* out, int ostride, double* part);
+// out[c * ostride] = sum over b < nb of part[b * k + c] (fixed order)
+void colfinalize(hipStream_t s, const double* part, int nb, int k, double* out, int ostride);
+// y = A x on the matrix cores (as MF<double>::apply with nt tiles, b null)
+// with the per-column x . y into out[c * ostride]: the application's
+// workgroups leave their partial sums in part (apply_dot_part_size doubles),
+// then colfinalize
+long apply_dot_part_size(int nt, int k, int nrows);
+void apply_dot(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W, const double* A,
+               const double* x, double* y, bool pool, double* part, double* out, int ostride);
 // a[:, c] -= mean(a[:, c]) for each of the k columns (deterministic); part
 // holds mean_zero_part_size(n, k) doubles
 // *out = sum of part[0 .. n) (fixed order)

@@ -263,34 +263,41 @@ __device__ __forceinline__ void tab_mac(typename Mf<T>::v4* acc, RowTab e, int n
 }
 
 // y = A x (b null) or y = b - A x; the row's ELL entries, rowptr and b_i in
-// one round trip, then the blocks
-template <typename T, int B, int NT, bool POOL>
+// one round trip, then the blocks.  DOT (y = A x only): also the per-column
+// sums of x . y over the workgroup's rows into part[blockIdx.x][0 .. k) (x_i
+// loaded in the accumulator layout beside the tables; lanes, then the four
+// wavefronts, summed in a fixed order)
+template <typename T, int B, int NT, bool POOL, bool DOT>
 __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, const int* __restrict__ rowptr,
                                                 const int* __restrict__ ell, int W, const T* __restrict__ A,
-                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y)
+                                                const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
+                                                double* __restrict__ part)
 {
     using V4 = typename Mf<T>::v4;
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));   // wave-uniform
-    const int i = w / ngroups, g = w - i * ngroups;
-    if (i >= nrows) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + wv);   // wave-uniform
+    const int i0 = w / ngroups, g = w - i0 * ngroups;
+    const bool valid = i0 < nrows;
+    if (!DOT && !valid) return;
+    const int i = valid ? i0 : 0;
     constexpr int RT = (B + 15) / 16;
     const int col0 = g * 16 * NT;
-    const int k0 = rowptr[i], nblk = rowptr[i + 1] - k0;
+    const int k0 = valid ? rowptr[i] : 0, nblk = valid ? rowptr[i + 1] - k0 : 0;
     const int* er = ell + (size_t)i * W * (POOL ? 2 : 1);
     RowTab e = ell_tab<POOL>(er, 0, lane, nblk, k0);
     V4 acc[RT * NT];
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int gc = col0 + NT * (lane & 15);   // this lane's NT output columns (k is a multiple of NT)
-    // b_i of the residual form, in flight with the tables
+    // b_i of the residual form (DOT: x_i), in flight with the tables
+    const T* own = DOT ? x : b;
     T bt[RT][4][NT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int r = 16 * rt + Mf<T>::row(lane, q);
-            if (b && r < B && gc < k) mf_ld<T, NT>(bt[rt][q], b + ((size_t)i * B + r) * k + gc);
+            if (own && valid && r < B && gc < k) mf_ld<T, NT>(bt[rt][q], own + ((size_t)i * B + r) * k + gc);
             else {
 #pragma unroll
                 for (int t = 0; t < NT; ++t) bt[rt][q][t] = (T)0;
@@ -301,19 +308,54 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
         if (m0 > 0) e = ell_tab<POOL>(er, m0, lane, nblk, k0);
         tab_mac<T, B, NT, -1, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, nullptr, 0);
     }
-    if (gc >= k) return;
+    if (valid && gc < k) {
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = 16 * rt + Mf<T>::row(lane, q);
-            if (r >= B) continue;
-            const size_t o = ((size_t)i * B + r) * k + gc;
-            T v[NT];
+            for (int q = 0; q < 4; ++q) {
+                const int r = 16 * rt + Mf<T>::row(lane, q);
+                if (r >= B) continue;
+                const size_t o = ((size_t)i * B + r) * k + gc;
+                T v[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) v[t] = b ? (T)(bt[rt][q][t] - acc[rt * NT + t][q]) : acc[rt * NT + t][q];
-            mf_st<T, NT>(y + o, v);
+                for (int t = 0; t < NT; ++t) v[t] = (!DOT && b) ? (T)(bt[rt][q][t] - acc[rt * NT + t][q]) : acc[rt * NT + t][q];
+                mf_st<T, NT>(y + o, v);
+            }
+    }
+    if constexpr (DOT) {
+        __shared__ double red[4][64];
+        double sdot[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) sdot[t] = 0;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (16 * rt + Mf<T>::row(lane, q) < B) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) sdot[t] += (double)bt[rt][q][t] * (double)acc[rt * NT + t][q];
+                }
+        // the four lanes n, n + 16, n + 32, n + 48 hold the same columns
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            sdot[t] += __shfl_xor(sdot[t], 16, 64);
+            sdot[t] += __shfl_xor(sdot[t], 32, 64);
         }
+        if (lane < 16) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) red[wv][NT * lane + t] = sdot[t];
+        }
+        __syncthreads();
+        const int c = threadIdx.x;
+        if (c < k) {
+            const int cg = c / (16 * NT), cl = c - cg * 16 * NT;
+            const int wb = w - wv;   // the workgroup's first wave
+            double tot = 0;
+            for (int v = 0; v < 4; ++v)
+                if ((wb + v) % ngroups == cg) tot += red[v][cl];
+            part[(size_t)blockIdx.x * k + c] = tot;
+        }
+    }
 }
 
 // The operands of a smoother's row that do not depend on its neighbour sum:
@@ -662,9 +704,30 @@ void MF<T>::apply(hipStream_t s, int B, int nt, int k, int nrows, const int* row
     nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
     const dim3 g(nblocks((long)nrows * ng));
-    MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT, PP>), g, dim3(256), 0, s, k, nrows,
-                                                                    ng, rowptr, ell, W, A, x, b, y)));
+    MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<T, BB, TT, PP, false>), g, dim3(256), 0, s, k,
+                                                                    nrows, ng, rowptr, ell, W, A, x, b, y, nullptr)));
     check("mf_apply");
+}
+
+long apply_dot_part_size(int nt, int k, int nrows)
+{
+    nt = fit_tiles(nt, k);
+    const long ng = (k + 16 * nt - 1) / (16 * nt);
+    return (long)nblocks((long)nrows * ng) * k;
+}
+
+void apply_dot(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W, const double* A,
+               const double* x, double* y, bool pool, double* part, double* out, int ostride)
+{
+    if (nrows <= 0 || k <= 0) return;
+    if (k > 256) throw std::runtime_error("apply_dot: more than 256 columns");
+    nt = fit_tiles(nt, k);
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    const unsigned nb = nblocks((long)nrows * ng);
+    MGLDG_MF_POOL(pool, MGLDG_MF_DISPATCH(B, nt, hipLaunchKernelGGL((mf_apply<double, BB, TT, PP, true>), dim3(nb), dim3(256), 0, s,
+                                                                    k, nrows, ng, rowptr, ell, W, A, x, nullptr, y, part)));
+    check("mf_apply (dot)");
+    colfinalize(s, part, (int)nb, k, out, ostride);
 }
 
 template <typename T>

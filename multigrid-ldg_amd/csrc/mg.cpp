@@ -1581,6 +1581,10 @@ void MGH::ensure_batch(int k, int precision)
     if (cut >= 0) Bgemm.alloc((size_t)dev::gemm_part_size(L[cut].nb * L[cut].B, k));
     bk = k; bprec = precision;
     tune_batch(k, precision);
+    // the CG product on MFMA tiles sums d.q in its own workgroups
+    const int nt0 = btune[0].empty() ? 0 : btune[0][0].apply;
+    if (nt0) Bdq.alloc((size_t)dev::apply_dot_part_size(nt0, k, L[0].nb));
+    else Bdq.release();
 }
 
 template <typename T>
@@ -1842,8 +1846,16 @@ std::vector<SolveStats> MGH::solve_batched(const double* b, double* x, int k, co
         }
         if (!any) break;
         if (i >= sp.maxit) { for (int c = 0; c < k; ++c) if (act[c]) st[c].status = 1; break; }
-        bapply<double>(0, k, Bd, nullptr, Bq);
-        dev::coldot(s, n, k, Bd, Bq, BS.p + dev::S_DQ, NS, Bpart);
+        // q = A d and d.q
+        const Level& l0 = L[0];
+        const int nt0 = btune[0].empty() ? 0 : btune[0][0].apply;
+        if (nt0 && Bdq.p) {
+            dev::apply_dot(s, l0.B, nt0, k, l0.nb, l0.A.rowptr, Sell(l0), l0.A.ellw, Mop<double>(l0), Bd, Bq, l0.npool > 0, Bdq,
+                           BS.p + dev::S_DQ, NS);
+        } else {
+            bapply<double>(0, k, Bd, nullptr, Bq);
+            dev::coldot(s, n, k, Bd, Bq, BS.p + dev::S_DQ, NS, Bpart);
+        }
         if (i % 50 == 0) {   // periodic true residual (krylov.h:47-52)
             dev::bpcg_x(s, n, k, BS, Bactive, Bd, x);
             residual();

@@ -216,6 +216,7 @@ struct MGH {
     int bk = 0, bprec = -1;
     std::vector<BatchLevel> BL;
     DBuf<double> Br, Bzold, Bd, Bq, BS, Bpart, Bgemm;
+    DBuf<double> Bdq;                   // partial sums of the fused product and d.q (dev::apply_dot)
     DBuf<float> Bzold32;
     DBuf<int> Bactive;
     hipGraphExec_t bgraph[2] = {nullptr, nullptr};
