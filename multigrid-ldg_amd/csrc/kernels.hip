@@ -2289,7 +2289,20 @@ __global__ __launch_bounds__(256) void k_colfinalize(const double* __restrict__ 
     __shared__ double red[3][256];
     const int c = blockIdx.x, tid = threadIdx.x;
     double t[3] = {0, 0, 0};
-    for (int b = tid; b < nb; b += 256)
+    // b = tid, tid + 256, ... in order; eight partials of each value loaded at once
+    int b = tid;
+    for (; b + 7 * 256 < nb; b += 8 * 256) {
+        double u[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) u[q][v] = v < nval ? part[((size_t)(b + q * 256) * nval + v) * k + c] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) t[v] += u[q][v];
+    }
+    for (; b < nb; b += 256)
         for (int v = 0; v < nval; ++v) t[v] += part[((size_t)b * nval + v) * k + c];
     for (int v = 0; v < nval; ++v) red[v][tid] = t[v];
     __syncthreads();
@@ -2489,10 +2502,47 @@ __global__ void k_bpcg_d(long n, int k, const double* S, const int* active, cons
     d[e] = (double)zs[e] + S[(size_t)c * S_COUNT + S_BETA] * d[e];
 }
 
+// the same, four elements of one row per thread (k a multiple of 4): 16-byte
+// accesses; the elements of inactive columns are written back unchanged
+template <typename T>
+__global__ void k_bpcg_d4(long n4, int k, const double* S, const int* active, const T* __restrict__ zs,
+                          double* __restrict__ d, int init)
+{
+    const long e4 = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e4 >= n4) return;
+    const long e = 4 * e4;
+    const int c = (int)(e % k);
+    T z[4];
+    double dv[4];
+    if constexpr (sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(zs + e);
+        z[0] = v.x; z[1] = v.y; z[2] = v.z; z[3] = v.w;
+    } else {
+        const double2 v0 = *reinterpret_cast<const double2*>(zs + e), v1 = *reinterpret_cast<const double2*>(zs + e + 2);
+        z[0] = v0.x; z[1] = v0.y; z[2] = v1.x; z[3] = v1.y;
+    }
+    if (!init) {
+        const double2 v0 = *reinterpret_cast<const double2*>(d + e), v1 = *reinterpret_cast<const double2*>(d + e + 2);
+        dv[0] = v0.x; dv[1] = v0.y; dv[2] = v1.x; dv[3] = v1.y;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (init) dv[u] = (double)z[u];
+        else if (active[c + u]) dv[u] = (double)z[u] + S[(size_t)(c + u) * S_COUNT + S_BETA] * dv[u];
+    }
+    *reinterpret_cast<double2*>(d + e) = make_double2(dv[0], dv[1]);
+    *reinterpret_cast<double2*>(d + e + 2) = make_double2(dv[2], dv[3]);
+}
+
 template <typename T>
 void bpcg_d(hipStream_t s, long n, int k, const double* S, const int* active, const T* zs, double* d, bool init)
 {
-    hipLaunchKernelGGL(k_bpcg_d<T>, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, S, active, zs, d, (int)init);
+    if (k % 4 == 0) {
+        const long n4 = n * k / 4;
+        hipLaunchKernelGGL(k_bpcg_d4<T>, dim3(nblk(n4, 256)), dim3(256), 0, s, n4, k, S, active, zs, d, (int)init);
+    } else {
+        hipLaunchKernelGGL(k_bpcg_d<T>, dim3(nblk(n * k, 256)), dim3(256), 0, s, n, k, S, active, zs, d, (int)init);
+    }
     check_launch("bpcg_d");
 }
 template void bpcg_d<float>(hipStream_t, long, int, const double*, const int*, const float*, double*, bool);
