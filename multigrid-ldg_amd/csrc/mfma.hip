@@ -559,6 +559,144 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
         }
 }
 
+// Quadtree restriction and prolongation on k columns (square B x B
+// interpolation blocks T_t, row-major in pool, at most kKidWidth children per
+// coarse row): one wavefront per coarse row c and column group, its children
+// from the kid table kpair ((child, block type) pairs, -1 past the last), so
+// that the tables take one round trip and every operand the next.
+//   restriction (restrictResidual, multigrid.h:414-419):  bc_c = sum_f T_{t(f)}^T r_f
+//   prolongation (correctSolution, multigrid.h:421-426):  xf_f += T_{t(f)} xc_c
+// Operands in the layout of the block kernels above: A[n][S q + s] per lane,
+// X[S q + s][gc .. gc + NT).
+template <typename T, int B, int NT>
+__device__ __forceinline__ void x_rows(T (&xb)[(B + 3) / 4][NT], const T* __restrict__ xj, int k, const LaneOff<T, B, NT>& lo)
+{
+#pragma unroll
+    for (int s = 0; s < (B + 3) / 4; ++s) {
+        if (lo.kin(s)) mf_ld<T, NT>(xb[s], xj + (size_t)s * k + lo.xl);
+        else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) xb[s][t] = (T)0;
+        }
+    }
+}
+
+// A operand of a row-major block (TR: of its transpose), zero outside B x B
+template <typename T, int B, bool TR>
+__device__ __forceinline__ void rm_a(T (&a)[(B + 3) / 4], const T* __restrict__ blk, int lane)
+{
+    constexpr int S = (B + 3) / 4;
+    const int n = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int c = S * q + s;
+        a[s] = (n < B && c < B) ? (TR ? blk[c * B + n] : blk[n * B + c]) : (T)0;
+    }
+}
+
+template <typename T, int B, int NT>
+__global__ __launch_bounds__(256) void mf_restrict(int k, int nc, int ngroups, const int* __restrict__ kpair,
+                                                   const T* __restrict__ pool, const T* __restrict__ r, T* __restrict__ bc)
+{
+    using V4 = typename Mf<T>::v4;
+    constexpr int S = (B + 3) / 4, KW = kKidWidth;
+    static_assert(B <= 16, "one row tile");
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));   // wave-uniform
+    const int c = w / ngroups, g = w - c * ngroups;
+    if (c >= nc) return;
+    const int col0 = g * 16 * NT;
+    const int2 kp = lane < KW ? reinterpret_cast<const int2*>(kpair)[(size_t)c * KW + lane] : make_int2(-1, 0);
+    const LaneOff<T, B, NT> lo(lane, k, col0);
+    T a[KW][S], xb[KW][S][NT];
+    bool use[KW];
+#pragma unroll
+    for (int m = 0; m < KW; ++m) {
+        const int f = __builtin_amdgcn_readlane(kp.x, m), ty = __builtin_amdgcn_readlane(kp.y, m);
+        use[m] = f >= 0;
+        if (use[m]) {
+            rm_a<T, B, true>(a[m], pool + (size_t)ty * B * B, lane);
+            x_rows<T, B, NT>(xb[m], r + (size_t)f * B * k, k, lo);
+        }
+    }
+    V4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < KW; ++m)
+        if (use[m]) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t] = Mf<T>::mac(a[m][s], xb[m][s][t], acc[t]);
+        }
+    const int gc = col0 + NT * (lane & 15);
+    if (gc >= k) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int rr = Mf<T>::row(lane, q);
+        if (rr >= B) continue;
+        T v[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) v[t] = acc[t][q];
+        mf_st<T, NT>(bc + ((size_t)c * B + rr) * k + gc, v);
+    }
+}
+
+template <typename T, int B, int NT>
+__global__ __launch_bounds__(256) void mf_prolong(int k, int nc, int ngroups, const int* __restrict__ kpair,
+                                                  const T* __restrict__ pool, const T* __restrict__ xc, T* __restrict__ xf)
+{
+    using V4 = typename Mf<T>::v4;
+    constexpr int S = (B + 3) / 4, KW = kKidWidth;
+    static_assert(B <= 16, "one row tile");
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));   // wave-uniform
+    const int c = w / ngroups, g = w - c * ngroups;
+    if (c >= nc) return;
+    const int col0 = g * 16 * NT;
+    const int gc = col0 + NT * (lane & 15);
+    const int2 kp = lane < KW ? reinterpret_cast<const int2*>(kpair)[(size_t)c * KW + lane] : make_int2(-1, 0);
+    const LaneOff<T, B, NT> lo(lane, k, col0);
+    T xb[S][NT];
+    x_rows<T, B, NT>(xb, xc + (size_t)c * B * k, k, lo);
+#pragma unroll
+    for (int m = 0; m < KW; ++m) {
+        const int f = __builtin_amdgcn_readlane(kp.x, m), ty = __builtin_amdgcn_readlane(kp.y, m);
+        if (f < 0) continue;
+        T a[S];
+        rm_a<T, B, false>(a, pool + (size_t)ty * B * B, lane);
+        T xo[4][NT];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = Mf<T>::row(lane, q);
+            if (rr < B && gc < k) mf_ld<T, NT>(xo[q], xf + ((size_t)f * B + rr) * k + gc);
+            else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) xo[q][t] = (T)0;
+            }
+        }
+        V4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = Mf<T>::mac(a[s], xb[s][t], acc[t]);
+        if (gc < k) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = Mf<T>::row(lane, q);
+                if (rr >= B) continue;
+                T v[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) v[t] = xo[q][t] + acc[t][q];
+                mf_st<T, NT>(xf + ((size_t)f * B + rr) * k + gc, v);
+            }
+        }
+    }
+}
+
 // W (n x n, row-major FP64) -> 16 x 4 tiles in the A-operand lane order:
 // Wt[(R * nK + K) * 64 + l] = W[16 R + l % 16][4 K + l / 16], zero-padded
 // (FP64, or rounded to FP32 for the FP32 V-cycle)
@@ -764,6 +902,41 @@ void MF<T>::jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* ro
                                                                                nrows, ng, rowptr, ell, W, A, Dinv, b, x, xout,
                                                                                omega, didx)));
     check("mf_jacobi");
+}
+
+bool mf_interp_supported(int B) { return B == 9 || B == 16; }
+
+#define MGLDG_MF_INTERP_DISPATCH(B, NT, ...)                                                   \
+    switch ((B) * 8 + (NT)) {                                                                   \
+        case 9 * 8 + 1: { constexpr int BB = 9, TT = 1; __VA_ARGS__; } break;                   \
+        case 9 * 8 + 2: { constexpr int BB = 9, TT = 2; __VA_ARGS__; } break;                   \
+        case 9 * 8 + 4: { constexpr int BB = 9, TT = 4; __VA_ARGS__; } break;                   \
+        case 16 * 8 + 1: { constexpr int BB = 16, TT = 1; __VA_ARGS__; } break;                 \
+        case 16 * 8 + 2: { constexpr int BB = 16, TT = 2; __VA_ARGS__; } break;                 \
+        case 16 * 8 + 4: { constexpr int BB = 16, TT = 4; __VA_ARGS__; } break;                 \
+        default: throw std::runtime_error("MFMA interpolation kernels: block size " + std::to_string(B) + " / tiles " + std::to_string(NT) + " not compiled"); \
+    }
+
+template <typename T>
+void MF<T>::restrict_(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const T* r, T* bc)
+{
+    if (nc <= 0 || k <= 0) return;
+    nt = fit_tiles(nt, k);
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    MGLDG_MF_INTERP_DISPATCH(B, nt, hipLaunchKernelGGL((mf_restrict<T, BB, TT>), dim3(nblocks((long)nc * ng)), dim3(256), 0, s,
+                                                       k, nc, ng, kpair, pool, r, bc));
+    check("mf_restrict");
+}
+
+template <typename T>
+void MF<T>::prolong(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const T* xc, T* xf)
+{
+    if (nc <= 0 || k <= 0) return;
+    nt = fit_tiles(nt, k);
+    const int ng = (k + 16 * nt - 1) / (16 * nt);
+    MGLDG_MF_INTERP_DISPATCH(B, nt, hipLaunchKernelGGL((mf_prolong<T, BB, TT>), dim3(nblocks((long)nc * ng)), dim3(256), 0, s,
+                                                       k, nc, ng, kpair, pool, xc, xf));
+    check("mf_prolong");
 }
 
 int gemm_tiles_size(int n) { return ((n + 15) / 16) * ((n + 3) / 4) * 64; }

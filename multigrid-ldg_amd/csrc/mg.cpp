@@ -1697,6 +1697,18 @@ void MGH::brelax(int l, int k, int n, bool reverse, bool zero_guess)
     }
 }
 
+// MFMA tiles for the batched restriction from / prolongation into level l:
+// quadtree levels (kid tables) whose operator application runs on MFMA
+// tiles take the same tile count; 0: the vector kernels
+template <typename T>
+int MGH::binterp(int l) const
+{
+    const Level& lv = L[l];
+    const auto& tune = btune[sizeof(T) == 4];
+    if (!lv.ilat || !dev::mf_interp_supported(lv.B) || (int)tune.size() <= l) return 0;
+    return tune[l].apply;
+}
+
 // Multigrid::vcycle (multigrid.h:361-383) on k columns, x0 = 0
 template <typename T>
 void MGH::bvcycle(int k)
@@ -1709,7 +1721,11 @@ void MGH::bvcycle(int k)
         brelax<T>(l, k, prm.npre, false, true);
         if (prm.npre == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B * k, (T)0, bx<T>(BL[l]));
         bapply<T>(l, k, bx<T>(BL[l]), bb<T>(BL[l]), br<T>(BL[l]));
-        dev::MOps<T>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
+        if (const int nt = binterp<T>(l))
+            dev::MF<T>::restrict_(s, lv.B, nt, k, lv.I.nc, lv.kpair, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
+        else
+            dev::MOps<T>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]),
+                                    bb<T>(BL[l + 1]));
     }
     Level& c = L[bottom];
     const int n = c.nb * c.B;
@@ -1728,7 +1744,10 @@ void MGH::bvcycle(int k)
     }
     for (int l = bottom - 1; l >= 0; --l) {
         Level& lv = L[l];
-        dev::MOps<T>::prolong(s, lv.B, L[l + 1].B, k, lv.nb, lv.parent, lv.type, Pool<T>(lv), bx<T>(BL[l + 1]), bx<T>(BL[l]));
+        if (const int nt = binterp<T>(l))
+            dev::MF<T>::prolong(s, lv.B, nt, k, lv.I.nc, lv.kpair, Pool<T>(lv), bx<T>(BL[l + 1]), bx<T>(BL[l]));
+        else
+            dev::MOps<T>::prolong(s, lv.B, L[l + 1].B, k, lv.nb, lv.parent, lv.type, Pool<T>(lv), bx<T>(BL[l + 1]), bx<T>(BL[l]));
         brelax<T>(l, k, prm.npost, true, false);
     }
 }

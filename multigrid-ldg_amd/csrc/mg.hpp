@@ -240,6 +240,7 @@ struct MGH {
     template <typename T> void bapply(int l, int k, const T* x, const T* b, T* y);
     template <typename T> void brelax(int l, int k, int n, bool reverse, bool zero_guess);
     template <typename T> void bvcycle(int k);
+    template <typename T> int binterp(int l) const;
     void bprecon(int precision, int k, bool use_graph);     // Bz = V(Br) on k columns
     std::vector<SolveStats> solve_batched(const double* b, double* x, int k, const SolveParams& sp);
     // lambda = A^{-1} dLdu (same MG-PCG), dLdf = M lambda, dLdg = B_g^T lambda
