@@ -75,10 +75,12 @@ MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X dense matrix-core pe
 
 def mfma_use(mg, plan, k):
     """Utilisation of the matrix cores by the fine-level batched operator
-    application (y = A x on k columns), from the autotuner's HIP-event timing
-    of the chosen kernel: useful FLOPs nnzb * 2 B^2 k (the 16 x 16 padding of
-    narrower blocks not counted) over time, against the dense MFMA peak; and
-    the algorithmic bytes (A once, x and y) over the same time."""
+    application (the residual form y = b - A x on k columns, as the autotuner
+    times it), from the autotuner's HIP-event timing of the chosen kernel:
+    useful FLOPs nnzb * 2 B^2 k (the 16 x 16 padding of narrower blocks not
+    counted) over time, against the dense MFMA peak; and the algorithmic
+    bytes (A and the column indices once; x, b and y once) over the same
+    time."""
     out = {}
     nnzb, B, nb = mg.level_nnzb[0], mg.level_B[0], mg.level_rows[0]
     for prec, size in (("fp64", 8), ("fp32", 4)):
@@ -88,11 +90,12 @@ def mfma_use(mg, plan, k):
             continue
         ms = lv["apply_ms"][{1: 1, 2: 2, 4: 3}[lv["apply"]]]
         flops = nnzb * 2.0 * B * B * k
-        byts = nnzb * B * B * size + 4.0 * nnzb + 2.0 * nb * B * k * size
+        byts = nnzb * B * B * size + 4.0 * nnzb + 3.0 * nb * B * k * size
         tf = flops / (ms * 1e-3) / 1e12
         out[prec] = {"kernel": f"mf_apply<{'float' if size == 4 else 'double'},{B},{lv['apply']}>", "ms": round(ms, 4),
                      "tflops": round(tf, 2), "peak_tflops": MFMA_PEAK_TFLOPS[prec],
-                     "frac": round(tf / MFMA_PEAK_TFLOPS[prec], 4), "gbs": round(byts / (ms * 1e-3) / 1e9, 1)}
+                     "frac": round(tf / MFMA_PEAK_TFLOPS[prec], 4), "gbs": round(byts / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     return out
 
 
@@ -266,7 +269,7 @@ def _oracle_job(job):
         b = pr.rhs(np.random.default_rng(0).standard_normal(pr.ndof), np.zeros(pr.nqb))
     t_setup = time.perf_counter() - t0
     if job.get("barrier") is not None:
-        job["barrier"].wait()
+        job["barrier"].wait(timeout=600)   # a sibling job that failed breaks the barrier instead of hanging it
     t0 = time.perf_counter()
     _, it, rr, _ = O.pcg(pr, mg, b, tol=1e-10, maxit=job.get("maxit", 500), criterion="true")
     return t_setup, time.perf_counter() - t0, it, rr, pr.ndof
@@ -329,6 +332,14 @@ def cpu_legs(gpu_iters, c2_one_core=None):
             ("c5", {"kind": "c5", "n": 17, "maxit": 3},
              "ball: Kuhn split of the jittered lattice of spacing 1/17 (1/15 of the workload), p=2, MGridGen "
              "hierarchy, multicolour GS")]
+    import oracle as O
+    if not O.mgridgen_available():      # the agglomerated hierarchies need the reference's MGridGen (oracle/_ref)
+        for name, _, what in subs:
+            if name in ("c3", "c5"):
+                out[name] = {"subset": what, "skipped": "oracle/_ref/libmgridgen.so not built"}
+                log(f"cpu {name}: skipped (MGridGen not built)")
+        subs = [q for q in subs if q[0] == "c4"]
+    log(f"cpu {', '.join(q[0] for q in subs)}: {len(subs)} subset solves started")
     res = _cpu_pool_run([j for _, j, _ in subs], True)
     for (name, job, what), (t_setup, ts, it, rr, ndof) in zip(subs, res):
         per_it = ts / max(it, 1)
