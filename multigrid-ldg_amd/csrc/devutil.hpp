@@ -27,6 +27,19 @@ __device__ __forceinline__ int xcd_block()
     return x * q + min(x, r) + (b >> 3);
 }
 
+// exactly N contiguous elements at p (no further) into o[0 .. N): one load
+// of the whole run (the hardware takes unaligned multi-dword loads).  A
+// struct of N elements, not an N-vector type: a 3-vector occupies 4.
+template <typename T, int N>
+__device__ __forceinline__ void uld(T* o, const T* p)
+{
+    struct Run { T v[N]; } w;
+    static_assert(sizeof(Run) == N * sizeof(T), "no padding");
+    __builtin_memcpy(&w, p, sizeof(Run));
+#pragma unroll
+    for (int e = 0; e < N; ++e) o[e] = w.v[e];
+}
+
 // position of the m-th summed block of a row: k0 + m, stepping over the
 // diagonal block at dp (dp < 0: no block is stepped over)
 __device__ __forceinline__ int blk_at(int k0, int m, int dp)

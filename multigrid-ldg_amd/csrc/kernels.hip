@@ -932,16 +932,23 @@ __global__ __launch_bounds__(256) void k_restrict(int nc, const int* __restrict_
 {
     using M = Map<T, BC>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = lane / BC, c = lane - slot * BC;
+    const int slot = lane / BC, c = lane - slot * BC, lane0 = slot * BC;
     const int I = blockIdx.x * M::RPB + wave * M::RPW + slot;
     if (slot >= M::RPW || I >= nc) return;
+    const int t0 = kid_off[I], nk = kid_off[I + 1] - t0;
     T acc = 0;
-    for (int kk = kid_off[I]; kk < kid_off[I + 1]; ++kk) {
-        const int k = kids[kk];
-        const T* Tb = pool + (size_t)type[k] * (BF * BC);
-        const T* rk = r + (size_t)k * BF;
+    // the slot's lanes load up to BC children and their block types at once
+    for (int m0 = 0; m0 < nk; m0 += BC) {
+        const int mk = c < nk - m0 ? kids[t0 + m0 + c] : 0;
+        const int mt = c < nk - m0 ? type[mk] : 0;
+        const int n = min(BC, nk - m0);
+        for (int m = 0; m < n; ++m) {
+            const int k = __shfl(mk, lane0 + m, 64), ty = __shfl(mt, lane0 + m, 64);
+            const T* Tb = pool + (size_t)ty * (BF * BC);
+            const T* rk = r + (size_t)k * BF;
 #pragma unroll
-        for (int q = 0; q < BF; ++q) acc += Tb[q * BC + c] * rk[q];
+            for (int q = 0; q < BF; ++q) acc += Tb[q * BC + c] * rk[q];
+        }
     }
     bc[(size_t)I * BC + c] = acc;
 }
@@ -957,9 +964,11 @@ __global__ __launch_bounds__(256) void k_prolong(int nf, const int* __restrict__
     if (slot >= M::RPW || k >= nf) return;
     const T* Tb = pool + (size_t)type[k] * (BF * BC) + r * BC;
     const T* xp = xc + (size_t)parent[k] * BC;
+    T tr[BC];
+    uld<T, BC>(tr, Tb);   // the lane's row of the block in one run
     T acc = 0;
 #pragma unroll
-    for (int c = 0; c < BC; ++c) acc += Tb[c] * xp[c];
+    for (int c = 0; c < BC; ++c) acc += tr[c] * xp[c];
     xf[(size_t)k * BF + r] += acc;
 }
 
@@ -1731,6 +1740,7 @@ __global__ __launch_bounds__(256) void mk_restrict_l(int k, int nc, const int* _
     extern __shared__ __align__(16) unsigned char mk_lds[];
     T* Ts = reinterpret_cast<T*>(mk_lds);
     __shared__ int maxkids;
+    __shared__ int kty[256];               // block type of child t of each of the G rows (-1: none)
     constexpr int TB = BF * BC;
     const int G = 256 / k;
     const int I0 = blockIdx.x * G, ng = min(G, nc - I0);
@@ -1748,10 +1758,20 @@ __global__ __launch_bounds__(256) void mk_restrict_l(int k, int nc, const int* _
 #pragma unroll
     for (int p = 0; p < BC; ++p) acc[p] = 0;
     for (int t = 0; t < mk; ++t) {
+        if (threadIdx.x < G) {
+            const int q = threadIdx.x;
+            int ty = -1;
+            if (q < ng) {
+                const int a = kid_off[I0 + q], b = kid_off[I0 + q + 1];
+                if (a + t < b) ty = type[kids[a + t]];
+            }
+            kty[q] = ty;
+        }
+        __syncthreads();
         for (int e = threadIdx.x; e < ng * TB; e += 256) {
             const int q = e / TB, o = e - q * TB;
-            const int a = kid_off[I0 + q], b = kid_off[I0 + q + 1];
-            Ts[e] = (a + t < b) ? pool[(size_t)type[kids[a + t]] * TB + o] : (T)0;
+            const int ty = kty[q];
+            Ts[e] = ty >= 0 ? pool[(size_t)ty * TB + o] : (T)0;
         }
         __syncthreads();
         if (valid && t0 + t < t1) {

@@ -275,10 +275,16 @@ void ProblemH::upload()
     }
     bf_off.upload(off.data(), off.size(), s); bf_list.upload(lst.data(), lst.size(), s);
     build_laplacian(s, N, b, T, G, M, A);
+    // M and G_d in the panel layout with their ELL tables (one column of the
+    // adjoint maps on the single-RHS row kernels)
+    Mpan.alloc(M.val.n);
+    dev::to_panel_f64(s, M.nnzb(), b, M.val, Mpan);
+    M.build_ell(s);
     // G_d (panel) and G_d^T (pattern transpose, transposed blocks, panel) for B_g and its adjoint
     for (int d = 0; d < N; ++d) {
         Gpan[d].alloc(G[d].val.n);
         dev::to_panel_f64(s, G[d].nnzb(), b, G[d].val, Gpan[d]);
+        G[d].build_ell(s);
         const HostBSR& Gh = H.G[d];
         HostBSR Tr;
         Tr.nbr = ne; Tr.nbc = ne; Tr.br = b; Tr.bc = b;
@@ -337,6 +343,20 @@ void ProblemH::adjoint_maps(const double* lam, int k, double* dLdf, double* dLdg
 {
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)ndof * k;
+    if (k == 1) {
+        if (dLdf) dev::Ops<double>::apply(s, b, ne, M.rowptr, M.ell, M.ellw, Mpan, lam, nullptr, dLdf, nullptr, nullptr, nullptr);
+        if (dLdg) {
+            for (int d = 0; d < N; ++d) {
+                if (Gl1[d].n < n) Gl1[d].alloc(n);
+                dev::Ops<double>::apply(s, b, ne, G[d].rowptr, G[d].ell, G[d].ellw, Gpan[d], lam, nullptr, Gl1[d], nullptr,
+                                        nullptr, nullptr);
+            }
+            dev::bc_adjoint(s, host->nbf, b, host->nq, 1, N, bleft, btype, bnormal, host->tauD, Lb, lam, Gl1[0],
+                            N > 1 ? Gl1[1].p : nullptr, N > 2 ? Gl1[2].p : nullptr, dLdg);
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        return;
+    }
     if (dLdf) dev::blockdiag(s, ne, b, M.val, lam, dLdf, k);
     if (dLdg) {
         DBuf<double> Gl[3];

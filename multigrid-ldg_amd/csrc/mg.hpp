@@ -74,6 +74,8 @@ struct ProblemH {
     DBuf<double> bnormal;              // per boundary face: outward unit normal (N)
     DBuf<int> bf_off, bf_list;         // boundary faces of each element (CSR)
     DBuf<double> Gpan[3];              // G_d in the solve (panel) layout
+    DBuf<double> Mpan;                 // M in the solve (panel) layout
+    DBuf<double> Gl1[3];               // G_d lambda of one column (adjoint_maps)
     DevOp GT[3];                       // G_d^T, values in the panel layout
     int N = 2, b = 0, ne = 0, nqb = 0;
     long ndof = 0;

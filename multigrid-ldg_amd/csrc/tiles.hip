@@ -56,19 +56,6 @@ template <> struct Tm<double> {
 // wave-uniform value (the same in every lane) into a scalar register
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// exactly N contiguous elements at p (no further) into o[0 .. N): one load
-// of the whole run (the hardware takes unaligned multi-dword loads).  A
-// struct of N elements, not an N-vector type: a 3-vector occupies 4.
-template <typename T, int N>
-__device__ __forceinline__ void uld(T* o, const T* p)
-{
-    struct Run { T v[N]; } w;
-    static_assert(sizeof(Run) == N * sizeof(T), "no padding");
-    __builtin_memcpy(&w, p, sizeof(Run));
-#pragma unroll
-    for (int e = 0; e < N; ++e) o[e] = w.v[e];
-}
-
 // the tile layout of a pool block: 256 elements; lane (n, q) holds four, and
 // each load instruction of a wavefront reads one contiguous kilobyte: FP32
 // element e at (n 4 + q) 4 + e, FP64 element e at ((e / 2) 64 + n 4 + q) 2 +
