@@ -13,8 +13,8 @@
 // that a lane's NT columns of X, b, x and y are contiguous; both come as
 // vector loads.  The accumulator lane layout is the instruction's (f32:
 // register e of lane l holds row 4 q + e; f64: row 4 e + q; tile column n).
-// The smoothers write the residual tile to LDS and read it back in the
-// B-operand layout for the D^-1 product.
+// The smoothers' D^-1 product takes the residual tile in the accumulators as
+// its B operand, K steps ordered by accumulator register (no LDS, no barrier).
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
@@ -362,12 +362,16 @@ __global__ __launch_bounds__(256) void mf_apply(int k, int nrows, int ngroups, c
 // b_i and x_i on this lane's rows and NT columns gc .. gc + NT (zero outside
 // the block, past k, or for an invalid wave), and the row's D^-1 entries in
 // the A-operand layout.  They are loaded before the neighbour sum, so that
-// they are in flight with it instead of after it.
+// they are in flight with it instead of after it.  The D^-1 product takes
+// the residual tile as it stands in the accumulators: in K step e of K tile
+// rt2 a lane supplies accumulator register e, i.e. row 16 rt2 + row(l, e)
+// of the residual, so dv[rt2][e][rt] = D^-1[16 rt + l % 16][16 rt2 + row(l, e)]
+// (zero outside B x B).
 template <typename T, int B, int NT>
 struct RowTile {
     static constexpr int RT = (B + 15) / 16, S = (B + 3) / 4;
     T bt[RT][4][NT], xt[RT][4][NT];
-    T dv[S][RT];
+    T dv[RT][4][RT];
 };
 
 template <typename T, int B, int NT, bool LOADX>
@@ -390,14 +394,17 @@ __device__ __forceinline__ void row_tile_load(RowTile<T, B, NT>& o, const T* __r
                 if (!LOADX || !in) o.xt[rr][e][t] = (T)0;
             }
         }
-    const int r16 = lane & 15, q = lane >> 4;
+    const int r16 = lane & 15;
 #pragma unroll
-    for (int s = 0; s < S; ++s)
+    for (int rt2 = 0; rt2 < RT; ++rt2)
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-            const int c = 4 * s + q, r = 16 * rt + r16;
-            o.dv[s][rt] = (r < B && c < B) ? panel_at<T, B>(D, r, c) : (T)0;
-        }
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const int c = 16 * rt2 + Mf<T>::row(lane, e), r = 16 * rt + r16;
+                o.dv[rt2][e][rt] = (r < B && c < B) ? panel_at<T, B>(D, r, c) : (T)0;
+            }
+    (void)S;
 }
 
 // res = b_i - acc (acc layout; zero where row_tile_load left b zero)
@@ -417,36 +424,26 @@ __device__ __forceinline__ void residual_tile(typename Mf<T>::v4* res, const typ
         }
 }
 
-// the D^-1 product of a smoother: r (acc layout, 16 RT x 16 NT) -> LDS -> D_i r
+// the D^-1 product of a smoother: D_i r with r in the accumulator layout
+// (16 RT x 16 NT), which is the B-operand layout of K steps ordered by
+// accumulator register (RowTile); no data leaves the lanes
 template <typename T, int B, int NT>
-__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* rt_in,
-                                           const RowTile<T, B, NT>& o, T* lds, int lane)
+__device__ __forceinline__ void dinv_times(typename Mf<T>::v4* out, const typename Mf<T>::v4* res,
+                                           const RowTile<T, B, NT>& o)
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
-    const int col_l = lane & 15;
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                lds[(16 * rt + Mf<T>::row(lane, e)) * (16 * NT) + 16 * t + col_l] = rt_in[rt * NT + t][e];
-    __syncthreads();
-    const int r16 = lane & 15, q = lane >> 4;
 #pragma unroll
     for (int t = 0; t < RT * NT; ++t) out[t] = V4{0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < (B + 3) / 4; ++s) {
-        const int c = 4 * s + q;
-        T xb[NT];
+    for (int rt2 = 0; rt2 < RT; ++rt2)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) xb[t] = (c < B) ? lds[c * (16 * NT) + 16 * t + r16] : (T)0;
+        for (int e = 0; e < 4; ++e)
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
+            for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) out[rt * NT + t] = Mf<T>::mac(o.dv[s][rt], xb[t], out[rt * NT + t]);
-    }
+                for (int t = 0; t < NT; ++t)
+                    out[rt * NT + t] = Mf<T>::mac(o.dv[rt2][e][rt], res[rt2 * NT + t][e], out[rt * NT + t]);
 }
 
 // damped block GS on the rows of one colour (multigrid.h:240-249):
@@ -462,15 +459,13 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
-    __shared__ T lds_all[4 * 16 * RT * 16 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    T* lds = lds_all + wv * 16 * RT * 16 * NT;
     const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + wv);   // wave-uniform
     const int ri = w / ngroups, g = w - ri * ngroups;
-    const bool valid = ri < nrows;          // every wave reaches the barrier in dinv_times
-    const int4 d = valid ? desc[ri] : make_int4(0, 0, 0, 0);
+    if (ri >= nrows) return;
+    const int4 d = desc[ri];
     const int i = d.x;
-    const bool reads = valid && !reads_none<SKIP>(cur);
+    const bool reads = !reads_none<SKIP>(cur);
     // off-diagonal blocks of the row (every row has its diagonal block)
     const int nblk = reads ? d.z - d.y - (POOL || d.w >= 0 ? 1 : 0) : 0;
     const int* gr = gcols + (size_t)ri * gw * (POOL ? 2 : 1);
@@ -481,17 +476,17 @@ __global__ __launch_bounds__(256) void mf_gs(int k, int nrows, int ngroups, cons
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
     RowTile<T, B, NT> rt;
-    row_tile_load<T, B, NT, SKIP == 0>(rt, b, x, Dinv + (size_t)(POOL ? d.w : i) * B * B, i, k, gc, valid, lane);
+    row_tile_load<T, B, NT, SKIP == 0>(rt, b, x, Dinv + (size_t)(POOL ? d.w : i) * B * B, i, k, gc, true, lane);
     const LaneOff<T, B, NT> lo(lane, k, col0);
     for (int m0 = 0; m0 < nblk; m0 += 64) {
         if (m0 > 0) e = gs_tab<POOL>(gr, m0, lane, nblk, d);
         tab_mac<T, B, NT, SKIP, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, color, cur);
     }
     V4 res[RT * NT];
-    residual_tile<T, B, NT>(res, acc, rt, k, gc, valid, lane);
+    residual_tile<T, B, NT>(res, acc, rt, k, gc, true, lane);
     V4 out[RT * NT];
-    dinv_times<T, B, NT>(out, res, rt, lds, lane);
-    if (!valid || gc >= k) return;
+    dinv_times<T, B, NT>(out, res, rt);
+    if (gc >= k) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
 #pragma unroll
@@ -518,14 +513,11 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
 {
     using V4 = typename Mf<T>::v4;
     constexpr int RT = (B + 15) / 16;
-    __shared__ T lds_all[4 * 16 * RT * 16 * NT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    T* lds = lds_all + wv * 16 * RT * 16 * NT;
     const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + wv);   // wave-uniform
-    const int i0 = w / ngroups, g = w - i0 * ngroups;
-    const bool valid = i0 < nrows;
-    const int i = valid ? i0 : 0;
-    const int k0 = valid ? rowptr[i] : 0, nblk = valid ? rowptr[i + 1] - k0 : 0;
+    const int i = w / ngroups, g = w - i * ngroups;
+    if (i >= nrows) return;
+    const int k0 = rowptr[i], nblk = rowptr[i + 1] - k0;
     const int* er = ell + (size_t)i * W * (POOL ? 2 : 1);
     RowTab e = ell_tab<POOL>(er, 0, lane, nblk, k0);
     const int col0 = g * 16 * NT, col_l = lane & 15;
@@ -534,17 +526,17 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
     for (int t = 0; t < RT * NT; ++t) acc[t] = V4{0, 0, 0, 0};
     const int gc = col0 + NT * col_l;   // this lane's NT columns (k is a multiple of NT)
     RowTile<T, B, NT> rt;
-    row_tile_load<T, B, NT, true>(rt, b, x, Dinv + (size_t)(POOL ? didx[i] : i) * B * B, i, k, gc, valid, lane);
+    row_tile_load<T, B, NT, true>(rt, b, x, Dinv + (size_t)(POOL ? didx[i] : i) * B * B, i, k, gc, true, lane);
     const LaneOff<T, B, NT> lo(lane, k, col0);
     for (int m0 = 0; m0 < nblk; m0 += 64) {
         if (m0 > 0) e = ell_tab<POOL>(er, m0, lane, nblk, k0);
         tab_mac<T, B, NT, -1, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, nullptr, 0);
     }
     V4 res[RT * NT];
-    residual_tile<T, B, NT>(res, acc, rt, k, gc, valid, lane);
+    residual_tile<T, B, NT>(res, acc, rt, k, gc, true, lane);
     V4 out[RT * NT];
-    dinv_times<T, B, NT>(out, res, rt, lds, lane);
-    if (!valid || gc >= k) return;
+    dinv_times<T, B, NT>(out, res, rt);
+    if (gc >= k) return;
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr)
 #pragma unroll

@@ -10,6 +10,9 @@
   Y = A X for every block size, tile count and column count the launchers
   accept, with every load inside its block; a wrong offset (planted below)
   must not.
+* The smoothers' D^-1 product (RowTile, dinv_times) reads the residual from
+  the accumulators, K steps ordered by accumulator register: emulated, it
+  must give D R for every block size in both register layouts.
 
 The emulation restates the kernels' address formulas; the GPU parity tests
 check the kernels themselves against the oracle."""
@@ -123,3 +126,54 @@ def test_a_planted_offset_error_is_caught():
     with pytest.raises(AssertionError):
         Y = emulate(4, 16, 2, 32, A, X, al_bias=1)
         assert np.allclose(Y, A @ X)
+
+
+def acc_row(ts, q, e):
+    """row of the 16x16 accumulator in register e of a lane in group q (mfma.hip Mf::row)"""
+    return 4 * q + e if ts == 4 else 4 * e + q
+
+
+def emulate_dinv(ts, B, Dm, R, row=acc_row):
+    """D R through the smoothers' D^-1 product (mfma.hip RowTile / dinv_times):
+    R (B x 16 columns of one tile) sits in the accumulator layout, lane (q, n)
+    register e of row tile rt2 holding R[16 rt2 + row(q, e)][n]; in K step e
+    of K tile rt2 that register is the B operand and dv[rt2][e][rt] =
+    D[16 rt + n][16 rt2 + row(q, e)] the A operand."""
+    RT = (B + 15) // 16
+    out = np.zeros((RT, 16, 16))
+    for rt2 in range(RT):
+        for e in range(4):
+            aop, bop = np.zeros((RT, 64)), np.zeros(64)
+            for lane in range(64):
+                n, q = lane & 15, lane >> 4
+                c = 16 * rt2 + row(ts, q, e)
+                bop[lane] = R[c, n] if c < B else 0.0
+                for rt in range(RT):
+                    r = 16 * rt + n
+                    aop[rt, lane] = Dm[r, c] if (r < B and c < B) else 0.0
+            for rt in range(RT):
+                out[rt] += aop[rt].reshape(4, 16).T @ bop.reshape(4, 16)
+    return np.concatenate(list(out))[:B]
+
+
+@pytest.mark.parametrize("ts,B", list(itertools.product((4, 8), (6, 8, 9, 10, 16, 27))))
+def test_dinv_product_reads_the_residual_in_the_accumulators(ts, B):
+    rng = np.random.default_rng(B)
+    Dm, R = rng.standard_normal((B, B)), rng.standard_normal((B, 16))
+    assert np.allclose(emulate_dinv(ts, B, Dm, R), Dm @ R, rtol=1e-12, atol=1e-12)
+
+
+def test_dinv_product_with_the_other_accumulator_layout_is_caught():
+    """A operand indexed by the FP64 register layout while the FP32 residual
+    sits in the FP32 one: the emulation must see the wrong product"""
+    rng = np.random.default_rng(1)
+    Dm, R = rng.standard_normal((16, 16)), rng.standard_normal((16, 16))
+    out = np.zeros((16, 16))
+    for e in range(4):
+        aop, bop = np.zeros(64), np.zeros(64)
+        for lane in range(64):
+            n, q = lane & 15, lane >> 4
+            bop[lane] = R[acc_row(4, q, e), n]
+            aop[lane] = Dm[n, acc_row(8, q, e)]
+        out += aop.reshape(4, 16).T @ bop.reshape(4, 16)
+    assert not np.allclose(out, Dm @ R)
