@@ -956,9 +956,9 @@ int gemm_slices(int n, int k)
 {
     const int ncw = gemm_ncw(k);
     const int tiles = ((n + 15) / 16) * (((k + 15) / 16 + ncw - 1) / ncw);
-    int sl = (2048 + tiles - 1) / tiles;          // about 2048 wavefronts
+    int sl = (8192 + tiles - 1) / tiles;          // about 8192 wavefronts
     const int nK = (n + 3) / 4;
-    if (sl > 32) sl = 32;
+    if (sl > 64) sl = 64;
     if (sl > nK) sl = nK;
     return sl < 1 ? 1 : sl;
 }
