@@ -108,6 +108,38 @@ void DevOp::build_ell(hipStream_t s)
 // or wavefronts); a pair's pool index then carries dev::kPidLower when the
 // column's colour precedes the row's, so that the first sweep from zero
 // knows which neighbours are already written without reading colours
+static bool pool_blocks(hipStream_t s, long nblk, int bs, const double* blocks, std::vector<int>& pidx,
+                        std::vector<int>& rep);
+
+// One-column products with a problem operator (adjoint maps): its distinct
+// blocks in the row-padded pool layout and (column, pool index) pairs at the
+// ELL stride, read by the pooled single-RHS operator kernel; false (nothing
+// built) when the blocks do not repeat.
+static bool pool_operator(hipStream_t s, DevOp& op, int b, DBuf<double>& pool, DBuf<int>& pairs)
+{
+    std::vector<int> pidx, rep;
+    if (!pool_blocks(s, op.nnzb(), b * b, op.val, pidx, rep)) return false;
+    const long np = (long)rep.size();
+    DBuf<int> rd; rd.upload(rep.data(), rep.size(), s);
+    DBuf<double> rm; rm.alloc((size_t)np * b * b);
+    dev::gather_blocks(s, (int)np, b * b, rd, op.val, rm);
+    pool.alloc((size_t)np * b * dev::pool_pad(8, b));
+    dev::to_rowpad_f64(s, np, b, rm, pool);
+    const int n = op.pat.nbr, W = op.ellw;
+    std::vector<int> pe((size_t)n * W * 2, 0);
+    for (int i = 0; i < n; ++i) {
+        int m = 0;
+        for (int k = op.pat.rowptr[i]; k < op.pat.rowptr[i + 1]; ++k, ++m) {
+            pe[((size_t)i * W + m) * 2] = op.pat.cols[k];
+            pe[((size_t)i * W + m) * 2 + 1] = pidx[k];
+        }
+        for (; m < W; ++m) pe[((size_t)i * W + m) * 2] = -1;
+    }
+    pairs.upload(pe.data(), pe.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+    return true;
+}
+
 static void gs_tables(const HostBSR& P, const std::vector<int>& rows, int w, hipStream_t s, DBuf<int>& desc_d,
                       DBuf<int>& gcols_d, const std::vector<int>* pidx = nullptr, const std::vector<int>* didx = nullptr,
                       const std::vector<int>* colour = nullptr)
@@ -275,16 +307,20 @@ void ProblemH::upload()
     }
     bf_off.upload(off.data(), off.size(), s); bf_list.upload(lst.data(), lst.size(), s);
     build_laplacian(s, N, b, T, G, M, A);
-    // M and G_d in the panel layout with their ELL tables (one column of the
-    // adjoint maps on the single-RHS row kernels)
-    Mpan.alloc(M.val.n);
-    dev::to_panel_f64(s, M.nnzb(), b, M.val, Mpan);
+    // M and G_d with their ELL tables for one column of the adjoint maps on
+    // the single-RHS row kernels: pooled where the blocks repeat (quadtree
+    // meshes), else in the panel layout
     M.build_ell(s);
+    if (!pool_operator(s, M, b, Mpool, Mpell)) {
+        Mpan.alloc(M.val.n);
+        dev::to_panel_f64(s, M.nnzb(), b, M.val, Mpan);
+    }
     // G_d (panel) and G_d^T (pattern transpose, transposed blocks, panel) for B_g and its adjoint
     for (int d = 0; d < N; ++d) {
         Gpan[d].alloc(G[d].val.n);
         dev::to_panel_f64(s, G[d].nnzb(), b, G[d].val, Gpan[d]);
         G[d].build_ell(s);
+        pool_operator(s, G[d], b, Gpool[d], Gpell[d]);
         const HostBSR& Gh = H.G[d];
         HostBSR Tr;
         Tr.nbr = ne; Tr.nbc = ne; Tr.br = b; Tr.bc = b;
@@ -344,12 +380,15 @@ void ProblemH::adjoint_maps(const double* lam, int k, double* dLdf, double* dLdg
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)ndof * k;
     if (k == 1) {
-        if (dLdf) dev::Ops<double>::apply(s, b, ne, M.rowptr, M.ell, M.ellw, Mpan, lam, nullptr, dLdf, nullptr, nullptr, nullptr);
+        auto apply1 = [&](const DevOp& op, const DBuf<double>& pool, const DBuf<int>& pell, const double* pan, double* y) {
+            if (pool.p) dev::Ops<double>::apply(s, b, ne, op.rowptr, pell, op.ellw, pool, lam, nullptr, y, nullptr, nullptr, nullptr, 1);
+            else dev::Ops<double>::apply(s, b, ne, op.rowptr, op.ell, op.ellw, pan, lam, nullptr, y, nullptr, nullptr, nullptr);
+        };
+        if (dLdf) apply1(M, Mpool, Mpell, Mpan, dLdf);
         if (dLdg) {
             for (int d = 0; d < N; ++d) {
                 if (Gl1[d].n < n) Gl1[d].alloc(n);
-                dev::Ops<double>::apply(s, b, ne, G[d].rowptr, G[d].ell, G[d].ellw, Gpan[d], lam, nullptr, Gl1[d], nullptr,
-                                        nullptr, nullptr);
+                apply1(G[d], Gpool[d], Gpell[d], Gpan[d], Gl1[d]);
             }
             dev::bc_adjoint(s, host->nbf, b, host->nq, 1, N, bleft, btype, bnormal, host->tauD, Lb, lam, Gl1[0],
                             N > 1 ? Gl1[1].p : nullptr, N > 2 ? Gl1[2].p : nullptr, dLdg);

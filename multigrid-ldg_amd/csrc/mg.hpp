@@ -74,7 +74,9 @@ struct ProblemH {
     DBuf<double> bnormal;              // per boundary face: outward unit normal (N)
     DBuf<int> bf_off, bf_list;         // boundary faces of each element (CSR)
     DBuf<double> Gpan[3];              // G_d in the solve (panel) layout
-    DBuf<double> Mpan;                 // M in the solve (panel) layout
+    DBuf<double> Mpan;                 // M in the solve (panel) layout (when M does not pool)
+    DBuf<double> Mpool, Gpool[3];      // distinct blocks of M / G_d, row-padded pool layout (when they repeat)
+    DBuf<int> Mpell, Gpell[3];         // their (column, pool index) pairs at the ELL stride
     DBuf<double> Gl1[3];               // G_d lambda of one column (adjoint_maps)
     DevOp GT[3];                       // G_d^T, values in the panel layout
     int N = 2, b = 0, ne = 0, nqb = 0;
