@@ -183,11 +183,15 @@ struct MF {
                          bool pool);
     static void jacobi(hipStream_t s, int B, int nt, int k, int nrows, const int* rowptr, const int* ell, int W,
                        const T* A, const T* Dinv, const T* b, const T* x, T* xout, T omega, const int* didx);
-    // quadtree restriction / prolongation over the nc coarse rows (kpair:
-    // (child, block type) pairs, kKidWidth per coarse row, -1 past the last;
-    // pool: the row-major B x B interpolation blocks); mf_interp_supported(B)
-    static void restrict_(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const T* r, T* bc);
+    // quadtree interpolation over the nc coarse rows (kpair: (child, block
+    // type) pairs, kKidWidth per coarse row, -1 past the last; pool: the
+    // row-major B x B interpolation blocks); mf_interp_supported(B):
+    // prolongation xf += T xc, and the restriction of the fine residual
+    // b - A x without storing it (A as for apply: rowptr / ell / W, apool:
+    // pooled)
     static void prolong(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const T* xc, T* xf);
+    static void resrestrict(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const int* rowptr,
+                            const int* ell, int W, const T* A, bool apool, const T* x, const T* b, T* bc);
 };
 bool mf_interp_supported(int B);
 // Y = W X on the matrix cores (FP64 16x16x4) for k columns: W in the 16 x 4

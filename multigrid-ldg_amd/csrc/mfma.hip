@@ -551,13 +551,13 @@ __global__ __launch_bounds__(256) void mf_jacobi(int k, int nrows, int ngroups, 
         }
 }
 
-// Quadtree restriction and prolongation on k columns (square B x B
-// interpolation blocks T_t, row-major in pool, at most kKidWidth children per
-// coarse row): one wavefront per coarse row c and column group, its children
-// from the kid table kpair ((child, block type) pairs, -1 past the last), so
-// that the tables take one round trip and every operand the next.
-//   restriction (restrictResidual, multigrid.h:414-419):  bc_c = sum_f T_{t(f)}^T r_f
+// Quadtree interpolation on k columns (square B x B interpolation blocks
+// T_t, row-major in pool, at most kKidWidth children per coarse row): one
+// wavefront per coarse row c and column group, its children from the kid
+// table kpair ((child, block type) pairs, -1 past the last), so that the
+// tables take one round trip and every operand the next.
 //   prolongation (correctSolution, multigrid.h:421-426):  xf_f += T_{t(f)} xc_c
+//   restriction of the residual: mf_resrestrict below
 // Operands in the layout of the block kernels above: A[n][S q + s] per lane,
 // X[S q + s][gc .. gc + NT).
 template <typename T, int B, int NT>
@@ -573,8 +573,8 @@ __device__ __forceinline__ void x_rows(T (&xb)[(B + 3) / 4][NT], const T* __rest
     }
 }
 
-// A operand of a row-major block (TR: of its transpose), zero outside B x B
-template <typename T, int B, bool TR>
+// A operand of a row-major block, zero outside B x B
+template <typename T, int B>
 __device__ __forceinline__ void rm_a(T (&a)[(B + 3) / 4], const T* __restrict__ blk, int lane)
 {
     constexpr int S = (B + 3) / 4;
@@ -582,47 +582,72 @@ __device__ __forceinline__ void rm_a(T (&a)[(B + 3) / 4], const T* __restrict__ 
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int c = S * q + s;
-        a[s] = (n < B && c < B) ? (TR ? blk[c * B + n] : blk[n * B + c]) : (T)0;
+        a[s] = (n < B && c < B) ? blk[n * B + c] : (T)0;
     }
 }
 
-template <typename T, int B, int NT>
-__global__ __launch_bounds__(256) void mf_restrict(int k, int nc, int ngroups, const int* __restrict__ kpair,
-                                                   const T* __restrict__ pool, const T* __restrict__ r, T* __restrict__ bc)
+// The residual restricted in one pass: bc_c = sum_f T_{t(f)}^T (b_f - A_f x)
+// (computeResidual and restrictResidual, multigrid.h:374-375, 414-419) for
+// the children f of coarse row c in kid-table order.  A child's residual
+// tile is formed as by mf_apply (its ELL row, its blocks in order) and taken
+// from the accumulators as the B operand of the T^T product, K steps ordered
+// by accumulator register (as the smoothers' D^-1 product): the fine
+// residual is never stored.
+template <typename T, int B, int NT, bool POOL>
+__global__ __launch_bounds__(256) void mf_resrestrict(int k, int nc, int ngroups, const int* __restrict__ kpair,
+                                                      const T* __restrict__ pool, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ ell, int W, const T* __restrict__ A,
+                                                      const T* __restrict__ x, const T* __restrict__ b,
+                                                      T* __restrict__ bc)
 {
     using V4 = typename Mf<T>::v4;
-    constexpr int S = (B + 3) / 4, KW = kKidWidth;
+    constexpr int KW = kKidWidth;
     static_assert(B <= 16, "one row tile");
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));   // wave-uniform
     const int c = w / ngroups, g = w - c * ngroups;
     if (c >= nc) return;
-    const int col0 = g * 16 * NT;
+    const int col0 = g * 16 * NT, n = lane & 15;
+    const int gc = col0 + NT * n;
     const int2 kp = lane < KW ? reinterpret_cast<const int2*>(kpair)[(size_t)c * KW + lane] : make_int2(-1, 0);
     const LaneOff<T, B, NT> lo(lane, k, col0);
-    T a[KW][S], xb[KW][S][NT];
-    bool use[KW];
+    V4 accc[NT];
 #pragma unroll
+    for (int t = 0; t < NT; ++t) accc[t] = V4{0, 0, 0, 0};
     for (int m = 0; m < KW; ++m) {
         const int f = __builtin_amdgcn_readlane(kp.x, m), ty = __builtin_amdgcn_readlane(kp.y, m);
-        use[m] = f >= 0;
-        if (use[m]) {
-            rm_a<T, B, true>(a[m], pool + (size_t)ty * B * B, lane);
-            x_rows<T, B, NT>(xb[m], r + (size_t)f * B * k, k, lo);
+        if (f < 0) continue;
+        const int k0 = rowptr[f], nblk = rowptr[f + 1] - k0;
+        const int* er = ell + (size_t)f * W * (POOL ? 2 : 1);
+        RowTab e = ell_tab<POOL>(er, 0, lane, nblk, k0);
+        T bt[4][NT];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = Mf<T>::row(lane, q);
+            if (r < B && gc < k) mf_ld<T, NT>(bt[q], b + ((size_t)f * B + r) * k + gc);
+            else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bt[q][t] = (T)0;
+            }
         }
+        T ta[4];   // T^T[n][row(l, q)] = T[row(l, q)][n]
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = Mf<T>::row(lane, q);
+            ta[q] = (n < B && r < B) ? pool[(size_t)ty * B * B + r * B + n] : (T)0;
+        }
+        V4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
+        for (int m0 = 0; m0 < nblk; m0 += 64) {
+            if (m0 > 0) e = ell_tab<POOL>(er, m0, lane, nblk, k0);
+            tab_mac<T, B, NT, -1, POOL>(acc, e, min(64, nblk - m0), A, x, k, lo, nullptr, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) accc[t] = Mf<T>::mac(ta[q], (T)(bt[q][t] - acc[t][q]), accc[t]);
     }
-    V4 acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = V4{0, 0, 0, 0};
-#pragma unroll
-    for (int m = 0; m < KW; ++m)
-        if (use[m]) {
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-#pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t] = Mf<T>::mac(a[m][s], xb[m][s][t], acc[t]);
-        }
-    const int gc = col0 + NT * (lane & 15);
     if (gc >= k) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -630,7 +655,7 @@ __global__ __launch_bounds__(256) void mf_restrict(int k, int nc, int ngroups, c
         if (rr >= B) continue;
         T v[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) v[t] = acc[t][q];
+        for (int t = 0; t < NT; ++t) v[t] = accc[t][q];
         mf_st<T, NT>(bc + ((size_t)c * B + rr) * k + gc, v);
     }
 }
@@ -657,7 +682,7 @@ __global__ __launch_bounds__(256) void mf_prolong(int k, int nc, int ngroups, co
         const int f = __builtin_amdgcn_readlane(kp.x, m), ty = __builtin_amdgcn_readlane(kp.y, m);
         if (f < 0) continue;
         T a[S];
-        rm_a<T, B, false>(a, pool + (size_t)ty * B * B, lane);
+        rm_a<T, B>(a, pool + (size_t)ty * B * B, lane);
         T xo[4][NT];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -910,14 +935,16 @@ bool mf_interp_supported(int B) { return B == 9 || B == 16; }
     }
 
 template <typename T>
-void MF<T>::restrict_(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const T* r, T* bc)
+void MF<T>::resrestrict(hipStream_t s, int B, int nt, int k, int nc, const int* kpair, const T* pool, const int* rowptr,
+                        const int* ell, int W, const T* A, bool apool, const T* x, const T* b, T* bc)
 {
     if (nc <= 0 || k <= 0) return;
     nt = fit_tiles(nt, k);
     const int ng = (k + 16 * nt - 1) / (16 * nt);
-    MGLDG_MF_INTERP_DISPATCH(B, nt, hipLaunchKernelGGL((mf_restrict<T, BB, TT>), dim3(nblocks((long)nc * ng)), dim3(256), 0, s,
-                                                       k, nc, ng, kpair, pool, r, bc));
-    check("mf_restrict");
+    MGLDG_MF_POOL(apool, MGLDG_MF_INTERP_DISPATCH(B, nt, hipLaunchKernelGGL((mf_resrestrict<T, BB, TT, PP>),
+                                                                            dim3(nblocks((long)nc * ng)), dim3(256), 0, s, k,
+                                                                            nc, ng, kpair, pool, rowptr, ell, W, A, x, b, bc)));
+    check("mf_resrestrict");
 }
 
 template <typename T>

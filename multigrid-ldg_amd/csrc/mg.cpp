@@ -1779,12 +1779,15 @@ void MGH::bvcycle(int k)
         Level& lv = L[l];
         brelax<T>(l, k, prm.npre, false, true);
         if (prm.npre == 0) dev::Ops<T>::fill(s, (long)lv.nb * lv.B * k, (T)0, bx<T>(BL[l]));
+        if (const int nt = binterp<T>(l)) {
+            // the residual restricted in one pass (the level's application is on MFMA tiles)
+            dev::MF<T>::resrestrict(s, lv.B, nt, k, lv.I.nc, lv.kpair, Pool<T>(lv), lv.A.rowptr, Sell(lv), lv.A.ellw, Mop<T>(lv),
+                                    lv.npool > 0, bx<T>(BL[l]), bb<T>(BL[l]), bb<T>(BL[l + 1]));
+            continue;
+        }
         bapply<T>(l, k, bx<T>(BL[l]), bb<T>(BL[l]), br<T>(BL[l]));
-        if (const int nt = binterp<T>(l))
-            dev::MF<T>::restrict_(s, lv.B, nt, k, lv.I.nc, lv.kpair, Pool<T>(lv), br<T>(BL[l]), bb<T>(BL[l + 1]));
-        else
-            dev::MOps<T>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]),
-                                    bb<T>(BL[l + 1]));
+        dev::MOps<T>::restrict_(s, lv.B, L[l + 1].B, k, lv.I.nc, lv.kid_off, lv.kids, lv.type, Pool<T>(lv), br<T>(BL[l]),
+                                bb<T>(BL[l + 1]));
     }
     Level& c = L[bottom];
     const int n = c.nb * c.B;
