@@ -1987,11 +1987,36 @@ __global__ __launch_bounds__(256) void k_gemv_f64(int n, const double* __restric
     acc = wave_sum(acc);
     if (lane == 0) y[row] = (T)acc;
 }
+// n even: 16-byte loads of W (every row starts on a 16-byte boundary).  W is
+// read once per call and is larger than the caches: nontemporal loads keep it
+// from evicting the operators and vectors of the rest of the V-cycle.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_f64x2(int n, const double* __restrict__ W, const T* __restrict__ x,
+                                                    T* __restrict__ y)
+{
+    typedef double dv2 __attribute__((ext_vector_type(2)));
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const dv2* w = reinterpret_cast<const dv2*>(W + (size_t)row * n);
+    double acc = 0;
+    for (int j = lane; j < n / 2; j += 64) {
+        const dv2 a = __builtin_nontemporal_load(w + j);
+        if constexpr (sizeof(T) == 8) {
+            const double2 c = reinterpret_cast<const double2*>(x)[j];
+            acc += a.x * c.x + a.y * c.y;
+        } else {
+            acc += a.x * (double)x[2 * j] + a.y * (double)x[2 * j + 1];
+        }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = (T)acc;
+}
 template <typename T>
 void gemv_f64(hipStream_t s, int n, const double* W, const T* x, T* y)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gemv_f64<T>, dim3(nblk(n, 4)), dim3(256), 0, s, n, W, x, y);
+    if (n % 2 == 0) hipLaunchKernelGGL(k_gemv_f64x2<T>, dim3(nblk(n, 4)), dim3(256), 0, s, n, W, x, y);
+    else hipLaunchKernelGGL(k_gemv_f64<T>, dim3(nblk(n, 4)), dim3(256), 0, s, n, W, x, y);
     check_launch("gemv_f64");
 }
 template void gemv_f64<float>(hipStream_t, int, const double*, const float*, float*);
@@ -2004,11 +2029,13 @@ __global__ __launch_bounds__(256) void k_gemv_w32(int n, const float* __restrict
 {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= n) return;
-    const float4* w = reinterpret_cast<const float4*>(W + (size_t)row * n);
+    typedef float fv4 __attribute__((ext_vector_type(4)));
+    const fv4* w = reinterpret_cast<const fv4*>(W + (size_t)row * n);
     const float4* xv = reinterpret_cast<const float4*>(x);
     double acc = 0;
     for (int j = lane; j < n / 4; j += 64) {
-        const float4 a = w[j], c = xv[j];
+        const fv4 a = __builtin_nontemporal_load(w + j);
+        const float4 c = xv[j];
         acc += (double)a.x * c.x + (double)a.y * c.y + (double)a.z * c.z + (double)a.w * c.w;
     }
     acc = wave_sum(acc);
