@@ -730,14 +730,16 @@ __global__ void k_tile_w(int n, int nR, int nK, const double* __restrict__ W, TW
 }
 
 // partial products of Y = W X over K slices, part[(slice * nRpad + row) * k + col].
-// A wavefront owns one 16-row tile of W, NCW consecutive 16-column tiles of
-// Y and one K slice: each W operand is loaded once for its NCW column tiles,
-// and four K steps of operands are loaded before their products, so
-// several loads are in flight per wavefront.  Products accumulate in K
-// order within the slice, on the MFMAs of W's type TW (the FP32 V-cycle
-// reads an FP32 copy of W: half the bytes, FP32 matrix-core rate); the
-// slices are summed in FP64.
-template <typename T, typename TW, int NCW>
+// A wavefront owns NRW consecutive 16-row tiles of W, NCW consecutive
+// 16-column tiles of Y and one K slice: each W operand is loaded once for its
+// NCW column tiles and each X operand once for its NRW row tiles, and four K
+// steps of operands are loaded before their products, so several loads are in
+// flight per wavefront.  W is read once per call and is larger than the
+// caches: its loads are nontemporal, which keeps it from evicting the rest of
+// the V-cycle's data.  Products accumulate in K order within the slice, on the
+// MFMAs of W's type TW (the FP32 V-cycle reads an FP32 copy of W: half the
+// bytes, FP32 matrix-core rate); the slices are summed in FP64.
+template <typename T, typename TW, int NCW, int NRW>
 __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int nslices, int kpers,
                                                const TW* __restrict__ Wt, const T* __restrict__ x,
                                                double* __restrict__ part)
@@ -745,25 +747,32 @@ __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int
     using V4 = typename Mf<TW>::v4;
     const int lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nC = (k + 15) / 16, ncg = (nC + NCW - 1) / NCW;
+    const int nC = (k + 15) / 16, ncg = (nC + NCW - 1) / NCW, nRg = (nR + NRW - 1) / NRW;
     const int tile = w / nslices, sl = w - tile * nslices;
-    if (tile >= nR * ncg) return;
-    const int R = tile / ncg, G = tile - R * ncg;
+    if (tile >= nRg * ncg) return;
+    const int R0 = (tile / ncg) * NRW, G = tile - (tile / ncg) * ncg;
     const int q = lane >> 4, cl = lane & 15;
     const int K0 = sl * kpers, K1 = min(nK, K0 + kpers);
-    V4 acc[NCW];
+    V4 acc[NRW][NCW];
 #pragma unroll
-    for (int t = 0; t < NCW; ++t) acc[t] = V4{0, 0, 0, 0};
-    const TW* wt = Wt + (size_t)R * nK * 64 + lane;
+    for (int r = 0; r < NRW; ++r)
+#pragma unroll
+        for (int t = 0; t < NCW; ++t) acc[r][t] = V4{0, 0, 0, 0};
+    // row tiles past the last one read row tile nR - 1 (their products are not stored)
+    const TW* wt[NRW];
+#pragma unroll
+    for (int r = 0; r < NRW; ++r) wt[r] = Wt + (size_t)min(R0 + r, nR - 1) * nK * 64 + lane;
     auto xval = [&](int K, int t) {
         const int c = 4 * K + q, col = 16 * (G * NCW + t) + cl;
         return (c < n && col < k) ? (TW)x[(size_t)c * k + col] : (TW)0;
     };
     int K = K0;
     for (; K + 4 <= K1; K += 4) {
-        TW a[4], b[4][NCW];
+        TW a[4][NRW], b[4][NCW];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) a[u] = wt[(size_t)(K + u) * 64];
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < NRW; ++r) a[u][r] = __builtin_nontemporal_load(wt[r] + (size_t)(K + u) * 64);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -771,21 +780,33 @@ __global__ __launch_bounds__(256) void mf_gemm(int n, int k, int nR, int nK, int
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int t = 0; t < NCW; ++t) acc[t] = Mf<TW>::mac(a[u], b[u][t], acc[t]);
+            for (int r = 0; r < NRW; ++r)
+#pragma unroll
+                for (int t = 0; t < NCW; ++t) acc[r][t] = Mf<TW>::mac(a[u][r], b[u][t], acc[r][t]);
     }
     for (; K < K1; ++K) {
-        const TW a = wt[(size_t)K * 64];
+        TW b[NCW];
 #pragma unroll
-        for (int t = 0; t < NCW; ++t) acc[t] = Mf<TW>::mac(a, xval(K, t), acc[t]);
+        for (int t = 0; t < NCW; ++t) b[t] = xval(K, t);
+#pragma unroll
+        for (int r = 0; r < NRW; ++r) {
+            const TW a = __builtin_nontemporal_load(wt[r] + (size_t)K * 64);
+#pragma unroll
+            for (int t = 0; t < NCW; ++t) acc[r][t] = Mf<TW>::mac(a, b[t], acc[r][t]);
+        }
     }
 #pragma unroll
-    for (int t = 0; t < NCW; ++t) {
-        const int col = 16 * (G * NCW + t) + cl;
-        if (col >= k) continue;
+    for (int r = 0; r < NRW; ++r) {
+        if (R0 + r >= nR) continue;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int r = 16 * R + Mf<TW>::row(lane, e);
-            part[((size_t)sl * (16 * nR) + r) * k + col] = (double)acc[t][e];
+        for (int t = 0; t < NCW; ++t) {
+            const int col = 16 * (G * NCW + t) + cl;
+            if (col >= k) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = 16 * (R0 + r) + Mf<TW>::row(lane, e);
+                part[((size_t)sl * (16 * nR) + row) * k + col] = (double)acc[r][t][e];
+            }
         }
     }
 }
@@ -979,11 +1000,14 @@ static int gemm_ncw(int k)
     return nC >= 4 ? 4 : (nC >= 2 ? 2 : 1);
 }
 
+// row tiles per wavefront of mf_gemm
+constexpr int kGemmNRW = 2;
+
 int gemm_slices(int n, int k)
 {
     const int ncw = gemm_ncw(k);
-    const int tiles = ((n + 15) / 16) * (((k + 15) / 16 + ncw - 1) / ncw);
-    int sl = (8192 + tiles - 1) / tiles;          // about 8192 wavefronts
+    const int tiles = ((n + 15) / 16 + kGemmNRW - 1) / kGemmNRW * (((k + 15) / 16 + ncw - 1) / ncw);
+    int sl = (8192 / kGemmNRW + tiles - 1) / tiles;   // about 8192 / kGemmNRW wavefronts
     const int nK = (n + 3) / 4;
     if (sl > 64) sl = 64;
     if (sl > nK) sl = nK;
@@ -998,11 +1022,12 @@ void gemm_mf(hipStream_t s, int n, int k, const TW* Wt, const T* x, T* y, double
     if (n <= 0 || k <= 0) return;
     const int nR = (n + 15) / 16, nK = (n + 3) / 4, nC = (k + 15) / 16;
     const int sl = gemm_slices(n, k), kpers = (nK + sl - 1) / sl;
-    const int ncw = gemm_ncw(k), ncg = (nC + ncw - 1) / ncw;
-    const dim3 g(nblocks((long)nR * ncg * sl));
-    if (ncw == 4) hipLaunchKernelGGL((mf_gemm<T, TW, 4>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
-    else if (ncw == 2) hipLaunchKernelGGL((mf_gemm<T, TW, 2>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
-    else hipLaunchKernelGGL((mf_gemm<T, TW, 1>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    const int ncw = gemm_ncw(k), ncg = (nC + ncw - 1) / ncw, nRg = (nR + kGemmNRW - 1) / kGemmNRW;
+    const dim3 g(nblocks((long)nRg * ncg * sl));
+    constexpr int R = kGemmNRW;
+    if (ncw == 4) hipLaunchKernelGGL((mf_gemm<T, TW, 4, R>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    else if (ncw == 2) hipLaunchKernelGGL((mf_gemm<T, TW, 2, R>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
+    else hipLaunchKernelGGL((mf_gemm<T, TW, 1, R>), g, dim3(256), 0, s, n, k, nR, nK, sl, kpers, Wt, x, part);
     check("mf_gemm");
     const long t = (long)n * k;
     hipLaunchKernelGGL(k_gemm_sum<T>, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, n, k, nR, sl, part, y);

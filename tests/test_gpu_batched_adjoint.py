@@ -36,6 +36,29 @@ def test_batched_matches_single(gctx, oracle, precision, collapse):
         assert rel(X[:, c], x1) < 1e-8
 
 
+@pytest.mark.parametrize("k", [5, 40, 70])
+def test_batched_collapsed_tail_equals_levelwise(gctx, oracle, k):
+    """One batched PCG step (x = alpha V(b)) with the coarse tail collapsed
+    into the MFMA GEMM equals the level-by-level V-cycle's: rounding apart
+    in FP64, within the FP32 copy's rounding in the FP32 V-cycle.  k = 5,
+    40, 70 run 1, 2 and 4 column tiles per wavefront, the last two with a
+    partial last tile or tile group."""
+    import mgldg
+    import torch
+    qt, pr, _, _ = build_pair(gctx, oracle, "adaptive", 2, rapid=True)
+    B = torch.tensor(pr.rhs(np.random.default_rng(23).standard_normal((pr.ndof, k))), device="cuda")
+    for prec, bound in (("fp64", 1e-12), ("mixed", 1e-5)):
+        out = []
+        for cdm in (0, 4096):
+            mg = mgldg.Multigrid(pr, "gs", npre=2, npost=2, precision="mixed", coarse_dense_max=cdm)
+            X = torch.zeros_like(B)
+            mgldg.pcg(mg, B, X, tol=0.0, maxit=1, criterion="true", precision=prec, flexible=prec == "mixed")
+            out.append(X.cpu().numpy())
+        assert np.all(np.isfinite(out[1]))
+        for c in range(k):
+            assert rel(out[1][:, c], out[0][:, c]) < bound, (prec, c)
+
+
 def test_batched_nonfinite_column_is_isolated(gctx, oracle):
     import mgldg
     import torch
